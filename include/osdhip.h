@@ -1,0 +1,153 @@
+/*
+ * osdhip.h -- C ABI of the MI355X (gfx950) Osd evaluator library
+ * (opensubdiv_amd/lib/libosdhip.so).
+ *
+ * Plain C: pointers, sizes and small POD structs only.  Every float / int
+ * pointer named src, dst, du, dv, duu, duv, dvv, sizes, offsets, indices,
+ * *weights, coords, arrays, params is a DEVICE pointer.  `stream` is a
+ * hipStream_t (NULL = the null stream).  Launches are asynchronous with
+ * respect to the host; call osdhip_stream_synchronize() before reading
+ * results.  No entry point throws.
+ *
+ * Return codes: OSDHIP_OK (0) on success; OSDHIP_ERR_ARGS (-1) when the
+ * arguments are rejected the way Osd::CpuEvaluator rejects them (length
+ * mismatch, NULL src/dst); OSDHIP_ERR_UNSUPPORTED (-2) for a request the
+ * library does not implement; a positive value is a hipError_t.
+ *
+ * Interfaces replaced (OpenSubdiv v3.6.0, paths relative to its tree):
+ *   osd/cudaEvaluator.cpp:33-68     extern "C" CudaEvalStencils /
+ *                                   CudaEvalPatches / CudaEvalPatchesWithDerivatives
+ *   osd/cudaEvaluator.h:159-178     CudaEvaluator::EvalStencils (raw)
+ *   osd/cudaEvaluator.h:272-296     ... with du/dv
+ *   osd/cudaEvaluator.h:437-470     ... with du/dv/duu/duv/dvv
+ *   osd/cudaEvaluator.h:700-830     CudaEvaluator::EvalPatches (raw, 3 arities)
+ *   osd/cudaEvaluator.cpp:377-380   CudaEvaluator::Synchronize
+ *   osd/cudaVertexBuffer.cpp:34-94  device buffer allocation / UpdateData
+ */
+#ifndef OSDHIP_H
+#define OSDHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OSDHIP_OK 0
+#define OSDHIP_ERR_ARGS (-1)
+#define OSDHIP_ERR_UNSUPPORTED (-2)
+
+/* Osd::BufferDescriptor (osd/bufferDescriptor.h:61-103), in floats. */
+typedef struct { int offset, length, stride; } osdhip_buffer_desc;
+
+/* Osd::PatchCoord (osd/types.h:42-64): 20 bytes, s/t in ptex-face space. */
+typedef struct { int arrayIndex, patchIndex, vertIndex; float s, t; } osdhip_patch_coord;
+
+/* Osd::PatchArray (osd/types.h:66-125): 6 ints. */
+typedef struct {
+    int regDesc, desc, numPatches, indexBase, stride, primitiveIdBase;
+} osdhip_patch_array;
+
+/* Osd::PatchParam (osd/types.h:127-130, far/patchParam.h:80-97): field0
+ * faceId[0:28] transition[28:32]; field1 depth[0:4] nonquad[4] regular[5]
+ * boundary[7:12] v[12:22] u[22:32]; then the crease sharpness. */
+typedef struct { unsigned int field0, field1; float sharpness; } osdhip_patch_param;
+
+/* Patch type ids (osd/patchBasisTypes.h:241-246). */
+#define OSDHIP_PATCH_QUADS 3
+#define OSDHIP_PATCH_TRIANGLES 4
+#define OSDHIP_PATCH_LOOP 5
+#define OSDHIP_PATCH_REGULAR 6
+#define OSDHIP_PATCH_GREGORY_BASIS 9
+#define OSDHIP_PATCH_GREGORY_TRIANGLE 10
+
+/* ---- stencils ----------------------------------------------------------
+ * dst row r (r = 0 .. end-start-1, relative to dstDesc.offset) =
+ *   sum_{j < sizes[i]} weights[offsets[i]+j] * src[indices[offsets[i]+j]]
+ * for i = start + r, accumulated in fp32.  The row convention follows
+ * CpuEvalStencils' generic path (osd/cpuKernel.cpp:104-121); every templated
+ * reference entry point passes start = 0, where all backends agree.
+ * end <= start is a no-op returning OSDHIP_OK; srcDesc.length != any output
+ * length returns OSDHIP_ERR_ARGS (osd/cpuEvaluator.cpp:46-47). */
+
+/* replaces CudaEvaluator::EvalStencils raw (osd/cudaEvaluator.h:159-178) and
+ * extern "C" CudaEvalStencils (osd/cudaEvaluator.cpp:33-40) */
+int osdhip_eval_stencils(const float *src, osdhip_buffer_desc srcDesc,
+                         float *dst, osdhip_buffer_desc dstDesc,
+                         const int *sizes, const int *offsets,
+                         const int *indices, const float *weights,
+                         int start, int end, void *stream);
+
+/* replaces CudaEvaluator::EvalStencils with du/dv (osd/cudaEvaluator.h:
+ * 272-296; osd/cudaEvaluator.cpp:172-214 makes 3 launches, this makes one).
+ * A NULL output (dst, du or dv) is skipped, as the CUDA backend skips it. */
+int osdhip_eval_stencils_d1(const float *src, osdhip_buffer_desc srcDesc,
+                            float *dst, osdhip_buffer_desc dstDesc,
+                            float *du, osdhip_buffer_desc duDesc,
+                            float *dv, osdhip_buffer_desc dvDesc,
+                            const int *sizes, const int *offsets,
+                            const int *indices, const float *weights,
+                            const float *duWeights, const float *dvWeights,
+                            int start, int end, void *stream);
+
+/* replaces CudaEvaluator::EvalStencils with 1st+2nd derivatives
+ * (osd/cudaEvaluator.h:437-470; osd/cudaEvaluator.cpp:217-292). */
+int osdhip_eval_stencils_d2(const float *src, osdhip_buffer_desc srcDesc,
+                            float *dst, osdhip_buffer_desc dstDesc,
+                            float *du, osdhip_buffer_desc duDesc,
+                            float *dv, osdhip_buffer_desc dvDesc,
+                            float *duu, osdhip_buffer_desc duuDesc,
+                            float *duv, osdhip_buffer_desc duvDesc,
+                            float *dvv, osdhip_buffer_desc dvvDesc,
+                            const int *sizes, const int *offsets,
+                            const int *indices, const float *weights,
+                            const float *duWeights, const float *dvWeights,
+                            const float *duuWeights, const float *duvWeights,
+                            const float *dvvWeights,
+                            int start, int end, void *stream);
+
+/* ---- patches -------------------------------------------------------------
+ * replaces CudaEvaluator::EvalPatches raw, all three arities
+ * (osd/cudaEvaluator.h:700-830) and extern "C" CudaEvalPatches /
+ * CudaEvalPatchesWithDerivatives (osd/cudaEvaluator.cpp:42-68).
+ * Output i (relative to each desc offset, with each desc stride) is the
+ * limit value / derivative at coords[i], as CpuEvaluator::EvalPatches
+ * (osd/cpuEvaluator.cpp:157-381).  Any output pointer may be NULL (skipped);
+ * src == NULL or an output length != srcDesc.length returns OSDHIP_ERR_ARGS.
+ * Supported patch types: REGULAR (B-spline, with boundary fold),
+ * GREGORY_BASIS, QUADS (bilinear); other types produce zeros. */
+int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
+                        float *dst, osdhip_buffer_desc dstDesc,
+                        float *du, osdhip_buffer_desc duDesc,
+                        float *dv, osdhip_buffer_desc dvDesc,
+                        float *duu, osdhip_buffer_desc duuDesc,
+                        float *duv, osdhip_buffer_desc duvDesc,
+                        float *dvv, osdhip_buffer_desc dvvDesc,
+                        int numPatchCoords,
+                        const osdhip_patch_coord *coords,
+                        const osdhip_patch_array *arrays,
+                        const int *patchIndices,
+                        const osdhip_patch_param *params,
+                        void *stream);
+
+/* ---- device memory and streams -------------------------------------------
+ * Used by the C++ HipVertexBuffer / HipStencilTable / HipPatchTable
+ * (mirroring osd/cudaVertexBuffer.cpp:34-94 and cudaEvaluator.cpp:75-145). */
+int osdhip_malloc(void **ptr, size_t bytes);
+int osdhip_free(void *ptr);
+int osdhip_memcpy_htod(void *dst, const void *src, size_t bytes, void *stream);
+int osdhip_memcpy_dtoh(void *dst, const void *src, size_t bytes, void *stream);
+int osdhip_memcpy_dtod(void *dst, const void *src, size_t bytes, void *stream);
+/* replaces CudaEvaluator::Synchronize (osd/cudaEvaluator.cpp:377-380) */
+int osdhip_stream_synchronize(void *stream);
+int osdhip_device_synchronize(void);
+/* Writes the device name (NUL-terminated) into buf; returns the device
+ * count, or a negative value when no device is visible. */
+int osdhip_device_info(char *buf, int buflen);
+/* Library version string, e.g. "osdhip 0.1 gfx950". */
+const char *osdhip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OSDHIP_H */

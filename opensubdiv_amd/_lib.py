@@ -1,0 +1,84 @@
+"""ctypes binding of the C ABI in include/osdhip.h (libosdhip.so) and of the
+host table compiler in include/osdfar.h (libosdfar.so).
+
+The libraries are built in-tree by ``opensubdiv_amd.build`` into
+``opensubdiv_amd/lib``.  Loading fails loudly when a library is missing:
+there is no CPU fallback for the evaluator.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+LIBDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+HIP_LIB = os.path.join(LIBDIR, "libosdhip.so")
+FAR_LIB = os.path.join(LIBDIR, "libosdfar.so")
+
+OK = 0
+ERR_ARGS = -1
+ERR_UNSUPPORTED = -2
+
+
+class Desc(C.Structure):
+    """osdhip_buffer_desc == Osd::BufferDescriptor (offset, length, stride)."""
+    _fields_ = [("offset", C.c_int), ("length", C.c_int), ("stride", C.c_int)]
+
+
+def desc(d) -> Desc:
+    if d is None:
+        return Desc(0, 0, 0)
+    if isinstance(d, Desc):
+        return d
+    return Desc(int(d.offset), int(d.length), int(d.stride))
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_D = Desc
+
+_HIP_SIGS = {
+    "osdhip_eval_stencils": [_vp, _D, _vp, _D, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "osdhip_eval_stencils_d1": [_vp, _D, _vp, _D, _vp, _D, _vp, _D,
+                                _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "osdhip_eval_stencils_d2": [_vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D,
+                                _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
+    "osdhip_eval_patches": [_vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D,
+                            _i, _vp, _vp, _vp, _vp, _vp],
+    "osdhip_malloc": [C.POINTER(C.c_void_p), C.c_size_t],
+    "osdhip_free": [_vp],
+    "osdhip_memcpy_htod": [_vp, _vp, C.c_size_t, _vp],
+    "osdhip_memcpy_dtoh": [_vp, _vp, C.c_size_t, _vp],
+    "osdhip_memcpy_dtod": [_vp, _vp, C.c_size_t, _vp],
+    "osdhip_stream_synchronize": [_vp],
+    "osdhip_device_synchronize": [],
+    "osdhip_device_info": [C.c_char_p, _i],
+    "osdhip_version": [],
+}
+
+_hip = None
+
+
+def hip_symbols():
+    return list(_HIP_SIGS)
+
+
+def load_hip():
+    """Load libosdhip.so (raises if it has not been built)."""
+    global _hip
+    if _hip is not None:
+        return _hip
+    if not os.path.exists(HIP_LIB):
+        raise RuntimeError(f"libosdhip.so not built ({HIP_LIB}); run opensubdiv_amd.build")
+    lib = C.CDLL(HIP_LIB)
+    for name, args in _HIP_SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = C.c_char_p if name == "osdhip_version" else C.c_int
+    _hip = lib
+    return lib
+
+
+def check(rc: int, what: str = "osdhip call"):
+    if rc != OK:
+        raise RuntimeError(f"{what} failed with code {rc}")
+    return rc

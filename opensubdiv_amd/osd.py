@@ -1,0 +1,285 @@
+"""Python mirror of the Osd HIP evaluator interface (tests, benchmark, and
+Python callers).  Mirrors the reference's static evaluator API
+(osd/cudaEvaluator.h:94-1265, osd/cpuEvaluator.h) with the same names,
+argument meaning and bool error results; device memory is held in torch
+tensors on ``cuda`` (plumbing only -- every computation runs in the HIP
+kernels of libosdhip.so, there is no CPU fallback).
+
+    vb = HipVertexBuffer.Create(3, nCoarse + nRefined)
+    vb.UpdateData(coarse_xyz, 0, nCoarse)
+    table = HipStencilTable.Create(farLikeTable)
+    HipEvaluator.EvalStencils(vb, BufferDescriptor(0, 3, 3),
+                              vb, BufferDescriptor(nCoarse * 3, 3, 3), table)
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+
+PATCH_COORD_DTYPE = np.dtype([("arrayIndex", "<i4"), ("patchIndex", "<i4"),
+                              ("vertIndex", "<i4"), ("s", "<f4"), ("t", "<f4")])
+PATCH_ARRAY_DTYPE = np.dtype([("regDesc", "<i4"), ("desc", "<i4"), ("numPatches", "<i4"),
+                              ("indexBase", "<i4"), ("stride", "<i4"),
+                              ("primitiveIdBase", "<i4")])
+PATCH_PARAM_DTYPE = np.dtype([("field0", "<u4"), ("field1", "<u4"), ("sharpness", "<f4")])
+
+QUADS, TRIANGLES, LOOP, REGULAR, GREGORY_BASIS, GREGORY_TRIANGLE = 3, 4, 5, 6, 9, 10
+
+
+@dataclass
+class BufferDescriptor:
+    """Osd::BufferDescriptor (osd/bufferDescriptor.h:61-103), in floats."""
+    offset: int = 0
+    length: int = 0
+    stride: int = 0
+
+    def GetLocalOffset(self) -> int:
+        return self.offset % self.stride if self.stride > 0 else 0
+
+    def IsValid(self) -> bool:
+        return self.length > 0 and self.length <= self.stride - self.GetLocalOffset()
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(x) -> int | None:
+    """Device pointer of a buffer-like object (torch tensor, HipVertexBuffer)."""
+    if x is None:
+        return None
+    if hasattr(x, "BindHipBuffer"):
+        return x.BindHipBuffer()
+    if hasattr(x, "data_ptr"):
+        if not x.is_cuda:
+            raise ValueError("device buffer expected")
+        return x.data_ptr()
+    return int(x)
+
+
+def _stream(ctx=None) -> int | None:
+    if ctx is not None:
+        return int(ctx)
+    torch = _torch()
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _device_tensor(a: np.ndarray):
+    torch = _torch()
+    a = np.ascontiguousarray(a)
+    if a.dtype.fields is not None:
+        a = a.view(np.int32 if a.dtype.itemsize % 4 == 0 else np.uint8)
+    return torch.from_numpy(a.copy()).to("cuda")
+
+
+class HipVertexBuffer:
+    """Osd::CudaVertexBuffer counterpart (osd/cudaVertexBuffer.h:37-68)."""
+
+    def __init__(self, numElements: int, numVertices: int):
+        torch = _torch()
+        self._n = int(numElements)
+        self._v = int(numVertices)
+        self.data = torch.zeros(self._n * self._v, dtype=torch.float32, device="cuda")
+
+    @classmethod
+    def Create(cls, numElements, numVertices, deviceContext=None):
+        return cls(numElements, numVertices)
+
+    def UpdateData(self, src, startVertex, numVertices, deviceContext=None):
+        torch = _torch()
+        a = np.ascontiguousarray(np.asarray(src, np.float32).reshape(-1))
+        n = int(numVertices) * self._n
+        self.data[startVertex * self._n: startVertex * self._n + n].copy_(
+            torch.from_numpy(a[:n]), non_blocking=False)
+
+    def GetNumElements(self):
+        return self._n
+
+    def GetNumVertices(self):
+        return self._v
+
+    def BindHipBuffer(self):
+        return self.data.data_ptr()
+
+    def numpy(self):
+        return self.data.cpu().numpy().reshape(self._v, self._n)
+
+
+class HipStencilTable:
+    """Device mirror of a (Limit)StencilTable (osd/cudaEvaluator.h:52-91).
+
+    Accepts any object with GetSizes/GetOffsets/GetControlIndices/GetWeights
+    (and optionally GetDuWeights ... GetDvvWeights) returning array-likes, or
+    a dict with keys sizes/offsets/indices/weights[/du/dv/duu/duv/dvv]."""
+
+    _KEYS = ["weights", "du", "dv", "duu", "duv", "dvv"]
+    _GETTERS = ["GetWeights", "GetDuWeights", "GetDvWeights", "GetDuuWeights",
+                "GetDuvWeights", "GetDvvWeights"]
+
+    def __init__(self, table):
+        get = (lambda k, g: table.get(k)) if isinstance(table, dict) else \
+              (lambda k, g: getattr(table, g)() if hasattr(table, g) else None)
+        sizes = np.asarray(get("sizes", "GetSizes"), np.int32)
+        offsets = get("offsets", "GetOffsets")
+        offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int32) \
+            if offsets is None or len(offsets) == 0 else np.asarray(offsets, np.int32)
+        self._n = int(sizes.shape[0])
+        self.sizes = _device_tensor(sizes)
+        self.offsets = _device_tensor(offsets)
+        self.indices = _device_tensor(np.asarray(get("indices", "GetControlIndices"), np.int32))
+        self.w = []
+        for k, g in zip(self._KEYS, self._GETTERS):
+            v = get(k, g)
+            self.w.append(None if v is None or len(v) == 0 else _device_tensor(np.asarray(v, np.float32)))
+
+    @classmethod
+    def Create(cls, table, deviceContext=None):
+        return cls(table)
+
+    def GetSizesBuffer(self): return self.sizes.data_ptr()
+    def GetOffsetsBuffer(self): return self.offsets.data_ptr()
+    def GetIndicesBuffer(self): return self.indices.data_ptr()
+    def _wp(self, k): return self.w[k].data_ptr() if self.w[k] is not None else None
+    def GetWeightsBuffer(self): return self._wp(0)
+    def GetDuWeightsBuffer(self): return self._wp(1)
+    def GetDvWeightsBuffer(self): return self._wp(2)
+    def GetDuuWeightsBuffer(self): return self._wp(3)
+    def GetDuvWeightsBuffer(self): return self._wp(4)
+    def GetDvvWeightsBuffer(self): return self._wp(5)
+    def GetNumStencils(self): return self._n
+
+
+class HipPatchTable:
+    """Device mirror of a flattened patch table (osd/cudaPatchTable.h:45-100):
+    PatchArray[], patch index buffer, PatchParam[] as CpuPatchTable lays them
+    out (osd/cpuPatchTable.cpp:35-158)."""
+
+    def __init__(self, arrays, indices, params, varying_arrays=None, varying_indices=None):
+        self.arrays = _device_tensor(np.asarray(arrays).astype(PATCH_ARRAY_DTYPE, copy=False))
+        self.indices = _device_tensor(np.asarray(indices, np.int32))
+        self.params = _device_tensor(np.asarray(params).astype(PATCH_PARAM_DTYPE, copy=False))
+        self.varying_arrays = None if varying_arrays is None else _device_tensor(
+            np.asarray(varying_arrays).astype(PATCH_ARRAY_DTYPE, copy=False))
+        self.varying_indices = None if varying_indices is None else _device_tensor(
+            np.asarray(varying_indices, np.int32))
+
+    @classmethod
+    def Create(cls, flat, deviceContext=None):
+        return cls(flat["arrays"], flat["indices"], flat["params"],
+                   flat.get("varying_arrays"), flat.get("varying_indices"))
+
+    def GetPatchArrayBuffer(self): return self.arrays.data_ptr()
+    def GetPatchIndexBuffer(self): return self.indices.data_ptr()
+    def GetPatchParamBuffer(self): return self.params.data_ptr()
+    def GetVaryingPatchArrayBuffer(self):
+        return None if self.varying_arrays is None else self.varying_arrays.data_ptr()
+    def GetVaryingPatchIndexBuffer(self):
+        return None if self.varying_indices is None else self.varying_indices.data_ptr()
+    def GetNumFVarChannels(self): return 0
+
+
+class HipEvaluator:
+    """Static evaluator (osd/cudaEvaluator.h:94-1265); all methods return
+    bool like the reference.  ``deviceContext`` may be a hipStream_t (int);
+    None = torch's current stream."""
+
+    @staticmethod
+    def EvalStencils(srcBuffer, srcDesc, dstBuffer, dstDesc, *rest):
+        """EvalStencils(src, srcDesc, dst, dstDesc, table[, instance, ctx])
+        EvalStencils(src, srcDesc, dst, dstDesc, du, duDesc, dv, dvDesc, table[, ...])
+        EvalStencils(src, srcDesc, dst, dstDesc, du, duDesc, dv, dvDesc,
+                     duu, duuDesc, duv, duvDesc, dvv, dvvDesc, table[, ...])"""
+        lib = _lib.load_hip()
+        outs = [(dstBuffer, dstDesc)]
+        rest = list(rest)
+        while rest and not isinstance(rest[0], HipStencilTable):
+            outs.append((rest[0], rest[1]))
+            rest = rest[2:]
+        table = rest[0]
+        ctx = rest[2] if len(rest) > 2 else None
+        if table.GetNumStencils() == 0:
+            return False                   # osd/cpuEvaluator.h:82-83
+        s = _stream(ctx)
+        common = (table.GetSizesBuffer(), table.GetOffsetsBuffer(), table.GetIndicesBuffer())
+        n = table.GetNumStencils()
+        if len(outs) == 1:
+            rc = lib.osdhip_eval_stencils(_ptr(srcBuffer), _lib.desc(srcDesc), _ptr(dstBuffer),
+                                          _lib.desc(dstDesc), *common, table.GetWeightsBuffer(),
+                                          0, n, s)
+        elif len(outs) == 3:
+            args = []
+            for b, d in outs:
+                args += [_ptr(b), _lib.desc(d)]
+            rc = lib.osdhip_eval_stencils_d1(_ptr(srcBuffer), _lib.desc(srcDesc), *args, *common,
+                                             table.GetWeightsBuffer(), table.GetDuWeightsBuffer(),
+                                             table.GetDvWeightsBuffer(), 0, n, s)
+        elif len(outs) == 6:
+            args = []
+            for b, d in outs:
+                args += [_ptr(b), _lib.desc(d)]
+            rc = lib.osdhip_eval_stencils_d2(_ptr(srcBuffer), _lib.desc(srcDesc), *args, *common,
+                                             table.GetWeightsBuffer(), table.GetDuWeightsBuffer(),
+                                             table.GetDvWeightsBuffer(), table.GetDuuWeightsBuffer(),
+                                             table.GetDuvWeightsBuffer(), table.GetDvvWeightsBuffer(),
+                                             0, n, s)
+        else:
+            raise TypeError("EvalStencils takes 1, 3 or 6 outputs")
+        return rc == _lib.OK
+
+    @staticmethod
+    def _patches(srcBuffer, srcDesc, outs, numPatchCoords, patchCoords, arrays, indices, params,
+                 ctx):
+        lib = _lib.load_hip()
+        args = []
+        for k in range(6):
+            if k < len(outs) and outs[k][0] is not None:
+                args += [_ptr(outs[k][0]), _lib.desc(outs[k][1])]
+            else:
+                args += [None, _lib.Desc(0, 0, 0)]
+        rc = lib.osdhip_eval_patches(_ptr(srcBuffer), _lib.desc(srcDesc), *args,
+                                     int(numPatchCoords), _ptr(patchCoords), arrays, indices,
+                                     params, _stream(ctx))
+        return rc == _lib.OK
+
+    @staticmethod
+    def EvalPatches(srcBuffer, srcDesc, dstBuffer, dstDesc, *rest):
+        """EvalPatches(src, srcDesc, dst, dstDesc, [du, duDesc, dv, dvDesc,
+        [duu, duuDesc, duv, duvDesc, dvv, dvvDesc,]] numPatchCoords,
+        patchCoords, patchTable[, instance, ctx])"""
+        outs = [(dstBuffer, dstDesc)]
+        rest = list(rest)
+        while rest and not isinstance(rest[0], int):
+            outs.append((rest[0], rest[1]))
+            rest = rest[2:]
+        n, coords, table = rest[0], rest[1], rest[2]
+        ctx = rest[4] if len(rest) > 4 else None
+        return HipEvaluator._patches(srcBuffer, srcDesc, outs, n, coords,
+                                     table.GetPatchArrayBuffer(), table.GetPatchIndexBuffer(),
+                                     table.GetPatchParamBuffer(), ctx)
+
+    @staticmethod
+    def EvalPatchesVarying(srcBuffer, srcDesc, dstBuffer, dstDesc, *rest):
+        """Varying evaluation through the table's varying arrays and indices
+        (osd/cudaEvaluator.h:857-1040)."""
+        outs = [(dstBuffer, dstDesc)]
+        rest = list(rest)
+        while rest and not isinstance(rest[0], int):
+            outs.append((rest[0], rest[1]))
+            rest = rest[2:]
+        n, coords, table = rest[0], rest[1], rest[2]
+        ctx = rest[4] if len(rest) > 4 else None
+        if table.GetVaryingPatchArrayBuffer() is None:
+            return False
+        return HipEvaluator._patches(srcBuffer, srcDesc, outs, n, coords,
+                                     table.GetVaryingPatchArrayBuffer(),
+                                     table.GetVaryingPatchIndexBuffer(),
+                                     table.GetPatchParamBuffer(), ctx)
+
+    @staticmethod
+    def Synchronize(deviceContext=None):
+        lib = _lib.load_hip()
+        return lib.osdhip_stream_synchronize(_stream(deviceContext)) == _lib.OK

@@ -1,0 +1,323 @@
+/*
+ * CPU oracle for the Osd evaluator hot path.  TEST INFRASTRUCTURE ONLY:
+ * loaded by tests/, the smoke check and the benchmark's CPU-baseline leg,
+ * never by the shipped evaluator.  See osd_oracle.h for the reference
+ * functions restated here and how each one is pinned.
+ */
+#include "osd_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define R float
+#define SFX f32
+#include "osd_oracle_basis.inc"
+#undef R
+#undef SFX
+
+#define R double
+#define SFX f64
+#include "osd_oracle_basis.inc"
+#undef R
+#undef SFX
+
+/* ----------------------------------------------------------------------- */
+/* Stencils: osd/cpuKernel.cpp:71-240.  Every output row is cleared, then
+ * accumulates w * src[index] in table order in the working precision, then
+ * is copied to dst row (i - start). */
+
+static int stencil_rows_f32(const float *src, oracle_desc sd,
+                            float *const *outs, const oracle_desc *ods,
+                            const float *const *ws, int nout,
+                            const int *sizes, const int *offsets,
+                            const int *indices, int start, int end)
+{
+    float acc[6][64];
+    int L = sd.length;
+    if (L > 64) return 0;
+    src += sd.offset;
+    for (int i = start; i < end; ++i) {
+        int off = offsets[i], n = sizes[i];
+        for (int o = 0; o < nout; ++o) memset(acc[o], 0, sizeof(float) * (size_t)L);
+        for (int j = 0; j < n; ++j) {
+            const float *p = src + (size_t)indices[off + j] * (size_t)sd.stride;
+            for (int o = 0; o < nout; ++o) {
+                float w = ws[o][off + j];
+                for (int k = 0; k < L; ++k) acc[o][k] += p[k] * w;
+            }
+        }
+        for (int o = 0; o < nout; ++o) {
+            float *d = outs[o] + ods[o].offset + (size_t)(i - start) * (size_t)ods[o].stride;
+            memcpy(d, acc[o], sizeof(float) * (size_t)L);
+        }
+    }
+    return 1;
+}
+
+int oracle_eval_stencils_f32(const float *src, oracle_desc srcDesc,
+                             float *dst, oracle_desc dstDesc,
+                             const int *sizes, const int *offsets,
+                             const int *indices, const float *weights,
+                             int start, int end)
+{
+    if (end <= start) return 1;
+    if (srcDesc.length != dstDesc.length) return 0;
+    float *outs[1] = { dst };
+    oracle_desc ods[1] = { dstDesc };
+    const float *ws[1] = { weights };
+    return stencil_rows_f32(src, srcDesc, outs, ods, ws, 1, sizes, offsets, indices, start, end);
+}
+
+int oracle_eval_stencils_d1_f32(const float *src, oracle_desc srcDesc,
+                                float *dst, oracle_desc dstDesc,
+                                float *du, oracle_desc duDesc,
+                                float *dv, oracle_desc dvDesc,
+                                const int *sizes, const int *offsets,
+                                const int *indices, const float *weights,
+                                const float *duWeights, const float *dvWeights,
+                                int start, int end)
+{
+    if (end <= start) return 1;
+    if (srcDesc.length != dstDesc.length || srcDesc.length != duDesc.length ||
+        srcDesc.length != dvDesc.length) return 0;
+    float *outs[3] = { dst, du, dv };
+    oracle_desc ods[3] = { dstDesc, duDesc, dvDesc };
+    const float *ws[3] = { weights, duWeights, dvWeights };
+    return stencil_rows_f32(src, srcDesc, outs, ods, ws, 3, sizes, offsets, indices, start, end);
+}
+
+int oracle_eval_stencils_d2_f32(const float *src, oracle_desc srcDesc,
+                                float *dst, oracle_desc dstDesc,
+                                float *du, oracle_desc duDesc,
+                                float *dv, oracle_desc dvDesc,
+                                float *duu, oracle_desc duuDesc,
+                                float *duv, oracle_desc duvDesc,
+                                float *dvv, oracle_desc dvvDesc,
+                                const int *sizes, const int *offsets,
+                                const int *indices, const float *weights,
+                                const float *duWeights, const float *dvWeights,
+                                const float *duuWeights, const float *duvWeights,
+                                const float *dvvWeights,
+                                int start, int end)
+{
+    if (end <= start) return 1;
+    int L = srcDesc.length;
+    if (L != dstDesc.length || L != duDesc.length || L != dvDesc.length ||
+        L != duuDesc.length || L != duvDesc.length || L != dvvDesc.length) return 0;
+    float *outs[6] = { dst, du, dv, duu, duv, dvv };
+    oracle_desc ods[6] = { dstDesc, duDesc, dvDesc, duuDesc, duvDesc, dvvDesc };
+    const float *ws[6] = { weights, duWeights, dvWeights, duuWeights, duvWeights, dvvWeights };
+    return stencil_rows_f32(src, srcDesc, outs, ods, ws, 6, sizes, offsets, indices, start, end);
+}
+
+int oracle_eval_stencils_f64(const double *src, oracle_desc srcDesc,
+                             double *dst, oracle_desc dstDesc,
+                             const int *sizes, const int *offsets,
+                             const int *indices, const double *weights,
+                             int start, int end)
+{
+    if (end <= start) return 1;
+    if (srcDesc.length != dstDesc.length) return 0;
+    int L = srcDesc.length;
+    if (L > 64) return 0;
+    src += srcDesc.offset;
+    dst += dstDesc.offset;
+    for (int i = start; i < end; ++i) {
+        double acc[64];
+        memset(acc, 0, sizeof(double) * (size_t)L);
+        int off = offsets[i];
+        for (int j = 0; j < sizes[i]; ++j) {
+            const double *p = src + (size_t)indices[off + j] * (size_t)srcDesc.stride;
+            double w = weights[off + j];
+            for (int k = 0; k < L; ++k) acc[k] += p[k] * w;
+        }
+        memcpy(dst + (size_t)(i - start) * (size_t)dstDesc.stride, acc, sizeof(double) * (size_t)L);
+    }
+    return 1;
+}
+
+int oracle_eval_stencils_omp_f32(const float *src, oracle_desc srcDesc,
+                                 float *dst, oracle_desc dstDesc,
+                                 const int *sizes, const int *offsets,
+                                 const int *indices, const float *weights,
+                                 int start, int end, int numThreads)
+{
+    if (end <= start) return 1;
+    if (srcDesc.length != dstDesc.length) return 0;
+    int L = srcDesc.length;
+    if (L > 64) return 0;
+    const float *s0 = src + srcDesc.offset;
+    float *d0 = dst + dstDesc.offset;
+#ifdef _OPENMP
+    if (numThreads > 0) omp_set_num_threads(numThreads);
+#else
+    (void)numThreads;
+#endif
+    /* osd/ompKernel.cpp:93 - default schedule, one stencil per iteration */
+#pragma omp parallel for
+    for (int i = start; i < end; ++i) {
+        float acc[64];
+        memset(acc, 0, sizeof(float) * (size_t)L);
+        int off = offsets[i];
+        for (int j = 0; j < sizes[i]; ++j) {
+            const float *p = s0 + (size_t)indices[off + j] * (size_t)srcDesc.stride;
+            float w = weights[off + j];
+            for (int k = 0; k < L; ++k) acc[k] += p[k] * w;
+        }
+        memcpy(d0 + (size_t)(i - start) * (size_t)dstDesc.stride, acc, sizeof(float) * (size_t)L);
+    }
+    return 1;
+}
+
+/* ----------------------------------------------------------------------- */
+/* Patches: osd/cpuEvaluator.cpp:157-381.  For each coord: pick the regular
+ * or irregular descriptor of its array by the PatchParam "regular" bit
+ * (cpuEvaluator.cpp:257-259), evaluate the basis, locate the patch's
+ * control points at indexBase + stride * (patchIndex - primitiveIdBase),
+ * and accumulate each requested output. */
+
+static int patch_checks(const void *src, oracle_desc sd, const void *const *outs,
+                        const oracle_desc *ods, int nout)
+{
+    if (!src) return 0;
+    for (int o = 0; o < nout; ++o)
+        if (outs[o] && ods[o].length != sd.length) return 0;
+    return 1;
+}
+
+int oracle_eval_patches_f32(const float *src, oracle_desc srcDesc,
+                            float *dst, oracle_desc dstDesc,
+                            float *du, oracle_desc duDesc,
+                            float *dv, oracle_desc dvDesc,
+                            float *duu, oracle_desc duuDesc,
+                            float *duv, oracle_desc duvDesc,
+                            float *dvv, oracle_desc dvvDesc,
+                            int numPatchCoords,
+                            const oracle_patch_coord *coords,
+                            const oracle_patch_array *arrays,
+                            const int *patchIndices,
+                            const oracle_patch_param *params)
+{
+    float *outs[6] = { dst, du, dv, duu, duv, dvv };
+    oracle_desc ods[6] = { dstDesc, duDesc, dvDesc, duuDesc, duvDesc, dvvDesc };
+    if (!patch_checks(src, srcDesc, (const void *const *)outs, ods, 6)) return 0;
+    int L = srcDesc.length;
+    if (L > 64) return 0;
+    int want1 = du || dv, want2 = duu || duv || dvv;
+    const float *s0 = src + srcDesc.offset;
+    for (int i = 0; i < numPatchCoords; ++i) {
+        const oracle_patch_coord *c = &coords[i];
+        const oracle_patch_array *a = &arrays[c->arrayIndex];
+        const oracle_patch_param *pp = &params[c->patchIndex];
+        int type = ((pp->field1 >> 5) & 1u) ? a->regDesc : a->desc;
+        float w[6][20];
+        int n = oracle_patch_basis_f32(type, pp->field0, pp->field1, c->s, c->t,
+                                       w[0], want1 || want2 ? w[1] : 0, want1 || want2 ? w[2] : 0,
+                                       want2 ? w[3] : 0, want2 ? w[4] : 0, want2 ? w[5] : 0);
+        const int *cvs = patchIndices + a->indexBase + a->stride * (c->patchIndex - a->primitiveIdBase);
+        for (int o = 0; o < 6; ++o) {
+            if (!outs[o]) continue;
+            float acc[64];
+            memset(acc, 0, sizeof(float) * (size_t)L);
+            for (int j = 0; j < n; ++j) {
+                const float *p = s0 + (size_t)cvs[j] * (size_t)srcDesc.stride;
+                for (int k = 0; k < L; ++k) acc[k] += p[k] * w[o][j];
+            }
+            memcpy(outs[o] + ods[o].offset + (size_t)i * (size_t)ods[o].stride, acc,
+                   sizeof(float) * (size_t)L);
+        }
+    }
+    return 1;
+}
+
+int oracle_eval_patches_f64(const double *src, oracle_desc srcDesc,
+                            double *dst, oracle_desc dstDesc,
+                            double *du, oracle_desc duDesc,
+                            double *dv, oracle_desc dvDesc,
+                            int numPatchCoords,
+                            const oracle_patch_coord *coords,
+                            const oracle_patch_array *arrays,
+                            const int *patchIndices,
+                            const oracle_patch_param *params)
+{
+    double *outs[3] = { dst, du, dv };
+    oracle_desc ods[3] = { dstDesc, duDesc, dvDesc };
+    if (!patch_checks(src, srcDesc, (const void *const *)outs, ods, 3)) return 0;
+    int L = srcDesc.length;
+    if (L > 64) return 0;
+    int want1 = du || dv;
+    const double *s0 = src + srcDesc.offset;
+    for (int i = 0; i < numPatchCoords; ++i) {
+        const oracle_patch_coord *c = &coords[i];
+        const oracle_patch_array *a = &arrays[c->arrayIndex];
+        const oracle_patch_param *pp = &params[c->patchIndex];
+        int type = ((pp->field1 >> 5) & 1u) ? a->regDesc : a->desc;
+        double w[3][20];
+        int n = oracle_patch_basis_f64(type, pp->field0, pp->field1, (double)c->s, (double)c->t,
+                                       w[0], want1 ? w[1] : 0, want1 ? w[2] : 0, 0, 0, 0);
+        const int *cvs = patchIndices + a->indexBase + a->stride * (c->patchIndex - a->primitiveIdBase);
+        for (int o = 0; o < 3; ++o) {
+            if (!outs[o]) continue;
+            double acc[64];
+            memset(acc, 0, sizeof(double) * (size_t)L);
+            for (int j = 0; j < n; ++j) {
+                const double *p = s0 + (size_t)cvs[j] * (size_t)srcDesc.stride;
+                for (int k = 0; k < L; ++k) acc[k] += p[k] * w[o][j];
+            }
+            memcpy(outs[o] + ods[o].offset + (size_t)i * (size_t)ods[o].stride, acc,
+                   sizeof(double) * (size_t)L);
+        }
+    }
+    return 1;
+}
+
+int oracle_eval_patches_omp_f32(const float *src, oracle_desc srcDesc,
+                                float *dst, oracle_desc dstDesc,
+                                float *du, oracle_desc duDesc,
+                                float *dv, oracle_desc dvDesc,
+                                int numPatchCoords,
+                                const oracle_patch_coord *coords,
+                                const oracle_patch_array *arrays,
+                                const int *patchIndices,
+                                const oracle_patch_param *params,
+                                int numThreads)
+{
+    float *outs[3] = { dst, du, dv };
+    oracle_desc ods[3] = { dstDesc, duDesc, dvDesc };
+    if (!patch_checks(src, srcDesc, (const void *const *)outs, ods, 3)) return 0;
+    int L = srcDesc.length;
+    if (L > 64) return 0;
+    int want1 = du || dv;
+    const float *s0 = src + srcDesc.offset;
+#ifdef _OPENMP
+    if (numThreads > 0) omp_set_num_threads(numThreads);
+#else
+    (void)numThreads;
+#endif
+#pragma omp parallel for
+    for (int i = 0; i < numPatchCoords; ++i) {
+        const oracle_patch_coord *c = &coords[i];
+        const oracle_patch_array *a = &arrays[c->arrayIndex];
+        const oracle_patch_param *pp = &params[c->patchIndex];
+        int type = ((pp->field1 >> 5) & 1u) ? a->regDesc : a->desc;
+        float w[3][20];
+        int n = oracle_patch_basis_f32(type, pp->field0, pp->field1, c->s, c->t,
+                                       w[0], want1 ? w[1] : 0, want1 ? w[2] : 0, 0, 0, 0);
+        const int *cvs = patchIndices + a->indexBase + a->stride * (c->patchIndex - a->primitiveIdBase);
+        for (int o = 0; o < 3; ++o) {
+            if (!outs[o]) continue;
+            float acc[64];
+            memset(acc, 0, sizeof(float) * (size_t)L);
+            for (int j = 0; j < n; ++j) {
+                const float *p = s0 + (size_t)cvs[j] * (size_t)srcDesc.stride;
+                for (int k = 0; k < L; ++k) acc[k] += p[k] * w[o][j];
+            }
+            memcpy(outs[o] + ods[o].offset + (size_t)i * (size_t)ods[o].stride, acc,
+                   sizeof(float) * (size_t)L);
+        }
+    }
+    return 1;
+}

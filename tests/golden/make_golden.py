@@ -1,0 +1,151 @@
+"""Writes the golden fixtures of tests/golden/*.npz from the REAL OpenSubdiv
+reference (Far factories + Osd::CpuEvaluator), built from the reference
+sources into oracle/_ref/libosdref.so by oracle/Makefile.
+
+Run in the build container (where the reference tree exists):
+    make -C oracle && python tests/golden/make_golden.py
+
+Each fixture holds inputs (tables, control points, coords) and the
+reference's outputs; nothing of the reference's source goes into them.
+  cube_l3.npz           cfg0: tutorial cube, seeded noise, uniform L3 stencils
+                        (generateIntermediateLevels=false), CpuEvaluator
+                        output, double table + double output (FP64 oracle)
+  cubesphere_l2.npz     creased cube-sphere n=4, uniform L2 stencils with
+                        intermediate levels, xyz + 4 primvars (stride 7)
+  cubesphere_limit.npz  LimitStencilTable (du/dv + 2nd) at seeded locations,
+                        CpuEvaluator outputs (1st and 2nd derivative arities)
+  cubesphere_patches.npz iso-3 feature-adaptive PatchTable (B-spline +
+                        GregoryBasis end caps), appended local-point
+                        stencils, 4096 PatchMap coords, CpuEvaluator
+                        EvalPatches P/du/dv and 2nd derivatives, and Far
+                        EvaluateBasis<double> weights for the first 256
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import ref  # noqa: E402
+from opensubdiv_amd import meshes  # noqa: E402
+from opensubdiv_amd.osd import PATCH_COORD_DTYPE  # noqa: E402
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, **arrays)
+    print(f"wrote {path} ({os.path.getsize(path)} bytes)")
+
+
+def cube_l3():
+    m = meshes.unit_cube(seed=0)
+    topo = ref.Topology(m)
+    t = ref.stencils_uniform(topo, 3)
+    td = ref.stencils_uniform(topo, 3, double=True)
+    src = m.positions.astype(np.float32)
+    nC, nS = 8, len(t["sizes"])
+    buf = np.zeros((nC + nS, 3), np.float32)
+    buf[:nC] = src
+    assert ref.cpu_eval_stencils(buf, (0, 3, 3), buf, (nC * 3, 3, 3), t)
+    out64 = np.zeros((nS, 3))
+    from oracle import eval_stencils_f64
+    srcd = m.positions.astype(np.float64)
+    assert eval_stencils_f64(srcd, (0, 3, 3), out64, (0, 3, 3), td["sizes"], td["offsets"],
+                             td["indices"], td["weights"])
+    save("cube_l3.npz", src=src, src64=srcd, sizes=t["sizes"], offsets=t["offsets"],
+         indices=t["indices"], weights=t["weights"], weights64=td["weights"],
+         cpu_out=buf[nC:].copy(), out64=out64)
+
+
+def cubesphere_l2():
+    m = meshes.cube_sphere(4, seed=11)
+    topo = ref.Topology(m)
+    t = ref.stencils_uniform(topo, 2, intermediate=True)
+    frames = meshes.deform_frames(m, 1, seed=5)[0]           # (V, 7)
+    nC, nS = m.nverts, len(t["sizes"])
+    buf = np.zeros((nC + nS, 7), np.float32)
+    buf[:nC] = frames
+    assert ref.cpu_eval_stencils(buf, (0, 7, 7), buf, (nC * 7, 7, 7), t)
+    # the same table on xyz only, sharing the stride-7 buffer (offset 0, length 3)
+    buf3 = np.zeros((nC + nS, 7), np.float32)
+    buf3[:nC] = frames
+    assert ref.cpu_eval_stencils(buf3, (0, 3, 7), buf3, (nC * 7, 3, 7), t)
+    save("cubesphere_l2.npz", face_verts=m.face_verts, crease_pairs=m.crease_pairs,
+         crease_weights=m.crease_weights, src=frames, sizes=t["sizes"], offsets=t["offsets"],
+         indices=t["indices"], weights=t["weights"], cpu_out7=buf[nC:].copy(),
+         cpu_out3=buf3[nC:, :3].copy())
+
+
+def cubesphere_limit():
+    m = meshes.cube_sphere(4, seed=12)
+    topo = ref.Topology(m)
+    f, s, t = meshes.random_patch_locations(m.nfaces, 512, seed=13)
+    order = np.argsort(f, kind="stable")
+    f, s, t = f[order], s[order], t[order]
+    lt = ref.stencils_limit(topo, 3, f, s, t, d2=True)
+    src = m.positions.astype(np.float32)
+    n = len(lt["sizes"])
+    outs = [np.zeros((n, 3), np.float32) for _ in range(6)]
+    assert ref.cpu_eval_stencils_d2(src, (0, 3, 3), outs, [(0, 3, 3)] * 6, lt)
+    outs1 = [np.zeros((n, 3), np.float32) for _ in range(3)]
+    assert ref.cpu_eval_stencils_d1(src, (0, 3, 3), outs1, [(0, 3, 3)] * 3, lt)
+    save("cubesphere_limit.npz", face_verts=m.face_verts, crease_pairs=m.crease_pairs,
+         crease_weights=m.crease_weights, src=src, ptex=f, s=s, t=t, sizes=lt["sizes"],
+         offsets=lt["offsets"], indices=lt["indices"], weights=lt["weights"], du=lt["du"],
+         dv=lt["dv"], duu=lt["duu"], duv=lt["duv"], dvv=lt["dvv"],
+         cpu_p=outs[0], cpu_du=outs[1], cpu_dv=outs[2], cpu_duu=outs[3], cpu_duv=outs[4],
+         cpu_dvv=outs[5], cpu1_p=outs1[0], cpu1_du=outs1[1], cpu1_dv=outs1[2])
+
+
+def cubesphere_patches():
+    m = meshes.cube_sphere(4, seed=14)
+    topo = ref.Topology(m)
+    P = ref.Patches(topo, 3)
+    st = P.stencils
+    nC, nT = P.num_coarse, P.num_vertices_total
+    src = np.zeros((nT, 3), np.float32)
+    src[:nC] = m.positions.astype(np.float32)
+    assert ref.cpu_eval_stencils(src, (0, 3, 3), src, (nC * 3, 3, 3), st)
+    f, s, t = meshes.random_patch_locations(m.nfaces, 4096, seed=15)
+    h = P.find(f, s, t)
+    assert (h[:, 0] >= 0).all()
+    coords = np.zeros(len(f), PATCH_COORD_DTYPE)
+    coords["arrayIndex"], coords["patchIndex"], coords["vertIndex"] = h[:, 0], h[:, 1], h[:, 2]
+    coords["s"], coords["t"] = s.astype(np.float32), t.astype(np.float32)
+    n = len(coords)
+    o = [np.zeros((n, 3), np.float32) for _ in range(6)]
+    assert ref.cpu_eval_patches(src, (0, 3, 3), o[:3], [(0, 3, 3)] * 3, coords, P.arrays,
+                                P.indices, P.params)
+    pos_only = np.zeros((n, 3), np.float32)
+    assert ref.cpu_eval_patches(src, (0, 3, 3), [pos_only], [(0, 3, 3)], coords, P.arrays,
+                                P.indices, P.params)
+    o2 = [np.zeros((n, 3), np.float32) for _ in range(6)]
+    assert ref.cpu_eval_patches_d2(src, (0, 3, 3), o2, [(0, 3, 3)] * 6, coords, P.arrays,
+                                   P.indices, P.params)
+    k = 256
+    npts, wP, wDs, wDt = P.eval_basis_f64(h[:k], coords["s"][:k].astype(np.float64),
+                                          coords["t"][:k].astype(np.float64))
+    save("cubesphere_patches.npz", face_verts=m.face_verts, crease_pairs=m.crease_pairs,
+         crease_weights=m.crease_weights, coarse=m.positions.astype(np.float32),
+         num_coarse=np.int32(nC), st_sizes=st["sizes"], st_offsets=st["offsets"],
+         st_indices=st["indices"], st_weights=st["weights"], src=src,
+         arrays=P.arrays.view(np.int32).reshape(-1, 6), indices=P.indices,
+         params=P.params.view(np.uint32).reshape(-1, 3), coords=coords.view(np.int32).reshape(-1, 5),
+         ptex=f, u=s, v=t,
+         cpu_p=o[0], cpu_du=o[1], cpu_dv=o[2], cpu_p_only=pos_only,
+         cpu2_p=o2[0], cpu2_du=o2[1], cpu2_dv=o2[2], cpu2_duu=o2[3], cpu2_duv=o2[4],
+         cpu2_dvv=o2[5], basis_n=npts, basis_wP=wP, basis_wDs=wDs, basis_wDt=wDt)
+
+
+if __name__ == "__main__":
+    if not ref.available():
+        sys.exit("oracle/_ref/libosdref.so not built (make -C oracle)")
+    cube_l3()
+    cubesphere_l2()
+    cubesphere_limit()
+    cubesphere_patches()

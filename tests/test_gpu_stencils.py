@@ -1,0 +1,204 @@
+"""GPU parity of HIP EvalStencils (C ABI) against the reference's golden
+outputs and the CPU oracle.  The HIP kernels accumulate w*x in table order
+without FMA, so the expected result is bit-exact equality."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import golden
+from gpu_util import dev, empty, host, ptr, stream
+from opensubdiv_amd import _lib
+
+pytestmark = pytest.mark.gpu
+D = _lib.Desc
+
+
+def eval1(lib, src_t, sd, dst_t, dd, tbl, start=0, end=None):
+    end = int(len(tbl["sizes"])) if end is None else end
+    return lib.osdhip_eval_stencils(ptr(src_t), D(*sd), ptr(dst_t), D(*dd), ptr(tbl["d_sizes"]),
+                                    ptr(tbl["d_offsets"]), ptr(tbl["d_indices"]),
+                                    ptr(tbl["d_w"][0]), start, end, stream())
+
+
+def upload(t, keys=("weights",)):
+    out = dict(t)
+    out["d_sizes"] = dev(np.asarray(t["sizes"], np.int32))
+    out["d_offsets"] = dev(np.asarray(t["offsets"], np.int32))
+    out["d_indices"] = dev(np.asarray(t["indices"], np.int32))
+    out["d_w"] = [dev(np.asarray(t[k], np.float32)) for k in keys]
+    return out
+
+
+def test_cube_l3_in_place_bit_exact(hip):
+    """osd_tutorial_0 call shape: one buffer, src {0,3,3}, dst {nCoarse*3,3,3}."""
+    g = golden("cube_l3.npz")
+    tbl = upload(g)
+    nS = len(g["sizes"])
+    buf = np.zeros((8 + nS, 3), np.float32)
+    buf[:8] = g["src"]
+    b = dev(buf)
+    assert eval1(hip, b, (0, 3, 3), b, (8 * 3, 3, 3), tbl) == 0
+    out = host(b, shape=(8 + nS, 3))
+    np.testing.assert_array_equal(out[8:], g["cpu_out"])
+    np.testing.assert_array_equal(out[:8], g["src"])
+
+
+def test_cubesphere_stride7_and_length3_in_stride7(hip):
+    g = golden("cubesphere_l2.npz")
+    tbl = upload(g)
+    n = len(g["sizes"])
+    src = dev(g["src"])
+    d7 = empty(n * 7)
+    assert eval1(hip, src, (0, 7, 7), d7, (0, 7, 7), tbl) == 0
+    np.testing.assert_array_equal(host(d7, shape=(n, 7)), g["cpu_out7"])
+    d3 = empty(n * 7, 0.0)
+    assert eval1(hip, src, (0, 3, 7), d3, (0, 3, 7), tbl) == 0
+    o = host(d3, shape=(n, 7))
+    np.testing.assert_array_equal(o[:, :3], g["cpu_out3"])
+    assert np.all(o[:, 3:] == 0)        # lanes outside the descriptor untouched
+
+
+def test_limit_stencils_fused_derivatives(hip):
+    g = golden("cubesphere_limit.npz")
+    keys = ("weights", "du", "dv", "duu", "duv", "dvv")
+    tbl = upload(g, keys)
+    n = len(g["sizes"])
+    src = dev(g["src"])
+    # first derivatives, interleaved P/du/dv in one buffer {0,3,9}/{3,3,9}/{6,3,9}
+    inter = empty(n * 9)
+    rc = hip.osdhip_eval_stencils_d1(ptr(src), D(0, 3, 3), ptr(inter), D(0, 3, 9), ptr(inter),
+                                     D(3, 3, 9), ptr(inter), D(6, 3, 9), ptr(tbl["d_sizes"]),
+                                     ptr(tbl["d_offsets"]), ptr(tbl["d_indices"]),
+                                     *[ptr(w) for w in tbl["d_w"][:3]], 0, n, stream())
+    assert rc == 0
+    o = host(inter, shape=(n, 9))
+    np.testing.assert_array_equal(o[:, 0:3], g["cpu1_p"])
+    np.testing.assert_array_equal(o[:, 3:6], g["cpu1_du"])
+    np.testing.assert_array_equal(o[:, 6:9], g["cpu1_dv"])
+    outs = [empty(n * 3) for _ in range(6)]
+    args = []
+    for t in outs:
+        args += [ptr(t), D(0, 3, 3)]
+    rc = hip.osdhip_eval_stencils_d2(ptr(src), D(0, 3, 3), *args, ptr(tbl["d_sizes"]),
+                                     ptr(tbl["d_offsets"]), ptr(tbl["d_indices"]),
+                                     *[ptr(w) for w in tbl["d_w"]], 0, n, stream())
+    assert rc == 0
+    for t, k in zip(outs, ("cpu_p", "cpu_du", "cpu_dv", "cpu_duu", "cpu_duv", "cpu_dvv")):
+        np.testing.assert_array_equal(host(t, shape=(n, 3)), g[k], err_msg=k)
+
+
+def random_table(rng, n, nsrc, max_size, nw=1, empty_frac=0.05):
+    sizes = rng.integers(1, max_size + 1, size=n).astype(np.int32)
+    sizes[rng.uniform(size=n) < empty_frac] = 0
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int32)
+    nnz = int(sizes.sum())
+    t = {"sizes": sizes, "offsets": offsets,
+         "indices": rng.integers(0, nsrc, size=nnz).astype(np.int32)}
+    for k in ("weights", "du", "dv", "duu", "duv", "dvv")[:nw]:
+        t[k] = rng.uniform(-1, 1, size=nnz).astype(np.float32)
+    return t
+
+
+@pytest.mark.parametrize("L", [1, 2, 3, 4, 5, 7, 8, 9, 12, 17])
+def test_random_tables_all_lengths_bit_exact(hip, L):
+    rng = np.random.default_rng(100 + L)
+    nsrc, n = 777, 5000
+    t = random_table(rng, n, nsrc, 24)
+    tbl = upload(t)
+    # src with an element offset and a padded stride; dst with its own layout
+    sstride, soff = L + 3, 2
+    dstride, doff = L + 1, 5 * (L + 1)
+    src = rng.normal(size=(nsrc, sstride)).astype(np.float32)
+    s_t = dev(src)
+    d_t = empty(doff + n * dstride)
+    assert eval1(hip, s_t, (soff, L, sstride), d_t, (doff, L, dstride), tbl) == 0
+    got = host(d_t)[doff:].reshape(n, dstride)[:, :L]
+    want = np.zeros((n, dstride), np.float32)
+    assert oracle.eval_stencils(src, (soff, L, sstride), want, (0, L, dstride), t["sizes"],
+                                t["offsets"], t["indices"], t["weights"])
+    np.testing.assert_array_equal(got, want[:, :L])
+
+
+@pytest.mark.parametrize("nw", [3, 6])
+@pytest.mark.parametrize("L", [3, 4, 7])
+def test_random_derivative_tables_bit_exact(hip, nw, L):
+    rng = np.random.default_rng(200 + nw * 10 + L)
+    nsrc, n = 500, 3000
+    t = random_table(rng, n, nsrc, 20, nw)
+    keys = ("weights", "du", "dv", "duu", "duv", "dvv")[:nw]
+    tbl = upload(t, keys)
+    src = rng.normal(size=(nsrc, L)).astype(np.float32)
+    s_t = dev(src)
+    outs = [empty(n * L) for _ in range(nw)]
+    args = []
+    for o in outs:
+        args += [ptr(o), D(0, L, L)]
+    fn = hip.osdhip_eval_stencils_d1 if nw == 3 else hip.osdhip_eval_stencils_d2
+    assert fn(ptr(s_t), D(0, L, L), *args, ptr(tbl["d_sizes"]), ptr(tbl["d_offsets"]),
+              ptr(tbl["d_indices"]), *[ptr(w) for w in tbl["d_w"]], 0, n, stream()) == 0
+    for k, o in enumerate(outs):
+        want = oracle.apply_stencils(src, t["sizes"], t["offsets"], t["indices"], t[keys[k]])
+        np.testing.assert_array_equal(host(o, shape=(n, L)), want, err_msg=keys[k])
+
+
+def test_large_stencils_take_the_unstaged_path(hip):
+    """Stencils of up to 200 entries: a wave's table slice exceeds the LDS
+    staging capacity and the kernel reads the table directly."""
+    rng = np.random.default_rng(7)
+    t = random_table(rng, 1000, 300, 200, 3)
+    keys = ("weights", "du", "dv")
+    tbl = upload(t, keys)
+    src = rng.normal(size=(300, 3)).astype(np.float32)
+    s_t = dev(src)
+    d = empty(1000 * 3)
+    assert eval1(hip, s_t, (0, 3, 3), d, (0, 3, 3), tbl) == 0
+    want = oracle.apply_stencils(src, t["sizes"], t["offsets"], t["indices"], t["weights"])
+    np.testing.assert_array_equal(host(d, shape=(1000, 3)), want)
+
+
+def test_start_end_subrange_writes_relative_rows(hip):
+    rng = np.random.default_rng(9)
+    t = random_table(rng, 4000, 100, 16)
+    tbl = upload(t)
+    src = rng.normal(size=(100, 3)).astype(np.float32)
+    s_t = dev(src)
+    full = oracle.apply_stencils(src, t["sizes"], t["offsets"], t["indices"], t["weights"])
+    for start, end in [(0, 1), (1, 2), (63, 65), (1000, 3999), (130, 4000)]:
+        d = empty(4000 * 3)
+        assert eval1(hip, s_t, (0, 3, 3), d, (0, 3, 3), tbl, start, end) == 0
+        o = host(d, shape=(4000, 3))
+        np.testing.assert_array_equal(o[:end - start], full[start:end])
+        assert np.all(np.isnan(o[end - start:]))
+
+
+def test_argument_errors_and_noops(hip):
+    rng = np.random.default_rng(11)
+    t = random_table(rng, 100, 10, 8)
+    tbl = upload(t)
+    s_t = dev(rng.normal(size=(10, 3)).astype(np.float32))
+    d = empty(300)
+    assert eval1(hip, s_t, (0, 3, 3), d, (0, 3, 3), tbl, 5, 5) == 0        # empty range
+    assert eval1(hip, s_t, (0, 3, 3), d, (0, 3, 3), tbl, 7, 2) == 0
+    assert np.all(np.isnan(host(d)))
+    assert eval1(hip, s_t, (0, 3, 3), d, (0, 2, 3), tbl) == _lib.ERR_ARGS  # length mismatch
+    assert eval1(hip, None, (0, 3, 3), d, (0, 3, 3), tbl) == _lib.ERR_ARGS
+    assert eval1(hip, s_t, (0, 3, 3), None, (0, 3, 3), tbl) == _lib.ERR_ARGS
+    assert eval1(hip, s_t, (0, 4, 3), d, (0, 4, 3), tbl) == _lib.ERR_ARGS  # length > stride
+    assert np.all(np.isnan(host(d)))
+
+
+def test_python_mirror_eval_stencils_on_vertex_buffer(hip):
+    """HipVertexBuffer / HipStencilTable / HipEvaluator.EvalStencils, as
+    Osd::Mesh::Refine calls it (in-place buffer)."""
+    from opensubdiv_amd import BufferDescriptor, HipEvaluator, HipStencilTable, HipVertexBuffer
+    g = golden("cube_l3.npz")
+    nS = len(g["sizes"])
+    vb = HipVertexBuffer.Create(3, 8 + nS)
+    vb.UpdateData(g["src"], 0, 8)
+    table = HipStencilTable.Create({k: g[k] for k in ("sizes", "offsets", "indices", "weights")})
+    assert HipEvaluator.EvalStencils(vb, BufferDescriptor(0, 3, 3), vb,
+                                     BufferDescriptor(8 * 3, 3, 3), table)
+    assert HipEvaluator.Synchronize()
+    np.testing.assert_array_equal(vb.numpy()[8:], g["cpu_out"])
+    assert not HipEvaluator.EvalStencils(vb, BufferDescriptor(0, 3, 3), vb,
+                                         BufferDescriptor(8 * 3, 2, 3), table)

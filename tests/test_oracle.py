@@ -1,0 +1,349 @@
+"""Pins the CPU oracle (oracle/osd_oracle.c) before it is trusted as the
+checker of the HIP evaluator:
+
+* golden vectors written by the real reference (Osd::CpuEvaluator, Far
+  StencilTableReal<double>, Far PatchTable::EvaluateBasis<double>);
+* closed forms computed by independent algorithms (Cox-de Boor recursion,
+  Bernstein polynomials);
+* invariants: partition of unity, zero-sum derivatives, finite differences,
+  the boundary-fold reflection identity, Gregory -> Bezier reduction,
+  PatchParam normalization / 2^depth scaling;
+* brute force for the stencil gather.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import bbox_rel_err, golden
+from opensubdiv_amd.osd import PATCH_ARRAY_DTYPE, PATCH_COORD_DTYPE, PATCH_PARAM_DTYPE
+
+REGULAR, GREGORY, QUADS = 6, 9, 3
+
+
+# ------------------------------------------------------------ golden vectors
+
+def test_stencils_cube_l3_bit_exact_vs_cpu_evaluator():
+    g = golden("cube_l3.npz")
+    out = oracle.apply_stencils(g["src"], g["sizes"], g["offsets"], g["indices"], g["weights"])
+    assert len(g["sizes"]) == 386                      # uniform L3 of the 8-vertex cube
+    np.testing.assert_array_equal(out, g["cpu_out"])
+
+
+def test_stencils_cube_l3_fp64_vs_far_double_table():
+    g = golden("cube_l3.npz")
+    out = oracle.apply_stencils(g["src64"], g["sizes"], g["offsets"], g["indices"],
+                                g["weights64"], np.float64)
+    np.testing.assert_allclose(out, g["out64"], rtol=0, atol=1e-15)
+    # float result within the contract of the double one
+    assert bbox_rel_err(g["cpu_out"], g["out64"]) < 1e-6
+
+
+def test_stencils_stride7_and_subrange_descriptors():
+    g = golden("cubesphere_l2.npz")
+    src = g["src"]
+    n = len(g["sizes"])
+    out7 = np.zeros((n, 7), np.float32)
+    assert oracle.eval_stencils(src, (0, 7, 7), out7, (0, 7, 7), g["sizes"], g["offsets"],
+                                g["indices"], g["weights"])
+    np.testing.assert_array_equal(out7, g["cpu_out7"])
+    out3 = np.zeros((n, 7), np.float32)
+    assert oracle.eval_stencils(src, (0, 3, 7), out3, (0, 3, 7), g["sizes"], g["offsets"],
+                                g["indices"], g["weights"])
+    np.testing.assert_array_equal(out3[:, :3], g["cpu_out3"])
+    assert np.all(out3[:, 3:] == 0)                    # untouched lanes of the stride
+
+
+def test_limit_stencils_d1_d2_vs_cpu_evaluator():
+    g = golden("cubesphere_limit.npz")
+    n = len(g["sizes"])
+    outs = [np.zeros((n, 3), np.float32) for _ in range(6)]
+    assert oracle.eval_stencils_d2(g["src"], (0, 3, 3), outs, [(0, 3, 3)] * 6, g["sizes"],
+                                   g["offsets"], g["indices"],
+                                   [g[k] for k in ("weights", "du", "dv", "duu", "duv", "dvv")])
+    for o, k in zip(outs, ("cpu_p", "cpu_du", "cpu_dv", "cpu_duu", "cpu_duv", "cpu_dvv")):
+        np.testing.assert_array_equal(o, g[k])
+    outs1 = [np.zeros((n, 3), np.float32) for _ in range(3)]
+    assert oracle.eval_stencils_d1(g["src"], (0, 3, 3), outs1, [(0, 3, 3)] * 3, g["sizes"],
+                                   g["offsets"], g["indices"], [g["weights"], g["du"], g["dv"]])
+    for o, k in zip(outs1, ("cpu1_p", "cpu1_du", "cpu1_dv")):
+        np.testing.assert_array_equal(o, g[k])
+
+
+def _patch_fixture():
+    g = golden("cubesphere_patches.npz")
+    coords = np.ascontiguousarray(g["coords"]).view(PATCH_COORD_DTYPE).reshape(-1)
+    arrays = np.ascontiguousarray(g["arrays"]).view(PATCH_ARRAY_DTYPE).reshape(-1)
+    params = np.ascontiguousarray(g["params"]).view(PATCH_PARAM_DTYPE).reshape(-1)
+    return g, coords, arrays, np.ascontiguousarray(g["indices"]), params
+
+
+def test_patches_vs_cpu_evaluator():
+    g, coords, arrays, idx, params = _patch_fixture()
+    assert set(arrays["desc"].tolist()) == {REGULAR, GREGORY}
+    n = len(coords)
+    o = [np.zeros((n, 3), np.float32) for _ in range(3)]
+    assert oracle.eval_patches(g["src"], (0, 3, 3), o, [(0, 3, 3)] * 3, coords, arrays, idx, params)
+    # same polynomial forms and operation order as the reference: bit-exact
+    for x, k in zip(o, ("cpu_p", "cpu_du", "cpu_dv")):
+        np.testing.assert_array_equal(x, g[k], err_msg=k)
+    o2 = [np.zeros((n, 3), np.float32) for _ in range(6)]
+    assert oracle.eval_patches(g["src"], (0, 3, 3), o2, [(0, 3, 3)] * 6, coords, arrays, idx,
+                               params)
+    for x, k in zip(o2, ("cpu2_p", "cpu2_du", "cpu2_dv", "cpu2_duu", "cpu2_duv", "cpu2_dvv")):
+        np.testing.assert_array_equal(x, g[k], err_msg=k)
+    p_only = np.zeros((n, 3), np.float32)
+    assert oracle.eval_patches(g["src"], (0, 3, 3), [p_only], [(0, 3, 3)], coords, arrays, idx,
+                               params)
+    np.testing.assert_array_equal(p_only, g["cpu_p_only"])
+
+
+def test_patch_basis_f64_vs_far_evaluate_basis():
+    g, coords, arrays, idx, params = _patch_fixture()
+    for i in range(len(g["basis_n"])):
+        c = coords[i]
+        a = arrays[c["arrayIndex"]]
+        p = params[c["patchIndex"]]
+        ptype = a["regDesc"] if (p["field1"] >> 5) & 1 else a["desc"]
+        n, wP, wDs, wDt, *_ = oracle.patch_basis(ptype, p["field0"], p["field1"],
+                                                 float(c["s"]), float(c["t"]), 1, np.float64)
+        assert n == g["basis_n"][i]
+        np.testing.assert_allclose(wP[:n], g["basis_wP"][i, :n], atol=1e-13)
+        np.testing.assert_allclose(wDs[:n], g["basis_wDs"][i, :n], atol=1e-11)
+        np.testing.assert_allclose(wDt[:n], g["basis_wDt"][i, :n], atol=1e-11)
+
+
+# --------------------------------------------- closed forms (independent)
+
+def cox_de_boor(t, deriv=0):
+    """Uniform cubic B-spline basis on [0,1] (knots ..., -3..4) by the
+    Cox-de Boor recursion; derivatives by the standard derivative formula."""
+    knots = np.arange(-3, 5, dtype=np.float64)
+
+    def N(i, p, x):
+        if p == 0:
+            return 1.0 if knots[i] <= x < knots[i + 1] else 0.0
+        a = (x - knots[i]) / (knots[i + p] - knots[i]) * N(i, p - 1, x)
+        b = (knots[i + p + 1] - x) / (knots[i + p + 1] - knots[i + 1]) * N(i + 1, p - 1, x)
+        return a + b
+
+    def dN(i, p, x, k):
+        if k == 0:
+            return N(i, p, x)
+        return p * (dN(i, p - 1, x, k - 1) / (knots[i + p] - knots[i]) -
+                    dN(i + 1, p - 1, x, k - 1) / (knots[i + p + 1] - knots[i + 1]))
+
+    x = min(max(t, 0.0), 1.0 - 1e-15)
+    return np.array([dN(i, 3, x, deriv) for i in range(4)])
+
+
+def bernstein(t, deriv=0):
+    def b(k, n, x):
+        if k < 0 or k > n:
+            return 0.0
+        return math.comb(n, k) * x ** k * (1 - x) ** (n - k)
+    if deriv == 0:
+        return np.array([b(k, 3, t) for k in range(4)])
+    if deriv == 1:
+        return np.array([3 * (b(k - 1, 2, t) - b(k, 2, t)) for k in range(4)])
+    return np.array([6 * (b(k - 2, 1, t) - 2 * b(k - 1, 1, t) + b(k, 1, t)) for k in range(4)])
+
+
+def field1(depth=0, u=0, v=0, boundary=0, regular=1, nonquad=0):
+    return (u << 22) | (v << 12) | (boundary << 7) | (regular << 5) | (nonquad << 4) | depth
+
+
+@pytest.mark.parametrize("s,t", [(0.0, 0.0), (0.3, 0.7), (0.999, 0.01), (0.5, 0.5), (0.125, 0.9)])
+def test_bspline_basis_is_cox_de_boor_tensor_product(s, t):
+    n, wP, wDs, wDt, wDss, wDst, wDtt = oracle.patch_basis(REGULAR, 0, field1(), s, t, 2,
+                                                            np.float64)
+    assert n == 16
+    bs = [cox_de_boor(s, k) for k in range(3)]
+    bt = [cox_de_boor(t, k) for k in range(3)]
+    # point (row i, column j) = 4i + j, column along s
+    np.testing.assert_allclose(wP[:16].reshape(4, 4), np.outer(bt[0], bs[0]), atol=1e-14)
+    np.testing.assert_allclose(wDs[:16].reshape(4, 4), np.outer(bt[0], bs[1]), atol=1e-13)
+    np.testing.assert_allclose(wDt[:16].reshape(4, 4), np.outer(bt[1], bs[0]), atol=1e-13)
+    np.testing.assert_allclose(wDss[:16].reshape(4, 4), np.outer(bt[0], bs[2]), atol=1e-12)
+    np.testing.assert_allclose(wDst[:16].reshape(4, 4), np.outer(bt[1], bs[1]), atol=1e-12)
+    np.testing.assert_allclose(wDtt[:16].reshape(4, 4), np.outer(bt[2], bs[0]), atol=1e-12)
+
+
+@pytest.mark.parametrize("s,t", [(0.2, 0.6), (0.0, 1.0), (0.75, 0.25)])
+def test_bilinear_basis_closed_form(s, t):
+    n, wP, wDs, wDt, wDss, wDst, wDtt = oracle.patch_basis(QUADS, 0, field1(), s, t, 2, np.float64)
+    assert n == 4
+    np.testing.assert_allclose(wP[:4], [(1 - s) * (1 - t), s * (1 - t), s * t, (1 - s) * t],
+                               atol=1e-15)
+    np.testing.assert_allclose(wDs[:4], [-(1 - t), 1 - t, t, -t], atol=1e-15)
+    np.testing.assert_allclose(wDt[:4], [-(1 - s), -s, s, 1 - s], atol=1e-15)
+    np.testing.assert_allclose(wDst[:4], [1, -1, 1, -1], atol=1e-15)
+    np.testing.assert_allclose(wDss[:4], 0, atol=0)
+
+
+GREG_BND = {0: (0, 0), 1: (1, 0), 7: (2, 0), 5: (3, 0), 2: (0, 1), 6: (3, 1), 16: (0, 2),
+            12: (3, 2), 15: (0, 3), 17: (1, 3), 11: (2, 3), 10: (3, 3)}
+GREG_INT = {(3, 4): (1, 1), (8, 9): (2, 1), (13, 14): (2, 2), (18, 19): (1, 2)}
+
+
+@pytest.mark.parametrize("s,t", [(0.3, 0.4), (0.9, 0.05), (0.5, 0.5), (0.01, 0.99)])
+def test_gregory_reduces_to_bicubic_bezier(s, t):
+    """With each interior pair merged, the Gregory basis is the Bernstein
+    tensor product (value and the approximate first/second derivatives)."""
+    n, wP, wDs, wDt, wDss, wDst, wDtt = oracle.patch_basis(GREGORY, 0, field1(regular=0), s, t, 2,
+                                                            np.float64)
+    assert n == 20
+    B = [bernstein(s, k) for k in range(3)]
+    T = [bernstein(t, k) for k in range(3)]
+    for (w, ds, dt) in [(wP, 0, 0), (wDs, 1, 0), (wDt, 0, 1), (wDss, 2, 0), (wDst, 1, 1),
+                        (wDtt, 0, 2)]:
+        for p, (c, r) in GREG_BND.items():
+            assert abs(w[p] - B[ds][c] * T[dt][r]) < 1e-12
+        for (p, m), (c, r) in GREG_INT.items():
+            assert abs(w[p] + w[m] - B[ds][c] * T[dt][r]) < 1e-12
+    # the rational blend is not trivial away from the corners
+    assert 0.0 < wP[3] / (wP[3] + wP[4]) < 1.0
+
+
+@pytest.mark.parametrize("ptype", [REGULAR, GREGORY, QUADS])
+@pytest.mark.parametrize("depth,u,v", [(0, 0, 0), (2, 1, 3), (3, 7, 0)])
+def test_partition_of_unity_and_zero_sum_derivatives(ptype, depth, u, v):
+    rng = np.random.default_rng(depth * 31 + ptype)
+    f1 = field1(depth, u, v, regular=int(ptype == REGULAR))
+    for _ in range(8):
+        s = (u + rng.uniform()) / (1 << depth)
+        t = (v + rng.uniform()) / (1 << depth)
+        n, wP, wDs, wDt, wDss, wDst, wDtt = oracle.patch_basis(ptype, 0, f1, s, t, 2, np.float64)
+        assert abs(wP[:n].sum() - 1) < 1e-13
+        for w in (wDs, wDt, wDss, wDst, wDtt):
+            assert abs(w[:n].sum()) < 1e-9
+
+
+@pytest.mark.parametrize("ptype", [REGULAR, QUADS])
+@pytest.mark.parametrize("boundary", [0, 1, 2, 4, 8, 3, 12, 9])
+def test_first_derivatives_match_finite_differences(ptype, boundary):
+    """Exact derivatives for the polynomial bases, including the 2^depth
+    scaling from PatchParam normalization (depth 2 here)."""
+    depth, u, v = 2, 1, 2
+    f1 = field1(depth, u, v, boundary=boundary if ptype == REGULAR else 0)
+    h = 1e-6
+    for s0, t0 in [(0.3, 0.6), (0.41, 0.52), (0.37, 0.74)]:
+        s, t = s0, t0
+        n, wP, wDs, wDt, *_ = oracle.patch_basis(ptype, 0, f1, s, t, 1, np.float64)
+        _, a, *_ = oracle.patch_basis(ptype, 0, f1, s + h, t, 0, np.float64)
+        _, b, *_ = oracle.patch_basis(ptype, 0, f1, s - h, t, 0, np.float64)
+        np.testing.assert_allclose(wDs[:n], (a - b)[:n] / (2 * h), atol=1e-6)
+        _, a, *_ = oracle.patch_basis(ptype, 0, f1, s, t + h, 0, np.float64)
+        _, b, *_ = oracle.patch_basis(ptype, 0, f1, s, t - h, 0, np.float64)
+        np.testing.assert_allclose(wDt[:n], (a - b)[:n] / (2 * h), atol=1e-6)
+
+
+@pytest.mark.parametrize("boundary", [1, 2, 4, 8, 3, 6, 12, 9, 15])
+def test_boundary_fold_is_reflection_of_phantom_points(boundary):
+    """Folded weights on the 16 points == unfolded weights on points whose
+    boundary row/column is replaced by the reflection 2*P1 - P2."""
+    rng = np.random.default_rng(boundary)
+    P = rng.normal(size=(4, 4, 3))              # [row(t), col(s)]
+    Q = P.copy()
+    if boundary & 1:
+        Q[0, :] = 2 * Q[1, :] - Q[2, :]
+    if boundary & 2:
+        Q[:, 3] = 2 * Q[:, 2] - Q[:, 1]
+    if boundary & 4:
+        Q[3, :] = 2 * Q[2, :] - Q[1, :]
+    if boundary & 8:
+        Q[:, 0] = 2 * Q[:, 1] - Q[:, 2]
+    if (boundary & 1) and (boundary & 2):       # corner: reflect the reflected row
+        Q[0, 3] = 2 * Q[0, 2] - Q[0, 1]
+    if (boundary & 4) and (boundary & 2):
+        Q[3, 3] = 2 * Q[3, 2] - Q[3, 1]
+    if (boundary & 4) and (boundary & 8):
+        Q[3, 0] = 2 * Q[3, 1] - Q[3, 2]
+    if (boundary & 1) and (boundary & 8):
+        Q[0, 0] = 2 * Q[0, 1] - Q[0, 2]
+    for s, t in [(0.2, 0.3), (0.7, 0.9), (0.5, 0.05)]:
+        _, wf, dsf, dtf, *_ = oracle.patch_basis(REGULAR, 0, field1(boundary=boundary), s, t, 1,
+                                                 np.float64)
+        _, w0, ds0, dt0, *_ = oracle.patch_basis(REGULAR, 0, field1(), s, t, 1, np.float64)
+        for a, b in [(wf, w0), (dsf, ds0), (dtf, dt0)]:
+            lhs = np.einsum("k,kd->d", a[:16], P.reshape(16, 3))
+            rhs = np.einsum("k,kd->d", b[:16], Q.reshape(16, 3))
+            np.testing.assert_allclose(lhs, rhs, atol=1e-12)
+        # folded weights vanish on the dropped row/column
+        W = wf[:16].reshape(4, 4)
+        if boundary & 1:
+            assert np.all(W[0] == 0)
+        if boundary & 8:
+            assert np.all(W[:, 0] == 0)
+
+
+def test_normalization_maps_ptex_space_to_patch_space():
+    """A depth-d patch at (u,v) evaluated at ptex (s,t) equals the depth-0
+    basis at (s 2^d - u, t 2^d - v), first derivatives x 2^d, second x 4^d."""
+    d, u, v = 3, 5, 2
+    for s_loc, t_loc in [(0.25, 0.75), (0.6, 0.1)]:
+        s = (u + s_loc) / 8
+        t = (v + t_loc) / 8
+        a = oracle.patch_basis(REGULAR, 0, field1(d, u, v), s, t, 2, np.float64)
+        b = oracle.patch_basis(REGULAR, 0, field1(), s_loc, t_loc, 2, np.float64)
+        np.testing.assert_allclose(a[1], b[1], atol=1e-13)
+        np.testing.assert_allclose(a[2], b[2] * 8, atol=1e-11)
+        np.testing.assert_allclose(a[3], b[3] * 8, atol=1e-11)
+        np.testing.assert_allclose(a[4], b[4] * 64, atol=1e-9)
+
+
+def test_unsupported_patch_type_returns_zero_points():
+    assert oracle.patch_basis(5, 0, field1(), 0.3, 0.3)[0] == 0
+
+
+# ---------------------------------------------------------------- brute force
+
+def random_csr(rng, n, nsrc, max_size=16):
+    sizes = rng.integers(0, max_size + 1, size=n).astype(np.int32)
+    offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int32)
+    nnz = int(sizes.sum())
+    idx = rng.integers(0, nsrc, size=nnz).astype(np.int32)
+    w = rng.uniform(-1, 1, size=nnz)
+    return sizes, offsets, idx, w
+
+
+@pytest.mark.parametrize("L", [1, 3, 4, 7, 8, 12])
+def test_stencils_brute_force(L):
+    rng = np.random.default_rng(L)
+    sizes, offsets, idx, w = random_csr(rng, 300, 50)
+    src = rng.normal(size=(50, L))
+    ref = np.zeros((300, L))
+    for i in range(300):
+        for j in range(sizes[i]):
+            ref[i] += w[offsets[i] + j] * src[idx[offsets[i] + j]]
+    out64 = oracle.apply_stencils(src, sizes, offsets, idx, w, np.float64)
+    np.testing.assert_allclose(out64, ref, atol=1e-12)
+    out32 = oracle.apply_stencils(src, sizes, offsets, idx, w.astype(np.float32))
+    assert bbox_rel_err(out32, ref) < 1e-5
+
+
+def test_stencils_start_end_row_convention_and_errors():
+    rng = np.random.default_rng(5)
+    sizes, offsets, idx, w = random_csr(rng, 40, 10)
+    w = w.astype(np.float32)
+    src = rng.normal(size=(10, 3)).astype(np.float32)
+    full = oracle.apply_stencils(src, sizes, offsets, idx, w)
+    out = np.zeros((40, 3), np.float32)
+    # CPU generic-path convention: stencils [start,end) -> rows 0..end-start-1
+    assert oracle.eval_stencils(src, (0, 3, 3), out, (0, 3, 3), sizes, offsets, idx, w, 10, 25)
+    np.testing.assert_array_equal(out[:15], full[10:25])
+    assert np.all(out[15:] == 0)
+    # no-op and length mismatch (osd/cpuEvaluator.cpp:46-47)
+    assert oracle.eval_stencils(src, (0, 3, 3), out, (0, 3, 3), sizes, offsets, idx, w, 5, 5)
+    assert not oracle.eval_stencils(src, (0, 3, 3), out, (0, 2, 3), sizes, offsets, idx, w, 0, 4)
+
+
+def test_planted_mistake_is_caught_by_the_closed_form():
+    """A plausible oracle bug (B-spline basis with columns reversed, i.e.
+    transposed s/t roles) must fail the Cox-de Boor check."""
+    n, wP, *_ = oracle.patch_basis(REGULAR, 0, field1(), 0.2, 0.7, 0, np.float64)
+    wrong = wP[:16].reshape(4, 4).T
+    exact = np.outer(cox_de_boor(0.7), cox_de_boor(0.2))
+    assert np.allclose(wP[:16].reshape(4, 4), exact, atol=1e-14)
+    assert not np.allclose(wrong, exact, atol=1e-6)
