@@ -1,0 +1,67 @@
+/*
+ * osdfar.h -- C ABI of the host table compiler (opensubdiv_amd/lib/libosdfar.so).
+ *
+ * Builds, on the host, the tables the HIP evaluator consumes, for Catmark
+ * meshes: uniform refinement with Far's vertex numbering, factorized stencil
+ * tables, limit stencil tables with derivatives, patch tables
+ * (B-spline patches + Gregory-basis end caps) and a patch map.  All pointers
+ * are HOST pointers.  Functions returning int return 0 on success and a
+ * negative value on invalid input.
+ *
+ * Interfaces restated (OpenSubdiv v3.6.0, paths relative to its tree):
+ *   far/topologyDescriptor.h:40-80      TopologyDescriptor (input mesh)
+ *   far/topologyRefiner.h:118-193       TopologyRefiner::RefineUniform
+ *   far/stencilTableFactory.h:55-130    StencilTableFactoryReal<REAL>::Create
+ *   far/stencilTable.h:141-297          StencilTableReal<REAL> (sizes/offsets/indices/weights)
+ */
+#ifndef OSDFAR_H
+#define OSDFAR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct osdfar_topology osdfar_topology;
+typedef struct osdfar_stencil_table osdfar_stencil_table;
+
+/* TopologyDescriptor + Sdc::Options::VtxBoundaryInterpolation
+ * (0 none, 1 edge only, 2 edge and corner); creasing is uniform. Returns
+ * NULL for input the compiler does not handle (non-manifold edges). */
+osdfar_topology *osdfar_topology_create(int numVertices, int numFaces,
+                                        const int *numVertsPerFace,
+                                        const int *vertIndicesPerFace,
+                                        int numCreases, const int *creaseVertexIndexPairs,
+                                        const float *creaseWeights,
+                                        int numCorners, const int *cornerVertexIndices,
+                                        const float *cornerWeights,
+                                        int boundaryInterpolation);
+void osdfar_topology_destroy(osdfar_topology *topo);
+
+/* TopologyRefiner::RefineUniform(level). */
+int osdfar_refine_uniform(osdfar_topology *topo, int maxLevel);
+/* Counts of a refinement level (level 0 = control mesh). */
+int osdfar_level_info(const osdfar_topology *topo, int level, int *numVertices,
+                      int *numFaces, int *numEdges);
+/* Face-vertex indices of a level (4 per face above level 0). */
+int osdfar_level_face_vertices(const osdfar_topology *topo, int level, int *out);
+
+/* StencilTableFactory::Create: mode 0 vertex / 1 varying; intermediate
+ * levels on/off; control-vertex identity stencils on/off; double weights
+ * when useDouble != 0. */
+osdfar_stencil_table *osdfar_stencil_table_create(const osdfar_topology *topo, int mode,
+                                                  int intermediateLevels, int controlVerts,
+                                                  int useDouble);
+/* numStencils, nnz (64-bit), control vertex count, weight arrays (1, 3, 6). */
+int osdfar_stencil_table_info(const osdfar_stencil_table *t, int *numStencils,
+                              long long *nnz, int *numControlVertices, int *numWeightArrays);
+/* Copies the arrays (each pointer may be NULL).  offsets32 requires
+ * nnz < 2^31.  which = 0 P, 1 du, 2 dv, 3 duu, 4 duv, 5 dvv. */
+int osdfar_stencil_table_copy(const osdfar_stencil_table *t, int *sizes, int *offsets32,
+                              long long *offsets64, int *indices, int which,
+                              float *weights, double *weights64);
+void osdfar_stencil_table_destroy(osdfar_stencil_table *t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

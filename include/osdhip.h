@@ -106,6 +106,41 @@ int osdhip_eval_stencils_d2(const float *src, osdhip_buffer_desc srcDesc,
                             const float *dvvWeights,
                             int start, int end, void *stream);
 
+/* ---- device stencil tables (HipStencilTable) ------------------------------
+ * replaces CudaStencilTable (osd/cudaEvaluator.h:52-91,
+ * osd/cudaEvaluator.cpp:75-145): owns the device copy of a Far
+ * (Limit)StencilTable.  Besides the Far CSR arrays (returned by
+ * osdhip_stencil_table_csr for the raw entry points) it holds a 64-row
+ * sliced layout (column-major per slice) that osdhip_stencil_table_eval
+ * streams with fully coalesced loads. */
+typedef struct osdhip_stencil_table osdhip_stencil_table;
+
+/* HOST arrays in; numWeightArrays = 1 (P), 3 (P, du, dv) or 6 (+ duu, duv,
+ * dvv); weights[k] = host pointer of array k; offsets may be NULL (then the
+ * exclusive prefix sum of sizes).  Returns NULL on invalid input or when a
+ * device allocation fails. */
+osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *sizes,
+                                                  const int *offsets, const int *indices,
+                                                  int numWeightArrays,
+                                                  const float *const *weights, void *stream);
+void osdhip_stencil_table_destroy(osdhip_stencil_table *table);
+int osdhip_stencil_table_info(const osdhip_stencil_table *table, int *numStencils,
+                              long long *numEntries, int *numWeightArrays,
+                              long long *deviceBytes);
+/* DEVICE pointers of the Far-layout arrays; weights receives 6 pointers
+ * (NULL for arrays the table does not have). */
+int osdhip_stencil_table_csr(const osdhip_stencil_table *table, const int **sizes,
+                             const int **offsets, const int **indices, const float **weights);
+/* EvalStencils over stencils [start, end) of a device table (templated
+ * CudaEvaluator::EvalStencils, osd/cudaEvaluator.h:125-143, 217-241,
+ * 352-385): numOutputs = 1, 3 or 6 (at most the table's weight arrays);
+ * outputs[k] / outputDescs[k] as the raw entry points; NULL outputs are
+ * skipped.  One launch for all outputs. */
+int osdhip_stencil_table_eval(const osdhip_stencil_table *table, const float *src,
+                              osdhip_buffer_desc srcDesc, int numOutputs, float *const *outputs,
+                              const osdhip_buffer_desc *outputDescs, int start, int end,
+                              void *stream);
+
 /* ---- patches -------------------------------------------------------------
  * replaces CudaEvaluator::EvalPatches raw, all three arities
  * (osd/cudaEvaluator.h:700-830) and extern "C" CudaEvalPatches /
@@ -129,6 +164,11 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
                         const int *patchIndices,
                         const osdhip_patch_param *params,
                         void *stream);
+
+/* Tuning knobs; returns the previous value (or an error code).
+ *   "stencil_kernel": 0 = sliced layout (default), 1 = CSR with LDS staging
+ *                     for osdhip_stencil_table_eval. */
+int osdhip_config(const char *key, int value);
 
 /* ---- device memory and streams -------------------------------------------
  * Used by the C++ HipVertexBuffer / HipStencilTable / HipPatchTable

@@ -44,6 +44,12 @@ _HIP_SIGS = {
                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "osdhip_eval_patches": [_vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D,
                             _i, _vp, _vp, _vp, _vp, _vp],
+    "osdhip_stencil_table_create": [_i, _vp, _vp, _vp, _i, _vp, _vp],
+    "osdhip_stencil_table_destroy": [_vp],
+    "osdhip_stencil_table_info": [_vp, _vp, _vp, _vp, _vp],
+    "osdhip_stencil_table_csr": [_vp, _vp, _vp, _vp, _vp],
+    "osdhip_stencil_table_eval": [_vp, _vp, _D, _i, _vp, _vp, _i, _i, _vp],
+    "osdhip_config": [C.c_char_p, _i],
     "osdhip_malloc": [C.POINTER(C.c_void_p), C.c_size_t],
     "osdhip_free": [_vp],
     "osdhip_memcpy_htod": [_vp, _vp, C.c_size_t, _vp],
@@ -73,7 +79,14 @@ def load_hip():
     for name, args in _HIP_SIGS.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        fn.restype = C.c_char_p if name == "osdhip_version" else C.c_int
+        if name == "osdhip_version":
+            fn.restype = C.c_char_p
+        elif name == "osdhip_stencil_table_create":
+            fn.restype = C.c_void_p
+        elif name == "osdhip_stencil_table_destroy":
+            fn.restype = None
+        else:
+            fn.restype = C.c_int
     _hip = lib
     return lib
 

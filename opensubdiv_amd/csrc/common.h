@@ -1,0 +1,41 @@
+// Shared device helpers for the gfx950 (CDNA4, wave64) kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace osdhip {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWave;
+constexpr int kNumXcd = 8;
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Wave-scope LDS hand-off: LDS traffic of one wave is processed in order,
+// so only the compiler has to be kept from reordering across this point.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Hardware dispatches work-group b to XCD (b % 8), each XCD with its own L2.
+// Remap so XCD x handles the contiguous logical range [x*per, (x+1)*per):
+// neighbouring stencils (sharing control points) then share one L2.
+// gridDim.x is always a multiple of 8.
+__device__ __forceinline__ int xcd_logical_block()
+{
+    const int per = (int)gridDim.x / kNumXcd;
+    return ((int)blockIdx.x % kNumXcd) * per + (int)blockIdx.x / kNumXcd;
+}
+
+inline int grid_for_blocks(int nblocks)
+{
+    return (nblocks + kNumXcd - 1) / kNumXcd * kNumXcd;
+}
+
+}  // namespace osdhip

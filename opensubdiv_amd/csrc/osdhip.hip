@@ -2,264 +2,70 @@
 // and the C ABI declared in include/osdhip.h.
 //
 // Stencil evaluation is a sparse gather (CSR rows x a dense multi-column
-// source), HBM-bound at < 0.3 FLOP/byte, so it uses no MFMA.  Each 64-lane
-// wavefront owns 64 consecutive stencils: it stages the wave's contiguous
-// slice of the index / weight streams into LDS with coalesced non-temporal
-// loads (the streams are read exactly once), then every lane reduces its own
-// stencil from LDS while gathering control points through L2.  All weight
-// arrays of a derivative table (P, du, dv [, duu, duv, dvv]) share the index
-// stream and are applied in ONE launch.  Work-groups are remapped so each of
-// the 8 XCDs (each with its own L2) walks a contiguous range of stencils.
+// source), HBM-bound at < 0.3 FLOP/byte, so it uses no MFMA.  Two kernels:
+//   * stencil_csr.h    - raw Far CSR tables (the raw EvalStencils entry
+//                        points): per-wave LDS staging of the table slice;
+//   * stencil_sliced.h - HipStencilTable's device layout (64-row column-major
+//                        slices): coalesced streaming without LDS.
+// All weight arrays of a derivative table (P, du, dv [, duu, duv, dvv])
+// share the index stream and are applied in ONE launch.
 //
-// Patch evaluation computes the B-spline / Gregory / bilinear basis in
-// registers per PatchCoord and gathers the 16 / 20 / 4 control points.
+// Patch evaluation (patch_kernels.h) computes the B-spline / Gregory /
+// bilinear basis in registers per PatchCoord and gathers 16 / 20 / 4 points.
 //
 // Parity: built with -ffp-contract=off (no fused multiply-add) and every
 // sum accumulates w * x in the reference's index order starting from 0, so
 // stencil and patch results equal Osd::CpuEvaluator's bit for bit.
 #include <hip/hip_runtime.h>
 
-#include <climits>
-#include <cstdint>
+#include <algorithm>
 #include <cstring>
+#include <new>
+#include <vector>
 
 #include "../../include/osdhip.h"
-#include "patch_basis_device.h"
+#include "common.h"
+#include "patch_kernels.h"
+#include "stencil_csr.h"
+#include "stencil_sliced.h"
 
 namespace osdhip {
 
-constexpr int kWave = 64;
-constexpr int kBlock = 256;
-constexpr int kWavesPerBlock = kBlock / kWave;
-constexpr int kNumXcd = 8;
+// Tuning knobs (osdhip_config).
+static int g_stencil_kernel = 0;     // table eval: 0 sliced, 1 CSR
 
-// Wave-scope LDS hand-off: LDS traffic of one wave is processed in order,
-// so only the compiler has to be kept from reordering across this point.
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// Hardware dispatches work-group b to XCD (b % 8).  Map it to a logical
-// block so that XCD x handles the contiguous range [x*per, (x+1)*per).
-// gridDim.x is always a multiple of 8.
-__device__ __forceinline__ int xcd_logical_block()
-{
-    const int per = (int)gridDim.x / kNumXcd;
-    return ((int)blockIdx.x % kNumXcd) * per + (int)blockIdx.x / kNumXcd;
-}
-
-struct StencilArgs {
-    const float *src;       // includes srcDesc.offset and the element window start
-    int srcStride;
-    float *dst[6];          // include their desc offsets (+ element window start)
-    int dstStride[6];
-    const int *sizes, *offsets, *indices;
-    const float *w[6];
-    int start, count;       // stencils [start, start+count); output row = i - start
-    int nblocks;            // logical blocks that hold stencils
-};
-
-// NW weight arrays (1: P, 3: P/du/dv, 6: + duu/duv/dvv); LC floats per vertex
-// handled by this launch; CAP = table entries staged per wave.
-template <int NW, int LC, int CAP>
-__global__ __launch_bounds__(kBlock) void k_stencils(StencilArgs a)
-{
-    __shared__ int s_idx[kWavesPerBlock][CAP];
-    __shared__ float s_w[kWavesPerBlock][NW][CAP];
-
-    const int lane = (int)threadIdx.x & (kWave - 1);
-    const int wv = (int)threadIdx.x / kWave;
-    const int lb = xcd_logical_block();
-    if (lb >= a.nblocks) return;
-    const int first = lb * kBlock + wv * kWave;     // relative to a.start
-    if (first >= a.count) return;                    // whole wave idle
-    const int cnt = min(kWave, a.count - first);
-    const bool active = lane < cnt;
-    const int i = a.start + first + lane;
-
-    int off = 0, sz = 0;
-    if (active) {
-        off = a.offsets[i];
-        sz = a.sizes[i];
-    }
-    // Table slice [lo, hi) covering every stencil of this wave.
-    int lo = active ? off : INT_MAX;
-    int hi = active ? off + sz : INT_MIN;
-#pragma unroll
-    for (int m = kWave / 2; m >= 1; m >>= 1) {
-        lo = min(lo, __shfl_xor(lo, m));
-        hi = max(hi, __shfl_xor(hi, m));
-    }
-    const int n = hi - lo;
-
-    float acc[NW][LC];
-#pragma unroll
-    for (int w = 0; w < NW; ++w)
-#pragma unroll
-        for (int k = 0; k < LC; ++k) acc[w][k] = 0.0f;
-
-    const float *__restrict__ src = a.src;
-    const size_t sstride = (size_t)a.srcStride;
-
-    if (n <= CAP) {
-        // Coalesced, streaming (non-temporal) copy of the slice into LDS.
-        for (int e = lane; e < n; e += kWave) {
-            s_idx[wv][e] = __builtin_nontemporal_load(a.indices + lo + e);
-#pragma unroll
-            for (int w = 0; w < NW; ++w)
-                s_w[wv][w][e] = __builtin_nontemporal_load(a.w[w] + lo + e);
-        }
-        wave_lds_sync();
-        const int base = off - lo;
-        for (int j = 0; j < sz; ++j) {
-            const int idx = s_idx[wv][base + j];
-            const float *p = src + (size_t)(unsigned)idx * sstride;
-            float v[LC];
-#pragma unroll
-            for (int k = 0; k < LC; ++k) v[k] = p[k];
-#pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                const float ww = s_w[wv][w][base + j];
-#pragma unroll
-                for (int k = 0; k < LC; ++k) acc[w][k] += v[k] * ww;
-            }
-        }
-    } else {
-        // Slice larger than the staging buffer: read the table directly.
-        for (int j = 0; j < sz; ++j) {
-            const int idx = a.indices[off + j];
-            const float *p = src + (size_t)(unsigned)idx * sstride;
-            float v[LC];
-#pragma unroll
-            for (int k = 0; k < LC; ++k) v[k] = p[k];
-#pragma unroll
-            for (int w = 0; w < NW; ++w) {
-                const float ww = a.w[w][off + j];
-#pragma unroll
-                for (int k = 0; k < LC; ++k) acc[w][k] += v[k] * ww;
-            }
-        }
-    }
-
-    if (active) {
-        const size_t row = (size_t)(first + lane);
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            float *d = a.dst[w];
-            if (!d) continue;
-            d += row * (size_t)a.dstStride[w];
-#pragma unroll
-            for (int k = 0; k < LC; ++k) d[k] = acc[w][k];
-        }
-    }
-}
-
-struct PatchArgs {
-    const float *src;       // includes srcDesc.offset and the element window start
-    int srcStride;
-    float *dst[6];          // P, du, dv, duu, duv, dvv (NULL = skip)
-    int dstStride[6];
-    int n;
-    const osdhip_patch_coord *coords;
-    const osdhip_patch_array *arrays;
-    const int *indices;
-    const osdhip_patch_param *params;
-};
-
-template <bool D1, bool D2, int LC>
-__global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
-{
-    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
-    if (i >= a.n) return;
-
-    const osdhip_patch_coord c = a.coords[i];
-    const osdhip_patch_array arr = a.arrays[c.arrayIndex];
-    const osdhip_patch_param pp = a.params[c.patchIndex];
-    const unsigned f1 = pp.field1;
-    const int type = ((f1 >> 5) & 1u) ? arr.regDesc : arr.desc;
-    const int depth = (int)(f1 & 0xfu);
-    const int nonquad = (int)((f1 >> 4) & 1u);
-    const int boundary = (int)((f1 >> 7) & 0x1fu);
-    const int pv = (int)((f1 >> 12) & 0x3ffu);
-    const int pu = (int)((f1 >> 22) & 0x3ffu);
-
-    // PatchParam::Normalize: ptex-face (s,t) -> patch-local (s,t)
-    const float frac_inv = (float)(1 << (depth - nonquad));
-    const float s = c.s * frac_inv - (float)pu;
-    const float t = c.t * frac_inv - (float)pv;
-
-    Basis<D1, D2> B;
-    B.n = 0;
-    if (type == OSDHIP_PATCH_REGULAR) basis_bspline<D1, D2>(s, t, boundary, B);
-    else if (type == OSDHIP_PATCH_GREGORY_BASIS) basis_gregory<D1, D2>(s, t, B);
-    else if (type == OSDHIP_PATCH_QUADS) basis_linear<D1, D2>(s, t, B);
-
-    const int *cvs = a.indices + arr.indexBase + arr.stride * (c.patchIndex - arr.primitiveIdBase);
-    constexpr int NO = 1 + (D1 ? 2 : 0) + (D2 ? 3 : 0);
-    float acc[NO][LC];
-#pragma unroll
-    for (int o = 0; o < NO; ++o)
-#pragma unroll
-        for (int k = 0; k < LC; ++k) acc[o][k] = 0.0f;
-
-    const size_t sstride = (size_t)a.srcStride;
-#pragma unroll
-    for (int j = 0; j < 20; ++j) {
-        if (j < B.n) {
-            const float *p = a.src + (size_t)(unsigned)cvs[j] * sstride;
-            float v[LC];
-#pragma unroll
-            for (int k = 0; k < LC; ++k) v[k] = p[k];
-#pragma unroll
-            for (int k = 0; k < LC; ++k) {
-                acc[0][k] += v[k] * B.p[j];
-                if constexpr (D1) {
-                    acc[1][k] += v[k] * B.ds[j];
-                    acc[2][k] += v[k] * B.dt[j];
-                }
-                if constexpr (D2) {
-                    acc[3][k] += v[k] * B.dss[j];
-                    acc[4][k] += v[k] * B.dst[j];
-                    acc[5][k] += v[k] * B.dtt[j];
-                }
-            }
-        }
-    }
-    // 2^depth (first) and 4^depth (second derivatives): exact power-of-two
-    // factors, applied to the sums instead of to every weight.
-    const float k1 = (float)(1 << depth);
-    const float k2 = k1 * k1;
-#pragma unroll
-    for (int o = 0; o < NO; ++o) {
-        float *d = a.dst[o];
-        if (!d) continue;
-        const float f = o == 0 ? 1.0f : (o < 3 ? k1 : k2);
-        d += (size_t)i * (size_t)a.dstStride[o];
-#pragma unroll
-        for (int k = 0; k < LC; ++k) d[k] = acc[o][k] * f;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// host-side dispatch
-
-template <int NW, int CAP>
-static int launch_stencils_lc(int lc, const StencilArgs &a, int grid, hipStream_t st)
+template <int NW, int CAP, bool VEC>
+static void launch_csr(int lc, const StencilArgs &a, int grid, hipStream_t st)
 {
     switch (lc) {
-    case 1: hipLaunchKernelGGL((k_stencils<NW, 1, CAP>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_stencils<NW, 2, CAP>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 3: hipLaunchKernelGGL((k_stencils<NW, 3, CAP>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 4: hipLaunchKernelGGL((k_stencils<NW, 4, CAP>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 5: hipLaunchKernelGGL((k_stencils<NW, 5, CAP>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 6: hipLaunchKernelGGL((k_stencils<NW, 6, CAP>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 7: hipLaunchKernelGGL((k_stencils<NW, 7, CAP>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 8: hipLaunchKernelGGL((k_stencils<NW, 8, CAP>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    default: return OSDHIP_ERR_UNSUPPORTED;
+    case 1: hipLaunchKernelGGL((k_stencils_csr<NW, 1, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_stencils_csr<NW, 2, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_stencils_csr<NW, 3, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_stencils_csr<NW, 4, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 5: hipLaunchKernelGGL((k_stencils_csr<NW, 5, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 6: hipLaunchKernelGGL((k_stencils_csr<NW, 6, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 7: hipLaunchKernelGGL((k_stencils_csr<NW, 7, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_stencils_csr<NW, 8, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
     }
+}
+
+template <int NW>
+static void launch_sliced(int lc, const SlicedArgs &a, int grid, hipStream_t st)
+{
+    switch (lc) {
+    case 1: hipLaunchKernelGGL((k_stencils_sliced<NW, 1>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_stencils_sliced<NW, 2>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_stencils_sliced<NW, 3>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_stencils_sliced<NW, 4>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 5: hipLaunchKernelGGL((k_stencils_sliced<NW, 5>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 6: hipLaunchKernelGGL((k_stencils_sliced<NW, 6>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 7: hipLaunchKernelGGL((k_stencils_sliced<NW, 7>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_stencils_sliced<NW, 8>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    }
+}
+
+static int last_error()
+{
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? OSDHIP_OK : (int)e;
 }
@@ -270,25 +76,32 @@ static bool desc_ok(const osdhip_buffer_desc &d)
            d.length <= d.stride - (d.offset % d.stride);
 }
 
-// NW outputs, outs[w]/ods[w] may have NULL pointers (skipped).
-template <int NW>
-static int eval_stencils_impl(const float *src, osdhip_buffer_desc sd,
-                              float *const *outs, const osdhip_buffer_desc *ods,
-                              const int *sizes, const int *offsets,
-                              const int *indices, const float *const *ws,
-                              int start, int end, void *stream)
+static bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+
+// Validates outputs against the source descriptor (CpuEvaluator's checks:
+// osd/cpuEvaluator.cpp:46-47, 70-72); NULL outputs are skipped.
+static bool outputs_ok(const osdhip_buffer_desc &sd, int nout, float *const *outs,
+                       const osdhip_buffer_desc *ods, const float *const *ws)
 {
-    if (end <= start) return OSDHIP_OK;
-    if (!src || !sizes || !offsets || !indices || start < 0) return OSDHIP_ERR_ARGS;
-    if (!desc_ok(sd)) return OSDHIP_ERR_ARGS;
+    if (!desc_ok(sd)) return false;
     bool any = false;
-    for (int w = 0; w < NW; ++w) {
+    for (int w = 0; w < nout; ++w) {
         if (!outs[w]) continue;
-        if (ods[w].length != sd.length || !desc_ok(ods[w]) || !ws[w]) return OSDHIP_ERR_ARGS;
+        if (ods[w].length != sd.length || !desc_ok(ods[w]) || (ws && !ws[w])) return false;
         any = true;
     }
-    if (!any) return OSDHIP_ERR_ARGS;
+    return any;
+}
 
+// NW outputs over a raw CSR table.
+template <int NW>
+static int eval_csr(const float *src, osdhip_buffer_desc sd, float *const *outs,
+                    const osdhip_buffer_desc *ods, const int *sizes, const int *offsets,
+                    const int *indices, const float *const *ws, int start, int end, void *stream)
+{
+    if (end <= start) return OSDHIP_OK;
+    if (!src || !sizes || !offsets || !indices || start < 0 || !ws[0]) return OSDHIP_ERR_ARGS;
+    if (!outputs_ok(sd, NW, outs, ods, ws)) return OSDHIP_ERR_ARGS;
     const int count = end - start;
     StencilArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -299,28 +112,37 @@ static int eval_stencils_impl(const float *src, osdhip_buffer_desc sd,
     a.start = start;
     a.count = count;
     a.nblocks = (count + kBlock - 1) / kBlock;
-    const int grid = (a.nblocks + kNumXcd - 1) / kNumXcd * kNumXcd;
+    bool vec = aligned16(indices);
+    for (int w = 0; w < NW; ++w) {
+        // a skipped output still needs a readable weight stream for the
+        // staging loop: reuse the P weights (their products are dropped)
+        a.w[w] = (outs[w] && ws[w]) ? ws[w] : ws[0];
+        vec = vec && aligned16(a.w[w]);
+    }
+    const int grid = grid_for_blocks(a.nblocks);
     hipStream_t st = (hipStream_t)stream;
     // Elements beyond 8 per vertex are handled by further launches over the
     // same stencils (windows of up to 8 floats).
     for (int e0 = 0; e0 < sd.length; e0 += 8) {
-        const int lc = sd.length - e0 < 8 ? sd.length - e0 : 8;
+        const int lc = std::min(8, sd.length - e0);
         a.src = src + sd.offset + e0;
         for (int w = 0; w < NW; ++w) {
             a.dst[w] = outs[w] ? outs[w] + ods[w].offset + e0 : nullptr;
             a.dstStride[w] = outs[w] ? ods[w].stride : 0;
-            // a skipped output still needs a readable weight stream for the
-            // staging loop; reuse the P weights (their products are dropped)
-            a.w[w] = ws[w] ? ws[w] : ws[0];
         }
-        int rc = launch_stencils_lc<NW, NW == 6 ? 512 : 1024>(lc, a, grid, st);
+        // SLOTS = CAP + 4 entries per wave: 32 KB per 256-thread block for
+        // NW = 1, i.e. five blocks per 160 KB CU
+        constexpr int CAP = NW == 6 ? 508 : 1020;
+        if (vec) launch_csr<NW, CAP, true>(lc, a, grid, st);
+        else launch_csr<NW, CAP, false>(lc, a, grid, st);
+        int rc = last_error();
         if (rc != OSDHIP_OK) return rc;
     }
     return OSDHIP_OK;
 }
 
 template <bool D1, bool D2>
-static int launch_patches_lc(int lc, const PatchArgs &a, hipStream_t st)
+static void launch_patches(int lc, const PatchArgs &a, hipStream_t st)
 {
     const int grid = (a.n + kBlock - 1) / kBlock;
     switch (lc) {
@@ -331,16 +153,47 @@ static int launch_patches_lc(int lc, const PatchArgs &a, hipStream_t st)
     case 5: hipLaunchKernelGGL((k_patches<D1, D2, 5>), dim3(grid), dim3(kBlock), 0, st, a); break;
     case 6: hipLaunchKernelGGL((k_patches<D1, D2, 6>), dim3(grid), dim3(kBlock), 0, st, a); break;
     case 7: hipLaunchKernelGGL((k_patches<D1, D2, 7>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 8: hipLaunchKernelGGL((k_patches<D1, D2, 8>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    default: return OSDHIP_ERR_UNSUPPORTED;
+    default: hipLaunchKernelGGL((k_patches<D1, D2, 8>), dim3(grid), dim3(kBlock), 0, st, a); break;
     }
-    hipError_t e = hipGetLastError();
-    return e == hipSuccess ? OSDHIP_OK : (int)e;
 }
 
 }  // namespace osdhip
 
 using namespace osdhip;
+
+// Device copy of a (limit) stencil table: the Far CSR arrays (for the raw
+// getters) plus the 64-row sliced layout the table kernel streams.
+struct osdhip_stencil_table {
+    int n = 0, nw = 0, nslices = 0;
+    long long nnz = 0, sliceEntries = 0, bytes = 0;
+    int *sizes = nullptr, *offsets = nullptr, *indices = nullptr;
+    float *w[6] = {};
+    long long *sliceOff = nullptr;
+    int *sliceWidth = nullptr;
+    int *sidx = nullptr;
+    float *sw[6] = {};
+};
+
+static void free_table(osdhip_stencil_table *t)
+{
+    if (!t) return;
+    void *ptrs[] = { t->sizes, t->offsets, t->indices, t->w[0], t->w[1], t->w[2], t->w[3],
+                     t->w[4], t->w[5], t->sliceOff, t->sliceWidth, t->sidx, t->sw[0], t->sw[1],
+                     t->sw[2], t->sw[3], t->sw[4], t->sw[5] };
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete t;
+}
+
+template <typename T>
+static bool upload(T **dptr, const T *host, size_t count, hipStream_t st, long long &bytes)
+{
+    *dptr = nullptr;
+    if (count == 0) return true;
+    if (hipMalloc((void **)dptr, count * sizeof(T)) != hipSuccess) { *dptr = nullptr; return false; }
+    bytes += (long long)(count * sizeof(T));
+    return hipMemcpyAsync(*dptr, host, count * sizeof(T), hipMemcpyHostToDevice, st) == hipSuccess;
+}
 
 extern "C" {
 
@@ -355,8 +208,7 @@ int osdhip_eval_stencils(const float *src, osdhip_buffer_desc srcDesc,
     float *outs[1] = { dst };
     osdhip_buffer_desc ods[1] = { dstDesc };
     const float *ws[1] = { weights };
-    return eval_stencils_impl<1>(src, srcDesc, outs, ods, sizes, offsets, indices, ws,
-                                 start, end, stream);
+    return eval_csr<1>(src, srcDesc, outs, ods, sizes, offsets, indices, ws, start, end, stream);
 }
 
 int osdhip_eval_stencils_d1(const float *src, osdhip_buffer_desc srcDesc,
@@ -372,9 +224,7 @@ int osdhip_eval_stencils_d1(const float *src, osdhip_buffer_desc srcDesc,
     float *outs[3] = { dst, du, dv };
     osdhip_buffer_desc ods[3] = { dstDesc, duDesc, dvDesc };
     const float *ws[3] = { weights, duWeights, dvWeights };
-    if (!ws[0]) return OSDHIP_ERR_ARGS;
-    return eval_stencils_impl<3>(src, srcDesc, outs, ods, sizes, offsets, indices, ws,
-                                 start, end, stream);
+    return eval_csr<3>(src, srcDesc, outs, ods, sizes, offsets, indices, ws, start, end, stream);
 }
 
 int osdhip_eval_stencils_d2(const float *src, osdhip_buffer_desc srcDesc,
@@ -395,9 +245,156 @@ int osdhip_eval_stencils_d2(const float *src, osdhip_buffer_desc srcDesc,
     float *outs[6] = { dst, du, dv, duu, duv, dvv };
     osdhip_buffer_desc ods[6] = { dstDesc, duDesc, dvDesc, duuDesc, duvDesc, dvvDesc };
     const float *ws[6] = { weights, duWeights, dvWeights, duuWeights, duvWeights, dvvWeights };
-    if (!ws[0]) return OSDHIP_ERR_ARGS;
-    return eval_stencils_impl<6>(src, srcDesc, outs, ods, sizes, offsets, indices, ws,
-                                 start, end, stream);
+    return eval_csr<6>(src, srcDesc, outs, ods, sizes, offsets, indices, ws, start, end, stream);
+}
+
+osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *sizes,
+                                                  const int *offsets, const int *indices,
+                                                  int numWeightArrays,
+                                                  const float *const *weights, void *stream)
+{
+    if (numStencils < 0 || (numStencils > 0 && (!sizes || !indices || !weights)) ||
+        (numWeightArrays != 1 && numWeightArrays != 3 && numWeightArrays != 6))
+        return nullptr;
+    for (int k = 0; k < numWeightArrays; ++k)
+        if (numStencils > 0 && !weights[k]) return nullptr;
+    osdhip_stencil_table *t = new (std::nothrow) osdhip_stencil_table;
+    if (!t) return nullptr;
+    t->n = numStencils;
+    t->nw = numWeightArrays;
+    hipStream_t st = (hipStream_t)stream;
+
+    // CSR mirror (offsets regenerated when absent: exclusive prefix of sizes)
+    std::vector<int> off(numStencils);
+    long long run = 0;
+    for (int i = 0; i < numStencils; ++i) {
+        if (sizes[i] < 0) { free_table(t); return nullptr; }
+        off[i] = offsets ? offsets[i] : (int)run;
+        run += sizes[i];
+    }
+    long long nnz = 0;
+    for (int i = 0; i < numStencils; ++i) nnz = std::max(nnz, (long long)off[i] + sizes[i]);
+    t->nnz = nnz;
+    bool ok = upload(&t->sizes, sizes, (size_t)numStencils, st, t->bytes) &&
+              upload(&t->offsets, off.data(), (size_t)numStencils, st, t->bytes) &&
+              upload(&t->indices, indices, (size_t)nnz, st, t->bytes);
+    for (int k = 0; ok && k < numWeightArrays; ++k)
+        ok = upload(&t->w[k], weights[k], (size_t)nnz, st, t->bytes);
+
+    // sliced layout: slice s holds rows [64s, 64s+64), column-major, padded
+    const int S = (numStencils + kWave - 1) / kWave;
+    t->nslices = S;
+    std::vector<long long> sOff(S + 1, 0);
+    std::vector<int> sW(S, 0);
+    for (int s = 0; s < S; ++s) {
+        int w = 0;
+        for (int r = s * kWave; r < std::min(numStencils, (s + 1) * kWave); ++r) w = std::max(w, sizes[r]);
+        sW[s] = w;
+        sOff[s + 1] = sOff[s] + (long long)w * kWave;
+    }
+    t->sliceEntries = sOff[S];
+    std::vector<int> pidx(ok ? (size_t)t->sliceEntries : 0, 0);
+    std::vector<float> pw;
+    if (ok) {
+        for (int s = 0; s < S; ++s)
+            for (int r = s * kWave; r < std::min(numStencils, (s + 1) * kWave); ++r)
+                for (int j = 0; j < sizes[r]; ++j)
+                    pidx[sOff[s] + (long long)j * kWave + (r - s * kWave)] = indices[off[r] + j];
+        ok = upload(&t->sliceOff, sOff.data(), (size_t)S, st, t->bytes) &&
+             upload(&t->sliceWidth, sW.data(), (size_t)S, st, t->bytes) &&
+             upload(&t->sidx, pidx.data(), pidx.size(), st, t->bytes) &&
+             hipStreamSynchronize(st) == hipSuccess;
+    }
+    for (int k = 0; ok && k < numWeightArrays; ++k) {
+        pw.assign((size_t)t->sliceEntries, 0.0f);
+        for (int s = 0; s < S; ++s)
+            for (int r = s * kWave; r < std::min(numStencils, (s + 1) * kWave); ++r)
+                for (int j = 0; j < sizes[r]; ++j)
+                    pw[sOff[s] + (long long)j * kWave + (r - s * kWave)] = weights[k][off[r] + j];
+        ok = upload(&t->sw[k], pw.data(), pw.size(), st, t->bytes) &&
+             hipStreamSynchronize(st) == hipSuccess;
+    }
+    if (!ok) { free_table(t); return nullptr; }
+    return t;
+}
+
+void osdhip_stencil_table_destroy(osdhip_stencil_table *t) { free_table(t); }
+
+int osdhip_stencil_table_info(const osdhip_stencil_table *t, int *numStencils,
+                              long long *numEntries, int *numWeightArrays, long long *deviceBytes)
+{
+    if (!t) return OSDHIP_ERR_ARGS;
+    if (numStencils) *numStencils = t->n;
+    if (numEntries) *numEntries = t->nnz;
+    if (numWeightArrays) *numWeightArrays = t->nw;
+    if (deviceBytes) *deviceBytes = t->bytes;
+    return OSDHIP_OK;
+}
+
+int osdhip_stencil_table_csr(const osdhip_stencil_table *t, const int **sizes, const int **offsets,
+                             const int **indices, const float **weights)
+{
+    if (!t) return OSDHIP_ERR_ARGS;
+    if (sizes) *sizes = t->sizes;
+    if (offsets) *offsets = t->offsets;
+    if (indices) *indices = t->indices;
+    if (weights)
+        for (int k = 0; k < 6; ++k) weights[k] = t->w[k];
+    return OSDHIP_OK;
+}
+
+int osdhip_stencil_table_eval(const osdhip_stencil_table *t, const float *src,
+                              osdhip_buffer_desc srcDesc, int numOutputs, float *const *outputs,
+                              const osdhip_buffer_desc *outputDescs, int start, int end,
+                              void *stream)
+{
+    if (!t || !outputs || !outputDescs) return OSDHIP_ERR_ARGS;
+    if (numOutputs != 1 && numOutputs != 3 && numOutputs != 6) return OSDHIP_ERR_ARGS;
+    if (numOutputs > t->nw) return OSDHIP_ERR_ARGS;
+    end = std::min(end, t->n);
+    if (end <= start) return OSDHIP_OK;
+    if (!src || start < 0 || !outputs_ok(srcDesc, numOutputs, outputs, outputDescs, nullptr))
+        return OSDHIP_ERR_ARGS;
+    if (g_stencil_kernel == 1) {
+        const float *ws[6] = { t->w[0], t->w[1], t->w[2], t->w[3], t->w[4], t->w[5] };
+        if (numOutputs == 1)
+            return eval_csr<1>(src, srcDesc, outputs, outputDescs, t->sizes, t->offsets, t->indices,
+                               ws, start, end, stream);
+        if (numOutputs == 3)
+            return eval_csr<3>(src, srcDesc, outputs, outputDescs, t->sizes, t->offsets, t->indices,
+                               ws, start, end, stream);
+        return eval_csr<6>(src, srcDesc, outputs, outputDescs, t->sizes, t->offsets, t->indices,
+                           ws, start, end, stream);
+    }
+    SlicedArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.srcStride = srcDesc.stride;
+    a.rowSize = t->sizes;
+    a.sliceOff = t->sliceOff;
+    a.sliceWidth = t->sliceWidth;
+    a.idx = t->sidx;
+    for (int k = 0; k < numOutputs; ++k) a.w[k] = t->sw[k];
+    a.rowBegin = start;
+    a.rowEnd = end;
+    a.sliceBegin = start / kWave;
+    a.nslices = (end + kWave - 1) / kWave - a.sliceBegin;
+    a.nblocks = (a.nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int grid = grid_for_blocks(a.nblocks);
+    hipStream_t st = (hipStream_t)stream;
+    for (int e0 = 0; e0 < srcDesc.length; e0 += 8) {
+        const int lc = std::min(8, srcDesc.length - e0);
+        a.src = src + srcDesc.offset + e0;
+        for (int k = 0; k < numOutputs; ++k) {
+            a.dst[k] = outputs[k] ? outputs[k] + outputDescs[k].offset + e0 : nullptr;
+            a.dstStride[k] = outputs[k] ? outputDescs[k].stride : 0;
+        }
+        if (numOutputs == 1) launch_sliced<1>(lc, a, grid, st);
+        else if (numOutputs == 3) launch_sliced<3>(lc, a, grid, st);
+        else launch_sliced<6>(lc, a, grid, st);
+        int rc = last_error();
+        if (rc != OSDHIP_OK) return rc;
+    }
+    return OSDHIP_OK;
 }
 
 int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
@@ -414,16 +411,10 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
                         const osdhip_patch_param *params,
                         void *stream)
 {
-    if (!src || !desc_ok(srcDesc)) return OSDHIP_ERR_ARGS;
+    if (!src) return OSDHIP_ERR_ARGS;
     float *outs[6] = { dst, du, dv, duu, duv, dvv };
     osdhip_buffer_desc ods[6] = { dstDesc, duDesc, dvDesc, duuDesc, duvDesc, dvvDesc };
-    bool any = false;
-    for (int o = 0; o < 6; ++o) {
-        if (!outs[o]) continue;
-        if (ods[o].length != srcDesc.length || !desc_ok(ods[o])) return OSDHIP_ERR_ARGS;
-        any = true;
-    }
-    if (!any) return OSDHIP_ERR_ARGS;
+    if (!outputs_ok(srcDesc, 6, outs, ods, nullptr)) return OSDHIP_ERR_ARGS;
     if (numPatchCoords <= 0) return OSDHIP_OK;
     if (!coords || !arrays || !patchIndices || !params) return OSDHIP_ERR_ARGS;
 
@@ -439,19 +430,31 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
     a.params = params;
     hipStream_t st = (hipStream_t)stream;
     for (int e0 = 0; e0 < srcDesc.length; e0 += 8) {
-        const int lc = srcDesc.length - e0 < 8 ? srcDesc.length - e0 : 8;
+        const int lc = std::min(8, srcDesc.length - e0);
         a.src = src + srcDesc.offset + e0;
         for (int o = 0; o < 6; ++o) {
             a.dst[o] = outs[o] ? outs[o] + ods[o].offset + e0 : nullptr;
             a.dstStride[o] = outs[o] ? ods[o].stride : 0;
         }
-        int rc;
-        if (d2) rc = launch_patches_lc<true, true>(lc, a, st);
-        else if (d1) rc = launch_patches_lc<true, false>(lc, a, st);
-        else rc = launch_patches_lc<false, false>(lc, a, st);
+        if (d2) launch_patches<true, true>(lc, a, st);
+        else if (d1) launch_patches<true, false>(lc, a, st);
+        else launch_patches<false, false>(lc, a, st);
+        int rc = last_error();
         if (rc != OSDHIP_OK) return rc;
     }
     return OSDHIP_OK;
+}
+
+int osdhip_config(const char *key, int value)
+{
+    if (!key) return OSDHIP_ERR_ARGS;
+    if (std::strcmp(key, "stencil_kernel") == 0) {
+        if (value != 0 && value != 1) return OSDHIP_ERR_ARGS;
+        int old = g_stencil_kernel;
+        g_stencil_kernel = value;
+        return old;
+    }
+    return OSDHIP_ERR_UNSUPPORTED;
 }
 
 int osdhip_malloc(void **ptr, size_t bytes)
@@ -512,6 +515,6 @@ int osdhip_device_info(char *buf, int buflen)
     return n;
 }
 
-const char *osdhip_version(void) { return "osdhip 0.1 gfx950"; }
+const char *osdhip_version(void) { return "osdhip 0.2 gfx950"; }
 
 }  // extern "C"

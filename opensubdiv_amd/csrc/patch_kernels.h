@@ -1,0 +1,100 @@
+// EvalPatches: limit evaluation at PatchCoords (osd/cpuEvaluator.cpp:
+// 157-381).  One lane per PatchCoord: PatchParam decode and Normalize, the
+// B-spline / Gregory / bilinear basis in registers (patch_basis_device.h),
+// then the 16 / 20 / 4-point gather for P and each requested derivative.
+// Sums accumulate in the reference's point order without FMA; derivative
+// sums are scaled by 2^depth / 4^depth (exact powers of two).
+#pragma once
+#include "../../include/osdhip.h"
+#include "common.h"
+#include "patch_basis_device.h"
+
+namespace osdhip {
+
+struct PatchArgs {
+    const float *src;       // includes srcDesc.offset and the element window start
+    int srcStride;
+    float *dst[6];          // P, du, dv, duu, duv, dvv (NULL = skip)
+    int dstStride[6];
+    int n;
+    const osdhip_patch_coord *coords;
+    const osdhip_patch_array *arrays;
+    const int *indices;
+    const osdhip_patch_param *params;
+};
+
+template <bool D1, bool D2, int LC>
+__global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
+{
+    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (i >= a.n) return;
+
+    const osdhip_patch_coord c = a.coords[i];
+    const osdhip_patch_array arr = a.arrays[c.arrayIndex];
+    const osdhip_patch_param pp = a.params[c.patchIndex];
+    const unsigned f1 = pp.field1;
+    const int type = ((f1 >> 5) & 1u) ? arr.regDesc : arr.desc;
+    const int depth = (int)(f1 & 0xfu);
+    const int nonquad = (int)((f1 >> 4) & 1u);
+    const int boundary = (int)((f1 >> 7) & 0x1fu);
+    const int pv = (int)((f1 >> 12) & 0x3ffu);
+    const int pu = (int)((f1 >> 22) & 0x3ffu);
+
+    // PatchParam::Normalize: ptex-face (s,t) -> patch-local (s,t)
+    const float frac_inv = (float)(1 << (depth - nonquad));
+    const float s = c.s * frac_inv - (float)pu;
+    const float t = c.t * frac_inv - (float)pv;
+
+    Basis<D1, D2> B;
+    B.n = 0;
+    if (type == OSDHIP_PATCH_REGULAR) basis_bspline<D1, D2>(s, t, boundary, B);
+    else if (type == OSDHIP_PATCH_GREGORY_BASIS) basis_gregory<D1, D2>(s, t, B);
+    else if (type == OSDHIP_PATCH_QUADS) basis_linear<D1, D2>(s, t, B);
+
+    const int *cvs = a.indices + arr.indexBase + arr.stride * (c.patchIndex - arr.primitiveIdBase);
+    constexpr int NO = 1 + (D1 ? 2 : 0) + (D2 ? 3 : 0);
+    float acc[NO][LC];
+#pragma unroll
+    for (int o = 0; o < NO; ++o)
+#pragma unroll
+        for (int k = 0; k < LC; ++k) acc[o][k] = 0.0f;
+
+    const size_t sstride = (size_t)a.srcStride;
+#pragma unroll
+    for (int j = 0; j < 20; ++j) {
+        if (j < B.n) {
+            const float *p = a.src + (size_t)(unsigned)cvs[j] * sstride;
+            float v[LC];
+#pragma unroll
+            for (int k = 0; k < LC; ++k) v[k] = p[k];
+#pragma unroll
+            for (int k = 0; k < LC; ++k) {
+                acc[0][k] += v[k] * B.p[j];
+                if constexpr (D1) {
+                    acc[1][k] += v[k] * B.ds[j];
+                    acc[2][k] += v[k] * B.dt[j];
+                }
+                if constexpr (D2) {
+                    acc[3][k] += v[k] * B.dss[j];
+                    acc[4][k] += v[k] * B.dst[j];
+                    acc[5][k] += v[k] * B.dtt[j];
+                }
+            }
+        }
+    }
+    // 2^depth (first) and 4^depth (second derivatives): exact power-of-two
+    // factors, applied to the sums instead of to every weight.
+    const float k1 = (float)(1 << depth);
+    const float k2 = k1 * k1;
+#pragma unroll
+    for (int o = 0; o < NO; ++o) {
+        float *d = a.dst[o];
+        if (!d) continue;
+        const float f = o == 0 ? 1.0f : (o < 3 ? k1 : k2);
+        d += (size_t)i * (size_t)a.dstStride[o];
+#pragma unroll
+        for (int k = 0; k < LC; ++k) d[k] = acc[o][k] * f;
+    }
+}
+
+}  // namespace osdhip

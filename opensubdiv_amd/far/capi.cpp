@@ -1,0 +1,146 @@
+// extern "C" layer of libosdfar (include/osdfar.h).
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "../../include/osdfar.h"
+#include "stencils.h"
+
+using namespace osdfar;
+
+struct osdfar_topology {
+    Refiner refiner;
+};
+
+struct osdfar_stencil_table {
+    bool isDouble = false;
+    StencilTable f;
+    StencilTableD d;
+    int numWeightArrays = 1;
+};
+
+namespace {
+template <typename REAL>
+static int copyTable(const StencilTableT<REAL> &s, int *sizes, int *offsets32, long long *offsets64,
+                     int *indices, int which, float *w, double *wd)
+{
+    const std::vector<REAL> *arr[6] = { &s.weights, &s.du, &s.dv, &s.duu, &s.duv, &s.dvv };
+    if (which < 0 || which > 5) return -1;
+    if (offsets32 && s.nnz() >= (size_t)0x7fffffff) return -2;
+    const size_t n = s.sizes.size();
+    if (sizes) std::memcpy(sizes, s.sizes.data(), n * sizeof(int));
+    if (offsets32) for (size_t i = 0; i < n; ++i) offsets32[i] = (int)s.offsets[i];
+    if (offsets64) for (size_t i = 0; i < n; ++i) offsets64[i] = (long long)s.offsets[i];
+    if (indices) std::memcpy(indices, s.indices.data(), s.nnz() * sizeof(int));
+    const std::vector<REAL> &src = *arr[which];
+    if ((w || wd) && src.size() != s.nnz()) return -3;
+    if (w) for (size_t i = 0; i < src.size(); ++i) w[i] = (float)src[i];
+    if (wd) for (size_t i = 0; i < src.size(); ++i) wd[i] = (double)src[i];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+osdfar_topology *osdfar_topology_create(int numVertices, int numFaces,
+                                        const int *numVertsPerFace,
+                                        const int *vertIndicesPerFace,
+                                        int numCreases, const int *creaseVertexIndexPairs,
+                                        const float *creaseWeights,
+                                        int numCorners, const int *cornerVertexIndices,
+                                        const float *cornerWeights,
+                                        int boundaryInterpolation)
+{
+    if (numVertices <= 0 || numFaces <= 0 || !numVertsPerFace || !vertIndicesPerFace) return nullptr;
+    TopologyDesc d;
+    d.numVertices = numVertices;
+    d.numFaces = numFaces;
+    d.numVertsPerFace = numVertsPerFace;
+    d.vertIndicesPerFace = vertIndicesPerFace;
+    d.numCreases = creaseVertexIndexPairs && creaseWeights ? numCreases : 0;
+    d.creaseVertexIndexPairs = creaseVertexIndexPairs;
+    d.creaseWeights = creaseWeights;
+    d.numCorners = cornerVertexIndices && cornerWeights ? numCorners : 0;
+    d.cornerVertexIndices = cornerVertexIndices;
+    d.cornerWeights = cornerWeights;
+    d.boundaryInterpolation = boundaryInterpolation;
+    std::unique_ptr<osdfar_topology> t(new (std::nothrow) osdfar_topology);
+    if (!t || !t->refiner.create(d)) return nullptr;
+    return t.release();
+}
+
+void osdfar_topology_destroy(osdfar_topology *topo) { delete topo; }
+
+int osdfar_refine_uniform(osdfar_topology *topo, int maxLevel)
+{
+    if (!topo || maxLevel < 0 || maxLevel > 10) return -1;
+    if (topo->refiner.maxLevel() > maxLevel) return -1;
+    topo->refiner.refineUniform(maxLevel);
+    return 0;
+}
+
+int osdfar_level_info(const osdfar_topology *topo, int level, int *nv, int *nf, int *ne)
+{
+    if (!topo || level < 0 || level > topo->refiner.maxLevel()) return -1;
+    const Level &L = topo->refiner.level(level);
+    if (nv) *nv = L.nv;
+    if (nf) *nf = L.nf;
+    if (ne) *ne = L.ne;
+    return 0;
+}
+
+int osdfar_level_face_vertices(const osdfar_topology *topo, int level, int *out)
+{
+    if (!topo || !out || level < 0 || level > topo->refiner.maxLevel()) return -1;
+    const Level &L = topo->refiner.level(level);
+    std::memcpy(out, L.fv.data(), L.fv.size() * sizeof(int));
+    return 0;
+}
+
+osdfar_stencil_table *osdfar_stencil_table_create(const osdfar_topology *topo, int mode,
+                                                  int intermediateLevels, int controlVerts,
+                                                  int useDouble)
+{
+    if (!topo || (mode != 0 && mode != 1)) return nullptr;
+    std::unique_ptr<osdfar_stencil_table> t(new (std::nothrow) osdfar_stencil_table);
+    if (!t) return nullptr;
+    StencilOptions o;
+    o.interpolationMode = mode;
+    o.generateIntermediateLevels = intermediateLevels != 0;
+    o.generateControlVerts = controlVerts != 0;
+    t->isDouble = useDouble != 0;
+    if (t->isDouble) buildStencilTable(topo->refiner, o, t->d);
+    else buildStencilTable(topo->refiner, o, t->f);
+    return t.release();
+}
+
+int osdfar_stencil_table_info(const osdfar_stencil_table *t, int *numStencils, long long *nnz,
+                              int *numControlVertices, int *numWeightArrays)
+{
+    if (!t) return -1;
+    if (t->isDouble) {
+        if (numStencils) *numStencils = t->d.numStencils();
+        if (nnz) *nnz = (long long)t->d.nnz();
+        if (numControlVertices) *numControlVertices = t->d.numControlVertices;
+    } else {
+        if (numStencils) *numStencils = t->f.numStencils();
+        if (nnz) *nnz = (long long)t->f.nnz();
+        if (numControlVertices) *numControlVertices = t->f.numControlVertices;
+    }
+    if (numWeightArrays) *numWeightArrays = t->numWeightArrays;
+    return 0;
+}
+
+int osdfar_stencil_table_copy(const osdfar_stencil_table *t, int *sizes, int *offsets32,
+                              long long *offsets64, int *indices, int which,
+                              float *weights, double *weights64)
+{
+    if (!t) return -1;
+    return t->isDouble ? copyTable(t->d, sizes, offsets32, offsets64, indices, which, weights, weights64)
+                       : copyTable(t->f, sizes, offsets32, offsets64, indices, which, weights, weights64);
+}
+
+void osdfar_stencil_table_destroy(osdfar_stencil_table *t) { delete t; }
+
+}  // extern "C"

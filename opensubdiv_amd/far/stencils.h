@@ -1,0 +1,54 @@
+// Factorized stencil tables (every refined vertex as a weighted sum of
+// control vertices), the output format of Far::StencilTableFactoryReal
+// (far/stencilTable.h:141-266): sizes / offsets / indices / weights, CSR.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "topology.h"
+
+namespace osdfar {
+
+template <typename REAL>
+struct StencilTableT {
+    int numControlVertices = 0;
+    std::vector<int> sizes;
+    std::vector<int64_t> offsets;     // 64-bit: batched tables exceed 2^31 entries
+    std::vector<int> indices;
+    std::vector<REAL> weights;
+    // optional derivative weights (limit stencils), same shape as weights
+    std::vector<REAL> du, dv, duu, duv, dvv;
+    int numStencils() const { return (int)sizes.size(); }
+    size_t nnz() const { return indices.size(); }
+};
+
+using StencilTable = StencilTableT<float>;
+using StencilTableD = StencilTableT<double>;
+
+enum InterpolationMode { INTERPOLATE_VERTEX = 0, INTERPOLATE_VARYING = 1 };
+
+struct StencilOptions {
+    int interpolationMode = INTERPOLATE_VERTEX;
+    bool generateIntermediateLevels = false;
+    bool generateControlVerts = false;
+};
+
+// One sparse row per vertex of a level, entries sorted by control index.
+struct LevelStencils {
+    std::vector<int64_t> off;   // nv + 1
+    std::vector<int> idx;
+    std::vector<double> w;
+    int size(int v) const { return (int)(off[v + 1] - off[v]); }
+};
+
+// Computes factorized stencils of every level 1..maxLevel of the refiner.
+// levelStencils[l] holds level l (levelStencils[0] is empty: identity).
+void computeLevelStencils(const Refiner &r, int mode, std::vector<LevelStencils> &levelStencils);
+
+// Far::StencilTableFactory::Create equivalent (stencils sorted by control
+// index inside each row, weights accumulated in double then rounded).
+template <typename REAL>
+void buildStencilTable(const Refiner &r, const StencilOptions &opt, StencilTableT<REAL> &out);
+
+}  // namespace osdfar

@@ -1,0 +1,89 @@
+// Host-side topology levels and uniform Catmull-Clark refinement for the
+// table compiler (libosdfar).  Scope: manifold Catmark meshes with n-gon
+// faces at level 0, semi-sharp and infinitely sharp creases, corner
+// sharpness, uniform creasing and boundary edges (sharpened).
+//
+// Numbering follows OpenSubdiv's Vtr uniform refinement so refined vertex i
+// here is refined vertex i of Far (vtr/refinement.cpp:253-270: children of
+// vertices, then of faces, then of edges; vtr/quadRefinement.cpp:125-345 for
+// child faces / edges; vtr/level.cpp:1640-1790 for level-0 edges).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace osdfar {
+
+constexpr float kSharpInfinite = 10.0f;   // Sdc::Crease::SHARPNESS_INFINITE
+
+enum Rule { RULE_SMOOTH = 1, RULE_DART = 2, RULE_CREASE = 4, RULE_CORNER = 8 };
+
+struct Level {
+    int nv = 0, nf = 0, ne = 0;
+    std::vector<int> fvOff;     // nf + 1
+    std::vector<int> fv;        // face vertices (counter-clockwise)
+    std::vector<int> fe;        // face edges, fe[k] joins fv[k] and fv[k+1]
+    std::vector<int> ev;        // 2 per edge
+    std::vector<int> efOff, ef; // edge -> faces
+    std::vector<int> veOff, ve; // vertex -> edges
+    std::vector<int> vfOff, vf; // vertex -> faces
+    std::vector<float> esharp, vsharp;
+
+    int faceSize(int f) const { return fvOff[f + 1] - fvOff[f]; }
+    int valence(int v) const { return veOff[v + 1] - veOff[v]; }
+    int otherVertex(int e, int v) const { return ev[2 * e] == v ? ev[2 * e + 1] : ev[2 * e]; }
+    Rule vertexRule(int v) const;
+    void buildIncidence();      // ef / ve / vf from fv / fe / ev
+};
+
+struct TopologyDesc {
+    int numVertices = 0, numFaces = 0;
+    const int *numVertsPerFace = nullptr;
+    const int *vertIndicesPerFace = nullptr;
+    int numCreases = 0;
+    const int *creaseVertexIndexPairs = nullptr;
+    const float *creaseWeights = nullptr;
+    int numCorners = 0;
+    const int *cornerVertexIndices = nullptr;
+    const float *cornerWeights = nullptr;
+    int boundaryInterpolation = 1;   // 0 none, 1 edge only, 2 edge and corner
+};
+
+// Where a refined face came from: the base (ptex) face, depth and integer
+// (u,v) of its parametric square inside the ptex face (quads only).
+struct FaceOrigin {
+    int ptex;
+    uint16_t u, v;
+};
+
+class Refiner {
+public:
+    // Returns false for input the compiler does not handle (non-manifold
+    // edges, out-of-range indices).
+    bool create(const TopologyDesc &d);
+    void refineUniform(int maxLevel);
+
+    int maxLevel() const { return (int)levels.size() - 1; }
+    const Level &level(int l) const { return levels[l]; }
+    int numVerticesTotal() const;
+    int ptexFaceCount() const { return ptexCount; }
+
+    std::vector<Level> levels;
+    // per level >= 1: origin of every face (empty for level 0)
+    std::vector<std::vector<FaceOrigin>> origins;
+    // ptex index of each base face (one per quad; n-gons have n ptex faces)
+    std::vector<int> ptexBase;
+    int ptexCount = 0;
+
+private:
+    void refineOnce();
+};
+
+inline float decrementSharpness(float s)
+{
+    if (s <= 0.0f) return 0.0f;
+    if (s >= kSharpInfinite) return kSharpInfinite;
+    if (s > 1.0f) return s - 1.0f;
+    return 0.0f;
+}
+
+}  // namespace osdfar

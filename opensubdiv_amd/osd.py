@@ -110,7 +110,9 @@ class HipVertexBuffer:
 
 
 class HipStencilTable:
-    """Device mirror of a (Limit)StencilTable (osd/cudaEvaluator.h:52-91).
+    """Device mirror of a (Limit)StencilTable (osd/cudaEvaluator.h:52-91),
+    built by osdhip_stencil_table_create: the Far CSR arrays plus the 64-row
+    sliced layout the table kernel streams.
 
     Accepts any object with GetSizes/GetOffsets/GetControlIndices/GetWeights
     (and optionally GetDuWeights ... GetDvvWeights) returning array-likes, or
@@ -120,37 +122,65 @@ class HipStencilTable:
     _GETTERS = ["GetWeights", "GetDuWeights", "GetDvWeights", "GetDuuWeights",
                 "GetDuvWeights", "GetDvvWeights"]
 
-    def __init__(self, table):
+    def __init__(self, table, stream=None):
+        import ctypes
         get = (lambda k, g: table.get(k)) if isinstance(table, dict) else \
               (lambda k, g: getattr(table, g)() if hasattr(table, g) else None)
-        sizes = np.asarray(get("sizes", "GetSizes"), np.int32)
+        sizes = np.ascontiguousarray(get("sizes", "GetSizes"), np.int32)
         offsets = get("offsets", "GetOffsets")
-        offsets = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int32) \
-            if offsets is None or len(offsets) == 0 else np.asarray(offsets, np.int32)
-        self._n = int(sizes.shape[0])
-        self.sizes = _device_tensor(sizes)
-        self.offsets = _device_tensor(offsets)
-        self.indices = _device_tensor(np.asarray(get("indices", "GetControlIndices"), np.int32))
-        self.w = []
+        offsets = None if offsets is None or len(offsets) == 0 else \
+            np.ascontiguousarray(offsets, np.int32)
+        indices = np.ascontiguousarray(get("indices", "GetControlIndices"), np.int32)
+        ws = []
         for k, g in zip(self._KEYS, self._GETTERS):
             v = get(k, g)
-            self.w.append(None if v is None or len(v) == 0 else _device_tensor(np.asarray(v, np.float32)))
+            if v is None or len(v) == 0:
+                break
+            ws.append(np.ascontiguousarray(v, np.float32))
+        nw = 6 if len(ws) >= 6 else (3 if len(ws) >= 3 else 1)
+        ws = ws[:nw]
+        lib = _lib.load_hip()
+        wptrs = (ctypes.c_void_p * nw)(*[w.ctypes.data for w in ws])
+        self._n = int(sizes.shape[0])
+        self._nw = nw
+        self.h = lib.osdhip_stencil_table_create(
+            self._n, sizes.ctypes.data, None if offsets is None else offsets.ctypes.data,
+            indices.ctypes.data, nw, ctypes.cast(wptrs, ctypes.c_void_p), _stream(stream))
+        if not self.h:
+            raise MemoryError("osdhip_stencil_table_create failed")
+        p = [ctypes.c_void_p() for _ in range(3)]
+        w6 = (ctypes.c_void_p * 6)()
+        lib.osdhip_stencil_table_csr(self.h, *[ctypes.byref(x) for x in p],
+                                     ctypes.cast(w6, ctypes.c_void_p))
+        self._csr = [x.value for x in p]
+        self._w = list(w6)
 
     @classmethod
     def Create(cls, table, deviceContext=None):
-        return cls(table)
+        return cls(table, deviceContext)
 
-    def GetSizesBuffer(self): return self.sizes.data_ptr()
-    def GetOffsetsBuffer(self): return self.offsets.data_ptr()
-    def GetIndicesBuffer(self): return self.indices.data_ptr()
-    def _wp(self, k): return self.w[k].data_ptr() if self.w[k] is not None else None
-    def GetWeightsBuffer(self): return self._wp(0)
-    def GetDuWeightsBuffer(self): return self._wp(1)
-    def GetDvWeightsBuffer(self): return self._wp(2)
-    def GetDuuWeightsBuffer(self): return self._wp(3)
-    def GetDuvWeightsBuffer(self): return self._wp(4)
-    def GetDvvWeightsBuffer(self): return self._wp(5)
+    def GetSizesBuffer(self): return self._csr[0]
+    def GetOffsetsBuffer(self): return self._csr[1]
+    def GetIndicesBuffer(self): return self._csr[2]
+    def GetWeightsBuffer(self): return self._w[0]
+    def GetDuWeightsBuffer(self): return self._w[1]
+    def GetDvWeightsBuffer(self): return self._w[2]
+    def GetDuuWeightsBuffer(self): return self._w[3]
+    def GetDuvWeightsBuffer(self): return self._w[4]
+    def GetDvvWeightsBuffer(self): return self._w[5]
     def GetNumStencils(self): return self._n
+    def GetNumWeightArrays(self): return self._nw
+
+    def device_bytes(self):
+        import ctypes
+        b = ctypes.c_longlong()
+        _lib.load_hip().osdhip_stencil_table_info(self.h, None, None, None, ctypes.byref(b))
+        return b.value
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib._hip is not None:
+            _lib._hip.osdhip_stencil_table_destroy(self.h)
+            self.h = None
 
 
 class HipPatchTable:
@@ -203,31 +233,15 @@ class HipEvaluator:
         ctx = rest[2] if len(rest) > 2 else None
         if table.GetNumStencils() == 0:
             return False                   # osd/cpuEvaluator.h:82-83
-        s = _stream(ctx)
-        common = (table.GetSizesBuffer(), table.GetOffsetsBuffer(), table.GetIndicesBuffer())
-        n = table.GetNumStencils()
-        if len(outs) == 1:
-            rc = lib.osdhip_eval_stencils(_ptr(srcBuffer), _lib.desc(srcDesc), _ptr(dstBuffer),
-                                          _lib.desc(dstDesc), *common, table.GetWeightsBuffer(),
-                                          0, n, s)
-        elif len(outs) == 3:
-            args = []
-            for b, d in outs:
-                args += [_ptr(b), _lib.desc(d)]
-            rc = lib.osdhip_eval_stencils_d1(_ptr(srcBuffer), _lib.desc(srcDesc), *args, *common,
-                                             table.GetWeightsBuffer(), table.GetDuWeightsBuffer(),
-                                             table.GetDvWeightsBuffer(), 0, n, s)
-        elif len(outs) == 6:
-            args = []
-            for b, d in outs:
-                args += [_ptr(b), _lib.desc(d)]
-            rc = lib.osdhip_eval_stencils_d2(_ptr(srcBuffer), _lib.desc(srcDesc), *args, *common,
-                                             table.GetWeightsBuffer(), table.GetDuWeightsBuffer(),
-                                             table.GetDvWeightsBuffer(), table.GetDuuWeightsBuffer(),
-                                             table.GetDuvWeightsBuffer(), table.GetDvvWeightsBuffer(),
-                                             0, n, s)
-        else:
+        if len(outs) not in (1, 3, 6):
             raise TypeError("EvalStencils takes 1, 3 or 6 outputs")
+        import ctypes
+        ptrs = (ctypes.c_void_p * len(outs))(*[_ptr(b) for b, _ in outs])
+        descs = (_lib.Desc * len(outs))(*[_lib.desc(d) for _, d in outs])
+        rc = lib.osdhip_stencil_table_eval(table.h, _ptr(srcBuffer), _lib.desc(srcDesc), len(outs),
+                                           ctypes.cast(ptrs, ctypes.c_void_p),
+                                           ctypes.cast(descs, ctypes.c_void_p), 0,
+                                           table.GetNumStencils(), _stream(ctx))
         return rc == _lib.OK
 
     @staticmethod

@@ -202,3 +202,68 @@ def test_python_mirror_eval_stencils_on_vertex_buffer(hip):
     np.testing.assert_array_equal(vb.numpy()[8:], g["cpu_out"])
     assert not HipEvaluator.EvalStencils(vb, BufferDescriptor(0, 3, 3), vb,
                                          BufferDescriptor(8 * 3, 2, 3), table)
+
+
+# ---------------------------------------------------------- device tables
+
+def table_eval(hip, h, src, sd, outs, descs, start, end):
+    import ctypes
+    ptrs = (ctypes.c_void_p * len(outs))(*[ptr(o) for o in outs])
+    ds = (D * len(outs))(*[D(*d) for d in descs])
+    return hip.osdhip_stencil_table_eval(h, ptr(src), D(*sd), len(outs),
+                                         ctypes.cast(ptrs, ctypes.c_void_p),
+                                         ctypes.cast(ds, ctypes.c_void_p), start, end, stream())
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("nw,L", [(1, 3), (1, 7), (1, 12), (3, 3), (3, 7), (6, 4)])
+def test_device_table_bit_exact(hip, kernel, nw, L):
+    """HipStencilTable device layout (sliced / CSR-staged kernels) against the
+    oracle, including sub-ranges that start and end inside a 64-row slice."""
+    from opensubdiv_amd import HipStencilTable
+    rng = np.random.default_rng(300 + 10 * nw + L + kernel)
+    nsrc, n = 600, 4099
+    t = random_table(rng, n, nsrc, 18, nw)
+    keys = ("weights", "du", "dv", "duu", "duv", "dvv")[:nw]
+    table = HipStencilTable.Create(t)
+    assert table.GetNumWeightArrays() == nw
+    src = rng.normal(size=(nsrc, L)).astype(np.float32)
+    s_t = dev(src)
+    old = hip.osdhip_config(b"stencil_kernel", kernel)
+    try:
+        for start, end in [(0, n), (37, 2000), (64, 128), (4000, 4099)]:
+            outs = [empty(n * L) for _ in range(nw)]
+            assert table_eval(hip, table.h, s_t, (0, L, L), outs, [(0, L, L)] * nw, start,
+                              end) == 0
+            for k in range(nw):
+                want = oracle.apply_stencils(src, t["sizes"], t["offsets"], t["indices"],
+                                             t[keys[k]])[start:end]
+                got = host(outs[k], shape=(n, L))
+                np.testing.assert_array_equal(got[:end - start], want, err_msg=f"{keys[k]}")
+                assert np.all(np.isnan(got[end - start:]))
+    finally:
+        hip.osdhip_config(b"stencil_kernel", old)
+
+
+def test_device_table_cfg0_and_errors(hip):
+    from opensubdiv_amd import HipStencilTable
+    g = golden("cube_l3.npz")
+    table = HipStencilTable.Create({k: g[k] for k in ("sizes", "offsets", "indices", "weights")})
+    n = table.GetNumStencils()
+    src = dev(g["src"])
+    out = empty(n * 3)
+    assert table_eval(hip, table.h, src, (0, 3, 3), [out], [(0, 3, 3)], 0, n) == 0
+    np.testing.assert_array_equal(host(out, shape=(n, 3)), g["cpu_out"])
+    # more outputs than weight arrays, length mismatch, bad output count
+    assert table_eval(hip, table.h, src, (0, 3, 3), [out] * 3, [(0, 3, 3)] * 3, 0, n) == \
+        _lib.ERR_ARGS
+    assert table_eval(hip, table.h, src, (0, 3, 3), [out], [(0, 2, 3)], 0, n) == _lib.ERR_ARGS
+    assert table_eval(hip, table.h, src, (0, 3, 3), [out, out], [(0, 3, 3)] * 2, 0, n) == \
+        _lib.ERR_ARGS
+    # the raw getters expose the Far layout on the device
+    out2 = empty(n * 3)
+    assert hip.osdhip_eval_stencils(ptr(src), D(0, 3, 3), ptr(out2), D(0, 3, 3),
+                                    table.GetSizesBuffer(), table.GetOffsetsBuffer(),
+                                    table.GetIndicesBuffer(), table.GetWeightsBuffer(), 0, n,
+                                    stream()) == 0
+    np.testing.assert_array_equal(host(out2, shape=(n, 3)), g["cpu_out"])
