@@ -61,6 +61,44 @@ int osdfar_stencil_table_copy(const osdfar_stencil_table *t, int *sizes, int *of
                               float *weights, double *weights64);
 void osdfar_stencil_table_destroy(osdfar_stencil_table *t);
 
+/* ---- feature-adaptive patch tables --------------------------------------
+ * PatchTableFactory::Create(refiner, Options(isoLevel)) with
+ * ENDCAP_GREGORY_BASIS (far/patchTableFactory.h:40-160) for closed,
+ * manifold Catmark meshes with quad base faces and semi-sharp creases
+ * (returns NULL otherwise).  Refines the topology uniformly to isoLevel.
+ * The patch points live in a vertex buffer laid out as
+ *   [control vertices | refined vertices | end-cap points]
+ * whose non-control rows are given by osdfar_patch_table_point_stencils. */
+typedef struct osdfar_patch_table osdfar_patch_table;
+osdfar_patch_table *osdfar_patch_table_create(osdfar_topology *topo, int isoLevel);
+void osdfar_patch_table_destroy(osdfar_patch_table *t);
+int osdfar_patch_table_info(const osdfar_patch_table *t, int *numControl, int *numRefined,
+                            int *numLocal, int *numArrays, int *numIndices, int *numPatches,
+                            int *maxDepth);
+/* Osd::CpuPatchTable layout (osd/cpuPatchTable.cpp:35-158): PatchArray
+ * (6 ints each), patch vertex indices, PatchParam (field0, field1, sharpness). */
+int osdfar_patch_table_copy(const osdfar_patch_table *t, int *arrays, int *indices, void *params);
+/* Stencils of the refined and end-cap points (Far's appended local-point
+ * stencil table, far/stencilTableFactory.cpp:256-408). */
+osdfar_stencil_table *osdfar_patch_table_point_stencils(const osdfar_patch_table *t, int useDouble);
+/* PatchMap::FindPatch (far/patchMap.h:193-231) for n (ptex face, u, v)
+ * queries; writes Osd::PatchCoord records {arrayIndex, patchIndex,
+ * vertIndex, s, t} (arrayIndex = -1 when no patch covers the location).
+ * Returns the number of locations found. */
+int osdfar_patch_map_find(const osdfar_patch_table *t, int n, const int *ptex, const double *u,
+                          const double *v, void *patchCoords);
+/* LimitStencilTableFactory::Create (far/stencilTableFactory.cpp:411-662):
+ * one stencil per location with P, du, dv (and duu, duv, dvv when
+ * secondDerivatives) weights, factorized to control vertices. */
+osdfar_stencil_table *osdfar_limit_stencil_table_create(const osdfar_patch_table *t, int n,
+                                                        const int *ptex, const double *u,
+                                                        const double *v, int secondDerivatives,
+                                                        int useDouble);
+/* One (ptex face, u, v) location per vertex of uniform level `level` of a
+ * refined topology (the vertex's corner of an incident refined face). */
+int osdfar_vertex_locations(const osdfar_topology *topo, int level, int *ptex, double *u,
+                            double *v);
+
 #ifdef __cplusplus
 }
 #endif

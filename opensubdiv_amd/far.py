@@ -34,6 +34,14 @@ def load_far():
         "osdfar_stencil_table_info": ([vp, vp, vp, vp, vp], i),
         "osdfar_stencil_table_copy": ([vp, vp, vp, vp, vp, i, vp, vp], i),
         "osdfar_stencil_table_destroy": ([vp], None),
+        "osdfar_patch_table_create": ([vp, i], vp),
+        "osdfar_patch_table_destroy": ([vp], None),
+        "osdfar_patch_table_info": ([vp, vp, vp, vp, vp, vp, vp, vp], i),
+        "osdfar_patch_table_copy": ([vp, vp, vp, vp], i),
+        "osdfar_patch_table_point_stencils": ([vp, i], vp),
+        "osdfar_patch_map_find": ([vp, i, vp, vp, vp, vp], i),
+        "osdfar_limit_stencil_table_create": ([vp, i, vp, vp, vp, i, i], vp),
+        "osdfar_vertex_locations": ([vp, i, vp, vp, vp], i),
     }
     for k, (a, r) in sig.items():
         f = getattr(L, k)
@@ -127,7 +135,82 @@ class Topology:
         finally:
             load_far().osdfar_stencil_table_destroy(h)
 
+    def vertex_locations(self, level):
+        """(ptex, u, v) of every vertex of uniform level `level`."""
+        nv, _, _ = self.level_info(level)
+        p = np.zeros(nv, np.int32)
+        u = np.zeros(nv, np.float64)
+        v = np.zeros(nv, np.float64)
+        assert load_far().osdfar_vertex_locations(self.h, level, _p(p), _p(u), _p(v)) == 0
+        return p, u, v
+
+    def patch_table(self, iso_level):
+        return PatchTable(self, iso_level)
+
     def __del__(self):
         if getattr(self, "h", None) and _far is not None:
             _far.osdfar_topology_destroy(self.h)
+            self.h = None
+
+
+class PatchTable:
+    """Feature-adaptive patch table (B-spline + Gregory end caps), its
+    patch-point stencils and patch map.  Vertex buffer layout:
+    [control | refined | end-cap points]."""
+
+    def __init__(self, topo, iso_level):
+        from .osd import PATCH_ARRAY_DTYPE, PATCH_PARAM_DTYPE
+        L = load_far()
+        self._topo = topo
+        self.h = L.osdfar_patch_table_create(topo.h, iso_level)
+        if not self.h:
+            raise ValueError("mesh outside the patch compiler's scope")
+        v = [C.c_int() for _ in range(7)]
+        L.osdfar_patch_table_info(self.h, *[C.byref(x) for x in v])
+        (self.num_control, self.num_refined, self.num_local, na, ni, npch,
+         self.max_depth) = [x.value for x in v]
+        self.arrays = np.zeros(na, PATCH_ARRAY_DTYPE)
+        self.indices = np.zeros(ni, np.int32)
+        self.params = np.zeros(npch, PATCH_PARAM_DTYPE)
+        L.osdfar_patch_table_copy(self.h, _p(self.arrays), _p(self.indices), _p(self.params))
+
+    @property
+    def num_vertices_total(self):
+        return self.num_control + self.num_refined + self.num_local
+
+    def point_stencils(self, double=False):
+        h = load_far().osdfar_patch_table_point_stencils(self.h, int(double))
+        try:
+            return _read_table(h, double)
+        finally:
+            load_far().osdfar_stencil_table_destroy(h)
+
+    def find(self, ptex, u, v):
+        """PatchMap::FindPatch -> PATCH_COORD_DTYPE records (arrayIndex -1
+        where no patch covers the location)."""
+        from .osd import PATCH_COORD_DTYPE
+        ptex = np.ascontiguousarray(ptex, np.int32)
+        u = np.ascontiguousarray(u, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        out = np.zeros(len(ptex), PATCH_COORD_DTYPE)
+        load_far().osdfar_patch_map_find(self.h, len(ptex), _p(ptex), _p(u), _p(v), _p(out))
+        return out
+
+    def limit_stencils(self, ptex, u, v, second=False, double=False):
+        ptex = np.ascontiguousarray(ptex, np.int32)
+        u = np.ascontiguousarray(u, np.float64)
+        v = np.ascontiguousarray(v, np.float64)
+        h = load_far().osdfar_limit_stencil_table_create(self.h, len(ptex), _p(ptex), _p(u),
+                                                         _p(v), int(second), int(double))
+        try:
+            return _read_table(h, double)
+        finally:
+            load_far().osdfar_stencil_table_destroy(h)
+
+    def flat(self):
+        return {"arrays": self.arrays, "indices": self.indices, "params": self.params}
+
+    def __del__(self):
+        if getattr(self, "h", None) and _far is not None:
+            _far.osdfar_patch_table_destroy(self.h)
             self.h = None

@@ -4,7 +4,7 @@
 #include <new>
 
 #include "../../include/osdfar.h"
-#include "stencils.h"
+#include "patches.h"
 
 using namespace osdfar;
 
@@ -17,6 +17,10 @@ struct osdfar_stencil_table {
     StencilTable f;
     StencilTableD d;
     int numWeightArrays = 1;
+};
+
+struct osdfar_patch_table {
+    PatchTable table;
 };
 
 namespace {
@@ -142,5 +146,101 @@ int osdfar_stencil_table_copy(const osdfar_stencil_table *t, int *sizes, int *of
 }
 
 void osdfar_stencil_table_destroy(osdfar_stencil_table *t) { delete t; }
+
+osdfar_patch_table *osdfar_patch_table_create(osdfar_topology *topo, int isoLevel)
+{
+    if (!topo || isoLevel < 0 || isoLevel > 10) return nullptr;
+    if (topo->refiner.maxLevel() > isoLevel) return nullptr;
+    std::unique_ptr<osdfar_patch_table> t(new (std::nothrow) osdfar_patch_table);
+    if (!t || !t->table.build(topo->refiner, isoLevel)) return nullptr;
+    return t.release();
+}
+
+void osdfar_patch_table_destroy(osdfar_patch_table *t) { delete t; }
+
+int osdfar_patch_table_info(const osdfar_patch_table *t, int *numControl, int *numRefined,
+                            int *numLocal, int *numArrays, int *numIndices, int *numPatches,
+                            int *maxDepth)
+{
+    if (!t) return -1;
+    const PatchTable &p = t->table;
+    if (numControl) *numControl = p.numControl;
+    if (numRefined) *numRefined = p.numRefined;
+    if (numLocal) *numLocal = p.numLocal;
+    if (numArrays) *numArrays = (int)p.arrays.size();
+    if (numIndices) *numIndices = (int)p.indices.size();
+    if (numPatches) *numPatches = (int)p.params.size();
+    if (maxDepth) *maxDepth = p.maxDepth;
+    return 0;
+}
+
+int osdfar_patch_table_copy(const osdfar_patch_table *t, int *arrays, int *indices, void *params)
+{
+    if (!t) return -1;
+    const PatchTable &p = t->table;
+    if (arrays) std::memcpy(arrays, p.arrays.data(), p.arrays.size() * sizeof(PatchArrayRec));
+    if (indices) std::memcpy(indices, p.indices.data(), p.indices.size() * sizeof(int));
+    if (params) std::memcpy(params, p.params.data(), p.params.size() * sizeof(PatchParamRec));
+    return 0;
+}
+
+osdfar_stencil_table *osdfar_patch_table_point_stencils(const osdfar_patch_table *t, int useDouble)
+{
+    if (!t) return nullptr;
+    std::unique_ptr<osdfar_stencil_table> s(new (std::nothrow) osdfar_stencil_table);
+    if (!s) return nullptr;
+    s->isDouble = useDouble != 0;
+    if (s->isDouble) t->table.pointStencilTable(s->d);
+    else t->table.pointStencilTable(s->f);
+    return s.release();
+}
+
+int osdfar_patch_map_find(const osdfar_patch_table *t, int n, const int *ptex, const double *u,
+                          const double *v, void *patchCoords)
+{
+    if (!t || n < 0 || (n > 0 && (!ptex || !u || !v || !patchCoords))) return -1;
+    struct Coord { int arrayIndex, patchIndex, vertIndex; float s, t; };
+    Coord *out = (Coord *)patchCoords;
+    int found = 0;
+#pragma omp parallel for reduction(+ : found)
+    for (int i = 0; i < n; ++i) {
+        PatchHandleRec h;
+        if (t->table.findPatch(ptex[i], u[i], v[i], h)) {
+            out[i] = { h.arrayIndex, h.patchIndex, h.vertIndex, (float)u[i], (float)v[i] };
+            ++found;
+        } else {
+            out[i] = { -1, -1, -1, (float)u[i], (float)v[i] };
+        }
+    }
+    return found;
+}
+
+osdfar_stencil_table *osdfar_limit_stencil_table_create(const osdfar_patch_table *t, int n,
+                                                        const int *ptex, const double *u,
+                                                        const double *v, int secondDerivatives,
+                                                        int useDouble)
+{
+    if (!t || n < 0 || (n > 0 && (!ptex || !u || !v))) return nullptr;
+    std::unique_ptr<osdfar_stencil_table> s(new (std::nothrow) osdfar_stencil_table);
+    if (!s) return nullptr;
+    s->isDouble = useDouble != 0;
+    s->numWeightArrays = secondDerivatives ? 6 : 3;
+    if (s->isDouble) t->table.limitStencils(n, ptex, u, v, secondDerivatives != 0, s->d);
+    else t->table.limitStencils(n, ptex, u, v, secondDerivatives != 0, s->f);
+    return s.release();
+}
+
+int osdfar_vertex_locations(const osdfar_topology *topo, int level, int *ptex, double *u,
+                            double *v)
+{
+    if (!topo || level < 0 || level > topo->refiner.maxLevel() || !ptex || !u || !v) return -1;
+    std::vector<int> p;
+    std::vector<double> uu, vv;
+    uniformVertexLocations(topo->refiner, level, p, uu, vv);
+    std::memcpy(ptex, p.data(), p.size() * sizeof(int));
+    std::memcpy(u, uu.data(), uu.size() * sizeof(double));
+    std::memcpy(v, vv.data(), vv.size() * sizeof(double));
+    return 0;
+}
 
 }  // extern "C"

@@ -1,0 +1,606 @@
+// Feature-adaptive patch tables, patch-point stencils, patch map and limit
+// stencils (see patches.h for the scope and the reference functions).
+#include "patches.h"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "basis.h"
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+namespace osdfar {
+
+namespace {
+
+constexpr int kRegular = 6, kGregory = 9;
+
+int otherFace(const Level &L, int e, int f)
+{
+    for (int k = L.efOff[e]; k < L.efOff[e + 1]; ++k)
+        if (L.ef[k] != f) return L.ef[k];
+    return -1;
+}
+
+int indexInFace(const Level &L, int f, int v)
+{
+    for (int k = L.fvOff[f]; k < L.fvOff[f + 1]; ++k)
+        if (L.fv[k] == v) return k - L.fvOff[f];
+    return -1;
+}
+
+// Counter-clockwise one-ring of interior vertex v starting at face f:
+// ring[2k] = vertex across edge k, ring[2k+1] = vertex of face k opposite v
+// (quads), face k lies between edges k and k+1, face 0 = f, edge 0 = the edge
+// from v to the next vertex of f.  faces[k] receives face k.  Returns the
+// valence, or -1 at a boundary.
+int orderedRing(const Level &L, int v, int f, std::vector<int> &ring, std::vector<int> *faces = nullptr)
+{
+    ring.clear();
+    if (faces) faces->clear();
+    int cur = f;
+    for (int guard = 0; guard < 256; ++guard) {
+        const int b = L.fvOff[cur], n = L.faceSize(cur);
+        const int j = indexInFace(L, cur, v);
+        ring.push_back(L.fv[b + (j + 1) % n]);
+        ring.push_back(L.fv[b + (j + 2) % n]);
+        if (faces) faces->push_back(cur);
+        const int ePrev = L.fe[b + (j + n - 1) % n];     // edge (v, next edge neighbour)
+        const int nxt = otherFace(L, ePrev, cur);
+        if (nxt < 0) return -1;
+        cur = nxt;
+        if (cur == f) return (int)ring.size() / 2;
+    }
+    return -1;
+}
+
+// Sparse row over level vertices.
+struct Row {
+    std::vector<std::pair<int, double>> e;
+    void add(int v, double w)
+    {
+        for (auto &x : e)
+            if (x.first == v) { x.second += w; return; }
+        e.push_back({ v, w });
+    }
+    void addRow(const Row &r, double s)
+    {
+        for (auto &x : r.e) add(x.first, x.second * s);
+    }
+};
+
+// Tangent scale of the interior limit tangent masks:
+// 16 / (n (cos(2pi/n) + 5 + sqrt((cos(2pi/n) + 9)(cos(2pi/n) + 1)))).
+double limitTangentCoefficient(int n)
+{
+    const double c = std::cos(2.0 * M_PI / n);
+    return 16.0 / (n * ((c + 5.0) + std::sqrt((c + 9.0) * (c + 1.0))));
+}
+
+// Limit point and the two edge tangent points of an interior corner of
+// valence n whose patch face is face 0 of its ring (far/catmarkPatchBuilder.cpp
+// CatmarkLimits::ComputeInteriorPointWeights with faceInRing = 0):
+// weights[0] for the corner, weights[1 + 2k] edge k, weights[2 + 2k] face k.
+void interiorPointWeights(int n, std::vector<double> &p, std::vector<double> &ep,
+                          std::vector<double> &em)
+{
+    const int W = 1 + 2 * n;
+    p.assign(W, 0.0);
+    ep.assign(W, 0.0);
+    em.assign(W, 0.0);
+    std::vector<double> tw(W, 0.0);
+    const double inv = 1.0 / n, invP5 = 1.0 / (n + 5.0);
+    const double pc = inv * invP5;
+    const double tc = limitTangentCoefficient(n) * 0.5 * invP5;
+    p[0] = n * invP5;
+    for (int i = 0; i < n; ++i) {
+        p[1 + 2 * i] = 4.0 * pc;
+        p[2 + 2 * i] = pc;
+        const int ip = (i + n - 1) % n, in = (i + 1) % n;
+        const double ci = tc * std::cos(2.0 * M_PI * i / n);
+        tw[1 + 2 * ip] += 2.0 * ci;
+        tw[2 + 2 * ip] += ci;
+        tw[1 + 2 * i] += 4.0 * ci;
+        tw[2 + 2 * i] += ci;
+        tw[1 + 2 * in] += 2.0 * ci;
+    }
+    // Ep: tangent along edge 0; Em: along edge 1 (rotate by one edge)
+    ep[0] = em[0] = p[0];
+    for (int i = 1; i < W; ++i) {
+        ep[i] = p[i] + tw[i];
+        int im = i + 2 * (n - 1);
+        if (im >= W) im -= W - 1;
+        em[i] = p[i] + tw[im];
+    }
+}
+
+// The 20 Gregory points of an end-cap face (Loop, Schaefer, Ni, Castano,
+// "Approximating subdivision surfaces with Gregory patches for hardware
+// tessellation", as converted in far/catmarkPatchBuilder.cpp:1003-1416) for
+// interior corners: rows over level vertices, point 5c+{0,1,2,3,4} =
+// P, E+, E-, F+, F- of corner c.
+bool gregoryPoints(const Level &L, int f, Row out[20])
+{
+    int cv[4];
+    std::vector<int> ring[4];
+    int val[4];
+    bool reg[4];
+    double cs[4];
+    for (int c = 0; c < 4; ++c) {
+        cv[c] = L.fv[L.fvOff[f] + c];
+        val[c] = orderedRing(L, cv[c], f, ring[c]);
+        if (val[c] < 3) return false;                 // boundary / valence 2: out of scope
+        reg[c] = val[c] == 4;
+        cs[c] = reg[c] ? 0.0 : std::cos(2.0 * M_PI / val[c]);
+    }
+    for (int c = 0; c < 4; ++c) {
+        Row &P = out[5 * c], &Ep = out[5 * c + 1], &Em = out[5 * c + 2];
+        const std::vector<int> &r = ring[c];
+        if (reg[c]) {
+            P.add(cv[c], 4.0 / 9.0);
+            for (int k = 0; k < 4; ++k) {
+                P.add(r[2 * k], 1.0 / 9.0);
+                P.add(r[2 * k + 1], 1.0 / 36.0);
+            }
+            Ep.add(cv[c], 4.0 / 9.0);
+            Ep.add(r[0], 2.0 / 9.0);
+            Ep.add(r[2], 1.0 / 9.0);
+            Ep.add(r[6], 1.0 / 9.0);
+            Ep.add(r[1], 1.0 / 18.0);
+            Ep.add(r[7], 1.0 / 18.0);
+            Em.add(cv[c], 4.0 / 9.0);
+            Em.add(r[2], 2.0 / 9.0);
+            Em.add(r[0], 1.0 / 9.0);
+            Em.add(r[4], 1.0 / 9.0);
+            Em.add(r[1], 1.0 / 18.0);
+            Em.add(r[3], 1.0 / 18.0);
+        } else {
+            std::vector<double> pw, epw, emw;
+            interiorPointWeights(val[c], pw, epw, emw);
+            P.add(cv[c], pw[0]);
+            Ep.add(cv[c], epw[0]);
+            Em.add(cv[c], emw[0]);
+            for (int i = 1; i < (int)pw.size(); ++i) {
+                P.add(r[i - 1], pw[i]);
+                Ep.add(r[i - 1], epw[i]);
+                Em.add(r[i - 1], emw[i]);
+            }
+        }
+    }
+    for (int c = 0; c < 4; ++c) {
+        const int cn = (c + 1) & 3, co = (c + 2) & 3, cp = (c + 3) & 3;
+        const std::vector<int> &r = ring[c];
+        const int n = val[c];
+        for (int side = 0; side < 2; ++side) {
+            Row &F = out[5 * c + 3 + side];
+            const int far = side == 0 ? cn : cp;
+            if (reg[c] && reg[far]) {
+                F.add(cv[c], 4.0 / 9.0);
+                F.add(cv[cp], 2.0 / 9.0);
+                F.add(cv[cn], 2.0 / 9.0);
+                F.add(cv[co], 1.0 / 9.0);
+                continue;
+            }
+            const double cN = cs[c], cF = cs[far];
+            const Row &eNear = out[5 * c + 1 + side];                 // E+ or E- of c
+            const Row &eFar = out[5 * far + (side == 0 ? 2 : 1)];    // E- of next / E+ of prev
+            F.addRow(out[5 * c], cF / 3.0);
+            F.addRow(eNear, (3.0 - 2.0 * cN - cF) / 3.0);
+            F.addRow(eFar, 2.0 * cN / 3.0);
+            const double sgn = side == 0 ? 1.0 : -1.0;
+            const int eInt = side, ePrev = (eInt + n - 1) % n, eNext = (eInt + 1) % n;
+            F.add(r[2 * ePrev], -sgn / 9.0);
+            F.add(r[2 * ePrev + 1], -sgn / 18.0);
+            F.add(r[2 * eInt + 1], sgn / 18.0);
+            F.add(r[2 * eNext], sgn / 9.0);
+        }
+    }
+    return true;
+}
+
+// Far's feature test for refining face f of a level (scope: manifold,
+// interior): non-quad, extraordinary corner, or a corner with vertex
+// sharpness or an incident sharp edge.
+bool faceHasFeatures(const Level &L, int f)
+{
+    if (L.faceSize(f) != 4) return true;
+    for (int k = L.fvOff[f]; k < L.fvOff[f + 1]; ++k) {
+        const int v = L.fv[k];
+        if (L.valence(v) != 4 || (L.vfOff[v + 1] - L.vfOff[v]) != 4) return true;
+        if (L.vsharp[v] > 0.0f) return true;
+        for (int i = L.veOff[v]; i < L.veOff[v + 1]; ++i)
+            if (L.esharp[L.ve[i]] > 0.0f) return true;
+    }
+    return false;
+}
+
+bool isExtraordinary(const Level &L, int f)
+{
+    for (int k = L.fvOff[f]; k < L.fvOff[f + 1]; ++k)
+        if (L.valence(L.fv[k]) != 4) return true;
+    return false;
+}
+
+// 16 B-spline points of an interior regular face (far/patchBuilder.cpp:
+// 878-968): corner c of the face sits at kCornerPoint[c][0], the vertices of
+// the face diagonally opposite at that corner fill the other three slots.
+const int kCornerPoint[4][4] = { { 5, 4, 0, 1 }, { 6, 2, 3, 7 }, { 10, 11, 15, 14 }, { 9, 13, 12, 8 } };
+
+bool regularPoints(const Level &L, int f, int pts[16])
+{
+    std::vector<int> ring;
+    for (int c = 0; c < 4; ++c) {
+        const int v = L.fv[L.fvOff[f] + c];
+        if (orderedRing(L, v, f, ring) != 4) return false;
+        pts[kCornerPoint[c][0]] = v;
+        pts[kCornerPoint[c][1]] = ring[4];
+        pts[kCornerPoint[c][2]] = ring[5];
+        pts[kCornerPoint[c][3]] = ring[6];
+    }
+    return true;
+}
+
+}  // namespace
+
+bool PatchTable::build(Refiner &refiner, int isoLevel)
+{
+    {
+        const Level &L0 = refiner.level(0);
+        for (int f = 0; f < L0.nf; ++f)
+            if (L0.faceSize(f) != 4) return false;             // quad base faces only
+        for (int e = 0; e < L0.ne; ++e)
+            if (L0.efOff[e + 1] - L0.efOff[e] != 2 || L0.esharp[e] >= kSharpInfinite) return false;
+        for (int v = 0; v < L0.nv; ++v)
+            if (L0.vsharp[v] >= kSharpInfinite) return false;
+    }
+    refiner.refineUniform(isoLevel);                           // may reallocate the levels
+    const int maxL = isoLevel;
+    std::vector<LevelStencils> LS;
+    computeLevelStencils(refiner, INTERPOLATE_VERTEX, LS);
+
+    numControl = refiner.level(0).nv;
+    maxDepth = 0;
+    struct Leaf { int level, face; bool regular; };
+    std::vector<Leaf> leaves;
+    std::vector<char> complete(refiner.level(0).nf, 1), next;
+    for (int d = 0; d <= maxL; ++d) {
+        const Level &L = refiner.level(d);
+        if (d < maxL) next.assign(refiner.level(d + 1).nf, 0);
+        for (int f = 0; f < L.nf; ++f) {
+            if (!complete[f]) continue;
+            if (d < maxL && faceHasFeatures(L, f)) {
+                for (int j = 0; j < 4; ++j) next[L.fvOff[f] + j] = 1;
+            } else {
+                leaves.push_back({ d, f, !isExtraordinary(L, f) });
+                maxDepth = std::max(maxDepth, d);
+            }
+        }
+        if (d < maxL) complete.swap(next);
+    }
+
+    // refined vertices referenced by B-spline patches
+    std::vector<std::vector<int>> bufIndex(maxL + 1);
+    for (int d = 1; d <= maxL; ++d) bufIndex[d].assign(refiner.level(d).nv, -1);
+    std::vector<int> regPts;
+    std::vector<Row> greg;
+    std::vector<const Leaf *> regLeaves, gregLeaves;
+    for (const Leaf &lf : leaves) {
+        if (lf.regular) {
+            int pts[16];
+            if (!regularPoints(refiner.level(lf.level), lf.face, pts)) return false;
+            if (lf.level > 0)
+                for (int p : pts) bufIndex[lf.level][p] = 0;
+            regPts.insert(regPts.end(), pts, pts + 16);
+            regLeaves.push_back(&lf);
+        } else {
+            Row rows[20];
+            if (!gregoryPoints(refiner.level(lf.level), lf.face, rows)) return false;
+            greg.insert(greg.end(), rows, rows + 20);
+            gregLeaves.push_back(&lf);
+        }
+    }
+    int next_index = numControl;
+    for (int d = 1; d <= maxL; ++d)
+        for (int &b : bufIndex[d])
+            if (b == 0) b = next_index++;
+    numRefined = next_index - numControl;
+    numLocal = (int)greg.size();
+
+    // patch arrays / indices / params, and point stencils
+    arrays.clear();
+    indices.clear();
+    params.clear();
+    patchDepth_.clear();
+    auto paramOf = [&](const Leaf &lf, bool regular) {
+        PatchParamRec p;
+        int ptex, u = 0, v = 0;
+        if (lf.level == 0) {
+            ptex = refiner.ptexBase[lf.face];
+        } else {
+            const FaceOrigin &o = refiner.origins[lf.level][lf.face];
+            ptex = o.ptex;
+            u = o.u;
+            v = o.v;
+        }
+        p.field0 = (uint32_t)ptex & 0xfffffffu;
+        p.field1 = (uint32_t)lf.level | ((uint32_t)(regular ? 1 : 0) << 5) |
+                   ((uint32_t)v << 12) | ((uint32_t)u << 22);
+        p.sharpness = 0.0f;
+        return p;
+    };
+    const int nReg = (int)regLeaves.size(), nGreg = (int)gregLeaves.size();
+    if (nReg) arrays.push_back({ kRegular, kRegular, nReg, 0, 16, 0 });
+    for (int i = 0; i < nReg; ++i) {
+        const Leaf &lf = *regLeaves[i];
+        for (int k = 0; k < 16; ++k) {
+            const int p = regPts[16 * i + k];
+            indices.push_back(lf.level == 0 ? p : bufIndex[lf.level][p]);
+        }
+        params.push_back(paramOf(lf, true));
+        patchDepth_.push_back(lf.level);
+    }
+    if (nGreg) arrays.push_back({ kGregory, kGregory, nGreg, (int)indices.size(), 20, nReg });
+    for (int i = 0; i < nGreg; ++i) {
+        for (int k = 0; k < 20; ++k) indices.push_back(numControl + numRefined + 20 * i + k);
+        params.push_back(paramOf(*gregLeaves[i], false));
+        patchDepth_.push_back(gregLeaves[i]->level);
+    }
+
+    // point stencils: refined vertices (level order), then end-cap points
+    LevelStencils &S = pointStencils;
+    S.off.assign(1, 0);
+    S.idx.clear();
+    S.w.clear();
+    for (int d = 1; d <= maxL; ++d)
+        for (int v = 0; v < refiner.level(d).nv; ++v)
+            if (bufIndex[d][v] >= 0) {
+                const LevelStencils &src = LS[d];
+                S.idx.insert(S.idx.end(), src.idx.begin() + src.off[v], src.idx.begin() + src.off[v + 1]);
+                S.w.insert(S.w.end(), src.w.begin() + src.off[v], src.w.begin() + src.off[v + 1]);
+                S.off.push_back((int64_t)S.idx.size());
+            }
+    {
+        const LevelStencils &src = LS[maxL];
+        std::vector<int> pos(numControl, -1), list;
+        std::vector<double> val;
+        for (const Row &row : greg) {
+            for (auto &x : row.e)
+                for (int64_t k = src.off[x.first]; k < src.off[x.first + 1]; ++k) {
+                    const int c = src.idx[k];
+                    if (pos[c] < 0) { pos[c] = (int)list.size(); list.push_back(c); val.push_back(0.0); }
+                    val[pos[c]] += x.second * src.w[k];
+                }
+            std::vector<int> ord(list.size());
+            for (size_t k = 0; k < ord.size(); ++k) ord[k] = (int)k;
+            std::sort(ord.begin(), ord.end(), [&](int a, int b) { return list[a] < list[b]; });
+            for (int k : ord) { S.idx.push_back(list[k]); S.w.push_back(val[k]); pos[list[k]] = -1; }
+            S.off.push_back((int64_t)S.idx.size());
+            list.clear();
+            val.clear();
+        }
+    }
+
+    // patch map: one quadtree per ptex face
+    ptexRoot.assign(refiner.ptexCount, -1);
+    nodes.clear();
+    auto rootOf = [&](int ptex) {
+        if (ptexRoot[ptex] < 0) {
+            ptexRoot[ptex] = (int)nodes.size();
+            nodes.push_back({ { -INT_MAX, -INT_MAX, -INT_MAX, -INT_MAX } });
+        }
+        return ptexRoot[ptex];
+    };
+    const int nPatches = (int)params.size();
+    for (int p = 0; p < nPatches; ++p) {
+        const uint32_t f1 = params[p].field1;
+        const int ptex = (int)(params[p].field0 & 0xfffffffu);
+        const int depth = (int)(f1 & 0xf), u = (int)((f1 >> 22) & 0x3ff), v = (int)((f1 >> 12) & 0x3ff);
+        int node = rootOf(ptex);
+        if (depth == 0) {
+            for (int q = 0; q < 4; ++q) nodes[node].child[q] = -(p + 1);
+            continue;
+        }
+        for (int k = depth - 1; k >= 0; --k) {
+            const int q = (((v >> k) & 1) << 1) | ((u >> k) & 1);
+            if (k == 0) {
+                nodes[node].child[q] = -(p + 1);
+            } else {
+                int c = nodes[node].child[q];
+                if (c < 0) {
+                    c = (int)nodes.size();
+                    nodes[node].child[q] = c;
+                    nodes.push_back({ { -INT_MAX, -INT_MAX, -INT_MAX, -INT_MAX } });
+                }
+                node = c;
+            }
+        }
+    }
+    return true;
+}
+
+bool PatchTable::findPatch(int ptex, double u, double v, PatchHandleRec &h) const
+{
+    if (ptex < 0 || ptex >= (int)ptexRoot.size() || ptexRoot[ptex] < 0) return false;
+    if (!(u >= 0.0 && u <= 1.0 && v >= 0.0 && v <= 1.0)) return false;
+    int node = ptexRoot[ptex];
+    double median = 0.5;
+    for (int depth = 0; depth <= maxDepth; ++depth, median *= 0.5) {
+        // far/patchMap.h:136-143: the upper half includes the median
+        const int uh = u >= median, vh = v >= median;
+        if (uh) u -= median;
+        if (vh) v -= median;
+        const int c = nodes[node].child[(vh << 1) | uh];
+        if (c == -INT_MAX) return false;
+        if (c < 0) {
+            const int p = -c - 1;
+            h.arrayIndex = (arrays.size() > 1 && p >= arrays[0].numPatches) ? 1 : 0;
+            h.patchIndex = p;
+            h.vertIndex = arrays[h.arrayIndex].indexBase +
+                          arrays[h.arrayIndex].stride * (p - arrays[h.arrayIndex].primitiveIdBase);
+            return true;
+        }
+        node = c;
+    }
+    return false;
+}
+
+const int *PatchTable::patchPoints(int p, int &n) const
+{
+    const int a = (arrays.size() > 1 && p >= arrays[0].numPatches) ? 1 : 0;
+    n = arrays[a].stride;
+    return indices.data() + arrays[a].indexBase + arrays[a].stride * (p - arrays[a].primitiveIdBase);
+}
+
+int PatchTable::evalBasis(int p, double u, double v, double *wP, double *wDu, double *wDv,
+                          double *wDuu, double *wDuv, double *wDvv) const
+{
+    const int a = (arrays.size() > 1 && p >= arrays[0].numPatches) ? 1 : 0;
+    return patchBasis<double>(arrays[a].desc, params[p].field1, u, v, wP, wDu, wDv, wDuu, wDuv, wDvv);
+}
+
+template <typename REAL>
+void PatchTable::pointStencilTable(StencilTableT<REAL> &out) const
+{
+    out = StencilTableT<REAL>();
+    out.numControlVertices = numControl;
+    const int n = (int)pointStencils.off.size() - 1;
+    out.sizes.resize(n);
+    out.offsets.resize(n);
+    for (int i = 0; i < n; ++i) {
+        out.sizes[i] = pointStencils.size(i);
+        out.offsets[i] = pointStencils.off[i];
+    }
+    out.indices = pointStencils.idx;
+    out.weights.assign(pointStencils.w.begin(), pointStencils.w.end());
+}
+
+template <typename REAL>
+int PatchTable::limitStencils(int n, const int *ptex, const double *u, const double *v,
+                              bool second, StencilTableT<REAL> &out) const
+{
+    out = StencilTableT<REAL>();
+    out.numControlVertices = numControl;
+    const int nw = second ? 6 : 3;
+    int T = 1;
+#ifdef _OPENMP
+    T = std::max(1, std::min(omp_get_max_threads(), (n + 4095) / 4096));
+#endif
+    std::vector<std::vector<int>> tIdx(T), tSz(T);
+    std::vector<std::vector<double>> tW(T);
+    int missing = 0;
+#pragma omp parallel num_threads(T) reduction(+ : missing)
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        const int lo = (int)((int64_t)n * t / T), hi = (int)((int64_t)n * (t + 1) / T);
+        std::vector<int> pos(numControl, -1), list;
+        std::vector<double> acc;    // nw per entry
+        double w[6][20];
+        for (int i = lo; i < hi; ++i) {
+            PatchHandleRec h;
+            if (!findPatch(ptex[i], u[i], v[i], h)) {
+                tSz[t].push_back(0);
+                ++missing;
+                continue;
+            }
+            const int np = evalBasis(h.patchIndex, u[i], v[i], w[0], w[1], w[2],
+                                     second ? w[3] : nullptr, second ? w[4] : nullptr,
+                                     second ? w[5] : nullptr);
+            int cnt;
+            const int *pts = patchPoints(h.patchIndex, cnt);
+            for (int j = 0; j < np; ++j) {
+                const int cvi = pts[j];
+                auto addc = [&](int c, double s) {
+                    if (pos[c] < 0) {
+                        pos[c] = (int)list.size();
+                        list.push_back(c);
+                        acc.insert(acc.end(), (size_t)nw, 0.0);
+                    }
+                    double *a = &acc[(size_t)pos[c] * nw];
+                    for (int k = 0; k < nw; ++k) a[k] += s * w[k][j];
+                };
+                if (cvi < numControl) {
+                    addc(cvi, 1.0);
+                } else {
+                    const int r = cvi - numControl;
+                    for (int64_t k = pointStencils.off[r]; k < pointStencils.off[r + 1]; ++k)
+                        addc(pointStencils.idx[k], pointStencils.w[k]);
+                }
+            }
+            std::vector<int> ord(list.size());
+            for (size_t k = 0; k < ord.size(); ++k) ord[k] = (int)k;
+            std::sort(ord.begin(), ord.end(), [&](int a, int b) { return list[a] < list[b]; });
+            for (int k : ord) {
+                tIdx[t].push_back(list[k]);
+                for (int q = 0; q < nw; ++q) tW[t].push_back(acc[(size_t)k * nw + q]);
+                pos[list[k]] = -1;
+            }
+            tSz[t].push_back((int)list.size());
+            list.clear();
+            acc.clear();
+        }
+    }
+    out.sizes.reserve(n);
+    out.offsets.reserve(n);
+    int64_t off = 0;
+    for (int t = 0; t < T; ++t)
+        for (int s : tSz[t]) { out.sizes.push_back(s); out.offsets.push_back(off); off += s; }
+    out.indices.reserve(off);
+    std::vector<REAL> *dst[6] = { &out.weights, &out.du, &out.dv, &out.duu, &out.duv, &out.dvv };
+    for (int k = 0; k < nw; ++k) dst[k]->reserve(off);
+    for (int t = 0; t < T; ++t) {
+        out.indices.insert(out.indices.end(), tIdx[t].begin(), tIdx[t].end());
+        const size_t m = tIdx[t].size();
+        for (size_t e = 0; e < m; ++e)
+            for (int k = 0; k < nw; ++k) dst[k]->push_back((REAL)tW[t][e * nw + k]);
+    }
+    return missing;
+}
+
+template void PatchTable::pointStencilTable<float>(StencilTable &) const;
+template void PatchTable::pointStencilTable<double>(StencilTableD &) const;
+template int PatchTable::limitStencils<float>(int, const int *, const double *, const double *,
+                                              bool, StencilTable &) const;
+template int PatchTable::limitStencils<double>(int, const int *, const double *, const double *,
+                                               bool, StencilTableD &) const;
+
+void uniformVertexLocations(const Refiner &r, int level, std::vector<int> &ptex,
+                            std::vector<double> &u, std::vector<double> &v)
+{
+    const Level &L = r.level(level);
+    ptex.assign(L.nv, -1);
+    u.assign(L.nv, 0.0);
+    v.assign(L.nv, 0.0);
+    const double scale = 1.0 / (double)(1 << level);
+    const int du[4] = { 0, 1, 1, 0 }, dv[4] = { 0, 0, 1, 1 };
+    for (int f = 0; f < L.nf; ++f) {
+        int p = 0, fu = 0, fv = 0, nonquadShift = 0;
+        if (level == 0) {
+            p = r.ptexBase[f];
+        } else {
+            const FaceOrigin &o = r.origins[level][f];
+            p = o.ptex;
+            fu = o.u;
+            fv = o.v;
+        }
+        (void)nonquadShift;
+        for (int k = 0; k < 4; ++k) {
+            const int vert = L.fv[L.fvOff[f] + k];
+            if (ptex[vert] >= 0) continue;
+            ptex[vert] = p;
+            u[vert] = (fu + du[k]) * scale;
+            v[vert] = (fv + dv[k]) * scale;
+        }
+    }
+}
+
+}  // namespace osdfar
