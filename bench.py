@@ -1,23 +1,33 @@
 """Benchmark of the MI355X Osd evaluator (one GPU).
 
-    python bench.py [--steps K] [--warmup W] [--workload cfg1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--only cfg1]
 
-Workload "cfg1" (default): cube-sphere, 6 x 128 x 128 quads (98,306
-vertices), float32 xyz on the unit sphere with 1% seeded radial noise, 5%
-of edges creased with sharpness U[0,4); uniform level-3 Catmull-Clark
-stencil table (6,291,458 stencils) built by the project's own table
-compiler.  One step = one EvalStencils pass over the whole table (one
-frame), inputs resident in HBM.  value = refined points per second.
+Workloads (synthetic, seeded; tables built by the project's own table
+compiler, which reproduces Far's tables):
 
-The JSON line carries the roofline of the dominant kernel (algorithmic HBM
+* cfg1 (headline): cube-sphere, 6 x 128 x 128 quads (98,306 vertices),
+  float32 xyz on the unit sphere with 1% seeded radial noise, 5% of edges
+  creased with sharpness U[0,4).  Limit stencil table with du/dv weights at
+  the 6,291,458 vertex locations of uniform level 3 (Far
+  LimitStencilTableFactory over the feature-adaptive iso-3 patch table).
+  One step = one fused EvalStencils launch producing P, dP/du, dP/dv for
+  every location (one frame, inputs resident in HBM).
+  value = limit points per second.
+* cfg1_refine: the uniform level-3 refinement table of the same mesh, P.
+* cfg2: EvalPatches (P, dP/du, dP/dv) at 4,194,304 PatchCoords (ptex face
+  uniform, (u,v) uniform [0,1)^2, PatchMap lookup) on the iso-3
+  feature-adaptive patch table (B-spline + Gregory end caps) of the mesh.
+
+The JSON line carries the roofline of the headline kernel (algorithmic HBM
 bytes per launch / average launch time from HIP events on the launch
-stream) and a CPU baseline (Osd::CpuEvaluator built from the reference
-sources when that library is present, else the C restatement), timed on
-this machine's host cores over the same table.
+stream) and the CPU baseline: Osd::CpuEvaluator built from the reference
+sources when that library is present (else the C restatement), timed on
+this machine's host cores over the same table; OmpEvaluator beside it.
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -30,66 +40,62 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0           # MI355X HBM3E, GB/s
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
+METRIC = "limit points/s + HBM GB/s vs roofline; served p50/p99 latency at fixed QPS"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_cfg1():
+# --------------------------------------------------------------- inputs
+
+def build_inputs(need_refine=True, need_patches=True):
     from opensubdiv_amd import far, meshes
     t0 = time.time()
     mesh = meshes.cube_sphere(128, seed=1, radial_noise=0.01, crease_fraction=0.05,
                               sharpness_max=4.0)
+    out = {"mesh": mesh}
     topo = far.Topology(mesh)
     topo.refine_uniform(3)
-    table = topo.stencil_table()
-    log(f"[bench] cfg1 table: {len(table['sizes'])} stencils, {len(table['indices'])} entries, "
-        f"built in {time.time() - t0:.1f}s")
-    return mesh, table
+    if need_refine:
+        out["refine"] = topo.stencil_table()
+    ptex, u, v = topo.vertex_locations(3)
+    pt = far.Topology(mesh).patch_table(3)
+    out["patches"] = pt
+    out["limit"] = pt.limit_stencils(ptex, u, v)
+    if need_patches:
+        out["points"] = pt.point_stencils()
+        f, s, t = meshes.random_patch_locations(mesh.nfaces, 4194304, seed=2)
+        out["coords"] = pt.find(f, s, t)
+        assert (out["coords"]["arrayIndex"] >= 0).all()
+    log(f"[bench] inputs built in {time.time() - t0:.1f}s: limit table "
+        f"{len(out['limit']['sizes'])} x {len(out['limit']['indices'])} entries, "
+        f"{len(pt.params)} patches")
+    return out
 
 
-def stencil_bytes(n_st, nnz, n_weights, L, n_src, n_out_arrays):
-    """Algorithmic HBM bytes of one EvalStencils launch: sizes + offsets
-    (4 B each per stencil), indices (4 B) and n_weights weight arrays per
-    entry, the outputs, and the control points read once."""
-    return 8 * n_st + (4 + 4 * n_weights) * nnz + 4 * L * n_out_arrays * n_st + 4 * L * n_src
+# ------------------------------------------------------- algorithmic bytes
+
+def stencil_bytes(n_st, nnz, n_weights, L, n_src, n_out):
+    """One EvalStencils launch: sizes (4 B / stencil), index + n_weights
+    weights (4 B each) per entry, n_out outputs of L floats, control points
+    read once."""
+    return 4 * n_st + (4 + 4 * n_weights) * nnz + 4 * L * n_out * n_st + 4 * L * n_src
 
 
-def cpu_baseline_stencils(src, table, L, budget_s=12.0, threads=0):
-    """Times CpuEvaluator::EvalStencils (threads=0: one thread) or
-    OmpEvaluator::EvalStencils (threads>0) over the full table, repeated
-    within a time budget; falls back to the C restatement when the reference
-    library is absent."""
-    import oracle
-    from oracle import ref
-    n = len(table["sizes"])
-    dst = np.zeros((n, L), np.float32)
-    if ref.available():
-        kind = "reference"
-        run = lambda: ref.cpu_eval_stencils(src, (0, L, L), dst, (0, L, L), table,  # noqa: E731
-                                            threads=threads)
-    else:
-        kind = "port"
-        run = lambda: oracle.eval_stencils(src, (0, L, L), dst, (0, L, L), table["sizes"],  # noqa
-                                           table["offsets"], table["indices"], table["weights"],
-                                           threads=threads)
-    run()                                   # warm-up
-    times = []
-    t_end = time.time() + budget_s
-    while time.time() < t_end and len(times) < 9:
-        t0 = time.perf_counter()
-        assert run()
-        times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
-    return {"value": n / med, "unit": "points/s", "cores": threads or 1, "kind": kind,
-            "sample": f"full table ({n} stencils) x {len(times)} runs, median {med * 1e3:.1f} ms",
-            "dst": dst}
+def patch_bytes(n_pc, L, n_out, n_indices, n_patches, n_arrays, n_cv):
+    """One EvalPatches launch: PatchCoords (20 B) and outputs per coord, the
+    patch tables (indices, 12 B params, 24 B arrays) and control points
+    counted once."""
+    return (20 + 4 * L * n_out) * n_pc + 4 * n_indices + 12 * n_patches + 24 * n_arrays + \
+        4 * L * n_cv
 
+
+# ---------------------------------------------------------------- timing
 
 def time_launches(step, steps, warmup, stream):
-    """Runs warmup + steps launches; returns (wall seconds of the timed
-    region, per-launch milliseconds from HIP events on `stream`)."""
+    """warmup + steps launches; returns (wall seconds of the timed region,
+    per-launch milliseconds from HIP events recorded on `stream`)."""
     import torch
     for _ in range(warmup):
         step()
@@ -107,95 +113,181 @@ def time_launches(step, steps, warmup, stream):
     return wall, [a.elapsed_time(b) for a, b in ev]
 
 
-def run_cfg1(args):
-    import ctypes
+def cpu_time(run, budget_s, max_runs=9):
+    run()                                   # warm-up
+    times = []
+    t_end = time.time() + budget_s
+    while time.time() < t_end and len(times) < max_runs:
+        t0 = time.perf_counter()
+        assert run()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), len(times)
 
+
+def traffic_for(key):
+    if os.path.exists(PROFILE_TRAFFIC):
+        tr = json.load(open(PROFILE_TRAFFIC)).get(key)
+        if tr:
+            return tr.get("bytes_per_launch")
+    return None
+
+
+# ------------------------------------------------------------- workloads
+
+def table_eval_fn(lib, table, src, srcdesc, outs, descs, stream):
+    D = type(srcdesc)
+    n = table.GetNumStencils()
+    optr = (ctypes.c_void_p * len(outs))(*[o.data_ptr() for o in outs])
+    dptr = (D * len(outs))(*descs)
+
+    def step():
+        rc = lib.osdhip_stencil_table_eval(table.h, src.data_ptr(), srcdesc, len(outs),
+                                           ctypes.cast(optr, ctypes.c_void_p),
+                                           ctypes.cast(dptr, ctypes.c_void_p), 0, n, stream)
+        if rc != 0:
+            raise RuntimeError(f"osdhip_stencil_table_eval failed: {rc}")
+    return step
+
+
+def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu):
+    """EvalStencils of a (limit) stencil table through HipStencilTable."""
     import torch
     from opensubdiv_amd import HipStencilTable, _lib
-    lib = _lib.load_hip()
-    mesh, table = build_cfg1()
-    L = 3
-    n_src = mesh.nverts
-    src_np = mesh.positions.astype(np.float32)
-    n = len(table["sizes"])
-    nnz = len(table["indices"])
-
-    d_src = torch.from_numpy(np.ascontiguousarray(src_np)).to("cuda")
-    d_dst = torch.empty(n * L, dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
     s = stream.cuda_stream
     D = _lib.Desc
+    L = src_np.shape[1]
+    n = len(table_np["sizes"])
+    nnz = len(table_np["indices"])
     t0 = time.time()
-    htable = HipStencilTable.Create(table, s)
-    log(f"[bench] device table: {htable.device_bytes() / 1e6:.0f} MB, {time.time() - t0:.1f}s")
-    outs = (ctypes.c_void_p * 1)(d_dst.data_ptr())
-    descs = (D * 1)(D(0, L, L))
-
-    def table_step():
-        rc = lib.osdhip_stencil_table_eval(htable.h, d_src.data_ptr(), D(0, L, L), 1,
-                                           ctypes.cast(outs, ctypes.c_void_p),
-                                           ctypes.cast(descs, ctypes.c_void_p), 0, n, s)
-        if rc != 0:
-            raise RuntimeError(f"osdhip_stencil_table_eval failed: {rc}")
-
-    def raw_step():
-        rc = lib.osdhip_eval_stencils(d_src.data_ptr(), D(0, L, L), d_dst.data_ptr(), D(0, L, L),
-                                      htable.GetSizesBuffer(), htable.GetOffsetsBuffer(),
-                                      htable.GetIndicesBuffer(), htable.GetWeightsBuffer(), 0, n, s)
-        if rc != 0:
-            raise RuntimeError(f"osdhip_eval_stencils failed: {rc}")
-
-    alg = stencil_bytes(n, nnz, 1, L, n_src, 1)
+    htable = HipStencilTable.Create(table_np, s)
+    log(f"[bench] {name}: device table {htable.device_bytes() / 1e6:.0f} MB in "
+        f"{time.time() - t0:.1f}s")
+    d_src = torch.from_numpy(np.ascontiguousarray(src_np)).to("cuda")
+    outs = [torch.empty(n * L, dtype=torch.float32, device="cuda") for _ in range(nw)]
+    step = table_eval_fn(lib, htable, d_src, D(0, L, L), outs, [D(0, L, L)] * nw, s)
+    alg = stencil_bytes(n, nnz, nw, L, src_np.shape[0], nw)
+    # autotune: a few untimed launches of each kernel variant, then the
+    # timed region runs exactly `steps` launches of the fastest
     variants = {}
-    results = {}
-    for name, kernel, fn in [("table_csr_lds", 1, table_step), ("raw_csr_lds", None, raw_step),
-                             ("table_sliced", 0, table_step)]:
-        if kernel is not None:
-            lib.osdhip_config(b"stencil_kernel", kernel)
-        wall, ms = time_launches(fn, args.steps, args.warmup, stream)
-        avg = float(np.mean(ms))
-        variants[name] = {"kernel_ms": avg, "GBps": alg / (avg * 1e-3) / 1e9,
-                          "points_per_s": n * args.steps / wall}
-        results[name] = (wall, ms, d_dst.cpu().numpy().reshape(n, L).copy())
-    best = "table_sliced"                       # HipEvaluator::EvalStencils path
-    wall, ms, got = results[best]
-    avg_ms = float(np.mean(ms))
-    achieved = alg / (avg_ms * 1e-3) / 1e9
-    out = {
-        "metric": "limit points/s + HBM GB/s vs roofline; served p50/p99 latency at fixed QPS",
-        "value": n * args.steps / wall,
-        "unit": "points/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "cfg1: cube-sphere 6x128x128 quads, 5% creased edges, uniform "
-                               "level-3 stencil table, xyz, one mesh per launch",
-                   "stencils": n, "table_entries": nnz, "control_points": n_src,
-                   "floats_per_point": L, "path": "HipStencilTable device layout"},
-    }
-    traffic = None
-    if os.path.exists(PROFILE_TRAFFIC):
-        tr = json.load(open(PROFILE_TRAFFIC)).get("cfg1_stencils")
-        if tr:
-            traffic = tr["bytes_per_launch"]
-    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                       "kernel": "k_stencils_sliced<1,3>", "algorithmic_bytes": alg,
-                       "kernel_ms": avg_ms}
-    out["variants"] = variants
-    if not args.no_cpu:
-        cb = cpu_baseline_stencils(src_np, table, L)
-        ref_out = cb.pop("dst")
-        nthreads = os.cpu_count() or 1
-        omp = cpu_baseline_stencils(src_np, table, L, budget_s=6.0, threads=min(nthreads, 16))
-        omp.pop("dst")
-        cb["omp"] = {k: omp[k] for k in ("value", "cores", "kind", "sample")}
-        out["cpu_baseline"] = cb
-        out["parity"] = {k: bool(np.array_equal(r[2], ref_out)) for k, r in results.items()}
-    else:
-        out["cpu_baseline"] = None
-    return out
+    for vname, kernel in (("csr_lds", 1), ("sliced", 0)):
+        lib.osdhip_config(b"stencil_kernel", kernel)
+        _, ms = time_launches(step, 5, 2, stream)
+        avg = float(np.median(ms))
+        variants[vname] = {"kernel_ms": avg, "GBps": alg / (avg * 1e-3) / 1e9}
+    best = min(variants, key=lambda k: variants[k]["kernel_ms"])
+    lib.osdhip_config(b"stencil_kernel", 0 if best == "sliced" else 1)
+    wall, ms = time_launches(step, args.steps, args.warmup, stream)
+    got = [o.cpu().numpy().reshape(n, L) for o in outs]
+    avg = float(np.mean(ms))
+    res = {"points_per_s": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
+           "kernel_ms": avg, "stencils": n, "table_entries": nnz, "floats_per_point": L,
+           "outputs": nw, "algorithmic_bytes": alg, "GBps": alg / (avg * 1e-3) / 1e9,
+           "kernel": f"{kernel_label} ({best})",
+           "autotune": variants}
+    if with_cpu:
+        from oracle import ref
+        import oracle
+        dst = [np.zeros((n, L), np.float32) for _ in range(nw)]
+        if ref.available():
+            kind = "reference"
+            if nw == 1:
+                run = lambda: ref.cpu_eval_stencils(src_np, (0, L, L), dst[0], (0, L, L),  # noqa
+                                                    table_np)
+            else:
+                run = lambda: ref.cpu_eval_stencils_d1(src_np, (0, L, L), dst, [(0, L, L)] * 3,  # noqa
+                                                       table_np)
+        else:
+            kind = "port"
+            if nw == 1:
+                run = lambda: oracle.eval_stencils(src_np, (0, L, L), dst[0], (0, L, L),  # noqa
+                                                   table_np["sizes"], table_np["offsets"],
+                                                   table_np["indices"], table_np["weights"])
+            else:
+                run = lambda: oracle.eval_stencils_d1(  # noqa
+                    src_np, (0, L, L), dst, [(0, L, L)] * 3, table_np["sizes"],
+                    table_np["offsets"], table_np["indices"],
+                    [table_np["weights"], table_np["du"], table_np["dv"]])
+        med, runs = cpu_time(run, args.cpu_budget)
+        res["cpu_baseline"] = {"value": n / med, "unit": "points/s", "cores": 1, "kind": kind,
+                               "sample": f"full table ({n} stencils, {nw} outputs) x {runs} "
+                                         f"runs, median {med * 1e3:.1f} ms"}
+        if nw == 1 and ref.available():
+            threads = min(os.cpu_count() or 1, 16)
+            omp = lambda: ref.cpu_eval_stencils(src_np, (0, L, L), dst[0], (0, L, L),  # noqa
+                                                table_np, threads=threads)
+            m2, r2 = cpu_time(omp, args.cpu_budget / 2)
+            res["cpu_baseline"]["omp"] = {"value": n / m2, "cores": threads,
+                                          "kind": "reference OmpEvaluator",
+                                          "sample": f"{r2} runs, median {m2 * 1e3:.1f} ms"}
+            want = [np.zeros((n, L), np.float32)]
+            ref.cpu_eval_stencils(src_np, (0, L, L), want[0], (0, L, L), table_np)
+        else:
+            want = dst
+        res["parity"] = {"bit_exact_vs_cpu": all(np.array_equal(a, b) for a, b in zip(got, want))}
+    del htable
+    return res
+
+
+def run_patches(args, lib, inputs, with_cpu):
+    import torch
+    from opensubdiv_amd import _lib
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+    D = _lib.Desc
+    mesh, pt, pts, coords = inputs["mesh"], inputs["patches"], inputs["points"], inputs["coords"]
+    import oracle
+    ctrl = mesh.positions.astype(np.float32)
+    refined = oracle.apply_stencils(ctrl, pts["sizes"], pts["offsets"], pts["indices"],
+                                    pts["weights"])
+    src_np = np.ascontiguousarray(np.concatenate([ctrl, refined]))
+    L, n = 3, len(coords)
+
+    def dev(a):
+        a = np.ascontiguousarray(a)
+        if a.dtype.fields is not None:
+            a = a.view(np.int32)
+        return torch.from_numpy(a.copy()).to("cuda")
+    d_src, d_coords = dev(src_np), dev(coords)
+    d_arrays, d_indices, d_params = dev(pt.arrays), dev(pt.indices), dev(pt.params)
+    outs = [torch.empty(n * L, dtype=torch.float32, device="cuda") for _ in range(3)]
+    d3 = D(0, L, L)
+    z = D(0, 0, 0)
+
+    def step():
+        rc = lib.osdhip_eval_patches(d_src.data_ptr(), d3, outs[0].data_ptr(), d3,
+                                     outs[1].data_ptr(), d3, outs[2].data_ptr(), d3,
+                                     None, z, None, z, None, z, n, d_coords.data_ptr(),
+                                     d_arrays.data_ptr(), d_indices.data_ptr(),
+                                     d_params.data_ptr(), s)
+        if rc != 0:
+            raise RuntimeError(f"osdhip_eval_patches failed: {rc}")
+    wall, ms = time_launches(step, args.steps, args.warmup, stream)
+    avg = float(np.mean(ms))
+    alg = patch_bytes(n, L, 3, len(pt.indices), len(pt.params), len(pt.arrays), len(src_np))
+    res = {"points_per_s": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
+           "kernel_ms": avg, "patch_coords": n, "patches": len(pt.params),
+           "gregory_patches": int(sum(a["numPatches"] for a in pt.arrays if a["desc"] == 9)),
+           "algorithmic_bytes": alg, "GBps": alg / (avg * 1e-3) / 1e9,
+           "kernel": "k_patches<true,false,3>"}
+    if with_cpu:
+        from oracle import ref
+        got = [o.cpu().numpy().reshape(n, L) for o in outs]
+        dst = [np.zeros((n, L), np.float32) for _ in range(3)]
+        if ref.available():
+            kind = "reference"
+            run = lambda: ref.cpu_eval_patches(src_np, (0, L, L), dst, [(0, L, L)] * 3,  # noqa
+                                               coords, pt.arrays, pt.indices, pt.params)
+        else:
+            kind = "port"
+            run = lambda: oracle.eval_patches(src_np, (0, L, L), dst, [(0, L, L)] * 3,  # noqa
+                                              coords, pt.arrays, pt.indices, pt.params)
+        med, runs = cpu_time(run, args.cpu_budget, max_runs=3)
+        res["cpu_baseline"] = {"value": n / med, "unit": "points/s", "cores": 1, "kind": kind,
+                               "sample": f"all {n} coords x {runs} runs, median "
+                                         f"{med * 1e3:.1f} ms"}
+        res["parity"] = {"bit_exact_vs_cpu": all(np.array_equal(a, b) for a, b in zip(got, dst))}
+    return res
 
 
 def main():
@@ -203,8 +295,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="cfg1", choices=["cfg1"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds per CPU baseline")
+    ap.add_argument("--only", default=None, choices=[None, "cfg1", "cfg1_refine", "cfg2"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -214,7 +307,45 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
-    out = run_cfg1(args)
+    from opensubdiv_amd import _lib
+    lib = _lib.load_hip()
+    name = ctypes.create_string_buffer(256)
+    lib.osdhip_device_info(name, 256)
+    want = {"cfg1", "cfg1_refine", "cfg2"} if args.only is None else {args.only, "cfg1"}
+    inputs = build_inputs(need_refine="cfg1_refine" in want, need_patches="cfg2" in want)
+    with_cpu = not args.no_cpu
+    src = inputs["mesh"].positions.astype(np.float32)
+    workloads = {}
+    workloads["cfg1"] = run_stencils(args, lib, "cfg1 limit stencils", inputs["limit"], src, 3,
+                                     "k_stencils_{sliced,csr}<3,3>", with_cpu)
+    if "cfg1_refine" in want:
+        workloads["cfg1_refine"] = run_stencils(args, lib, "cfg1 refinement", inputs["refine"],
+                                                src, 1, "k_stencils_{sliced,csr}<1,3>", with_cpu)
+    if "cfg2" in want:
+        workloads["cfg2"] = run_patches(args, lib, inputs, with_cpu)
+    head = workloads["cfg1"]
+    out = {
+        "metric": METRIC,
+        "value": head["points_per_s"],
+        "unit": "points/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "cfg1: cube-sphere 6x128x128 quads (98,306 verts), 5% creased "
+                               "edges, limit stencils (P, du, dv) at the 6,291,458 uniform "
+                               "level-3 vertex locations, one fused launch per frame",
+                   "stencils": head["stencils"], "table_entries": head["table_entries"],
+                   "floats_per_point": 3, "device": name.value.decode()},
+        "roofline": {"bound": "hbm", "achieved": head["GBps"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": head["GBps"] / HBM_PEAK_GBS,
+                     "traffic": traffic_for("cfg1_limit"), "kernel": head["kernel"],
+                     "algorithmic_bytes": head["algorithmic_bytes"],
+                     "kernel_ms": head["kernel_ms"]},
+        "cpu_baseline": head.get("cpu_baseline"),
+        "parity": head.get("parity"),
+        "workloads": {k: v for k, v in workloads.items() if k != "cfg1"},
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -508,8 +508,7 @@ int PatchTable::limitStencils(int n, const int *ptex, const double *u, const dou
         for (int i = lo; i < hi; ++i) {
             PatchHandleRec h;
             if (!findPatch(ptex[i], u[i], v[i], h)) {
-                tSz[t].push_back(0);
-                ++missing;
+                ++missing;                   // no stencil, as Far drops the location
                 continue;
             }
             const int np = evalBasis(h.patchIndex, u[i], v[i], w[0], w[1], w[2],

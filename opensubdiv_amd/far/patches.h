@@ -69,7 +69,7 @@ public:
 
     // LimitStencilTableFactory equivalent: one stencil (P, du, dv[, duu,
     // duv, dvv]) per (ptex, u, v) location, factorized to control vertices.
-    // Locations outside every patch give empty stencils; returns their count.
+    // Locations outside every patch get no stencil (as in Far); returns their count.
     template <typename REAL>
     int limitStencils(int n, const int *ptex, const double *u, const double *v, bool second,
                       StencilTableT<REAL> &out) const;

@@ -360,7 +360,9 @@ int ref_limit_points_uniform(void *topo, int level, const double *coarse,
                              double *pos, double *tan1, double *tan2)
 {
     Far::TopologyRefiner *r = ((Topo *)topo)->makeRefiner();
-    r->RefineUniform(Far::TopologyRefiner::UniformOptions(level));
+    Far::TopologyRefiner::UniformOptions uo(level);
+    uo.fullTopologyInLastLevel = true;      // required by PrimvarRefiner::Limit
+    r->RefineUniform(uo);
     int total = r->GetNumVerticesTotal();
     std::vector<V3> v(total);
     std::memcpy(v.data(), coarse, sizeof(double) * 3 * r->GetLevel(0).GetNumVertices());
