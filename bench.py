@@ -59,10 +59,13 @@ def build_inputs(need_refine=True, need_patches=True):
     topo.refine_uniform(3)
     if need_refine:
         out["refine"] = topo.stencil_table()
+        log(f"[bench] refinement table built ({time.time() - t0:.1f}s)")
     ptex, u, v = topo.vertex_locations(3)
     pt = far.Topology(mesh).patch_table(3)
     out["patches"] = pt
+    log(f"[bench] patch table built ({time.time() - t0:.1f}s)")
     out["limit"] = pt.limit_stencils(ptex, u, v)
+    log(f"[bench] limit table built ({time.time() - t0:.1f}s)")
     if need_patches:
         out["points"] = pt.point_stencils()
         f, s, t = meshes.random_patch_locations(mesh.nfaces, 4194304, seed=2)
@@ -124,10 +127,12 @@ def cpu_time(run, budget_s, max_runs=9):
     return float(np.median(times)), len(times)
 
 
-def traffic_for(key):
+def traffic_for(key, kernel):
+    """HBM bytes per launch from the committed counter profile, when it was
+    measured on the same kernel (template arguments included)."""
     if os.path.exists(PROFILE_TRAFFIC):
         tr = json.load(open(PROFILE_TRAFFIC)).get(key)
-        if tr:
+        if tr and tr.get("kernel_short") == kernel:
             return tr.get("bytes_per_launch")
     return None
 
@@ -149,6 +154,11 @@ def table_eval_fn(lib, table, src, srcdesc, outs, descs, stream):
     return step
 
 
+def kernel_labels(nw, lc):
+    return {"rec_g8": f"k_stencils_rec<{nw},{lc},8>", "rec_g4": f"k_stencils_rec<{nw},{lc},4>",
+            "split": f"k_stencils_sliced<{nw},{lc}>", "csr_lds": f"k_stencils_csr<{nw},{lc}>"}
+
+
 def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu):
     """EvalStencils of a (limit) stencil table through HipStencilTable."""
     import torch
@@ -160,30 +170,54 @@ def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu):
     n = len(table_np["sizes"])
     nnz = len(table_np["indices"])
     t0 = time.time()
+    lib.osdhip_config(b"table_layout", 0)
     htable = HipStencilTable.Create(table_np, s)
     log(f"[bench] {name}: device table {htable.device_bytes() / 1e6:.0f} MB in "
         f"{time.time() - t0:.1f}s")
     d_src = torch.from_numpy(np.ascontiguousarray(src_np)).to("cuda")
     outs = [torch.empty(n * L, dtype=torch.float32, device="cuda") for _ in range(nw)]
-    step = table_eval_fn(lib, htable, d_src, D(0, L, L), outs, [D(0, L, L)] * nw, s)
     alg = stencil_bytes(n, nnz, nw, L, src_np.shape[0], nw)
-    # autotune: a few untimed launches of each kernel variant, then the
-    # timed region runs exactly `steps` launches of the fastest
+    # autotune: a few untimed launches of each kernel variant (record slices
+    # with gather groups of 8 / 4, split-stream slices, CSR with LDS
+    # staging), then the timed region runs exactly `steps` launches of the
+    # fastest
+    VARIANTS = {"rec_g8": (0, 8, 0), "rec_g4": (0, 4, 0), "split": (1, 8, 0),
+                "csr_lds": (0, 8, 1)}
     variants = {}
-    for vname, kernel in (("csr_lds", 1), ("sliced", 0)):
+    for vname, (layout, group, kernel) in VARIANTS.items():
+        lib.osdhip_config(b"slice_group", group)
         lib.osdhip_config(b"stencil_kernel", kernel)
-        _, ms = time_launches(step, 5, 2, stream)
+        tab = htable
+        if layout != 0:
+            lib.osdhip_config(b"table_layout", layout)
+            tab = HipStencilTable.Create(table_np, s)
+            lib.osdhip_config(b"table_layout", 0)
+        fn = table_eval_fn(lib, tab, d_src, D(0, L, L), outs, [D(0, L, L)] * nw, s)
+        _, ms = time_launches(fn, 5, 2, stream)
         avg = float(np.median(ms))
         variants[vname] = {"kernel_ms": avg, "GBps": alg / (avg * 1e-3) / 1e9}
+        if tab is not htable:
+            del tab, fn
+            torch.cuda.synchronize()
     best = min(variants, key=lambda k: variants[k]["kernel_ms"])
-    lib.osdhip_config(b"stencil_kernel", 0 if best == "sliced" else 1)
+    layout, group, kernel = VARIANTS[best]
+    if layout != 0:
+        del htable
+        lib.osdhip_config(b"table_layout", layout)
+        htable = HipStencilTable.Create(table_np, s)
+        lib.osdhip_config(b"table_layout", 0)
+    lib.osdhip_config(b"slice_group", group)
+    lib.osdhip_config(b"stencil_kernel", kernel)
+    step = table_eval_fn(lib, htable, d_src, D(0, L, L), outs, [D(0, L, L)] * nw, s)
     wall, ms = time_launches(step, args.steps, args.warmup, stream)
+    lib.osdhip_config(b"slice_group", 8)
+    lib.osdhip_config(b"stencil_kernel", 0)
     got = [o.cpu().numpy().reshape(n, L) for o in outs]
     avg = float(np.mean(ms))
     res = {"points_per_s": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
            "kernel_ms": avg, "stencils": n, "table_entries": nnz, "floats_per_point": L,
            "outputs": nw, "algorithmic_bytes": alg, "GBps": alg / (avg * 1e-3) / 1e9,
-           "kernel": f"{kernel_label} ({best})",
+           "kernel": f"{kernel_label[best]} ({best})",
            "autotune": variants}
     if with_cpu:
         from oracle import ref
@@ -236,11 +270,23 @@ def run_patches(args, lib, inputs, with_cpu):
     s = stream.cuda_stream
     D = _lib.Desc
     mesh, pt, pts, coords = inputs["mesh"], inputs["patches"], inputs["points"], inputs["coords"]
-    import oracle
+    from opensubdiv_amd import HipStencilTable
     ctrl = mesh.positions.astype(np.float32)
-    refined = oracle.apply_stencils(ctrl, pts["sizes"], pts["offsets"], pts["indices"],
-                                    pts["weights"])
-    src_np = np.ascontiguousarray(np.concatenate([ctrl, refined]))
+    # vertex buffer [control | refined | end-cap points], filled on the device
+    # by the patch-point stencils (the Osd::Mesh::Refine step)
+    nctrl, npts = len(ctrl), len(pts["sizes"])
+    d_buf = torch.zeros((nctrl + npts) * 3, dtype=torch.float32, device="cuda")
+    d_buf[:nctrl * 3] = torch.from_numpy(np.ascontiguousarray(ctrl).reshape(-1)).to("cuda")
+    ptab = HipStencilTable.Create(pts, s)
+    outp = (ctypes.c_void_p * 1)(d_buf.data_ptr())
+    descp = (D * 1)(D(nctrl * 3, 3, 3))
+    rc = lib.osdhip_stencil_table_eval(ptab.h, d_buf.data_ptr(), D(0, 3, 3), 1,
+                                       ctypes.cast(outp, ctypes.c_void_p),
+                                       ctypes.cast(descp, ctypes.c_void_p), 0, npts, s)
+    assert rc == 0, rc
+    torch.cuda.synchronize()
+    del ptab
+    d_src = d_buf
     L, n = 3, len(coords)
 
     def dev(a):
@@ -248,7 +294,7 @@ def run_patches(args, lib, inputs, with_cpu):
         if a.dtype.fields is not None:
             a = a.view(np.int32)
         return torch.from_numpy(a.copy()).to("cuda")
-    d_src, d_coords = dev(src_np), dev(coords)
+    d_coords = dev(coords)
     d_arrays, d_indices, d_params = dev(pt.arrays), dev(pt.indices), dev(pt.params)
     outs = [torch.empty(n * L, dtype=torch.float32, device="cuda") for _ in range(3)]
     d3 = D(0, L, L)
@@ -262,15 +308,28 @@ def run_patches(args, lib, inputs, with_cpu):
                                      d_params.data_ptr(), s)
         if rc != 0:
             raise RuntimeError(f"osdhip_eval_patches failed: {rc}")
+    alg = patch_bytes(n, L, 3, len(pt.indices), len(pt.params), len(pt.arrays), nctrl + npts)
+    # autotune: caller order vs patch-sorted order (radix sort inside each
+    # launch, included in the timed step)
+    variants = {}
+    for vname, smin in (("caller_order", 0), ("patch_sorted", 16384)):
+        lib.osdhip_config(b"patch_sort_min", smin)
+        _, ms = time_launches(step, 5, 2, stream)
+        avg = float(np.median(ms))
+        variants[vname] = {"step_ms": avg, "GBps": alg / (avg * 1e-3) / 1e9}
+    best = min(variants, key=lambda k: variants[k]["step_ms"])
+    lib.osdhip_config(b"patch_sort_min", 0 if best == "caller_order" else 16384)
     wall, ms = time_launches(step, args.steps, args.warmup, stream)
     avg = float(np.mean(ms))
-    alg = patch_bytes(n, L, 3, len(pt.indices), len(pt.params), len(pt.arrays), len(src_np))
+    lib.osdhip_config(b"patch_sort_min", 0)
     res = {"points_per_s": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
            "kernel_ms": avg, "patch_coords": n, "patches": len(pt.params),
            "gregory_patches": int(sum(a["numPatches"] for a in pt.arrays if a["desc"] == 9)),
            "algorithmic_bytes": alg, "GBps": alg / (avg * 1e-3) / 1e9,
-           "kernel": "k_patches<true,false,3>"}
+           "kernel": f"k_patches<true,false,3> ({best})", "autotune": variants}
     if with_cpu:
+        src_np = d_src.cpu().numpy().reshape(-1, 3)
+        import oracle
         from oracle import ref
         got = [o.cpu().numpy().reshape(n, L) for o in outs]
         dst = [np.zeros((n, L), np.float32) for _ in range(3)]
@@ -290,6 +349,22 @@ def run_patches(args, lib, inputs, with_cpu):
     return res
 
 
+def copy_probe(args):
+    """1 GiB device-to-device copy: reads and writes exactly 2^30 bytes per
+    launch, so its FETCH_SIZE / WRITE_SIZE calibrate the counters and its
+    rate is the measured copy bandwidth (read + write bytes / time)."""
+    import torch
+    n = 1 << 28
+    a = torch.rand(n, device="cuda")
+    b = torch.empty_like(a)
+    stream = torch.cuda.current_stream()
+    wall, ms = time_launches(lambda: b.copy_(a), args.steps, args.warmup, stream)
+    kms = float(np.median(ms))
+    print(json.dumps({"probe": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
+                      "kernel_ms": kms, "GBps": 8 * n / (kms * 1e-3) / 1e9,
+                      "steps": args.steps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -298,6 +373,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds per CPU baseline")
     ap.add_argument("--only", default=None, choices=[None, "cfg1", "cfg1_refine", "cfg2"])
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="time a plain device-to-device copy of 1 GiB instead (HBM copy "
+                         "bandwidth; the reference kernel for FETCH_SIZE/WRITE_SIZE scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -307,6 +385,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe:
+        copy_probe(args)
+        return
     from opensubdiv_amd import _lib
     lib = _lib.load_hip()
     name = ctypes.create_string_buffer(256)
@@ -317,10 +398,10 @@ def main():
     src = inputs["mesh"].positions.astype(np.float32)
     workloads = {}
     workloads["cfg1"] = run_stencils(args, lib, "cfg1 limit stencils", inputs["limit"], src, 3,
-                                     "k_stencils_{sliced,csr}<3,3>", with_cpu)
+                                     kernel_labels(3, 3), with_cpu)
     if "cfg1_refine" in want:
         workloads["cfg1_refine"] = run_stencils(args, lib, "cfg1 refinement", inputs["refine"],
-                                                src, 1, "k_stencils_{sliced,csr}<1,3>", with_cpu)
+                                                src, 1, kernel_labels(1, 3), with_cpu)
     if "cfg2" in want:
         workloads["cfg2"] = run_patches(args, lib, inputs, with_cpu)
     head = workloads["cfg1"]
@@ -339,7 +420,8 @@ def main():
                    "floats_per_point": 3, "device": name.value.decode()},
         "roofline": {"bound": "hbm", "achieved": head["GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["GBps"] / HBM_PEAK_GBS,
-                     "traffic": traffic_for("cfg1_limit"), "kernel": head["kernel"],
+                     "traffic": traffic_for("cfg1_limit", head["kernel"].split(" ")[0]),
+                     "kernel": head["kernel"],
                      "algorithmic_bytes": head["algorithmic_bytes"],
                      "kernel_ms": head["kernel_ms"]},
         "cpu_baseline": head.get("cpu_baseline"),

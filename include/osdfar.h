@@ -24,6 +24,11 @@ extern "C" {
 typedef struct osdfar_topology osdfar_topology;
 typedef struct osdfar_stencil_table osdfar_stencil_table;
 
+/* Host threads the calling thread's builds use (0 = all available); a
+ * worker pool that compiles several meshes at once sets 1 in each worker.
+ * Returns 0.  No counterpart in the reference (Far is single-threaded). */
+int osdfar_set_threads(int n);
+
 /* TopologyDescriptor + Sdc::Options::VtxBoundaryInterpolation
  * (0 none, 1 edge only, 2 edge and corner); creasing is uniform. Returns
  * NULL for input the compiler does not handle (non-manifold edges). */

@@ -166,8 +166,24 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
                         void *stream);
 
 /* Tuning knobs; returns the previous value (or an error code).
- *   "stencil_kernel": 0 = sliced layout (default), 1 = CSR with LDS staging
- *                     for osdhip_stencil_table_eval. */
+ *   "stencil_kernel": 0 = slice layout (default), 1 = CSR with LDS staging
+ *                     for osdhip_stencil_table_eval.
+ *   "table_layout":   slice layout of tables created afterwards: 0 = one
+ *                     interleaved record {index, weights} per entry
+ *                     (default), 1 = index and weights in separate streams.
+ *   "slice_group":    entries per gather group of the record kernel, 4 or 8
+ *                     (default 8).
+ *   "table_csr_mirror": 1 (default) = tables created afterwards keep the Far
+ *                     CSR arrays on the device (osdhip_stencil_table_csr,
+ *                     the CSR kernel); 0 = only the slice layout is kept
+ *                     (the CSR getters then return NULL and the table is
+ *                     always evaluated with the slice kernels).
+ *   "patch_sort_min": osdhip_eval_patches evaluates coord sets of at least
+ *                     this many coords in patch order (a radix sort in
+ *                     stream-ordered scratch memory); 0 = never (default:
+ *                     on MI355X the sort costs more than the locality it
+ *                     buys for uniformly random coords).  Results are
+ *                     identical either way. */
 int osdhip_config(const char *key, int value);
 
 /* ---- device memory and streams -------------------------------------------
@@ -186,6 +202,70 @@ int osdhip_device_synchronize(void);
 int osdhip_device_info(char *buf, int buflen);
 /* Library version string, e.g. "osdhip 0.1 gfx950". */
 const char *osdhip_version(void);
+
+/* ---- batched evaluation -----------------------------------------------
+ * No counterpart in the reference (its evaluators take one mesh and one
+ * frame per call).  For callers that evaluate many meshes, each with many
+ * frames of control points, per step: the batch captures all of it into
+ * one HIP graph (phase A: every stencil segment; phase B: every patch
+ * segment, which may read the vertex buffers phase A writes), replayed by
+ * osdhip_batch_launch.  Element (v, f, k) of a multi-frame buffer is at
+ * base + v * vertexStride + f * frameStride + k (floats); `length` floats
+ * (1..8) per element.  Results equal osdhip_stencil_table_eval /
+ * osdhip_eval_patches of each (segment, frame) bit for bit.  The batch
+ * keeps the pointers it is given: buffers and tables must outlive it. */
+typedef struct osdhip_batch osdhip_batch;
+
+typedef struct {
+    const osdhip_stencil_table *table;   /* one weight array, record layout */
+    const float *src;
+    long long srcVertexStride, srcFrameStride;
+    float *dst;                          /* row r of the table -> element r */
+    long long dstVertexStride, dstFrameStride;
+    int frames;
+} osdhip_batch_stencils;
+
+typedef struct {
+    int numCoords;
+    const osdhip_patch_coord *coords;
+    const osdhip_patch_array *arrays;
+    const int *patchIndices;
+    const osdhip_patch_param *params;
+    const float *src;                    /* patch points [control | refined | end cap] */
+    long long srcVertexStride, srcFrameStride;
+    float *P, *dPdu, *dPdv;              /* NULL = not written; coord c -> element c */
+    long long outVertexStride, outFrameStride;
+    int frames;
+} osdhip_batch_patches;
+
+/* NULL on invalid segments (tables not in the record layout or with more
+ * than one weight array, strides below `length`) or allocation failure. */
+osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
+                                  const osdhip_batch_stencils *stencils,
+                                  int numPatchSegments, const osdhip_batch_patches *patches,
+                                  void *stream);
+void osdhip_batch_destroy(osdhip_batch *batch);
+/* One graph launch (reset, phase A, phase B) on `stream`. */
+int osdhip_batch_launch(osdhip_batch *batch, void *stream);
+/* The same kernels without the graph: phases 1 = A, 2 = B, 3 = both. */
+int osdhip_batch_run(osdhip_batch *batch, int phases, void *stream);
+/* The same work one mesh and one frame per launch through
+ * osdhip_stencil_table_eval and osdhip_eval_patches (the plain evaluator
+ * path), for comparison: every stencil segment, then every patch segment. */
+int osdhip_batch_run_plain(osdhip_batch *batch, void *stream);
+/* Variant knobs: "a_frames" x "a_group" in {1x4, 1x8, 2x4, 4x2} (frames
+ * per chunk, gather group), "a_item_slices" (64-row slices per work item),
+ * "a_persistent" / "b_persistent" (0 = one block per item, 1 = fixed grid
+ * claiming items from a device counter), "b_frames" in {1, 2, 4}.  The
+ * graph is re-captured on the next launch. */
+int osdhip_batch_set(osdhip_batch *batch, const char *key, int value);
+int osdhip_batch_get(const osdhip_batch *batch, const char *key);
+int osdhip_batch_info(const osdhip_batch *batch, long long *stencilRows, long long *frameRows,
+                      long long *patchCoords, long long *patchFrames, long long *tableBytes);
+/* Autotuner: times every variant of each phase (median of `reps` runs with
+ * HIP events on `stream`), keeps the fastest and re-captures the graph.
+ * `report` (optional) receives the timings as JSON. */
+int osdhip_batch_tune(osdhip_batch *batch, int reps, void *stream, char *report, int reportBytes);
 
 #ifdef __cplusplus
 }

@@ -32,6 +32,25 @@ def desc(d) -> Desc:
     return Desc(int(d.offset), int(d.length), int(d.stride))
 
 
+class BatchStencils(C.Structure):
+    """osdhip_batch_stencils."""
+    _fields_ = [("table", C.c_void_p), ("src", C.c_void_p),
+                ("srcVertexStride", C.c_longlong), ("srcFrameStride", C.c_longlong),
+                ("dst", C.c_void_p),
+                ("dstVertexStride", C.c_longlong), ("dstFrameStride", C.c_longlong),
+                ("frames", C.c_int)]
+
+
+class BatchPatches(C.Structure):
+    """osdhip_batch_patches."""
+    _fields_ = [("numCoords", C.c_int), ("coords", C.c_void_p), ("arrays", C.c_void_p),
+                ("patchIndices", C.c_void_p), ("params", C.c_void_p), ("src", C.c_void_p),
+                ("srcVertexStride", C.c_longlong), ("srcFrameStride", C.c_longlong),
+                ("P", C.c_void_p), ("dPdu", C.c_void_p), ("dPdv", C.c_void_p),
+                ("outVertexStride", C.c_longlong), ("outFrameStride", C.c_longlong),
+                ("frames", C.c_int)]
+
+
 _vp = C.c_void_p
 _i = C.c_int
 _D = Desc
@@ -59,6 +78,15 @@ _HIP_SIGS = {
     "osdhip_device_synchronize": [],
     "osdhip_device_info": [C.c_char_p, _i],
     "osdhip_version": [],
+    "osdhip_batch_create": [_i, _i, _vp, _i, _vp, _vp],
+    "osdhip_batch_destroy": [_vp],
+    "osdhip_batch_launch": [_vp, _vp],
+    "osdhip_batch_run": [_vp, _i, _vp],
+    "osdhip_batch_run_plain": [_vp, _vp],
+    "osdhip_batch_set": [_vp, C.c_char_p, _i],
+    "osdhip_batch_get": [_vp, C.c_char_p],
+    "osdhip_batch_info": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "osdhip_batch_tune": [_vp, _i, _vp, _vp, _i],
 }
 
 _hip = None
@@ -81,9 +109,9 @@ def load_hip():
         fn.argtypes = args
         if name == "osdhip_version":
             fn.restype = C.c_char_p
-        elif name == "osdhip_stencil_table_create":
+        elif name in ("osdhip_stencil_table_create", "osdhip_batch_create"):
             fn.restype = C.c_void_p
-        elif name == "osdhip_stencil_table_destroy":
+        elif name in ("osdhip_stencil_table_destroy", "osdhip_batch_destroy"):
             fn.restype = None
         else:
             fn.restype = C.c_int

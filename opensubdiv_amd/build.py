@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "lib")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(ROOT, "include")]
@@ -36,11 +36,20 @@ def _run(cmd):
 
 
 def build_hip(verbose=False):
-    os.makedirs(LIB, exist_ok=True)
-    src = os.path.join(HERE, "csrc", "osdhip.hip")
+    """Every csrc/*.hip to an object (in parallel), then one shared library."""
+    os.makedirs(os.path.join(LIB, "obj"), exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(HERE, "csrc", "*.hip")))
+    objs = [os.path.join(LIB, "obj", os.path.basename(s)[:-4] + ".o") for s in srcs]
+    procs = [(subprocess.Popen([HIPCC, *HIP_FLAGS, "-c", "-o", o, s], stdout=subprocess.PIPE,
+                               stderr=subprocess.STDOUT, text=True), s) for s, o in zip(srcs, objs)]
+    for p, s in procs:
+        text, _ = p.communicate()
+        if p.returncode != 0:
+            text = "\n".join(l for l in text.splitlines() if "packed-fp32-ops" not in l)
+            raise RuntimeError(f"build failed: {s}\n{text}")
     out = os.path.join(LIB, "libosdhip.so")
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-o", tmp, src])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, out)
     if verbose:
         print("built", out)

@@ -1,0 +1,545 @@
+// Batched evaluation (osdhip_batch_*): segmented stencil and patch work of
+// many meshes and frames, captured once into a HIP graph of
+//   [counter reset] -> phase A (k_batch_stencils) -> phase B (k_batch_patches)
+// and replayed by osdhip_batch_launch.  Tuning knobs select the kernel
+// variant (frames per chunk, gather group, slices per work item, static or
+// persistent scheduling); osdhip_batch_tune times the variants on the
+// batch's own data and keeps the fastest.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/osdhip.h"
+#include "batch_kernels.h"
+#include "common.h"
+#include "table.h"
+
+using namespace osdhip;
+
+struct BatchConfig {
+    int aFrames = 2, aGroup = 4, aItemSlices = 8, aPersistent = 0;
+    int bFrames = 1, bPersistent = 0;
+};
+
+struct osdhip_batch {
+    int length = 0;
+    int nS = 0, nP = 0;
+    long long stencilRows = 0, frameRows = 0, patchCoords = 0, patchFrames = 0, tableBytes = 0;
+    BatchStencilSeg *dS = nullptr;
+    BatchPatchSeg *dP = nullptr;
+    std::vector<BatchStencilSeg> hS;
+    std::vector<BatchPatchSeg> hP;
+    std::vector<const osdhip_stencil_table *> tables;   // per stencil segment (plain path)
+    BatchItem *itemsA = nullptr, *itemsB = nullptr;
+    int nItemsA = 0, nItemsB = 0, itemsASlices = 0;
+    unsigned *counters = nullptr;          // [0] phase A, [1] phase B
+    int numCUs = 256;
+    BatchConfig cfg;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+template <int LC, int FC, int G>
+void launchA(const BatchArgs &a, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_batch_stencils<LC, FC, G>), dim3(grid), dim3(kBlock), 0, st, a);
+}
+
+template <int FC, int G>
+bool launchA_lc(int lc, const BatchArgs &a, int grid, hipStream_t st)
+{
+    switch (lc) {
+    case 1: launchA<1, FC, G>(a, grid, st); return true;
+    case 2: launchA<2, FC, G>(a, grid, st); return true;
+    case 3: launchA<3, FC, G>(a, grid, st); return true;
+    case 4: launchA<4, FC, G>(a, grid, st); return true;
+    case 5: launchA<5, FC, G>(a, grid, st); return true;
+    case 6: launchA<6, FC, G>(a, grid, st); return true;
+    case 7: launchA<7, FC, G>(a, grid, st); return true;
+    case 8: launchA<8, FC, G>(a, grid, st); return true;
+    default: return false;
+    }
+}
+
+// phase A variants: (frames per chunk, gather group)
+bool launchA_variant(int lc, int fc, int g, const BatchArgs &a, int grid, hipStream_t st)
+{
+    if (fc == 1 && g == 4) return launchA_lc<1, 4>(lc, a, grid, st);
+    if (fc == 1 && g == 8) return launchA_lc<1, 8>(lc, a, grid, st);
+    if (fc == 2 && g == 4) return launchA_lc<2, 4>(lc, a, grid, st);
+    if (fc == 4 && g == 2) return launchA_lc<4, 2>(lc, a, grid, st);
+    return false;
+}
+
+template <int LC, int FC>
+void launchB(const BatchArgs &a, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_batch_patches<LC, FC>), dim3(grid), dim3(kBlock), 0, st, a);
+}
+
+template <int FC>
+bool launchB_lc(int lc, const BatchArgs &a, int grid, hipStream_t st)
+{
+    switch (lc) {
+    case 1: launchB<1, FC>(a, grid, st); return true;
+    case 2: launchB<2, FC>(a, grid, st); return true;
+    case 3: launchB<3, FC>(a, grid, st); return true;
+    case 4: launchB<4, FC>(a, grid, st); return true;
+    case 5: launchB<5, FC>(a, grid, st); return true;
+    case 6: launchB<6, FC>(a, grid, st); return true;
+    case 7: launchB<7, FC>(a, grid, st); return true;
+    case 8: launchB<8, FC>(a, grid, st); return true;
+    default: return false;
+    }
+}
+
+bool launchB_variant(int lc, int fc, const BatchArgs &a, int grid, hipStream_t st)
+{
+    if (fc == 1) return launchB_lc<1>(lc, a, grid, st);
+    if (fc == 2) return launchB_lc<2>(lc, a, grid, st);
+    if (fc == 4) return launchB_lc<4>(lc, a, grid, st);
+    return false;
+}
+
+bool validA(int fc, int g)
+{
+    return (fc == 1 && (g == 4 || g == 8)) || (fc == 2 && g == 4) || (fc == 4 && g == 2);
+}
+
+// (Re)builds the phase-A item list for cfg.aItemSlices.
+int buildItemsA(osdhip_batch *b)
+{
+    if (b->itemsA && b->itemsASlices == b->cfg.aItemSlices) return OSDHIP_OK;
+    std::vector<BatchItem> items;
+    // segments with the most work per slice first (persistent balance)
+    std::vector<int> order(b->nS);
+    for (int i = 0; i < b->nS; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return (long long)b->hS[x].frames * b->hS[x].nslices >
+               (long long)b->hS[y].frames * b->hS[y].nslices;
+    });
+    const int per = b->cfg.aItemSlices;
+    for (int s : order)
+        for (int f = 0; f < b->hS[s].nslices; f += per)
+            items.push_back({ s, f, std::min(per, b->hS[s].nslices - f), 0 });
+    if (b->itemsA) {
+        (void)hipDeviceSynchronize();     // no queued kernel still reads the old list
+        (void)hipFree(b->itemsA);
+    }
+    b->itemsA = nullptr;
+    b->nItemsA = (int)items.size();
+    b->itemsASlices = per;
+    if (items.empty()) return OSDHIP_OK;
+    if (hipMalloc(&b->itemsA, items.size() * sizeof(BatchItem)) != hipSuccess) return OSDHIP_ERR_ARGS;
+    if (hipMemcpy(b->itemsA, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice) !=
+        hipSuccess)
+        return OSDHIP_ERR_ARGS;
+    return OSDHIP_OK;
+}
+
+int gridFor(const osdhip_batch *b, int nitems, bool persistent)
+{
+    if (!persistent) return nitems;
+    // persistent: enough blocks to fill every CU a few times over
+    return std::max(1, std::min(nitems, b->numCUs * 8));
+}
+
+int enqueuePhaseA(osdhip_batch *b, hipStream_t st)
+{
+    if (b->nItemsA == 0) return OSDHIP_OK;
+    BatchArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.sseg = b->dS;
+    a.items = b->itemsA;
+    a.nitems = b->nItemsA;
+    a.persistent = b->cfg.aPersistent;
+    a.counter = b->counters;
+    if (!launchA_variant(b->length, b->cfg.aFrames, b->cfg.aGroup, a,
+                         gridFor(b, b->nItemsA, a.persistent != 0), st))
+        return OSDHIP_ERR_UNSUPPORTED;
+    return last_error();
+}
+
+int enqueuePhaseB(osdhip_batch *b, hipStream_t st)
+{
+    if (b->nItemsB == 0) return OSDHIP_OK;
+    BatchArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.pseg = b->dP;
+    a.items = b->itemsB;
+    a.nitems = b->nItemsB;
+    a.persistent = b->cfg.bPersistent;
+    a.counter = b->counters + 1;
+    if (!launchB_variant(b->length, b->cfg.bFrames, a, gridFor(b, b->nItemsB, a.persistent != 0), st))
+        return OSDHIP_ERR_UNSUPPORTED;
+    return last_error();
+}
+
+int enqueueAll(osdhip_batch *b, int phases, hipStream_t st)
+{
+    int rc = OSDHIP_OK;
+    if ((b->cfg.aPersistent && (phases & 1)) || (b->cfg.bPersistent && (phases & 2))) {
+        hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->counters, 2);
+        rc = last_error();
+    }
+    if (rc == OSDHIP_OK && (phases & 1)) rc = enqueuePhaseA(b, st);
+    if (rc == OSDHIP_OK && (phases & 2)) rc = enqueuePhaseB(b, st);
+    return rc;
+}
+
+void dropGraph(osdhip_batch *b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+int capture(osdhip_batch *b)
+{
+    dropGraph(b);
+    int rc = buildItemsA(b);
+    if (rc != OSDHIP_OK) return rc;
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return last_error();
+    rc = (int)hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (rc == OSDHIP_OK) {
+        rc = enqueueAll(b, 3, cs);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(cs, &g);
+        if (rc == OSDHIP_OK) rc = (int)e;
+        b->graph = g;
+        if (rc == OSDHIP_OK) rc = (int)hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    }
+    (void)hipStreamDestroy(cs);
+    if (rc != OSDHIP_OK) dropGraph(b);
+    return rc;
+}
+
+void freeBatch(osdhip_batch *b)
+{
+    if (!b) return;
+    dropGraph(b);
+    void *p[] = { b->dS, b->dP, b->itemsA, b->itemsB, b->counters };
+    for (void *x : p)
+        if (x) (void)hipFree(x);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
+                                  const osdhip_batch_stencils *stencils, int numPatchSegments,
+                                  const osdhip_batch_patches *patches, void *stream)
+{
+    if (length < 1 || length > 8 || numStencilSegments < 0 || numPatchSegments < 0 ||
+        (numStencilSegments > 0 && !stencils) || (numPatchSegments > 0 && !patches))
+        return nullptr;
+    osdhip_batch *b = new (std::nothrow) osdhip_batch;
+    if (!b) return nullptr;
+    b->length = length;
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+        b->numCUs = std::max(1, prop.multiProcessorCount);
+
+    for (int i = 0; i < numStencilSegments; ++i) {
+        const osdhip_batch_stencils &s = stencils[i];
+        const osdhip_stencil_table *t = s.table;
+        if (!t || t->layout != 0 || t->nw != 1 || s.frames < 1 || !s.src || !s.dst ||
+            s.srcVertexStride < length || s.dstVertexStride < length) {
+            freeBatch(b);
+            return nullptr;
+        }
+        if (t->n == 0) continue;
+        BatchStencilSeg g;
+        std::memset(&g, 0, sizeof(g));
+        g.rec = t->rec;
+        g.sliceOff = t->sliceOff;
+        g.sliceWidth = t->sliceWidth;
+        g.rowSize = t->rowSize;
+        g.src = s.src;
+        g.srcV = s.srcVertexStride;
+        g.srcF = s.srcFrameStride;
+        g.dst = s.dst;
+        g.dstV = s.dstVertexStride;
+        g.dstF = s.dstFrameStride;
+        g.nrows = t->n;
+        g.nslices = t->nslices;
+        g.frames = s.frames;
+        b->hS.push_back(g);
+        b->tables.push_back(t);
+        b->stencilRows += t->n;
+        b->frameRows += (long long)t->n * s.frames;
+        b->tableBytes += 4LL * t->n + 8LL * t->nnz;
+    }
+    std::vector<BatchItem> itemsB;
+    for (int i = 0; i < numPatchSegments; ++i) {
+        const osdhip_batch_patches &p = patches[i];
+        if (p.numCoords < 0 || p.frames < 1 || !p.src || p.srcVertexStride < length ||
+            (p.numCoords > 0 && (!p.coords || !p.arrays || !p.patchIndices || !p.params)) ||
+            (p.P && p.outVertexStride < length) || (p.dPdu && p.outVertexStride < length) ||
+            (p.dPdv && p.outVertexStride < length)) {
+            freeBatch(b);
+            return nullptr;
+        }
+        if (p.numCoords == 0) continue;
+        BatchPatchSeg g;
+        std::memset(&g, 0, sizeof(g));
+        g.coords = p.coords;
+        g.arrays = p.arrays;
+        g.indices = p.patchIndices;
+        g.params = p.params;
+        g.src = p.src;
+        g.srcV = p.srcVertexStride;
+        g.srcF = p.srcFrameStride;
+        g.out[0] = p.P;
+        g.out[1] = p.dPdu;
+        g.out[2] = p.dPdv;
+        g.outV = p.outVertexStride;
+        g.outF = p.outFrameStride;
+        g.ncoords = p.numCoords;
+        g.frames = p.frames;
+        const int seg = (int)b->hP.size();
+        b->hP.push_back(g);
+        for (int c = 0; c < p.numCoords; c += kBlock)
+            itemsB.push_back({ seg, c, std::min(kBlock, p.numCoords - c), 0 });
+        b->patchCoords += p.numCoords;
+        b->patchFrames += (long long)p.numCoords * p.frames;
+    }
+    b->nS = (int)b->hS.size();
+    b->nP = (int)b->hP.size();
+    b->nItemsB = (int)itemsB.size();
+    bool ok = hipMalloc(&b->counters, 2 * sizeof(unsigned)) == hipSuccess;
+    if (ok && b->nS)
+        ok = hipMalloc(&b->dS, b->hS.size() * sizeof(BatchStencilSeg)) == hipSuccess &&
+             hipMemcpy(b->dS, b->hS.data(), b->hS.size() * sizeof(BatchStencilSeg),
+                       hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && b->nP)
+        ok = hipMalloc(&b->dP, b->hP.size() * sizeof(BatchPatchSeg)) == hipSuccess &&
+             hipMemcpy(b->dP, b->hP.data(), b->hP.size() * sizeof(BatchPatchSeg),
+                       hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && b->nItemsB)
+        ok = hipMalloc(&b->itemsB, itemsB.size() * sizeof(BatchItem)) == hipSuccess &&
+             hipMemcpy(b->itemsB, itemsB.data(), itemsB.size() * sizeof(BatchItem),
+                       hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) ok = capture(b) == OSDHIP_OK;
+    (void)stream;
+    if (!ok) {
+        freeBatch(b);
+        return nullptr;
+    }
+    return b;
+}
+
+void osdhip_batch_destroy(osdhip_batch *b) { freeBatch(b); }
+
+int osdhip_batch_launch(osdhip_batch *b, void *stream)
+{
+    if (!b) return OSDHIP_ERR_ARGS;
+    if (!b->exec) {
+        int rc = capture(b);
+        if (rc != OSDHIP_OK) return rc;
+    }
+    return (int)hipGraphLaunch(b->exec, (hipStream_t)stream);
+}
+
+int osdhip_batch_run(osdhip_batch *b, int phases, void *stream)
+{
+    if (!b || phases < 1 || phases > 3) return OSDHIP_ERR_ARGS;
+    int rc = buildItemsA(b);
+    if (rc != OSDHIP_OK) return rc;
+    return enqueueAll(b, phases, (hipStream_t)stream);
+}
+
+int osdhip_batch_run_plain(osdhip_batch *b, void *stream)
+{
+    if (!b) return OSDHIP_ERR_ARGS;
+    const int L = b->length;
+    // Osd buffer descriptors hold int offsets and strides
+    const long long lim = 0x7fffffffLL;
+    for (const BatchStencilSeg &g : b->hS)
+        if (g.srcV > lim || g.dstV > lim || (g.frames - 1) * g.srcF > lim || (g.frames - 1) * g.dstF > lim)
+            return OSDHIP_ERR_UNSUPPORTED;
+    for (const BatchPatchSeg &g : b->hP)
+        if (g.srcV > lim || g.outV > lim || (g.frames - 1) * g.srcF > lim || (g.frames - 1) * g.outF > lim)
+            return OSDHIP_ERR_UNSUPPORTED;
+    for (int i = 0; i < b->nS; ++i) {
+        const BatchStencilSeg &g = b->hS[i];
+        for (int f = 0; f < g.frames; ++f) {
+            // frame f as an Osd buffer: offset f * frameStride, stride vertexStride
+            osdhip_buffer_desc sd = { (int)(f * g.srcF), L, (int)g.srcV };
+            osdhip_buffer_desc dd = { (int)(f * g.dstF), L, (int)g.dstV };
+            float *outs[1] = { g.dst };
+            int rc = osdhip_stencil_table_eval(b->tables[i], g.src, sd, 1, outs, &dd, 0,
+                                               g.nrows, stream);
+            if (rc != OSDHIP_OK) return rc;
+        }
+    }
+    for (int i = 0; i < b->nP; ++i) {
+        const BatchPatchSeg &g = b->hP[i];
+        for (int f = 0; f < g.frames; ++f) {
+            osdhip_buffer_desc sd = { (int)(f * g.srcF), L, (int)g.srcV };
+            osdhip_buffer_desc od = { (int)(f * g.outF), L, (int)g.outV };
+            osdhip_buffer_desc z = { 0, 0, 0 };
+            int rc = osdhip_eval_patches(g.src, sd, g.out[0], g.out[0] ? od : z, g.out[1],
+                                         g.out[1] ? od : z, g.out[2], g.out[2] ? od : z, nullptr,
+                                         z, nullptr, z, nullptr, z, g.ncoords, g.coords, g.arrays,
+                                         g.indices, g.params, stream);
+            if (rc != OSDHIP_OK) return rc;
+        }
+    }
+    return OSDHIP_OK;
+}
+
+int osdhip_batch_set(osdhip_batch *b, const char *key, int value)
+{
+    if (!b || !key) return OSDHIP_ERR_ARGS;
+    BatchConfig c = b->cfg;
+    if (!std::strcmp(key, "a_frames")) c.aFrames = value;
+    else if (!std::strcmp(key, "a_group")) c.aGroup = value;
+    else if (!std::strcmp(key, "a_item_slices")) c.aItemSlices = value;
+    else if (!std::strcmp(key, "a_persistent")) c.aPersistent = value;
+    else if (!std::strcmp(key, "b_frames")) c.bFrames = value;
+    else if (!std::strcmp(key, "b_persistent")) c.bPersistent = value;
+    else return OSDHIP_ERR_UNSUPPORTED;
+    if (!validA(c.aFrames, c.aGroup) || c.aItemSlices < 1 || c.aItemSlices > 1024 ||
+        (c.aPersistent != 0 && c.aPersistent != 1) ||
+        (c.bFrames != 1 && c.bFrames != 2 && c.bFrames != 4) ||
+        (c.bPersistent != 0 && c.bPersistent != 1))
+        return OSDHIP_ERR_ARGS;
+    b->cfg = c;
+    dropGraph(b);          // re-captured on the next launch
+    return OSDHIP_OK;
+}
+
+int osdhip_batch_get(const osdhip_batch *b, const char *key)
+{
+    if (!b || !key) return OSDHIP_ERR_ARGS;
+    if (!std::strcmp(key, "a_frames")) return b->cfg.aFrames;
+    if (!std::strcmp(key, "a_group")) return b->cfg.aGroup;
+    if (!std::strcmp(key, "a_item_slices")) return b->cfg.aItemSlices;
+    if (!std::strcmp(key, "a_persistent")) return b->cfg.aPersistent;
+    if (!std::strcmp(key, "b_frames")) return b->cfg.bFrames;
+    if (!std::strcmp(key, "b_persistent")) return b->cfg.bPersistent;
+    return OSDHIP_ERR_UNSUPPORTED;
+}
+
+int osdhip_batch_info(const osdhip_batch *b, long long *stencilRows, long long *frameRows,
+                      long long *patchCoords, long long *patchFrames, long long *tableBytes)
+{
+    if (!b) return OSDHIP_ERR_ARGS;
+    if (stencilRows) *stencilRows = b->stencilRows;
+    if (frameRows) *frameRows = b->frameRows;
+    if (patchCoords) *patchCoords = b->patchCoords;
+    if (patchFrames) *patchFrames = b->patchFrames;
+    if (tableBytes) *tableBytes = b->tableBytes;
+    return OSDHIP_OK;
+}
+
+int osdhip_batch_tune(osdhip_batch *b, int reps, void *stream, char *report, int reportBytes)
+{
+    if (!b || reps < 1) return OSDHIP_ERR_ARGS;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return last_error();
+    std::string rep = "{";
+    auto timePhase = [&](int phase, float &best) -> int {
+        // one untimed run, then the median of reps timed runs
+        int rc = osdhip_batch_run(b, phase, stream);
+        std::vector<float> ms;
+        for (int r = 0; rc == OSDHIP_OK && r < reps; ++r) {
+            (void)hipEventRecord(e0, st);
+            rc = osdhip_batch_run(b, phase, stream);
+            (void)hipEventRecord(e1, st);
+            if (rc == OSDHIP_OK) rc = (int)hipEventSynchronize(e1);
+            float t = 0.0f;
+            if (rc == OSDHIP_OK) rc = (int)hipEventElapsedTime(&t, e0, e1);
+            ms.push_back(t);
+        }
+        if (rc != OSDHIP_OK) return rc;
+        std::sort(ms.begin(), ms.end());
+        best = ms[ms.size() / 2];
+        return OSDHIP_OK;
+    };
+    int rc = OSDHIP_OK;
+    char buf[160];
+    // phase A: (frames per chunk, gather group) x item slices x scheduling
+    if (b->nS) {
+        const int fg[4][2] = { { 1, 4 }, { 1, 8 }, { 2, 4 }, { 4, 2 } };
+        const int slices[3] = { 4, 16, 64 };
+        BatchConfig bestc = b->cfg;
+        float bestms = 1e30f;
+        rep += "\"phase_a\":[";
+        bool first = true;
+        for (auto &v : fg)
+            for (int sl : slices)
+                for (int pers = 0; pers < 2; ++pers) {
+                    b->cfg.aFrames = v[0];
+                    b->cfg.aGroup = v[1];
+                    b->cfg.aItemSlices = sl;
+                    b->cfg.aPersistent = pers;
+                    float t = 0.0f;
+                    rc = timePhase(1, t);
+                    if (rc != OSDHIP_OK) break;
+                    std::snprintf(buf, sizeof(buf),
+                                  "%s{\"frames\":%d,\"group\":%d,\"slices\":%d,\"persistent\":%d,\"ms\":%.4f}",
+                                  first ? "" : ",", v[0], v[1], sl, pers, t);
+                    rep += buf;
+                    first = false;
+                    if (t < bestms) { bestms = t; bestc = b->cfg; }
+                }
+        rep += "]";
+        BatchConfig keep = b->cfg;
+        keep.aFrames = bestc.aFrames;
+        keep.aGroup = bestc.aGroup;
+        keep.aItemSlices = bestc.aItemSlices;
+        keep.aPersistent = bestc.aPersistent;
+        b->cfg = keep;
+    }
+    // phase B: frames per chunk x scheduling (needs phase A's outputs: run it once first)
+    if (rc == OSDHIP_OK && b->nP) {
+        if (b->nS) rc = osdhip_batch_run(b, 1, stream);
+        const int fcs[3] = { 1, 2, 4 };
+        BatchConfig bestc = b->cfg;
+        float bestms = 1e30f;
+        rep += b->nS ? ",\"phase_b\":[" : "\"phase_b\":[";
+        bool first = true;
+        for (int fc : fcs)
+            for (int pers = 0; rc == OSDHIP_OK && pers < 2; ++pers) {
+                b->cfg.bFrames = fc;
+                b->cfg.bPersistent = pers;
+                float t = 0.0f;
+                rc = timePhase(2, t);
+                if (rc != OSDHIP_OK) break;
+                std::snprintf(buf, sizeof(buf), "%s{\"frames\":%d,\"persistent\":%d,\"ms\":%.4f}",
+                              first ? "" : ",", fc, pers, t);
+                rep += buf;
+                first = false;
+                if (t < bestms) { bestms = t; bestc = b->cfg; }
+            }
+        rep += "]";
+        b->cfg.bFrames = bestc.bFrames;
+        b->cfg.bPersistent = bestc.bPersistent;
+    }
+    rep += "}";
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == OSDHIP_OK) rc = capture(b);
+    if (report && reportBytes > 0) {
+        std::strncpy(report, rep.c_str(), (size_t)reportBytes - 1);
+        report[reportBytes - 1] = '\0';
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -17,6 +17,7 @@
 // sum accumulates w * x in the reference's index order starting from 0, so
 // stencil and patch results equal Osd::CpuEvaluator's bit for bit.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -28,11 +29,39 @@
 #include "patch_kernels.h"
 #include "stencil_csr.h"
 #include "stencil_sliced.h"
+#include "table.h"
+#include "table_build.h"
 
 namespace osdhip {
 
 // Tuning knobs (osdhip_config).
 static int g_stencil_kernel = 0;     // table eval: 0 sliced, 1 CSR
+static int g_patch_sort_min = 0;     // EvalPatches: patch-sort coords when n >= this (0 = never)
+static int g_table_layout = 0;       // new tables: 0 record-interleaved slices, 1 split streams
+static int g_csr_mirror = 1;         // new tables keep the Far CSR arrays on the device
+static int g_slice_group = 8;        // record kernel: entries per gather group (4 or 8)
+
+template <int NWT, int G>
+static void launch_rec_g(int lc, const RecArgs &a, int grid, hipStream_t st)
+{
+    switch (lc) {
+    case 1: hipLaunchKernelGGL((k_stencils_rec<NWT, 1, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_stencils_rec<NWT, 2, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_stencils_rec<NWT, 3, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_stencils_rec<NWT, 4, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 5: hipLaunchKernelGGL((k_stencils_rec<NWT, 5, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 6: hipLaunchKernelGGL((k_stencils_rec<NWT, 6, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 7: hipLaunchKernelGGL((k_stencils_rec<NWT, 7, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_stencils_rec<NWT, 8, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    }
+}
+
+template <int NWT>
+static void launch_rec(int lc, int group, const RecArgs &a, int grid, hipStream_t st)
+{
+    if (group == 4) launch_rec_g<NWT, 4>(lc, a, grid, st);
+    else launch_rec_g<NWT, 8>(lc, a, grid, st);
+}
 
 template <int NW, int CAP, bool VEC>
 static void launch_csr(int lc, const StencilArgs &a, int grid, hipStream_t st)
@@ -64,11 +93,6 @@ static void launch_sliced(int lc, const SlicedArgs &a, int grid, hipStream_t st)
     }
 }
 
-static int last_error()
-{
-    hipError_t e = hipGetLastError();
-    return e == hipSuccess ? OSDHIP_OK : (int)e;
-}
 
 static bool desc_ok(const osdhip_buffer_desc &d)
 {
@@ -161,28 +185,26 @@ static void launch_patches(int lc, const PatchArgs &a, hipStream_t st)
 
 using namespace osdhip;
 
-// Device copy of a (limit) stencil table: the Far CSR arrays (for the raw
-// getters) plus the 64-row sliced layout the table kernel streams.
-struct osdhip_stencil_table {
-    int n = 0, nw = 0, nslices = 0;
-    long long nnz = 0, sliceEntries = 0, bytes = 0;
-    int *sizes = nullptr, *offsets = nullptr, *indices = nullptr;
-    float *w[6] = {};
-    long long *sliceOff = nullptr;
-    int *sliceWidth = nullptr;
-    int *sidx = nullptr;
-    float *sw[6] = {};
-};
 
 static void free_table(osdhip_stencil_table *t)
 {
     if (!t) return;
     void *ptrs[] = { t->sizes, t->offsets, t->indices, t->w[0], t->w[1], t->w[2], t->w[3],
-                     t->w[4], t->w[5], t->sliceOff, t->sliceWidth, t->sidx, t->sw[0], t->sw[1],
-                     t->sw[2], t->sw[3], t->sw[4], t->sw[5] };
+                     t->w[4], t->w[5], t->sliceOff, t->sliceWidth, t->rowSize, t->rec, t->sidx,
+                     t->sw[0], t->sw[1], t->sw[2], t->sw[3], t->sw[4], t->sw[5] };
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete t;
+}
+
+template <typename T>
+static bool dalloc(T **dptr, size_t count, long long &bytes)
+{
+    *dptr = nullptr;
+    if (count == 0) return true;
+    if (hipMalloc((void **)dptr, count * sizeof(T)) != hipSuccess) { *dptr = nullptr; return false; }
+    bytes += (long long)(count * sizeof(T));
+    return true;
 }
 
 template <typename T>
@@ -281,40 +303,81 @@ osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *si
     for (int k = 0; ok && k < numWeightArrays; ++k)
         ok = upload(&t->w[k], weights[k], (size_t)nnz, st, t->bytes);
 
-    // sliced layout: slice s holds rows [64s, 64s+64), column-major, padded
+    // slice layout, built on the device from the CSR mirror
     const int S = (numStencils + kWave - 1) / kWave;
+    const int rows = S * kWave;
     t->nslices = S;
-    std::vector<long long> sOff(S + 1, 0);
-    std::vector<int> sW(S, 0);
-    for (int s = 0; s < S; ++s) {
-        int w = 0;
-        for (int r = s * kWave; r < std::min(numStencils, (s + 1) * kWave); ++r) w = std::max(w, sizes[r]);
-        sW[s] = w;
-        sOff[s + 1] = sOff[s] + (long long)w * kWave;
+    t->layout = g_table_layout;
+    long long *entries = nullptr;
+    void *scanTmp = nullptr;
+    size_t scanBytes = 0;
+    long long dummy = 0;
+    if (ok && S > 0) {
+        ok = dalloc(&t->sliceOff, (size_t)S, t->bytes) && dalloc(&t->sliceWidth, (size_t)S, t->bytes) &&
+             dalloc(&t->rowSize, (size_t)rows, t->bytes) && dalloc(&entries, (size_t)S, dummy);
+        if (ok) {
+            const int gs = (S + kBlock - 1) / kBlock, gr = (rows + kBlock - 1) / kBlock;
+            hipLaunchKernelGGL(k_slice_widths, dim3(gs), dim3(kBlock), 0, st, t->sizes, numStencils,
+                               S, t->sliceWidth, entries);
+            hipLaunchKernelGGL(k_pad_sizes, dim3(gr), dim3(kBlock), 0, st, t->sizes, numStencils,
+                               rows, t->rowSize);
+            ok = hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, entries, t->sliceOff, S, st) ==
+                     hipSuccess &&
+                 hipMalloc(&scanTmp, scanBytes) == hipSuccess &&
+                 hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, entries, t->sliceOff, S, st) ==
+                     hipSuccess;
+        }
+        long long lastOff = 0, lastEntries = 0;
+        if (ok)
+            ok = hipMemcpyAsync(&lastOff, t->sliceOff + (S - 1), sizeof(long long),
+                                hipMemcpyDeviceToHost, st) == hipSuccess &&
+                 hipMemcpyAsync(&lastEntries, entries + (S - 1), sizeof(long long),
+                                hipMemcpyDeviceToHost, st) == hipSuccess &&
+                 hipStreamSynchronize(st) == hipSuccess;
+        t->sliceEntries = lastOff + lastEntries;
+        const int gr = (rows + kBlock - 1) / kBlock;
+        const float *const *W = t->w;
+        if (ok && t->layout == 0) {
+            ok = dalloc(&t->rec, (size_t)t->sliceEntries * rec_dwords(numWeightArrays), t->bytes);
+            if (ok) {
+                if (numWeightArrays == 1)
+                    hipLaunchKernelGGL(k_fill_records<1>, dim3(gr), dim3(kBlock), 0, st, t->sizes,
+                                       t->offsets, t->indices, W[0], W[1], W[2], W[3], W[4], W[5],
+                                       numStencils, rows, t->sliceOff, t->sliceWidth, t->rec);
+                else if (numWeightArrays == 3)
+                    hipLaunchKernelGGL(k_fill_records<3>, dim3(gr), dim3(kBlock), 0, st, t->sizes,
+                                       t->offsets, t->indices, W[0], W[1], W[2], W[3], W[4], W[5],
+                                       numStencils, rows, t->sliceOff, t->sliceWidth, t->rec);
+                else
+                    hipLaunchKernelGGL(k_fill_records<6>, dim3(gr), dim3(kBlock), 0, st, t->sizes,
+                                       t->offsets, t->indices, W[0], W[1], W[2], W[3], W[4], W[5],
+                                       numStencils, rows, t->sliceOff, t->sliceWidth, t->rec);
+            }
+        } else if (ok) {
+            ok = dalloc(&t->sidx, (size_t)t->sliceEntries, t->bytes);
+            for (int k = 0; ok && k < numWeightArrays; ++k)
+                ok = dalloc(&t->sw[k], (size_t)t->sliceEntries, t->bytes);
+            if (ok)
+                hipLaunchKernelGGL(k_fill_split, dim3(gr), dim3(kBlock), 0, st, t->sizes, t->offsets,
+                                   t->indices, W[0], W[1], W[2], W[3], W[4], W[5], numWeightArrays,
+                                   numStencils, rows, t->sliceOff, t->sliceWidth, t->sidx, t->sw[0],
+                                   t->sw[1], t->sw[2], t->sw[3], t->sw[4], t->sw[5]);
+        }
+        ok = ok && hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
     }
-    t->sliceEntries = sOff[S];
-    std::vector<int> pidx(ok ? (size_t)t->sliceEntries : 0, 0);
-    std::vector<float> pw;
-    if (ok) {
-        for (int s = 0; s < S; ++s)
-            for (int r = s * kWave; r < std::min(numStencils, (s + 1) * kWave); ++r)
-                for (int j = 0; j < sizes[r]; ++j)
-                    pidx[sOff[s] + (long long)j * kWave + (r - s * kWave)] = indices[off[r] + j];
-        ok = upload(&t->sliceOff, sOff.data(), (size_t)S, st, t->bytes) &&
-             upload(&t->sliceWidth, sW.data(), (size_t)S, st, t->bytes) &&
-             upload(&t->sidx, pidx.data(), pidx.size(), st, t->bytes) &&
-             hipStreamSynchronize(st) == hipSuccess;
-    }
-    for (int k = 0; ok && k < numWeightArrays; ++k) {
-        pw.assign((size_t)t->sliceEntries, 0.0f);
-        for (int s = 0; s < S; ++s)
-            for (int r = s * kWave; r < std::min(numStencils, (s + 1) * kWave); ++r)
-                for (int j = 0; j < sizes[r]; ++j)
-                    pw[sOff[s] + (long long)j * kWave + (r - s * kWave)] = weights[k][off[r] + j];
-        ok = upload(&t->sw[k], pw.data(), pw.size(), st, t->bytes) &&
-             hipStreamSynchronize(st) == hipSuccess;
-    }
+    if (entries) (void)hipFree(entries);
+    if (scanTmp) (void)hipFree(scanTmp);
+    if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
     if (!ok) { free_table(t); return nullptr; }
+    if (!g_csr_mirror) {
+        // slice layout only: the CSR arrays were needed to build it
+        void *csr[] = { t->offsets, t->indices, t->w[0], t->w[1], t->w[2], t->w[3], t->w[4], t->w[5] };
+        for (void *p : csr)
+            if (p) (void)hipFree(p);
+        t->bytes -= 4LL * numStencils + 4LL * nnz * (1 + numWeightArrays);
+        t->offsets = t->indices = nullptr;
+        for (int k = 0; k < 6; ++k) t->w[k] = nullptr;
+    }
     return t;
 }
 
@@ -355,7 +418,7 @@ int osdhip_stencil_table_eval(const osdhip_stencil_table *t, const float *src,
     if (end <= start) return OSDHIP_OK;
     if (!src || start < 0 || !outputs_ok(srcDesc, numOutputs, outputs, outputDescs, nullptr))
         return OSDHIP_ERR_ARGS;
-    if (g_stencil_kernel == 1) {
+    if (g_stencil_kernel == 1 && t->indices) {
         const float *ws[6] = { t->w[0], t->w[1], t->w[2], t->w[3], t->w[4], t->w[5] };
         if (numOutputs == 1)
             return eval_csr<1>(src, srcDesc, outputs, outputDescs, t->sizes, t->offsets, t->indices,
@@ -366,21 +429,54 @@ int osdhip_stencil_table_eval(const osdhip_stencil_table *t, const float *src,
         return eval_csr<6>(src, srcDesc, outputs, outputDescs, t->sizes, t->offsets, t->indices,
                            ws, start, end, stream);
     }
+    const int sliceBegin = start / kWave;
+    const int nslices = (end + kWave - 1) / kWave - sliceBegin;
+    const int nblocks = (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int grid = grid_for_blocks(nblocks);
+    hipStream_t st = (hipStream_t)stream;
+    if (t->layout == 0) {
+        RecArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.srcStride = srcDesc.stride;
+        a.rowSize = t->rowSize;
+        a.sliceOff = t->sliceOff;
+        a.sliceWidth = t->sliceWidth;
+        a.rec = t->rec;
+        a.rowBegin = start;
+        a.rowEnd = end;
+        a.sliceBegin = sliceBegin;
+        a.nslices = nslices;
+        a.nblocks = nblocks;
+        for (int e0 = 0; e0 < srcDesc.length; e0 += 8) {
+            const int lc = std::min(8, srcDesc.length - e0);
+            a.src = src + srcDesc.offset + e0;
+            for (int k = 0; k < numOutputs; ++k) {
+                a.dst[k] = outputs[k] ? outputs[k] + outputDescs[k].offset + e0 : nullptr;
+                a.dstStride[k] = outputs[k] ? outputDescs[k].stride : 0;
+            }
+            // the kernel computes every weight array of the table; only the
+            // requested outputs are written
+            if (t->nw == 1) launch_rec<1>(lc, g_slice_group, a, grid, st);
+            else if (t->nw == 3) launch_rec<3>(lc, g_slice_group, a, grid, st);
+            else launch_rec<6>(lc, g_slice_group, a, grid, st);
+            int rc = last_error();
+            if (rc != OSDHIP_OK) return rc;
+        }
+        return OSDHIP_OK;
+    }
     SlicedArgs a;
     std::memset(&a, 0, sizeof(a));
     a.srcStride = srcDesc.stride;
-    a.rowSize = t->sizes;
+    a.rowSize = t->rowSize;
     a.sliceOff = t->sliceOff;
     a.sliceWidth = t->sliceWidth;
     a.idx = t->sidx;
     for (int k = 0; k < numOutputs; ++k) a.w[k] = t->sw[k];
     a.rowBegin = start;
     a.rowEnd = end;
-    a.sliceBegin = start / kWave;
-    a.nslices = (end + kWave - 1) / kWave - a.sliceBegin;
-    a.nblocks = (a.nslices + kWavesPerBlock - 1) / kWavesPerBlock;
-    const int grid = grid_for_blocks(a.nblocks);
-    hipStream_t st = (hipStream_t)stream;
+    a.sliceBegin = sliceBegin;
+    a.nslices = nslices;
+    a.nblocks = nblocks;
     for (int e0 = 0; e0 < srcDesc.length; e0 += 8) {
         const int lc = std::min(8, srcDesc.length - e0);
         a.src = src + srcDesc.offset + e0;
@@ -429,7 +525,34 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
     a.indices = patchIndices;
     a.params = params;
     hipStream_t st = (hipStream_t)stream;
-    for (int e0 = 0; e0 < srcDesc.length; e0 += 8) {
+
+    // Patch-ordered evaluation for large coord sets: radix-sort (patch key,
+    // coord index) pairs in stream-ordered scratch memory, then evaluate in
+    // that order.  Outputs still go to each coord's own row.
+    void *ws = nullptr;
+    if (g_patch_sort_min > 0 && numPatchCoords >= g_patch_sort_min) {
+        const int n = numPatchCoords;
+        size_t tmpBytes = 0;
+        hipcub::DoubleBuffer<unsigned> kb(nullptr, nullptr);
+        hipcub::DoubleBuffer<int> vb(nullptr, nullptr);
+        hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, kb, vb, n, 0, 24, st);
+        const size_t arr = ((size_t)n * 4 + 255) & ~(size_t)255;
+        if (hipMallocAsync(&ws, 4 * arr + tmpBytes, st) == hipSuccess) {
+            char *b = (char *)ws;
+            kb = hipcub::DoubleBuffer<unsigned>((unsigned *)b, (unsigned *)(b + arr));
+            vb = hipcub::DoubleBuffer<int>((int *)(b + 2 * arr), (int *)(b + 3 * arr));
+            hipLaunchKernelGGL(k_patch_keys, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                               coords, n, kb.Current(), vb.Current());
+            if (hipcub::DeviceRadixSort::SortPairs(b + 4 * arr, tmpBytes, kb, vb, n, 0, 24, st) ==
+                hipSuccess)
+                a.order = vb.Current();
+        } else {
+            (void)hipGetLastError();
+            ws = nullptr;
+        }
+    }
+    int rc = OSDHIP_OK;
+    for (int e0 = 0; e0 < srcDesc.length && rc == OSDHIP_OK; e0 += 8) {
         const int lc = std::min(8, srcDesc.length - e0);
         a.src = src + srcDesc.offset + e0;
         for (int o = 0; o < 6; ++o) {
@@ -439,10 +562,13 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
         if (d2) launch_patches<true, true>(lc, a, st);
         else if (d1) launch_patches<true, false>(lc, a, st);
         else launch_patches<false, false>(lc, a, st);
-        int rc = last_error();
-        if (rc != OSDHIP_OK) return rc;
+        rc = last_error();
     }
-    return OSDHIP_OK;
+    if (ws) {
+        int rf = (int)hipFreeAsync(ws, st);
+        if (rc == OSDHIP_OK) rc = rf;
+    }
+    return rc;
 }
 
 int osdhip_config(const char *key, int value)
@@ -452,6 +578,30 @@ int osdhip_config(const char *key, int value)
         if (value != 0 && value != 1) return OSDHIP_ERR_ARGS;
         int old = g_stencil_kernel;
         g_stencil_kernel = value;
+        return old;
+    }
+    if (std::strcmp(key, "table_layout") == 0) {
+        if (value != 0 && value != 1) return OSDHIP_ERR_ARGS;
+        int old = g_table_layout;
+        g_table_layout = value;
+        return old;
+    }
+    if (std::strcmp(key, "table_csr_mirror") == 0) {
+        if (value != 0 && value != 1) return OSDHIP_ERR_ARGS;
+        int old = g_csr_mirror;
+        g_csr_mirror = value;
+        return old;
+    }
+    if (std::strcmp(key, "slice_group") == 0) {
+        if (value != 4 && value != 8) return OSDHIP_ERR_ARGS;
+        int old = g_slice_group;
+        g_slice_group = value;
+        return old;
+    }
+    if (std::strcmp(key, "patch_sort_min") == 0) {
+        if (value < 0) return OSDHIP_ERR_ARGS;
+        int old = g_patch_sort_min;
+        g_patch_sort_min = value;
         return old;
     }
     return OSDHIP_ERR_UNSUPPORTED;

@@ -21,13 +21,50 @@ struct PatchArgs {
     const osdhip_patch_array *arrays;
     const int *indices;
     const osdhip_patch_param *params;
+    const int *order;       // evaluation order (coord index per lane), NULL = identity
 };
+
+// acc[o][k] += v[k] * weight_o[j], in the reference's point order.
+template <bool D1, bool D2, int LC, int NO>
+__device__ __forceinline__ void accumulate_point(const float (&v)[LC], const Basis<D1, D2> &B,
+                                                 int j, float (&acc)[NO][LC])
+{
+#pragma unroll
+    for (int k = 0; k < LC; ++k) {
+        acc[0][k] += v[k] * B.p[j];
+        if constexpr (D1) {
+            acc[1][k] += v[k] * B.ds[j];
+            acc[2][k] += v[k] * B.dt[j];
+        }
+        if constexpr (D2) {
+            acc[3][k] += v[k] * B.dss[j];
+            acc[4][k] += v[k] * B.dst[j];
+            acc[5][k] += v[k] * B.dtt[j];
+        }
+    }
+}
+
+// Sort keys for the patch-ordered evaluation: the low 24 bits of the patch
+// index (any key gives a valid permutation; these give full locality up to
+// 16.7 M patches), value = coord index.
+__global__ __launch_bounds__(kBlock) void k_patch_keys(const osdhip_patch_coord *coords, int n,
+                                                       unsigned *keys, int *vals)
+{
+    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (i >= n) return;
+    keys[i] = (unsigned)coords[i].patchIndex & 0xffffffu;
+    vals[i] = i;
+}
 
 template <bool D1, bool D2, int LC>
 __global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
 {
-    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
-    if (i >= a.n) return;
+    const int lane_i = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (lane_i >= a.n) return;
+    // Each coord's result depends on that coord only, so any order gives the
+    // same bits; a patch-sorted order makes neighbouring lanes share control
+    // points and index rows in L2.
+    const int i = a.order ? a.order[lane_i] : lane_i;
 
     const osdhip_patch_coord c = a.coords[i];
     const osdhip_patch_array arr = a.arrays[c.arrayIndex];
@@ -45,12 +82,13 @@ __global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
     const float s = c.s * frac_inv - (float)pu;
     const float t = c.t * frac_inv - (float)pv;
 
-    Basis<D1, D2> B;
-    B.n = 0;
-    if (type == OSDHIP_PATCH_REGULAR) basis_bspline<D1, D2>(s, t, boundary, B);
-    else if (type == OSDHIP_PATCH_GREGORY_BASIS) basis_gregory<D1, D2>(s, t, B);
-    else if (type == OSDHIP_PATCH_QUADS) basis_linear<D1, D2>(s, t, B);
-
+    // The point count follows from the type alone (REGULAR 16, GREGORY_BASIS
+    // 20, QUADS 4, anything else 0 -> zero outputs, as the reference's
+    // Clear() without AddWithWeight), so the index and control-point loads
+    // are issued before the basis is evaluated and overlap with it.
+    const int n = type == OSDHIP_PATCH_REGULAR ? 16
+                : type == OSDHIP_PATCH_GREGORY_BASIS ? 20
+                : type == OSDHIP_PATCH_QUADS ? 4 : 0;
     const int *cvs = a.indices + arr.indexBase + arr.stride * (c.patchIndex - arr.primitiveIdBase);
     constexpr int NO = 1 + (D1 ? 2 : 0) + (D2 ? 3 : 0);
     float acc[NO][LC];
@@ -60,27 +98,41 @@ __global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
         for (int k = 0; k < LC; ++k) acc[o][k] = 0.0f;
 
     const size_t sstride = (size_t)a.srcStride;
+    Basis<D1, D2> B;
+    if (n >= 16) {
+        float v[16][LC];
 #pragma unroll
-    for (int j = 0; j < 20; ++j) {
-        if (j < B.n) {
+        for (int j = 0; j < 16; ++j) {
             const float *p = a.src + (size_t)(unsigned)cvs[j] * sstride;
-            float v[LC];
 #pragma unroll
-            for (int k = 0; k < LC; ++k) v[k] = p[k];
-#pragma unroll
-            for (int k = 0; k < LC; ++k) {
-                acc[0][k] += v[k] * B.p[j];
-                if constexpr (D1) {
-                    acc[1][k] += v[k] * B.ds[j];
-                    acc[2][k] += v[k] * B.dt[j];
-                }
-                if constexpr (D2) {
-                    acc[3][k] += v[k] * B.dss[j];
-                    acc[4][k] += v[k] * B.dst[j];
-                    acc[5][k] += v[k] * B.dtt[j];
-                }
-            }
+            for (int k = 0; k < LC; ++k) v[j][k] = p[k];
         }
+        if (n == 16) basis_bspline<D1, D2>(s, t, boundary, B);
+        else basis_gregory<D1, D2>(s, t, B);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) accumulate_point<D1, D2, LC>(v[j], B, j, acc);
+        if (n == 20) {
+            float v4[4][LC];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float *p = a.src + (size_t)(unsigned)cvs[16 + j] * sstride;
+#pragma unroll
+                for (int k = 0; k < LC; ++k) v4[j][k] = p[k];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) accumulate_point<D1, D2, LC>(v4[j], B, 16 + j, acc);
+        }
+    } else if (n == 4) {
+        float v[4][LC];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float *p = a.src + (size_t)(unsigned)cvs[j] * sstride;
+#pragma unroll
+            for (int k = 0; k < LC; ++k) v[j][k] = p[k];
+        }
+        basis_linear<D1, D2>(s, t, B);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) accumulate_point<D1, D2, LC>(v[j], B, j, acc);
     }
     // 2^depth (first) and 4^depth (second derivatives): exact power-of-two
     // factors, applied to the sums instead of to every weight.

@@ -98,4 +98,130 @@ __global__ __launch_bounds__(kBlock) void k_stencils_sliced(SlicedArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Record-interleaved slices (the default HipStencilTable layout).
+//
+// Same 64-row slices, but entry j of row r is ONE record of R dwords,
+// {index, w0 [, w1, w2 [, w3, w4, w5, pad]]} (R = 2 / 4 / 8 for 1 / 3 / 6
+// weight arrays), stored at (sliceOffset[s] + 64*j + r%64) * R.  Lane r reads
+// its whole entry with one 8- or 16-byte load and a wave reads one
+// contiguous 64*R*4-byte block per column: a single sequential stream per
+// wave instead of 1 + NW streams.  Padding records (j >= size, and rows past
+// the table end) hold index 0 and zero weights, so every gather address is
+// valid and the gathers are issued without branches; only the accumulation
+// is predicated (j < size), which keeps the sums bit-identical to
+// CpuEvalStencils.
+
+template <int NWT> struct RecShape;
+template <> struct RecShape<1> { static constexpr int R = 2; };
+template <> struct RecShape<3> { static constexpr int R = 4; };
+template <> struct RecShape<6> { static constexpr int R = 8; };
+
+struct RecArgs {
+    const float *src;       // includes srcDesc.offset and the element window start
+    int srcStride;
+    float *dst[6];          // NULL = output not requested
+    int dstStride[6];
+    const int *rowSize;     // per slice row (padded to a multiple of 64)
+    const long long *sliceOff;
+    const int *sliceWidth;
+    const unsigned *rec;
+    int rowBegin, rowEnd;   // stencils [rowBegin, rowEnd); output row = row - rowBegin
+    int sliceBegin, nslices;
+    int nblocks;
+};
+
+template <int NWT, int G>
+__device__ __forceinline__ void load_record(const unsigned *p, int &idx, float (&w)[NWT][G], int u)
+{
+    if constexpr (NWT == 1) {
+        const v2i q = __builtin_nontemporal_load((const v2i *)p);
+        idx = q.x;
+        w[0][u] = __int_as_float(q.y);
+    } else if constexpr (NWT == 3) {
+        const v4i q = __builtin_nontemporal_load((const v4i *)p);
+        idx = q.x;
+        w[0][u] = __int_as_float(q.y);
+        w[1][u] = __int_as_float(q.z);
+        w[2][u] = __int_as_float(q.w);
+    } else {
+        const v4i q = __builtin_nontemporal_load((const v4i *)p);
+        const v4i r = __builtin_nontemporal_load((const v4i *)p + 1);
+        idx = q.x;
+        w[0][u] = __int_as_float(q.y);
+        w[1][u] = __int_as_float(q.z);
+        w[2][u] = __int_as_float(q.w);
+        w[3][u] = __int_as_float(r.x);
+        w[4][u] = __int_as_float(r.y);
+        w[5][u] = __int_as_float(r.z);
+    }
+}
+
+template <int NWT, int LC, int G>
+__global__ __launch_bounds__(kBlock) void k_stencils_rec(RecArgs a)
+{
+    constexpr int R = RecShape<NWT>::R;
+    const int lane = (int)threadIdx.x & (kWave - 1);
+    const int lb = xcd_logical_block();
+    if (lb >= a.nblocks) return;
+    const int sl = lb * kWavesPerBlock + (int)threadIdx.x / kWave;
+    if (sl >= a.nslices) return;                     // whole wave idle
+    const int s = a.sliceBegin + sl;
+    const int row = s * kWave + lane;
+    const bool active = row >= a.rowBegin && row < a.rowEnd;
+    const int sz = active ? a.rowSize[row] : 0;
+    const int W = a.sliceWidth[s];
+    const unsigned *rp = a.rec + (size_t)(a.sliceOff[s] + lane) * R;
+
+    float acc[NWT][LC];
+#pragma unroll
+    for (int w = 0; w < NWT; ++w)
+#pragma unroll
+        for (int k = 0; k < LC; ++k) acc[w][k] = 0.0f;
+
+    const float *__restrict__ src = a.src;
+    const size_t sstride = (size_t)a.srcStride;
+    for (int j = 0; j < W; j += G) {
+        int ii[G];
+        float ww[NWT][G];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            ii[u] = 0;
+#pragma unroll
+            for (int w = 0; w < NWT; ++w) ww[w][u] = 0.0f;
+            if (j + u < W)                           // wave-uniform
+                load_record<NWT, G>(rp + (size_t)(j + u) * (kWave * R), ii[u], ww, u);
+        }
+        float v[G][LC];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const float *p = src + (size_t)(unsigned)ii[u] * sstride;
+#pragma unroll
+            for (int k = 0; k < LC; ++k) v[u][k] = p[k];
+        }
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const bool take = j + u < sz;
+#pragma unroll
+            for (int w = 0; w < NWT; ++w)
+#pragma unroll
+                for (int k = 0; k < LC; ++k) {
+                    const float t = acc[w][k] + v[u][k] * ww[w][u];
+                    acc[w][k] = take ? t : acc[w][k];
+                }
+        }
+    }
+    if (active) {
+        const size_t r = (size_t)(row - a.rowBegin);
+#pragma unroll
+        for (int w = 0; w < NWT; ++w) {
+            float *d = a.dst[w];
+            if (!d) continue;
+            d += r * (size_t)a.dstStride[w];
+#pragma unroll
+            for (int k = 0; k < LC; ++k) d[k] = acc[w][k];
+        }
+    }
+}
+
 }  // namespace osdhip

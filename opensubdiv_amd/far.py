@@ -25,6 +25,7 @@ def load_far():
         raise RuntimeError(f"libosdfar.so not built ({_lib.FAR_LIB}); run opensubdiv_amd.build")
     L = C.CDLL(_lib.FAR_LIB)
     sig = {
+        "osdfar_set_threads": ([i], i),
         "osdfar_topology_create": ([i, i, vp, vp, i, vp, vp, i, vp, vp, i], vp),
         "osdfar_topology_destroy": ([vp], None),
         "osdfar_refine_uniform": ([vp, i], i),

@@ -28,7 +28,7 @@ template <typename REAL>
 static int copyTable(const StencilTableT<REAL> &s, int *sizes, int *offsets32, long long *offsets64,
                      int *indices, int which, float *w, double *wd)
 {
-    const std::vector<REAL> *arr[6] = { &s.weights, &s.du, &s.dv, &s.duu, &s.duv, &s.dvv };
+    const hvec<REAL> *arr[6] = { &s.weights, &s.du, &s.dv, &s.duu, &s.duv, &s.dvv };
     if (which < 0 || which > 5) return -1;
     if (offsets32 && s.nnz() >= (size_t)0x7fffffff) return -2;
     const size_t n = s.sizes.size();
@@ -36,7 +36,7 @@ static int copyTable(const StencilTableT<REAL> &s, int *sizes, int *offsets32, l
     if (offsets32) for (size_t i = 0; i < n; ++i) offsets32[i] = (int)s.offsets[i];
     if (offsets64) for (size_t i = 0; i < n; ++i) offsets64[i] = (long long)s.offsets[i];
     if (indices) std::memcpy(indices, s.indices.data(), s.nnz() * sizeof(int));
-    const std::vector<REAL> &src = *arr[which];
+    const hvec<REAL> &src = *arr[which];
     if ((w || wd) && src.size() != s.nnz()) return -3;
     if (w) for (size_t i = 0; i < src.size(); ++i) w[i] = (float)src[i];
     if (wd) for (size_t i = 0; i < src.size(); ++i) wd[i] = (double)src[i];
@@ -46,6 +46,12 @@ static int copyTable(const StencilTableT<REAL> &s, int *sizes, int *offsets32, l
 }  // namespace
 
 extern "C" {
+
+int osdfar_set_threads(int n)
+{
+    setCompilerThreads(n);
+    return 0;
+}
 
 osdfar_topology *osdfar_topology_create(int numVertices, int numFaces,
                                         const int *numVertsPerFace,

@@ -477,7 +477,7 @@ void PatchTable::pointStencilTable(StencilTableT<REAL> &out) const
         out.sizes[i] = pointStencils.size(i);
         out.offsets[i] = pointStencils.off[i];
     }
-    out.indices = pointStencils.idx;
+    out.indices.assign(pointStencils.idx.begin(), pointStencils.idx.end());
     out.weights.assign(pointStencils.w.begin(), pointStencils.w.end());
 }
 
@@ -554,7 +554,7 @@ int PatchTable::limitStencils(int n, const int *ptex, const double *u, const dou
     for (int t = 0; t < T; ++t)
         for (int s : tSz[t]) { out.sizes.push_back(s); out.offsets.push_back(off); off += s; }
     out.indices.reserve(off);
-    std::vector<REAL> *dst[6] = { &out.weights, &out.du, &out.dv, &out.duu, &out.duv, &out.dvv };
+    hvec<REAL> *dst[6] = { &out.weights, &out.du, &out.dv, &out.duu, &out.duv, &out.dvv };
     for (int k = 0; k < nw; ++k) dst[k]->reserve(off);
     for (int t = 0; t < T; ++t) {
         out.indices.insert(out.indices.end(), tIdx[t].begin(), tIdx[t].end());

@@ -8,6 +8,11 @@
 // mask-weighted sum of its parents' rows, so every row refers to control
 // vertices only (far/stencilTableFactory.cpp:78-151).  Rows are accumulated
 // in double with a sparse accumulator and stored sorted by control index.
+//
+// Rows of one level are computed in dynamically scheduled chunks into
+// per-thread buffers; the level's arrays are then assembled by the same
+// threads (parallel first touch, no serial copies).  The last level of a
+// table is written straight into the table's arrays.
 #include "stencils.h"
 
 #include <algorithm>
@@ -18,17 +23,38 @@
 
 namespace osdfar {
 
+static int g_threads = 0;
+static thread_local int t_threads = -1;
+
+void setCompilerThreads(int n) { t_threads = n < 0 ? -1 : n; }
+
+int compilerThreads()
+{
+    int n = t_threads >= 0 ? t_threads : g_threads;
+#ifdef _OPENMP
+    if (n <= 0) n = omp_get_max_threads();
+#else
+    n = 1;
+#endif
+    return std::max(1, n);
+}
+
 namespace {
 
+// Sparse accumulator over control vertices: pos[c] is c's slot in the
+// current row or -1.
 struct Accum {
     std::vector<int> pos, list;
     std::vector<double> val;
-    std::vector<std::pair<int, double>> tmp;
-    explicit Accum(int nControl) : pos(nControl, -1) {}
-
-    void add(int i, double w)
+    explicit Accum(int nControl) : pos(nControl, -1)
     {
-        int p = pos[i];
+        list.reserve(64);
+        val.reserve(64);
+    }
+
+    inline void add(int i, double w)
+    {
+        const int p = pos[i];
         if (p < 0) {
             pos[i] = (int)list.size();
             list.push_back(i);
@@ -38,77 +64,159 @@ struct Accum {
         }
     }
     // parent row v (identity at level 0) scaled by w
-    void addRow(const LevelStencils *S, int v, double w)
+    inline void addRow(const LevelStencils *S, int v, double w)
     {
         if (!S) { add(v, w); return; }
-        for (int64_t k = S->off[v]; k < S->off[v + 1]; ++k) add(S->idx[k], w * S->w[k]);
+        const int64_t k1 = S->off[v + 1];
+        const int *ix = S->idx.data();
+        const double *ww = S->w.data();
+        for (int64_t k = S->off[v]; k < k1; ++k) add(ix[k], w * ww[k]);
     }
-    void flush(std::vector<int> &idx, std::vector<double> &w)
+    // Appends the row sorted by control index and resets the accumulator.
+    int flush(std::vector<int> &idx, std::vector<double> &w)
     {
-        tmp.resize(list.size());
-        for (size_t k = 0; k < list.size(); ++k) tmp[k] = { list[k], val[k] };
-        std::sort(tmp.begin(), tmp.end(),
-                  [](const std::pair<int, double> &a, const std::pair<int, double> &b) {
-                      return a.first < b.first;
-                  });
-        for (auto &e : tmp) {
-            idx.push_back(e.first);
-            w.push_back(e.second);
-            pos[e.first] = -1;
+        const int n = (int)list.size();
+        int *L = list.data();
+        double *V = val.data();
+        if (n <= 48) {
+            for (int a = 1; a < n; ++a) {              // insertion sort
+                const int ki = L[a];
+                const double kv = V[a];
+                int b = a - 1;
+                while (b >= 0 && L[b] > ki) { L[b + 1] = L[b]; V[b + 1] = V[b]; --b; }
+                L[b + 1] = ki;
+                V[b + 1] = kv;
+            }
+            for (int a = 0; a < n; ++a) {
+                idx.push_back(L[a]);
+                w.push_back(V[a]);
+                pos[L[a]] = -1;
+            }
+        } else {
+            std::vector<int> ord(n);
+            for (int a = 0; a < n; ++a) ord[a] = a;
+            std::sort(ord.begin(), ord.end(), [&](int x, int y) { return L[x] < L[y]; });
+            for (int a : ord) {
+                idx.push_back(L[a]);
+                w.push_back(V[a]);
+                pos[L[a]] = -1;
+            }
         }
         list.clear();
         val.clear();
+        return n;
     }
 };
 
-// Runs row(i, acc) for i in [0, n) in parallel contiguous chunks and
-// gathers the rows, in order, into out.
-template <class F>
-void parallelRows(int n, int nControl, F row, LevelStencils &out)
+// Per-thread row buffers, kept across levels and calls so their pages are
+// faulted in once per thread.
+struct RowBuffers {
+    std::vector<int> idx, sz;
+    std::vector<double> w;
+};
+static thread_local RowBuffers tl_rows;
+
+// Destination of parallelRows: begin(n, total) once, then row(i, offset,
+// size) for every row and entries(at, idx, w, count) for every chunk.
+struct LevelSink {
+    LevelStencils &S;
+    void begin(int n, int64_t total)
+    {
+        S.off.resize((size_t)n + 1);
+        S.off[n] = total;
+        S.idx.resize((size_t)total);
+        S.w.resize((size_t)total);
+    }
+    void row(int i, int64_t off, int) { S.off[i] = off; }
+    void entries(int64_t at, const int *ix, const double *w, size_t cnt)
+    {
+        std::memcpy(S.idx.data() + at, ix, cnt * sizeof(int));
+        std::memcpy(S.w.data() + at, w, cnt * sizeof(double));
+    }
+};
+
+template <typename REAL>
+struct TableSink {           // rows [rb, rb + n) of a table, entries from eb
+    StencilTableT<REAL> &T;
+    int64_t rb, eb;
+    void begin(int, int64_t total)
+    {
+        T.indices.resize((size_t)(eb + total));
+        T.weights.resize((size_t)(eb + total));
+    }
+    void row(int i, int64_t off, int size)
+    {
+        T.sizes[rb + i] = size;
+        T.offsets[rb + i] = eb + off;
+    }
+    void entries(int64_t at, const int *ix, const double *w, size_t cnt)
+    {
+        std::memcpy(T.indices.data() + eb + at, ix, cnt * sizeof(int));
+        REAL *d = T.weights.data() + eb + at;
+        for (size_t k = 0; k < cnt; ++k) d[k] = (REAL)w[k];
+    }
+};
+
+// Runs row(i, acc) for i in [0, n) over chunks of rows scheduled
+// dynamically, then assembles the rows, in order, into the sink (each
+// thread copies the chunks it owns).  avgHint: expected entries per row.
+template <class F, class Sink>
+void parallelRows(int n, int nControl, double avgHint, F row, Sink &sink)
 {
-    int T = 1;
-#ifdef _OPENMP
-    T = std::max(1, std::min(omp_get_max_threads(), (n + 1023) / 1024));
-#endif
-    std::vector<std::vector<int>> idx(T), sz(T);
-    std::vector<std::vector<double>> w(T);
+    constexpr int kChunk = 2048;
+    const int nch = (n + kChunk - 1) / kChunk;
+    const int T = std::max(1, std::min(compilerThreads(), nch));
+    struct Chunk { int owner; int64_t at, count, sizeAt, dst; };
+    std::vector<Chunk> ch((size_t)nch);
+    std::vector<const RowBuffers *> owner((size_t)T, nullptr);
+    int64_t total = 0;
 #pragma omp parallel num_threads(T)
     {
         int t = 0;
 #ifdef _OPENMP
         t = omp_get_thread_num();
 #endif
-        int lo = (int)((int64_t)n * t / T), hi = (int)((int64_t)n * (t + 1) / T);
+        RowBuffers &B = tl_rows;
+        owner[t] = &B;
+        B.idx.clear();
+        B.w.clear();
+        B.sz.clear();
+        const size_t est = (size_t)((double)n / T * avgHint * 1.15) + 1024;
+        if (B.idx.capacity() < est) { B.idx.reserve(est); B.w.reserve(est); }
         Accum acc(nControl);
-        sz[t].reserve(hi - lo);
-        for (int i = lo; i < hi; ++i) {
-            row(i, acc);
-            size_t before = idx[t].size();
-            acc.flush(idx[t], w[t]);
-            sz[t].push_back((int)(idx[t].size() - before));
+#pragma omp for schedule(dynamic, 1)
+        for (int c = 0; c < nch; ++c) {
+            Chunk &k = ch[c];
+            k.owner = t;
+            k.at = (int64_t)B.idx.size();
+            k.sizeAt = (int64_t)B.sz.size();
+            const int hi = std::min(n, (c + 1) * kChunk);
+            for (int i = c * kChunk; i < hi; ++i) {
+                row(i, acc);
+                B.sz.push_back(acc.flush(B.idx, B.w));
+            }
+            k.count = (int64_t)B.idx.size() - k.at;
+        }
+#pragma omp single
+        {
+            for (int c = 0; c < nch; ++c) { ch[c].dst = total; total += ch[c].count; }
+            sink.begin(n, total);
+        }
+        // copy out (any thread may copy any chunk from its owner's buffers)
+#pragma omp for schedule(dynamic, 1)
+        for (int c = 0; c < nch; ++c) {
+            const Chunk &k = ch[c];
+            const RowBuffers &O = *owner[k.owner];
+            int64_t o = k.dst;
+            const int hi = std::min(n, (c + 1) * kChunk);
+            for (int i = c * kChunk; i < hi; ++i) {
+                const int z = O.sz[(size_t)(k.sizeAt + (i - c * kChunk))];
+                sink.row(i, o, z);
+                o += z;
+            }
+            sink.entries(k.dst, O.idx.data() + k.at, O.w.data() + k.at, (size_t)k.count);
         }
     }
-    out.off.assign(n + 1, 0);
-    int r = 0;
-    for (int t = 0; t < T; ++t)
-        for (int s : sz[t]) { out.off[r + 1] = out.off[r] + s; ++r; }
-    out.idx.resize(out.off[n]);
-    out.w.resize(out.off[n]);
-    int64_t o = 0;
-    for (int t = 0; t < T; ++t) {
-        std::memcpy(out.idx.data() + o, idx[t].data(), idx[t].size() * sizeof(int));
-        std::memcpy(out.w.data() + o, w[t].data(), w[t].size() * sizeof(double));
-        o += (int64_t)idx[t].size();
-    }
-}
-
-void append(LevelStencils &dst, const LevelStencils &src)
-{
-    int64_t base = dst.off.empty() ? 0 : dst.off.back();
-    if (dst.off.empty()) dst.off.push_back(0);
-    for (size_t i = 1; i < src.off.size(); ++i) dst.off.push_back(base + src.off[i]);
-    dst.idx.insert(dst.idx.end(), src.idx.begin(), src.idx.end());
-    dst.w.insert(dst.w.end(), src.w.begin(), src.w.end());
 }
 
 Rule ruleFrom(float vs, int n, const float *es)
@@ -119,176 +227,223 @@ Rule ruleFrom(float vs, int n, const float *es)
     return sharp > 2 ? RULE_CORNER : (Rule)(1 << sharp);
 }
 
-// Vertex-vertex mask weights: vertex, per incident edge (opposite vertex),
-// per incident face (face point).  Returns whether face weights are used.
+// Vertex-vertex mask: weight of the vertex, per incident edge (opposite
+// vertex) and per incident face (face point).  Fixed capacity: valences
+// above kMaxValence use heap storage.
+constexpr int kMaxValence = 64;
 struct VMask {
     double v;
-    std::vector<double> e, f;
+    double *e, *f;
+    double eb[kMaxValence], fb[kMaxValence];
+    std::vector<double> eh, fh;
+    void init(int n, int nf)
+    {
+        if (n <= kMaxValence && nf <= kMaxValence) {
+            e = eb;
+            f = fb;
+        } else {
+            eh.assign(n, 0.0);
+            fh.assign(nf, 0.0);
+            e = eh.data();
+            f = fh.data();
+        }
+        v = 0.0;
+        std::fill(e, e + n, 0.0);
+        std::fill(f, f + nf, 0.0);
+    }
 };
 
-void smoothVertexMask(int n, VMask &m)
+void smoothVertexMask(int n, int nf, VMask &m)
 {
     m.v = (double)(n - 2) / (double)n;
-    double fw = 1.0 / (double)(n * n);
-    std::fill(m.e.begin(), m.e.end(), fw);
-    std::fill(m.f.begin(), m.f.end(), fw);
+    const double fw = 1.0 / (double)(n * n);
+    std::fill(m.e, m.e + n, fw);
+    std::fill(m.f, m.f + nf, fw);
 }
 
-void creaseVertexMask(int n, const float *es, VMask &m)
+void creaseVertexMask(int n, int nf, const float *es, VMask &m)
 {
     m.v = 0.75;
-    std::fill(m.e.begin(), m.e.end(), 0.0);
-    std::fill(m.f.begin(), m.f.end(), 0.0);
+    std::fill(m.e, m.e + n, 0.0);
+    std::fill(m.f, m.f + nf, 0.0);
     int found = 0;
     for (int i = 0; i < n && found < 2; ++i)
         if (es[i] > 0.0f) { m.e[i] = 0.125; ++found; }
 }
 
-void cornerVertexMask(VMask &m)
+void cornerVertexMask(int n, int nf, VMask &m)
 {
     m.v = 1.0;
-    std::fill(m.e.begin(), m.e.end(), 0.0);
-    std::fill(m.f.begin(), m.f.end(), 0.0);
+    std::fill(m.e, m.e + n, 0.0);
+    std::fill(m.f, m.f + nf, 0.0);
 }
 
-void ruleMask(Rule r, int n, const float *es, VMask &m)
+void ruleMask(Rule r, int n, int nf, const float *es, VMask &m)
 {
-    if (r == RULE_SMOOTH || r == RULE_DART) smoothVertexMask(n, m);
-    else if (r == RULE_CREASE) creaseVertexMask(n, es, m);
-    else cornerVertexMask(m);
+    if (r == RULE_SMOOTH || r == RULE_DART) smoothVertexMask(n, nf, m);
+    else if (r == RULE_CREASE) creaseVertexMask(n, nf, es, m);
+    else cornerVertexMask(n, nf, m);
 }
 
 }  // namespace
 
-void computeLevelStencils(const Refiner &r, int mode, std::vector<LevelStencils> &LS)
+// Rows of level l (parent rows PS, NULL at l == 1) into the sink; returns
+// the level's row count.
+template <class Sink>
+static int levelRows(const Refiner &r, int mode, int l, const LevelStencils *PS, double avgPrev,
+                     Sink &sink)
 {
     const int nControl = r.level(0).nv;
+    const Level &P = r.level(l - 1);
+    LevelStencils FS;
+    LevelSink fsink{ FS };
+
+    // face points: average of the face's vertices
+    parallelRows(P.nf, nControl, 2.0 * avgPrev + 4.0, [&](int f, Accum &acc) {
+        const int n = P.faceSize(f);
+        const double w = 1.0 / (double)n;
+        for (int k = P.fvOff[f]; k < P.fvOff[f + 1]; ++k) acc.addRow(PS, P.fv[k], w);
+    }, fsink);
+
+    // one pass over the level's rows in Far order:
+    // [vertex points | face points (copied) | edge points]
+    const int nv = P.nv, nf = P.nf;
+    auto vertexRow = [&](int v, Accum &acc) {
+        if (mode == INTERPOLATE_VARYING) { acc.addRow(PS, v, 1.0); return; }
+        const int n = P.valence(v);
+        const int nfv = P.vfOff[v + 1] - P.vfOff[v];
+        float pesb[kMaxValence], cesb[kMaxValence];
+        const int nn = std::min(n, kMaxValence);
+        for (int i = 0; i < nn; ++i) {
+            pesb[i] = P.esharp[P.ve[P.veOff[v] + i]];
+            cesb[i] = decrementSharpness(pesb[i]);
+        }
+        const float pvs = P.vsharp[v], cvs = decrementSharpness(pvs);
+        const Rule pr = ruleFrom(pvs, nn, pesb);
+        VMask m;
+        m.init(n, nfv);
+        if (pr == RULE_SMOOTH || pr == RULE_DART) {
+            smoothVertexMask(n, nfv, m);
+        } else {
+            ruleMask(pr, nn, nfv, pesb, m);
+            const Rule cr = ruleFrom(cvs, nn, cesb);
+            if (cr != pr) {
+                // fractional blend (sdc/crease.cpp:62-95, uniform creasing)
+                int cnt = 0;
+                double sum = 0.0;
+                if (pvs > 0.0f && cvs <= 0.0f) { cnt = 1; sum = pvs; }
+                for (int i = 0; i < nn; ++i)
+                    if (pesb[i] > 0.0f && pesb[i] <= 1.0f) { sum += pesb[i]; ++cnt; }
+                const double pw = cnt ? std::min(1.0, sum / cnt) : 0.0;
+                VMask cm;
+                cm.init(n, nfv);
+                ruleMask(cr, nn, nfv, cesb, cm);
+                const double cw = 1.0 - pw;
+                m.v = pw * m.v + cw * cm.v;
+                for (int i = 0; i < n; ++i) m.e[i] = pw * m.e[i] + cw * cm.e[i];
+                for (int i = 0; i < nfv; ++i) m.f[i] = pw * m.f[i] + cw * cm.f[i];
+            }
+        }
+        for (int i = 0; i < nfv; ++i)
+            if (m.f[i] != 0.0) acc.addRow(&FS, P.vf[P.vfOff[v] + i], m.f[i]);
+        for (int i = 0; i < n; ++i)
+            if (m.e[i] != 0.0) acc.addRow(PS, P.otherVertex(P.ve[P.veOff[v] + i], v), m.e[i]);
+        acc.addRow(PS, v, m.v);
+    };
+    auto edgeRow = [&](int e, Accum &acc) {
+        const int a = P.ev[2 * e], b = P.ev[2 * e + 1];
+        const int nfe = P.efOff[e + 1] - P.efOff[e];
+        const float s = P.esharp[e];
+        if (mode == INTERPOLATE_VARYING || (s > 0.0f && (decrementSharpness(s) > 0.0f || nfe != 2))) {
+            // crease (child vertex is a crease) or linear varying rule
+            acc.addRow(PS, a, 0.5);
+            acc.addRow(PS, b, 0.5);
+            return;
+        }
+        double vw = 0.25, fw = nfe == 2 ? 0.25 : 0.5 / (double)nfe;
+        if (s > 0.0f) {
+            // parent crease, child smooth: blend by the edge sharpness
+            const double p = (double)s, c = 1.0 - p;
+            vw = p * 0.5 + c * vw;
+            fw *= c;
+        }
+        acc.addRow(PS, a, vw);
+        acc.addRow(PS, b, vw);
+        for (int k = P.efOff[e]; k < P.efOff[e + 1]; ++k) acc.addRow(&FS, P.ef[k], fw);
+    };
+    const int nrows = nv + nf + P.ne;
+    parallelRows(nrows, nControl, 3.0 * avgPrev + 6.0, [&](int i, Accum &acc) {
+        if (i < nv) {
+            vertexRow(i, acc);
+        } else if (i < nv + nf) {
+            const int f = i - nv;           // copy (already sorted)
+            for (int64_t k = FS.off[f]; k < FS.off[f + 1]; ++k) acc.add(FS.idx[k], FS.w[k]);
+        } else {
+            edgeRow(i - nv - nf, acc);
+        }
+    }, sink);
+    return nrows;
+}
+
+void computeLevelStencils(const Refiner &r, int mode, std::vector<LevelStencils> &LS)
+{
     LS.assign(r.maxLevel() + 1, LevelStencils());
+    double avgPrev = 1.0;                       // entries per row of the parent level
     for (int l = 1; l <= r.maxLevel(); ++l) {
-        const Level &P = r.level(l - 1);
-        const LevelStencils *PS = (l == 1) ? nullptr : &LS[l - 1];
-        LevelStencils FS, ES, VS;
-
-        // face points: average of the face's vertices
-        parallelRows(P.nf, nControl, [&](int f, Accum &acc) {
-            const int n = P.faceSize(f);
-            const double w = 1.0 / (double)n;
-            for (int k = P.fvOff[f]; k < P.fvOff[f + 1]; ++k) acc.addRow(PS, P.fv[k], w);
-        }, FS);
-
-        // edge points
-        parallelRows(P.ne, nControl, [&](int e, Accum &acc) {
-            const int a = P.ev[2 * e], b = P.ev[2 * e + 1];
-            const int nf = P.efOff[e + 1] - P.efOff[e];
-            const float s = P.esharp[e];
-            if (mode == INTERPOLATE_VARYING || (s > 0.0f && (decrementSharpness(s) > 0.0f || nf != 2))) {
-                // crease (child vertex is a crease) or linear varying rule
-                acc.addRow(PS, a, 0.5);
-                acc.addRow(PS, b, 0.5);
-                return;
-            }
-            double vw = 0.25, fw = nf == 2 ? 0.25 : 0.5 / (double)nf;
-            if (s > 0.0f) {
-                // parent crease, child smooth: blend by the edge sharpness
-                const double p = (double)s, c = 1.0 - p;
-                vw = p * 0.5 + c * vw;
-                fw *= c;
-            }
-            acc.addRow(PS, a, vw);
-            acc.addRow(PS, b, vw);
-            for (int k = P.efOff[e]; k < P.efOff[e + 1]; ++k) acc.addRow(&FS, P.ef[k], fw);
-        }, ES);
-
-        // vertex points
-        parallelRows(P.nv, nControl, [&](int v, Accum &acc) {
-            if (mode == INTERPOLATE_VARYING) { acc.addRow(PS, v, 1.0); return; }
-            const int n = P.valence(v);
-            const int nf = P.vfOff[v + 1] - P.vfOff[v];
-            float pes[64], ces[64];
-            const int nn = std::min(n, 64);
-            for (int i = 0; i < nn; ++i) {
-                pes[i] = P.esharp[P.ve[P.veOff[v] + i]];
-                ces[i] = decrementSharpness(pes[i]);
-            }
-            const float pvs = P.vsharp[v], cvs = decrementSharpness(pvs);
-            const Rule pr = ruleFrom(pvs, nn, pes);
-            VMask m;
-            m.e.assign(n, 0.0);
-            m.f.assign(nf, 0.0);
-            if (pr == RULE_SMOOTH || pr == RULE_DART) {
-                smoothVertexMask(n, m);
-            } else {
-                ruleMask(pr, nn, pes, m);
-                const Rule cr = ruleFrom(cvs, nn, ces);
-                if (cr != pr) {
-                    // fractional blend (sdc/crease.cpp:62-95, uniform creasing)
-                    int cnt = 0;
-                    double sum = 0.0;
-                    if (pvs > 0.0f && cvs <= 0.0f) { cnt = 1; sum = pvs; }
-                    for (int i = 0; i < nn; ++i)
-                        if (pes[i] > 0.0f && pes[i] <= 1.0f) { sum += pes[i]; ++cnt; }
-                    double pw = cnt ? std::min(1.0, sum / cnt) : 0.0;
-                    VMask cm;
-                    cm.e.assign(n, 0.0);
-                    cm.f.assign(nf, 0.0);
-                    ruleMask(cr, nn, ces, cm);
-                    const double cw = 1.0 - pw;
-                    m.v = pw * m.v + cw * cm.v;
-                    for (int i = 0; i < n; ++i) m.e[i] = pw * m.e[i] + cw * cm.e[i];
-                    for (int i = 0; i < nf; ++i) m.f[i] = pw * m.f[i] + cw * cm.f[i];
-                }
-            }
-            for (int i = 0; i < nf; ++i)
-                if (m.f[i] != 0.0) acc.addRow(&FS, P.vf[P.vfOff[v] + i], m.f[i]);
-            for (int i = 0; i < n; ++i)
-                if (m.e[i] != 0.0) acc.addRow(PS, P.otherVertex(P.ve[P.veOff[v] + i], v), m.e[i]);
-            acc.addRow(PS, v, m.v);
-        }, VS);
-
-        LevelStencils &C = LS[l];
-        append(C, VS);
-        append(C, FS);
-        append(C, ES);
+        LevelSink sink{ LS[l] };
+        const int nrows = levelRows(r, mode, l, l == 1 ? nullptr : &LS[l - 1], avgPrev, sink);
+        avgPrev = nrows ? (double)LS[l].idx.size() / nrows : 1.0;
     }
 }
 
 template <typename REAL>
 void buildStencilTable(const Refiner &r, const StencilOptions &opt, StencilTableT<REAL> &out)
 {
-    std::vector<LevelStencils> LS;
-    computeLevelStencils(r, opt.interpolationMode, LS);
     out = StencilTableT<REAL>();
     out.numControlVertices = r.level(0).nv;
-    size_t total = 0;
-    int rows = 0;
-    const int first = opt.generateIntermediateLevels ? 1 : r.maxLevel();
-    for (int l = std::max(first, 1); l <= r.maxLevel(); ++l) {
-        total += LS[l].idx.size();
-        rows += r.level(l).nv;
+    const int maxL = r.maxLevel();
+    const int first = std::max(opt.generateIntermediateLevels ? 1 : maxL, 1);
+    const int nc = opt.generateControlVerts ? out.numControlVertices : 0;
+    int rows = nc;
+    for (int l = first; l <= maxL; ++l) rows += r.level(l).nv;
+    out.sizes.resize(rows);
+    out.offsets.resize(rows);
+    out.indices.resize(nc);
+    out.weights.resize(nc);
+    for (int v = 0; v < nc; ++v) {
+        out.sizes[v] = 1;
+        out.offsets[v] = v;
+        out.indices[v] = v;
+        out.weights[v] = (REAL)1;
     }
-    if (opt.generateControlVerts) { total += out.numControlVertices; rows += out.numControlVertices; }
-    out.sizes.reserve(rows);
-    out.offsets.reserve(rows);
-    out.indices.reserve(total);
-    out.weights.reserve(total);
-    int64_t off = 0;
-    if (opt.generateControlVerts) {
-        for (int v = 0; v < out.numControlVertices; ++v) {
-            out.sizes.push_back(1);
-            out.offsets.push_back(off++);
-            out.indices.push_back(v);
-            out.weights.push_back((REAL)1);
+    if (maxL < 1) return;
+    // levels 1..maxL: kept levels are appended to the table as they are
+    // produced; a parent level is dropped once its children exist
+    std::vector<LevelStencils> LS(maxL + 1);
+    double avgPrev = 1.0;
+    int64_t rb = nc;
+    for (int l = 1; l <= maxL; ++l) {
+        const LevelStencils *PS = l == 1 ? nullptr : &LS[l - 1];
+        int nrows;
+        if (l == maxL) {
+            TableSink<REAL> sink{ out, rb, (int64_t)out.indices.size() };
+            nrows = levelRows(r, opt.interpolationMode, l, PS, avgPrev, sink);
+        } else {
+            LevelSink sink{ LS[l] };
+            nrows = levelRows(r, opt.interpolationMode, l, PS, avgPrev, sink);
+            avgPrev = nrows ? (double)LS[l].idx.size() / nrows : 1.0;
+            if (l >= first) {
+                TableSink<REAL> ts{ out, rb, (int64_t)out.indices.size() };
+                const LevelStencils &S = LS[l];
+                ts.begin(nrows, (int64_t)S.idx.size());
+                const int T = compilerThreads();
+#pragma omp parallel for num_threads(T) schedule(static)
+                for (int v = 0; v < nrows; ++v) ts.row(v, S.off[v], S.size(v));
+                ts.entries(0, S.idx.data(), S.w.data(), S.idx.size());
+            }
         }
-    }
-    for (int l = std::max(first, 1); l <= r.maxLevel(); ++l) {
-        const LevelStencils &S = LS[l];
-        for (int v = 0; v < r.level(l).nv; ++v) {
-            out.sizes.push_back(S.size(v));
-            out.offsets.push_back(off);
-            off += S.size(v);
-        }
-        out.indices.insert(out.indices.end(), S.idx.begin(), S.idx.end());
-        for (double w : S.w) out.weights.push_back((REAL)w);
+        if (l >= first) rb += nrows;                    // rows of kept levels only
+        if (l >= 2) LS[l - 1] = LevelStencils();        // parent no longer needed
     }
 }
 

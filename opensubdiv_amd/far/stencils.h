@@ -4,21 +4,40 @@
 #pragma once
 #include <cstddef>
 #include <cstdint>
+#include <memory>
+#include <utility>
 #include <vector>
 
 #include "topology.h"
 
 namespace osdfar {
 
+// Allocator that leaves new elements uninitialised, so large row arrays are
+// first touched by the threads that fill them.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U> struct rebind { using other = NoInitAlloc<U>; };
+    NoInitAlloc() = default;
+    template <class U> NoInitAlloc(const NoInitAlloc<U> &) {}
+    template <class U> void construct(U *p) noexcept { ::new ((void *)p) U; }
+    template <class U, class... A> void construct(U *p, A &&...a)
+    {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+
+template <class T>
+using hvec = std::vector<T, NoInitAlloc<T>>;
+
 template <typename REAL>
 struct StencilTableT {
     int numControlVertices = 0;
-    std::vector<int> sizes;
-    std::vector<int64_t> offsets;     // 64-bit: batched tables exceed 2^31 entries
-    std::vector<int> indices;
-    std::vector<REAL> weights;
+    hvec<int> sizes;
+    hvec<int64_t> offsets;     // 64-bit: batched tables exceed 2^31 entries
+    hvec<int> indices;
+    hvec<REAL> weights;
     // optional derivative weights (limit stencils), same shape as weights
-    std::vector<REAL> du, dv, duu, duv, dvv;
+    hvec<REAL> du, dv, duu, duv, dvv;
     int numStencils() const { return (int)sizes.size(); }
     size_t nnz() const { return indices.size(); }
 };
@@ -37,10 +56,15 @@ struct StencilOptions {
 // One sparse row per vertex of a level, entries sorted by control index.
 struct LevelStencils {
     std::vector<int64_t> off;   // nv + 1
-    std::vector<int> idx;
-    std::vector<double> w;
+    hvec<int> idx;
+    hvec<double> w;
     int size(int v) const { return (int)(off[v + 1] - off[v]); }
 };
+
+// Host threads used by the table compiler (0 = all available).  Callers
+// that build many meshes concurrently set 1 per worker.
+void setCompilerThreads(int n);
+int compilerThreads();
 
 // Computes factorized stencils of every level 1..maxLevel of the refiner.
 // levelStencils[l] holds level l (levelStencils[0] is empty: identity).

@@ -43,3 +43,31 @@ def test_far_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(_lib.FAR_LIB)
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
+
+
+def test_batch_struct_layouts_match_the_header(tmp_path):
+    """ctypes mirrors of osdhip_batch_stencils / osdhip_batch_patches have
+    the C header's sizes and field offsets (compiled with gcc)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not found")
+    structs = {"osdhip_batch_stencils": _lib.BatchStencils,
+               "osdhip_batch_patches": _lib.BatchPatches}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "osdhip.h"', "int main(void){"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([gcc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

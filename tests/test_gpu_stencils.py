@@ -215,34 +215,55 @@ def table_eval(hip, h, src, sd, outs, descs, start, end):
                                          ctypes.cast(ds, ctypes.c_void_p), start, end, stream())
 
 
-@pytest.mark.parametrize("kernel", [0, 1])
+# (table_layout, slice_group, stencil_kernel): record slices with gather
+# groups of 8 and 4, split-stream slices, and the CSR kernel on the mirror
+VARIANTS = {"rec8": (0, 8, 0), "rec4": (0, 4, 0), "split": (1, 8, 0), "csr": (0, 8, 1)}
+
+
+@pytest.mark.parametrize("variant", sorted(VARIANTS))
 @pytest.mark.parametrize("nw,L", [(1, 3), (1, 7), (1, 12), (3, 3), (3, 7), (6, 4)])
-def test_device_table_bit_exact(hip, kernel, nw, L):
-    """HipStencilTable device layout (sliced / CSR-staged kernels) against the
-    oracle, including sub-ranges that start and end inside a 64-row slice."""
+def test_device_table_bit_exact(hip, variant, nw, L):
+    """HipStencilTable device layouts (record / split slices, CSR-staged
+    kernel) against the oracle, including sub-ranges that start and end
+    inside a 64-row slice, empty rows, and P-only evaluation of a derivative
+    table."""
     from opensubdiv_amd import HipStencilTable
-    rng = np.random.default_rng(300 + 10 * nw + L + kernel)
+    layout, group, kernel = VARIANTS[variant]
+    rng = np.random.default_rng(300 + 10 * nw + L + 7 * layout + group + kernel)
     nsrc, n = 600, 4099
     t = random_table(rng, n, nsrc, 18, nw)
     keys = ("weights", "du", "dv", "duu", "duv", "dvv")[:nw]
-    table = HipStencilTable.Create(t)
-    assert table.GetNumWeightArrays() == nw
-    src = rng.normal(size=(nsrc, L)).astype(np.float32)
-    s_t = dev(src)
-    old = hip.osdhip_config(b"stencil_kernel", kernel)
+    old = [hip.osdhip_config(b"table_layout", layout), hip.osdhip_config(b"slice_group", group),
+           hip.osdhip_config(b"stencil_kernel", kernel)]
     try:
-        for start, end in [(0, n), (37, 2000), (64, 128), (4000, 4099)]:
-            outs = [empty(n * L) for _ in range(nw)]
-            assert table_eval(hip, table.h, s_t, (0, L, L), outs, [(0, L, L)] * nw, start,
+        table = HipStencilTable.Create(t)
+        assert table.GetNumWeightArrays() == nw
+        src = rng.normal(size=(nsrc, L)).astype(np.float32)
+        s_t = dev(src)
+        cases = [(0, n, nw), (37, 2000, nw), (64, 128, nw), (4000, 4099, nw)]
+        if nw > 1:
+            cases.append((5, 3000, 1))          # P only from a derivative table
+        for start, end, nout in cases:
+            outs = [empty(n * L) for _ in range(nout)]
+            assert table_eval(hip, table.h, s_t, (0, L, L), outs, [(0, L, L)] * nout, start,
                               end) == 0
-            for k in range(nw):
+            for k in range(nout):
                 want = oracle.apply_stencils(src, t["sizes"], t["offsets"], t["indices"],
                                              t[keys[k]])[start:end]
                 got = host(outs[k], shape=(n, L))
-                np.testing.assert_array_equal(got[:end - start], want, err_msg=f"{keys[k]}")
+                np.testing.assert_array_equal(got[:end - start], want,
+                                              err_msg=f"{keys[k]} [{start},{end})")
                 assert np.all(np.isnan(got[end - start:]))
     finally:
-        hip.osdhip_config(b"stencil_kernel", old)
+        hip.osdhip_config(b"table_layout", old[0])
+        hip.osdhip_config(b"slice_group", old[1])
+        hip.osdhip_config(b"stencil_kernel", old[2])
+
+
+def test_device_table_knob_errors(hip):
+    assert hip.osdhip_config(b"table_layout", 2) == _lib.ERR_ARGS
+    assert hip.osdhip_config(b"slice_group", 5) == _lib.ERR_ARGS
+    assert hip.osdhip_config(b"no_such_knob", 0) == _lib.ERR_UNSUPPORTED
 
 
 def test_device_table_cfg0_and_errors(hip):
