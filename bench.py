@@ -349,6 +349,155 @@ def run_patches(args, lib, inputs, with_cpu):
     return res
 
 
+def buffer_checksums(dms):
+    """Exact order-independent checksum of every output buffer: the sum of
+    its float bit patterns as int64 (equal bits -> equal sums)."""
+    import torch
+    chunk = 1 << 26
+    out = []
+    for d in dms:
+        bufs = [d.refined, d.vbuf[d.nctrl * d.spec.frames * d.length:]] + d.out
+        sums = []
+        for b in bufs:
+            v = b.view(torch.int32)
+            sums.append(sum(int(v[i:i + chunk].to(torch.int64).sum().item())
+                            for i in range(0, v.numel(), chunk)))
+        out.append(sums)
+    return out
+
+
+def run_batched(args, lib, device):
+    """cfg3: 256 cube-sphere meshes (n log-uniform in [8,128], 5% creased
+    edges, level 2-4, 1-16 frames of xyz + 4 primvars, 64K-1M patch-ordered
+    PatchCoords) in one graph launch per step, against the same work one
+    mesh and one frame per launch."""
+    import torch
+    from opensubdiv_amd import workloads
+    from opensubdiv_amd.batch import Batch
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+    L = 7
+    free, total = torch.cuda.mem_get_info()
+    budget = int(args.budget_frac * total)
+    seed, specs = workloads.sample_fitting(args.meshes, args.seed, budget, L)
+    log(f"[cfg3] seed {seed}: {len(specs)} meshes, estimated "
+        f"{sum(workloads.estimate_device_bytes(x, L) for x in specs) / 1e9:.1f} GB of "
+        f"{budget / 1e9:.1f} GB budget")
+    t0 = time.time()
+    lib.osdhip_config(b"table_csr_mirror", 0)
+    workers = min(16, os.cpu_count() or 1)
+    dms = workloads.build_device_meshes(specs, L, s, workers=workers, log=log)
+    lib.osdhip_config(b"table_csr_mirror", 1)
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    dev_bytes = sum(d.device_bytes() for d in dms)
+    log(f"[cfg3] built and uploaded in {build_s:.1f}s, {dev_bytes / 1e9:.1f} GB on the device")
+    batch = Batch(L, [x for d in dms for x in d.stencil_segments()],
+                  [d.patch_segment() for d in dms], s)
+    t0 = time.time()
+    report = batch.tune(reps=2, stream=s)
+    log(f"[cfg3] tuned in {time.time() - t0:.1f}s: {batch.config()}")
+    points = sum(d.points_per_step() for d in dms)
+    alg = sum(d.algorithmic_bytes() for d in dms)
+    wall, ms = time_launches(lambda: batch.launch(s), args.steps, args.warmup, stream)
+    kms = float(np.median(ms))
+    sums_batched = buffer_checksums(dms)
+    plain_steps = max(2, min(args.steps, 5))
+    pwall, pms = time_launches(lambda: batch.run_plain(s), plain_steps, 1, stream)
+    sums_plain = buffer_checksums(dms)
+    pkms = float(np.median(pms))
+    res = {
+        "metric": METRIC, "value": points * args.steps / wall, "unit": "points/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"cfg3: {len(dms)} cube-sphere meshes (seed {seed}), n log-uniform "
+                               "[8,128], 5% creased edges, refinement level 2-4, 1-16 frames of "
+                               "xyz + 4 primvars, 64K-1M PatchCoords each; one graph launch per "
+                               "step (stencils then patches)",
+                   "meshes": len(dms), "seed": seed, "floats_per_point": L, "device": device,
+                   "refined_rows_per_step": sum(d.spec.frames * d.nrefine for d in dms),
+                   "patch_points_per_step": sum(d.spec.frames * d.ncoords for d in dms),
+                   "table_entries": sum(d.nnz_refine + d.nnz_points for d in dms),
+                   "device_bytes": dev_bytes, "build_s": build_s},
+        "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_batch_stencils + k_batch_patches (graph)",
+                     "algorithmic_bytes": alg, "kernel_ms": kms},
+        "plain": {"points_per_s": points * plain_steps / pwall, "step_ms": pkms,
+                  "launches_per_step": sum(2 * d.spec.frames + d.spec.frames for d in dms),
+                  "speedup_batched": pkms / kms},
+        "autotune": report, "tuned": batch.config(),
+        "parity": {"batched_equals_plain_checksums": sums_batched == sums_plain},
+    }
+    if not args.no_cpu:
+        res["cpu_baseline"] = batched_cpu_baseline(dms, specs, args.cpu_budget)
+    batch.close()
+    return res
+
+
+def batched_cpu_baseline(dms, specs, budget_s):
+    """Reference CpuEvaluator on a bounded sample of the batched workload:
+    the refinement stencils and EvalPatches of the smallest meshes, one
+    frame each, until about `budget_s` seconds of CPU work."""
+    from oracle import ref
+    import oracle
+    from opensubdiv_amd import far, meshes
+    t_end = time.time() + budget_s
+    pts, secs, used = 0, 0.0, 0
+    for d in sorted(dms, key=lambda x: x.nrefine):
+        if time.time() > t_end:
+            break
+        sp, L, F = d.spec, d.length, d.spec.frames
+        mesh = meshes.cube_sphere(sp.n, seed=sp.seed, radial_noise=0.01, crease_fraction=0.05,
+                                  sharpness_max=4.0)
+        topo = far.Topology(mesh)
+        topo.refine_uniform(sp.level)
+        table = topo.stencil_table()
+        vb = d.vbuf.cpu().numpy().reshape(-1, F, L)[:, 0, :].copy()
+        ctrl = np.ascontiguousarray(vb[:d.nctrl])
+        dst = np.zeros((d.nrefine, L), np.float32)
+        n = d.ncoords
+        outs = [np.zeros((n, L), np.float32) for _ in range(3)]
+        coords = d.coords.cpu().numpy().view(oracle_coord_dtype())
+        arrays = d.arrays.cpu().numpy().view(oracle_array_dtype())
+        params = d.params.cpu().numpy().view(oracle_param_dtype())
+        indices = d.indices.cpu().numpy()
+        t0 = time.perf_counter()
+        if ref.available():
+            ref.cpu_eval_stencils(ctrl, (0, L, L), dst, (0, L, L), table)
+            ref.cpu_eval_patches(vb, (0, L, L), outs, [(0, L, L)] * 3, coords, arrays, indices,
+                                 params)
+            kind = "reference"
+        else:
+            oracle.eval_stencils(ctrl, (0, L, L), dst, (0, L, L), table["sizes"],
+                                 table["offsets"], table["indices"], table["weights"])
+            oracle.eval_patches(vb, (0, L, L), outs, [(0, L, L)] * 3, coords, arrays, indices,
+                                params)
+            kind = "port"
+        secs += time.perf_counter() - t0
+        pts += d.nrefine + n
+        used += 1
+    return {"value": pts / secs if secs else None, "unit": "points/s", "cores": 1, "kind": kind,
+            "sample": f"{used} smallest meshes, frame 0: refinement stencils + EvalPatches "
+                      f"(P, dPdu, dPdv) of {pts} points in {secs:.1f}s"}
+
+
+def oracle_coord_dtype():
+    from opensubdiv_amd.osd import PATCH_COORD_DTYPE
+    return PATCH_COORD_DTYPE
+
+
+def oracle_array_dtype():
+    from opensubdiv_amd.osd import PATCH_ARRAY_DTYPE
+    return PATCH_ARRAY_DTYPE
+
+
+def oracle_param_dtype():
+    from opensubdiv_amd.osd import PATCH_PARAM_DTYPE
+    return PATCH_PARAM_DTYPE
+
+
 def copy_probe(args):
     """1 GiB device-to-device copy: reads and writes exactly 2^30 bytes per
     launch, so its FETCH_SIZE / WRITE_SIZE calibrate the counters and its
@@ -372,7 +521,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds per CPU baseline")
-    ap.add_argument("--only", default=None, choices=[None, "cfg1", "cfg1_refine", "cfg2"])
+    ap.add_argument("--only", default=None, choices=[None, "cfg1", "cfg1_refine", "cfg2", "cfg3"])
+    ap.add_argument("--meshes", type=int, default=256, help="cfg3: meshes in the batch")
+    ap.add_argument("--seed", type=int, default=0, help="cfg3: first sample seed")
+    ap.add_argument("--budget-frac", type=float, default=0.85,
+                    help="cfg3: HBM fraction the sampled batch may occupy")
     ap.add_argument("--copy-probe", action="store_true",
                     help="time a plain device-to-device copy of 1 GiB instead (HBM copy "
                          "bandwidth; the reference kernel for FETCH_SIZE/WRITE_SIZE scaling)")
@@ -392,6 +545,9 @@ def main():
     lib = _lib.load_hip()
     name = ctypes.create_string_buffer(256)
     lib.osdhip_device_info(name, 256)
+    if args.only == "cfg3":
+        print(json.dumps(run_batched(args, lib, name.value.decode())), flush=True)
+        return
     want = {"cfg1", "cfg1_refine", "cfg2"} if args.only is None else {args.only, "cfg1"}
     inputs = build_inputs(need_refine="cfg1_refine" in want, need_patches="cfg2" in want)
     with_cpu = not args.no_cpu

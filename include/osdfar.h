@@ -42,6 +42,14 @@ osdfar_topology *osdfar_topology_create(int numVertices, int numFaces,
                                         int boundaryInterpolation);
 void osdfar_topology_destroy(osdfar_topology *topo);
 
+/* Refines uniformly to `level` and keeps every level's factorized vertex
+ * stencils with the topology, so a later osdfar_stencil_table_create
+ * (vertex mode) and osdfar_patch_table_create (isoLevel == level) reuse
+ * them instead of recomputing.  osdfar_topology_drop_stencils frees them.
+ * No counterpart in the reference. */
+int osdfar_topology_cache_stencils(osdfar_topology *topo, int level);
+int osdfar_topology_drop_stencils(osdfar_topology *topo);
+
 /* TopologyRefiner::RefineUniform(level). */
 int osdfar_refine_uniform(osdfar_topology *topo, int maxLevel);
 /* Counts of a refinement level (level 0 = control mesh). */
@@ -56,6 +64,12 @@ int osdfar_level_face_vertices(const osdfar_topology *topo, int level, int *out)
 osdfar_stencil_table *osdfar_stencil_table_create(const osdfar_topology *topo, int mode,
                                                   int intermediateLevels, int controlVerts,
                                                   int useDouble);
+/* Pointers to a float table's own sizes / indices / weights (valid until
+ * the table is destroyed), for uploads without a copy.  -1 for double
+ * tables. */
+int osdfar_stencil_table_data(const osdfar_stencil_table *t, const int **sizes,
+                              const int **indices, const float **weights);
+
 /* numStencils, nnz (64-bit), control vertex count, weight arrays (1, 3, 6). */
 int osdfar_stencil_table_info(const osdfar_stencil_table *t, int *numStencils,
                               long long *nnz, int *numControlVertices, int *numWeightArrays);

@@ -26,6 +26,9 @@ def load_far():
     L = C.CDLL(_lib.FAR_LIB)
     sig = {
         "osdfar_set_threads": ([i], i),
+        "osdfar_topology_cache_stencils": ([vp, i], i),
+        "osdfar_topology_drop_stencils": ([vp], i),
+        "osdfar_stencil_table_data": ([vp, vp, vp, vp], i),
         "osdfar_topology_create": ([i, i, vp, vp, i, vp, vp, i, vp, vp, i], vp),
         "osdfar_topology_destroy": ([vp], None),
         "osdfar_refine_uniform": ([vp, i], i),
@@ -93,6 +96,35 @@ def _read_table(h, double=False):
     return out
 
 
+class RawStencilTable:
+    """A compiled float stencil table left in the compiler's memory: its
+    sizes / indices / weights pointers feed a device upload without a host
+    copy (HipStencilTable accepts it).  close() frees it."""
+
+    def __init__(self, h):
+        L = load_far()
+        self.h = h
+        ns, nnz, nc, nw = C.c_int(), C.c_longlong(), C.c_int(), C.c_int()
+        L.osdfar_stencil_table_info(h, C.byref(ns), C.byref(nnz), C.byref(nc), C.byref(nw))
+        self.num_stencils, self.nnz, self.num_control = ns.value, nnz.value, nc.value
+        p = [C.c_void_p() for _ in range(3)]
+        assert L.osdfar_stencil_table_data(h, *[C.byref(x) for x in p]) == 0
+        self.sizes_ptr, self.indices_ptr, self.weights_ptr = (x.value for x in p)
+
+    def close(self):
+        if getattr(self, "h", None) and _far is not None:
+            _far.osdfar_stencil_table_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def set_threads(n):
+    """Host threads the calling thread's table builds use (0 = all)."""
+    load_far().osdfar_set_threads(int(n))
+
+
 class Topology:
     """Control mesh + uniform refinement levels (TopologyRefiner)."""
 
@@ -116,6 +148,14 @@ class Topology:
     def refine_uniform(self, level):
         assert load_far().osdfar_refine_uniform(self.h, level) == 0
 
+    def cache_stencils(self, level):
+        """Refine to `level` and keep the level stencils, shared by the next
+        stencil_table() and patch_table(level) calls."""
+        assert load_far().osdfar_topology_cache_stencils(self.h, level) == 0
+
+    def drop_stencils(self):
+        load_far().osdfar_topology_drop_stencils(self.h)
+
     def level_info(self, level):
         nv, nf, ne = C.c_int(), C.c_int(), C.c_int()
         assert load_far().osdfar_level_info(self.h, level, C.byref(nv), C.byref(nf),
@@ -127,6 +167,13 @@ class Topology:
         out = np.zeros(nf * 4, np.int32)
         assert load_far().osdfar_level_face_vertices(self.h, level, _p(out)) == 0
         return out.reshape(nf, 4)
+
+    def stencil_table_raw(self, mode=0, intermediate=False, control=False):
+        """Float table kept in the compiler's memory (RawStencilTable)."""
+        h = load_far().osdfar_stencil_table_create(self.h, mode, int(intermediate), int(control), 0)
+        if not h:
+            raise MemoryError("osdfar_stencil_table_create failed")
+        return RawStencilTable(h)
 
     def stencil_table(self, mode=0, intermediate=False, control=False, double=False):
         h = load_far().osdfar_stencil_table_create(self.h, mode, int(intermediate), int(control),

@@ -10,6 +10,7 @@ using namespace osdfar;
 
 struct osdfar_topology {
     Refiner refiner;
+    std::vector<LevelStencils> levels;     // cached vertex-mode level stencils
 };
 
 struct osdfar_stencil_table {
@@ -82,6 +83,21 @@ osdfar_topology *osdfar_topology_create(int numVertices, int numFaces,
 
 void osdfar_topology_destroy(osdfar_topology *topo) { delete topo; }
 
+int osdfar_topology_cache_stencils(osdfar_topology *topo, int level)
+{
+    if (!topo || level < 0 || level > 10 || topo->refiner.maxLevel() > level) return -1;
+    topo->refiner.refineUniform(level);
+    computeLevelStencils(topo->refiner, INTERPOLATE_VERTEX, topo->levels);
+    return 0;
+}
+
+int osdfar_topology_drop_stencils(osdfar_topology *topo)
+{
+    if (!topo) return -1;
+    std::vector<LevelStencils>().swap(topo->levels);
+    return 0;
+}
+
 int osdfar_refine_uniform(osdfar_topology *topo, int maxLevel)
 {
     if (!topo || maxLevel < 0 || maxLevel > 10) return -1;
@@ -120,9 +136,25 @@ osdfar_stencil_table *osdfar_stencil_table_create(const osdfar_topology *topo, i
     o.generateIntermediateLevels = intermediateLevels != 0;
     o.generateControlVerts = controlVerts != 0;
     t->isDouble = useDouble != 0;
-    if (t->isDouble) buildStencilTable(topo->refiner, o, t->d);
-    else buildStencilTable(topo->refiner, o, t->f);
+    const bool cached = mode == 0 && (int)topo->levels.size() == topo->refiner.maxLevel() + 1;
+    if (cached) {
+        if (t->isDouble) stencilTableFromLevels(topo->refiner, topo->levels, o, t->d);
+        else stencilTableFromLevels(topo->refiner, topo->levels, o, t->f);
+    } else {
+        if (t->isDouble) buildStencilTable(topo->refiner, o, t->d);
+        else buildStencilTable(topo->refiner, o, t->f);
+    }
     return t.release();
+}
+
+int osdfar_stencil_table_data(const osdfar_stencil_table *t, const int **sizes,
+                              const int **indices, const float **weights)
+{
+    if (!t || t->isDouble) return -1;
+    if (sizes) *sizes = t->f.sizes.data();
+    if (indices) *indices = t->f.indices.data();
+    if (weights) *weights = t->f.weights.data();
+    return 0;
 }
 
 int osdfar_stencil_table_info(const osdfar_stencil_table *t, int *numStencils, long long *nnz,
@@ -158,7 +190,10 @@ osdfar_patch_table *osdfar_patch_table_create(osdfar_topology *topo, int isoLeve
     if (!topo || isoLevel < 0 || isoLevel > 10) return nullptr;
     if (topo->refiner.maxLevel() > isoLevel) return nullptr;
     std::unique_ptr<osdfar_patch_table> t(new (std::nothrow) osdfar_patch_table);
-    if (!t || !t->table.build(topo->refiner, isoLevel)) return nullptr;
+    const bool cached = (int)topo->levels.size() == topo->refiner.maxLevel() + 1 &&
+                        topo->refiner.maxLevel() == isoLevel;
+    if (!t || !t->table.build(topo->refiner, isoLevel, cached ? &topo->levels : nullptr))
+        return nullptr;
     return t.release();
 }
 

@@ -249,7 +249,7 @@ bool regularPoints(const Level &L, int f, int pts[16])
 
 }  // namespace
 
-bool PatchTable::build(Refiner &refiner, int isoLevel)
+bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelStencils> *levels)
 {
     {
         const Level &L0 = refiner.level(0);
@@ -262,8 +262,12 @@ bool PatchTable::build(Refiner &refiner, int isoLevel)
     }
     refiner.refineUniform(isoLevel);                           // may reallocate the levels
     const int maxL = isoLevel;
-    std::vector<LevelStencils> LS;
-    computeLevelStencils(refiner, INTERPOLATE_VERTEX, LS);
+    std::vector<LevelStencils> ownLS;
+    if (!levels || (int)levels->size() <= maxL) {
+        computeLevelStencils(refiner, INTERPOLATE_VERTEX, ownLS);
+        levels = &ownLS;
+    }
+    const std::vector<LevelStencils> &LS = *levels;
 
     numControl = refiner.level(0).nv;
     maxDepth = 0;

@@ -39,8 +39,11 @@ struct PatchHandleRec {
 class PatchTable {
 public:
     // Returns false for meshes outside the compiler's scope (non-quad base
-    // faces, boundaries / infinitely sharp features).
-    bool build(Refiner &refiner, int isoLevel);
+    // faces, boundaries / infinitely sharp features).  `levels`, when given,
+    // holds the vertex-mode level stencils of the refiner up to at least
+    // isoLevel (computed here otherwise).
+    bool build(Refiner &refiner, int isoLevel,
+               const std::vector<LevelStencils> *levels = nullptr);
 
     int numControl = 0, numRefined = 0, numLocal = 0, maxDepth = 0;
     std::vector<PatchArrayRec> arrays;      // [REGULAR (16), GREGORY_BASIS (20)]

@@ -447,6 +447,56 @@ void buildStencilTable(const Refiner &r, const StencilOptions &opt, StencilTable
     }
 }
 
+template <typename REAL>
+void stencilTableFromLevels(const Refiner &r, const std::vector<LevelStencils> &LS,
+                            const StencilOptions &opt, StencilTableT<REAL> &out)
+{
+    out = StencilTableT<REAL>();
+    out.numControlVertices = r.level(0).nv;
+    const int maxL = r.maxLevel();
+    const int first = std::max(opt.generateIntermediateLevels ? 1 : maxL, 1);
+    const int nc = opt.generateControlVerts ? out.numControlVertices : 0;
+    int rows = nc;
+    for (int l = first; l <= maxL; ++l) rows += r.level(l).nv;
+    out.sizes.resize(rows);
+    out.offsets.resize(rows);
+    out.indices.resize(nc);
+    out.weights.resize(nc);
+    for (int v = 0; v < nc; ++v) {
+        out.sizes[v] = 1;
+        out.offsets[v] = v;
+        out.indices[v] = v;
+        out.weights[v] = (REAL)1;
+    }
+    int64_t rb = nc;
+    const int T = compilerThreads();
+    for (int l = first; l <= maxL; ++l) {
+        const LevelStencils &S = LS[l];
+        const int nrows = r.level(l).nv;
+        TableSink<REAL> ts{ out, rb, (int64_t)out.indices.size() };
+        ts.begin(nrows, (int64_t)S.idx.size());
+#pragma omp parallel num_threads(T)
+        {
+#pragma omp for schedule(static)
+            for (int v = 0; v < nrows; ++v) ts.row(v, S.off[v], S.size(v));
+            // entries in parallel contiguous chunks
+            int t = 0, nt = 1;
+#ifdef _OPENMP
+            t = omp_get_thread_num();
+            nt = omp_get_num_threads();
+#endif
+            const size_t n = S.idx.size();
+            const size_t lo = n * t / nt, hi = n * (t + 1) / nt;
+            ts.entries((int64_t)lo, S.idx.data() + lo, S.w.data() + lo, hi - lo);
+        }
+        rb += nrows;
+    }
+}
+
+template void stencilTableFromLevels<float>(const Refiner &, const std::vector<LevelStencils> &,
+                                            const StencilOptions &, StencilTable &);
+template void stencilTableFromLevels<double>(const Refiner &, const std::vector<LevelStencils> &,
+                                             const StencilOptions &, StencilTableD &);
 template void buildStencilTable<float>(const Refiner &, const StencilOptions &, StencilTable &);
 template void buildStencilTable<double>(const Refiner &, const StencilOptions &, StencilTableD &);
 

@@ -75,4 +75,10 @@ void computeLevelStencils(const Refiner &r, int mode, std::vector<LevelStencils>
 template <typename REAL>
 void buildStencilTable(const Refiner &r, const StencilOptions &opt, StencilTableT<REAL> &out);
 
+// The same table from level stencils already computed for the refiner
+// (computeLevelStencils, vertex mode; levels.size() > r.maxLevel()).
+template <typename REAL>
+void stencilTableFromLevels(const Refiner &r, const std::vector<LevelStencils> &levels,
+                            const StencilOptions &opt, StencilTableT<REAL> &out);
+
 }  // namespace osdfar

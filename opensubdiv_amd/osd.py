@@ -124,6 +124,18 @@ class HipStencilTable:
 
     def __init__(self, table, stream=None):
         import ctypes
+        if hasattr(table, "weights_ptr"):            # far.RawStencilTable
+            lib = _lib.load_hip()
+            wptrs = (ctypes.c_void_p * 1)(table.weights_ptr)
+            self._n, self._nw = int(table.num_stencils), 1
+            self.h = lib.osdhip_stencil_table_create(self._n, table.sizes_ptr, None,
+                                                     table.indices_ptr, 1,
+                                                     ctypes.cast(wptrs, ctypes.c_void_p),
+                                                     _stream(stream))
+            if not self.h:
+                raise MemoryError("osdhip_stencil_table_create failed")
+            self._csr_pointers(lib)
+            return
         get = (lambda k, g: table.get(k)) if isinstance(table, dict) else \
               (lambda k, g: getattr(table, g)() if hasattr(table, g) else None)
         sizes = np.ascontiguousarray(get("sizes", "GetSizes"), np.int32)
@@ -148,6 +160,10 @@ class HipStencilTable:
             indices.ctypes.data, nw, ctypes.cast(wptrs, ctypes.c_void_p), _stream(stream))
         if not self.h:
             raise MemoryError("osdhip_stencil_table_create failed")
+        self._csr_pointers(lib)
+
+    def _csr_pointers(self, lib):
+        import ctypes
         p = [ctypes.c_void_p() for _ in range(3)]
         w6 = (ctypes.c_void_p * 6)()
         lib.osdhip_stencil_table_csr(self.h, *[ctypes.byref(x) for x in p],

@@ -1,0 +1,206 @@
+"""Multi-mesh workloads: many cube-sphere meshes, each with many frames.
+
+``sample_meshes`` draws the batched configuration: per mesh a cube-sphere
+with n x n quads per cube face, n = floor(8 * 16^U) (log-uniform in
+[8, 128]), 5 % of edges creased with sharpness U[0, 4), uniform refinement
+level U{2, 3, 4}, U{1..16} frames of control points (xyz displaced along the
+normal by 0.1 sin(2 pi f <d, P> + phi), plus 4 primvars U[0, 1)), and
+log-uniform 64K..1M PatchCoords (random ptex face, (s, t) ~ U[0, 1)^2) on
+the feature-adaptive patch table of the same isolation level.
+
+``compile_mesh`` builds one mesh's host data with the table compiler
+(libosdfar): the level-l refinement table, the adaptive patch table and its
+patch-point stencils, and the coords (listed in patch order -- the same
+random set, ordered so neighbouring coords share control points).
+``DeviceMesh`` uploads it: one vertex buffer [control | patch points] with
+frames interleaved per vertex (element (v, f, k) at (v * F + f) * L + k), a
+refinement output buffer and P / dPdu / dPdv buffers in the same layout,
+and the segments a ``Batch`` (or the one-mesh-per-launch path) evaluates.
+"""
+from __future__ import annotations
+
+import math
+from concurrent.futures import ThreadPoolExecutor, as_completed
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import far, meshes
+
+
+@dataclass
+class MeshSpec:
+    index: int
+    n: int
+    level: int
+    frames: int
+    ncoords: int
+    seed: int
+
+
+def sample_meshes(count=256, seed=0, n_range=(8, 128), levels=(2, 4), frames=(1, 16),
+                  coords=(65536, 1 << 20)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(math.floor(n_range[0] * (n_range[1] / n_range[0]) ** rng.uniform()))
+        lvl = int(rng.integers(levels[0], levels[1] + 1))
+        nf = int(rng.integers(frames[0], frames[1] + 1))
+        nc = int(math.exp(rng.uniform(math.log(coords[0]), math.log(coords[1]))))
+        out.append(MeshSpec(i, n, lvl, nf, nc, int(seed) * 100003 + 7919 * i + 1))
+    return out
+
+
+def spec_rows(s: MeshSpec):
+    """Closed forms: control vertices and level-l vertices of the cube-sphere
+    (V = F + 2 for a closed genus-0 quad mesh)."""
+    return 6 * s.n * s.n + 2, 6 * s.n * s.n * 4 ** s.level + 2
+
+
+def estimate_device_bytes(s: MeshSpec, length=7):
+    """Upper estimate of one mesh's device footprint (tables, buffers,
+    outputs) used to fit a sample into an HBM budget: 16 table entries per
+    refined vertex, patch points <= 25 % of the level-l vertices with 12
+    entries each, one patch per 2 patch points."""
+    nc, nr = spec_rows(s)
+    npts = int(0.25 * nr)
+    F, L = s.frames, length
+    tables = 8.4 * 16 * nr + 8 * nr + 8.4 * 12 * npts + 8 * npts
+    patches = (npts // 2) * (16 * 4 + 12) + 20 * s.ncoords
+    buffers = 4 * F * L * (nc + npts + nr + 3 * s.ncoords)
+    return int(tables + patches + buffers)
+
+
+def sample_fitting(count, seed, budget_bytes, length=7, max_tries=64):
+    """First seed >= `seed` whose sample fits the budget; returns (seed, specs)."""
+    for k in range(max_tries):
+        specs = sample_meshes(count, seed + k)
+        if sum(estimate_device_bytes(s, length) for s in specs) <= budget_bytes:
+            return seed + k, specs
+    raise ValueError("no sample fits the budget")
+
+
+@dataclass
+class HostMesh:
+    spec: MeshSpec
+    length: int
+    nctrl: int
+    refine: object          # far.RawStencilTable
+    points: dict            # patch-point stencils (numpy)
+    arrays: np.ndarray
+    indices: np.ndarray
+    params: np.ndarray
+    coords: np.ndarray      # PATCH_COORD_DTYPE, in patch order
+    frames: np.ndarray      # (F, nctrl, L) float32
+
+    def release(self):
+        if self.refine is not None:
+            self.refine.close()
+            self.refine = None
+
+
+def compile_mesh(spec: MeshSpec, length=7, threads=1) -> HostMesh:
+    far.set_threads(threads)
+    mesh = meshes.cube_sphere(spec.n, seed=spec.seed, radial_noise=0.01, crease_fraction=0.05,
+                              sharpness_max=4.0)
+    topo = far.Topology(mesh)
+    topo.cache_stencils(spec.level)
+    refine = topo.stencil_table_raw()
+    pt = topo.patch_table(spec.level)
+    topo.drop_stencils()
+    points = pt.point_stencils()
+    f, s, t = meshes.random_patch_locations(mesh.nfaces, spec.ncoords, seed=spec.seed + 1)
+    coords = pt.find(f, s, t)
+    coords = coords[np.argsort(coords["patchIndex"], kind="stable")]
+    frames = meshes.deform_frames(mesh, spec.frames, seed=spec.seed + 2,
+                                  extra_primvars=length - 3)
+    return HostMesh(spec, length, mesh.nverts, refine, points, pt.arrays, pt.indices, pt.params,
+                    coords, frames)
+
+
+class DeviceMesh:
+    """One mesh's device data and its batch segments (see module docstring)."""
+
+    def __init__(self, h: HostMesh, stream=None):
+        import torch
+        from .osd import HipStencilTable, _device_tensor
+        s = h.spec
+        F, L = s.frames, h.length
+        self.spec, self.length = s, L
+        self.nctrl = h.nctrl
+        self.npts = int(len(h.points["sizes"]))
+        self.nrefine = int(h.refine.num_stencils)
+        self.nnz_refine = int(h.refine.nnz)
+        self.nnz_points = int(len(h.points["indices"]))
+        self.ncoords = int(len(h.coords))
+        self.npatches = int(len(h.params))
+        self.npatch_indices = int(len(h.indices))
+        dev = "cuda"
+        self.vbuf = torch.empty((self.nctrl + self.npts) * F * L, dtype=torch.float32, device=dev)
+        ctrl = np.ascontiguousarray(h.frames.transpose(1, 0, 2)).reshape(-1)
+        self.vbuf[:ctrl.size].copy_(torch.from_numpy(ctrl))
+        self.refined = torch.empty(self.nrefine * F * L, dtype=torch.float32, device=dev)
+        self.out = [torch.empty(self.ncoords * F * L, dtype=torch.float32, device=dev)
+                    for _ in range(3)]
+        self.refine_table = HipStencilTable(h.refine, stream)
+        self.point_table = HipStencilTable(h.points, stream)
+        self.coords = _device_tensor(h.coords)
+        self.arrays = _device_tensor(h.arrays)
+        self.indices = _device_tensor(h.indices)
+        self.params = _device_tensor(h.params)
+
+    def stencil_segments(self):
+        F, L = self.spec.frames, self.length
+        v = F * L
+        return [dict(table=self.refine_table, src=self.vbuf, src_v=v, src_f=L,
+                     dst=self.refined, dst_v=v, dst_f=L, frames=F),
+                dict(table=self.point_table, src=self.vbuf, src_v=v, src_f=L,
+                     dst=self.vbuf, dst_off=self.nctrl * v, dst_v=v, dst_f=L, frames=F)]
+
+    def patch_segment(self):
+        F, L = self.spec.frames, self.length
+        v = F * L
+        return dict(n=self.ncoords, coords=self.coords, arrays=self.arrays,
+                    indices=self.indices, params=self.params, src=self.vbuf, src_v=v,
+                    src_f=L, P=self.out[0], dPdu=self.out[1], dPdv=self.out[2], out_v=v,
+                    out_f=L, frames=F)
+
+    def device_bytes(self):
+        t = self.refine_table.device_bytes() + self.point_table.device_bytes()
+        b = 4 * (self.vbuf.numel() + self.refined.numel() + sum(o.numel() for o in self.out))
+        return t + b + 4 * (self.coords.numel() + self.arrays.numel() + self.indices.numel() +
+                            self.params.numel())
+
+    def points_per_step(self):
+        """Stencil outputs (refined vertices) and PatchCoords, per frame."""
+        return self.spec.frames * (self.nrefine + self.ncoords)
+
+    def algorithmic_bytes(self):
+        """Minimum HBM traffic of one step: tables, coords and patch tables
+        once (frames are extra columns); per frame, control points read and
+        refined / patch points / P, dPdu, dPdv written once, and the patch
+        points read once by the patch phase."""
+        F, L = self.spec.frames, self.length
+        e = 4 * L
+        once = (4 * self.nrefine + 8 * self.nnz_refine + 4 * self.npts + 8 * self.nnz_points +
+                20 * self.ncoords + 4 * self.npatch_indices + 12 * self.npatches)
+        per_frame = (e * (self.nctrl + self.nrefine) + e * (self.nctrl + self.npts) +
+                     e * (self.nctrl + self.npts) + 3 * e * self.ncoords)
+        return once + F * per_frame
+
+
+def build_device_meshes(specs, length=7, stream=None, workers=8, log=None):
+    """Compiles the meshes on `workers` host threads (largest first) and
+    uploads each as it completes; returns DeviceMeshes in spec order."""
+    order = sorted(specs, key=lambda s: -spec_rows(s)[1])
+    done = {}
+    with ThreadPoolExecutor(max_workers=max(1, int(workers))) as ex:
+        futs = {ex.submit(compile_mesh, s, length): s for s in order}
+        for k, fut in enumerate(as_completed(futs)):
+            h = fut.result()
+            done[h.spec.index] = DeviceMesh(h, stream)
+            h.release()
+            del h
+            if log and (k + 1) % 16 == 0:
+                log(f"[workload] {k + 1}/{len(specs)} meshes on the device")
+    return [done[s.index] for s in specs]

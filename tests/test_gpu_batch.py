@@ -1,0 +1,185 @@
+"""GPU parity of the batched engine (osdhip_batch_*): many meshes x many
+frames in one graph launch.
+
+Every kernel variant (frames per chunk, gather group, slices per item,
+static / persistent scheduling), the captured graph (replayed several times
+with other work in between) and the autotuned configuration must produce
+the same bits as the one-mesh-per-launch path (osdhip_stencil_table_eval /
+osdhip_eval_patches per mesh and frame), which in turn equals the oracle
+(CpuEvaluator order) on the same inputs."""
+import numpy as np
+import pytest
+
+import oracle
+from opensubdiv_amd import _lib, far, meshes
+from opensubdiv_amd.batch import Batch
+from opensubdiv_amd.workloads import DeviceMesh, MeshSpec, compile_mesh
+
+pytestmark = pytest.mark.gpu
+
+# small meshes: n x n quads per cube face, refinement / isolation level,
+# frames, coords -- Gregory end caps at the valence-3 corners
+SPECS = [MeshSpec(0, 3, 2, 3, 700, 11), MeshSpec(1, 2, 1, 1, 300, 12),
+         MeshSpec(2, 5, 2, 5, 2500, 13), MeshSpec(3, 4, 3, 2, 1800, 14),
+         MeshSpec(4, 2, 2, 4, 64, 15)]
+
+
+def build(length):
+    import torch
+    old = _lib.load_hip().osdhip_config(b"table_csr_mirror", 0)
+    try:
+        hosts = [compile_mesh(s, length) for s in SPECS]
+        dms = [DeviceMesh(h) for h in hosts]
+    finally:
+        _lib.load_hip().osdhip_config(b"table_csr_mirror", old)
+    for h in hosts:
+        h.release()
+    torch.cuda.synchronize()
+    return hosts, dms
+
+
+def outputs(dms):
+    import torch
+    torch.cuda.synchronize()
+    res = []
+    for d in dms:
+        res.append([d.refined.cpu().numpy().copy(),
+                    d.vbuf[d.nctrl * d.spec.frames * d.length:].cpu().numpy().copy()] +
+                   [o.cpu().numpy().copy() for o in d.out])
+    return res
+
+
+def poison(dms):
+    import torch
+    for d in dms:
+        d.refined.fill_(float("nan"))
+        d.vbuf[d.nctrl * d.spec.frames * d.length:].fill_(float("nan"))
+        for o in d.out:
+            o.fill_(float("nan"))
+    torch.cuda.synchronize()
+
+
+def same(a, b, what):
+    for i, (x, y) in enumerate(zip(a, b)):
+        for k, (u, v) in enumerate(zip(x, y)):
+            assert np.array_equal(u.view(np.uint32), v.view(np.uint32)), f"{what}: mesh {i} buffer {k}"
+
+
+@pytest.fixture(scope="module", params=[7, 3])
+def workload(request):
+    length = request.param
+    hosts, dms = build(length)
+    batch = Batch(length, [s for d in dms for s in d.stencil_segments()],
+                  [d.patch_segment() for d in dms])
+    yield length, hosts, dms, batch
+    batch.close()
+
+
+def test_plain_path_matches_oracle(workload):
+    """One mesh and frame per launch, checked against the oracle's
+    CpuEvaluator-order sums on the same inputs (refinement, patch points,
+    P / dPdu / dPdv)."""
+    length, hosts, dms, batch = workload
+    poison(dms)
+    batch.run_plain()
+    got = outputs(dms)
+    for h, d, g in zip(hosts, dms, got):
+        s = h.spec
+        F, L = s.frames, length
+        mesh = meshes.cube_sphere(s.n, seed=s.seed, radial_noise=0.01, crease_fraction=0.05,
+                                  sharpness_max=4.0)
+        topo = far.Topology(mesh)
+        topo.refine_uniform(s.level)
+        ref = topo.stencil_table()
+        pts = h.points
+        refined = g[0].reshape(-1, F, L)
+        ptsout = g[1].reshape(-1, F, L)
+        for f in range(F):
+            ctrl = np.ascontiguousarray(h.frames[f])
+            want = oracle.apply_stencils(ctrl, ref["sizes"], ref["offsets"], ref["indices"],
+                                         ref["weights"])
+            np.testing.assert_array_equal(refined[:, f], want, err_msg=f"mesh {s.index} refine f{f}")
+            wp = oracle.apply_stencils(ctrl, pts["sizes"], pts["offsets"], pts["indices"],
+                                       pts["weights"])
+            np.testing.assert_array_equal(ptsout[:, f], wp, err_msg=f"mesh {s.index} points f{f}")
+            src = np.ascontiguousarray(np.concatenate([ctrl, wp]))
+            n = len(h.coords)
+            dst = [np.zeros((n, L), np.float32) for _ in range(3)]
+            assert oracle.eval_patches(src, (0, L, L), dst, [(0, L, L)] * 3, h.coords, h.arrays,
+                                       h.indices, h.params)
+            for k in range(3):
+                np.testing.assert_array_equal(g[2 + k].reshape(-1, F, L)[:, f], dst[k],
+                                              err_msg=f"mesh {s.index} patch output {k} f{f}")
+
+
+def test_graph_launch_matches_plain_and_replays(workload):
+    length, hosts, dms, batch = workload
+    poison(dms)
+    batch.run_plain()
+    plain = outputs(dms)
+    for rep in range(3):
+        poison(dms)
+        batch.launch()
+        same(plain, outputs(dms), f"graph replay {rep}")
+        batch.run_plain()          # other work on the stream between replays
+
+
+VARIANTS = [dict(a_frames=1, a_group=4), dict(a_frames=1, a_group=8),
+            dict(a_frames=2, a_group=4), dict(a_frames=4, a_group=2)]
+
+
+@pytest.mark.parametrize("variant", range(len(VARIANTS)))
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_every_variant_matches_plain(workload, variant, persistent):
+    length, hosts, dms, batch = workload
+    poison(dms)
+    batch.run_plain()
+    plain = outputs(dms)
+    for k, v in VARIANTS[variant].items():
+        assert batch.set(k, v) == 0
+    for slices, bfr in ((1, 1), (3, 2), (16, 4)):
+        assert batch.set("a_item_slices", slices) == 0
+        assert batch.set("a_persistent", persistent) == 0
+        assert batch.set("b_frames", bfr) == 0
+        assert batch.set("b_persistent", persistent) == 0
+        poison(dms)
+        batch.run(3)
+        same(plain, outputs(dms), f"eager {VARIANTS[variant]} slices {slices} b_frames {bfr}")
+        poison(dms)
+        batch.launch()
+        same(plain, outputs(dms), f"graph {VARIANTS[variant]} slices {slices} b_frames {bfr}")
+
+
+def test_tuned_configuration_matches_plain(workload):
+    length, hosts, dms, batch = workload
+    poison(dms)
+    batch.run_plain()
+    plain = outputs(dms)
+    report = batch.tune(reps=1)
+    assert len(report["phase_a"]) == 24 and len(report["phase_b"]) == 6
+    poison(dms)
+    batch.launch()
+    same(plain, outputs(dms), f"tuned {batch.config()}")
+
+
+def test_batch_knob_and_segment_errors(workload):
+    length, hosts, dms, batch = workload
+    assert batch.set("a_frames", 3) == _lib.ERR_ARGS
+    assert batch.set("b_frames", 8) == _lib.ERR_ARGS
+    assert batch.set("a_persistent", 2) == _lib.ERR_ARGS
+    assert batch.set("nope", 1) == _lib.ERR_UNSUPPORTED
+    info = batch.info()
+    assert info["stencil_rows"] == sum(d.nrefine + d.npts for d in dms)
+    assert info["patch_coords"] == sum(d.ncoords for d in dms)
+    # a derivative (3-weight) table is not a batch stencil segment
+    from opensubdiv_amd import HipStencilTable
+    rng = np.random.default_rng(3)
+    t = {"sizes": np.full(10, 2, np.int32), "offsets": np.arange(0, 20, 2, dtype=np.int32),
+         "indices": rng.integers(0, 4, 20).astype(np.int32)}
+    for k in ("weights", "du", "dv"):
+        t[k] = rng.uniform(size=20).astype(np.float32)
+    t3 = HipStencilTable(t)
+    seg = dict(dms[0].stencil_segments()[0])
+    seg["table"] = t3
+    with pytest.raises(ValueError):
+        Batch(length, [seg], [])

@@ -1,0 +1,59 @@
+"""Host side of the multi-mesh workloads (no GPU): the sampled distribution,
+the closed-form sizes, the budget fit and one compiled mesh."""
+import math
+
+import numpy as np
+
+from opensubdiv_amd import workloads
+from opensubdiv_amd.workloads import MeshSpec
+
+
+def test_sample_is_deterministic_and_in_range():
+    a = workloads.sample_meshes(256, seed=0)
+    b = workloads.sample_meshes(256, seed=0)
+    assert a == b and len(a) == 256
+    assert all(8 <= s.n <= 128 and 2 <= s.level <= 4 and 1 <= s.frames <= 16 and
+               65536 <= s.ncoords <= (1 << 20) for s in a)
+    # log-uniform n: median near 8 * 16^0.5 = 32
+    med = float(np.median([s.n for s in a]))
+    assert 22 <= med <= 45
+    assert len({s.seed for s in a}) == 256
+    assert workloads.sample_meshes(256, seed=1) != a
+
+
+def test_closed_form_rows_match_the_compiler():
+    from opensubdiv_amd import far, meshes
+    s = MeshSpec(0, 4, 2, 1, 100, 5)
+    nc, nr = workloads.spec_rows(s)
+    m = meshes.cube_sphere(4, seed=5)
+    assert m.nverts == nc
+    t = far.Topology(m)
+    t.refine_uniform(2)
+    assert t.level_info(2)[0] == nr
+
+
+def test_budget_fit_picks_the_first_fitting_seed():
+    specs = workloads.sample_meshes(8, seed=3)
+    need = sum(workloads.estimate_device_bytes(s) for s in specs)
+    seed, got = workloads.sample_fitting(8, 3, need)
+    assert seed == 3 and got == specs
+    seed2, got2 = workloads.sample_fitting(8, 3, need - 1, max_tries=200)
+    assert seed2 > 3
+    assert sum(workloads.estimate_device_bytes(s) for s in got2) <= need - 1
+
+
+def test_compile_mesh_outputs():
+    s = MeshSpec(0, 3, 2, 3, 500, 9)
+    h = workloads.compile_mesh(s, length=7)
+    try:
+        nc, nr = workloads.spec_rows(s)
+        assert h.nctrl == nc and h.refine.num_stencils == nr
+        assert h.frames.shape == (3, nc, 7) and h.frames.dtype == np.float32
+        assert len(h.coords) == 500 and np.all(np.diff(h.coords["patchIndex"]) >= 0)
+        assert h.coords["arrayIndex"].min() >= 0
+        assert np.all((h.frames[:, :, 3:] >= 0) & (h.frames[:, :, 3:] < 1))
+        # frames differ by the sinusoidal displacement only along the normal
+        d = h.frames[1, :, :3] - h.frames[0, :, :3]
+        assert np.abs(d).max() <= 0.2 + 1e-6 and math.isfinite(float(d.sum()))
+    finally:
+        h.release()
