@@ -156,6 +156,7 @@ def table_eval_fn(lib, table, src, srcdesc, outs, descs, stream):
 
 def kernel_labels(nw, lc):
     return {"rec_g8": f"k_stencils_rec<{nw},{lc},8>", "rec_g4": f"k_stencils_rec<{nw},{lc},4>",
+            "rec_g16": f"k_stencils_rec<{nw},{lc},16>",
             "split": f"k_stencils_sliced<{nw},{lc}>", "csr_lds": f"k_stencils_csr<{nw},{lc}>"}
 
 
@@ -181,8 +182,8 @@ def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu):
     # with gather groups of 8 / 4, split-stream slices, CSR with LDS
     # staging), then the timed region runs exactly `steps` launches of the
     # fastest
-    VARIANTS = {"rec_g8": (0, 8, 0), "rec_g4": (0, 4, 0), "split": (1, 8, 0),
-                "csr_lds": (0, 8, 1)}
+    VARIANTS = {"rec_g8": (0, 8, 0), "rec_g4": (0, 4, 0), "rec_g16": (0, 16, 0),
+                "split": (1, 8, 0), "csr_lds": (0, 8, 1)}
     variants = {}
     for vname, (layout, group, kernel) in VARIANTS.items():
         lib.osdhip_config(b"slice_group", group)
@@ -260,6 +261,50 @@ def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu):
             want = dst
         res["parity"] = {"bit_exact_vs_cpu": all(np.array_equal(a, b) for a, b in zip(got, want))}
     del htable
+    return res
+
+
+def run_cube(args, lib, with_cpu):
+    """cfg0: the tutorial cube (osd_tutorial_0.cpp:43-62, xyz + U(-0.05, 0.05)
+    noise, seed 0), uniform level-3 stencils (386 rows), evaluated in place as
+    the tutorial does (src {0,3,3}, dst {8*3,3,3} in one buffer).  A CPU
+    configuration: the point of comparison is CpuEvaluator's time per call;
+    the HIP launch is reported beside it."""
+    import torch
+    from opensubdiv_amd import HipStencilTable, _lib, far, meshes
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+    D = _lib.Desc
+    m = meshes.unit_cube(seed=0)
+    topo = far.Topology(m)
+    topo.refine_uniform(3)
+    t = topo.stencil_table()
+    n = len(t["sizes"])
+    buf = np.zeros((8 + n, 3), np.float32)
+    buf[:8] = m.positions.astype(np.float32)
+    d_buf = torch.from_numpy(buf).to("cuda")
+    table = HipStencilTable.Create(t, s)
+    step = table_eval_fn(lib, table, d_buf, D(0, 3, 3), [d_buf.view(-1)[8 * 3:]], [D(0, 3, 3)], s)
+    reps = 200
+    wall, ms = time_launches(step, reps, 20, stream)
+    res = {"stencils": n, "table_entries": int(len(t["indices"])),
+           "hip_us_per_call": float(np.median(ms)) * 1e3,
+           "hip_points_per_s": n / (float(np.median(ms)) * 1e-3)}
+    if with_cpu:
+        from oracle import ref
+        got = d_buf.cpu().numpy()
+        host = buf.copy()
+        if ref.available():
+            ref.cpu_eval_stencils(host, (0, 3, 3), host, (8 * 3, 3, 3), t)
+            calls = 2000
+            t0 = time.perf_counter()
+            for _ in range(calls):
+                ref.cpu_eval_stencils(host, (0, 3, 3), host, (8 * 3, 3, 3), t)
+            dt = (time.perf_counter() - t0) / calls
+            res["cpu_baseline"] = {"value": n / dt, "unit": "points/s", "us_per_call": dt * 1e6,
+                                   "cores": 1, "kind": "reference",
+                                   "sample": f"{calls} calls (includes the ctypes call overhead)"}
+            res["parity"] = {"bit_exact_vs_cpu": bool(np.array_equal(got, host))}
     return res
 
 
@@ -377,23 +422,40 @@ def run_batched(args, lib, device):
     stream = torch.cuda.current_stream()
     s = stream.cuda_stream
     L = 7
-    free, total = torch.cuda.mem_get_info()
-    budget = int(args.budget_frac * total)
-    seed, specs = workloads.sample_fitting(args.meshes, args.seed, budget, L)
-    log(f"[cfg3] seed {seed}: {len(specs)} meshes, estimated "
-        f"{sum(workloads.estimate_device_bytes(x, L) for x in specs) / 1e9:.1f} GB of "
-        f"{budget / 1e9:.1f} GB budget")
+    seed = args.seed
+    specs = workloads.sample_meshes(args.meshes, seed)
     t0 = time.time()
     lib.osdhip_config(b"table_csr_mirror", 0)
     workers = min(16, os.cpu_count() or 1)
-    dms = workloads.build_device_meshes(specs, L, s, workers=workers, log=log)
+    dms = workloads.build_device_meshes(specs, L, s, workers=workers, log=log, own_outputs=False)
     lib.osdhip_config(b"table_csr_mirror", 1)
     torch.cuda.synchronize()
     build_s = time.time() - t0
+    # outputs (refined vertices, P / dPdu / dPdv of every frame) stream
+    # through one arena of the memory left after the resident tables and
+    # vertex buffers; meshes fill stages in order
+    free, total = torch.cuda.mem_get_info()
+    reserve = 8 << 30
+    arena_floats = max(0, int(free - reserve) // 4)
+    arena, nstages = workloads.stage_outputs(dms, arena_floats)
     dev_bytes = sum(d.device_bytes() for d in dms)
-    log(f"[cfg3] built and uploaded in {build_s:.1f}s, {dev_bytes / 1e9:.1f} GB on the device")
+    out_bytes = 4 * sum(d.output_floats() for d in dms)
+    log(f"[cfg3] {len(dms)} meshes built and uploaded in {build_s:.1f}s; resident "
+        f"{(dev_bytes - out_bytes) / 1e9:.1f} GB, outputs {out_bytes / 1e9:.1f} GB per step "
+        f"through a {4 * arena.numel() / 1e9:.1f} GB arena in {nstages} stage(s)")
     batch = Batch(L, [x for d in dms for x in d.stencil_segments()],
                   [d.patch_segment() for d in dms], s)
+    # parity at full size: every stage, batched kernels vs one mesh and
+    # frame per launch, exact checksums of every output buffer
+    parity = True
+    for st in range(nstages):
+        members = [d for d in dms if d.stage == st]
+        batch.run_stage(st, 3, s)
+        a = buffer_checksums(members)
+        batch.run_plain_stage(st, s)
+        b = buffer_checksums(members)
+        parity &= a == b
+    log(f"[cfg3] parity batched == plain: {parity}")
     t0 = time.time()
     report = batch.tune(reps=2, stream=s)
     log(f"[cfg3] tuned in {time.time() - t0:.1f}s: {batch.config()}")
@@ -401,10 +463,8 @@ def run_batched(args, lib, device):
     alg = sum(d.algorithmic_bytes() for d in dms)
     wall, ms = time_launches(lambda: batch.launch(s), args.steps, args.warmup, stream)
     kms = float(np.median(ms))
-    sums_batched = buffer_checksums(dms)
-    plain_steps = max(2, min(args.steps, 5))
+    plain_steps = max(2, min(args.steps, 3))
     pwall, pms = time_launches(lambda: batch.run_plain(s), plain_steps, 1, stream)
-    sums_plain = buffer_checksums(dms)
     pkms = float(np.median(pms))
     res = {
         "metric": METRIC, "value": points * args.steps / wall, "unit": "points/s",
@@ -416,6 +476,7 @@ def run_batched(args, lib, device):
                                "xyz + 4 primvars, 64K-1M PatchCoords each; one graph launch per "
                                "step (stencils then patches)",
                    "meshes": len(dms), "seed": seed, "floats_per_point": L, "device": device,
+                   "stages": nstages, "output_bytes_per_step": out_bytes,
                    "refined_rows_per_step": sum(d.spec.frames * d.nrefine for d in dms),
                    "patch_points_per_step": sum(d.spec.frames * d.ncoords for d in dms),
                    "table_entries": sum(d.nnz_refine + d.nnz_points for d in dms),
@@ -428,12 +489,91 @@ def run_batched(args, lib, device):
                   "launches_per_step": sum(2 * d.spec.frames + d.spec.frames for d in dms),
                   "speedup_batched": pkms / kms},
         "autotune": report, "tuned": batch.config(),
-        "parity": {"batched_equals_plain_checksums": sums_batched == sums_plain},
+        "parity": {"batched_equals_plain_checksums": parity,
+                   "method": "per stage: int64 sums of every output buffer's bit patterns"},
     }
     if not args.no_cpu:
         res["cpu_baseline"] = batched_cpu_baseline(dms, specs, args.cpu_budget)
     batch.close()
     return res
+
+
+def run_served(args, lib, device):
+    """cfg4: a pool of 32 resident topologies drawn like cfg3; requests (one
+    new frame of xyz + 4 primvars, then the topology's refinement or its
+    limit points) arrive as a Poisson process and are served with
+    continuous batching under fixed device / pinned-host budgets."""
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+    from opensubdiv_amd import serve, workloads
+    L = 7
+    specs = workloads.sample_meshes(32, seed=args.seed + 1)
+    t0 = time.time()
+    workers = min(16, os.cpu_count() or 1)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        hosts = list(ex.map(lambda sp: workloads.compile_mesh(sp, L), specs))
+    lib.osdhip_config(b"table_csr_mirror", 0)
+    residents = serve.residents_from_hosts(hosts)
+    lib.osdhip_config(b"table_csr_mirror", 1)
+    for h in hosts:
+        h.release()
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    dev_budget, host_budget = 32 << 30, 24 << 30
+    log(f"[cfg4] {len(residents)} resident topologies in {build_s:.1f}s; budgets "
+        f"{dev_budget >> 30} GiB device, {host_budget >> 30} GiB pinned host")
+    server = serve.Server(residents, L, dev_budget, host_budget, slots=args.slots)
+
+    def points(r):
+        t = residents[r.topo]
+        return t.nrefine if r.kind == "refine" else t.ncoords
+
+    # capacity: a saturating burst (all arrive at t = 0), after a warm-up burst
+    for rep in range(2):
+        burst = serve.poisson_requests(96, 1e9, len(residents), seed=100 + rep)
+        for r in burst:
+            r.arrival = 0.0
+        tb = time.perf_counter()
+        server.run(burst, max_seconds=300)
+        burst_s = time.perf_counter() - tb
+    capacity = len(burst) / burst_s
+    rate = args.load * capacity
+    reqs = serve.poisson_requests(args.requests, rate, len(residents), seed=7)
+    keep = {r.rid for r in reqs[:16]}
+    server.epochs = []
+    tr = time.perf_counter()
+    done, results = server.run(reqs, keep=keep, max_seconds=600)
+    wall = time.perf_counter() - tr
+    lat = np.array([r.done - r.arrival for r in done]) * 1e3
+    pts = sum(points(r) for r in done)
+    makespan = max(r.done for r in done) - min(r.arrival for r in done)
+    # the same requests alone (one per epoch, nothing else in flight)
+    same = True
+    for r in [x for x in done if x.rid in keep]:
+        r2 = serve.Request(r.rid, r.topo, r.kind, 0.0, r.frame)
+        _, alone = server.run([r2], keep={r.rid}, max_seconds=60)
+        same &= bool(np.array_equal(alone[r.rid].view(np.uint32), results[r.rid].view(np.uint32)))
+    return {
+        "metric": METRIC, "value": pts / makespan, "unit": "points/s", "n_gpus": 1,
+        "steps": len(done), "warmup": 2, "ms_per_step": wall * 1e3 / len(done),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "cfg4: 32 resident cube-sphere topologies drawn like cfg3 "
+                               "(n log-uniform [8,128], level 2-4, 64K-1M PatchCoords); each "
+                               "request = one new frame (xyz + 4 primvars) + refinement or limit "
+                               "points (50/50); Poisson arrivals; continuous batching",
+                   "topologies": len(residents), "floats_per_point": L, "device": device,
+                   "device_budget_bytes": dev_budget, "host_budget_bytes": host_budget,
+                   "slots": args.slots, "build_s": build_s},
+        "served": {"requests": len(done), "rate_per_s": rate, "capacity_per_s": capacity,
+                   "load": args.load, "p50_ms": float(np.percentile(lat, 50)),
+                   "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+                   "max_ms": float(lat.max()), "epochs": len(server.epochs),
+                   "mean_requests_per_epoch": float(np.mean(server.epochs)),
+                   "max_requests_per_epoch": int(np.max(server.epochs)),
+                   "latency": "arrival -> results copied to pinned host memory"},
+        "parity": {"alone_equals_mix_bits": same, "checked_requests": len(keep)},
+    }
 
 
 def batched_cpu_baseline(dms, specs, budget_s):
@@ -509,8 +649,13 @@ def copy_probe(args):
     stream = torch.cuda.current_stream()
     wall, ms = time_launches(lambda: b.copy_(a), args.steps, args.warmup, stream)
     kms = float(np.median(ms))
+    out = torch.empty((), device="cuda")
+    _, rms = time_launches(lambda: torch.sum(a, out=out), args.steps, args.warmup, stream)
+    rkms = float(np.median(rms))
     print(json.dumps({"probe": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
                       "kernel_ms": kms, "GBps": 8 * n / (kms * 1e-3) / 1e9,
+                      "read_probe": {"bytes_read": 4 * n, "kernel_ms": rkms,
+                                     "GBps": 4 * n / (rkms * 1e-3) / 1e9, "op": "torch.sum"},
                       "steps": args.steps}), flush=True)
 
 
@@ -521,14 +666,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds per CPU baseline")
-    ap.add_argument("--only", default=None, choices=[None, "cfg1", "cfg1_refine", "cfg2", "cfg3"])
+    ap.add_argument("--only", default=None, choices=[None, "cfg1", "cfg1_refine", "cfg2", "cfg3", "cfg4"])
+    ap.add_argument("--requests", type=int, default=1500, help="cfg4: requests served")
+    ap.add_argument("--load", type=float, default=0.7,
+                    help="cfg4: arrival rate as a fraction of the measured capacity")
+    ap.add_argument("--slots", type=int, default=4, help="cfg4: epochs in flight")
     ap.add_argument("--meshes", type=int, default=256, help="cfg3: meshes in the batch")
     ap.add_argument("--seed", type=int, default=0, help="cfg3: first sample seed")
-    ap.add_argument("--budget-frac", type=float, default=0.85,
-                    help="cfg3: HBM fraction the sampled batch may occupy")
+
     ap.add_argument("--copy-probe", action="store_true",
-                    help="time a plain device-to-device copy of 1 GiB instead (HBM copy "
-                         "bandwidth; the reference kernel for FETCH_SIZE/WRITE_SIZE scaling)")
+                    help="time a plain device-to-device copy of 1 GiB and a 1 GiB read "
+                         "(torch.sum) instead: measured HBM bandwidths, and the reference "
+                         "kernel for FETCH_SIZE/WRITE_SIZE scaling")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -548,6 +697,9 @@ def main():
     if args.only == "cfg3":
         print(json.dumps(run_batched(args, lib, name.value.decode())), flush=True)
         return
+    if args.only == "cfg4":
+        print(json.dumps(run_served(args, lib, name.value.decode())), flush=True)
+        return
     want = {"cfg1", "cfg1_refine", "cfg2"} if args.only is None else {args.only, "cfg1"}
     inputs = build_inputs(need_refine="cfg1_refine" in want, need_patches="cfg2" in want)
     with_cpu = not args.no_cpu
@@ -560,6 +712,8 @@ def main():
                                                 src, 1, kernel_labels(1, 3), with_cpu)
     if "cfg2" in want:
         workloads["cfg2"] = run_patches(args, lib, inputs, with_cpu)
+    if args.only is None:
+        workloads["cfg0"] = run_cube(args, lib, with_cpu)
     head = workloads["cfg1"]
     out = {
         "metric": METRIC,

@@ -171,8 +171,8 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
  *   "table_layout":   slice layout of tables created afterwards: 0 = one
  *                     interleaved record {index, weights} per entry
  *                     (default), 1 = index and weights in separate streams.
- *   "slice_group":    entries per gather group of the record kernel, 4 or 8
- *                     (default 8).
+ *   "slice_group":    entries per gather group of the record kernel, 4, 8
+ *                     or 16 (default 8).
  *   "table_csr_mirror": 1 (default) = tables created afterwards keep the Far
  *                     CSR arrays on the device (osdhip_stencil_table_csr,
  *                     the CSR kernel); 0 = only the slice layout is kept
@@ -223,6 +223,7 @@ typedef struct {
     float *dst;                          /* row r of the table -> element r */
     long long dstVertexStride, dstFrameStride;
     int frames;
+    int stage;                           /* see below */
 } osdhip_batch_stencils;
 
 typedef struct {
@@ -236,7 +237,14 @@ typedef struct {
     float *P, *dPdu, *dPdv;              /* NULL = not written; coord c -> element c */
     long long outVertexStride, outFrameStride;
     int frames;
+    int stage;
 } osdhip_batch_patches;
+
+/* Stages: a launch runs stage 0, 1, ... in order, and within a stage every
+ * stencil segment before any patch segment, so segments of different
+ * stages may share output memory (a batch whose outputs exceed the device
+ * can stream them through one arena, each output still written once per
+ * launch).  osdhip_batch_epoch ignores `stage` (an epoch is one stage). */
 
 /* NULL on invalid segments (tables not in the record layout or with more
  * than one weight array, strides below `length`) or allocation failure. */
@@ -247,14 +255,19 @@ osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
 void osdhip_batch_destroy(osdhip_batch *batch);
 /* One graph launch (reset, phase A, phase B) on `stream`. */
 int osdhip_batch_launch(osdhip_batch *batch, void *stream);
-/* The same kernels without the graph: phases 1 = A, 2 = B, 3 = both. */
+/* The same kernels without the graph: phases 1 = A, 2 = B, 3 = both; every
+ * stage in order, or one stage (osdhip_batch_run_stage). */
 int osdhip_batch_run(osdhip_batch *batch, int phases, void *stream);
+int osdhip_batch_run_stage(osdhip_batch *batch, int stage, int phases, void *stream);
+int osdhip_batch_num_stages(const osdhip_batch *batch);
 /* The same work one mesh and one frame per launch through
  * osdhip_stencil_table_eval and osdhip_eval_patches (the plain evaluator
  * path), for comparison: every stencil segment, then every patch segment. */
 int osdhip_batch_run_plain(osdhip_batch *batch, void *stream);
+int osdhip_batch_run_plain_stage(osdhip_batch *batch, int stage, void *stream);
 /* Variant knobs: "a_frames" x "a_group" in {1x4, 1x8, 2x4, 4x2} (frames
- * per chunk, gather group), "a_item_slices" (64-row slices per work item),
+ * per chunk, gather group; other pairs make launches return
+ * OSDHIP_ERR_UNSUPPORTED), "a_item_slices" (64-row slices per work item),
  * "a_persistent" / "b_persistent" (0 = one block per item, 1 = fixed grid
  * claiming items from a device counter), "b_frames" in {1, 2, 4}.  The
  * graph is re-captured on the next launch. */
@@ -266,6 +279,23 @@ int osdhip_batch_info(const osdhip_batch *batch, long long *stencilRows, long lo
  * HIP events on `stream`), keeps the fastest and re-captures the graph.
  * `report` (optional) receives the timings as JSON. */
 int osdhip_batch_tune(osdhip_batch *batch, int reps, void *stream, char *report, int reportBytes);
+
+/* One-shot epoch of batch work, for serving loops whose segments change
+ * every epoch: no graph, no allocation.  The segment and work-item tables
+ * are written into `staging` (pinned host memory, hipHostMalloc), copied to
+ * `scratch` (device memory) on `stream`, then phase A and phase B run on
+ * `stream` (static scheduling, one frame per chunk, gather group 8).  Both
+ * buffers need osdhip_batch_epoch_bytes() bytes and must not be reused
+ * until the work queued on `stream` has completed.  Results equal
+ * osdhip_stencil_table_eval / osdhip_eval_patches per segment and frame.
+ * No counterpart in the reference. */
+long long osdhip_batch_epoch_bytes(int length, int numStencilSegments,
+                                   const osdhip_batch_stencils *stencils, int numPatchSegments,
+                                   const osdhip_batch_patches *patches, int itemSlices);
+int osdhip_batch_epoch(int length, int numStencilSegments, const osdhip_batch_stencils *stencils,
+                       int numPatchSegments, const osdhip_batch_patches *patches, int itemSlices,
+                       void *staging, long long stagingBytes, void *scratch,
+                       long long scratchBytes, void *stream);
 
 #ifdef __cplusplus
 }

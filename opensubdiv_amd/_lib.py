@@ -38,7 +38,7 @@ class BatchStencils(C.Structure):
                 ("srcVertexStride", C.c_longlong), ("srcFrameStride", C.c_longlong),
                 ("dst", C.c_void_p),
                 ("dstVertexStride", C.c_longlong), ("dstFrameStride", C.c_longlong),
-                ("frames", C.c_int)]
+                ("frames", C.c_int), ("stage", C.c_int)]
 
 
 class BatchPatches(C.Structure):
@@ -48,7 +48,7 @@ class BatchPatches(C.Structure):
                 ("srcVertexStride", C.c_longlong), ("srcFrameStride", C.c_longlong),
                 ("P", C.c_void_p), ("dPdu", C.c_void_p), ("dPdv", C.c_void_p),
                 ("outVertexStride", C.c_longlong), ("outFrameStride", C.c_longlong),
-                ("frames", C.c_int)]
+                ("frames", C.c_int), ("stage", C.c_int)]
 
 
 _vp = C.c_void_p
@@ -83,10 +83,15 @@ _HIP_SIGS = {
     "osdhip_batch_launch": [_vp, _vp],
     "osdhip_batch_run": [_vp, _i, _vp],
     "osdhip_batch_run_plain": [_vp, _vp],
+    "osdhip_batch_run_stage": [_vp, _i, _i, _vp],
+    "osdhip_batch_run_plain_stage": [_vp, _i, _vp],
+    "osdhip_batch_num_stages": [_vp],
     "osdhip_batch_set": [_vp, C.c_char_p, _i],
     "osdhip_batch_get": [_vp, C.c_char_p],
     "osdhip_batch_info": [_vp, _vp, _vp, _vp, _vp, _vp],
     "osdhip_batch_tune": [_vp, _i, _vp, _vp, _i],
+    "osdhip_batch_epoch_bytes": [_i, _i, _vp, _i, _vp, _i],
+    "osdhip_batch_epoch": [_i, _i, _vp, _i, _vp, _i, _vp, C.c_longlong, _vp, C.c_longlong, _vp],
 }
 
 _hip = None
@@ -109,6 +114,8 @@ def load_hip():
         fn.argtypes = args
         if name == "osdhip_version":
             fn.restype = C.c_char_p
+        elif name == "osdhip_batch_epoch_bytes":
+            fn.restype = C.c_longlong
         elif name in ("osdhip_stencil_table_create", "osdhip_batch_create"):
             fn.restype = C.c_void_p
         elif name in ("osdhip_stencil_table_destroy", "osdhip_batch_destroy"):

@@ -10,6 +10,7 @@ Stencil segment (dict):
     dst     device tensor; element (row, f, k) at dst + row*dst_v + f*dst_f + k
     frames, src_v, src_f, dst_v, dst_f
     src_off / dst_off (optional): float offsets into src / dst
+    stage (optional, default 0): stages run in order within a launch
 
 Patch segment (dict):
     coords, arrays, indices, params   device tensors (Osd layouts)
@@ -41,14 +42,15 @@ class Batch:
             S[i] = _lib.BatchStencils(s["table"].h, _p(s["src"], s.get("src_off", 0)),
                                       int(s["src_v"]), int(s["src_f"]),
                                       _p(s["dst"], s.get("dst_off", 0)),
-                                      int(s["dst_v"]), int(s["dst_f"]), int(s["frames"]))
+                                      int(s["dst_v"]), int(s["dst_f"]), int(s["frames"]),
+                                      int(s.get("stage", 0)))
         P = (_lib.BatchPatches * max(1, len(patches)))()
         for i, p in enumerate(patches):
             P[i] = _lib.BatchPatches(int(p["n"]), _p(p["coords"]), _p(p["arrays"]),
                                      _p(p["indices"]), _p(p["params"]), _p(p["src"]),
                                      int(p["src_v"]), int(p["src_f"]), _p(p.get("P")),
                                      _p(p.get("dPdu")), _p(p.get("dPdv")), int(p["out_v"]),
-                                     int(p["out_f"]), int(p["frames"]))
+                                     int(p["out_f"]), int(p["frames"]), int(p.get("stage", 0)))
         self.h = self._lib.osdhip_batch_create(int(length), len(stencils),
                                                ctypes.cast(S, ctypes.c_void_p), len(patches),
                                                ctypes.cast(P, ctypes.c_void_p), stream)
@@ -63,6 +65,17 @@ class Batch:
 
     def run_plain(self, stream=None):
         _lib.check(self._lib.osdhip_batch_run_plain(self.h, stream), "osdhip_batch_run_plain")
+
+    def run_stage(self, stage, phases=3, stream=None):
+        _lib.check(self._lib.osdhip_batch_run_stage(self.h, int(stage), int(phases), stream),
+                   "osdhip_batch_run_stage")
+
+    def run_plain_stage(self, stage, stream=None):
+        _lib.check(self._lib.osdhip_batch_run_plain_stage(self.h, int(stage), stream),
+                   "osdhip_batch_run_plain_stage")
+
+    def num_stages(self):
+        return self._lib.osdhip_batch_num_stages(self.h)
 
     def set(self, key, value):
         return self._lib.osdhip_batch_set(self.h, key.encode(), int(value))

@@ -35,9 +35,12 @@ struct osdhip_batch {
     std::vector<BatchStencilSeg> hS;
     std::vector<BatchPatchSeg> hP;
     std::vector<const osdhip_stencil_table *> tables;   // per stencil segment (plain path)
+    std::vector<int> sStage, pStage;       // stage of each segment
+    int nStages = 1;
     BatchItem *itemsA = nullptr, *itemsB = nullptr;
     int nItemsA = 0, nItemsB = 0, itemsASlices = 0;
-    unsigned *counters = nullptr;          // [0] phase A, [1] phase B
+    std::vector<int> beginA, countA, beginB, countB;    // item ranges per stage
+    unsigned *counters = nullptr;          // [2 * stage] phase A, [2 * stage + 1] phase B
     int numCUs = 256;
     BatchConfig cfg;
     hipGraph_t graph = nullptr;
@@ -118,17 +121,26 @@ int buildItemsA(osdhip_batch *b)
 {
     if (b->itemsA && b->itemsASlices == b->cfg.aItemSlices) return OSDHIP_OK;
     std::vector<BatchItem> items;
-    // segments with the most work per slice first (persistent balance)
+    // by stage; within a stage, segments with the most work first
+    // (persistent balance)
     std::vector<int> order(b->nS);
     for (int i = 0; i < b->nS; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (b->sStage[x] != b->sStage[y]) return b->sStage[x] < b->sStage[y];
         return (long long)b->hS[x].frames * b->hS[x].nslices >
                (long long)b->hS[y].frames * b->hS[y].nslices;
     });
     const int per = b->cfg.aItemSlices;
-    for (int s : order)
-        for (int f = 0; f < b->hS[s].nslices; f += per)
+    b->beginA.assign(b->nStages, 0);
+    b->countA.assign(b->nStages, 0);
+    for (int s : order) {
+        const int st = b->sStage[s];
+        if (b->countA[st] == 0) b->beginA[st] = (int)items.size();
+        for (int f = 0; f < b->hS[s].nslices; f += per) {
             items.push_back({ s, f, std::min(per, b->hS[s].nslices - f), 0 });
+            b->countA[st]++;
+        }
+    }
     if (b->itemsA) {
         (void)hipDeviceSynchronize();     // no queued kernel still reads the old list
         (void)hipFree(b->itemsA);
@@ -151,46 +163,53 @@ int gridFor(const osdhip_batch *b, int nitems, bool persistent)
     return std::max(1, std::min(nitems, b->numCUs * 8));
 }
 
-int enqueuePhaseA(osdhip_batch *b, hipStream_t st)
+int enqueuePhaseA(osdhip_batch *b, int stage, hipStream_t st)
 {
-    if (b->nItemsA == 0) return OSDHIP_OK;
+    const int n = b->countA[stage];
+    if (n == 0) return OSDHIP_OK;
     BatchArgs a;
     std::memset(&a, 0, sizeof(a));
     a.sseg = b->dS;
-    a.items = b->itemsA;
-    a.nitems = b->nItemsA;
+    a.items = b->itemsA + b->beginA[stage];
+    a.nitems = n;
     a.persistent = b->cfg.aPersistent;
-    a.counter = b->counters;
+    a.counter = b->counters + 2 * stage;
     if (!launchA_variant(b->length, b->cfg.aFrames, b->cfg.aGroup, a,
-                         gridFor(b, b->nItemsA, a.persistent != 0), st))
+                         gridFor(b, n, a.persistent != 0), st))
         return OSDHIP_ERR_UNSUPPORTED;
     return last_error();
 }
 
-int enqueuePhaseB(osdhip_batch *b, hipStream_t st)
+int enqueuePhaseB(osdhip_batch *b, int stage, hipStream_t st)
 {
-    if (b->nItemsB == 0) return OSDHIP_OK;
+    const int n = b->countB[stage];
+    if (n == 0) return OSDHIP_OK;
     BatchArgs a;
     std::memset(&a, 0, sizeof(a));
     a.pseg = b->dP;
-    a.items = b->itemsB;
-    a.nitems = b->nItemsB;
+    a.items = b->itemsB + b->beginB[stage];
+    a.nitems = n;
     a.persistent = b->cfg.bPersistent;
-    a.counter = b->counters + 1;
-    if (!launchB_variant(b->length, b->cfg.bFrames, a, gridFor(b, b->nItemsB, a.persistent != 0), st))
+    a.counter = b->counters + 2 * stage + 1;
+    if (!launchB_variant(b->length, b->cfg.bFrames, a, gridFor(b, n, a.persistent != 0), st))
         return OSDHIP_ERR_UNSUPPORTED;
     return last_error();
 }
 
-int enqueueAll(osdhip_batch *b, int phases, hipStream_t st)
+// stage < 0: every stage in order
+int enqueueAll(osdhip_batch *b, int phases, hipStream_t st, int stage = -1)
 {
     int rc = OSDHIP_OK;
     if ((b->cfg.aPersistent && (phases & 1)) || (b->cfg.bPersistent && (phases & 2))) {
-        hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, b->counters, 2);
+        const int nc = 2 * b->nStages;
+        hipLaunchKernelGGL(k_batch_reset, dim3((nc + 63) / 64), dim3(64), 0, st, b->counters, nc);
         rc = last_error();
     }
-    if (rc == OSDHIP_OK && (phases & 1)) rc = enqueuePhaseA(b, st);
-    if (rc == OSDHIP_OK && (phases & 2)) rc = enqueuePhaseB(b, st);
+    for (int s = stage < 0 ? 0 : stage; rc == OSDHIP_OK && s < (stage < 0 ? b->nStages : stage + 1);
+         ++s) {
+        if (phases & 1) rc = enqueuePhaseA(b, s, st);
+        if (rc == OSDHIP_OK && (phases & 2)) rc = enqueuePhaseB(b, s, st);
+    }
     return rc;
 }
 
@@ -276,7 +295,13 @@ osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
         g.nrows = t->n;
         g.nslices = t->nslices;
         g.frames = s.frames;
+        if (s.stage < 0) {
+            freeBatch(b);
+            return nullptr;
+        }
         b->hS.push_back(g);
+        b->sStage.push_back(s.stage);
+        b->nStages = std::max(b->nStages, s.stage + 1);
         b->tables.push_back(t);
         b->stencilRows += t->n;
         b->frameRows += (long long)t->n * s.frames;
@@ -293,6 +318,10 @@ osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
             return nullptr;
         }
         if (p.numCoords == 0) continue;
+        if (p.stage < 0) {
+            freeBatch(b);
+            return nullptr;
+        }
         BatchPatchSeg g;
         std::memset(&g, 0, sizeof(g));
         g.coords = p.coords;
@@ -309,17 +338,28 @@ osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
         g.outF = p.outFrameStride;
         g.ncoords = p.numCoords;
         g.frames = p.frames;
-        const int seg = (int)b->hP.size();
         b->hP.push_back(g);
-        for (int c = 0; c < p.numCoords; c += kBlock)
-            itemsB.push_back({ seg, c, std::min(kBlock, p.numCoords - c), 0 });
+        b->pStage.push_back(p.stage);
+        b->nStages = std::max(b->nStages, p.stage + 1);
         b->patchCoords += p.numCoords;
         b->patchFrames += (long long)p.numCoords * p.frames;
     }
     b->nS = (int)b->hS.size();
     b->nP = (int)b->hP.size();
+    b->beginB.assign(b->nStages, 0);
+    b->countB.assign(b->nStages, 0);
+    for (int st = 0; st < b->nStages; ++st) {
+        b->beginB[st] = (int)itemsB.size();
+        for (int seg = 0; seg < b->nP; ++seg) {
+            if (b->pStage[seg] != st) continue;
+            const int nc = b->hP[seg].ncoords;
+            for (int c = 0; c < nc; c += kBlock)
+                itemsB.push_back({ seg, c, std::min(kBlock, nc - c), 0 });
+        }
+        b->countB[st] = (int)itemsB.size() - b->beginB[st];
+    }
     b->nItemsB = (int)itemsB.size();
-    bool ok = hipMalloc(&b->counters, 2 * sizeof(unsigned)) == hipSuccess;
+    bool ok = hipMalloc(&b->counters, 2 * (size_t)b->nStages * sizeof(unsigned)) == hipSuccess;
     if (ok && b->nS)
         ok = hipMalloc(&b->dS, b->hS.size() * sizeof(BatchStencilSeg)) == hipSuccess &&
              hipMemcpy(b->dS, b->hS.data(), b->hS.size() * sizeof(BatchStencilSeg),
@@ -364,6 +404,26 @@ int osdhip_batch_run(osdhip_batch *b, int phases, void *stream)
 int osdhip_batch_run_plain(osdhip_batch *b, void *stream)
 {
     if (!b) return OSDHIP_ERR_ARGS;
+    for (int s = 0; s < b->nStages; ++s) {
+        const int rc = osdhip_batch_run_plain_stage(b, s, stream);
+        if (rc != OSDHIP_OK) return rc;
+    }
+    return OSDHIP_OK;
+}
+
+int osdhip_batch_num_stages(const osdhip_batch *b) { return b ? b->nStages : OSDHIP_ERR_ARGS; }
+
+int osdhip_batch_run_stage(osdhip_batch *b, int stage, int phases, void *stream)
+{
+    if (!b || stage < 0 || stage >= b->nStages || phases < 1 || phases > 3) return OSDHIP_ERR_ARGS;
+    int rc = buildItemsA(b);
+    if (rc != OSDHIP_OK) return rc;
+    return enqueueAll(b, phases, (hipStream_t)stream, stage);
+}
+
+int osdhip_batch_run_plain_stage(osdhip_batch *b, int stage, void *stream)
+{
+    if (!b || stage < 0 || stage >= b->nStages) return OSDHIP_ERR_ARGS;
     const int L = b->length;
     // Osd buffer descriptors hold int offsets and strides
     const long long lim = 0x7fffffffLL;
@@ -375,6 +435,7 @@ int osdhip_batch_run_plain(osdhip_batch *b, void *stream)
             return OSDHIP_ERR_UNSUPPORTED;
     for (int i = 0; i < b->nS; ++i) {
         const BatchStencilSeg &g = b->hS[i];
+        if (b->sStage[i] != stage) continue;
         for (int f = 0; f < g.frames; ++f) {
             // frame f as an Osd buffer: offset f * frameStride, stride vertexStride
             osdhip_buffer_desc sd = { (int)(f * g.srcF), L, (int)g.srcV };
@@ -387,6 +448,7 @@ int osdhip_batch_run_plain(osdhip_batch *b, void *stream)
     }
     for (int i = 0; i < b->nP; ++i) {
         const BatchPatchSeg &g = b->hP[i];
+        if (b->pStage[i] != stage) continue;
         for (int f = 0; f < g.frames; ++f) {
             osdhip_buffer_desc sd = { (int)(f * g.srcF), L, (int)g.srcV };
             osdhip_buffer_desc od = { (int)(f * g.outF), L, (int)g.outV };
@@ -412,7 +474,11 @@ int osdhip_batch_set(osdhip_batch *b, const char *key, int value)
     else if (!std::strcmp(key, "b_frames")) c.bFrames = value;
     else if (!std::strcmp(key, "b_persistent")) c.bPersistent = value;
     else return OSDHIP_ERR_UNSUPPORTED;
-    if (!validA(c.aFrames, c.aGroup) || c.aItemSlices < 1 || c.aItemSlices > 1024 ||
+    // each knob within its own domain; the (frames, group) pair is checked
+    // when the phase is launched
+    if ((c.aFrames != 1 && c.aFrames != 2 && c.aFrames != 4) ||
+        (c.aGroup != 2 && c.aGroup != 4 && c.aGroup != 8) || c.aItemSlices < 1 ||
+        c.aItemSlices > 1024 ||
         (c.aPersistent != 0 && c.aPersistent != 1) ||
         (c.bFrames != 1 && c.bFrames != 2 && c.bFrames != 4) ||
         (c.bPersistent != 0 && c.bPersistent != 1))
@@ -540,6 +606,152 @@ int osdhip_batch_tune(osdhip_batch *b, int reps, void *stream, char *report, int
         report[reportBytes - 1] = '\0';
     }
     return rc;
+}
+
+}  // extern "C"
+
+// ---- one-shot epochs (serving) ---------------------------------------------
+
+namespace {
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct EpochPlan {
+    size_t segA = 0, segB = 0, itemsA = 0, itemsB = 0, total = 0;
+    int nItemsA = 0, nItemsB = 0;
+};
+
+bool planEpoch(int length, int nS, const osdhip_batch_stencils *st, int nP,
+               const osdhip_batch_patches *pt, int itemSlices, EpochPlan &p)
+{
+    if (length < 1 || length > 8 || nS < 0 || nP < 0 || itemSlices < 1 ||
+        (nS && !st) || (nP && !pt))
+        return false;
+    long long ia = 0, ib = 0;
+    for (int i = 0; i < nS; ++i) {
+        const osdhip_stencil_table *t = st[i].table;
+        if (!t || t->layout != 0 || t->nw != 1 || st[i].frames < 1 || !st[i].src || !st[i].dst ||
+            st[i].srcVertexStride < length || st[i].dstVertexStride < length)
+            return false;
+        ia += (t->nslices + itemSlices - 1) / itemSlices;
+    }
+    for (int i = 0; i < nP; ++i) {
+        const osdhip_batch_patches &q = pt[i];
+        if (q.numCoords < 0 || q.frames < 1 || !q.src || q.srcVertexStride < length ||
+            (q.numCoords > 0 && (!q.coords || !q.arrays || !q.patchIndices || !q.params)) ||
+            ((q.P || q.dPdu || q.dPdv) && q.outVertexStride < length))
+            return false;
+        ib += (q.numCoords + kBlock - 1) / kBlock;
+    }
+    if (ia > 0x7fffffff || ib > 0x7fffffff) return false;
+    p.nItemsA = (int)ia;
+    p.nItemsB = (int)ib;
+    p.segA = 0;
+    p.segB = align256((size_t)nS * sizeof(BatchStencilSeg));
+    p.itemsA = p.segB + align256((size_t)nP * sizeof(BatchPatchSeg));
+    p.itemsB = p.itemsA + align256((size_t)ia * sizeof(BatchItem));
+    p.total = p.itemsB + align256((size_t)ib * sizeof(BatchItem));
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+long long osdhip_batch_epoch_bytes(int length, int numStencilSegments,
+                                   const osdhip_batch_stencils *stencils, int numPatchSegments,
+                                   const osdhip_batch_patches *patches, int itemSlices)
+{
+    EpochPlan p;
+    if (!planEpoch(length, numStencilSegments, stencils, numPatchSegments, patches, itemSlices, p))
+        return OSDHIP_ERR_ARGS;
+    return (long long)p.total;
+}
+
+int osdhip_batch_epoch(int length, int numStencilSegments, const osdhip_batch_stencils *stencils,
+                       int numPatchSegments, const osdhip_batch_patches *patches, int itemSlices,
+                       void *staging, long long stagingBytes, void *scratch,
+                       long long scratchBytes, void *stream)
+{
+    EpochPlan p;
+    if (!planEpoch(length, numStencilSegments, stencils, numPatchSegments, patches, itemSlices, p))
+        return OSDHIP_ERR_ARGS;
+    if (!staging || !scratch || stagingBytes < (long long)p.total || scratchBytes < (long long)p.total)
+        return OSDHIP_ERR_ARGS;
+    char *h = (char *)staging;
+    BatchStencilSeg *hs = (BatchStencilSeg *)(h + p.segA);
+    BatchPatchSeg *hp = (BatchPatchSeg *)(h + p.segB);
+    BatchItem *ha = (BatchItem *)(h + p.itemsA);
+    BatchItem *hb = (BatchItem *)(h + p.itemsB);
+    int na = 0, nb = 0;
+    for (int i = 0; i < numStencilSegments; ++i) {
+        const osdhip_batch_stencils &s = stencils[i];
+        const osdhip_stencil_table *t = s.table;
+        BatchStencilSeg g;
+        std::memset(&g, 0, sizeof(g));
+        g.rec = t->rec;
+        g.sliceOff = t->sliceOff;
+        g.sliceWidth = t->sliceWidth;
+        g.rowSize = t->rowSize;
+        g.src = s.src;
+        g.srcV = s.srcVertexStride;
+        g.srcF = s.srcFrameStride;
+        g.dst = s.dst;
+        g.dstV = s.dstVertexStride;
+        g.dstF = s.dstFrameStride;
+        g.nrows = t->n;
+        g.nslices = t->nslices;
+        g.frames = s.frames;
+        hs[i] = g;
+        for (int f = 0; f < t->nslices; f += itemSlices)
+            ha[na++] = { i, f, std::min(itemSlices, t->nslices - f), 0 };
+    }
+    for (int i = 0; i < numPatchSegments; ++i) {
+        const osdhip_batch_patches &q = patches[i];
+        BatchPatchSeg g;
+        std::memset(&g, 0, sizeof(g));
+        g.coords = q.coords;
+        g.arrays = q.arrays;
+        g.indices = q.patchIndices;
+        g.params = q.params;
+        g.src = q.src;
+        g.srcV = q.srcVertexStride;
+        g.srcF = q.srcFrameStride;
+        g.out[0] = q.P;
+        g.out[1] = q.dPdu;
+        g.out[2] = q.dPdv;
+        g.outV = q.outVertexStride;
+        g.outF = q.outFrameStride;
+        g.ncoords = q.numCoords;
+        g.frames = q.frames;
+        hp[i] = g;
+        for (int c = 0; c < q.numCoords; c += kBlock)
+            hb[nb++] = { i, c, std::min(kBlock, q.numCoords - c), 0 };
+    }
+    hipStream_t st = (hipStream_t)stream;
+    if (hipMemcpyAsync(scratch, staging, p.total, hipMemcpyHostToDevice, st) != hipSuccess)
+        return last_error();
+    char *d = (char *)scratch;
+    BatchArgs a;
+    std::memset(&a, 0, sizeof(a));
+    if (na) {
+        a.sseg = (const BatchStencilSeg *)(d + p.segA);
+        a.items = (const BatchItem *)(d + p.itemsA);
+        a.nitems = na;
+        if (!launchA_variant(length, 1, 8, a, na, st)) return OSDHIP_ERR_UNSUPPORTED;
+        int rc = last_error();
+        if (rc != OSDHIP_OK) return rc;
+    }
+    if (nb) {
+        std::memset(&a, 0, sizeof(a));
+        a.pseg = (const BatchPatchSeg *)(d + p.segB);
+        a.items = (const BatchItem *)(d + p.itemsB);
+        a.nitems = nb;
+        if (!launchB_variant(length, 1, a, nb, st)) return OSDHIP_ERR_UNSUPPORTED;
+        int rc = last_error();
+        if (rc != OSDHIP_OK) return rc;
+    }
+    return OSDHIP_OK;
 }
 
 }  // extern "C"

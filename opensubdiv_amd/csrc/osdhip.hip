@@ -39,7 +39,7 @@ static int g_stencil_kernel = 0;     // table eval: 0 sliced, 1 CSR
 static int g_patch_sort_min = 0;     // EvalPatches: patch-sort coords when n >= this (0 = never)
 static int g_table_layout = 0;       // new tables: 0 record-interleaved slices, 1 split streams
 static int g_csr_mirror = 1;         // new tables keep the Far CSR arrays on the device
-static int g_slice_group = 8;        // record kernel: entries per gather group (4 or 8)
+static int g_slice_group = 8;        // record kernel: entries per gather group (4, 8, 16)
 
 template <int NWT, int G>
 static void launch_rec_g(int lc, const RecArgs &a, int grid, hipStream_t st)
@@ -60,6 +60,7 @@ template <int NWT>
 static void launch_rec(int lc, int group, const RecArgs &a, int grid, hipStream_t st)
 {
     if (group == 4) launch_rec_g<NWT, 4>(lc, a, grid, st);
+    else if (group == 16) launch_rec_g<NWT, 16>(lc, a, grid, st);
     else launch_rec_g<NWT, 8>(lc, a, grid, st);
 }
 
@@ -593,7 +594,7 @@ int osdhip_config(const char *key, int value)
         return old;
     }
     if (std::strcmp(key, "slice_group") == 0) {
-        if (value != 4 && value != 8) return OSDHIP_ERR_ARGS;
+        if (value != 4 && value != 8 && value != 16) return OSDHIP_ERR_ARGS;
         int old = g_slice_group;
         g_slice_group = value;
         return old;

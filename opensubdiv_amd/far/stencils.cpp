@@ -115,6 +115,7 @@ struct RowBuffers {
     std::vector<double> w;
 };
 static thread_local RowBuffers tl_rows;
+constexpr size_t kKeepBytes = size_t(256) << 20;
 
 // Destination of parallelRows: begin(n, total) once, then row(i, offset,
 // size) for every row and entries(at, idx, w, count) for every chunk.
@@ -215,6 +216,13 @@ void parallelRows(int n, int nControl, double avgHint, F row, Sink &sink)
                 o += z;
             }
             sink.entries(k.dst, O.idx.data() + k.at, O.w.data() + k.at, (size_t)k.count);
+        }
+#pragma omp barrier
+        // keep small buffers for the next level / call; give large ones back
+        if (B.idx.capacity() * (sizeof(int) + sizeof(double)) > kKeepBytes) {
+            std::vector<int>().swap(B.idx);
+            std::vector<double>().swap(B.w);
+            std::vector<int>().swap(B.sz);
         }
     }
 }

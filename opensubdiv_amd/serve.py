@@ -1,0 +1,297 @@
+"""Serving loop: continuous batching of evaluation requests over topologies
+whose tables stay resident in HBM.
+
+A request names a resident topology, carries one new frame of control
+points (L floats per vertex) and asks for either the topology's uniform
+refinement (its level-l stencil table) or its limit points (P, dPdu, dPdv
+at the topology's resident PatchCoords, through the patch-point stencils and
+the patch table).  The server:
+
+* admits waiting requests in arrival order while their device buffers fit
+  the device budget and their host result buffers fit the pinned host
+  budget (first-fit arenas; no allocation per request);
+* batches everything admitted since the last launch into one epoch
+  (osdhip_batch_epoch: phase A stencils, phase B patches) on a free stream
+  slot, with the frame uploads before it and the result downloads after it
+  on the same stream;
+* keeps up to `slots` epochs in flight, so requests that arrive while others
+  are on the GPU are admitted as soon as a slot and memory free up, and their
+  work overlaps on the device;
+* completes a request when its download has finished; latency = completion
+  time - arrival time.
+
+Every request's result is the same bits alone and in any mix: each output
+element depends only on its own segment's table, frame and coords.
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+
+class Arena:
+    """First-fit allocator over [0, size) in `align`-unit blocks."""
+
+    def __init__(self, size, align=64):
+        self.size, self.align = int(size), int(align)
+        self.free = [(0, self.size)]             # sorted (offset, length)
+
+    def alloc(self, n):
+        n = (int(n) + self.align - 1) // self.align * self.align
+        for k, (o, ln) in enumerate(self.free):
+            if ln >= n:
+                if ln == n:
+                    self.free.pop(k)
+                else:
+                    self.free[k] = (o + n, ln - n)
+                return o, n
+        return None
+
+    def release(self, o, n):
+        self.free.append((o, n))
+        self.free.sort()
+        merged = []
+        for a, ln in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == a:
+                merged[-1] = (merged[-1][0], merged[-1][1] + ln)
+            else:
+                merged.append((a, ln))
+        self.free = merged
+
+    def used(self):
+        return self.size - sum(ln for _, ln in self.free)
+
+
+@dataclass
+class Resident:
+    """A topology's resident device data (from workloads.DeviceMesh-like
+    compilation with one frame)."""
+    index: int
+    nctrl: int
+    npts: int
+    nrefine: int
+    ncoords: int
+    refine_table: object
+    point_table: object
+    coords: object
+    arrays: object
+    indices: object
+    params: object
+    frames: np.ndarray       # (K, nctrl, L) pinned-able host frames to cycle through
+
+
+@dataclass
+class Request:
+    rid: int
+    topo: int
+    kind: str                # "refine" or "limit"
+    arrival: float
+    frame: int = 0
+    admitted: float = 0.0
+    done: float = 0.0
+    dev: list = field(default_factory=list)      # device arena blocks
+    host: tuple | None = None                    # host arena block
+    points: int = 0
+
+
+class Server:
+    def __init__(self, residents, length, device_budget, host_budget, slots=4, item_slices=8):
+        import torch
+        self.torch = torch
+        self.lib = _lib.load_hip()
+        self.res = residents
+        self.L = int(length)
+        self.dev = torch.empty(int(device_budget) // 4, dtype=torch.float32, device="cuda")
+        self.dev_arena = Arena(self.dev.numel(), 64)
+        self.host = torch.empty(int(host_budget) // 4, dtype=torch.float32, pin_memory=True)
+        self.host_arena = Arena(self.host.numel(), 64)
+        self.item_slices = int(item_slices)
+        self.epochs = []                         # requests per launched epoch
+        self.slots = []
+        for _ in range(int(slots)):
+            self.slots.append({"stream": torch.cuda.Stream(), "event": torch.cuda.Event(),
+                               "reqs": None, "staging": None, "scratch": None, "cap": 0})
+        # host frames in pinned memory, one block per topology
+        self.frames = [torch.from_numpy(np.ascontiguousarray(r.frames)).pin_memory()
+                       for r in residents]
+
+    # ---- sizes ----------------------------------------------------------
+    def dev_floats(self, r: Request):
+        t = self.res[r.topo]
+        if r.kind == "refine":
+            return [t.nctrl * self.L, t.nrefine * self.L]
+        return [(t.nctrl + t.npts) * self.L] + [t.ncoords * self.L] * 3
+
+    def host_floats(self, r: Request):
+        t = self.res[r.topo]
+        return (t.nrefine if r.kind == "refine" else 3 * t.ncoords) * self.L
+
+    def admit(self, r: Request):
+        blocks = []
+        for n in self.dev_floats(r):
+            b = self.dev_arena.alloc(n)
+            if b is None:
+                for x in blocks:
+                    self.dev_arena.release(*x)
+                return False
+            blocks.append(b)
+        h = self.host_arena.alloc(self.host_floats(r))
+        if h is None:
+            for x in blocks:
+                self.dev_arena.release(*x)
+            return False
+        r.dev, r.host = blocks, h
+        return True
+
+    def release(self, r: Request):
+        for x in r.dev:
+            self.dev_arena.release(*x)
+        self.host_arena.release(*r.host)
+        r.dev, r.host = [], None
+
+    # ---- one epoch ------------------------------------------------------
+    def _segments(self, reqs):
+        L, base = self.L, self.dev.data_ptr()
+        S, P = [], []
+        for r in reqs:
+            t = self.res[r.topo]
+            vb = base + 4 * r.dev[0][0]
+            if r.kind == "refine":
+                S.append(_lib.BatchStencils(t.refine_table.h, vb, L, 0, base + 4 * r.dev[1][0],
+                                            L, 0, 1))
+            else:
+                S.append(_lib.BatchStencils(t.point_table.h, vb, L, 0, vb + 4 * t.nctrl * L,
+                                            L, 0, 1))
+                P.append(_lib.BatchPatches(t.ncoords, t.coords.data_ptr(), t.arrays.data_ptr(),
+                                           t.indices.data_ptr(), t.params.data_ptr(), vb, L, 0,
+                                           base + 4 * r.dev[1][0], base + 4 * r.dev[2][0],
+                                           base + 4 * r.dev[3][0], L, 0, 1))
+        SA = (_lib.BatchStencils * max(1, len(S)))(*S)
+        PA = (_lib.BatchPatches * max(1, len(P)))(*P)
+        return len(S), SA, len(P), PA
+
+    def launch(self, slot, reqs):
+        torch = self.torch
+        nS, SA, nP, PA = self._segments(reqs)
+        need = self.lib.osdhip_batch_epoch_bytes(self.L, nS, ctypes.cast(SA, ctypes.c_void_p), nP,
+                                                 ctypes.cast(PA, ctypes.c_void_p),
+                                                 self.item_slices)
+        _lib.check(0 if need > 0 else need, "osdhip_batch_epoch_bytes")
+        if need > slot["cap"]:
+            # grow this slot's staging / scratch (only between its epochs)
+            cap = max(need, 2 * slot["cap"], 1 << 20)
+            slot["staging"] = torch.empty(cap, dtype=torch.uint8, pin_memory=True)
+            slot["scratch"] = torch.empty(cap, dtype=torch.uint8, device="cuda")
+            slot["cap"] = cap
+        st = slot["stream"]
+        with torch.cuda.stream(st):
+            for r in reqs:
+                t = self.res[r.topo]
+                src = self.frames[r.topo][r.frame % self.frames[r.topo].shape[0]].reshape(-1)
+                o, _ = r.dev[0]
+                self.dev[o:o + t.nctrl * self.L].copy_(src, non_blocking=True)
+            rc = self.lib.osdhip_batch_epoch(self.L, nS, ctypes.cast(SA, ctypes.c_void_p), nP,
+                                             ctypes.cast(PA, ctypes.c_void_p), self.item_slices,
+                                             slot["staging"].data_ptr(), slot["cap"],
+                                             slot["scratch"].data_ptr(), slot["cap"],
+                                             st.cuda_stream)
+            _lib.check(rc, "osdhip_batch_epoch")
+            for r in reqs:
+                ho, _ = r.host
+                if r.kind == "refine":
+                    o, _ = r.dev[1]
+                    n = self.res[r.topo].nrefine * self.L
+                    self.host[ho:ho + n].copy_(self.dev[o:o + n], non_blocking=True)
+                else:
+                    n = self.res[r.topo].ncoords * self.L
+                    for k in range(3):
+                        o, _ = r.dev[1 + k]
+                        self.host[ho + k * n:ho + (k + 1) * n].copy_(self.dev[o:o + n],
+                                                                     non_blocking=True)
+            slot["event"].record(st)
+        slot["reqs"] = reqs
+        self.epochs.append(len(reqs))
+
+    def result(self, r: Request):
+        """Host copy of a completed request's output (before release)."""
+        ho, _ = r.host
+        n = self.host_floats(r)
+        return self.host[ho:ho + n].numpy().copy()
+
+    # ---- the loop -------------------------------------------------------
+    def run(self, requests, keep=None, max_seconds=600.0):
+        """Serves `requests` (sorted by arrival, seconds from start) in real
+        time.  `keep`: set of request ids whose results are returned.
+        Returns (completed requests, results)."""
+        pending = list(requests)
+        waiting, done, results = [], [], {}
+        t0 = time.perf_counter()
+        i = 0
+        while len(done) < len(requests):
+            now = time.perf_counter() - t0
+            if now > max_seconds:
+                raise TimeoutError("serving loop exceeded its time limit")
+            while i < len(pending) and pending[i].arrival <= now:
+                waiting.append(pending[i])
+                i += 1
+            # completions
+            for sl in self.slots:
+                if sl["reqs"] is not None and sl["event"].query():
+                    tdone = time.perf_counter() - t0
+                    for r in sl["reqs"]:
+                        r.done = tdone
+                        if keep is not None and r.rid in keep:
+                            results[r.rid] = self.result(r)
+                        self.release(r)
+                        done.append(r)
+                    sl["reqs"] = None
+            # admission + launch on a free slot
+            free = [sl for sl in self.slots if sl["reqs"] is None]
+            if waiting and free:
+                batch = []
+                while waiting and self.admit(waiting[0]):
+                    r = waiting.pop(0)
+                    r.admitted = now
+                    batch.append(r)
+                if batch:
+                    self.launch(free[0], batch)
+                elif not any(sl["reqs"] is not None for sl in self.slots):
+                    raise MemoryError(f"request {waiting[0].rid} exceeds the serving budgets")
+            if i < len(pending) and not waiting and all(sl["reqs"] is None for sl in self.slots):
+                # idle until the next arrival
+                dt = pending[i].arrival - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(min(dt, 0.001))
+        return done, results
+
+
+def poisson_requests(n, rate, ntopo, seed, limit_fraction=0.5, nframes=8):
+    """n requests, exponential inter-arrival times of mean 1/rate, uniform
+    topology, kind limit with probability `limit_fraction`."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    reqs = []
+    for k in range(n):
+        reqs.append(Request(k, int(rng.integers(0, ntopo)),
+                            "limit" if rng.uniform() < limit_fraction else "refine",
+                            float(t[k]), int(rng.integers(0, nframes))))
+    return reqs
+
+
+def residents_from_hosts(hosts, stream=None):
+    """Resident topologies from compiled workloads.HostMesh objects (their
+    `frames` become the frame pool requests cycle through)."""
+    from .osd import HipStencilTable, _device_tensor
+    out = []
+    for h in hosts:
+        out.append(Resident(h.spec.index, h.nctrl, int(len(h.points["sizes"])),
+                            int(h.refine.num_stencils), int(len(h.coords)),
+                            HipStencilTable(h.refine, stream), HipStencilTable(h.points, stream),
+                            _device_tensor(h.coords), _device_tensor(h.arrays),
+                            _device_tensor(h.indices), _device_tensor(h.params), h.frames))
+    return out

@@ -57,18 +57,26 @@ def spec_rows(s: MeshSpec):
     return 6 * s.n * s.n + 2, 6 * s.n * s.n * 4 ** s.level + 2
 
 
+# Measured on creased cube-spheres (5 % creased edges) by level: table
+# entries per refined vertex, patch points per level-l vertex, entries per
+# patch point, patches per patch point.
+_LEVEL_STATS = {1: (6.3, 0.9, 7.0, 0.5), 2: (10.6, 0.52, 8.8, 0.51), 3: (13.2, 0.22, 10.3, 0.45),
+                4: (14.6, 0.075, 11.0, 0.43)}
+
+
 def estimate_device_bytes(s: MeshSpec, length=7):
-    """Upper estimate of one mesh's device footprint (tables, buffers,
-    outputs) used to fit a sample into an HBM budget: 16 table entries per
-    refined vertex, patch points <= 25 % of the level-l vertices with 12
-    entries each, one patch per 2 patch points."""
+    """Estimate of one mesh's device footprint (tables in the slice layout
+    with 15 % padding, vertex / output buffers, patch tables and coords),
+    from the measured per-level statistics plus 5 %; used to fit a sample
+    into an HBM budget."""
     nc, nr = spec_rows(s)
-    npts = int(0.25 * nr)
+    epr, ppr, eppt, patches_pp = _LEVEL_STATS.get(s.level, _LEVEL_STATS[4])
+    npts = int(ppr * nr) + 480
     F, L = s.frames, length
-    tables = 8.4 * 16 * nr + 8 * nr + 8.4 * 12 * npts + 8 * npts
-    patches = (npts // 2) * (16 * 4 + 12) + 20 * s.ncoords
+    tables = 1.15 * 8 * (epr * nr + eppt * npts) + 12 * (nr + npts)
+    patches = int(patches_pp * npts) * (16 * 4 + 12) + 20 * s.ncoords
     buffers = 4 * F * L * (nc + npts + nr + 3 * s.ncoords)
-    return int(tables + patches + buffers)
+    return int(1.05 * (tables + patches + buffers))
 
 
 def sample_fitting(count, seed, budget_bytes, length=7, max_tries=64):
@@ -121,7 +129,7 @@ def compile_mesh(spec: MeshSpec, length=7, threads=1) -> HostMesh:
 class DeviceMesh:
     """One mesh's device data and its batch segments (see module docstring)."""
 
-    def __init__(self, h: HostMesh, stream=None):
+    def __init__(self, h: HostMesh, stream=None, own_outputs=True):
         import torch
         from .osd import HipStencilTable, _device_tensor
         s = h.spec
@@ -139,9 +147,11 @@ class DeviceMesh:
         self.vbuf = torch.empty((self.nctrl + self.npts) * F * L, dtype=torch.float32, device=dev)
         ctrl = np.ascontiguousarray(h.frames.transpose(1, 0, 2)).reshape(-1)
         self.vbuf[:ctrl.size].copy_(torch.from_numpy(ctrl))
-        self.refined = torch.empty(self.nrefine * F * L, dtype=torch.float32, device=dev)
-        self.out = [torch.empty(self.ncoords * F * L, dtype=torch.float32, device=dev)
-                    for _ in range(3)]
+        self.stage = 0
+        self.refined = None
+        self.out = None
+        if own_outputs:
+            self.bind_outputs(torch.empty(self.output_floats(), dtype=torch.float32, device=dev))
         self.refine_table = HipStencilTable(h.refine, stream)
         self.point_table = HipStencilTable(h.points, stream)
         self.coords = _device_tensor(h.coords)
@@ -149,13 +159,29 @@ class DeviceMesh:
         self.indices = _device_tensor(h.indices)
         self.params = _device_tensor(h.params)
 
+    def output_floats(self):
+        """Refined vertices + P, dPdu, dPdv for every frame."""
+        F, L = self.spec.frames, self.length
+        return (self.nrefine + 3 * self.ncoords) * F * L
+
+    def bind_outputs(self, region, stage=0):
+        """Places the outputs in `region` (a float tensor of output_floats()
+        elements, possibly a view of a shared arena) and tags the segments
+        with `stage`."""
+        F, L = self.spec.frames, self.length
+        nr, nc = self.nrefine * F * L, self.ncoords * F * L
+        self.refined = region[:nr]
+        self.out = [region[nr + k * nc:nr + (k + 1) * nc] for k in range(3)]
+        self.stage = int(stage)
+
     def stencil_segments(self):
         F, L = self.spec.frames, self.length
         v = F * L
         return [dict(table=self.refine_table, src=self.vbuf, src_v=v, src_f=L,
-                     dst=self.refined, dst_v=v, dst_f=L, frames=F),
+                     dst=self.refined, dst_v=v, dst_f=L, frames=F, stage=self.stage),
                 dict(table=self.point_table, src=self.vbuf, src_v=v, src_f=L,
-                     dst=self.vbuf, dst_off=self.nctrl * v, dst_v=v, dst_f=L, frames=F)]
+                     dst=self.vbuf, dst_off=self.nctrl * v, dst_v=v, dst_f=L, frames=F,
+                     stage=self.stage)]
 
     def patch_segment(self):
         F, L = self.spec.frames, self.length
@@ -163,11 +189,11 @@ class DeviceMesh:
         return dict(n=self.ncoords, coords=self.coords, arrays=self.arrays,
                     indices=self.indices, params=self.params, src=self.vbuf, src_v=v,
                     src_f=L, P=self.out[0], dPdu=self.out[1], dPdv=self.out[2], out_v=v,
-                    out_f=L, frames=F)
+                    out_f=L, frames=F, stage=self.stage)
 
     def device_bytes(self):
         t = self.refine_table.device_bytes() + self.point_table.device_bytes()
-        b = 4 * (self.vbuf.numel() + self.refined.numel() + sum(o.numel() for o in self.out))
+        b = 4 * (self.vbuf.numel() + self.output_floats())
         return t + b + 4 * (self.coords.numel() + self.arrays.numel() + self.indices.numel() +
                             self.params.numel())
 
@@ -189,16 +215,40 @@ class DeviceMesh:
         return once + F * per_frame
 
 
-def build_device_meshes(specs, length=7, stream=None, workers=8, log=None):
+def stage_outputs(dms, arena_floats):
+    """Assigns meshes, in order, to stages whose outputs fit one arena of
+    `arena_floats` floats (a mesh that fits nowhere alone raises), allocates
+    the arena and binds every mesh's outputs into it; meshes of different
+    stages share the arena.  Returns (arena, number of stages)."""
+    import torch
+    need = [d.output_floats() for d in dms]
+    if max(need) > arena_floats:
+        raise MemoryError("one mesh's outputs exceed the output arena")
+    stage, used, place = 0, 0, []
+    for n in need:
+        n64 = (n + 63) // 64 * 64
+        if used + n64 > arena_floats:
+            stage, used = stage + 1, 0
+        place.append((stage, used))
+        used += n64
+    peak = max(o + (n + 63) // 64 * 64 for (st, o), n in zip(place, need))
+    arena = torch.empty(peak, dtype=torch.float32, device="cuda")
+    for d, (st, o), n in zip(dms, place, need):
+        d.bind_outputs(arena[o:o + n], st)
+    return arena, stage + 1
+
+
+def build_device_meshes(specs, length=7, stream=None, workers=8, log=None, own_outputs=True):
     """Compiles the meshes on `workers` host threads (largest first) and
-    uploads each as it completes; returns DeviceMeshes in spec order."""
+    uploads each as it completes; returns DeviceMeshes in spec order (with
+    their own output buffers, or none for stage_outputs)."""
     order = sorted(specs, key=lambda s: -spec_rows(s)[1])
     done = {}
     with ThreadPoolExecutor(max_workers=max(1, int(workers))) as ex:
         futs = {ex.submit(compile_mesh, s, length): s for s in order}
         for k, fut in enumerate(as_completed(futs)):
             h = fut.result()
-            done[h.spec.index] = DeviceMesh(h, stream)
+            done[h.spec.index] = DeviceMesh(h, stream, own_outputs)
             h.release()
             del h
             if log and (k + 1) % 16 == 0:

@@ -165,7 +165,14 @@ def test_tuned_configuration_matches_plain(workload):
 def test_batch_knob_and_segment_errors(workload):
     length, hosts, dms, batch = workload
     assert batch.set("a_frames", 3) == _lib.ERR_ARGS
+    assert batch.set("a_group", 16) == _lib.ERR_ARGS
     assert batch.set("b_frames", 8) == _lib.ERR_ARGS
+    # a pair that has no kernel instance is refused at launch
+    cfg = batch.config()
+    assert batch.set("a_frames", 4) == 0 and batch.set("a_group", 8) == 0
+    with pytest.raises(RuntimeError):
+        batch.run(1)
+    assert batch.set("a_frames", cfg["a_frames"]) == 0 and batch.set("a_group", cfg["a_group"]) == 0
     assert batch.set("a_persistent", 2) == _lib.ERR_ARGS
     assert batch.set("nope", 1) == _lib.ERR_UNSUPPORTED
     info = batch.info()
@@ -183,3 +190,45 @@ def test_batch_knob_and_segment_errors(workload):
     seg["table"] = t3
     with pytest.raises(ValueError):
         Batch(length, [seg], [])
+
+
+def test_staged_outputs_share_an_arena():
+    """Meshes in several stages whose outputs share one arena (each output
+    still written once per launch): every stage equals the plain path bit
+    for bit right after it runs, and the arena after a graph launch equals
+    the arena after the plain path (the last writes of each region)."""
+    import torch
+    from opensubdiv_amd.workloads import stage_outputs
+    length = 7
+    old = _lib.load_hip().osdhip_config(b"table_csr_mirror", 0)
+    try:
+        hosts = [compile_mesh(s, length) for s in SPECS]
+        dms = [DeviceMesh(h, own_outputs=False) for h in hosts]
+    finally:
+        _lib.load_hip().osdhip_config(b"table_csr_mirror", old)
+    need = [d.output_floats() for d in dms]
+    arena, nstages = stage_outputs(dms, max(need) + 64)
+    assert nstages >= 3 and arena.numel() < sum(need)
+    batch = Batch(length, [s for d in dms for s in d.stencil_segments()],
+                  [d.patch_segment() for d in dms])
+    assert batch.num_stages() == nstages
+    for st in range(nstages):
+        members = [d for d in dms if d.stage == st]
+        assert members
+        arena.fill_(float("nan"))
+        batch.run_plain_stage(st)
+        want = outputs(members)
+        arena.fill_(float("nan"))
+        batch.run_stage(st)
+        same(want, outputs(members), f"stage {st} batched")
+    arena.fill_(float("nan"))
+    batch.run_plain()
+    torch.cuda.synchronize()
+    final_plain = arena.cpu().numpy().copy()
+    arena.fill_(float("nan"))
+    batch.launch()
+    torch.cuda.synchronize()
+    assert np.array_equal(arena.cpu().numpy().view(np.uint32), final_plain.view(np.uint32))
+    batch.close()
+    for h in hosts:
+        h.release()

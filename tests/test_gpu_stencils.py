@@ -217,7 +217,8 @@ def table_eval(hip, h, src, sd, outs, descs, start, end):
 
 # (table_layout, slice_group, stencil_kernel): record slices with gather
 # groups of 8 and 4, split-stream slices, and the CSR kernel on the mirror
-VARIANTS = {"rec8": (0, 8, 0), "rec4": (0, 4, 0), "split": (1, 8, 0), "csr": (0, 8, 1)}
+VARIANTS = {"rec8": (0, 8, 0), "rec4": (0, 4, 0), "rec16": (0, 16, 0), "split": (1, 8, 0),
+            "csr": (0, 8, 1)}
 
 
 @pytest.mark.parametrize("variant", sorted(VARIANTS))
@@ -263,6 +264,7 @@ def test_device_table_bit_exact(hip, variant, nw, L):
 def test_device_table_knob_errors(hip):
     assert hip.osdhip_config(b"table_layout", 2) == _lib.ERR_ARGS
     assert hip.osdhip_config(b"slice_group", 5) == _lib.ERR_ARGS
+    assert hip.osdhip_config(b"slice_group", 32) == _lib.ERR_ARGS
     assert hip.osdhip_config(b"no_such_knob", 0) == _lib.ERR_UNSUPPORTED
 
 
