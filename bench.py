@@ -401,7 +401,7 @@ def buffer_checksums(dms):
     chunk = 1 << 26
     out = []
     for d in dms:
-        bufs = [d.refined, d.vbuf[d.nctrl * d.spec.frames * d.length:]] + d.out
+        bufs = [d.refined, d.vbuf[d.nctrl * d.spec.frames * d.lp:]] + d.out
         sums = []
         for b in bufs:
             v = b.view(torch.int32)
@@ -522,7 +522,8 @@ def run_served(args, lib, device):
     dev_budget, host_budget = 32 << 30, 24 << 30
     log(f"[cfg4] {len(residents)} resident topologies in {build_s:.1f}s; budgets "
         f"{dev_budget >> 30} GiB device, {host_budget >> 30} GiB pinned host")
-    server = serve.Server(residents, L, dev_budget, host_budget, slots=args.slots)
+    server = serve.Server(residents, L, dev_budget, host_budget, slots=args.slots,
+                          max_epoch_points=int(args.epoch_points))
 
     def points(r):
         t = residents[r.topo]
@@ -540,7 +541,7 @@ def run_served(args, lib, device):
     rate = args.load * capacity
     reqs = serve.poisson_requests(args.requests, rate, len(residents), seed=7)
     keep = {r.rid for r in reqs[:16]}
-    server.epochs = []
+    server.epochs, server.epoch_points = [], []
     tr = time.perf_counter()
     done, results = server.run(reqs, keep=keep, max_seconds=600)
     wall = time.perf_counter() - tr
@@ -564,7 +565,8 @@ def run_served(args, lib, device):
                                "points (50/50); Poisson arrivals; continuous batching",
                    "topologies": len(residents), "floats_per_point": L, "device": device,
                    "device_budget_bytes": dev_budget, "host_budget_bytes": host_budget,
-                   "slots": args.slots, "build_s": build_s},
+                   "slots": args.slots, "max_epoch_points": int(args.epoch_points),
+                   "build_s": build_s},
         "served": {"requests": len(done), "rate_per_s": rate, "capacity_per_s": capacity,
                    "load": args.load, "p50_ms": float(np.percentile(lat, 50)),
                    "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
@@ -594,7 +596,7 @@ def batched_cpu_baseline(dms, specs, budget_s):
         topo = far.Topology(mesh)
         topo.refine_uniform(sp.level)
         table = topo.stencil_table()
-        vb = d.vbuf.cpu().numpy().reshape(-1, F, L)[:, 0, :].copy()
+        vb = np.ascontiguousarray(d.vbuf.cpu().numpy().reshape(-1, F, d.lp)[:, 0, :L])
         ctrl = np.ascontiguousarray(vb[:d.nctrl])
         dst = np.zeros((d.nrefine, L), np.float32)
         n = d.ncoords
@@ -649,8 +651,7 @@ def copy_probe(args):
     stream = torch.cuda.current_stream()
     wall, ms = time_launches(lambda: b.copy_(a), args.steps, args.warmup, stream)
     kms = float(np.median(ms))
-    out = torch.empty((), device="cuda")
-    _, rms = time_launches(lambda: torch.sum(a, out=out), args.steps, args.warmup, stream)
+    _, rms = time_launches(lambda: a.sum(), args.steps, args.warmup, stream)
     rkms = float(np.median(rms))
     print(json.dumps({"probe": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
                       "kernel_ms": kms, "GBps": 8 * n / (kms * 1e-3) / 1e9,
@@ -671,6 +672,8 @@ def main():
     ap.add_argument("--load", type=float, default=0.7,
                     help="cfg4: arrival rate as a fraction of the measured capacity")
     ap.add_argument("--slots", type=int, default=4, help="cfg4: epochs in flight")
+    ap.add_argument("--epoch-points", type=float, default=32e6,
+                    help="cfg4: cap on the points batched into one epoch")
     ap.add_argument("--meshes", type=int, default=256, help="cfg3: meshes in the batch")
     ap.add_argument("--seed", type=int, default=0, help="cfg3: first sample seed")
 

@@ -267,7 +267,9 @@ int osdhip_batch_run_plain(osdhip_batch *batch, void *stream);
 int osdhip_batch_run_plain_stage(osdhip_batch *batch, int stage, void *stream);
 /* Variant knobs: "a_frames" x "a_group" in {1x4, 1x8, 2x4, 4x2} (frames
  * per chunk, gather group; other pairs make launches return
- * OSDHIP_ERR_UNSUPPORTED), "a_item_slices" (64-row slices per work item),
+ * OSDHIP_ERR_UNSUPPORTED) or a_frames 0 (a slice's records stay in
+ * registers for all frames; slices wider than 16 entries take the chunked
+ * path), "a_item_slices" (64-row slices per work item),
  * "a_persistent" / "b_persistent" (0 = one block per item, 1 = fixed grid
  * claiming items from a device counter), "b_frames" in {1, 2, 4}.  The
  * graph is re-captured on the next launch. */

@@ -21,6 +21,8 @@
 
 using namespace osdhip;
 
+constexpr int kCounters = kNumXcd;      // claim counters per (stage, phase)
+
 struct BatchConfig {
     int aFrames = 2, aGroup = 4, aItemSlices = 8, aPersistent = 0;
     int bFrames = 1, bPersistent = 0;
@@ -40,7 +42,7 @@ struct osdhip_batch {
     BatchItem *itemsA = nullptr, *itemsB = nullptr;
     int nItemsA = 0, nItemsB = 0, itemsASlices = 0;
     std::vector<int> beginA, countA, beginB, countB;    // item ranges per stage
-    unsigned *counters = nullptr;          // [2 * stage] phase A, [2 * stage + 1] phase B
+    unsigned *counters = nullptr;          // kCounters per (stage, phase)
     int numCUs = 256;
     BatchConfig cfg;
     hipGraph_t graph = nullptr;
@@ -71,9 +73,32 @@ bool launchA_lc(int lc, const BatchArgs &a, int grid, hipStream_t st)
     }
 }
 
-// phase A variants: (frames per chunk, gather group)
+template <int LC>
+void launchRR(const BatchArgs &a, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL((k_batch_stencils_rr<LC>), dim3(grid), dim3(kBlock), 0, st, a);
+}
+
+bool launchRR_lc(int lc, const BatchArgs &a, int grid, hipStream_t st)
+{
+    switch (lc) {
+    case 1: launchRR<1>(a, grid, st); return true;
+    case 2: launchRR<2>(a, grid, st); return true;
+    case 3: launchRR<3>(a, grid, st); return true;
+    case 4: launchRR<4>(a, grid, st); return true;
+    case 5: launchRR<5>(a, grid, st); return true;
+    case 6: launchRR<6>(a, grid, st); return true;
+    case 7: launchRR<7>(a, grid, st); return true;
+    case 8: launchRR<8>(a, grid, st); return true;
+    default: return false;
+    }
+}
+
+// phase A variants: (frames per chunk, gather group); frames 0 = the
+// records of a slice stay in registers for all frames (group ignored)
 bool launchA_variant(int lc, int fc, int g, const BatchArgs &a, int grid, hipStream_t st)
 {
+    if (fc == 0) return launchRR_lc(lc, a, grid, st);
     if (fc == 1 && g == 4) return launchA_lc<1, 4>(lc, a, grid, st);
     if (fc == 1 && g == 8) return launchA_lc<1, 8>(lc, a, grid, st);
     if (fc == 2 && g == 4) return launchA_lc<2, 4>(lc, a, grid, st);
@@ -81,9 +106,15 @@ bool launchA_variant(int lc, int fc, int g, const BatchArgs &a, int grid, hipStr
     return false;
 }
 
+// phase B: the regular-interior kernel, then the general one (other coords);
+// both claim the same items, so the persistent counters of the second
+// launch are reset between them
 template <int LC, int FC>
 void launchB(const BatchArgs &a, int grid, hipStream_t st)
 {
+    hipLaunchKernelGGL((k_batch_patches_regular<LC, FC>), dim3(grid), dim3(kBlock), 0, st, a);
+    if (a.persistent)
+        hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, a.counter, kNumXcd);
     hipLaunchKernelGGL((k_batch_patches<LC, FC>), dim3(grid), dim3(kBlock), 0, st, a);
 }
 
@@ -113,7 +144,7 @@ bool launchB_variant(int lc, int fc, const BatchArgs &a, int grid, hipStream_t s
 
 bool validA(int fc, int g)
 {
-    return (fc == 1 && (g == 4 || g == 8)) || (fc == 2 && g == 4) || (fc == 4 && g == 2);
+    return fc == 0 || (fc == 1 && (g == 4 || g == 8)) || (fc == 2 && g == 4) || (fc == 4 && g == 2);
 }
 
 // (Re)builds the phase-A item list for cfg.aItemSlices.
@@ -156,11 +187,13 @@ int buildItemsA(osdhip_batch *b)
     return OSDHIP_OK;
 }
 
+// Static: one block per item, rounded up to whole XCD groups (the block ->
+// item map needs a multiple of 8).  Persistent: enough blocks to fill every
+// CU a few times over.
 int gridFor(const osdhip_batch *b, int nitems, bool persistent)
 {
-    if (!persistent) return nitems;
-    // persistent: enough blocks to fill every CU a few times over
-    return std::max(1, std::min(nitems, b->numCUs * 8));
+    if (!persistent) return grid_for_blocks(nitems);
+    return grid_for_blocks(std::max(1, std::min(nitems, b->numCUs * 8)));
 }
 
 int enqueuePhaseA(osdhip_batch *b, int stage, hipStream_t st)
@@ -173,7 +206,7 @@ int enqueuePhaseA(osdhip_batch *b, int stage, hipStream_t st)
     a.items = b->itemsA + b->beginA[stage];
     a.nitems = n;
     a.persistent = b->cfg.aPersistent;
-    a.counter = b->counters + 2 * stage;
+    a.counter = b->counters + kCounters * (2 * stage);
     if (!launchA_variant(b->length, b->cfg.aFrames, b->cfg.aGroup, a,
                          gridFor(b, n, a.persistent != 0), st))
         return OSDHIP_ERR_UNSUPPORTED;
@@ -190,7 +223,7 @@ int enqueuePhaseB(osdhip_batch *b, int stage, hipStream_t st)
     a.items = b->itemsB + b->beginB[stage];
     a.nitems = n;
     a.persistent = b->cfg.bPersistent;
-    a.counter = b->counters + 2 * stage + 1;
+    a.counter = b->counters + kCounters * (2 * stage + 1);
     if (!launchB_variant(b->length, b->cfg.bFrames, a, gridFor(b, n, a.persistent != 0), st))
         return OSDHIP_ERR_UNSUPPORTED;
     return last_error();
@@ -201,7 +234,7 @@ int enqueueAll(osdhip_batch *b, int phases, hipStream_t st, int stage = -1)
 {
     int rc = OSDHIP_OK;
     if ((b->cfg.aPersistent && (phases & 1)) || (b->cfg.bPersistent && (phases & 2))) {
-        const int nc = 2 * b->nStages;
+        const int nc = kCounters * 2 * b->nStages;
         hipLaunchKernelGGL(k_batch_reset, dim3((nc + 63) / 64), dim3(64), 0, st, b->counters, nc);
         rc = last_error();
     }
@@ -359,7 +392,8 @@ osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
         b->countB[st] = (int)itemsB.size() - b->beginB[st];
     }
     b->nItemsB = (int)itemsB.size();
-    bool ok = hipMalloc(&b->counters, 2 * (size_t)b->nStages * sizeof(unsigned)) == hipSuccess;
+    bool ok = hipMalloc(&b->counters, kCounters * 2 * (size_t)b->nStages * sizeof(unsigned)) ==
+              hipSuccess;
     if (ok && b->nS)
         ok = hipMalloc(&b->dS, b->hS.size() * sizeof(BatchStencilSeg)) == hipSuccess &&
              hipMemcpy(b->dS, b->hS.data(), b->hS.size() * sizeof(BatchStencilSeg),
@@ -425,24 +459,23 @@ int osdhip_batch_run_plain_stage(osdhip_batch *b, int stage, void *stream)
 {
     if (!b || stage < 0 || stage >= b->nStages) return OSDHIP_ERR_ARGS;
     const int L = b->length;
-    // Osd buffer descriptors hold int offsets and strides
+    // Osd buffer descriptors hold int strides; each frame is passed as its
+    // own buffer (base pointer of the frame), as a caller with one buffer per
+    // frame would
     const long long lim = 0x7fffffffLL;
     for (const BatchStencilSeg &g : b->hS)
-        if (g.srcV > lim || g.dstV > lim || (g.frames - 1) * g.srcF > lim || (g.frames - 1) * g.dstF > lim)
-            return OSDHIP_ERR_UNSUPPORTED;
+        if (g.srcV > lim || g.dstV > lim) return OSDHIP_ERR_UNSUPPORTED;
     for (const BatchPatchSeg &g : b->hP)
-        if (g.srcV > lim || g.outV > lim || (g.frames - 1) * g.srcF > lim || (g.frames - 1) * g.outF > lim)
-            return OSDHIP_ERR_UNSUPPORTED;
+        if (g.srcV > lim || g.outV > lim) return OSDHIP_ERR_UNSUPPORTED;
     for (int i = 0; i < b->nS; ++i) {
         const BatchStencilSeg &g = b->hS[i];
         if (b->sStage[i] != stage) continue;
         for (int f = 0; f < g.frames; ++f) {
-            // frame f as an Osd buffer: offset f * frameStride, stride vertexStride
-            osdhip_buffer_desc sd = { (int)(f * g.srcF), L, (int)g.srcV };
-            osdhip_buffer_desc dd = { (int)(f * g.dstF), L, (int)g.dstV };
-            float *outs[1] = { g.dst };
-            int rc = osdhip_stencil_table_eval(b->tables[i], g.src, sd, 1, outs, &dd, 0,
-                                               g.nrows, stream);
+            osdhip_buffer_desc sd = { 0, L, (int)g.srcV };
+            osdhip_buffer_desc dd = { 0, L, (int)g.dstV };
+            float *outs[1] = { g.dst + f * g.dstF };
+            int rc = osdhip_stencil_table_eval(b->tables[i], g.src + f * g.srcF, sd, 1, outs, &dd,
+                                               0, g.nrows, stream);
             if (rc != OSDHIP_OK) return rc;
         }
     }
@@ -450,13 +483,15 @@ int osdhip_batch_run_plain_stage(osdhip_batch *b, int stage, void *stream)
         const BatchPatchSeg &g = b->hP[i];
         if (b->pStage[i] != stage) continue;
         for (int f = 0; f < g.frames; ++f) {
-            osdhip_buffer_desc sd = { (int)(f * g.srcF), L, (int)g.srcV };
-            osdhip_buffer_desc od = { (int)(f * g.outF), L, (int)g.outV };
+            osdhip_buffer_desc sd = { 0, L, (int)g.srcV };
+            osdhip_buffer_desc od = { 0, L, (int)g.outV };
             osdhip_buffer_desc z = { 0, 0, 0 };
-            int rc = osdhip_eval_patches(g.src, sd, g.out[0], g.out[0] ? od : z, g.out[1],
-                                         g.out[1] ? od : z, g.out[2], g.out[2] ? od : z, nullptr,
-                                         z, nullptr, z, nullptr, z, g.ncoords, g.coords, g.arrays,
-                                         g.indices, g.params, stream);
+            float *o[3];
+            for (int k = 0; k < 3; ++k) o[k] = g.out[k] ? g.out[k] + f * g.outF : nullptr;
+            int rc = osdhip_eval_patches(g.src + f * g.srcF, sd, o[0], o[0] ? od : z, o[1],
+                                         o[1] ? od : z, o[2], o[2] ? od : z, nullptr, z, nullptr,
+                                         z, nullptr, z, g.ncoords, g.coords, g.arrays, g.indices,
+                                         g.params, stream);
             if (rc != OSDHIP_OK) return rc;
         }
     }
@@ -476,7 +511,7 @@ int osdhip_batch_set(osdhip_batch *b, const char *key, int value)
     else return OSDHIP_ERR_UNSUPPORTED;
     // each knob within its own domain; the (frames, group) pair is checked
     // when the phase is launched
-    if ((c.aFrames != 1 && c.aFrames != 2 && c.aFrames != 4) ||
+    if ((c.aFrames != 0 && c.aFrames != 1 && c.aFrames != 2 && c.aFrames != 4) ||
         (c.aGroup != 2 && c.aGroup != 4 && c.aGroup != 8) || c.aItemSlices < 1 ||
         c.aItemSlices > 1024 ||
         (c.aPersistent != 0 && c.aPersistent != 1) ||
@@ -541,7 +576,7 @@ int osdhip_batch_tune(osdhip_batch *b, int reps, void *stream, char *report, int
     char buf[160];
     // phase A: (frames per chunk, gather group) x item slices x scheduling
     if (b->nS) {
-        const int fg[4][2] = { { 1, 4 }, { 1, 8 }, { 2, 4 }, { 4, 2 } };
+        const int fg[5][2] = { { 0, 4 }, { 1, 4 }, { 1, 8 }, { 2, 4 }, { 4, 2 } };
         const int slices[3] = { 4, 16, 64 };
         BatchConfig bestc = b->cfg;
         float bestms = 1e30f;
@@ -738,7 +773,7 @@ int osdhip_batch_epoch(int length, int numStencilSegments, const osdhip_batch_st
         a.sseg = (const BatchStencilSeg *)(d + p.segA);
         a.items = (const BatchItem *)(d + p.itemsA);
         a.nitems = na;
-        if (!launchA_variant(length, 1, 8, a, na, st)) return OSDHIP_ERR_UNSUPPORTED;
+        if (!launchA_variant(length, 1, 8, a, grid_for_blocks(na), st)) return OSDHIP_ERR_UNSUPPORTED;
         int rc = last_error();
         if (rc != OSDHIP_OK) return rc;
     }
@@ -747,7 +782,7 @@ int osdhip_batch_epoch(int length, int numStencilSegments, const osdhip_batch_st
         a.pseg = (const BatchPatchSeg *)(d + p.segB);
         a.items = (const BatchItem *)(d + p.itemsB);
         a.nitems = nb;
-        if (!launchB_variant(length, 1, a, nb, st)) return OSDHIP_ERR_UNSUPPORTED;
+        if (!launchB_variant(length, 1, a, grid_for_blocks(nb), st)) return OSDHIP_ERR_UNSUPPORTED;
         int rc = last_error();
         if (rc != OSDHIP_OK) return rc;
     }

@@ -67,7 +67,7 @@ struct BatchArgs {
     const BatchItem *items;
     int nitems;
     int persistent;
-    unsigned *counter;            // persistent: next item
+    unsigned *counter;            // persistent: one claim counter per XCD queue
 };
 
 __global__ void k_batch_reset(unsigned *counters, int n)
@@ -76,18 +76,41 @@ __global__ void k_batch_reset(unsigned *counters, int n)
     if (i < n) counters[i] = 0u;
 }
 
-// Next item of this block: static -> blockIdx.x once; persistent -> claimed
-// from the counter.  Returns -1 when done.  Must be reached by every thread
-// of the block (it contains __syncthreads).
+// Next item of this block.  Items are split into kNumXcd contiguous ranges,
+// one per XCD (the hardware places block b on XCD b % 8), so neighbouring
+// slices -- which share control points -- are processed by one XCD and
+// share its L2.  Static: block b takes item xcd_logical_block() once.
+// Persistent: block b claims items from its XCD's range through that
+// range's counter and, when the range is exhausted, from the other ranges in
+// turn; it returns -1 once every range is exhausted.  Must be reached by
+// every thread of the block (it contains __syncthreads).
 __device__ __forceinline__ int batch_next_item(const BatchArgs &a, int iter)
 {
-    if (!a.persistent) return iter == 0 && (int)blockIdx.x < a.nitems ? (int)blockIdx.x : -1;
-    __shared__ int slot;
+    if (!a.persistent) {
+        if (iter) return -1;
+        const int lb = xcd_logical_block();
+        return lb < a.nitems ? lb : -1;
+    }
+    __shared__ int slot, queue;
     __syncthreads();                          // previous item's readers are done
-    if (threadIdx.x == 0) slot = (int)atomicAdd(a.counter, 1u);
+    if (threadIdx.x == 0) {
+        if (iter == 0) queue = (int)(blockIdx.x % kNumXcd);
+        const int per = (a.nitems + kNumXcd - 1) / kNumXcd;
+        int it = -1;
+        for (int k = 0; k < kNumXcd && it < 0; ++k) {
+            const int q = (queue + k) % kNumXcd;
+            const int lo = q * per, n = min(a.nitems, lo + per) - lo;
+            if (n <= 0) continue;
+            const int c = (int)atomicAdd(a.counter + q, 1u);
+            if (c < n) {
+                it = lo + c;
+                queue = q;
+            }
+        }
+        slot = it;
+    }
     __syncthreads();
-    const int it = slot;
-    return it < a.nitems ? it : -1;
+    return slot;
 }
 
 template <int LC, int FC, int G>
@@ -165,6 +188,218 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils(BatchArgs a)
     }
 }
 
+// Phase A with the slice's records held in registers for all frames: a
+// slice of width W <= 16 loads its (index, weight) records once, then every
+// frame gathers and accumulates from registers, so the table leaves HBM
+// exactly once per step whatever the frame count.  Wider slices take the
+// chunked path (one frame per chunk, gather group 4).
+template <int LC>
+__global__ __launch_bounds__(kBlock) void k_batch_stencils_rr(BatchArgs a)
+{
+    constexpr int MAXW = 16, GG = 4;
+    const int lane = (int)threadIdx.x & (kWave - 1);
+    const int wv = (int)threadIdx.x / kWave;
+    for (int iter = 0;; ++iter) {
+        const int itx = batch_next_item(a, iter);
+        if (itx < 0) break;
+        const BatchItem item = a.items[itx];
+        const BatchStencilSeg sg = a.sseg[item.seg];
+        for (int sl = wv; sl < item.count; sl += kWavesPerBlock) {
+            const int s = item.first + sl;
+            const int row = s * kWave + lane;
+            const bool active = row < sg.nrows;
+            const int sz = sg.rowSize[row];
+            const int W = sg.sliceWidth[s];
+            const unsigned *rp = sg.rec + (size_t)(sg.sliceOff[s] + lane) * 2;
+            if (W <= MAXW) {
+                int ii[MAXW];
+                float ww[1][MAXW];
+#pragma unroll
+                for (int j = 0; j < MAXW; ++j) {
+                    ii[j] = 0;
+                    ww[0][j] = 0.0f;
+                    if (j < W)                                   // wave-uniform
+                        load_record<1, MAXW>(rp + (size_t)j * (kWave * 2), ii[j], ww, j);
+                }
+                for (int f = 0; f < sg.frames; ++f) {
+                    float acc[LC];
+#pragma unroll
+                    for (int k = 0; k < LC; ++k) acc[k] = 0.0f;
+                    const float *base = sg.src + (size_t)f * sg.srcF;
+#pragma unroll
+                    for (int j0 = 0; j0 < MAXW; j0 += GG) {
+                        if (j0 < W) {                            // wave-uniform
+                            float v[GG][LC];
+#pragma unroll
+                            for (int u = 0; u < GG; ++u) {
+                                const float *p = base + (size_t)(unsigned)ii[j0 + u] * sg.srcV;
+#pragma unroll
+                                for (int k = 0; k < LC; ++k) v[u][k] = p[k];
+                            }
+#pragma unroll
+                            for (int u = 0; u < GG; ++u) {
+                                const bool take = j0 + u < sz;
+#pragma unroll
+                                for (int k = 0; k < LC; ++k) {
+                                    const float t = acc[k] + v[u][k] * ww[0][j0 + u];
+                                    acc[k] = take ? t : acc[k];
+                                }
+                            }
+                        }
+                    }
+                    if (active) {
+                        float *d = sg.dst + (size_t)row * sg.dstV + (size_t)f * sg.dstF;
+#pragma unroll
+                        for (int k = 0; k < LC; ++k) d[k] = acc[k];
+                    }
+                }
+                continue;
+            }
+            for (int f = 0; f < sg.frames; ++f) {
+                float acc[LC];
+#pragma unroll
+                for (int k = 0; k < LC; ++k) acc[k] = 0.0f;
+                for (int j = 0; j < W; j += GG) {
+                    int ii[GG];
+                    float ww[1][GG];
+#pragma unroll
+                    for (int u = 0; u < GG; ++u) {
+                        ii[u] = 0;
+                        ww[0][u] = 0.0f;
+                        if (j + u < W)
+                            load_record<1, GG>(rp + (size_t)(j + u) * (kWave * 2), ii[u], ww, u);
+                    }
+                    float v[GG][LC];
+#pragma unroll
+                    for (int u = 0; u < GG; ++u) {
+                        const float *p = sg.src + (size_t)(unsigned)ii[u] * sg.srcV +
+                                         (size_t)f * sg.srcF;
+#pragma unroll
+                        for (int k = 0; k < LC; ++k) v[u][k] = p[k];
+                    }
+#pragma unroll
+                    for (int u = 0; u < GG; ++u) {
+                        const bool take = j + u < sz;
+#pragma unroll
+                        for (int k = 0; k < LC; ++k) {
+                            const float t = acc[k] + v[u][k] * ww[0][u];
+                            acc[k] = take ? t : acc[k];
+                        }
+                    }
+                }
+                if (active) {
+                    float *d = sg.dst + (size_t)row * sg.dstV + (size_t)f * sg.dstF;
+#pragma unroll
+                    for (int k = 0; k < LC; ++k) d[k] = acc[k];
+                }
+            }
+        }
+    }
+}
+
+// acc[o][q][k] += x * w_o[j] for frames q of a chunk, reference order.
+template <int LC, int FC>
+__device__ __forceinline__ void batch_patch_point(const float *p, long long fstride, int nq,
+                                                  float wp, float wu, float wv,
+                                                  float (&acc)[3][FC][LC])
+{
+#pragma unroll
+    for (int q = 0; q < FC; ++q) {
+        if (q < nq) {                                   // uniform per segment
+#pragma unroll
+            for (int k = 0; k < LC; ++k) {
+                const float x = p[q * fstride + k];
+                acc[0][q][k] += x * wp;
+                acc[1][q][k] += x * wu;
+                acc[2][q][k] += x * wv;
+            }
+        }
+    }
+}
+
+// Phase B in two kernels over the same items, each with its own register
+// budget:
+//   k_batch_patches_regular  regular B-spline patches without boundary (nearly
+//                            all coords): the 16 weights are formed in-line
+//                            from the separable curves -- the same single
+//                            products basis_bspline stores -- one row of 4
+//                            points at a time, so few gathers are live;
+//   k_batch_patches          every other coord (Gregory, boundary, bilinear),
+//                            skipping the regular-interior ones.
+__device__ __forceinline__ float pick4(const float (&v)[4], int r)
+{
+    return r == 0 ? v[0] : r == 1 ? v[1] : r == 2 ? v[2] : v[3];
+}
+
+template <int LC, int FC>
+__global__ __launch_bounds__(kBlock) void k_batch_patches_regular(BatchArgs a)
+{
+    for (int iter = 0;; ++iter) {
+        const int itx = batch_next_item(a, iter);
+        if (itx < 0) break;
+        const BatchItem item = a.items[itx];
+        const BatchPatchSeg sg = a.pseg[item.seg];
+        const int i = item.first + (int)threadIdx.x;
+        if (i >= item.first + item.count) continue;
+
+        const osdhip_patch_coord c = sg.coords[i];
+        const osdhip_patch_array arr = sg.arrays[c.arrayIndex];
+        const osdhip_patch_param pp = sg.params[c.patchIndex];
+        const unsigned f1 = pp.field1;
+        const int type = ((f1 >> 5) & 1u) ? arr.regDesc : arr.desc;
+        const int boundary = (int)((f1 >> 7) & 0x1fu);
+        if (type != OSDHIP_PATCH_REGULAR || boundary != 0) continue;
+        const int depth = (int)(f1 & 0xfu);
+        const int nonquad = (int)((f1 >> 4) & 1u);
+        const int pv = (int)((f1 >> 12) & 0x3ffu);
+        const int pu = (int)((f1 >> 22) & 0x3ffu);
+        const float frac_inv = (float)(1 << (depth - nonquad));
+        const float s = c.s * frac_inv - (float)pu;
+        const float t = c.t * frac_inv - (float)pv;
+        const int *cvs = sg.indices + arr.indexBase + arr.stride * (c.patchIndex - arr.primitiveIdBase);
+        Curve cs, ct;
+        bspline_curve(s, cs);
+        bspline_curve(t, ct);
+        const float k1 = (float)(1 << depth);
+
+        for (int f0 = 0; f0 < sg.frames; f0 += FC) {
+            const int nq = min(FC, sg.frames - f0);
+            float acc[3][FC][LC];
+#pragma unroll
+            for (int o = 0; o < 3; ++o)
+#pragma unroll
+                for (int q = 0; q < FC; ++q)
+#pragma unroll
+                    for (int k = 0; k < LC; ++k) acc[o][q][k] = 0.0f;
+            const float *base = sg.src + (size_t)f0 * sg.srcF;
+#pragma unroll 1
+            for (int r = 0; r < 4; ++r) {                 // point j = 4 r + q
+                const float tb = pick4(ct.b, r), td = pick4(ct.d, r);
+                int idx[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) idx[q] = cvs[4 * r + q];
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    batch_patch_point<LC, FC>(base + (size_t)(unsigned)idx[q] * sg.srcV, sg.srcF,
+                                              nq, cs.b[q] * tb, cs.d[q] * tb, cs.b[q] * td, acc);
+            }
+#pragma unroll
+            for (int q = 0; q < FC; ++q) {
+                if (q >= nq) break;
+#pragma unroll
+                for (int o = 0; o < 3; ++o) {
+                    float *d = sg.out[o];
+                    if (!d) continue;
+                    const float f = o == 0 ? 1.0f : k1;
+                    d += (size_t)i * sg.outV + (size_t)(f0 + q) * sg.outF;
+#pragma unroll
+                    for (int k = 0; k < LC; ++k) d[k] = acc[o][q][k] * f;
+                }
+            }
+        }
+    }
+}
+
 template <int LC, int FC>
 __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
 {
@@ -184,6 +419,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
         const int depth = (int)(f1 & 0xfu);
         const int nonquad = (int)((f1 >> 4) & 1u);
         const int boundary = (int)((f1 >> 7) & 0x1fu);
+        if (type == OSDHIP_PATCH_REGULAR && boundary == 0) continue;   // the regular kernel's
         const int pv = (int)((f1 >> 12) & 0x3ffu);
         const int pu = (int)((f1 >> 22) & 0x3ffu);
         const float frac_inv = (float)(1 << (depth - nonquad));
@@ -197,19 +433,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
         if (n == 16) basis_bspline<true, false>(s, t, boundary, B);
         else if (n == 20) basis_gregory<true, false>(s, t, B);
         else if (n == 4) basis_linear<true, false>(s, t, B);
-        else {
-#pragma unroll
-            for (int j = 0; j < 20; ++j) B.p[j] = B.ds[j] = B.dt[j] = 0.0f;
-        }
-        // points 0..15 are gathered unconditionally (index 0 past the
-        // patch's own points) and accumulated only where j < n, so the
-        // gathers issue without branches; 16..19 belong to Gregory patches
         int cv[20];
 #pragma unroll
         for (int j = 0; j < 20; ++j) cv[j] = j < n ? cvs[j] : 0;
         const float k1 = (float)(1 << depth);
 
         for (int f0 = 0; f0 < sg.frames; f0 += FC) {
+            const int nq = min(FC, sg.frames - f0);
             float acc[3][FC][LC];
 #pragma unroll
             for (int o = 0; o < 3; ++o)
@@ -217,34 +447,29 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
                 for (int q = 0; q < FC; ++q)
 #pragma unroll
                     for (int k = 0; k < LC; ++k) acc[o][q][k] = 0.0f;
-            auto point = [&](int j) {
-                const float *p = sg.src + (size_t)(unsigned)cv[j] * sg.srcV + (size_t)f0 * sg.srcF;
-                const bool take = j < n;
+            const float *base = sg.src + (size_t)f0 * sg.srcF;
+            // the point count follows from the type: 16 / 20 points without
+            // per-point predicates, bilinear separately (reference order)
+            if (n >= 16) {
 #pragma unroll
-                for (int q = 0; q < FC; ++q) {
-                    if (f0 + q < sg.frames) {                   // uniform per segment
+                for (int j = 0; j < 16; ++j)
+                    batch_patch_point<LC, FC>(base + (size_t)(unsigned)cv[j] * sg.srcV, sg.srcF,
+                                              nq, B.p[j], B.ds[j], B.dt[j], acc);
+                if (n == 20) {
 #pragma unroll
-                        for (int k = 0; k < LC; ++k) {
-                            const float x = p[q * sg.srcF + k];
-                            const float a0 = acc[0][q][k] + x * B.p[j];
-                            const float a1 = acc[1][q][k] + x * B.ds[j];
-                            const float a2 = acc[2][q][k] + x * B.dt[j];
-                            acc[0][q][k] = take ? a0 : acc[0][q][k];
-                            acc[1][q][k] = take ? a1 : acc[1][q][k];
-                            acc[2][q][k] = take ? a2 : acc[2][q][k];
-                        }
-                    }
+                    for (int j = 16; j < 20; ++j)
+                        batch_patch_point<LC, FC>(base + (size_t)(unsigned)cv[j] * sg.srcV,
+                                                  sg.srcF, nq, B.p[j], B.ds[j], B.dt[j], acc);
                 }
-            };
+            } else if (n == 4) {
 #pragma unroll
-            for (int j = 0; j < 16; ++j) point(j);
-            if (n > 16) {
-#pragma unroll
-                for (int j = 16; j < 20; ++j) point(j);
+                for (int j = 0; j < 4; ++j)
+                    batch_patch_point<LC, FC>(base + (size_t)(unsigned)cv[j] * sg.srcV, sg.srcF,
+                                              nq, B.p[j], B.ds[j], B.dt[j], acc);
             }
 #pragma unroll
             for (int q = 0; q < FC; ++q) {
-                if (f0 + q >= sg.frames) break;
+                if (q >= nq) break;
 #pragma unroll
                 for (int o = 0; o < 3; ++o) {
                     float *d = sg.out[o];

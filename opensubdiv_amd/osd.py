@@ -204,7 +204,8 @@ class HipPatchTable:
     PatchArray[], patch index buffer, PatchParam[] as CpuPatchTable lays them
     out (osd/cpuPatchTable.cpp:35-158)."""
 
-    def __init__(self, arrays, indices, params, varying_arrays=None, varying_indices=None):
+    def __init__(self, arrays, indices, params, varying_arrays=None, varying_indices=None,
+                 fvar_channels=()):
         self.arrays = _device_tensor(np.asarray(arrays).astype(PATCH_ARRAY_DTYPE, copy=False))
         self.indices = _device_tensor(np.asarray(indices, np.int32))
         self.params = _device_tensor(np.asarray(params).astype(PATCH_PARAM_DTYPE, copy=False))
@@ -212,11 +213,17 @@ class HipPatchTable:
             np.asarray(varying_arrays).astype(PATCH_ARRAY_DTYPE, copy=False))
         self.varying_indices = None if varying_indices is None else _device_tensor(
             np.asarray(varying_indices, np.int32))
+        # face-varying channels: (arrays, indices, params) per channel
+        self.fvar = [(_device_tensor(np.asarray(a).astype(PATCH_ARRAY_DTYPE, copy=False)),
+                      _device_tensor(np.asarray(i, np.int32)),
+                      _device_tensor(np.asarray(p).astype(PATCH_PARAM_DTYPE, copy=False)))
+                     for a, i, p in fvar_channels]
 
     @classmethod
     def Create(cls, flat, deviceContext=None):
         return cls(flat["arrays"], flat["indices"], flat["params"],
-                   flat.get("varying_arrays"), flat.get("varying_indices"))
+                   flat.get("varying_arrays"), flat.get("varying_indices"),
+                   flat.get("fvar_channels", ()))
 
     def GetPatchArrayBuffer(self): return self.arrays.data_ptr()
     def GetPatchIndexBuffer(self): return self.indices.data_ptr()
@@ -225,7 +232,10 @@ class HipPatchTable:
         return None if self.varying_arrays is None else self.varying_arrays.data_ptr()
     def GetVaryingPatchIndexBuffer(self):
         return None if self.varying_indices is None else self.varying_indices.data_ptr()
-    def GetNumFVarChannels(self): return 0
+    def GetNumFVarChannels(self): return len(self.fvar)
+    def GetFVarPatchArrayBuffer(self, fvarChannel=0): return self.fvar[fvarChannel][0].data_ptr()
+    def GetFVarPatchIndexBuffer(self, fvarChannel=0): return self.fvar[fvarChannel][1].data_ptr()
+    def GetFVarPatchParamBuffer(self, fvarChannel=0): return self.fvar[fvarChannel][2].data_ptr()
 
 
 class HipEvaluator:
@@ -308,6 +318,27 @@ class HipEvaluator:
                                      table.GetVaryingPatchArrayBuffer(),
                                      table.GetVaryingPatchIndexBuffer(),
                                      table.GetPatchParamBuffer(), ctx)
+
+    @staticmethod
+    def EvalPatchesFaceVarying(srcBuffer, srcDesc, dstBuffer, dstDesc, *rest):
+        """Face-varying evaluation through channel `fvarChannel`'s arrays,
+        indices and params (osd/cudaEvaluator.h:1050-1253):
+        EvalPatchesFaceVarying(src, srcDesc, dst, dstDesc, [du, duDesc, dv,
+        dvDesc, [...]] numPatchCoords, patchCoords, patchTable, fvarChannel
+        [, instance, ctx])"""
+        outs = [(dstBuffer, dstDesc)]
+        rest = list(rest)
+        while rest and not isinstance(rest[0], int):
+            outs.append((rest[0], rest[1]))
+            rest = rest[2:]
+        n, coords, table, ch = rest[0], rest[1], rest[2], rest[3]
+        ctx = rest[5] if len(rest) > 5 else None
+        if ch < 0 or ch >= table.GetNumFVarChannels():
+            return False
+        return HipEvaluator._patches(srcBuffer, srcDesc, outs, n, coords,
+                                     table.GetFVarPatchArrayBuffer(ch),
+                                     table.GetFVarPatchIndexBuffer(ch),
+                                     table.GetFVarPatchParamBuffer(ch), ctx)
 
     @staticmethod
     def Synchronize(deviceContext=None):

@@ -10,7 +10,8 @@ the patch table).  The server:
 * admits waiting requests in arrival order while their device buffers fit
   the device budget and their host result buffers fit the pinned host
   budget (first-fit arenas; no allocation per request);
-* batches everything admitted since the last launch into one epoch
+* batches the requests admitted since the last launch (up to a cap on the
+  points per epoch) into one epoch
   (osdhip_batch_epoch: phase A stencils, phase B patches) on a free stream
   slot, with the frame uploads before it and the result downloads after it
   on the same stream;
@@ -100,7 +101,8 @@ class Request:
 
 
 class Server:
-    def __init__(self, residents, length, device_budget, host_budget, slots=4, item_slices=8):
+    def __init__(self, residents, length, device_budget, host_budget, slots=4, item_slices=8,
+                 max_epoch_points=None):
         import torch
         self.torch = torch
         self.lib = _lib.load_hip()
@@ -111,7 +113,11 @@ class Server:
         self.host = torch.empty(int(host_budget) // 4, dtype=torch.float32, pin_memory=True)
         self.host_arena = Arena(self.host.numel(), 64)
         self.item_slices = int(item_slices)
+        # an epoch takes waiting requests (in order) until their points reach
+        # this cap, so one huge epoch does not hold every slot's successors
+        self.max_epoch_points = max_epoch_points
         self.epochs = []                         # requests per launched epoch
+        self.epoch_points = []                   # points per launched epoch
         self.slots = []
         for _ in range(int(slots)):
             self.slots.append({"stream": torch.cuda.Stream(), "event": torch.cuda.Event(),
@@ -126,6 +132,10 @@ class Server:
         if r.kind == "refine":
             return [t.nctrl * self.L, t.nrefine * self.L]
         return [(t.nctrl + t.npts) * self.L] + [t.ncoords * self.L] * 3
+
+    def points(self, r: Request):
+        t = self.res[r.topo]
+        return t.nrefine if r.kind == "refine" else t.ncoords
 
     def host_floats(self, r: Request):
         t = self.res[r.topo]
@@ -216,6 +226,7 @@ class Server:
             slot["event"].record(st)
         slot["reqs"] = reqs
         self.epochs.append(len(reqs))
+        self.epoch_points.append(sum(self.points(r) for r in reqs))
 
     def result(self, r: Request):
         """Host copy of a completed request's output (before release)."""
@@ -253,10 +264,13 @@ class Server:
             # admission + launch on a free slot
             free = [sl for sl in self.slots if sl["reqs"] is None]
             if waiting and free:
-                batch = []
-                while waiting and self.admit(waiting[0]):
+                batch, pts = [], 0
+                while waiting and (not batch or self.max_epoch_points is None or
+                                   pts + self.points(waiting[0]) <= self.max_epoch_points) \
+                        and self.admit(waiting[0]):
                     r = waiting.pop(0)
                     r.admitted = now
+                    pts += self.points(r)
                     batch.append(r)
                 if batch:
                     self.launch(free[0], batch)

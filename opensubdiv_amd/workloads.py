@@ -13,9 +13,13 @@ the feature-adaptive patch table of the same isolation level.
 patch-point stencils, and the coords (listed in patch order -- the same
 random set, ordered so neighbouring coords share control points).
 ``DeviceMesh`` uploads it: one vertex buffer [control | patch points] with
-frames interleaved per vertex (element (v, f, k) at (v * F + f) * L + k), a
-refinement output buffer and P / dPdu / dPdv buffers in the same layout,
-and the segments a ``Batch`` (or the one-mesh-per-launch path) evaluates.
+frames interleaved per vertex and each frame's record padded to a multiple
+of 4 floats (element (v, f, k) at (v * F + f) * Lp + k, Lp = 8 for L = 7:
+aligned 32-byte records, four frames of a vertex in one 128-byte line for
+the gathers), frame-major
+refinement and P / dPdu / dPdv outputs (element (r, f, k) at
+(f * rows + r) * L + k: a wave writes contiguous runs), and the segments a
+``Batch`` (or the one-mesh-per-launch path) evaluates.
 """
 from __future__ import annotations
 
@@ -135,6 +139,7 @@ class DeviceMesh:
         s = h.spec
         F, L = s.frames, h.length
         self.spec, self.length = s, L
+        self.lp = Lp = (L + 3) // 4 * 4
         self.nctrl = h.nctrl
         self.npts = int(len(h.points["sizes"]))
         self.nrefine = int(h.refine.num_stencils)
@@ -144,9 +149,10 @@ class DeviceMesh:
         self.npatches = int(len(h.params))
         self.npatch_indices = int(len(h.indices))
         dev = "cuda"
-        self.vbuf = torch.empty((self.nctrl + self.npts) * F * L, dtype=torch.float32, device=dev)
-        ctrl = np.ascontiguousarray(h.frames.transpose(1, 0, 2)).reshape(-1)
-        self.vbuf[:ctrl.size].copy_(torch.from_numpy(ctrl))
+        self.vbuf = torch.zeros((self.nctrl + self.npts) * F * Lp, dtype=torch.float32, device=dev)
+        ctrl = np.zeros((self.nctrl, F, Lp), np.float32)
+        ctrl[:, :, :L] = h.frames.transpose(1, 0, 2)
+        self.vbuf[:ctrl.size].copy_(torch.from_numpy(ctrl.reshape(-1)))
         self.stage = 0
         self.refined = None
         self.out = None
@@ -175,21 +181,22 @@ class DeviceMesh:
         self.stage = int(stage)
 
     def stencil_segments(self):
-        F, L = self.spec.frames, self.length
-        v = F * L
-        return [dict(table=self.refine_table, src=self.vbuf, src_v=v, src_f=L,
-                     dst=self.refined, dst_v=v, dst_f=L, frames=F, stage=self.stage),
-                dict(table=self.point_table, src=self.vbuf, src_v=v, src_f=L,
-                     dst=self.vbuf, dst_off=self.nctrl * v, dst_v=v, dst_f=L, frames=F,
+        F, L, Lp = self.spec.frames, self.length, self.lp
+        v = F * Lp
+        return [dict(table=self.refine_table, src=self.vbuf, src_v=v, src_f=Lp,
+                     dst=self.refined, dst_v=L, dst_f=self.nrefine * L, frames=F,
+                     stage=self.stage),
+                dict(table=self.point_table, src=self.vbuf, src_v=v, src_f=Lp,
+                     dst=self.vbuf, dst_off=self.nctrl * v, dst_v=v, dst_f=Lp, frames=F,
                      stage=self.stage)]
 
     def patch_segment(self):
-        F, L = self.spec.frames, self.length
-        v = F * L
+        F, L, Lp = self.spec.frames, self.length, self.lp
+        v = F * Lp
         return dict(n=self.ncoords, coords=self.coords, arrays=self.arrays,
                     indices=self.indices, params=self.params, src=self.vbuf, src_v=v,
-                    src_f=L, P=self.out[0], dPdu=self.out[1], dPdv=self.out[2], out_v=v,
-                    out_f=L, frames=F, stage=self.stage)
+                    src_f=Lp, P=self.out[0], dPdu=self.out[1], dPdv=self.out[2], out_v=L,
+                    out_f=self.ncoords * L, frames=F, stage=self.stage)
 
     def device_bytes(self):
         t = self.refine_table.device_bytes() + self.point_table.device_bytes()

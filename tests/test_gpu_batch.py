@@ -44,7 +44,7 @@ def outputs(dms):
     res = []
     for d in dms:
         res.append([d.refined.cpu().numpy().copy(),
-                    d.vbuf[d.nctrl * d.spec.frames * d.length:].cpu().numpy().copy()] +
+                    d.vbuf[d.nctrl * d.spec.frames * d.lp:].cpu().numpy().copy()] +
                    [o.cpu().numpy().copy() for o in d.out])
     return res
 
@@ -53,7 +53,7 @@ def poison(dms):
     import torch
     for d in dms:
         d.refined.fill_(float("nan"))
-        d.vbuf[d.nctrl * d.spec.frames * d.length:].fill_(float("nan"))
+        d.vbuf[d.nctrl * d.spec.frames * d.lp:].fill_(float("nan"))
         for o in d.out:
             o.fill_(float("nan"))
     torch.cuda.synchronize()
@@ -92,8 +92,8 @@ def test_plain_path_matches_oracle(workload):
         topo.refine_uniform(s.level)
         ref = topo.stencil_table()
         pts = h.points
-        refined = g[0].reshape(-1, F, L)
-        ptsout = g[1].reshape(-1, F, L)
+        refined = g[0].reshape(F, -1, L).transpose(1, 0, 2)      # frame-major outputs
+        ptsout = g[1].reshape(-1, F, d.lp)[:, :, :L]             # frames interleaved, padded
         for f in range(F):
             ctrl = np.ascontiguousarray(h.frames[f])
             want = oracle.apply_stencils(ctrl, ref["sizes"], ref["offsets"], ref["indices"],
@@ -108,7 +108,7 @@ def test_plain_path_matches_oracle(workload):
             assert oracle.eval_patches(src, (0, L, L), dst, [(0, L, L)] * 3, h.coords, h.arrays,
                                        h.indices, h.params)
             for k in range(3):
-                np.testing.assert_array_equal(g[2 + k].reshape(-1, F, L)[:, f], dst[k],
+                np.testing.assert_array_equal(g[2 + k].reshape(F, -1, L)[f], dst[k],
                                               err_msg=f"mesh {s.index} patch output {k} f{f}")
 
 
@@ -125,7 +125,8 @@ def test_graph_launch_matches_plain_and_replays(workload):
 
 
 VARIANTS = [dict(a_frames=1, a_group=4), dict(a_frames=1, a_group=8),
-            dict(a_frames=2, a_group=4), dict(a_frames=4, a_group=2)]
+            dict(a_frames=2, a_group=4), dict(a_frames=4, a_group=2),
+            dict(a_frames=0, a_group=4)]
 
 
 @pytest.mark.parametrize("variant", range(len(VARIANTS)))
@@ -156,7 +157,7 @@ def test_tuned_configuration_matches_plain(workload):
     batch.run_plain()
     plain = outputs(dms)
     report = batch.tune(reps=1)
-    assert len(report["phase_a"]) == 24 and len(report["phase_b"]) == 6
+    assert len(report["phase_a"]) == 30 and len(report["phase_b"]) == 6
     poison(dms)
     batch.launch()
     same(plain, outputs(dms), f"tuned {batch.config()}")

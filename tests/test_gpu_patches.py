@@ -176,6 +176,17 @@ def test_python_mirror_eval_patches(hip):
     assert HipEvaluator.Synchronize()
     np.testing.assert_array_equal(host(P, shape=(n, 3)), g["cpu_p"])
     np.testing.assert_array_equal(host(dv, shape=(n, 3)), g["cpu_dv"])
+    # a face-varying channel holding the same arrays evaluates identically;
+    # a channel that does not exist returns False
+    arrays = g["arrays"].view(PATCH_ARRAY_DTYPE).reshape(-1)
+    params = g["params"].view(PATCH_PARAM_DTYPE).reshape(-1)
+    ft = HipPatchTable(arrays, g["indices"], params, fvar_channels=[(arrays, g["indices"], params)])
+    assert ft.GetNumFVarChannels() == 1
+    P2 = empty(n * 3)
+    assert HipEvaluator.EvalPatchesFaceVarying(src, d, P2, d, n, coords, ft, 0)
+    assert HipEvaluator.Synchronize()
+    np.testing.assert_array_equal(host(P2, shape=(n, 3)), g["cpu_p"])
+    assert not HipEvaluator.EvalPatchesFaceVarying(src, d, P2, d, n, coords, ft, 1)
 
 
 @pytest.mark.parametrize("deriv", [0, 1, 2])

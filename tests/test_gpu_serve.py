@@ -107,3 +107,18 @@ def test_tight_budget_serialises_but_completes(residents):
         r.arrival = 0.0
         _, alone = server.run([r], keep={r.rid}, max_seconds=30)
         assert np.array_equal(alone[r.rid].view(np.uint32), res_t[r.rid].view(np.uint32))
+
+
+def test_epoch_cap_splits_the_burst_with_same_results(residents):
+    reqs = make_requests()
+    base = Server(residents, L, device_budget=64 << 20, host_budget=64 << 20, slots=3)
+    _, want = base.run(reqs, keep={r.rid for r in reqs}, max_seconds=60)
+    capped = Server(residents, L, device_budget=64 << 20, host_budget=64 << 20, slots=3,
+                    max_epoch_points=3000)
+    reqs2 = make_requests()
+    done, got = capped.run(reqs2, keep={r.rid for r in reqs2}, max_seconds=60)
+    assert len(done) == len(reqs2)
+    for n, pts in zip(capped.epochs, capped.epoch_points):
+        assert n == 1 or pts <= 3000
+    for r in reqs2:
+        assert np.array_equal(want[r.rid].view(np.uint32), got[r.rid].view(np.uint32))
