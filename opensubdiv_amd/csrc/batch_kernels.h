@@ -57,6 +57,110 @@ struct BatchPatchSeg {
     int ncoords, frames;
 };
 
+// Device views of the segments with global-qualified pointers (the
+// descriptors are read from memory, so the compiler would otherwise treat
+// their pointers as generic and emit flat accesses).
+#define G_ OSDHIP_GLOBAL
+struct GStencilSeg {
+    const G_ unsigned *rec;
+    const G_ long long *sliceOff;
+    const G_ int *sliceWidth;
+    const G_ int *rowSize;
+    const G_ float *src;
+    long long srcV, srcF;
+    G_ float *dst;
+    long long dstV, dstF;
+    int nrows, nslices, frames;
+};
+
+struct GPatchSeg {
+    const G_ osdhip_patch_coord *coords;
+    const G_ osdhip_patch_array *arrays;
+    const G_ int *indices;
+    const G_ osdhip_patch_param *params;
+    const G_ float *src;
+    long long srcV, srcF;
+    G_ float *out[3];
+    long long outV, outF;
+    int ncoords, frames;
+};
+
+__device__ __forceinline__ GStencilSeg gseg(const BatchStencilSeg &s)
+{
+    GStencilSeg g;
+    g.rec = gptr(s.rec);
+    g.sliceOff = gptr(s.sliceOff);
+    g.sliceWidth = gptr(s.sliceWidth);
+    g.rowSize = gptr(s.rowSize);
+    g.src = gptr(s.src);
+    g.srcV = s.srcV;
+    g.srcF = s.srcF;
+    g.dst = gptr_w(s.dst);
+    g.dstV = s.dstV;
+    g.dstF = s.dstF;
+    g.nrows = s.nrows;
+    g.nslices = s.nslices;
+    g.frames = s.frames;
+    return g;
+}
+
+__device__ __forceinline__ GPatchSeg gseg(const BatchPatchSeg &s)
+{
+    GPatchSeg g;
+    g.coords = gptr(s.coords);
+    g.arrays = gptr(s.arrays);
+    g.indices = gptr(s.indices);
+    g.params = gptr(s.params);
+    g.src = gptr(s.src);
+    g.srcV = s.srcV;
+    g.srcF = s.srcF;
+    for (int k = 0; k < 3; ++k) g.out[k] = s.out[k] ? gptr_w(s.out[k]) : nullptr;
+    g.outV = s.outV;
+    g.outF = s.outF;
+    g.ncoords = s.ncoords;
+    g.frames = s.frames;
+    return g;
+}
+
+// Field-wise loads of the Osd structs through global pointers.
+__device__ __forceinline__ osdhip_patch_coord ld(const G_ osdhip_patch_coord *p)
+{
+    osdhip_patch_coord c;
+    c.arrayIndex = p->arrayIndex;
+    c.patchIndex = p->patchIndex;
+    c.vertIndex = p->vertIndex;
+    c.s = p->s;
+    c.t = p->t;
+    return c;
+}
+__device__ __forceinline__ osdhip_patch_array ld(const G_ osdhip_patch_array *p)
+{
+    osdhip_patch_array a;
+    a.regDesc = p->regDesc;
+    a.desc = p->desc;
+    a.numPatches = p->numPatches;
+    a.indexBase = p->indexBase;
+    a.stride = p->stride;
+    a.primitiveIdBase = p->primitiveIdBase;
+    return a;
+}
+__device__ __forceinline__ osdhip_patch_param ld(const G_ osdhip_patch_param *p)
+{
+    osdhip_patch_param q;
+    q.field0 = p->field0;
+    q.field1 = p->field1;
+    q.sharpness = p->sharpness;
+    return q;
+}
+
+// One (index, weight) record from a global pointer (non-temporal).
+__device__ __forceinline__ void load_record_g(const OSDHIP_GLOBAL unsigned *p, int &idx, float &w)
+{
+    const v2i q = __builtin_nontemporal_load((const OSDHIP_GLOBAL v2i *)p);
+    idx = q.x;
+    w = __int_as_float(q.y);
+}
+
 struct BatchItem {
     int seg, first, count, pad;   // phase A: slices [first, first+count); B: coords
 };
@@ -122,14 +226,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils(BatchArgs a)
         const int itx = batch_next_item(a, iter);
         if (itx < 0) break;
         const BatchItem item = a.items[itx];
-        const BatchStencilSeg sg = a.sseg[item.seg];
+        const GStencilSeg sg = gseg(a.sseg[item.seg]);
         for (int sl = wv; sl < item.count; sl += kWavesPerBlock) {
             const int s = item.first + sl;
             const int row = s * kWave + lane;
             const bool active = row < sg.nrows;
             const int sz = sg.rowSize[row];
             const int W = sg.sliceWidth[s];
-            const unsigned *rp = sg.rec + (size_t)(sg.sliceOff[s] + lane) * 2;
+            const G_ unsigned *rp = sg.rec + (size_t)(sg.sliceOff[s] + lane) * 2;
             for (int f0 = 0; f0 < sg.frames; f0 += FC) {
                 float acc[FC][LC];
 #pragma unroll
@@ -144,13 +248,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils(BatchArgs a)
                         ii[u] = 0;
                         ww[0][u] = 0.0f;
                         if (j + u < W)                          // wave-uniform
-                            load_record<1, G>(rp + (size_t)(j + u) * (kWave * 2), ii[u], ww, u);
+                            load_record_g(rp + (size_t)(j + u) * (kWave * 2), ii[u], ww[0][u]);
                     }
                     float v[G][FC][LC];
 #pragma unroll
                     for (int u = 0; u < G; ++u) {
-                        const float *p = sg.src + (size_t)(unsigned)ii[u] * sg.srcV +
-                                         (size_t)f0 * sg.srcF;
+                        const G_ float *p = sg.src + (size_t)(unsigned)ii[u] * sg.srcV +
+                                                       (size_t)f0 * sg.srcF;
 #pragma unroll
                         for (int c = 0; c < FC; ++c) {
                             if (f0 + c < sg.frames) {           // wave-uniform
@@ -178,7 +282,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils(BatchArgs a)
 #pragma unroll
                     for (int c = 0; c < FC; ++c) {
                         if (f0 + c >= sg.frames) break;
-                        float *d = sg.dst + (size_t)row * sg.dstV + (size_t)(f0 + c) * sg.dstF;
+                        G_ float *d =
+                            sg.dst + (size_t)row * sg.dstV + (size_t)(f0 + c) * sg.dstF;
 #pragma unroll
                         for (int k = 0; k < LC; ++k) d[k] = acc[c][k];
                     }
@@ -203,14 +308,14 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils_rr(BatchArgs a)
         const int itx = batch_next_item(a, iter);
         if (itx < 0) break;
         const BatchItem item = a.items[itx];
-        const BatchStencilSeg sg = a.sseg[item.seg];
+        const GStencilSeg sg = gseg(a.sseg[item.seg]);
         for (int sl = wv; sl < item.count; sl += kWavesPerBlock) {
             const int s = item.first + sl;
             const int row = s * kWave + lane;
             const bool active = row < sg.nrows;
             const int sz = sg.rowSize[row];
             const int W = sg.sliceWidth[s];
-            const unsigned *rp = sg.rec + (size_t)(sg.sliceOff[s] + lane) * 2;
+            const G_ unsigned *rp = sg.rec + (size_t)(sg.sliceOff[s] + lane) * 2;
             if (W <= MAXW) {
                 int ii[MAXW];
                 float ww[1][MAXW];
@@ -219,20 +324,20 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils_rr(BatchArgs a)
                     ii[j] = 0;
                     ww[0][j] = 0.0f;
                     if (j < W)                                   // wave-uniform
-                        load_record<1, MAXW>(rp + (size_t)j * (kWave * 2), ii[j], ww, j);
+                        load_record_g(rp + (size_t)j * (kWave * 2), ii[j], ww[0][j]);
                 }
                 for (int f = 0; f < sg.frames; ++f) {
                     float acc[LC];
 #pragma unroll
                     for (int k = 0; k < LC; ++k) acc[k] = 0.0f;
-                    const float *base = sg.src + (size_t)f * sg.srcF;
+                    const G_ float *base = sg.src + (size_t)f * sg.srcF;
 #pragma unroll
                     for (int j0 = 0; j0 < MAXW; j0 += GG) {
                         if (j0 < W) {                            // wave-uniform
                             float v[GG][LC];
 #pragma unroll
                             for (int u = 0; u < GG; ++u) {
-                                const float *p = base + (size_t)(unsigned)ii[j0 + u] * sg.srcV;
+                                const G_ float *p = base + (size_t)(unsigned)ii[j0 + u] * sg.srcV;
 #pragma unroll
                                 for (int k = 0; k < LC; ++k) v[u][k] = p[k];
                             }
@@ -248,7 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils_rr(BatchArgs a)
                         }
                     }
                     if (active) {
-                        float *d = sg.dst + (size_t)row * sg.dstV + (size_t)f * sg.dstF;
+                        G_ float *d = sg.dst + (size_t)row * sg.dstV + (size_t)f * sg.dstF;
 #pragma unroll
                         for (int k = 0; k < LC; ++k) d[k] = acc[k];
                     }
@@ -267,12 +372,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils_rr(BatchArgs a)
                         ii[u] = 0;
                         ww[0][u] = 0.0f;
                         if (j + u < W)
-                            load_record<1, GG>(rp + (size_t)(j + u) * (kWave * 2), ii[u], ww, u);
+                            load_record_g(rp + (size_t)(j + u) * (kWave * 2), ii[u], ww[0][u]);
                     }
                     float v[GG][LC];
 #pragma unroll
                     for (int u = 0; u < GG; ++u) {
-                        const float *p = sg.src + (size_t)(unsigned)ii[u] * sg.srcV +
+                        const G_ float *p = sg.src + (size_t)(unsigned)ii[u] * sg.srcV +
                                          (size_t)f * sg.srcF;
 #pragma unroll
                         for (int k = 0; k < LC; ++k) v[u][k] = p[k];
@@ -288,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils_rr(BatchArgs a)
                     }
                 }
                 if (active) {
-                    float *d = sg.dst + (size_t)row * sg.dstV + (size_t)f * sg.dstF;
+                    G_ float *d = sg.dst + (size_t)row * sg.dstV + (size_t)f * sg.dstF;
 #pragma unroll
                     for (int k = 0; k < LC; ++k) d[k] = acc[k];
                 }
@@ -299,7 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils_rr(BatchArgs a)
 
 // acc[o][q][k] += x * w_o[j] for frames q of a chunk, reference order.
 template <int LC, int FC>
-__device__ __forceinline__ void batch_patch_point(const float *p, long long fstride, int nq,
+__device__ __forceinline__ void batch_patch_point(const G_ float *p, long long fstride, int nq,
                                                   float wp, float wu, float wv,
                                                   float (&acc)[3][FC][LC])
 {
@@ -338,13 +443,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches_regular(BatchArgs a)
         const int itx = batch_next_item(a, iter);
         if (itx < 0) break;
         const BatchItem item = a.items[itx];
-        const BatchPatchSeg sg = a.pseg[item.seg];
+        const GPatchSeg sg = gseg(a.pseg[item.seg]);
         const int i = item.first + (int)threadIdx.x;
         if (i >= item.first + item.count) continue;
 
-        const osdhip_patch_coord c = sg.coords[i];
-        const osdhip_patch_array arr = sg.arrays[c.arrayIndex];
-        const osdhip_patch_param pp = sg.params[c.patchIndex];
+        const osdhip_patch_coord c = ld(sg.coords + i);
+        const osdhip_patch_array arr = ld(sg.arrays + c.arrayIndex);
+        const osdhip_patch_param pp = ld(sg.params + c.patchIndex);
         const unsigned f1 = pp.field1;
         const int type = ((f1 >> 5) & 1u) ? arr.regDesc : arr.desc;
         const int boundary = (int)((f1 >> 7) & 0x1fu);
@@ -356,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches_regular(BatchArgs a)
         const float frac_inv = (float)(1 << (depth - nonquad));
         const float s = c.s * frac_inv - (float)pu;
         const float t = c.t * frac_inv - (float)pv;
-        const int *cvs = sg.indices + arr.indexBase + arr.stride * (c.patchIndex - arr.primitiveIdBase);
+        const G_ int *cvs = sg.indices + arr.indexBase + arr.stride * (c.patchIndex - arr.primitiveIdBase);
         Curve cs, ct;
         bspline_curve(s, cs);
         bspline_curve(t, ct);
@@ -371,7 +476,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches_regular(BatchArgs a)
                 for (int q = 0; q < FC; ++q)
 #pragma unroll
                     for (int k = 0; k < LC; ++k) acc[o][q][k] = 0.0f;
-            const float *base = sg.src + (size_t)f0 * sg.srcF;
+            const G_ float *base = sg.src + (size_t)f0 * sg.srcF;
 #pragma unroll 1
             for (int r = 0; r < 4; ++r) {                 // point j = 4 r + q
                 const float tb = pick4(ct.b, r), td = pick4(ct.d, r);
@@ -388,7 +493,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches_regular(BatchArgs a)
                 if (q >= nq) break;
 #pragma unroll
                 for (int o = 0; o < 3; ++o) {
-                    float *d = sg.out[o];
+                    G_ float *d = sg.out[o];
                     if (!d) continue;
                     const float f = o == 0 ? 1.0f : k1;
                     d += (size_t)i * sg.outV + (size_t)(f0 + q) * sg.outF;
@@ -407,13 +512,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
         const int itx = batch_next_item(a, iter);
         if (itx < 0) break;
         const BatchItem item = a.items[itx];
-        const BatchPatchSeg sg = a.pseg[item.seg];
+        const GPatchSeg sg = gseg(a.pseg[item.seg]);
         const int i = item.first + (int)threadIdx.x;
         if (i >= item.first + item.count) continue;
 
-        const osdhip_patch_coord c = sg.coords[i];
-        const osdhip_patch_array arr = sg.arrays[c.arrayIndex];
-        const osdhip_patch_param pp = sg.params[c.patchIndex];
+        const osdhip_patch_coord c = ld(sg.coords + i);
+        const osdhip_patch_array arr = ld(sg.arrays + c.arrayIndex);
+        const osdhip_patch_param pp = ld(sg.params + c.patchIndex);
         const unsigned f1 = pp.field1;
         const int type = ((f1 >> 5) & 1u) ? arr.regDesc : arr.desc;
         const int depth = (int)(f1 & 0xfu);
@@ -428,7 +533,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
         const int n = type == OSDHIP_PATCH_REGULAR ? 16
                     : type == OSDHIP_PATCH_GREGORY_BASIS ? 20
                     : type == OSDHIP_PATCH_QUADS ? 4 : 0;
-        const int *cvs = sg.indices + arr.indexBase + arr.stride * (c.patchIndex - arr.primitiveIdBase);
+        const G_ int *cvs = sg.indices + arr.indexBase + arr.stride * (c.patchIndex - arr.primitiveIdBase);
         Basis<true, false> B;
         if (n == 16) basis_bspline<true, false>(s, t, boundary, B);
         else if (n == 20) basis_gregory<true, false>(s, t, B);
@@ -447,7 +552,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
                 for (int q = 0; q < FC; ++q)
 #pragma unroll
                     for (int k = 0; k < LC; ++k) acc[o][q][k] = 0.0f;
-            const float *base = sg.src + (size_t)f0 * sg.srcF;
+            const G_ float *base = sg.src + (size_t)f0 * sg.srcF;
             // the point count follows from the type: 16 / 20 points without
             // per-point predicates, bilinear separately (reference order)
             if (n >= 16) {
@@ -472,7 +577,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
                 if (q >= nq) break;
 #pragma unroll
                 for (int o = 0; o < 3; ++o) {
-                    float *d = sg.out[o];
+                    G_ float *d = sg.out[o];
                     if (!d) continue;
                     const float f = o == 0 ? 1.0f : k1;
                     d += (size_t)i * sg.outV + (size_t)(f0 + q) * sg.outF;

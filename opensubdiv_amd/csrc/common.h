@@ -34,6 +34,21 @@ __device__ __forceinline__ int xcd_logical_block()
     return ((int)blockIdx.x % kNumXcd) * per + (int)blockIdx.x / kNumXcd;
 }
 
+// Pointers read from device memory (segment tables) are generic to the
+// compiler, which then emits flat loads that also count against lgkmcnt and
+// force conservative waits; these casts restore global-memory accesses.
+#define OSDHIP_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const OSDHIP_GLOBAL T *gptr(const T *p)
+{
+    return (const OSDHIP_GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ OSDHIP_GLOBAL T *gptr_w(T *p)
+{
+    return (OSDHIP_GLOBAL T *)p;
+}
+
 inline int grid_for_blocks(int nblocks)
 {
     return (nblocks + kNumXcd - 1) / kNumXcd * kNumXcd;

@@ -25,13 +25,14 @@ def residents():
 
 
 def make_requests():
-    # a burst: several requests per topology, both kinds, different frames
+    # several requests per topology, both kinds, different frames: the first
+    # six arrive together (one multi-request epoch), the rest staggered
     out = []
     k = 0
     for rep in range(3):
         for t in range(len(SPECS)):
             for kind in ("refine", "limit"):
-                out.append(Request(k, t, kind, 0.0005 * k, (k + rep) % 4))
+                out.append(Request(k, t, kind, 0.0 if k < 6 else 0.0002 * k, (k + rep) % 4))
                 k += 1
     return out
 
