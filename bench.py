@@ -426,9 +426,11 @@ def run_batched(args, lib, device):
     specs = workloads.sample_meshes(args.meshes, seed)
     t0 = time.time()
     lib.osdhip_config(b"table_csr_mirror", 0)
+    old_layout = lib.osdhip_config(b"table_layout", args.table_layout)
     workers = min(16, os.cpu_count() or 1)
     dms = workloads.build_device_meshes(specs, L, s, workers=workers, log=log, own_outputs=False)
     lib.osdhip_config(b"table_csr_mirror", 1)
+    lib.osdhip_config(b"table_layout", old_layout)
     torch.cuda.synchronize()
     build_s = time.time() - t0
     # outputs (refined vertices, P / dPdu / dPdv of every frame) stream
@@ -476,7 +478,8 @@ def run_batched(args, lib, device):
                                "xyz + 4 primvars, 64K-1M PatchCoords each; one graph launch per "
                                "step (stencils then patches)",
                    "meshes": len(dms), "seed": seed, "floats_per_point": L, "device": device,
-                   "stages": nstages, "output_bytes_per_step": out_bytes,
+                   "stages": nstages, "table_layout": args.table_layout,
+                   "output_bytes_per_step": out_bytes,
                    "refined_rows_per_step": sum(d.spec.frames * d.nrefine for d in dms),
                    "patch_points_per_step": sum(d.spec.frames * d.ncoords for d in dms),
                    "table_entries": sum(d.nnz_refine + d.nnz_points for d in dms),
@@ -513,8 +516,10 @@ def run_served(args, lib, device):
     with ThreadPoolExecutor(max_workers=workers) as ex:
         hosts = list(ex.map(lambda sp: workloads.compile_mesh(sp, L), specs))
     lib.osdhip_config(b"table_csr_mirror", 0)
+    old_layout = lib.osdhip_config(b"table_layout", args.table_layout)
     residents = serve.residents_from_hosts(hosts)
     lib.osdhip_config(b"table_csr_mirror", 1)
+    lib.osdhip_config(b"table_layout", old_layout)
     for h in hosts:
         h.release()
     torch.cuda.synchronize()
@@ -566,6 +571,7 @@ def run_served(args, lib, device):
                    "topologies": len(residents), "floats_per_point": L, "device": device,
                    "device_budget_bytes": dev_budget, "host_budget_bytes": host_budget,
                    "slots": args.slots, "max_epoch_points": int(args.epoch_points),
+                   "table_layout": args.table_layout,
                    "build_s": build_s},
         "served": {"requests": len(done), "rate_per_s": rate, "capacity_per_s": capacity,
                    "load": args.load, "p50_ms": float(np.percentile(lat, 50)),
@@ -676,6 +682,8 @@ def main():
                     help="cfg4: cap on the points batched into one epoch")
     ap.add_argument("--meshes", type=int, default=256, help="cfg3: meshes in the batch")
     ap.add_argument("--seed", type=int, default=0, help="cfg3: first sample seed")
+    ap.add_argument("--table-layout", type=int, default=2, choices=[0, 2],
+                    help="cfg3/cfg4: stencil table layout (0 record slices, 2 locality-ordered)")
 
     ap.add_argument("--copy-probe", action="store_true",
                     help="time a plain device-to-device copy of 1 GiB and a 1 GiB read "

@@ -127,6 +127,12 @@ void osdhip_stencil_table_destroy(osdhip_stencil_table *table);
 int osdhip_stencil_table_info(const osdhip_stencil_table *table, int *numStencils,
                               long long *numEntries, int *numWeightArrays,
                               long long *deviceBytes);
+/* The table's device layout: layout (see "table_layout" below), number of
+ * 64-row slices and, for layout 2, the slices whose control vertices are
+ * staged in LDS and the total length of their control lists.  No
+ * counterpart in the reference: a diagnostic of this library's layouts. */
+int osdhip_stencil_table_layout(const osdhip_stencil_table *table, int *layout, int *numSlices,
+                                int *listedSlices, long long *listEntries);
 /* DEVICE pointers of the Far-layout arrays; weights receives 6 pointers
  * (NULL for arrays the table does not have). */
 int osdhip_stencil_table_csr(const osdhip_stencil_table *table, const int **sizes,
@@ -170,9 +176,13 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
  *                     for osdhip_stencil_table_eval.
  *   "table_layout":   slice layout of tables created afterwards: 0 = one
  *                     interleaved record {index, weights} per entry
- *                     (default), 1 = index and weights in separate streams.
+ *                     (default), 1 = index and weights in separate streams,
+ *                     2 = locality-ordered slices: rows grouped by their
+ *                     dominant control vertex, each slice's distinct
+ *                     control vertices staged in LDS (results identical;
+ *                     for multi-frame batches and large refinement tables).
  *   "slice_group":    entries per gather group of the record kernel, 4, 8
- *                     or 16 (default 8).
+ *                     or 16 (default 8; layout 2 uses 4, or 8 for 8 and 16).
  *   "table_csr_mirror": 1 (default) = tables created afterwards keep the Far
  *                     CSR arrays on the device (osdhip_stencil_table_csr,
  *                     the CSR kernel); 0 = only the slice layout is kept

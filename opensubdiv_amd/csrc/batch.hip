@@ -31,6 +31,7 @@ struct BatchConfig {
 struct osdhip_batch {
     int length = 0;
     int nS = 0, nP = 0;
+    bool tiled = false;                    // stencil tables in layout 2
     long long stencilRows = 0, frameRows = 0, patchCoords = 0, patchFrames = 0, tableBytes = 0;
     BatchStencilSeg *dS = nullptr;
     BatchPatchSeg *dP = nullptr;
@@ -94,10 +95,40 @@ bool launchRR_lc(int lc, const BatchArgs &a, int grid, hipStream_t st)
     }
 }
 
+template <int FC, int G>
+bool launchT_lc(int lc, const BatchArgs &a, int grid, hipStream_t st)
+{
+    switch (lc) {
+    case 1: hipLaunchKernelGGL((k_batch_stencils_tiled<1, FC, G>), dim3(grid), dim3(kBlock), 0, st, a); return true;
+    case 2: hipLaunchKernelGGL((k_batch_stencils_tiled<2, FC, G>), dim3(grid), dim3(kBlock), 0, st, a); return true;
+    case 3: hipLaunchKernelGGL((k_batch_stencils_tiled<3, FC, G>), dim3(grid), dim3(kBlock), 0, st, a); return true;
+    case 4: hipLaunchKernelGGL((k_batch_stencils_tiled<4, FC, G>), dim3(grid), dim3(kBlock), 0, st, a); return true;
+    case 5: hipLaunchKernelGGL((k_batch_stencils_tiled<5, FC, G>), dim3(grid), dim3(kBlock), 0, st, a); return true;
+    case 6: hipLaunchKernelGGL((k_batch_stencils_tiled<6, FC, G>), dim3(grid), dim3(kBlock), 0, st, a); return true;
+    case 7: hipLaunchKernelGGL((k_batch_stencils_tiled<7, FC, G>), dim3(grid), dim3(kBlock), 0, st, a); return true;
+    case 8: hipLaunchKernelGGL((k_batch_stencils_tiled<8, FC, G>), dim3(grid), dim3(kBlock), 0, st, a); return true;
+    default: return false;
+    }
+}
+
+// phase A variants on locality-ordered tables: (frames per chunk, group)
+bool launchT_variant(int lc, int fc, int g, const BatchArgs &a, int grid, hipStream_t st)
+{
+    if (fc == 1 && g == 4) return launchT_lc<1, 4>(lc, a, grid, st);
+    if (fc == 1 && g == 8) return launchT_lc<1, 8>(lc, a, grid, st);
+    if (fc == 2 && g == 4) return launchT_lc<2, 4>(lc, a, grid, st);
+    if (fc == 2 && g == 8) return launchT_lc<2, 8>(lc, a, grid, st);
+    if (fc == 4 && g == 4) return launchT_lc<4, 4>(lc, a, grid, st);
+    if (fc == 4 && g == 8) return launchT_lc<4, 8>(lc, a, grid, st);
+    return false;
+}
+
 // phase A variants: (frames per chunk, gather group); frames 0 = the
 // records of a slice stay in registers for all frames (group ignored)
-bool launchA_variant(int lc, int fc, int g, const BatchArgs &a, int grid, hipStream_t st)
+bool launchA_variant(bool tiled, int lc, int fc, int g, const BatchArgs &a, int grid,
+                     hipStream_t st)
 {
+    if (tiled) return launchT_variant(lc, fc, g, a, grid, st);
     if (fc == 0) return launchRR_lc(lc, a, grid, st);
     if (fc == 1 && g == 4) return launchA_lc<1, 4>(lc, a, grid, st);
     if (fc == 1 && g == 8) return launchA_lc<1, 8>(lc, a, grid, st);
@@ -140,11 +171,6 @@ bool launchB_variant(int lc, int fc, const BatchArgs &a, int grid, hipStream_t s
     if (fc == 2) return launchB_lc<2>(lc, a, grid, st);
     if (fc == 4) return launchB_lc<4>(lc, a, grid, st);
     return false;
-}
-
-bool validA(int fc, int g)
-{
-    return fc == 0 || (fc == 1 && (g == 4 || g == 8)) || (fc == 2 && g == 4) || (fc == 4 && g == 2);
 }
 
 // (Re)builds the phase-A item list for cfg.aItemSlices.
@@ -207,7 +233,7 @@ int enqueuePhaseA(osdhip_batch *b, int stage, hipStream_t st)
     a.nitems = n;
     a.persistent = b->cfg.aPersistent;
     a.counter = b->counters + kCounters * (2 * stage);
-    if (!launchA_variant(b->length, b->cfg.aFrames, b->cfg.aGroup, a,
+    if (!launchA_variant(b->tiled, b->length, b->cfg.aFrames, b->cfg.aGroup, a,
                          gridFor(b, n, a.persistent != 0), st))
         return OSDHIP_ERR_UNSUPPORTED;
     return last_error();
@@ -307,11 +333,13 @@ osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
     for (int i = 0; i < numStencilSegments; ++i) {
         const osdhip_batch_stencils &s = stencils[i];
         const osdhip_stencil_table *t = s.table;
-        if (!t || t->layout != 0 || t->nw != 1 || s.frames < 1 || !s.src || !s.dst ||
-            s.srcVertexStride < length || s.dstVertexStride < length) {
+        if (!t || (t->layout != 0 && t->layout != 2) || t->nw != 1 || s.frames < 1 || !s.src ||
+            !s.dst || s.srcVertexStride < length || s.dstVertexStride < length ||
+            (i > 0 && (t->layout == 2) != b->tiled)) {
             freeBatch(b);
             return nullptr;
         }
+        b->tiled = t->layout == 2;
         if (t->n == 0) continue;
         BatchStencilSeg g;
         std::memset(&g, 0, sizeof(g));
@@ -319,6 +347,10 @@ osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
         g.sliceOff = t->sliceOff;
         g.sliceWidth = t->sliceWidth;
         g.rowSize = t->rowSize;
+        g.perm = t->perm;
+        g.nuniq = t->nuniq;
+        g.ctrlOff = t->ctrlOff;
+        g.ctrl = t->ctrl;
         g.src = s.src;
         g.srcV = s.srcVertexStride;
         g.srcF = s.srcFrameStride;
@@ -576,7 +608,11 @@ int osdhip_batch_tune(osdhip_batch *b, int reps, void *stream, char *report, int
     char buf[160];
     // phase A: (frames per chunk, gather group) x item slices x scheduling
     if (b->nS) {
-        const int fg[5][2] = { { 0, 4 }, { 1, 4 }, { 1, 8 }, { 2, 4 }, { 4, 2 } };
+        const std::vector<std::pair<int, int>> fg =
+            b->tiled ? std::vector<std::pair<int, int>>{ { 1, 4 }, { 1, 8 }, { 2, 4 }, { 2, 8 },
+                                                         { 4, 4 }, { 4, 8 } }
+                     : std::vector<std::pair<int, int>>{ { 0, 4 }, { 1, 4 }, { 1, 8 }, { 2, 4 },
+                                                         { 4, 2 } };
         const int slices[3] = { 4, 16, 64 };
         BatchConfig bestc = b->cfg;
         float bestms = 1e30f;
@@ -585,8 +621,8 @@ int osdhip_batch_tune(osdhip_batch *b, int reps, void *stream, char *report, int
         for (auto &v : fg)
             for (int sl : slices)
                 for (int pers = 0; pers < 2; ++pers) {
-                    b->cfg.aFrames = v[0];
-                    b->cfg.aGroup = v[1];
+                    b->cfg.aFrames = v.first;
+                    b->cfg.aGroup = v.second;
                     b->cfg.aItemSlices = sl;
                     b->cfg.aPersistent = pers;
                     float t = 0.0f;
@@ -594,7 +630,7 @@ int osdhip_batch_tune(osdhip_batch *b, int reps, void *stream, char *report, int
                     if (rc != OSDHIP_OK) break;
                     std::snprintf(buf, sizeof(buf),
                                   "%s{\"frames\":%d,\"group\":%d,\"slices\":%d,\"persistent\":%d,\"ms\":%.4f}",
-                                  first ? "" : ",", v[0], v[1], sl, pers, t);
+                                  first ? "" : ",", v.first, v.second, sl, pers, t);
                     rep += buf;
                     first = false;
                     if (t < bestms) { bestms = t; bestc = b->cfg; }
@@ -665,8 +701,9 @@ bool planEpoch(int length, int nS, const osdhip_batch_stencils *st, int nP,
     long long ia = 0, ib = 0;
     for (int i = 0; i < nS; ++i) {
         const osdhip_stencil_table *t = st[i].table;
-        if (!t || t->layout != 0 || t->nw != 1 || st[i].frames < 1 || !st[i].src || !st[i].dst ||
-            st[i].srcVertexStride < length || st[i].dstVertexStride < length)
+        if (!t || (t->layout != 0 && t->layout != 2) || t->nw != 1 || st[i].frames < 1 ||
+            !st[i].src || !st[i].dst || st[i].srcVertexStride < length ||
+            st[i].dstVertexStride < length || (t->layout == 2) != (st[0].table->layout == 2))
             return false;
         ia += (t->nslices + itemSlices - 1) / itemSlices;
     }
@@ -728,6 +765,10 @@ int osdhip_batch_epoch(int length, int numStencilSegments, const osdhip_batch_st
         g.sliceOff = t->sliceOff;
         g.sliceWidth = t->sliceWidth;
         g.rowSize = t->rowSize;
+        g.perm = t->perm;
+        g.nuniq = t->nuniq;
+        g.ctrlOff = t->ctrlOff;
+        g.ctrl = t->ctrl;
         g.src = s.src;
         g.srcV = s.srcVertexStride;
         g.srcF = s.srcFrameStride;
@@ -773,7 +814,9 @@ int osdhip_batch_epoch(int length, int numStencilSegments, const osdhip_batch_st
         a.sseg = (const BatchStencilSeg *)(d + p.segA);
         a.items = (const BatchItem *)(d + p.itemsA);
         a.nitems = na;
-        if (!launchA_variant(length, 1, 8, a, grid_for_blocks(na), st)) return OSDHIP_ERR_UNSUPPORTED;
+        const bool tiled = stencils[0].table->layout == 2;
+        if (!launchA_variant(tiled, length, 1, tiled ? 4 : 8, a, grid_for_blocks(na), st))
+            return OSDHIP_ERR_UNSUPPORTED;
         int rc = last_error();
         if (rc != OSDHIP_OK) return rc;
     }

@@ -30,6 +30,7 @@
 #include "common.h"
 #include "patch_basis_device.h"
 #include "stencil_sliced.h"
+#include "stencil_tiled.h"
 
 namespace osdhip {
 
@@ -37,7 +38,11 @@ struct BatchStencilSeg {
     const unsigned *rec;          // record layout, one weight array (R = 2)
     const long long *sliceOff;
     const int *sliceWidth;
-    const int *rowSize;           // padded to whole slices
+    const int *rowSize;           // layout 0: padded to whole slices
+    const int *perm;              // layout 2: slice row -> table row
+    const int *nuniq;             // layout 2: listed control vertices per slice
+    const long long *ctrlOff;
+    const int *ctrl;
     const float *src;
     long long srcV, srcF;
     float *dst;
@@ -66,6 +71,10 @@ struct GStencilSeg {
     const G_ long long *sliceOff;
     const G_ int *sliceWidth;
     const G_ int *rowSize;
+    const G_ int *perm;
+    const G_ int *nuniq;
+    const G_ long long *ctrlOff;
+    const G_ int *ctrl;
     const G_ float *src;
     long long srcV, srcF;
     G_ float *dst;
@@ -85,13 +94,18 @@ struct GPatchSeg {
     int ncoords, frames;
 };
 
-__device__ __forceinline__ GStencilSeg gseg(const BatchStencilSeg &s)
+#define C_ OSDHIP_CONST
+__device__ __forceinline__ GStencilSeg gseg(const C_ BatchStencilSeg &s)
 {
     GStencilSeg g;
     g.rec = gptr(s.rec);
     g.sliceOff = gptr(s.sliceOff);
     g.sliceWidth = gptr(s.sliceWidth);
     g.rowSize = gptr(s.rowSize);
+    g.perm = gptr(s.perm);
+    g.nuniq = gptr(s.nuniq);
+    g.ctrlOff = gptr(s.ctrlOff);
+    g.ctrl = gptr(s.ctrl);
     g.src = gptr(s.src);
     g.srcV = s.srcV;
     g.srcF = s.srcF;
@@ -104,7 +118,7 @@ __device__ __forceinline__ GStencilSeg gseg(const BatchStencilSeg &s)
     return g;
 }
 
-__device__ __forceinline__ GPatchSeg gseg(const BatchPatchSeg &s)
+__device__ __forceinline__ GPatchSeg gseg(const C_ BatchPatchSeg &s)
 {
     GPatchSeg g;
     g.coords = gptr(s.coords);
@@ -165,6 +179,16 @@ struct BatchItem {
     int seg, first, count, pad;   // phase A: slices [first, first+count); B: coords
 };
 
+__device__ __forceinline__ BatchItem ld(const C_ BatchItem *p)
+{
+    BatchItem it;
+    it.seg = p->seg;
+    it.first = p->first;
+    it.count = p->count;
+    it.pad = 0;
+    return it;
+}
+
 struct BatchArgs {
     const BatchStencilSeg *sseg;
     const BatchPatchSeg *pseg;
@@ -187,7 +211,8 @@ __global__ void k_batch_reset(unsigned *counters, int n)
 // Persistent: block b claims items from its XCD's range through that
 // range's counter and, when the range is exhausted, from the other ranges in
 // turn; it returns -1 once every range is exhausted.  Must be reached by
-// every thread of the block (it contains __syncthreads).
+// every thread of the block (it contains __syncthreads).  The result is
+// wave-uniform.
 __device__ __forceinline__ int batch_next_item(const BatchArgs &a, int iter)
 {
     if (!a.persistent) {
@@ -214,19 +239,20 @@ __device__ __forceinline__ int batch_next_item(const BatchArgs &a, int iter)
         slot = it;
     }
     __syncthreads();
-    return slot;
+    // block-uniform: the item's descriptors are then read with scalar loads
+    return __builtin_amdgcn_readfirstlane(slot);
 }
 
 template <int LC, int FC, int G>
 __global__ __launch_bounds__(kBlock) void k_batch_stencils(BatchArgs a)
 {
     const int lane = (int)threadIdx.x & (kWave - 1);
-    const int wv = (int)threadIdx.x / kWave;
+    const int wv = wave_in_block();
     for (int iter = 0;; ++iter) {
         const int itx = batch_next_item(a, iter);
         if (itx < 0) break;
-        const BatchItem item = a.items[itx];
-        const GStencilSeg sg = gseg(a.sseg[item.seg]);
+        const BatchItem item = ld(cptr(a.items) + itx);
+        const GStencilSeg sg = gseg(cptr(a.sseg)[item.seg]);
         for (int sl = wv; sl < item.count; sl += kWavesPerBlock) {
             const int s = item.first + sl;
             const int row = s * kWave + lane;
@@ -244,12 +270,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils(BatchArgs a)
                     int ii[G];
                     float ww[1][G];
 #pragma unroll
-                    for (int u = 0; u < G; ++u) {
-                        ii[u] = 0;
-                        ww[0][u] = 0.0f;
-                        if (j + u < W)                          // wave-uniform
-                            load_record_g(rp + (size_t)(j + u) * (kWave * 2), ii[u], ww[0][u]);
-                    }
+                    for (int u = 0; u < G; ++u)   // past the width: last column, not accumulated
+                        load_record_g(rp + (size_t)min(j + u, W - 1) * (kWave * 2), ii[u], ww[0][u]);
                     float v[G][FC][LC];
 #pragma unroll
                     for (int u = 0; u < G; ++u) {
@@ -293,6 +315,168 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils(BatchArgs a)
     }
 }
 
+// A layout-2 slice with more distinct control vertices than LDS slots:
+// records hold control indices (padding: kTilePad, skipped); every frame in
+// turn gathers from memory and writes its results.
+template <int LC, int G>
+__device__ __forceinline__ void tiled_gather_slice(const GStencilSeg &sg, const G_ unsigned *rp, int W,
+                                                int row, bool active)
+{
+    for (int f = 0; f < sg.frames; ++f) {
+        float acc[LC];
+#pragma unroll
+        for (int k = 0; k < LC; ++k) acc[k] = 0.0f;
+        for (int j = 0; j < W; j += G) {
+            int ii[G];
+            float ww[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                load_record_g(rp + (size_t)min(j + u, W - 1) * (kWave * 2), ii[u], ww[u]);
+                ii[u] = j + u < W ? ii[u] : (int)kTilePad;
+            }
+            float v[G][LC];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const unsigned ix = (unsigned)ii[u] == kTilePad ? 0u : (unsigned)ii[u];
+                const G_ float *p = sg.src + (size_t)ix * sg.srcV + (size_t)f * sg.srcF;
+#pragma unroll
+                for (int k = 0; k < LC; ++k) v[u][k] = p[k];
+            }
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const bool take = (unsigned)ii[u] != kTilePad;
+#pragma unroll
+                for (int k = 0; k < LC; ++k) {
+                    const float t = acc[k] + v[u][k] * ww[u];
+                    acc[k] = take ? t : acc[k];
+                }
+            }
+        }
+        if (active) {
+            G_ float *d = sg.dst + (size_t)row * sg.dstV + (size_t)f * sg.dstF;
+#pragma unroll
+            for (int k = 0; k < LC; ++k) d[k] = acc[k];
+        }
+    }
+}
+
+// Records held in registers across every frame chunk (slices up to this
+// width: the table then leaves memory once per step).
+constexpr int kTiledRegRecords = 16;
+
+// One listed layout-2 slice for every frame, FC frames per chunk: the
+// wave stages its listed control vertices' frames into LDS (lane u ->
+// slot u), then accumulates each row's entries from LDS.  RES: the slice's
+// (up to N) records are loaded once into registers; otherwise they are read
+// N per group for every chunk.
+template <int LC, int FC, int N, bool RES, int G>
+__device__ __forceinline__ void tiled_lds_slice(const GStencilSeg &sg, float (*Ls)[kTileSlots],
+                                                const G_ unsigned *rp, int W, int U, int vtx,
+                                                int row, bool active)
+{
+    const int lane = (int)threadIdx.x & (kWave - 1);
+    int slot[N];
+    float ww[N];
+    auto load = [&](int j0) {
+#pragma unroll
+        for (int u = 0; u < N; ++u) {
+            // unconditional loads; past the width: the zero slot
+            load_record_g(rp + (size_t)min(j0 + u, W - 1) * (kWave * 2), slot[u], ww[u]);
+            slot[u] = j0 + u < W ? slot[u] : kTileZero;
+            ww[u] = j0 + u < W ? ww[u] : 0.0f;
+        }
+    };
+    if (RES) load(0);
+    for (int f0 = 0; f0 < sg.frames; f0 += FC) {
+        float acc[FC][LC];
+#pragma unroll
+        for (int c = 0; c < FC; ++c)
+#pragma unroll
+            for (int k = 0; k < LC; ++k) acc[c][k] = 0.0f;
+        wave_lds_sync();                                 // previous chunk's reads are done
+        if (lane < U) {
+            const G_ float *p = sg.src + (size_t)(unsigned)vtx * sg.srcV + (size_t)f0 * sg.srcF;
+#pragma unroll
+            for (int c = 0; c < FC; ++c) {
+                if (f0 + c < sg.frames) {                   // wave-uniform
+#pragma unroll
+                    for (int k = 0; k < LC; ++k) Ls[c * LC + k][lane] = p[c * sg.srcF + k];
+                }
+            }
+        }
+        wave_lds_sync();
+        for (int j0 = 0; j0 < (RES ? 1 : W); j0 += N) {
+            if (!RES) load(j0);
+#pragma unroll
+            for (int u = 0; u < N; ++u) {
+                if (RES && u >= W) break;                   // wave-uniform
+#pragma unroll
+                for (int c = 0; c < FC; ++c)
+#pragma unroll
+                    for (int k = 0; k < LC; ++k)
+                        acc[c][k] = acc[c][k] + Ls[c * LC + k][slot[u]] * ww[u];
+                // one entry's LDS reads in flight at a time: hoisting all
+                // of them would cut occupancy
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        if (active) {
+#pragma unroll
+            for (int c = 0; c < FC; ++c) {
+                if (f0 + c >= sg.frames) break;
+                G_ float *d = sg.dst + (size_t)row * sg.dstV + (size_t)(f0 + c) * sg.dstF;
+#pragma unroll
+                for (int k = 0; k < LC; ++k) d[k] = acc[c][k];
+            }
+        }
+    }
+}
+
+// Phase A on locality-ordered tables (layout 2): per slice and chunk of FC
+// frames, the wave stages the slice's listed control vertices (one per
+// lane, FC frames each) into its LDS region, element (c, k) of slot u at
+// lds[c * LC + k][u], then accumulates every entry from LDS; a slice's
+// records stay in registers for all its chunks (slices of width up to
+// kTiledRegRecords) or are re-read per chunk.  Results go to the table row
+// perm[slice row].  Unlisted slices gather from memory.
+template <int LC, int FC, int G>
+__global__ __launch_bounds__(kBlock) void k_batch_stencils_tiled(BatchArgs a)
+{
+    __shared__ float lds[kWavesPerBlock][FC * LC][kTileSlots];
+    const int lane = (int)threadIdx.x & (kWave - 1);
+    const int wv = wave_in_block();
+    float(*Ls)[kTileSlots] = lds[wv];
+    if (lane == kTileZero) {
+#pragma unroll
+        for (int q = 0; q < FC * LC; ++q) Ls[q][kTileZero] = 0.0f;
+    }
+    for (int iter = 0;; ++iter) {
+        const int itx = batch_next_item(a, iter);
+        if (itx < 0) break;
+        const BatchItem item = ld(cptr(a.items) + itx);
+        const GStencilSeg sg = gseg(cptr(a.sseg)[item.seg]);
+        for (int sl = wv; sl < item.count; sl += kWavesPerBlock) {
+            const int s = item.first + sl;
+            const int row = sg.perm[(long long)s * kWave + lane];
+            const bool active = row < sg.nrows;
+            const int W = sg.sliceWidth[s];
+            const int U = sg.nuniq[s];
+            const G_ unsigned *rp = sg.rec + (size_t)(sg.sliceOff[s] + lane) * 2;
+            if (U < 0) {
+                // rare: more distinct control vertices than LDS slots; one
+                // frame at a time, gathers from memory
+                tiled_gather_slice<LC, G>(sg, rp, W, row, active);
+                continue;
+            }
+            const int vtx = lane < U ? sg.ctrl[sg.ctrlOff[s] + lane] : 0;
+            if (W <= kTiledRegRecords)
+                tiled_lds_slice<LC, FC, kTiledRegRecords, true, G>(sg, Ls, rp, W, U, vtx, row, active);
+            else
+                tiled_lds_slice<LC, FC, G, false, G>(sg, Ls, rp, W, U, vtx, row, active);
+        }
+    }
+}
+
 // Phase A with the slice's records held in registers for all frames: a
 // slice of width W <= 16 loads its (index, weight) records once, then every
 // frame gathers and accumulates from registers, so the table leaves HBM
@@ -303,12 +487,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils_rr(BatchArgs a)
 {
     constexpr int MAXW = 16, GG = 4;
     const int lane = (int)threadIdx.x & (kWave - 1);
-    const int wv = (int)threadIdx.x / kWave;
+    const int wv = wave_in_block();
     for (int iter = 0;; ++iter) {
         const int itx = batch_next_item(a, iter);
         if (itx < 0) break;
-        const BatchItem item = a.items[itx];
-        const GStencilSeg sg = gseg(a.sseg[item.seg]);
+        const BatchItem item = ld(cptr(a.items) + itx);
+        const GStencilSeg sg = gseg(cptr(a.sseg)[item.seg]);
         for (int sl = wv; sl < item.count; sl += kWavesPerBlock) {
             const int s = item.first + sl;
             const int row = s * kWave + lane;
@@ -320,12 +504,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils_rr(BatchArgs a)
                 int ii[MAXW];
                 float ww[1][MAXW];
 #pragma unroll
-                for (int j = 0; j < MAXW; ++j) {
-                    ii[j] = 0;
-                    ww[0][j] = 0.0f;
-                    if (j < W)                                   // wave-uniform
-                        load_record_g(rp + (size_t)j * (kWave * 2), ii[j], ww[0][j]);
-                }
+                for (int j = 0; j < MAXW; ++j)    // past the width: last column, not accumulated
+                    load_record_g(rp + (size_t)min(j, W - 1) * (kWave * 2), ii[j], ww[0][j]);
                 for (int f = 0; f < sg.frames; ++f) {
                     float acc[LC];
 #pragma unroll
@@ -368,12 +548,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils_rr(BatchArgs a)
                     int ii[GG];
                     float ww[1][GG];
 #pragma unroll
-                    for (int u = 0; u < GG; ++u) {
-                        ii[u] = 0;
-                        ww[0][u] = 0.0f;
-                        if (j + u < W)
-                            load_record_g(rp + (size_t)(j + u) * (kWave * 2), ii[u], ww[0][u]);
-                    }
+                    for (int u = 0; u < GG; ++u)
+                        load_record_g(rp + (size_t)min(j + u, W - 1) * (kWave * 2), ii[u], ww[0][u]);
                     float v[GG][LC];
 #pragma unroll
                     for (int u = 0; u < GG; ++u) {
@@ -442,8 +618,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches_regular(BatchArgs a)
     for (int iter = 0;; ++iter) {
         const int itx = batch_next_item(a, iter);
         if (itx < 0) break;
-        const BatchItem item = a.items[itx];
-        const GPatchSeg sg = gseg(a.pseg[item.seg]);
+        const BatchItem item = ld(cptr(a.items) + itx);
+        const GPatchSeg sg = gseg(cptr(a.pseg)[item.seg]);
         const int i = item.first + (int)threadIdx.x;
         if (i >= item.first + item.count) continue;
 
@@ -511,8 +687,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
     for (int iter = 0;; ++iter) {
         const int itx = batch_next_item(a, iter);
         if (itx < 0) break;
-        const BatchItem item = a.items[itx];
-        const GPatchSeg sg = gseg(a.pseg[item.seg]);
+        const BatchItem item = ld(cptr(a.items) + itx);
+        const GPatchSeg sg = gseg(cptr(a.pseg)[item.seg]);
         const int i = item.first + (int)threadIdx.x;
         if (i >= item.first + item.count) continue;
 

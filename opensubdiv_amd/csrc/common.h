@@ -15,6 +15,14 @@ typedef int v2i __attribute__((ext_vector_type(2)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
+// Index of this wavefront in its block, as a wave-uniform (scalar) value:
+// slice indices, widths and loop bounds derived from it then live in SGPRs
+// and guard scalar branches, so a wave's record loads issue back to back.
+__device__ __forceinline__ int wave_in_block()
+{
+    return __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
+}
+
 // Wave-scope LDS hand-off: LDS traffic of one wave is processed in order,
 // so only the compiler has to be kept from reordering across this point.
 __device__ __forceinline__ void wave_lds_sync()
@@ -47,6 +55,17 @@ template <class T>
 __device__ __forceinline__ OSDHIP_GLOBAL T *gptr_w(T *p)
 {
     return (OSDHIP_GLOBAL T *)p;
+}
+
+// Read-only launch descriptors (segment tables, item lists): read through
+// the constant address space, i.e. with scalar loads into SGPRs, since the
+// compiler cannot prove on its own that the kernel's stores leave them
+// unchanged.
+#define OSDHIP_CONST __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const OSDHIP_CONST T *cptr(const T *p)
+{
+    return (const OSDHIP_CONST T *)p;
 }
 
 inline int grid_for_blocks(int nblocks)

@@ -29,6 +29,7 @@
 #include "patch_kernels.h"
 #include "stencil_csr.h"
 #include "stencil_sliced.h"
+#include "stencil_tiled.h"
 #include "table.h"
 #include "table_build.h"
 
@@ -37,7 +38,8 @@ namespace osdhip {
 // Tuning knobs (osdhip_config).
 static int g_stencil_kernel = 0;     // table eval: 0 sliced, 1 CSR
 static int g_patch_sort_min = 0;     // EvalPatches: patch-sort coords when n >= this (0 = never)
-static int g_table_layout = 0;       // new tables: 0 record-interleaved slices, 1 split streams
+static int g_table_layout = 0;       // new tables: 0 record-interleaved slices, 1 split streams,
+                                     // 2 locality-ordered slices
 static int g_csr_mirror = 1;         // new tables keep the Far CSR arrays on the device
 static int g_slice_group = 8;        // record kernel: entries per gather group (4, 8, 16)
 
@@ -62,6 +64,28 @@ static void launch_rec(int lc, int group, const RecArgs &a, int grid, hipStream_
     if (group == 4) launch_rec_g<NWT, 4>(lc, a, grid, st);
     else if (group == 16) launch_rec_g<NWT, 16>(lc, a, grid, st);
     else launch_rec_g<NWT, 8>(lc, a, grid, st);
+}
+
+template <int NWT, int G>
+static void launch_tiled_g(int lc, const TiledArgs &a, int grid, hipStream_t st)
+{
+    switch (lc) {
+    case 1: hipLaunchKernelGGL((k_stencils_tiled<NWT, 1, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_stencils_tiled<NWT, 2, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_stencils_tiled<NWT, 3, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_stencils_tiled<NWT, 4, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 5: hipLaunchKernelGGL((k_stencils_tiled<NWT, 5, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 6: hipLaunchKernelGGL((k_stencils_tiled<NWT, 6, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 7: hipLaunchKernelGGL((k_stencils_tiled<NWT, 7, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_stencils_tiled<NWT, 8, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    }
+}
+
+template <int NWT>
+static void launch_tiled(int lc, int group, const TiledArgs &a, int grid, hipStream_t st)
+{
+    if (group == 4) launch_tiled_g<NWT, 4>(lc, a, grid, st);
+    else launch_tiled_g<NWT, 8>(lc, a, grid, st);
 }
 
 template <int NW, int CAP, bool VEC>
@@ -192,7 +216,8 @@ static void free_table(osdhip_stencil_table *t)
     if (!t) return;
     void *ptrs[] = { t->sizes, t->offsets, t->indices, t->w[0], t->w[1], t->w[2], t->w[3],
                      t->w[4], t->w[5], t->sliceOff, t->sliceWidth, t->rowSize, t->rec, t->sidx,
-                     t->sw[0], t->sw[1], t->sw[2], t->sw[3], t->sw[4], t->sw[5] };
+                     t->sw[0], t->sw[1], t->sw[2], t->sw[3], t->sw[4], t->sw[5], t->perm,
+                     t->nuniq, t->ctrlOff, t->ctrl };
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     delete t;
@@ -216,6 +241,92 @@ static bool upload(T **dptr, const T *host, size_t count, hipStream_t st, long l
     if (hipMalloc((void **)dptr, count * sizeof(T)) != hipSuccess) { *dptr = nullptr; return false; }
     bytes += (long long)(count * sizeof(T));
     return hipMemcpyAsync(*dptr, host, count * sizeof(T), hipMemcpyHostToDevice, st) == hipSuccess;
+}
+
+// Exclusive prefix sum of n counts into out; returns the total (-1 on error).
+template <typename T>
+static long long exclusive_scan(const T *counts, T *out, int n, hipStream_t st)
+{
+    size_t bytes = 0;
+    void *tmp = nullptr;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, counts, out, n, st) != hipSuccess ||
+        hipMalloc(&tmp, std::max<size_t>(bytes, 1)) != hipSuccess)
+        return -1;
+    T last[2] = { 0, 0 };
+    const bool ok =
+        hipcub::DeviceScan::ExclusiveSum(tmp, bytes, counts, out, n, st) == hipSuccess &&
+        hipMemcpyAsync(&last[0], out + (n - 1), sizeof(T), hipMemcpyDeviceToHost, st) == hipSuccess &&
+        hipMemcpyAsync(&last[1], counts + (n - 1), sizeof(T), hipMemcpyDeviceToHost, st) ==
+            hipSuccess &&
+        hipStreamSynchronize(st) == hipSuccess;
+    (void)hipFree(tmp);
+    return ok ? (long long)last[0] + (long long)last[1] : -1;
+}
+
+// Layout 2 from the uploaded CSR arrays: dominant-vertex row order (stable
+// radix sort), per-slice widths and control lists (count pass, scans, fill
+// pass).
+static bool build_tiled(osdhip_stencil_table *t, hipStream_t st)
+{
+    const int n = t->n, S = t->nslices, rows = S * kWave;
+    long long scratchBytes = 0;
+    unsigned *keys = nullptr, *keysOut = nullptr;
+    int *vals = nullptr;
+    long long *entries = nullptr, *listLen = nullptr;
+    void *sortTmp = nullptr;
+    size_t sortBytes = 0;
+    const int gr = (rows + kBlock - 1) / kBlock, gs = (S + kWavesPerBlock - 1) / kWavesPerBlock;
+    const float *const *W = t->w;
+    bool ok = dalloc(&keys, (size_t)rows, scratchBytes) && dalloc(&keysOut, (size_t)rows, scratchBytes) &&
+              dalloc(&vals, (size_t)rows, scratchBytes) && dalloc(&entries, (size_t)S, scratchBytes) &&
+              dalloc(&listLen, (size_t)S, scratchBytes) && dalloc(&t->perm, (size_t)rows, t->bytes) &&
+              dalloc(&t->sliceWidth, (size_t)S, t->bytes) && dalloc(&t->sliceOff, (size_t)S, t->bytes) &&
+              dalloc(&t->nuniq, (size_t)S, t->bytes) && dalloc(&t->ctrlOff, (size_t)S, t->bytes);
+    if (ok) {
+        hipLaunchKernelGGL(k_row_dominant, dim3(gr), dim3(kBlock), 0, st, t->sizes, t->offsets,
+                           t->indices, W[0], n, rows, keys, vals);
+        ok = hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, keys, keysOut, vals, t->perm, rows,
+                                                0, 32, st) == hipSuccess &&
+             hipMalloc(&sortTmp, std::max<size_t>(sortBytes, 1)) == hipSuccess &&
+             hipcub::DeviceRadixSort::SortPairs(sortTmp, sortBytes, keys, keysOut, vals, t->perm,
+                                                rows, 0, 32, st) == hipSuccess;
+    }
+#define OSDHIP_TILED(NWT, FILL)                                                                 \
+    hipLaunchKernelGGL((k_tiled_slices<NWT, FILL>), dim3(gs), dim3(kBlock), 0, st, t->perm,     \
+                       t->sizes, t->offsets, t->indices, W[0], W[1], W[2], W[3], W[4], W[5], n, \
+                       S, t->sliceWidth, entries, t->nuniq, listLen, t->sliceOff, t->ctrlOff,   \
+                       t->ctrl, t->rec)
+    if (ok) {
+        if (t->nw == 1) OSDHIP_TILED(1, false);
+        else if (t->nw == 3) OSDHIP_TILED(3, false);
+        else OSDHIP_TILED(6, false);
+        ok = hipGetLastError() == hipSuccess;
+    }
+    if (ok) {
+        std::vector<int> nu((size_t)S);
+        ok = hipMemcpyAsync(nu.data(), t->nuniq, (size_t)S * sizeof(int), hipMemcpyDeviceToHost,
+                            st) == hipSuccess &&
+             hipStreamSynchronize(st) == hipSuccess;
+        t->listedSlices = (int)std::count_if(nu.begin(), nu.end(), [](int u) { return u >= 0; });
+    }
+    if (ok) {
+        t->sliceEntries = exclusive_scan(entries, t->sliceOff, S, st);
+        t->ctrlLen = exclusive_scan(listLen, t->ctrlOff, S, st);
+        ok = t->sliceEntries >= 0 && t->ctrlLen >= 0 &&
+             dalloc(&t->rec, (size_t)t->sliceEntries * rec_dwords(t->nw), t->bytes) &&
+             dalloc(&t->ctrl, (size_t)t->ctrlLen, t->bytes);
+    }
+    if (ok) {
+        if (t->nw == 1) OSDHIP_TILED(1, true);
+        else if (t->nw == 3) OSDHIP_TILED(3, true);
+        else OSDHIP_TILED(6, true);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
+    }
+#undef OSDHIP_TILED
+    void *tmp[] = { keys, keysOut, vals, entries, listLen, sortTmp };
+    for (void *p : tmp)
+        if (p) (void)hipFree(p);
+    return ok;
 }
 
 extern "C" {
@@ -313,7 +424,9 @@ osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *si
     void *scanTmp = nullptr;
     size_t scanBytes = 0;
     long long dummy = 0;
-    if (ok && S > 0) {
+    if (ok && S > 0 && t->layout == 2) {
+        ok = build_tiled(t, st);
+    } else if (ok && S > 0) {
         ok = dalloc(&t->sliceOff, (size_t)S, t->bytes) && dalloc(&t->sliceWidth, (size_t)S, t->bytes) &&
              dalloc(&t->rowSize, (size_t)rows, t->bytes) && dalloc(&entries, (size_t)S, dummy);
         if (ok) {
@@ -395,6 +508,17 @@ int osdhip_stencil_table_info(const osdhip_stencil_table *t, int *numStencils,
     return OSDHIP_OK;
 }
 
+int osdhip_stencil_table_layout(const osdhip_stencil_table *t, int *layout, int *numSlices,
+                                int *listedSlices, long long *listEntries)
+{
+    if (!t) return OSDHIP_ERR_ARGS;
+    if (layout) *layout = t->layout;
+    if (numSlices) *numSlices = t->nslices;
+    if (listedSlices) *listedSlices = t->listedSlices;
+    if (listEntries) *listEntries = t->ctrlLen;
+    return OSDHIP_OK;
+}
+
 int osdhip_stencil_table_csr(const osdhip_stencil_table *t, const int **sizes, const int **offsets,
                              const int **indices, const float **weights)
 {
@@ -435,6 +559,40 @@ int osdhip_stencil_table_eval(const osdhip_stencil_table *t, const float *src,
     const int nblocks = (nslices + kWavesPerBlock - 1) / kWavesPerBlock;
     const int grid = grid_for_blocks(nblocks);
     hipStream_t st = (hipStream_t)stream;
+    if (t->layout == 2) {
+        // a row range maps to slices anywhere in the locality order: every
+        // slice is visited and rows outside [start, end) are not written
+        TiledArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.srcStride = srcDesc.stride;
+        a.perm = t->perm;
+        a.nuniq = t->nuniq;
+        a.ctrlOff = t->ctrlOff;
+        a.ctrl = t->ctrl;
+        a.sliceOff = t->sliceOff;
+        a.sliceWidth = t->sliceWidth;
+        a.rec = t->rec;
+        a.rowBegin = start;
+        a.rowEnd = end;
+        a.nslices = t->nslices;
+        a.nblocks = (t->nslices + kWavesPerBlock - 1) / kWavesPerBlock;
+        const int tgrid = grid_for_blocks(a.nblocks);
+        for (int e0 = 0; e0 < srcDesc.length; e0 += 8) {
+            const int lc = std::min(8, srcDesc.length - e0);
+            a.src = src + srcDesc.offset + e0;
+            for (int k = 0; k < numOutputs; ++k) {
+                a.dst[k] = outputs[k] ? outputs[k] + outputDescs[k].offset + e0 : nullptr;
+                a.dstStride[k] = outputs[k] ? outputDescs[k].stride : 0;
+            }
+            const int g = g_slice_group == 4 ? 4 : 8;
+            if (t->nw == 1) launch_tiled<1>(lc, g, a, tgrid, st);
+            else if (t->nw == 3) launch_tiled<3>(lc, g, a, tgrid, st);
+            else launch_tiled<6>(lc, g, a, tgrid, st);
+            int rc = last_error();
+            if (rc != OSDHIP_OK) return rc;
+        }
+        return OSDHIP_OK;
+    }
     if (t->layout == 0) {
         RecArgs a;
         std::memset(&a, 0, sizeof(a));
@@ -582,7 +740,7 @@ int osdhip_config(const char *key, int value)
         return old;
     }
     if (std::strcmp(key, "table_layout") == 0) {
-        if (value != 0 && value != 1) return OSDHIP_ERR_ARGS;
+        if (value < 0 || value > 2) return OSDHIP_ERR_ARGS;
         int old = g_table_layout;
         g_table_layout = value;
         return old;

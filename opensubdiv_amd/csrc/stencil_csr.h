@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_stencils_csr(StencilArgs a)
     __shared__ __attribute__((aligned(16))) float s_w[kWavesPerBlock][NW][SLOTS];
 
     const int lane = (int)threadIdx.x & (kWave - 1);
-    const int wv = (int)threadIdx.x / kWave;
+    const int wv = wave_in_block();
     const int lb = xcd_logical_block();
     if (lb >= a.nblocks) return;
     const int first = lb * kBlock + wv * kWave;     // relative to a.start

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kBlock) void k_stencils_sliced(SlicedArgs a)
     const int lane = (int)threadIdx.x & (kWave - 1);
     const int lb = xcd_logical_block();
     if (lb >= a.nblocks) return;
-    const int sl = lb * kWavesPerBlock + (int)threadIdx.x / kWave;
+    const int sl = lb * kWavesPerBlock + wave_in_block();
     if (sl >= a.nslices) return;                     // whole wave idle
     const int s = a.sliceBegin + sl;
     const int row = s * kWave + lane;
@@ -59,12 +59,12 @@ __global__ __launch_bounds__(kBlock) void k_stencils_sliced(SlicedArgs a)
         float ww[NW][kSliceGroup];
 #pragma unroll
         for (int u = 0; u < kSliceGroup; ++u) {
-            if (j + u < W) {                          // wave-uniform
-                const long long e = eo + (long long)(j + u) * kWave;
-                ii[u] = __builtin_nontemporal_load(a.idx + e);
+            // columns past the width re-read the last one (never accumulated:
+            // j + u >= W >= size), so the loads are unconditional
+            const long long e = eo + (long long)min(j + u, W - 1) * kWave;
+            ii[u] = __builtin_nontemporal_load(a.idx + e);
 #pragma unroll
-                for (int w = 0; w < NW; ++w) ww[w][u] = __builtin_nontemporal_load(a.w[w] + e);
-            }
+            for (int w = 0; w < NW; ++w) ww[w][u] = __builtin_nontemporal_load(a.w[w] + e);
         }
         float v[kSliceGroup][LC];
 #pragma unroll
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void k_stencils_rec(RecArgs a)
     const int lane = (int)threadIdx.x & (kWave - 1);
     const int lb = xcd_logical_block();
     if (lb >= a.nblocks) return;
-    const int sl = lb * kWavesPerBlock + (int)threadIdx.x / kWave;
+    const int sl = lb * kWavesPerBlock + wave_in_block();
     if (sl >= a.nslices) return;                     // whole wave idle
     const int s = a.sliceBegin + sl;
     const int row = s * kWave + lane;
@@ -185,13 +185,8 @@ __global__ __launch_bounds__(kBlock) void k_stencils_rec(RecArgs a)
         int ii[G];
         float ww[NWT][G];
 #pragma unroll
-        for (int u = 0; u < G; ++u) {
-            ii[u] = 0;
-#pragma unroll
-            for (int w = 0; w < NWT; ++w) ww[w][u] = 0.0f;
-            if (j + u < W)                           // wave-uniform
-                load_record<NWT, G>(rp + (size_t)(j + u) * (kWave * R), ii[u], ww, u);
-        }
+        for (int u = 0; u < G; ++u)       // past the width: the last column again, not accumulated
+            load_record<NWT, G>(rp + (size_t)min(j + u, W - 1) * (kWave * R), ii[u], ww, u);
         float v[G][LC];
 #pragma unroll
         for (int u = 0; u < G; ++u) {

@@ -15,9 +15,19 @@ struct osdhip_stencil_table {
     long long *sliceOff = nullptr;
     int *sliceWidth = nullptr;
     int *rowSize = nullptr;          // sizes padded to nslices * 64 rows
-    unsigned *rec = nullptr;         // layout 0: records of rec_dwords(nw)
+    unsigned *rec = nullptr;         // layouts 0, 2: records of rec_dwords(nw)
     int *sidx = nullptr;             // layout 1: split streams
     float *sw[6] = {};
+    // layout 2 (locality-ordered slices): slice row i holds Far row perm[i]
+    // (>= n: padding); a slice with nuniq[s] >= 0 lists its distinct control
+    // vertices at ctrl[ctrlOff[s] ...] and its records hold positions in
+    // that list; nuniq[s] < 0: records hold control indices
+    int *perm = nullptr;
+    int *nuniq = nullptr;
+    long long *ctrlOff = nullptr;
+    int *ctrl = nullptr;
+    long long ctrlLen = 0;
+    int listedSlices = 0;
 };
 
 namespace osdhip {

@@ -3,6 +3,7 @@
 #pragma once
 #include "common.h"
 #include "stencil_sliced.h"
+#include "stencil_tiled.h"
 
 namespace osdhip {
 
@@ -78,6 +79,142 @@ __global__ __launch_bounds__(kBlock) void k_fill_split(
         const size_t e = (size_t)(sliceOff[s] + (long long)j * kWave + lane);
         sidx[e] = j < sz ? indices[off + j] : 0;
         for (int k = 0; k < nw; ++k) ds[k][e] = j < sz ? ws[k][off + j] : 0.0f;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Locality-ordered slices (layout 2).
+//
+// Rows are ordered by their dominant control vertex (the entry of largest
+// |weight|, the first one on ties; a stable sort keeps Far order among
+// equal keys), so the 64 rows of a slice surround a few control vertices and
+// share most of their support: 26-38 distinct control vertices per slice on
+// creased cube-spheres at levels 2-4, against 200+ in Far order.  Per slice
+// the distinct control vertices are listed in ascending order; when there
+// are at most kTileZero of them the slice's records hold list positions
+// (LDS slots), otherwise control indices.
+
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+    for (int m = kWave / 2; m; m >>= 1) v = max(v, __shfl_xor(v, m, kWave));
+    return v;
+}
+
+__device__ __forceinline__ unsigned wave_min_u(unsigned v)
+{
+#pragma unroll
+    for (int m = kWave / 2; m; m >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, m, kWave));
+    return v;
+}
+
+// Sort key per slice row: the dominant control vertex (rows past n and
+// empty rows: 0xffffffff, after every real key); value: the row.
+__global__ __launch_bounds__(kBlock) void k_row_dominant(const int *sizes, const int *offsets,
+                                                         const int *indices, const float *w0, int n,
+                                                         int rows, unsigned *keys, int *vals)
+{
+    const int r = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (r >= rows) return;
+    unsigned key = 0xffffffffu;
+    if (r < n && sizes[r] > 0) {
+        const long long off = offsets[r];
+        key = (unsigned)indices[off];
+        float best = fabsf(w0[off]);
+        for (int j = 1; j < sizes[r]; ++j) {
+            const float a = fabsf(w0[off + j]);
+            if (a > best) {
+                best = a;
+                key = (unsigned)indices[off + j];
+            }
+        }
+    }
+    keys[r] = key;
+    vals[r] = r;
+}
+
+// One wave per slice of the ordered rows perm[].  Count pass (FILL false):
+// slice width, record entries (64 * width), distinct control vertices
+// (nuniq = -1 when more than kTileZero) and list length.  Fill pass: the
+// ascending control list and the records -- {list position, weights} for a
+// listed slice (padding: {kTileZero, 0}), {control index, weights} otherwise
+// (padding: {kTilePad, 0}).  Distinct vertices are extracted in ascending
+// order by repeated wave minima over the slice's entries.
+template <int NWT, bool FILL>
+__global__ __launch_bounds__(kBlock) void k_tiled_slices(
+    const int *perm, const int *sizes, const int *offsets, const int *indices, const float *w0,
+    const float *w1, const float *w2, const float *w3, const float *w4, const float *w5, int n, int S,
+    int *width, long long *entries, int *nuniq, long long *listLen, const long long *sliceOff,
+    const long long *ctrlOff, int *ctrl, unsigned *rec)
+{
+    constexpr int R = RecShape<NWT>::R;
+    const int lane = (int)threadIdx.x & (kWave - 1);
+    const int s = (int)blockIdx.x * kWavesPerBlock + wave_in_block();
+    if (s >= S) return;                                // whole wave
+    const int r = perm[(long long)s * kWave + lane];
+    const bool valid = r >= 0 && r < n;
+    const int sz = valid ? sizes[r] : 0;
+    const long long off = valid ? (long long)offsets[r] : 0;
+    const float *ws[6] = { w0, w1, w2, w3, w4, w5 };
+    const int W = FILL ? width[s] : wave_max_i(sz);
+    const bool listed = FILL ? nuniq[s] >= 0 : true;
+    unsigned *rp = FILL ? rec + (size_t)(sliceOff[s] + lane) * R : nullptr;
+    int U = 0;
+    bool over = false;
+    if (listed) {
+        long long last = -1;
+        for (;;) {
+            unsigned m = 0xffffffffu;
+            for (int j = 0; j < sz; ++j) {
+                const unsigned x = (unsigned)indices[off + j];
+                if ((long long)x > last) m = min(m, x);
+            }
+            m = wave_min_u(m);
+            if (m == 0xffffffffu) break;
+            if (U == kTileZero) {
+                over = true;
+                break;
+            }
+            if (FILL) {
+                if (lane == 0) ctrl[ctrlOff[s] + U] = (int)m;
+                for (int j = 0; j < sz; ++j) {
+                    if ((unsigned)indices[off + j] != m) continue;
+                    unsigned *d = rp + (size_t)j * (kWave * R);
+                    d[0] = (unsigned)U;
+#pragma unroll
+                    for (int k = 0; k < NWT; ++k) d[1 + k] = __float_as_uint(ws[k][off + j]);
+#pragma unroll
+                    for (int k = 1 + NWT; k < R; ++k) d[k] = 0u;
+                }
+            }
+            last = m;
+            ++U;
+        }
+    }
+    if (!FILL) {
+        if (lane == 0) {
+            width[s] = W;
+            entries[s] = (long long)W * kWave;
+            nuniq[s] = over ? -1 : U;
+            listLen[s] = over ? 0 : U;
+        }
+        return;
+    }
+    for (int j = 0; j < W; ++j) {
+        unsigned *d = rp + (size_t)j * (kWave * R);
+        if (listed && j < sz) continue;                // written above
+        unsigned q[R];
+#pragma unroll
+        for (int k = 0; k < R; ++k) q[k] = 0u;
+        if (j < sz) {
+            q[0] = (unsigned)indices[off + j];
+#pragma unroll
+            for (int k = 0; k < NWT; ++k) q[1 + k] = __float_as_uint(ws[k][off + j]);
+        } else {
+            q[0] = listed ? (unsigned)kTileZero : kTilePad;
+        }
+#pragma unroll
+        for (int k = 0; k < R; ++k) d[k] = q[k];
     }
 }
 

@@ -193,6 +193,14 @@ class HipStencilTable:
         _lib.load_hip().osdhip_stencil_table_info(self.h, None, None, None, ctypes.byref(b))
         return b.value
 
+    def layout(self):
+        """Device layout: dict(layout, slices, listed_slices, list_entries)."""
+        import ctypes
+        v = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()]
+        _lib.check(_lib.load_hip().osdhip_stencil_table_layout(self.h, *[ctypes.byref(x) for x in v]),
+                   "osdhip_stencil_table_layout")
+        return dict(zip(("layout", "slices", "listed_slices", "list_entries"), (x.value for x in v)))
+
     def __del__(self):
         if getattr(self, "h", None) and _lib._hip is not None:
             _lib._hip.osdhip_stencil_table_destroy(self.h)

@@ -24,14 +24,17 @@ SPECS = [MeshSpec(0, 3, 2, 3, 700, 11), MeshSpec(1, 2, 1, 1, 300, 12),
          MeshSpec(4, 2, 2, 4, 64, 15)]
 
 
-def build(length):
+def build(length, layout=0):
     import torch
-    old = _lib.load_hip().osdhip_config(b"table_csr_mirror", 0)
+    lib = _lib.load_hip()
+    old = lib.osdhip_config(b"table_csr_mirror", 0)
+    old_layout = lib.osdhip_config(b"table_layout", layout)
     try:
         hosts = [compile_mesh(s, length) for s in SPECS]
         dms = [DeviceMesh(h) for h in hosts]
     finally:
-        _lib.load_hip().osdhip_config(b"table_csr_mirror", old)
+        lib.osdhip_config(b"table_csr_mirror", old)
+        lib.osdhip_config(b"table_layout", old_layout)
     for h in hosts:
         h.release()
     torch.cuda.synchronize()
@@ -65,12 +68,20 @@ def same(a, b, what):
             assert np.array_equal(u.view(np.uint32), v.view(np.uint32)), f"{what}: mesh {i} buffer {k}"
 
 
-@pytest.fixture(scope="module", params=[7, 3])
+# (floats per vertex, table layout): record slices and locality-ordered
+# slices (LDS-staged control points)
+@pytest.fixture(scope="module", params=[(7, 0), (3, 0), (7, 2), (3, 2)],
+                ids=["L7-rec", "L3-rec", "L7-tiled", "L3-tiled"])
 def workload(request):
-    length = request.param
-    hosts, dms = build(length)
+    length, layout = request.param
+    hosts, dms = build(length, layout)
+    if layout == 2:
+        for d in dms:
+            info = d.refine_table.layout()
+            assert info["layout"] == 2 and info["listed_slices"] >= 0.9 * info["slices"]
     batch = Batch(length, [s for d in dms for s in d.stencil_segments()],
                   [d.patch_segment() for d in dms])
+    batch.layout = layout
     yield length, hosts, dms, batch
     batch.close()
 
@@ -124,19 +135,26 @@ def test_graph_launch_matches_plain_and_replays(workload):
         batch.run_plain()          # other work on the stream between replays
 
 
-VARIANTS = [dict(a_frames=1, a_group=4), dict(a_frames=1, a_group=8),
-            dict(a_frames=2, a_group=4), dict(a_frames=4, a_group=2),
-            dict(a_frames=0, a_group=4)]
+VARIANTS = {0: [dict(a_frames=1, a_group=4), dict(a_frames=1, a_group=8),
+                dict(a_frames=2, a_group=4), dict(a_frames=4, a_group=2),
+                dict(a_frames=0, a_group=4)],
+            2: [dict(a_frames=1, a_group=4), dict(a_frames=1, a_group=8),
+                dict(a_frames=2, a_group=4), dict(a_frames=2, a_group=8),
+                dict(a_frames=4, a_group=4), dict(a_frames=4, a_group=8)]}
+TUNED_A = {0: 30, 2: 36}
 
 
-@pytest.mark.parametrize("variant", range(len(VARIANTS)))
+@pytest.mark.parametrize("variant", range(6))
 @pytest.mark.parametrize("persistent", [0, 1])
 def test_every_variant_matches_plain(workload, variant, persistent):
     length, hosts, dms, batch = workload
+    variants = VARIANTS[batch.layout]
+    if variant >= len(variants):
+        pytest.skip("no such phase-A variant for this table layout")
     poison(dms)
     batch.run_plain()
     plain = outputs(dms)
-    for k, v in VARIANTS[variant].items():
+    for k, v in variants[variant].items():
         assert batch.set(k, v) == 0
     for slices, bfr in ((1, 1), (3, 2), (16, 4)):
         assert batch.set("a_item_slices", slices) == 0
@@ -145,10 +163,10 @@ def test_every_variant_matches_plain(workload, variant, persistent):
         assert batch.set("b_persistent", persistent) == 0
         poison(dms)
         batch.run(3)
-        same(plain, outputs(dms), f"eager {VARIANTS[variant]} slices {slices} b_frames {bfr}")
+        same(plain, outputs(dms), f"eager {variants[variant]} slices {slices} b_frames {bfr}")
         poison(dms)
         batch.launch()
-        same(plain, outputs(dms), f"graph {VARIANTS[variant]} slices {slices} b_frames {bfr}")
+        same(plain, outputs(dms), f"graph {variants[variant]} slices {slices} b_frames {bfr}")
 
 
 def test_tuned_configuration_matches_plain(workload):
@@ -157,7 +175,7 @@ def test_tuned_configuration_matches_plain(workload):
     batch.run_plain()
     plain = outputs(dms)
     report = batch.tune(reps=1)
-    assert len(report["phase_a"]) == 30 and len(report["phase_b"]) == 6
+    assert len(report["phase_a"]) == TUNED_A[batch.layout] and len(report["phase_b"]) == 6
     poison(dms)
     batch.launch()
     same(plain, outputs(dms), f"tuned {batch.config()}")
@@ -170,7 +188,8 @@ def test_batch_knob_and_segment_errors(workload):
     assert batch.set("b_frames", 8) == _lib.ERR_ARGS
     # a pair that has no kernel instance is refused at launch
     cfg = batch.config()
-    assert batch.set("a_frames", 4) == 0 and batch.set("a_group", 8) == 0
+    bad = (4, 8) if batch.layout == 0 else (0, 4)
+    assert batch.set("a_frames", bad[0]) == 0 and batch.set("a_group", bad[1]) == 0
     with pytest.raises(RuntimeError):
         batch.run(1)
     assert batch.set("a_frames", cfg["a_frames"]) == 0 and batch.set("a_group", cfg["a_group"]) == 0
@@ -191,6 +210,19 @@ def test_batch_knob_and_segment_errors(workload):
     seg["table"] = t3
     with pytest.raises(ValueError):
         Batch(length, [seg], [])
+    # tables of both slice layouts in one batch
+    lib = _lib.load_hip()
+    old = lib.osdhip_config(b"table_layout", 2 - batch.layout)
+    try:
+        t1 = {"sizes": t["sizes"], "offsets": t["offsets"], "indices": t["indices"],
+              "weights": t["weights"]}
+        other = HipStencilTable(t1)
+    finally:
+        lib.osdhip_config(b"table_layout", old)
+    seg2 = dict(seg)
+    seg2["table"] = other
+    with pytest.raises(ValueError):
+        Batch(length, [dms[0].stencil_segments()[0], seg2], [])
 
 
 def test_staged_outputs_share_an_arena():

@@ -15,10 +15,15 @@ SPECS = [MeshSpec(0, 3, 2, 4, 900, 21), MeshSpec(1, 2, 3, 4, 400, 22),
 L = 7
 
 
-@pytest.fixture(scope="module")
-def residents():
+@pytest.fixture(scope="module", params=[0, 2], ids=["rec", "tiled"])
+def residents(request):
     hosts = [compile_mesh(s, L) for s in SPECS]
-    res = residents_from_hosts(hosts)
+    lib = _lib.load_hip()
+    old = lib.osdhip_config(b"table_layout", request.param)
+    try:
+        res = residents_from_hosts(hosts)
+    finally:
+        lib.osdhip_config(b"table_layout", old)
     yield res
     for h in hosts:
         h.release()

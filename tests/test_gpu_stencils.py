@@ -216,9 +216,10 @@ def table_eval(hip, h, src, sd, outs, descs, start, end):
 
 
 # (table_layout, slice_group, stencil_kernel): record slices with gather
-# groups of 8 and 4, split-stream slices, and the CSR kernel on the mirror
+# groups of 8 and 4, split-stream slices, the CSR kernel on the mirror, and
+# locality-ordered slices (random tables: mostly gathers from memory)
 VARIANTS = {"rec8": (0, 8, 0), "rec4": (0, 4, 0), "rec16": (0, 16, 0), "split": (1, 8, 0),
-            "csr": (0, 8, 1)}
+            "csr": (0, 8, 1), "tiled8": (2, 8, 0), "tiled4": (2, 4, 0)}
 
 
 @pytest.mark.parametrize("variant", sorted(VARIANTS))
@@ -262,7 +263,8 @@ def test_device_table_bit_exact(hip, variant, nw, L):
 
 
 def test_device_table_knob_errors(hip):
-    assert hip.osdhip_config(b"table_layout", 2) == _lib.ERR_ARGS
+    assert hip.osdhip_config(b"table_layout", 3) == _lib.ERR_ARGS
+    assert hip.osdhip_config(b"table_layout", -1) == _lib.ERR_ARGS
     assert hip.osdhip_config(b"slice_group", 5) == _lib.ERR_ARGS
     assert hip.osdhip_config(b"slice_group", 32) == _lib.ERR_ARGS
     assert hip.osdhip_config(b"no_such_knob", 0) == _lib.ERR_UNSUPPORTED
@@ -290,3 +292,88 @@ def test_device_table_cfg0_and_errors(hip):
                                     table.GetIndicesBuffer(), table.GetWeightsBuffer(), 0, n,
                                     stream()) == 0
     np.testing.assert_array_equal(host(out2, shape=(n, 3)), g["cpu_out"])
+
+
+def _tiled_table(hip, t):
+    from opensubdiv_amd import HipStencilTable
+    old = hip.osdhip_config(b"table_layout", 2)
+    try:
+        return HipStencilTable.Create(t)
+    finally:
+        hip.osdhip_config(b"table_layout", old)
+
+
+@pytest.mark.parametrize("group", [4, 8])
+def test_tiled_layout_mesh_tables_stage_in_lds(hip, group):
+    """Locality-ordered slices on cube-sphere tables: every slice of the
+    level-2 refinement table lists its control vertices (LDS path), the
+    limit table mixes listed and gathering slices, and P / all six limit outputs,
+    full tables and sub-ranges, equal the reference's CpuEvaluator outputs
+    bit for bit."""
+    old = hip.osdhip_config(b"slice_group", group)
+    try:
+        g = golden("cubesphere_l2.npz")
+        table = _tiled_table(hip, {k: g[k] for k in ("sizes", "offsets", "indices", "weights")})
+        info = table.layout()
+        assert info["layout"] == 2 and info["listed_slices"] == info["slices"] > 0
+        n = table.GetNumStencils()
+        src = dev(g["src"])
+        d7 = empty(n * 7)
+        assert table_eval(hip, table.h, src, (0, 7, 7), [d7], [(0, 7, 7)], 0, n) == 0
+        np.testing.assert_array_equal(host(d7, shape=(n, 7)), g["cpu_out7"])
+        d3 = empty(n * 7, 0.0)
+        assert table_eval(hip, table.h, src, (0, 3, 7), [d3], [(0, 3, 7)], 0, n) == 0
+        o = host(d3, shape=(n, 7))
+        np.testing.assert_array_equal(o[:, :3], g["cpu_out3"])
+        assert np.all(o[:, 3:] == 0)
+        for start, end in [(0, 1), (63, 65), (100, 1500), (1900, n)]:
+            d = empty(n * 7)
+            assert table_eval(hip, table.h, src, (0, 7, 7), [d], [(0, 7, 7)], start, end) == 0
+            o = host(d, shape=(n, 7))
+            np.testing.assert_array_equal(o[:end - start], g["cpu_out7"][start:end])
+            assert np.all(np.isnan(o[end - start:]))
+
+        g = golden("cubesphere_limit.npz")
+        keys = ("weights", "du", "dv", "duu", "duv", "dvv")
+        table = _tiled_table(hip, {k: g[k] for k in ("sizes", "offsets", "indices") + keys})
+        info = table.layout()
+        assert 0 < info["listed_slices"] < info["slices"]      # both paths in one table
+        n = table.GetNumStencils()
+        src = dev(g["src"])
+        outs = [empty(n * 3) for _ in range(6)]
+        assert table_eval(hip, table.h, src, (0, 3, 3), outs, [(0, 3, 3)] * 6, 0, n) == 0
+        for k, name in enumerate(("cpu_p", "cpu_du", "cpu_dv", "cpu_duu", "cpu_duv", "cpu_dvv")):
+            np.testing.assert_array_equal(host(outs[k], shape=(n, 3)), g[name], err_msg=name)
+    finally:
+        hip.osdhip_config(b"slice_group", old)
+
+
+@pytest.mark.parametrize("nw,L", [(1, 3), (1, 8), (3, 5), (6, 2)])
+def test_tiled_layout_random_tables_both_paths(hip, nw, L):
+    """Random tables over 40 control vertices (every slice listed, padding
+    records on the zero slot) and over 5000 (most slices gather from
+    memory, padding records skipped), with empty rows, against the oracle."""
+    keys = ("weights", "du", "dv", "duu", "duv", "dvv")[:nw]
+    for nsrc, listed in ((40, "all"), (5000, "few")):
+        rng = np.random.default_rng(900 + 10 * nw + L + nsrc)
+        n = 3001
+        t = random_table(rng, n, nsrc, 14, nw)
+        table = _tiled_table(hip, t)
+        info = table.layout()
+        if listed == "all":
+            assert info["listed_slices"] == info["slices"]
+        else:
+            assert info["listed_slices"] < info["slices"] // 2
+        src = rng.normal(size=(nsrc, L)).astype(np.float32)
+        s_t = dev(src)
+        for start, end in [(0, n), (17, 2222)]:
+            outs = [empty(n * L) for _ in range(nw)]
+            assert table_eval(hip, table.h, s_t, (0, L, L), outs, [(0, L, L)] * nw, start,
+                              end) == 0
+            for k in range(nw):
+                want = oracle.apply_stencils(src, t["sizes"], t["offsets"], t["indices"],
+                                             t[keys[k]])[start:end]
+                got = host(outs[k], shape=(n, L))
+                np.testing.assert_array_equal(got[:end - start], want,
+                                              err_msg=f"{keys[k]} nsrc {nsrc} [{start},{end})")
+                assert np.all(np.isnan(got[end - start:]))
