@@ -364,17 +364,65 @@ __device__ __forceinline__ void tiled_gather_slice(const GStencilSeg &sg, const 
 // width: the table then leaves memory once per step).
 constexpr int kTiledRegRecords = 16;
 
+// LDS floats per wave of the layout-2 batch kernel: the staging area
+// (FC * LC values of 64 slots) doubles as the store transpose, whose rows
+// have an odd stride so that both its writes and its reads are free of bank
+// conflicts.
+template <int LC, int FC>
+struct TiledArea {
+    static constexpr int Q = FC * LC;                 // values per row and chunk
+    static constexpr int TP = Q | 1;                  // transpose row stride (odd)
+    static constexpr int SEG = Q <= 8 ? 8 : Q <= 16 ? 16 : Q <= 32 ? 32 : 64;   // lanes per row
+    static constexpr int RPI = kWave / SEG;           // rows per store instruction
+    static constexpr int FLOATS = (Q > TP ? Q : TP) * kWave;
+};
+
+// Stores a chunk's results for vertex-major outputs (dstF == LC): each lane
+// parks its row's FC * LC values in LDS, then every store instruction
+// writes RPI rows, SEG consecutive lanes per row covering the row's
+// contiguous run of FC * LC floats.
+template <int LC, int FC>
+__device__ __forceinline__ void tiled_store_runs(const GStencilSeg &sg, float *area,
+                                                 const int *rowsL, const float (&acc)[FC][LC],
+                                                 int f0)
+{
+    using A = TiledArea<LC, FC>;
+    const int lane = (int)threadIdx.x & (kWave - 1);
+    wave_lds_sync();                                   // compute reads of the area are done
+#pragma unroll
+    for (int c = 0; c < FC; ++c)
+#pragma unroll
+        for (int k = 0; k < LC; ++k) area[lane * A::TP + c * LC + k] = acc[c][k];
+    wave_lds_sync();
+    const int q = lane % A::SEG, hi = lane / A::SEG;
+    const int nq = min(FC, sg.frames - f0) * LC;       // valid values per row
+    G_ float *base = sg.dst + (size_t)f0 * LC + q;
+#pragma unroll 4
+    for (int it = 0; it < kWave / A::RPI; ++it) {
+        const int r = it * A::RPI + hi;
+        const int rr = rowsL[r];
+        if (q < nq && rr >= 0) base[(size_t)rr * sg.dstV] = area[r * A::TP + q];
+    }
+}
+
 // One listed layout-2 slice for every frame, FC frames per chunk: the
 // wave stages its listed control vertices' frames into LDS (lane u ->
 // slot u), then accumulates each row's entries from LDS.  RES: the slice's
 // (up to N) records are loaded once into registers; otherwise they are read
 // N per group for every chunk.
 template <int LC, int FC, int N, bool RES, int G>
-__device__ __forceinline__ void tiled_lds_slice(const GStencilSeg &sg, float (*Ls)[kTileSlots],
+__device__ __forceinline__ void tiled_lds_slice(const GStencilSeg &sg, float *area, int *rowsL,
                                                 const G_ unsigned *rp, int W, int U, int vtx,
                                                 int row, bool active)
 {
     const int lane = (int)threadIdx.x & (kWave - 1);
+    float(*Ls)[kTileSlots] = (float(*)[kTileSlots])area;
+    // vertex-major outputs with a row's frames adjacent: results leave
+    // through LDS so that consecutive lanes store one row's run (measured
+    // faster than per-lane stores for that layout; frame-major outputs keep
+    // per-lane stores, which are faster still)
+    const bool runs = sg.dstF == LC && sg.frames > 1;
+    if (runs) rowsL[lane] = active ? row : -1;
     int slot[N];
     float ww[N];
     auto load = [&](int j0) {
@@ -403,6 +451,9 @@ __device__ __forceinline__ void tiled_lds_slice(const GStencilSeg &sg, float (*L
                     for (int k = 0; k < LC; ++k) Ls[c * LC + k][lane] = p[c * sg.srcF + k];
                 }
             }
+        } else if (lane == kTileZero) {                  // the store transpose reuses the area
+#pragma unroll
+            for (int q = 0; q < FC * LC; ++q) Ls[q][kTileZero] = 0.0f;
         }
         wave_lds_sync();
         for (int j0 = 0; j0 < (RES ? 1 : W); j0 += N) {
@@ -420,7 +471,9 @@ __device__ __forceinline__ void tiled_lds_slice(const GStencilSeg &sg, float (*L
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
-        if (active) {
+        if (runs) {
+            tiled_store_runs<LC, FC>(sg, area, rowsL, acc, f0);
+        } else if (active) {
 #pragma unroll
             for (int c = 0; c < FC; ++c) {
                 if (f0 + c >= sg.frames) break;
@@ -442,14 +495,10 @@ __device__ __forceinline__ void tiled_lds_slice(const GStencilSeg &sg, float (*L
 template <int LC, int FC, int G>
 __global__ __launch_bounds__(kBlock) void k_batch_stencils_tiled(BatchArgs a)
 {
-    __shared__ float lds[kWavesPerBlock][FC * LC][kTileSlots];
+    __shared__ float lds[kWavesPerBlock][TiledArea<LC, FC>::FLOATS];
+    __shared__ int rowsLds[kWavesPerBlock][kWave];
     const int lane = (int)threadIdx.x & (kWave - 1);
     const int wv = wave_in_block();
-    float(*Ls)[kTileSlots] = lds[wv];
-    if (lane == kTileZero) {
-#pragma unroll
-        for (int q = 0; q < FC * LC; ++q) Ls[q][kTileZero] = 0.0f;
-    }
     for (int iter = 0;; ++iter) {
         const int itx = batch_next_item(a, iter);
         if (itx < 0) break;
@@ -470,9 +519,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils_tiled(BatchArgs a)
             }
             const int vtx = lane < U ? sg.ctrl[sg.ctrlOff[s] + lane] : 0;
             if (W <= kTiledRegRecords)
-                tiled_lds_slice<LC, FC, kTiledRegRecords, true, G>(sg, Ls, rp, W, U, vtx, row, active);
+                tiled_lds_slice<LC, FC, kTiledRegRecords, true, G>(sg, lds[wv], rowsLds[wv], rp, W,
+                                                                   U, vtx, row, active);
             else
-                tiled_lds_slice<LC, FC, G, false, G>(sg, Ls, rp, W, U, vtx, row, active);
+                tiled_lds_slice<LC, FC, G, false, G>(sg, lds[wv], rowsLds[wv], rp, W, U, vtx, row,
+                                                     active);
         }
     }
 }

@@ -18,8 +18,9 @@ of 4 floats (element (v, f, k) at (v * F + f) * Lp + k, Lp = 8 for L = 7:
 aligned 32-byte records, four frames of a vertex in one 128-byte line for
 the gathers), frame-major
 refinement and P / dPdu / dPdv outputs (element (r, f, k) at
-(f * rows + r) * L + k: a wave writes contiguous runs), and the segments a
-``Batch`` (or the one-mesh-per-launch path) evaluates.
+(f * rows + r) * L + k: rows of one frame adjacent, as in per-frame vertex
+buffers; measured faster than interleaving a row's frames), and the
+segments a ``Batch`` (or the one-mesh-per-launch path) evaluates.
 """
 from __future__ import annotations
 

@@ -169,6 +169,51 @@ def test_every_variant_matches_plain(workload, variant, persistent):
         same(plain, outputs(dms), f"graph {variants[variant]} slices {slices} b_frames {bfr}")
 
 
+def test_vertex_major_refined_outputs(workload):
+    """Refined outputs with a row's frames adjacent (element (r, f, k) at
+    (r * F + f) * L + k; the layout-2 kernel stores them through LDS as row
+    runs) equal the plain path and the frame-major results, for every
+    frames-per-chunk variant."""
+    import torch
+    length, hosts, dms, batch = workload
+    poison(dms)
+    batch.run_plain()
+    ref = outputs(dms)
+    segs, outs = [], []
+    for d in dms:
+        s0, s1 = d.stencil_segments()
+        F = d.spec.frames
+        o = torch.full((d.nrefine * F * length,), float("nan"), device="cuda")
+        segs += [dict(s0, dst=o, dst_v=F * length, dst_f=length), s1]
+        outs.append(o)
+    b2 = Batch(length, segs, [])
+    try:
+        for fc in (1, 2, 4):
+            assert b2.set("a_frames", fc) == 0
+            assert b2.set("a_group", 2 if fc == 4 and batch.layout == 0 else 4) == 0
+            for o in outs:
+                o.fill_(float("nan"))
+            b2.run(1)
+            torch.cuda.synchronize()
+            for d, o, r in zip(dms, outs, ref):
+                F = d.spec.frames
+                got = o.cpu().numpy().reshape(-1, F, length)
+                want = r[0].reshape(F, -1, length).transpose(1, 0, 2)
+                assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), \
+                    f"mesh {d.spec.index} frames per chunk {fc}"
+        for o in outs:
+            o.fill_(float("nan"))
+        b2.run_plain()
+        torch.cuda.synchronize()
+        for d, o, r in zip(dms, outs, ref):
+            F = d.spec.frames
+            want = r[0].reshape(F, -1, length).transpose(1, 0, 2)
+            assert np.array_equal(o.cpu().numpy().reshape(-1, F, length).view(np.uint32),
+                                  want.view(np.uint32))
+    finally:
+        b2.close()
+
+
 def test_tuned_configuration_matches_plain(workload):
     length, hosts, dms, batch = workload
     poison(dms)
