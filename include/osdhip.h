@@ -227,7 +227,7 @@ const char *osdhip_version(void);
 typedef struct osdhip_batch osdhip_batch;
 
 typedef struct {
-    const osdhip_stencil_table *table;   /* one weight array, record layout */
+    const osdhip_stencil_table *table;   /* one weight array; layout 0 or 2 (all segments alike) */
     const float *src;
     long long srcVertexStride, srcFrameStride;
     float *dst;                          /* row r of the table -> element r */
@@ -256,8 +256,10 @@ typedef struct {
  * can stream them through one arena, each output still written once per
  * launch).  osdhip_batch_epoch ignores `stage` (an epoch is one stage). */
 
-/* NULL on invalid segments (tables not in the record layout or with more
- * than one weight array, strides below `length`) or allocation failure. */
+/* NULL on invalid segments (tables in the split-stream layout, tables of
+ * both slice layouts, more than one weight array, strides below `length`)
+ * or allocation failure.  For every patch segment the batch lists, per 64
+ * coords, the distinct points of its regular patches (staged phase B). */
 osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
                                   const osdhip_batch_stencils *stencils,
                                   int numPatchSegments, const osdhip_batch_patches *patches,
@@ -275,14 +277,17 @@ int osdhip_batch_num_stages(const osdhip_batch *batch);
  * path), for comparison: every stencil segment, then every patch segment. */
 int osdhip_batch_run_plain(osdhip_batch *batch, void *stream);
 int osdhip_batch_run_plain_stage(osdhip_batch *batch, int stage, void *stream);
-/* Variant knobs: "a_frames" x "a_group" in {1x4, 1x8, 2x4, 4x2} (frames
- * per chunk, gather group; other pairs make launches return
- * OSDHIP_ERR_UNSUPPORTED) or a_frames 0 (a slice's records stay in
- * registers for all frames; slices wider than 16 entries take the chunked
- * path), "a_item_slices" (64-row slices per work item),
- * "a_persistent" / "b_persistent" (0 = one block per item, 1 = fixed grid
- * claiming items from a device counter), "b_frames" in {1, 2, 4}.  The
- * graph is re-captured on the next launch. */
+/* Variant knobs: "a_frames" x "a_group" (frames per chunk, gather group;
+ * layout-0 tables: {1x4, 1x8, 2x4, 4x2} or a_frames 0 = a slice's records
+ * stay in registers for all frames; layout-2 tables: {1, 2, 4} x {4, 8};
+ * other pairs make launches return OSDHIP_ERR_UNSUPPORTED),
+ * "a_item_slices" (64-row slices per work item), "a_persistent" /
+ * "b_persistent" (0 = one block per item, 1 = fixed grid claiming items
+ * from a device counter), "b_frames" in {1, 2, 4}, "b_stage" (1 = waves
+ * whose regular-patch coords use at most 256 distinct points read them
+ * from LDS, 0 = gathers only).  The graph is re-captured on the next
+ * launch.  osdhip_batch_get also reads "staged_coords": coords in waves
+ * that can be staged. */
 int osdhip_batch_set(osdhip_batch *batch, const char *key, int value);
 int osdhip_batch_get(const osdhip_batch *batch, const char *key);
 int osdhip_batch_info(const osdhip_batch *batch, long long *stencilRows, long long *frameRows,

@@ -32,7 +32,8 @@ def _p(t, off=0):
 
 
 class Batch:
-    KNOBS = ("a_frames", "a_group", "a_item_slices", "a_persistent", "b_frames", "b_persistent")
+    KNOBS = ("a_frames", "a_group", "a_item_slices", "a_persistent", "b_frames", "b_persistent",
+             "b_stage")
 
     def __init__(self, length, stencils=(), patches=(), stream=None):
         self._lib = _lib.load_hip()

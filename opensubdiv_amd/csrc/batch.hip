@@ -6,6 +6,7 @@
 // persistent scheduling); osdhip_batch_tune times the variants on the
 // batch's own data and keeps the fastest.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstdio>
@@ -25,7 +26,7 @@ constexpr int kCounters = kNumXcd;      // claim counters per (stage, phase)
 
 struct BatchConfig {
     int aFrames = 2, aGroup = 4, aItemSlices = 8, aPersistent = 0;
-    int bFrames = 1, bPersistent = 0;
+    int bFrames = 1, bPersistent = 0, bStage = 1;
 };
 
 struct osdhip_batch {
@@ -42,12 +43,18 @@ struct osdhip_batch {
     int nStages = 1;
     BatchItem *itemsA = nullptr, *itemsB = nullptr;
     int nItemsA = 0, nItemsB = 0, itemsASlices = 0;
-    std::vector<int> beginA, countA, beginB, countB;    // item ranges per stage
+    std::vector<int> beginA, countA;       // phase-A item ranges per stage
+    // phase-B item ranges per (launch kind, stage): each launch visits only
+    // the items with waves of its kind
+    std::vector<int> beginB[5], countB[5];
+    std::vector<std::vector<int>> hNuniq, hNgen;   // per patch segment, while building
     unsigned *counters = nullptr;          // kCounters per (stage, phase)
     int numCUs = 256;
     BatchConfig cfg;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<void *> prep;              // patch point staging lists
+    long long stagedCoords = 0;            // coords in listed waves
 };
 
 namespace {
@@ -137,40 +144,126 @@ bool launchA_variant(bool tiled, int lc, int fc, int g, const BatchArgs &a, int 
     return false;
 }
 
-// phase B: the regular-interior kernel, then the general one (other coords);
-// both claim the same items, so the persistent counters of the second
-// launch are reset between them
-template <int LC, int FC>
-void launchB(const BatchArgs &a, int grid, hipStream_t st)
+// Phase-B launch kinds, each over its own item list: every regular-interior
+// coord gathering (kB_All, b_stage 0), or waves staged in 128 / 256 LDS
+// slots and the remaining regular waves (b_stage 1); then the other coords
+// (kB_Gen).
+enum { kB_All = 0, kB_S128 = 1, kB_S256 = 2, kB_Unl = 3, kB_Gen = 4, kBKinds = 5 };
+
+struct PhaseB {
+    BatchArgs a[kBKinds];
+    int grid[kBKinds];
+};
+
+// persistent launches reset their claim counters first
+template <class K>
+void launchKind(const PhaseB &L, int kind, K kernel, hipStream_t st)
 {
-    hipLaunchKernelGGL((k_batch_patches_regular<LC, FC>), dim3(grid), dim3(kBlock), 0, st, a);
+    const BatchArgs &a = L.a[kind];
+    if (a.nitems == 0) return;
     if (a.persistent)
         hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, a.counter, kNumXcd);
-    hipLaunchKernelGGL((k_batch_patches<LC, FC>), dim3(grid), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(kernel, dim3(L.grid[kind]), dim3(kBlock), 0, st, a);
+}
+
+template <int LC, int FC>
+void launchB(const PhaseB &L, bool staged, hipStream_t st)
+{
+    if (staged) {
+        launchKind(L, kB_S128, k_batch_patches_regular<LC, FC, kStagedOnly, 128>, st);
+        // 256 slots at most 2 frames per chunk (LDS: 4 waves x FC x LC x 256 floats)
+        launchKind(L, kB_S256, k_batch_patches_regular<LC, (FC > 2 ? 2 : FC), kStagedOnly, 256>, st);
+        launchKind(L, kB_Unl, k_batch_patches_regular<LC, FC, kGatherUnlisted>, st);
+    } else {
+        launchKind(L, kB_All, k_batch_patches_regular<LC, FC, kGatherAll>, st);
+    }
+    // the few other coords (Gregory, boundary, bilinear): one frame per chunk
+    // keeps the general kernel's register budget lower
+    launchKind(L, kB_Gen, k_batch_patches<LC, 1>, st);
 }
 
 template <int FC>
-bool launchB_lc(int lc, const BatchArgs &a, int grid, hipStream_t st)
+bool launchB_lc(int lc, const PhaseB &L, bool staged, hipStream_t st)
 {
     switch (lc) {
-    case 1: launchB<1, FC>(a, grid, st); return true;
-    case 2: launchB<2, FC>(a, grid, st); return true;
-    case 3: launchB<3, FC>(a, grid, st); return true;
-    case 4: launchB<4, FC>(a, grid, st); return true;
-    case 5: launchB<5, FC>(a, grid, st); return true;
-    case 6: launchB<6, FC>(a, grid, st); return true;
-    case 7: launchB<7, FC>(a, grid, st); return true;
-    case 8: launchB<8, FC>(a, grid, st); return true;
+    case 1: launchB<1, FC>(L, staged, st); return true;
+    case 2: launchB<2, FC>(L, staged, st); return true;
+    case 3: launchB<3, FC>(L, staged, st); return true;
+    case 4: launchB<4, FC>(L, staged, st); return true;
+    case 5: launchB<5, FC>(L, staged, st); return true;
+    case 6: launchB<6, FC>(L, staged, st); return true;
+    case 7: launchB<7, FC>(L, staged, st); return true;
+    case 8: launchB<8, FC>(L, staged, st); return true;
     default: return false;
     }
 }
 
-bool launchB_variant(int lc, int fc, const BatchArgs &a, int grid, hipStream_t st)
+bool launchB_variant(int lc, int fc, bool staged, const PhaseB &L, hipStream_t st)
 {
-    if (fc == 1) return launchB_lc<1>(lc, a, grid, st);
-    if (fc == 2) return launchB_lc<2>(lc, a, grid, st);
-    if (fc == 4) return launchB_lc<4>(lc, a, grid, st);
+    if (fc == 1) return launchB_lc<1>(lc, L, staged, st);
+    if (fc == 2) return launchB_lc<2>(lc, L, staged, st);
+    if (fc == 4) return launchB_lc<4>(lc, L, staged, st);
     return false;
+}
+
+// Lists, per 64-coord wave of a patch segment, the distinct points of its
+// regular-interior coords and each coord's 16 slot bytes (k_patch_slots
+// count pass, scan, fill pass); the lists stay with the batch.
+bool preparePatchStaging(osdhip_batch *b, BatchPatchSeg &g)
+{
+    const int nw = (g.ncoords + kWave - 1) / kWave;
+    if (nw == 0) return true;
+    int *nuniq = nullptr, *pts = nullptr, *ngen = nullptr;
+    long long *listLen = nullptr, *ptsOff = nullptr;
+    unsigned *slots = nullptr;
+    void *tmp = nullptr;
+    size_t tmpBytes = 0;
+    const int grid = (nw + kWavesPerBlock - 1) / kWavesPerBlock;
+    bool ok = hipMalloc(&nuniq, (size_t)nw * sizeof(int)) == hipSuccess &&
+              hipMalloc(&ngen, (size_t)nw * sizeof(int)) == hipSuccess &&
+              hipMalloc(&listLen, (size_t)nw * sizeof(long long)) == hipSuccess &&
+              hipMalloc(&ptsOff, (size_t)nw * sizeof(long long)) == hipSuccess &&
+              hipMalloc(&slots, (size_t)g.ncoords * 4 * sizeof(unsigned)) == hipSuccess;
+    long long total = 0;
+    if (ok) {
+        hipLaunchKernelGGL(k_patch_slots<false>, dim3(grid), dim3(kBlock), 0, 0, g, nuniq, listLen,
+                           ptsOff, pts, slots, ngen);
+        ok = hipGetLastError() == hipSuccess &&
+             hipcub::DeviceScan::ExclusiveSum(nullptr, tmpBytes, listLen, ptsOff, nw) == hipSuccess &&
+             hipMalloc(&tmp, std::max<size_t>(tmpBytes, 1)) == hipSuccess &&
+             hipcub::DeviceScan::ExclusiveSum(tmp, tmpBytes, listLen, ptsOff, nw) == hipSuccess;
+    }
+    long long lastOff = 0, lastLen = 0;
+    std::vector<int> nu((size_t)nw), ng((size_t)nw);
+    if (ok)
+        ok = hipMemcpy(&lastOff, ptsOff + (nw - 1), sizeof(long long), hipMemcpyDeviceToHost) ==
+                 hipSuccess &&
+             hipMemcpy(&lastLen, listLen + (nw - 1), sizeof(long long), hipMemcpyDeviceToHost) ==
+                 hipSuccess &&
+             hipMemcpy(nu.data(), nuniq, (size_t)nw * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess &&
+             hipMemcpy(ng.data(), ngen, (size_t)nw * sizeof(int), hipMemcpyDeviceToHost) == hipSuccess;
+    total = lastOff + lastLen;
+    if (ok && total > 0) ok = hipMalloc(&pts, (size_t)total * sizeof(int)) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_patch_slots<true>, dim3(grid), dim3(kBlock), 0, 0, g, nuniq, listLen,
+                           ptsOff, pts, slots, ngen);
+        ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+    }
+    if (tmp) (void)hipFree(tmp);
+    if (listLen) (void)hipFree(listLen);
+    for (void *p : { (void *)nuniq, (void *)ptsOff, (void *)pts, (void *)slots, (void *)ngen })
+        if (p) b->prep.push_back(p);
+    if (!ok) return false;
+    for (int w = 0; w < nw; ++w)
+        if (nu[w] >= 0) b->stagedCoords += std::min(kWave, g.ncoords - w * kWave);
+    b->hNuniq.push_back(std::move(nu));
+    b->hNgen.push_back(std::move(ng));
+    g.nuniq = nuniq;
+    g.ptsOff = ptsOff;
+    g.pts = pts;
+    g.slots = slots;
+    g.ngen = ngen;
+    return true;
 }
 
 // (Re)builds the phase-A item list for cfg.aItemSlices.
@@ -241,16 +334,21 @@ int enqueuePhaseA(osdhip_batch *b, int stage, hipStream_t st)
 
 int enqueuePhaseB(osdhip_batch *b, int stage, hipStream_t st)
 {
-    const int n = b->countB[stage];
-    if (n == 0) return OSDHIP_OK;
-    BatchArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.pseg = b->dP;
-    a.items = b->itemsB + b->beginB[stage];
-    a.nitems = n;
-    a.persistent = b->cfg.bPersistent;
-    a.counter = b->counters + kCounters * (2 * stage + 1);
-    if (!launchB_variant(b->length, b->cfg.bFrames, a, gridFor(b, n, a.persistent != 0), st))
+    PhaseB L;
+    std::memset(&L, 0, sizeof(L));
+    int total = 0;
+    for (int k = 0; k < kBKinds; ++k) {
+        BatchArgs &a = L.a[k];
+        a.pseg = b->dP;
+        a.items = b->itemsB + b->beginB[k][stage];
+        a.nitems = b->countB[k][stage];
+        a.persistent = b->cfg.bPersistent;
+        a.counter = b->counters + kCounters * (2 * stage + 1);
+        L.grid[k] = gridFor(b, a.nitems, a.persistent != 0);
+        total += a.nitems;
+    }
+    if (total == 0) return OSDHIP_OK;
+    if (!launchB_variant(b->length, b->cfg.bFrames, b->cfg.bStage != 0, L, st))
         return OSDHIP_ERR_UNSUPPORTED;
     return last_error();
 }
@@ -307,6 +405,8 @@ void freeBatch(osdhip_batch *b)
     dropGraph(b);
     void *p[] = { b->dS, b->dP, b->itemsA, b->itemsB, b->counters };
     for (void *x : p)
+        if (x) (void)hipFree(x);
+    for (void *x : b->prep)
         if (x) (void)hipFree(x);
     delete b;
 }
@@ -411,19 +511,40 @@ osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
     }
     b->nS = (int)b->hS.size();
     b->nP = (int)b->hP.size();
-    b->beginB.assign(b->nStages, 0);
-    b->countB.assign(b->nStages, 0);
-    for (int st = 0; st < b->nStages; ++st) {
-        b->beginB[st] = (int)itemsB.size();
-        for (int seg = 0; seg < b->nP; ++seg) {
-            if (b->pStage[seg] != st) continue;
-            const int nc = b->hP[seg].ncoords;
-            for (int c = 0; c < nc; c += kBlock)
-                itemsB.push_back({ seg, c, std::min(kBlock, nc - c), 0 });
+    // per patch segment: point lists of its 64-coord waves (staged phase B)
+    for (int i = 0; i < b->nP; ++i) {
+        if (!preparePatchStaging(b, b->hP[i])) {
+            freeBatch(b);
+            return nullptr;
         }
-        b->countB[st] = (int)itemsB.size() - b->beginB[st];
+    }
+    // phase-B item lists per launch kind and stage, from the per-wave counts
+    // of the staging pass (an item: kBlock coords = 4 waves)
+    for (int k = 0; k < kBKinds; ++k) {
+        b->beginB[k].assign(b->nStages, 0);
+        b->countB[k].assign(b->nStages, 0);
+        for (int st = 0; st < b->nStages; ++st) {
+            b->beginB[k][st] = (int)itemsB.size();
+            for (int seg = 0; seg < b->nP; ++seg) {
+                if (b->pStage[seg] != st) continue;
+                const int nc = b->hP[seg].ncoords;
+                const std::vector<int> &nu = b->hNuniq[seg], &ng = b->hNgen[seg];
+                for (int c = 0; c < nc; c += kBlock) {
+                    bool take = k == kB_All;
+                    for (int w = c / kWave; !take && w < (std::min(nc, c + kBlock) + kWave - 1) / kWave; ++w)
+                        take = k == kB_S128 ? nu[w] > 0 && nu[w] <= 128
+                             : k == kB_S256 ? nu[w] > 128
+                             : k == kB_Unl ? nu[w] < 0
+                             : ng[w] > 0;
+                    if (take) itemsB.push_back({ seg, c, std::min(kBlock, nc - c), 0 });
+                }
+            }
+            b->countB[k][st] = (int)itemsB.size() - b->beginB[k][st];
+        }
     }
     b->nItemsB = (int)itemsB.size();
+    b->hNuniq.clear();
+    b->hNgen.clear();
     bool ok = hipMalloc(&b->counters, kCounters * 2 * (size_t)b->nStages * sizeof(unsigned)) ==
               hipSuccess;
     if (ok && b->nS)
@@ -540,6 +661,7 @@ int osdhip_batch_set(osdhip_batch *b, const char *key, int value)
     else if (!std::strcmp(key, "a_persistent")) c.aPersistent = value;
     else if (!std::strcmp(key, "b_frames")) c.bFrames = value;
     else if (!std::strcmp(key, "b_persistent")) c.bPersistent = value;
+    else if (!std::strcmp(key, "b_stage")) c.bStage = value;
     else return OSDHIP_ERR_UNSUPPORTED;
     // each knob within its own domain; the (frames, group) pair is checked
     // when the phase is launched
@@ -548,7 +670,7 @@ int osdhip_batch_set(osdhip_batch *b, const char *key, int value)
         c.aItemSlices > 1024 ||
         (c.aPersistent != 0 && c.aPersistent != 1) ||
         (c.bFrames != 1 && c.bFrames != 2 && c.bFrames != 4) ||
-        (c.bPersistent != 0 && c.bPersistent != 1))
+        (c.bPersistent != 0 && c.bPersistent != 1) || (c.bStage != 0 && c.bStage != 1))
         return OSDHIP_ERR_ARGS;
     b->cfg = c;
     dropGraph(b);          // re-captured on the next launch
@@ -564,6 +686,9 @@ int osdhip_batch_get(const osdhip_batch *b, const char *key)
     if (!std::strcmp(key, "a_persistent")) return b->cfg.aPersistent;
     if (!std::strcmp(key, "b_frames")) return b->cfg.bFrames;
     if (!std::strcmp(key, "b_persistent")) return b->cfg.bPersistent;
+    if (!std::strcmp(key, "b_stage")) return b->cfg.bStage;
+    if (!std::strcmp(key, "staged_coords"))
+        return (int)std::min<long long>(b->stagedCoords, 0x7fffffffLL);
     return OSDHIP_ERR_UNSUPPORTED;
 }
 
@@ -652,21 +777,25 @@ int osdhip_batch_tune(osdhip_batch *b, int reps, void *stream, char *report, int
         rep += b->nS ? ",\"phase_b\":[" : "\"phase_b\":[";
         bool first = true;
         for (int fc : fcs)
-            for (int pers = 0; rc == OSDHIP_OK && pers < 2; ++pers) {
-                b->cfg.bFrames = fc;
-                b->cfg.bPersistent = pers;
-                float t = 0.0f;
-                rc = timePhase(2, t);
-                if (rc != OSDHIP_OK) break;
-                std::snprintf(buf, sizeof(buf), "%s{\"frames\":%d,\"persistent\":%d,\"ms\":%.4f}",
-                              first ? "" : ",", fc, pers, t);
-                rep += buf;
-                first = false;
-                if (t < bestms) { bestms = t; bestc = b->cfg; }
-            }
+            for (int pers = 0; rc == OSDHIP_OK && pers < 2; ++pers)
+                for (int stg = 0; rc == OSDHIP_OK && stg < 2; ++stg) {
+                    b->cfg.bFrames = fc;
+                    b->cfg.bPersistent = pers;
+                    b->cfg.bStage = stg;
+                    float t = 0.0f;
+                    rc = timePhase(2, t);
+                    if (rc != OSDHIP_OK) break;
+                    std::snprintf(buf, sizeof(buf),
+                                  "%s{\"frames\":%d,\"persistent\":%d,\"stage\":%d,\"ms\":%.4f}",
+                                  first ? "" : ",", fc, pers, stg, t);
+                    rep += buf;
+                    first = false;
+                    if (t < bestms) { bestms = t; bestc = b->cfg; }
+                }
         rep += "]";
         b->cfg.bFrames = bestc.bFrames;
         b->cfg.bPersistent = bestc.bPersistent;
+        b->cfg.bStage = bestc.bStage;
     }
     rep += "}";
     (void)hipEventDestroy(e0);
@@ -821,11 +950,17 @@ int osdhip_batch_epoch(int length, int numStencilSegments, const osdhip_batch_st
         if (rc != OSDHIP_OK) return rc;
     }
     if (nb) {
-        std::memset(&a, 0, sizeof(a));
-        a.pseg = (const BatchPatchSeg *)(d + p.segB);
-        a.items = (const BatchItem *)(d + p.itemsB);
-        a.nitems = nb;
-        if (!launchB_variant(length, 1, a, grid_for_blocks(nb), st)) return OSDHIP_ERR_UNSUPPORTED;
+        // every item for the gathering regular launch and the general one
+        PhaseB L;
+        std::memset(&L, 0, sizeof(L));
+        for (int k : { (int)kB_All, (int)kB_Gen }) {
+            L.a[k].pseg = (const BatchPatchSeg *)(d + p.segB);
+            L.a[k].items = (const BatchItem *)(d + p.itemsB);
+            L.a[k].nitems = nb;
+            L.grid[k] = grid_for_blocks(nb);
+        }
+        if (!launchB_variant(length, 1, false, L, st))
+            return OSDHIP_ERR_UNSUPPORTED;
         int rc = last_error();
         if (rc != OSDHIP_OK) return rc;
     }

@@ -60,6 +60,15 @@ struct BatchPatchSeg {
     float *out[3];                // P, dPdu, dPdv (NULL = skip)
     long long outV, outF;
     int ncoords, frames;
+    // point staging of regular-interior coords, per wave of 64 coords
+    // (NULL: gathers only): nuniq[w] distinct points listed at
+    // pts[ptsOff[w] ...] (nuniq < 0: more than kPatchSlots), and per coord
+    // the list positions of its 16 points, one byte each
+    const int *nuniq;
+    const long long *ptsOff;
+    const int *pts;
+    const unsigned *slots;        // 4 words per coord
+    const int *ngen;              // per wave: coords that are not regular-interior
 };
 
 // Device views of the segments with global-qualified pointers (the
@@ -92,6 +101,11 @@ struct GPatchSeg {
     G_ float *out[3];
     long long outV, outF;
     int ncoords, frames;
+    const G_ int *nuniq;
+    const G_ long long *ptsOff;
+    const G_ int *pts;
+    const G_ unsigned *slots;
+    const G_ int *ngen;
 };
 
 #define C_ OSDHIP_CONST
@@ -133,6 +147,11 @@ __device__ __forceinline__ GPatchSeg gseg(const C_ BatchPatchSeg &s)
     g.outF = s.outF;
     g.ncoords = s.ncoords;
     g.frames = s.frames;
+    g.nuniq = gptr(s.nuniq);
+    g.ptsOff = gptr(s.ptsOff);
+    g.pts = gptr(s.pts);
+    g.slots = gptr(s.slots);
+    g.ngen = gptr(s.ngen);
     return g;
 }
 
@@ -629,6 +648,85 @@ __global__ __launch_bounds__(kBlock) void k_batch_stencils_rr(BatchArgs a)
     }
 }
 
+// Most points listed per wave of 64 coords (slot bytes: up to 256); the
+// staged regular-patch kernel comes in 128- and 256-slot variants.
+constexpr int kPatchSlots = 256;
+
+// One wave per 64 coords of a patch segment (coords [64 w, 64 w + 64)):
+// count pass (FILL false) -- distinct points of the wave's regular-interior
+// coords (nuniq = -1 when more than kPatchSlots), list length and the
+// number of other coords (ngen); fill pass -- the ascending point list and
+// each regular-interior coord's 16 slot bytes.
+// Distinct points are extracted in ascending order by repeated wave minima.
+template <bool FILL>
+__global__ __launch_bounds__(kBlock) void k_patch_slots(BatchPatchSeg sg, int *nuniq,
+                                                        long long *listLen, const long long *ptsOff,
+                                                        int *pts, unsigned *slots, int *ngen)
+{
+    const int lane = (int)threadIdx.x & (kWave - 1);
+    const int w = (int)blockIdx.x * kWavesPerBlock + wave_in_block();
+    const int nw = (sg.ncoords + kWave - 1) / kWave;
+    if (w >= nw) return;                                  // whole wave
+    if (FILL && nuniq[w] < 0) return;
+    const int i = w * kWave + lane;
+    bool reg = false;
+    int idx[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) idx[j] = 0;
+    if (i < sg.ncoords) {
+        const osdhip_patch_coord c = sg.coords[i];
+        const osdhip_patch_array arr = sg.arrays[c.arrayIndex];
+        const unsigned f1 = sg.params[c.patchIndex].field1;
+        const int type = ((f1 >> 5) & 1u) ? arr.regDesc : arr.desc;
+        reg = type == OSDHIP_PATCH_REGULAR && ((f1 >> 7) & 0x1fu) == 0;
+        if (reg) {
+            const int *cvs = sg.indices + arr.indexBase + arr.stride * (c.patchIndex - arr.primitiveIdBase);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) idx[j] = cvs[j];
+        }
+    }
+    unsigned sw[4] = { 0u, 0u, 0u, 0u };
+    long long last = -1;
+    int U = 0;
+    bool over = false;
+    for (;;) {
+        unsigned m = 0xffffffffu;
+        if (reg) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if ((long long)(unsigned)idx[j] > last) m = min(m, (unsigned)idx[j]);
+        }
+        m = wave_min_u(m);
+        if (m == 0xffffffffu) break;
+        if (U == kPatchSlots) {
+            over = true;
+            break;
+        }
+        if (FILL) {
+            if (lane == 0) pts[ptsOff[w] + U] = (int)m;
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (reg && (unsigned)idx[j] == m) sw[j / 4] |= (unsigned)U << (8 * (j % 4));
+        }
+        last = m;
+        ++U;
+    }
+    if (!FILL) {
+        const bool gen = i < sg.ncoords && !reg;
+        const int ng = __popcll(__ballot(gen));
+        if (lane == 0) {
+            nuniq[w] = over ? -1 : U;
+            listLen[w] = over ? 0 : U;
+            ngen[w] = ng;
+        }
+        return;
+    }
+    if (reg) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) slots[(size_t)i * 4 + r] = sw[r];
+    }
+}
+
 // acc[o][q][k] += x * w_o[j] for frames q of a chunk, reference order.
 template <int LC, int FC>
 __device__ __forceinline__ void batch_patch_point(const G_ float *p, long long fstride, int nq,
@@ -663,39 +761,159 @@ __device__ __forceinline__ float pick4(const float (&v)[4], int r)
     return r == 0 ? v[0] : r == 1 ? v[1] : r == 2 ? v[2] : v[3];
 }
 
+// A regular-interior coord (REGULAR type, no boundary) and its parameters.
+struct RegularCoord {
+    bool ok;
+    float s, t, k1;
+    const G_ int *cvs;
+};
+
+__device__ __forceinline__ RegularCoord regular_coord(const GPatchSeg &sg, int i)
+{
+    RegularCoord rc;
+    const osdhip_patch_coord c = ld(sg.coords + i);
+    const osdhip_patch_array arr = ld(sg.arrays + c.arrayIndex);
+    const osdhip_patch_param pp = ld(sg.params + c.patchIndex);
+    const unsigned f1 = pp.field1;
+    const int type = ((f1 >> 5) & 1u) ? arr.regDesc : arr.desc;
+    const int boundary = (int)((f1 >> 7) & 0x1fu);
+    rc.ok = type == OSDHIP_PATCH_REGULAR && boundary == 0;
+    const int depth = (int)(f1 & 0xfu);
+    const int nonquad = (int)((f1 >> 4) & 1u);
+    const int pv = (int)((f1 >> 12) & 0x3ffu);
+    const int pu = (int)((f1 >> 22) & 0x3ffu);
+    const float frac_inv = (float)(1 << (depth - nonquad));
+    rc.s = c.s * frac_inv - (float)pu;
+    rc.t = c.t * frac_inv - (float)pv;
+    rc.k1 = (float)(1 << depth);
+    rc.cvs = sg.indices + arr.indexBase + arr.stride * (c.patchIndex - arr.primitiveIdBase);
+    return rc;
+}
+
 template <int LC, int FC>
+__device__ __forceinline__ void store_patch_chunk(const GPatchSeg &sg, int i, int f0, int nq,
+                                                  float k1, const float (&acc)[3][FC][LC])
+{
+#pragma unroll
+    for (int q = 0; q < FC; ++q) {
+        if (q >= nq) break;
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            G_ float *d = sg.out[o];
+            if (!d) continue;
+            const float f = o == 0 ? 1.0f : k1;
+            d += (size_t)i * sg.outV + (size_t)(f0 + q) * sg.outF;
+#pragma unroll
+            for (int k = 0; k < LC; ++k) d[k] = acc[o][q][k] * f;
+        }
+    }
+}
+
+// Waves of 64 coords in three modes, each launch with its own register
+// budget: kGatherAll gathers every regular coord's points from memory;
+// kStagedOnly takes the waves whose coords reference at most kPatchSlots
+// distinct points (listed by k_patch_slots): the wave copies those points'
+// frames into LDS -- element (q, k) of slot u at lds[q * LC + k][u], one or
+// two slots per lane -- and reads each coord's 16 points from LDS through
+// its slot bytes; kGatherUnlisted gathers for the other waves.  Same
+// products and order in every mode.
+enum { kGatherAll = 0, kStagedOnly = 1, kGatherUnlisted = 2 };
+
+template <int LC, int FC, int MODE, int SLOTS = 128>
 __global__ __launch_bounds__(kBlock) void k_batch_patches_regular(BatchArgs a)
 {
+    constexpr bool STAGED = MODE == kStagedOnly;
+    __shared__ float lds[STAGED ? kWavesPerBlock : 1][STAGED ? FC * LC : 1][STAGED ? SLOTS : 1];
+    const int lane = (int)threadIdx.x & (kWave - 1);
+    const int wv = wave_in_block();
     for (int iter = 0;; ++iter) {
         const int itx = batch_next_item(a, iter);
         if (itx < 0) break;
         const BatchItem item = ld(cptr(a.items) + itx);
         const GPatchSeg sg = gseg(cptr(a.pseg)[item.seg]);
         const int i = item.first + (int)threadIdx.x;
-        if (i >= item.first + item.count) continue;
+        const bool inRange = i < item.first + item.count;
+        // the wave's entry in the staging lists (items start at multiples of
+        // kBlock coords); a wave past the segment's end has none
+        const bool waveHasCoords = wv * kWave < item.count;
+        const int U = (MODE != kGatherAll && sg.nuniq && waveHasCoords)
+                          ? sg.nuniq[(item.first >> 6) + wv] : -1;
+        // staged variants: 128 slots take U <= 128, 256 slots 128 < U <= 256
+        if (MODE == kStagedOnly && (U < 0 || U > SLOTS || (SLOTS > 128 && U <= 128))) continue;
+        if (MODE == kGatherUnlisted && U >= 0) continue;
 
-        const osdhip_patch_coord c = ld(sg.coords + i);
-        const osdhip_patch_array arr = ld(sg.arrays + c.arrayIndex);
-        const osdhip_patch_param pp = ld(sg.params + c.patchIndex);
-        const unsigned f1 = pp.field1;
-        const int type = ((f1 >> 5) & 1u) ? arr.regDesc : arr.desc;
-        const int boundary = (int)((f1 >> 7) & 0x1fu);
-        if (type != OSDHIP_PATCH_REGULAR || boundary != 0) continue;
-        const int depth = (int)(f1 & 0xfu);
-        const int nonquad = (int)((f1 >> 4) & 1u);
-        const int pv = (int)((f1 >> 12) & 0x3ffu);
-        const int pu = (int)((f1 >> 22) & 0x3ffu);
-        const float frac_inv = (float)(1 << (depth - nonquad));
-        const float s = c.s * frac_inv - (float)pu;
-        const float t = c.t * frac_inv - (float)pv;
-        const G_ int *cvs = sg.indices + arr.indexBase + arr.stride * (c.patchIndex - arr.primitiveIdBase);
+        if (!STAGED) {                                    // gathers from memory
+            if (!inRange) continue;
+            const RegularCoord rc = regular_coord(sg, i);
+            if (!rc.ok) continue;
+            Curve cs, ct;
+            bspline_curve(rc.s, cs);
+            bspline_curve(rc.t, ct);
+            for (int f0 = 0; f0 < sg.frames; f0 += FC) {
+                const int nq = min(FC, sg.frames - f0);
+                float acc[3][FC][LC];
+#pragma unroll
+                for (int o = 0; o < 3; ++o)
+#pragma unroll
+                    for (int q = 0; q < FC; ++q)
+#pragma unroll
+                        for (int k = 0; k < LC; ++k) acc[o][q][k] = 0.0f;
+                const G_ float *base = sg.src + (size_t)f0 * sg.srcF;
+#pragma unroll 1
+                for (int r = 0; r < 4; ++r) {             // point j = 4 r + q
+                    const float tb = pick4(ct.b, r), td = pick4(ct.d, r);
+                    int idx[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) idx[q] = rc.cvs[4 * r + q];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        batch_patch_point<LC, FC>(base + (size_t)(unsigned)idx[q] * sg.srcV,
+                                                  sg.srcF, nq, cs.b[q] * tb, cs.d[q] * tb,
+                                                  cs.b[q] * td, acc);
+                }
+                store_patch_chunk<LC, FC>(sg, i, f0, nq, rc.k1, acc);
+            }
+            continue;
+        }
+
+        // staged wave: every lane takes part in the copies
+        float(*Ls)[STAGED ? SLOTS : 1] = lds[STAGED ? wv : 0];
+        RegularCoord rc;
+        rc.ok = false;
+        if (inRange) rc = regular_coord(sg, i);
+        unsigned sw[4] = { 0u, 0u, 0u, 0u };
+        if (rc.ok) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) sw[r] = sg.slots[(size_t)i * 4 + r];
+        }
         Curve cs, ct;
-        bspline_curve(s, cs);
-        bspline_curve(t, ct);
-        const float k1 = (float)(1 << depth);
-
+        bspline_curve(rc.ok ? rc.s : 0.0f, cs);
+        bspline_curve(rc.ok ? rc.t : 0.0f, ct);
+        const G_ int *lst = sg.pts + sg.ptsOff[(item.first >> 6) + wv];
+        constexpr int PER = STAGED ? SLOTS / kWave : 1;   // slots per lane
+        int vv[PER];
+#pragma unroll
+        for (int h = 0; h < PER; ++h) vv[h] = lane + h * kWave < U ? lst[lane + h * kWave] : 0;
         for (int f0 = 0; f0 < sg.frames; f0 += FC) {
             const int nq = min(FC, sg.frames - f0);
+            wave_lds_sync();                              // previous chunk's reads are done
+#pragma unroll
+            for (int h = 0; h < PER; ++h) {
+                const int u = lane + h * kWave;
+                if (u < U) {
+                    const G_ float *p = sg.src + (size_t)(unsigned)vv[h] * sg.srcV +
+                                        (size_t)f0 * sg.srcF;
+#pragma unroll
+                    for (int q = 0; q < FC; ++q) {
+                        if (q < nq) {                     // wave-uniform
+#pragma unroll
+                            for (int k = 0; k < LC; ++k) Ls[q * LC + k][u] = p[q * sg.srcF + k];
+                        }
+                    }
+                }
+            }
+            wave_lds_sync();
+            if (!rc.ok) continue;
             float acc[3][FC][LC];
 #pragma unroll
             for (int o = 0; o < 3; ++o)
@@ -703,31 +921,29 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches_regular(BatchArgs a)
                 for (int q = 0; q < FC; ++q)
 #pragma unroll
                     for (int k = 0; k < LC; ++k) acc[o][q][k] = 0.0f;
-            const G_ float *base = sg.src + (size_t)f0 * sg.srcF;
-#pragma unroll 1
-            for (int r = 0; r < 4; ++r) {                 // point j = 4 r + q
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {                 // point j = 4 r + q, in order
                 const float tb = pick4(ct.b, r), td = pick4(ct.d, r);
-                int idx[4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q) idx[q] = cvs[4 * r + q];
+                for (int qq = 0; qq < 4; ++qq) {
+                    const int slot = (int)((sw[r] >> (8 * qq)) & 0xffu);
+                    const float wp = cs.b[qq] * tb, wu = cs.d[qq] * tb, wvv = cs.b[qq] * td;
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    batch_patch_point<LC, FC>(base + (size_t)(unsigned)idx[q] * sg.srcV, sg.srcF,
-                                              nq, cs.b[q] * tb, cs.d[q] * tb, cs.b[q] * td, acc);
-            }
+                    for (int q = 0; q < FC; ++q) {
+                        if (q < nq) {
 #pragma unroll
-            for (int q = 0; q < FC; ++q) {
-                if (q >= nq) break;
-#pragma unroll
-                for (int o = 0; o < 3; ++o) {
-                    G_ float *d = sg.out[o];
-                    if (!d) continue;
-                    const float f = o == 0 ? 1.0f : k1;
-                    d += (size_t)i * sg.outV + (size_t)(f0 + q) * sg.outF;
-#pragma unroll
-                    for (int k = 0; k < LC; ++k) d[k] = acc[o][q][k] * f;
+                            for (int k = 0; k < LC; ++k) {
+                                const float x = Ls[q * LC + k][slot];
+                                acc[0][q][k] += x * wp;
+                                acc[1][q][k] += x * wu;
+                                acc[2][q][k] += x * wvv;
+                            }
+                        }
+                    }
+                    __builtin_amdgcn_sched_barrier(0);    // one point's reads in flight
                 }
             }
+            store_patch_chunk<LC, FC>(sg, i, f0, nq, rc.k1, acc);
         }
     }
 }
@@ -735,6 +951,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches_regular(BatchArgs a)
 template <int LC, int FC>
 __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
 {
+    const int wv = wave_in_block();
     for (int iter = 0;; ++iter) {
         const int itx = batch_next_item(a, iter);
         if (itx < 0) break;
@@ -742,6 +959,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
         const GPatchSeg sg = gseg(cptr(a.pseg)[item.seg]);
         const int i = item.first + (int)threadIdx.x;
         if (i >= item.first + item.count) continue;
+        // waves without such coords (counted when the batch was built)
+        if (sg.ngen && sg.ngen[(item.first >> 6) + wv] == 0) continue;
 
         const osdhip_patch_coord c = ld(sg.coords + i);
         const osdhip_patch_array arr = ld(sg.arrays + c.arrayIndex);

@@ -23,6 +23,21 @@ __device__ __forceinline__ int wave_in_block()
     return __builtin_amdgcn_readfirstlane((int)threadIdx.x / kWave);
 }
 
+// Wave-wide reductions (every lane receives the result).
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+    for (int m = kWave / 2; m; m >>= 1) v = max(v, __shfl_xor(v, m, kWave));
+    return v;
+}
+
+__device__ __forceinline__ unsigned wave_min_u(unsigned v)
+{
+#pragma unroll
+    for (int m = kWave / 2; m; m >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, m, kWave));
+    return v;
+}
+
 // Wave-scope LDS hand-off: LDS traffic of one wave is processed in order,
 // so only the compiler has to be kept from reordering across this point.
 __device__ __forceinline__ void wave_lds_sync()

@@ -94,20 +94,6 @@ __global__ __launch_bounds__(kBlock) void k_fill_split(
 // are at most kTileZero of them the slice's records hold list positions
 // (LDS slots), otherwise control indices.
 
-__device__ __forceinline__ int wave_max_i(int v)
-{
-#pragma unroll
-    for (int m = kWave / 2; m; m >>= 1) v = max(v, __shfl_xor(v, m, kWave));
-    return v;
-}
-
-__device__ __forceinline__ unsigned wave_min_u(unsigned v)
-{
-#pragma unroll
-    for (int m = kWave / 2; m; m >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, m, kWave));
-    return v;
-}
-
 // Sort key per slice row: the dominant control vertex (rows past n and
 // empty rows: 0xffffffff, after every real key); value: the row.
 __global__ __launch_bounds__(kBlock) void k_row_dominant(const int *sizes, const int *offsets,

@@ -18,10 +18,12 @@ from opensubdiv_amd.workloads import DeviceMesh, MeshSpec, compile_mesh
 pytestmark = pytest.mark.gpu
 
 # small meshes: n x n quads per cube face, refinement / isolation level,
-# frames, coords -- Gregory end caps at the valence-3 corners
+# frames, coords -- Gregory end caps at the valence-3 corners; the last one
+# has fewer coords than patches, so its waves of coords reference too many
+# points to be staged in LDS
 SPECS = [MeshSpec(0, 3, 2, 3, 700, 11), MeshSpec(1, 2, 1, 1, 300, 12),
          MeshSpec(2, 5, 2, 5, 2500, 13), MeshSpec(3, 4, 3, 2, 1800, 14),
-         MeshSpec(4, 2, 2, 4, 64, 15)]
+         MeshSpec(4, 2, 2, 4, 64, 15), MeshSpec(5, 12, 2, 2, 700, 16)]
 
 
 def build(length, layout=0):
@@ -82,6 +84,8 @@ def workload(request):
     batch = Batch(length, [s for d in dms for s in d.stencil_segments()],
                   [d.patch_segment() for d in dms])
     batch.layout = layout
+    # some waves of coords are staged, some are not
+    assert 0 < batch.get("staged_coords") < sum(d.ncoords for d in dms)
     yield length, hosts, dms, batch
     batch.close()
 
@@ -156,17 +160,18 @@ def test_every_variant_matches_plain(workload, variant, persistent):
     plain = outputs(dms)
     for k, v in variants[variant].items():
         assert batch.set(k, v) == 0
-    for slices, bfr in ((1, 1), (3, 2), (16, 4)):
+    for slices, bfr, stg in ((1, 1, 1), (3, 2, 0), (16, 4, 1), (2, 2, 1)):
         assert batch.set("a_item_slices", slices) == 0
         assert batch.set("a_persistent", persistent) == 0
         assert batch.set("b_frames", bfr) == 0
         assert batch.set("b_persistent", persistent) == 0
+        assert batch.set("b_stage", stg) == 0
         poison(dms)
         batch.run(3)
-        same(plain, outputs(dms), f"eager {variants[variant]} slices {slices} b_frames {bfr}")
+        same(plain, outputs(dms), f"eager {variants[variant]} slices {slices} b_frames {bfr} stage {stg}")
         poison(dms)
         batch.launch()
-        same(plain, outputs(dms), f"graph {variants[variant]} slices {slices} b_frames {bfr}")
+        same(plain, outputs(dms), f"graph {variants[variant]} slices {slices} b_frames {bfr} stage {stg}")
 
 
 def test_vertex_major_refined_outputs(workload):
@@ -220,7 +225,7 @@ def test_tuned_configuration_matches_plain(workload):
     batch.run_plain()
     plain = outputs(dms)
     report = batch.tune(reps=1)
-    assert len(report["phase_a"]) == TUNED_A[batch.layout] and len(report["phase_b"]) == 6
+    assert len(report["phase_a"]) == TUNED_A[batch.layout] and len(report["phase_b"]) == 12
     poison(dms)
     batch.launch()
     same(plain, outputs(dms), f"tuned {batch.config()}")
@@ -239,6 +244,7 @@ def test_batch_knob_and_segment_errors(workload):
         batch.run(1)
     assert batch.set("a_frames", cfg["a_frames"]) == 0 and batch.set("a_group", cfg["a_group"]) == 0
     assert batch.set("a_persistent", 2) == _lib.ERR_ARGS
+    assert batch.set("b_stage", 2) == _lib.ERR_ARGS
     assert batch.set("nope", 1) == _lib.ERR_UNSUPPORTED
     info = batch.info()
     assert info["stencil_rows"] == sum(d.nrefine + d.npts for d in dms)
