@@ -144,9 +144,11 @@ class HipStencilTable:
             np.ascontiguousarray(offsets, np.int32)
         indices = np.ascontiguousarray(get("indices", "GetControlIndices"), np.int32)
         ws = []
-        for k, g in zip(self._KEYS, self._GETTERS):
+        for i, (k, g) in enumerate(zip(self._KEYS, self._GETTERS)):
             v = get(k, g)
-            if v is None or len(v) == 0:
+            # derivative arrays are optional (absent or empty); the position
+            # weights are always passed, also when every stencil is empty
+            if v is None or (i > 0 and len(v) == 0):
                 break
             ws.append(np.ascontiguousarray(v, np.float32))
         nw = 6 if len(ws) >= 6 else (3 if len(ws) >= 3 else 1)

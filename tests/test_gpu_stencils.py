@@ -377,3 +377,34 @@ def test_tiled_layout_random_tables_both_paths(hip, nw, L):
                 np.testing.assert_array_equal(got[:end - start], want,
                                               err_msg=f"{keys[k]} nsrc {nsrc} [{start},{end})")
                 assert np.all(np.isnan(got[end - start:]))
+
+
+def test_tiled_layout_edge_tables(hip):
+    """Layout-2 builds of edge cases against the oracle: one row, a row
+    repeating a control index, fewer rows than a slice, one row past a slice,
+    every row empty, and exactly one full slice."""
+    cases = []
+    rng = np.random.default_rng(77)
+    cases.append({"sizes": np.array([3], np.int32), "offsets": np.array([0], np.int32),
+                  "indices": np.array([2, 0, 2], np.int32),
+                  "weights": np.array([0.25, 0.5, -0.75], np.float32)})
+    for n in (5, 63, 64, 65):
+        t = random_table(rng, n, 30, 9)
+        cases.append(t)
+    blank = {"sizes": np.zeros(70, np.int32), "offsets": np.zeros(70, np.int32),
+             "indices": np.zeros(0, np.int32), "weights": np.zeros(0, np.float32)}
+    cases.append(blank)
+    L = 5
+    for k, t in enumerate(cases):
+        table = _tiled_table(hip, t)
+        n = table.GetNumStencils()
+        info = table.layout()
+        assert info["layout"] == 2 and info["slices"] == (n + 63) // 64, f"case {k}"
+        src = rng.normal(size=(40, L)).astype(np.float32)
+        s_t = dev(src)
+        out = empty(max(1, n) * L)
+        assert table_eval(hip, table.h, s_t, (0, L, L), [out], [(0, L, L)], 0, n) == 0
+        if n == 0:
+            continue
+        want = oracle.apply_stencils(src, t["sizes"], t["offsets"], t["indices"], t["weights"])
+        np.testing.assert_array_equal(host(out, shape=(n, L)), want, err_msg=f"case {k}")
