@@ -921,26 +921,53 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches_regular(BatchArgs a)
                 for (int q = 0; q < FC; ++q)
 #pragma unroll
                     for (int k = 0; k < LC; ++k) acc[o][q][k] = 0.0f;
+            // the curves' values pass through an empty asm each chunk, so the
+            // 48 weight products are formed per chunk instead of being held
+            // in registers across the frame loop (they are the same products)
+            float cb[4], cd[4], tbv[4], tdv[4];
+            unsigned swv[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                cb[r] = cs.b[r];
+                cd[r] = cs.d[r];
+                tbv[r] = ct.b[r];
+                tdv[r] = ct.d[r];
+                swv[r] = sw[r];
+                asm volatile("" : "+v"(cb[r]), "+v"(cd[r]), "+v"(tbv[r]), "+v"(tdv[r]), "+v"(swv[r]));
+            }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {                 // point j = 4 r + q, in order
-                const float tb = pick4(ct.b, r), td = pick4(ct.d, r);
+                const float tb = tbv[r], td = tdv[r];
 #pragma unroll
                 for (int qq = 0; qq < 4; ++qq) {
-                    const int slot = (int)((sw[r] >> (8 * qq)) & 0xffu);
-                    const float wp = cs.b[qq] * tb, wu = cs.d[qq] * tb, wvv = cs.b[qq] * td;
+                    const int slot = (int)((swv[r] >> (8 * qq)) & 0xffu);
+                    const float wp = cb[qq] * tb, wu = cd[qq] * tb, wvv = cb[qq] * td;
+                    // every frame of the chunk without branches, so the point's
+                    // LDS reads issue together (frames past the end are not stored)
+                    float x[FC][LC];
 #pragma unroll
-                    for (int q = 0; q < FC; ++q) {
-                        if (q < nq) {
+                    for (int q = 0; q < FC; ++q)
 #pragma unroll
-                            for (int k = 0; k < LC; ++k) {
-                                const float x = Ls[q * LC + k][slot];
-                                acc[0][q][k] += x * wp;
-                                acc[1][q][k] += x * wu;
-                                acc[2][q][k] += x * wvv;
-                            }
+                        for (int k = 0; k < LC; ++k) x[q][k] = Ls[q * LC + k][slot];
+#pragma unroll
+                    for (int q = 0; q < FC; ++q)
+#pragma unroll
+                        for (int k = 0; k < LC; ++k) {
+                            acc[0][q][k] += x[q][k] * wp;
+                            acc[1][q][k] += x[q][k] * wu;
+                            acc[2][q][k] += x[q][k] * wvv;
                         }
-                    }
-                    __builtin_amdgcn_sched_barrier(0);    // one point's reads in flight
+                    // one point's reads in flight: the point's sums are taken
+                    // as asm inputs before a memory clobber, so its products
+                    // are formed before the next point's LDS reads issue
+                    // (otherwise every point's 7 * FC values are held at once)
+#pragma unroll
+                    for (int o = 0; o < 3; ++o)
+#pragma unroll
+                        for (int q = 0; q < FC; ++q)
+#pragma unroll
+                            for (int k = 0; k < LC; ++k) asm volatile("" ::"v"(acc[o][q][k]));
+                    asm volatile("" ::: "memory");
                 }
             }
             store_patch_chunk<LC, FC>(sg, i, f0, nq, rc.k1, acc);
