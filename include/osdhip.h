@@ -171,6 +171,18 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
                         const osdhip_patch_param *params,
                         void *stream);
 
+/* ---- the step after EvalPatches: limit normals ------------------------------
+ * normals[i] = cross(dPdu[i], dPdv[i]) over xyz (the first three elements of
+ * each descriptor), scaled to unit length by 1 / sqrtf(dot) when `normalize`
+ * is nonzero: the steps the reference's tutorial uses for limit normals
+ * (tutorials/far/tutorial_2_3/far_tutorial_2_3.cpp:73-87, 404-412); Osd has no
+ * evaluator entry point for them.  Same results as those steps, bit for bit.
+ * OSDHIP_ERR_ARGS when a descriptor has fewer than 3 elements or a buffer
+ * is NULL with count > 0. */
+int osdhip_limit_normals(const float *dPdu, osdhip_buffer_desc duDesc, const float *dPdv,
+                         osdhip_buffer_desc dvDesc, float *normals, osdhip_buffer_desc nDesc,
+                         int count, int normalize, void *stream);
+
 /* Tuning knobs; returns the previous value (or an error code).
  *   "stencil_kernel": 0 = slice layout (default), 1 = CSR with LDS staging
  *                     for osdhip_stencil_table_eval.

@@ -68,6 +68,7 @@ _HIP_SIGS = {
     "osdhip_stencil_table_info": [_vp, _vp, _vp, _vp, _vp],
     "osdhip_stencil_table_csr": [_vp, _vp, _vp, _vp, _vp],
     "osdhip_stencil_table_layout": [_vp, _vp, _vp, _vp, _vp],
+    "osdhip_limit_normals": [_vp, _D, _vp, _D, _vp, _D, _i, _i, _vp],
     "osdhip_stencil_table_eval": [_vp, _vp, _D, _i, _vp, _vp, _i, _i, _vp],
     "osdhip_config": [C.c_char_p, _i],
     "osdhip_malloc": [C.POINTER(C.c_void_p), C.c_size_t],

@@ -730,6 +730,21 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
     return rc;
 }
 
+int osdhip_limit_normals(const float *dPdu, osdhip_buffer_desc duDesc, const float *dPdv,
+                         osdhip_buffer_desc dvDesc, float *normals, osdhip_buffer_desc nDesc,
+                         int count, int normalize, void *stream)
+{
+    if (count <= 0) return OSDHIP_OK;
+    if (!dPdu || !dPdv || !normals || duDesc.length < 3 || dvDesc.length < 3 || nDesc.length < 3 ||
+        !desc_ok(duDesc) || !desc_ok(dvDesc) || !desc_ok(nDesc))
+        return OSDHIP_ERR_ARGS;
+    const int grid = (int)(((long long)count + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_limit_normals, dim3(grid), dim3(kBlock), 0, (hipStream_t)stream, dPdu,
+                       duDesc.offset, duDesc.stride, dPdv, dvDesc.offset, dvDesc.stride, normals,
+                       nDesc.offset, nDesc.stride, count, normalize ? 1 : 0);
+    return last_error();
+}
+
 int osdhip_config(const char *key, int value)
 {
     if (!key) return OSDHIP_ERR_ARGS;

@@ -149,4 +149,33 @@ __global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
     }
 }
 
+// Limit normals from EvalPatches' first derivatives, one thread per point:
+// n = cross(du, dv) component by component as written (no fused
+// multiply-add), then n * (1 / sqrtf(dot)) when normalizing (IEEE division
+// and square root), which are the reference tutorial's steps.
+__global__ __launch_bounds__(kBlock) void k_limit_normals(const float *du, int duOff, int duStride,
+                                                          const float *dv, int dvOff, int dvStride,
+                                                          float *n, int nOff, int nStride,
+                                                          int count, int normalize)
+{
+    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (i >= count) return;
+    const float *a = du + duOff + (size_t)i * (size_t)duStride;
+    const float *b = dv + dvOff + (size_t)i * (size_t)dvStride;
+    const float a0 = a[0], a1 = a[1], a2 = a[2], b0 = b[0], b1 = b[1], b2 = b[2];
+    float c0 = a1 * b2 - a2 * b1;
+    float c1 = a2 * b0 - a0 * b2;
+    float c2 = a0 * b1 - a1 * b0;
+    if (normalize) {
+        const float rn = 1.0f / sqrtf(c0 * c0 + c1 * c1 + c2 * c2);
+        c0 *= rn;
+        c1 *= rn;
+        c2 *= rn;
+    }
+    float *o = n + nOff + (size_t)i * (size_t)nStride;
+    o[0] = c0;
+    o[1] = c1;
+    o[2] = c2;
+}
+
 }  // namespace osdhip

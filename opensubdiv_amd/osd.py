@@ -248,6 +248,19 @@ class HipPatchTable:
     def GetFVarPatchParamBuffer(self, fvarChannel=0): return self.fvar[fvarChannel][2].data_ptr()
 
 
+def limit_normals(du, duDesc, dv, dvDesc, normals, normalsDesc, count, normalize=True,
+                  deviceContext=None):
+    """Limit normals after EvalPatches: cross(dPdu, dPdv) over xyz, scaled to
+    unit length when `normalize` -- the reference tutorial's steps
+    (tutorials/far/tutorial_2_3/far_tutorial_2_3.cpp:73-87) on the device
+    (osdhip_limit_normals).  Returns bool."""
+    lib = _lib.load_hip()
+    rc = lib.osdhip_limit_normals(_ptr(du), _lib.desc(duDesc), _ptr(dv), _lib.desc(dvDesc),
+                                  _ptr(normals), _lib.desc(normalsDesc), int(count),
+                                  int(bool(normalize)), _stream(deviceContext))
+    return rc == _lib.OK
+
+
 class HipEvaluator:
     """Static evaluator (osd/cudaEvaluator.h:94-1265); all methods return
     bool like the reference.  ``deviceContext`` may be a hipStream_t (int);

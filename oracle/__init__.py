@@ -70,6 +70,7 @@ def lib():
                                         i, vp, vp, vp, vp],
             "oracle_eval_patches_f64": [vp, dd, vp, dd, vp, dd, vp, dd, i, vp, vp, vp, vp],
             "oracle_eval_patches_omp_f32": [vp, dd, vp, dd, vp, dd, vp, dd, i, vp, vp, vp, vp, i],
+            "oracle_limit_normals_f32": [vp, dd, vp, dd, vp, dd, i, i],
         }
         for k, a in sigs.items():
             f = getattr(L, k)
@@ -179,3 +180,12 @@ def eval_patches_f64(src, sd, outs, descs, coords, arrays, indices, params):
         a += [_p(o), D(d)]
     return bool(lib().oracle_eval_patches_f64(_p(src), D(sd), *a, len(coords), _p(coords),
                                               _p(arrays), _p(indices), _p(params)))
+
+
+# ---------------------------------------------------------- limit normals
+
+def limit_normals(du, du_desc, dv, dv_desc, n, n_desc, count, normalize=True):
+    """float32 cross(dPdu, dPdv) [normalized], the reference tutorial's
+    steps (far_tutorial_2_3.cpp:73-87). Returns bool."""
+    return bool(lib().oracle_limit_normals_f32(_p(du), D(du_desc), _p(dv), D(dv_desc), _p(n),
+                                               D(n_desc), int(count), int(bool(normalize))))

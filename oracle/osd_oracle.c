@@ -6,6 +6,7 @@
  */
 #include "osd_oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -318,6 +319,34 @@ int oracle_eval_patches_omp_f32(const float *src, oracle_desc srcDesc,
             memcpy(outs[o] + ods[o].offset + (size_t)i * (size_t)ods[o].stride, acc,
                    sizeof(float) * (size_t)L);
         }
+    }
+    return 1;
+}
+
+/* ---- limit normals (tutorials/far/tutorial_2_3/far_tutorial_2_3.cpp:73-87) -- */
+
+int oracle_limit_normals_f32(const float *du, oracle_desc duDesc, const float *dv,
+                             oracle_desc dvDesc, float *n, oracle_desc nDesc, int count,
+                             int normalize)
+{
+    if (duDesc.length < 3 || dvDesc.length < 3 || nDesc.length < 3) return 0;
+    for (int i = 0; i < count; ++i) {
+        const float *a = du + duDesc.offset + (size_t)i * (size_t)duDesc.stride;
+        const float *b = dv + dvDesc.offset + (size_t)i * (size_t)dvDesc.stride;
+        float *o = n + nDesc.offset + (size_t)i * (size_t)nDesc.stride;
+        float c[3];
+        c[0] = a[1] * b[2] - a[2] * b[1];   /* cross(v1, v2): lines 82-87 */
+        c[1] = a[2] * b[0] - a[0] * b[2];
+        c[2] = a[0] * b[1] - a[1] * b[0];
+        if (normalize) {                    /* normalize(n): lines 73-79 */
+            const float rn = 1.0f / sqrtf(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+            c[0] *= rn;
+            c[1] *= rn;
+            c[2] *= rn;
+        }
+        o[0] = c[0];
+        o[1] = c[1];
+        o[2] = c[2];
     }
     return 1;
 }

@@ -19,6 +19,7 @@
  *   osd/patchBasis.h:1447-1610      dispatch, normalization, 2^depth scaling
  *   osd/patchBasisTypes.h:298-397   PatchParam bitfield decode and Normalize
  *   far/stencilTable.h:616-674      StencilTableReal<double>::UpdateValues
+ *   tutorials/far/tutorial_2_3/far_tutorial_2_3.cpp:73-87  limit normal (cross, normalize)
  *
  * Parity pins (each function is checked by tests/ against at least one of):
  *   - golden vectors written by the real reference (tests/golden/*.npz),
@@ -154,6 +155,16 @@ int oracle_eval_patches_omp_f32(const float *src, oracle_desc srcDesc,
                                 const int *patchIndices,
                                 const oracle_patch_param *params,
                                 int numThreads);
+
+/* --- the step after EvalPatches: limit normals ------------------------- */
+
+/* N = cross(dPdu, dPdv) over xyz (the first three elements), optionally
+ * scaled to unit length as 1 / sqrtf(dot) -- the reference tutorial's
+ * `cross` and `normalize` (tutorials/far/tutorial_2_3/far_tutorial_2_3.cpp:73-87).
+ * Returns 0 when a descriptor has fewer than 3 elements, 1 otherwise. */
+int oracle_limit_normals_f32(const float *du, oracle_desc duDesc, const float *dv,
+                             oracle_desc dvDesc, float *n, oracle_desc nDesc, int count,
+                             int normalize);
 
 #ifdef __cplusplus
 }

@@ -220,3 +220,60 @@ def test_patch_sorted_order_bit_exact(hip, deriv):
         assert hip.osdhip_config(b"patch_sort_min", -1) == _lib.ERR_ARGS
     finally:
         hip.osdhip_config(b"patch_sort_min", old)
+
+
+def normals(lib, du, dd, dv, vd, n, nd, count, normalize):
+    return lib.osdhip_limit_normals(ptr(du), D(*dd), ptr(dv), D(*vd), ptr(n), D(*nd), count,
+                                    int(normalize), stream())
+
+
+@pytest.mark.parametrize("normalize", [False, True])
+def test_limit_normals_after_golden_patches(hip, normalize):
+    """EvalPatches on the golden cube-sphere coords, interleaved P/du/dv in one
+    buffer, then limit normals into a padded buffer: bit-exact against the
+    oracle's restatement of the reference tutorial's cross / normalize applied
+    to the reference's own dPdu / dPdv."""
+    g, src, coords, arrays, indices, params = golden_patches()
+    n = len(g["coords"])
+    inter = empty(n * 9)
+    assert eval_patches(hip, src, (0, 3, 3), [inter, inter, inter], [(0, 3, 9), (3, 3, 9),
+                                                                     (6, 3, 9)],
+                        n, coords, arrays, indices, params) == 0
+    nrm = empty(n * 4 + 1)
+    assert normals(hip, inter, (3, 3, 9), inter, (6, 3, 9), nrm, (1, 3, 4), n, normalize) == 0
+    want = np.full((n, 3), np.nan, np.float32)
+    du, dv = np.ascontiguousarray(g["cpu_du"]), np.ascontiguousarray(g["cpu_dv"])
+    assert oracle.limit_normals(du, (0, 3, 3), dv, (0, 3, 3), want, (0, 3, 3), n, normalize)
+    got = host(nrm)[1:].reshape(n, 4)
+    np.testing.assert_array_equal(got[:, :3], want)
+    assert np.all(np.isnan(got[:, 3]))                    # the padding element is untouched
+
+
+def test_limit_normals_random_and_errors(hip):
+    """Random tangents with offsets and strides, normalised and not, with a
+    zero normal (0 * (1 / sqrtf(0)) = NaN, as the reference tutorial's
+    normalize gives); argument errors; the Python mirror."""
+    rng = np.random.default_rng(5)
+    n = 10007
+    du = rng.normal(size=(n, 5)).astype(np.float32)
+    dv = rng.normal(size=(n, 4)).astype(np.float32)
+    du[17] = 0.0
+    d_du, d_dv = dev(du), dev(dv)
+    for normalize in (False, True):
+        out = empty(n * 3)
+        assert normals(hip, d_du, (1, 3, 5), d_dv, (0, 3, 4), out, (0, 3, 3), n, normalize) == 0
+        want = np.zeros((n, 3), np.float32)
+        assert oracle.limit_normals(du, (1, 3, 5), dv, (0, 3, 4), want, (0, 3, 3), n, normalize)
+        np.testing.assert_array_equal(host(out, shape=(n, 3)), want)
+        assert np.isnan(want[17]).all() == normalize
+    out = empty(n * 3)
+    assert normals(hip, d_du, (0, 2, 5), d_dv, (0, 3, 4), out, (0, 3, 3), n, True) == \
+        _lib.ERR_ARGS
+    assert normals(hip, None, (0, 3, 5), d_dv, (0, 3, 4), out, (0, 3, 3), n, True) == \
+        _lib.ERR_ARGS
+    assert normals(hip, d_du, (0, 3, 5), d_dv, (0, 3, 4), out, (0, 3, 3), 0, True) == 0
+    assert np.isnan(host(out)).all()                      # count 0 writes nothing
+    from opensubdiv_amd.osd import BufferDescriptor as BD, limit_normals
+    assert limit_normals(d_du, BD(1, 3, 5), d_dv, BD(0, 3, 4), out, BD(0, 3, 3), n)
+    np.testing.assert_array_equal(host(out, shape=(n, 3)), want)
+    assert not limit_normals(d_du, BD(1, 3, 5), d_dv, BD(0, 2, 4), out, BD(0, 3, 3), n)

@@ -347,3 +347,61 @@ def test_planted_mistake_is_caught_by_the_closed_form():
     exact = np.outer(cox_de_boor(0.7), cox_de_boor(0.2))
     assert np.allclose(wP[:16].reshape(4, 4), exact, atol=1e-14)
     assert not np.allclose(wrong, exact, atol=1e-6)
+
+
+# ------------------------------------------------------------- limit normals
+
+def _normals(du, dv, normalize, desc=(0, 3, 3)):
+    n = len(du)
+    out = np.zeros((n, 3), np.float32)
+    assert oracle.limit_normals(du, desc, dv, desc, out, (0, 3, 3), n, normalize)
+    return out
+
+
+def test_limit_normals_closed_forms():
+    """Right-handed basis (e1 x e2 = e3 and cyclic), scaling, unit length."""
+    e = np.eye(3, dtype=np.float32)
+    du = np.stack([e[0], e[1], e[2], 2 * e[1]]).astype(np.float32)
+    dv = np.stack([e[1], e[2], e[0], 3 * e[2]]).astype(np.float32)
+    np.testing.assert_array_equal(_normals(du, dv, False), [e[2], e[0], e[1], [6, 0, 0]])
+    np.testing.assert_array_equal(_normals(du, dv, True), [e[2], e[0], e[1], e[0]])
+
+
+def test_limit_normals_invariants_against_numpy_cross():
+    """Against numpy's cross in float64: anti-commutative, orthogonal to both
+    tangents, unit length after normalising; descriptor offsets and strides."""
+    rng = np.random.default_rng(11)
+    n = 4096
+    du = rng.normal(size=(n, 5)).astype(np.float32)
+    dv = rng.normal(size=(n, 5)).astype(np.float32)
+    out = np.zeros((n, 3), np.float32)
+    assert oracle.limit_normals(du, (2, 3, 5), dv, (1, 3, 5), out, (0, 3, 3), n, False)
+    a, b = du[:, 2:5].astype(np.float64), dv[:, 1:4].astype(np.float64)
+    ref = np.cross(a, b)
+    assert np.allclose(out, ref, rtol=1e-5, atol=1e-5)
+    rev = np.zeros_like(out)
+    assert oracle.limit_normals(dv, (1, 3, 5), du, (2, 3, 5), rev, (0, 3, 3), n, False)
+    np.testing.assert_array_equal(rev, -out)
+    unit = np.zeros_like(out)
+    assert oracle.limit_normals(du, (2, 3, 5), dv, (1, 3, 5), unit, (0, 3, 3), n, True)
+    assert np.abs(np.linalg.norm(unit, axis=1) - 1).max() < 1e-6
+    assert np.abs((unit * a).sum(1)).max() < 1e-5 * np.abs(a).max()
+    assert np.abs((unit * b).sum(1)).max() < 1e-5 * np.abs(b).max()
+    assert np.allclose(unit, ref / np.linalg.norm(ref, axis=1)[:, None], atol=1e-6)
+    # length < 3 in any descriptor is rejected; count 0 is a no-op
+    assert not oracle.limit_normals(du, (0, 2, 5), dv, (0, 3, 5), out, (0, 3, 3), n, True)
+    assert not oracle.limit_normals(du, (0, 3, 5), dv, (0, 3, 5), out, (0, 2, 3), n, True)
+    assert oracle.limit_normals(du, (0, 3, 5), dv, (0, 3, 5), out, (0, 3, 3), 0, True)
+
+
+def test_limit_normals_point_outward_on_the_golden_sphere():
+    """The reference's own dPdu / dPdv of the cube-sphere limit surface give
+    unit normals within 10 degrees of the radial direction; a planted swap of
+    the operands (dv x du) flips every one of them and fails this check."""
+    g = golden("cubesphere_patches.npz")
+    du, dv = np.ascontiguousarray(g["cpu_du"]), np.ascontiguousarray(g["cpu_dv"])
+    p = g["cpu_p"].astype(np.float64)
+    radial = p / np.linalg.norm(p, axis=1)[:, None]
+    cos = (_normals(du, dv, True) * radial).sum(1)
+    assert cos.min() > math.cos(math.radians(10))
+    assert ((_normals(dv, du, True) * radial).sum(1) < 0).all()
