@@ -411,6 +411,18 @@ def buffer_checksums(dms):
     return out
 
 
+def table_cache(args):
+    """The --table-cache directory as a TableCache (None without one)."""
+    if not args.table_cache:
+        return None
+    from opensubdiv_amd.table_cache import TableCache
+    return TableCache(args.table_cache)
+
+
+def cache_summary(cache):
+    return None if cache is None else {"hits": cache.hits, "misses": cache.misses}
+
+
 def run_batched(args, lib, device):
     """cfg3: 256 cube-sphere meshes (n log-uniform in [8,128], 5% creased
     edges, level 2-4, 1-16 frames of xyz + 4 primvars, 64K-1M patch-ordered
@@ -428,7 +440,9 @@ def run_batched(args, lib, device):
     lib.osdhip_config(b"table_csr_mirror", 0)
     old_layout = lib.osdhip_config(b"table_layout", args.table_layout)
     workers = min(16, os.cpu_count() or 1)
-    dms = workloads.build_device_meshes(specs, L, s, workers=workers, log=log, own_outputs=False)
+    cache = table_cache(args)
+    dms = workloads.build_device_meshes(specs, L, s, workers=workers, log=log, own_outputs=False,
+                                        cache=cache)
     lib.osdhip_config(b"table_csr_mirror", 1)
     lib.osdhip_config(b"table_layout", old_layout)
     torch.cuda.synchronize()
@@ -483,7 +497,8 @@ def run_batched(args, lib, device):
                    "refined_rows_per_step": sum(d.spec.frames * d.nrefine for d in dms),
                    "patch_points_per_step": sum(d.spec.frames * d.ncoords for d in dms),
                    "table_entries": sum(d.nnz_refine + d.nnz_points for d in dms),
-                   "device_bytes": dev_bytes, "build_s": build_s},
+                   "device_bytes": dev_bytes, "build_s": build_s,
+                   "table_cache": cache_summary(cache)},
         "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                      "traffic": None, "kernel": "k_batch_stencils + k_batch_patches (graph)",
@@ -513,8 +528,9 @@ def run_served(args, lib, device):
     specs = workloads.sample_meshes(32, seed=args.seed + 1)
     t0 = time.time()
     workers = min(16, os.cpu_count() or 1)
+    cache = table_cache(args)
     with ThreadPoolExecutor(max_workers=workers) as ex:
-        hosts = list(ex.map(lambda sp: workloads.compile_mesh(sp, L), specs))
+        hosts = list(ex.map(lambda sp: workloads.compile_mesh(sp, L, 1, cache), specs))
     lib.osdhip_config(b"table_csr_mirror", 0)
     old_layout = lib.osdhip_config(b"table_layout", args.table_layout)
     residents = serve.residents_from_hosts(hosts)
@@ -572,7 +588,7 @@ def run_served(args, lib, device):
                    "device_budget_bytes": dev_budget, "host_budget_bytes": host_budget,
                    "slots": args.slots, "max_epoch_points": int(args.epoch_points),
                    "table_layout": args.table_layout,
-                   "build_s": build_s},
+                   "build_s": build_s, "table_cache": cache_summary(cache)},
         "served": {"requests": len(done), "rate_per_s": rate, "capacity_per_s": capacity,
                    "load": args.load, "p50_ms": float(np.percentile(lat, 50)),
                    "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
@@ -682,6 +698,10 @@ def main():
                     help="cfg4: cap on the points batched into one epoch")
     ap.add_argument("--meshes", type=int, default=256, help="cfg3: meshes in the batch")
     ap.add_argument("--seed", type=int, default=0, help="cfg3: first sample seed")
+    ap.add_argument("--table-cache", default=None,
+                    help="cfg3/cfg4: directory of compiled-table files; tables found there "
+                         "(same topology and recipe) are mapped instead of compiled, the "
+                         "rest are compiled and stored")
     ap.add_argument("--table-layout", type=int, default=2, choices=[0, 2],
                     help="cfg3/cfg4: stencil table layout (0 record slices, 2 locality-ordered)")
 

@@ -118,6 +118,54 @@ osdfar_stencil_table *osdfar_limit_stencil_table_create(const osdfar_patch_table
 int osdfar_vertex_locations(const osdfar_topology *topo, int level, int *ptex, double *u,
                             double *v);
 
+/* ---- table files and topology keys --------------------------------------
+ * No counterpart in the reference: Far rebuilds every table from the
+ * topology on each run.  A table file holds named, typed arrays (sections)
+ * and the 64-bit key of what they were built from, so a process that keeps
+ * many topologies resident loads their tables instead of recompiling them.
+ *
+ * osdfar_topology_key hashes a TopologyDescriptor's arrays (the arguments
+ * of osdfar_topology_create), the boundary rule and a caller's recipe
+ * string naming what was built (levels, options, sampled locations); 0 for
+ * invalid input.  osdfar_hash64 is XXH64 (the published xxHash 64-bit
+ * algorithm) of nbytes bytes.
+ *
+ * osdfar_table_file_write writes the sections (names unique, 1-31 bytes)
+ * under a temporary name and renames it onto `path`, so readers see no
+ * file or a whole one; 0, -1 for invalid arguments, -2 for I/O errors.
+ * osdfar_table_file_open maps the file read-only and returns NULL when it
+ * is missing, truncated, of another format version or key, or has a
+ * corrupt directory -- or, when verify != 0, a section whose payload hash
+ * does not match.  Section data pointers stay valid until
+ * osdfar_table_file_close; each section is 256-byte aligned in the map. */
+enum { OSDFAR_I32 = 1, OSDFAR_I64 = 2, OSDFAR_F32 = 3, OSDFAR_F64 = 4, OSDFAR_BYTES = 5 };
+typedef struct osdfar_section {
+    const char *name;
+    int dtype;              /* OSDFAR_* */
+    long long count;        /* elements */
+    const void *data;
+} osdfar_section;
+typedef struct osdfar_table_file osdfar_table_file;
+
+unsigned long long osdfar_hash64(const void *data, long long nbytes, unsigned long long seed);
+unsigned long long osdfar_topology_key(int numVertices, int numFaces, const int *numVertsPerFace,
+                                       const int *vertIndicesPerFace, int numCreases,
+                                       const int *creaseVertexIndexPairs,
+                                       const float *creaseWeights, int numCorners,
+                                       const int *cornerVertexIndices, const float *cornerWeights,
+                                       int boundaryInterpolation, const char *recipe);
+int osdfar_table_file_write(const char *path, unsigned long long key, int numSections,
+                            const osdfar_section *sections);
+osdfar_table_file *osdfar_table_file_open(const char *path, unsigned long long key, int verify);
+/* Number of sections (-1 for NULL); section i's name, dtype, count, data. */
+int osdfar_table_file_sections(const osdfar_table_file *f);
+int osdfar_table_file_section(const osdfar_table_file *f, int i, const char **name, int *dtype,
+                              long long *count, const void **data);
+/* The section called `name`: 0, or -2 when there is none. */
+int osdfar_table_file_find(const osdfar_table_file *f, const char *name, int *dtype,
+                           long long *count, const void **data);
+void osdfar_table_file_close(osdfar_table_file *f);
+
 #ifdef __cplusplus
 }
 #endif

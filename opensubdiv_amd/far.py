@@ -46,6 +46,15 @@ def load_far():
         "osdfar_patch_map_find": ([vp, i, vp, vp, vp, vp], i),
         "osdfar_limit_stencil_table_create": ([vp, i, vp, vp, vp, i, i], vp),
         "osdfar_vertex_locations": ([vp, i, vp, vp, vp], i),
+        "osdfar_hash64": ([vp, ll, C.c_ulonglong], C.c_ulonglong),
+        "osdfar_topology_key": ([i, i, vp, vp, i, vp, vp, i, vp, vp, i, C.c_char_p],
+                                C.c_ulonglong),
+        "osdfar_table_file_write": ([C.c_char_p, C.c_ulonglong, i, vp], i),
+        "osdfar_table_file_open": ([C.c_char_p, C.c_ulonglong, i], vp),
+        "osdfar_table_file_sections": ([vp], i),
+        "osdfar_table_file_section": ([vp, i, vp, vp, vp, vp], i),
+        "osdfar_table_file_find": ([vp, C.c_char_p, vp, vp, vp], i),
+        "osdfar_table_file_close": ([vp], None),
     }
     for k, (a, r) in sig.items():
         f = getattr(L, k)
@@ -262,3 +271,96 @@ class PatchTable:
         if getattr(self, "h", None) and _far is not None:
             _far.osdfar_patch_table_destroy(self.h)
             self.h = None
+
+
+# ---- table files and topology keys (include/osdfar.h) -------------------
+
+class Section(C.Structure):
+    """osdfar_section."""
+    _fields_ = [("name", C.c_char_p), ("dtype", C.c_int), ("count", C.c_longlong),
+                ("data", C.c_void_p)]
+
+
+_DTYPES = {1: np.int32, 2: np.int64, 3: np.float32, 4: np.float64, 5: np.uint8}
+_CODES = {np.dtype(v): k for k, v in _DTYPES.items()}
+
+
+def hash64(data, seed=0):
+    """XXH64 of a bytes-like object or a contiguous array's bytes."""
+    a = np.ascontiguousarray(np.frombuffer(data, np.uint8) if isinstance(data, (bytes, bytearray))
+                             else data)
+    return int(load_far().osdfar_hash64(a.ctypes.data if a.nbytes else None, a.nbytes, seed))
+
+
+def topology_key(mesh, recipe="", boundary=1):
+    """64-bit key of a mesh's topology (the arrays Topology() compiles) and a
+    recipe string naming what was built from it."""
+    fv = np.ascontiguousarray(mesh.face_verts, np.int32).reshape(-1)
+    nvpf = np.full(mesh.nfaces, mesh.face_verts.shape[1], np.int32)
+    cp = np.ascontiguousarray(mesh.crease_pairs, np.int32).reshape(-1)
+    cw = np.ascontiguousarray(mesh.crease_weights, np.float32)
+    cv = np.ascontiguousarray(mesh.corner_verts, np.int32)
+    cwv = np.ascontiguousarray(mesh.corner_weights, np.float32)
+    return int(load_far().osdfar_topology_key(mesh.nverts, mesh.nfaces, _p(nvpf), _p(fv), len(cw),
+                                              _p(cp), _p(cw), len(cv), _p(cv), _p(cwv), boundary,
+                                              recipe.encode()))
+
+
+def write_table_file(path, key, arrays):
+    """Writes {name: array} (int32/int64/float32/float64; other dtypes as
+    raw bytes) as a table file.  Returns the C status (0 = written)."""
+    keep, secs = [], (Section * max(1, len(arrays)))()
+    for k, (name, a) in enumerate(arrays.items()):
+        a = np.ascontiguousarray(a)
+        code = _CODES.get(a.dtype)
+        if code is None:
+            a, code = a.reshape(-1).view(np.uint8), 5
+        keep.append(a)
+        secs[k] = Section(name.encode(), code, a.size, a.ctypes.data if a.size else None)
+    return load_far().osdfar_table_file_write(os.fspath(path).encode(), key, len(arrays), secs)
+
+
+class TableFile:
+    """A table file mapped read-only: arrays(name) are numpy views of the
+    map, valid until close()."""
+
+    def __init__(self, h):
+        self.h = h
+
+    @classmethod
+    def open(cls, path, key, verify=True):
+        h = load_far().osdfar_table_file_open(os.fspath(path).encode(), key, int(verify))
+        return cls(h) if h else None
+
+    def names(self):
+        L, out = load_far(), []
+        for k in range(L.osdfar_table_file_sections(self.h)):
+            nm = C.c_char_p()
+            assert L.osdfar_table_file_section(self.h, k, C.byref(nm), None, None, None) == 0
+            out.append(nm.value.decode())
+        return out
+
+    def array(self, name, dtype=None):
+        dt, n, ptr = C.c_int(), C.c_longlong(), C.c_void_p()
+        if load_far().osdfar_table_file_find(self.h, name.encode(), C.byref(dt), C.byref(n),
+                                             C.byref(ptr)) != 0:
+            raise KeyError(name)
+        base = np.dtype(_DTYPES[dt.value])
+        nbytes = n.value * base.itemsize
+        buf = (C.c_ubyte * nbytes).from_address(ptr.value) if nbytes else bytearray()
+        a = np.frombuffer(buf, base)
+        return a.view(dtype) if dtype is not None else a
+
+    def close(self):
+        if getattr(self, "h", None) and _far is not None:
+            _far.osdfar_table_file_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()

@@ -105,17 +105,58 @@ class HostMesh:
     params: np.ndarray
     coords: np.ndarray      # PATCH_COORD_DTYPE, in patch order
     frames: np.ndarray      # (F, nctrl, L) float32
+    source: object = None   # table_cache TableFile the arrays above view, if loaded
 
     def release(self):
         if self.refine is not None:
             self.refine.close()
             self.refine = None
+        if self.source is not None:
+            self.source.close()
+            self.source = None
 
 
-def compile_mesh(spec: MeshSpec, length=7, threads=1) -> HostMesh:
+def mesh_recipe(spec: MeshSpec) -> str:
+    """What compile_mesh builds from a spec's topology (the table cache's
+    recipe string): everything but the frames, which are not cached."""
+    return (f"cube-sphere tables v1: refine vertex L{spec.level}; patches iso {spec.level} "
+            f"gregory-basis end caps; coords {spec.ncoords} random @ seed {spec.seed + 1} "
+            f"in patch order")
+
+
+def compile_mesh(spec: MeshSpec, length=7, threads=1, cache=None) -> HostMesh:
+    """One mesh's host data; with a table_cache.TableCache, the tables and
+    coords come from its file for this topology and recipe when there is a
+    valid one, and are compiled and stored otherwise."""
     far.set_threads(threads)
     mesh = meshes.cube_sphere(spec.n, seed=spec.seed, radial_noise=0.01, crease_fraction=0.05,
                               sharpness_max=4.0)
+    frames = meshes.deform_frames(mesh, spec.frames, seed=spec.seed + 2,
+                                  extra_primvars=length - 3)
+    key = cache.key(mesh, mesh_recipe(spec)) if cache is not None else None
+    if cache is not None:
+        tf = cache.load(key)
+        if tf is not None:
+            from .osd import PATCH_ARRAY_DTYPE, PATCH_COORD_DTYPE, PATCH_PARAM_DTYPE
+            from .table_cache import MappedStencilTable
+            try:
+                points = far.StencilTable(
+                    {k: tf.array("points." + k) for k in ("sizes", "indices", "weights")},
+                    num_control=mesh.nverts)
+                off = np.zeros(len(points["sizes"]), np.int64)
+                np.cumsum(points["sizes"][:-1], out=off[1:])
+                points["offsets"] = off.astype(np.int32) if len(points["indices"]) < 2 ** 31 \
+                    else off
+                h = HostMesh(spec, length, mesh.nverts, MappedStencilTable(tf, "refine",
+                                                                           mesh.nverts),
+                             points, tf.array("patch.arrays", PATCH_ARRAY_DTYPE),
+                             tf.array("patch.indices"),
+                             tf.array("patch.params", PATCH_PARAM_DTYPE),
+                             tf.array("coords", PATCH_COORD_DTYPE), frames, tf)
+            except (KeyError, ValueError):
+                tf.close()
+            else:
+                return h
     topo = far.Topology(mesh)
     topo.cache_stencils(spec.level)
     refine = topo.stencil_table_raw()
@@ -125,8 +166,13 @@ def compile_mesh(spec: MeshSpec, length=7, threads=1) -> HostMesh:
     f, s, t = meshes.random_patch_locations(mesh.nfaces, spec.ncoords, seed=spec.seed + 1)
     coords = pt.find(f, s, t)
     coords = coords[np.argsort(coords["patchIndex"], kind="stable")]
-    frames = meshes.deform_frames(mesh, spec.frames, seed=spec.seed + 2,
-                                  extra_primvars=length - 3)
+    if cache is not None:
+        from .table_cache import raw_table_arrays
+        arrays = raw_table_arrays(refine, "refine")
+        arrays.update({"points." + k: points[k] for k in ("sizes", "indices", "weights")})
+        arrays.update({"patch.arrays": pt.arrays, "patch.indices": pt.indices,
+                       "patch.params": pt.params, "coords": coords})
+        cache.store(key, arrays)
     return HostMesh(spec, length, mesh.nverts, refine, points, pt.arrays, pt.indices, pt.params,
                     coords, frames)
 
@@ -246,14 +292,16 @@ def stage_outputs(dms, arena_floats):
     return arena, stage + 1
 
 
-def build_device_meshes(specs, length=7, stream=None, workers=8, log=None, own_outputs=True):
-    """Compiles the meshes on `workers` host threads (largest first) and
-    uploads each as it completes; returns DeviceMeshes in spec order (with
-    their own output buffers, or none for stage_outputs)."""
+def build_device_meshes(specs, length=7, stream=None, workers=8, log=None, own_outputs=True,
+                        cache=None):
+    """Compiles the meshes on `workers` host threads (largest first), or
+    loads them from a table_cache.TableCache, and uploads each as it
+    completes; returns DeviceMeshes in spec order (with their own output
+    buffers, or none for stage_outputs)."""
     order = sorted(specs, key=lambda s: -spec_rows(s)[1])
     done = {}
     with ThreadPoolExecutor(max_workers=max(1, int(workers))) as ex:
-        futs = {ex.submit(compile_mesh, s, length): s for s in order}
+        futs = {ex.submit(compile_mesh, s, length, 1, cache): s for s in order}
         for k, fut in enumerate(as_completed(futs)):
             h = fut.result()
             done[h.spec.index] = DeviceMesh(h, stream, own_outputs)

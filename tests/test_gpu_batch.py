@@ -316,3 +316,41 @@ def test_staged_outputs_share_an_arena():
     batch.close()
     for h in hosts:
         h.release()
+
+
+def test_meshes_loaded_from_the_table_cache_match_compiled(tmp_path):
+    """Meshes whose tables and coords come from table files (stored by a
+    first compile, then mapped) evaluate to the same bits as freshly
+    compiled ones, on the plain path and in a graph launch."""
+    from opensubdiv_amd.table_cache import TableCache
+    length = 7
+    cache = TableCache(tmp_path)
+    lib = _lib.load_hip()
+    old = lib.osdhip_config(b"table_csr_mirror", 0)
+    old_layout = lib.osdhip_config(b"table_layout", 2)
+    try:
+        for s in SPECS:
+            compile_mesh(s, length, cache=cache).release()
+        assert (cache.hits, cache.misses) == (0, len(SPECS))
+        cached = [compile_mesh(s, length, cache=cache) for s in SPECS]
+        assert cache.hits == len(SPECS)
+        dms = [DeviceMesh(h) for h in cached]
+    finally:
+        lib.osdhip_config(b"table_csr_mirror", old)
+        lib.osdhip_config(b"table_layout", old_layout)
+    for h in cached:
+        h.release()
+    _, fresh = build(length, 2)
+    results = []
+    for group in (dms, fresh):
+        batch = Batch(length, [s for d in group for s in d.stencil_segments()],
+                      [d.patch_segment() for d in group])
+        poison(group)
+        batch.run_plain()
+        plain = outputs(group)
+        poison(group)
+        batch.launch()
+        same(plain, outputs(group), "graph launch")
+        results.append(plain)
+        batch.close()
+    same(results[0], results[1], "cached vs compiled")
