@@ -82,6 +82,10 @@ __global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
     const float s = c.s * frac_inv - (float)pu;
     const float t = c.t * frac_inv - (float)pv;
 
+    // A negative point index (INDEX_INVALID of a patch built without
+    // fillMissingBoundaryPoints, far/patchBuilder.h:177-183) reads point 0
+    // instead of memory before the buffer; such points carry zero weight
+    // after the boundary fold.
     // The point count follows from the type alone (REGULAR 16, GREGORY_BASIS
     // 20, QUADS 4, anything else 0 -> zero outputs, as the reference's
     // Clear() without AddWithWeight), so the index and control-point loads
@@ -103,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
         float v[16][LC];
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
-            const float *p = a.src + (size_t)(unsigned)cvs[j] * sstride;
+            const float *p = a.src + (size_t)max(cvs[j], 0) * sstride;
 #pragma unroll
             for (int k = 0; k < LC; ++k) v[j][k] = p[k];
         }
@@ -115,7 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
             float v4[4][LC];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float *p = a.src + (size_t)(unsigned)cvs[16 + j] * sstride;
+                const float *p = a.src + (size_t)max(cvs[16 + j], 0) * sstride;
 #pragma unroll
                 for (int k = 0; k < LC; ++k) v4[j][k] = p[k];
             }
@@ -126,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
         float v[4][LC];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const float *p = a.src + (size_t)(unsigned)cvs[j] * sstride;
+            const float *p = a.src + (size_t)max(cvs[j], 0) * sstride;
 #pragma unroll
             for (int k = 0; k < LC; ++k) v[j][k] = p[k];
         }

@@ -277,3 +277,41 @@ def test_limit_normals_random_and_errors(hip):
     assert limit_normals(d_du, BD(1, 3, 5), d_dv, BD(0, 3, 4), out, BD(0, 3, 3), n)
     np.testing.assert_array_equal(host(out, shape=(n, 3)), want)
     assert not limit_normals(d_du, BD(1, 3, 5), d_dv, BD(0, 2, 4), out, BD(0, 3, 3), n)
+
+
+def test_invalid_phantom_indices_read_no_memory_outside_the_buffer(hip):
+    """Boundary patches whose phantom points carry INDEX_INVALID (-1), as a
+    PatchBuilder without fillMissingBoundaryPoints leaves them: the fold
+    gives those points zero weight, and the kernel reads point 0 in their
+    place, so the results equal the oracle's on the same table with any
+    valid fill (here the random indices)."""
+    rng = np.random.default_rng(77)
+    nsrc, n, L = 900, 4000, 3
+    arrays, indices, params = synthetic_patch_table(rng, nsrc)
+    holes = indices.copy()
+    phantom = {1: [0, 1, 2, 3], 2: [3, 7, 11, 15], 4: [12, 13, 14, 15], 8: [0, 4, 8, 12]}
+    count = 0
+    for arr in arrays:
+        if arr["regDesc"] != 6:
+            continue
+        for p in range(arr["numPatches"]):
+            pi = arr["primitiveIdBase"] + p
+            f1 = int(params[pi]["field1"])
+            if not (f1 >> 5) & 1:
+                continue
+            base = arr["indexBase"] + arr["stride"] * p
+            for bit, pts in phantom.items():
+                if (f1 >> 7) & bit:
+                    holes[base + np.array(pts)] = -1
+                    count += 1
+    assert count > 100
+    coords = synthetic_coords(rng, arrays, params, n)
+    src = rng.normal(size=(nsrc, L)).astype(np.float32)
+    want = [np.zeros((n, L), np.float32) for _ in range(3)]
+    assert oracle.eval_patches(src, (0, L, L), want, [(0, L, L)] * 3, coords, arrays, indices,
+                               params)
+    outs = [empty(n * L) for _ in range(3)]
+    assert eval_patches(hip, dev(src), (0, L, L), outs, [(0, L, L)] * 3, n, dev(coords),
+                        dev(arrays), dev(holes), dev(params)) == 0
+    for k in range(3):
+        np.testing.assert_array_equal(host(outs[k], shape=(n, L)), want[k], err_msg=f"output {k}")
