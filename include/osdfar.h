@@ -30,7 +30,8 @@ typedef struct osdfar_stencil_table osdfar_stencil_table;
 int osdfar_set_threads(int n);
 
 /* TopologyDescriptor + Sdc::Options::VtxBoundaryInterpolation
- * (0 none, 1 edge only, 2 edge and corner); creasing is uniform. Returns
+ * (0 none, 1 edge only, 2 edge and corner); creasing is uniform unless
+ * osdfar_topology_set_creasing selects Chaikin. Returns
  * NULL for input the compiler does not handle (non-manifold edges). */
 osdfar_topology *osdfar_topology_create(int numVertices, int numFaces,
                                         const int *numVertsPerFace,
@@ -41,6 +42,11 @@ osdfar_topology *osdfar_topology_create(int numVertices, int numFaces,
                                         const float *cornerWeights,
                                         int boundaryInterpolation);
 void osdfar_topology_destroy(osdfar_topology *topo);
+
+/* Sdc::Options::SetCreasingMethod (sdc/options.h:66-72): 0 uniform
+ * (default), 1 Chaikin.  Before the first refinement only: -2 afterwards,
+ * -1 for an unknown method. */
+int osdfar_topology_set_creasing(osdfar_topology *topo, int method);
 
 /* Refines uniformly to `level` and keeps every level's factorized vertex
  * stencils with the topology, so a later osdfar_stencil_table_create

@@ -31,6 +31,7 @@ def load_far():
         "osdfar_stencil_table_data": ([vp, vp, vp, vp], i),
         "osdfar_topology_create": ([i, i, vp, vp, i, vp, vp, i, vp, vp, i], vp),
         "osdfar_topology_destroy": ([vp], None),
+        "osdfar_topology_set_creasing": ([vp, i], i),
         "osdfar_refine_uniform": ([vp, i], i),
         "osdfar_level_info": ([vp, i, vp, vp, vp], i),
         "osdfar_level_face_vertices": ([vp, i, vp], i),
@@ -137,7 +138,7 @@ def set_threads(n):
 class Topology:
     """Control mesh + uniform refinement levels (TopologyRefiner)."""
 
-    def __init__(self, mesh, boundary=1):
+    def __init__(self, mesh, boundary=1, creasing=0):
         L = load_far()
         fv = np.ascontiguousarray(mesh.face_verts, np.int32).reshape(-1)
         nvpf = np.full(mesh.nfaces, mesh.face_verts.shape[1], np.int32)
@@ -152,6 +153,8 @@ class Topology:
                                           _p(cwv) if len(cv) else None, boundary)
         if not self.h:
             raise ValueError("topology rejected by the table compiler")
+        if creasing and L.osdfar_topology_set_creasing(self.h, int(creasing)) != 0:
+            raise ValueError(f"creasing method {creasing} not supported")
         self.num_control = mesh.nverts
 
     def refine_uniform(self, level):

@@ -81,6 +81,14 @@ osdfar_topology *osdfar_topology_create(int numVertices, int numFaces,
     return t.release();
 }
 
+int osdfar_topology_set_creasing(osdfar_topology *topo, int method)
+{
+    if (!topo || (method != CREASE_UNIFORM && method != CREASE_CHAIKIN)) return -1;
+    if (topo->refiner.maxLevel() > 0 || !topo->levels.empty()) return -2;
+    topo->refiner.creasing = method;
+    return 0;
+}
+
 void osdfar_topology_destroy(osdfar_topology *topo) { delete topo; }
 
 int osdfar_topology_cache_stencils(osdfar_topology *topo, int level)

@@ -1,8 +1,8 @@
 // Factorized stencils of uniformly refined Catmark levels.
 //
 // Masks restate Sdc's Catmark rules (sdc/catmarkScheme.h:71-210) with
-// uniform creasing and fractional blending (sdc/scheme.h:392-600,
-// sdc/crease.cpp:44-100), applied level by level as Far's PrimvarRefiner
+// uniform or Chaikin creasing and fractional blending (sdc/scheme.h:392-600,
+// sdc/crease.cpp:44-198), applied level by level as Far's PrimvarRefiner
 // does (far/primvarRefiner.h:475-660): face points, then edge points (which
 // use the new face points), then vertex points.  Each new vertex's row is the
 // mask-weighted sum of its parents' rows, so every row refers to control
@@ -322,10 +322,8 @@ static int levelRows(const Refiner &r, int mode, int l, const LevelStencils *PS,
         const int nfv = P.vfOff[v + 1] - P.vfOff[v];
         float pesb[kMaxValence], cesb[kMaxValence];
         const int nn = std::min(n, kMaxValence);
-        for (int i = 0; i < nn; ++i) {
-            pesb[i] = P.esharp[P.ve[P.veOff[v] + i]];
-            cesb[i] = decrementSharpness(pesb[i]);
-        }
+        for (int i = 0; i < nn; ++i) pesb[i] = P.esharp[P.ve[P.veOff[v] + i]];
+        subdivideSharpnessAroundVertex(r.creasing, nn, pesb, cesb);
         const float pvs = P.vsharp[v], cvs = decrementSharpness(pvs);
         const Rule pr = ruleFrom(pvs, nn, pesb);
         VMask m;
@@ -336,12 +334,13 @@ static int levelRows(const Refiner &r, int mode, int l, const LevelStencils *PS,
             ruleMask(pr, nn, nfv, pesb, m);
             const Rule cr = ruleFrom(cvs, nn, cesb);
             if (cr != pr) {
-                // fractional blend (sdc/crease.cpp:62-95, uniform creasing)
+                // fractional blend (sdc/crease.cpp:62-101): the parent
+                // sharpness of the vertex and of every edge that turns smooth
                 int cnt = 0;
                 double sum = 0.0;
                 if (pvs > 0.0f && cvs <= 0.0f) { cnt = 1; sum = pvs; }
                 for (int i = 0; i < nn; ++i)
-                    if (pesb[i] > 0.0f && pesb[i] <= 1.0f) { sum += pesb[i]; ++cnt; }
+                    if (pesb[i] > 0.0f && cesb[i] <= 0.0f) { sum += pesb[i]; ++cnt; }
                 const double pw = cnt ? std::min(1.0, sum / cnt) : 0.0;
                 VMask cm;
                 cm.init(n, nfv);
@@ -358,11 +357,18 @@ static int levelRows(const Refiner &r, int mode, int l, const LevelStencils *PS,
             if (m.e[i] != 0.0) acc.addRow(PS, P.otherVertex(P.ve[P.veOff[v] + i], v), m.e[i]);
         acc.addRow(PS, v, m.v);
     };
+    // the two child edges of parent edge e are child edges eFromE + 2e + {0,1}
+    const Level &C = r.level(l);
+    const size_t eFromE = P.fv.size();
     auto edgeRow = [&](int e, Accum &acc) {
         const int a = P.ev[2 * e], b = P.ev[2 * e + 1];
         const int nfe = P.efOff[e + 1] - P.efOff[e];
         const float s = P.esharp[e];
-        if (mode == INTERPOLATE_VARYING || (s > 0.0f && (decrementSharpness(s) > 0.0f || nfe != 2))) {
+        // the child vertex is a crease when both child edges stay sharp
+        // (vtr/refinement.cpp:960-993); uniform creasing: when s > 1
+        const bool childCrease = C.esharp[eFromE + 2 * (size_t)e] > 0.0f &&
+                                 C.esharp[eFromE + 2 * (size_t)e + 1] > 0.0f;
+        if (mode == INTERPOLATE_VARYING || (s > 0.0f && (childCrease || nfe != 2))) {
             // crease (child vertex is a crease) or linear varying rule
             acc.addRow(PS, a, 0.5);
             acc.addRow(PS, b, 0.5);
@@ -370,7 +376,9 @@ static int levelRows(const Refiner &r, int mode, int l, const LevelStencils *PS,
         }
         double vw = 0.25, fw = nfe == 2 ? 0.25 : 0.5 / (double)nfe;
         if (s > 0.0f) {
-            // parent crease, child smooth: blend by the edge sharpness
+            // parent crease, child smooth or dart: blend by the edge's
+            // sharpness, unclamped (sdc/scheme.h:451-470); above 1 only
+            // with Chaikin creasing, where one child edge can turn smooth
             const double p = (double)s, c = 1.0 - p;
             vw = p * 0.5 + c * vw;
             fw *= c;

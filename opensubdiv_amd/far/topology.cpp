@@ -110,6 +110,8 @@ bool Refiner::create(const TopologyDesc &d)
         for (int v = 0; v < L.nv; ++v)
             if (L.vfOff[v + 1] - L.vfOff[v] == 1) L.vsharp[v] = kSharpInfinite;
 
+    creasing = d.creasing == CREASE_CHAIKIN ? CREASE_CHAIKIN : CREASE_UNIFORM;
+
     // ptex faces (far/ptexIndices.cpp): one per quad, one per corner of an n-gon
     ptexBase.resize(L.nf);
     ptexCount = 0;
@@ -210,8 +212,22 @@ void Refiner::refineOnce()
             C.ev[2 * (eFromE + 2 * e + j) + 1] = vFromV + P.ev[2 * e + j];
         }
     C.esharp.assign(C.ne, 0.0f);
-    for (int e = 0; e < P.ne; ++e)
-        C.esharp[eFromE + 2 * e] = C.esharp[eFromE + 2 * e + 1] = decrementSharpness(P.esharp[e]);
+    if (creasing == CREASE_CHAIKIN) {
+        // the child edge at parent vertex v subdivides the parent's sharpness
+        // with the edges around v (vtr/refinement.cpp:870-930)
+        std::vector<float> ps, cs;
+        for (int v = 0; v < P.nv; ++v) {
+            const int n = P.valence(v);
+            ps.resize(n);
+            cs.resize(n);
+            for (int i = 0; i < n; ++i) ps[i] = P.esharp[P.ve[P.veOff[v] + i]];
+            subdivideSharpnessAroundVertex(creasing, n, ps.data(), cs.data());
+            for (int i = 0; i < n; ++i) C.esharp[edgeChild(P.ve[P.veOff[v] + i], v)] = cs[i];
+        }
+    } else {
+        for (int e = 0; e < P.ne; ++e)
+            C.esharp[eFromE + 2 * e] = C.esharp[eFromE + 2 * e + 1] = decrementSharpness(P.esharp[e]);
+    }
     C.vsharp.assign(C.nv, 0.0f);
     for (int v = 0; v < P.nv; ++v) C.vsharp[vFromV + v] = decrementSharpness(P.vsharp[v]);
     C.buildIncidence();

@@ -1,7 +1,7 @@
 // Host-side topology levels and uniform Catmull-Clark refinement for the
 // table compiler (libosdfar).  Scope: manifold Catmark meshes with n-gon
 // faces at level 0, semi-sharp and infinitely sharp creases, corner
-// sharpness, uniform creasing and boundary edges (sharpened).
+// sharpness, uniform or Chaikin creasing and boundary edges (sharpened).
 //
 // Numbering follows OpenSubdiv's Vtr uniform refinement so refined vertex i
 // here is refined vertex i of Far (vtr/refinement.cpp:253-270: children of
@@ -16,6 +16,7 @@ namespace osdfar {
 constexpr float kSharpInfinite = 10.0f;   // Sdc::Crease::SHARPNESS_INFINITE
 
 enum Rule { RULE_SMOOTH = 1, RULE_DART = 2, RULE_CREASE = 4, RULE_CORNER = 8 };
+enum Creasing { CREASE_UNIFORM = 0, CREASE_CHAIKIN = 1 };   // Sdc::Options::CreasingMethod
 
 struct Level {
     int nv = 0, nf = 0, ne = 0;
@@ -46,6 +47,7 @@ struct TopologyDesc {
     const int *cornerVertexIndices = nullptr;
     const float *cornerWeights = nullptr;
     int boundaryInterpolation = 1;   // 0 none, 1 edge only, 2 edge and corner
+    int creasing = CREASE_UNIFORM;
 };
 
 // Where a refined face came from: the base (ptex) face, depth and integer
@@ -73,6 +75,7 @@ public:
     // ptex index of each base face (one per quad; n-gons have n ptex faces)
     std::vector<int> ptexBase;
     int ptexCount = 0;
+    int creasing = CREASE_UNIFORM;
 
 private:
     void refineOnce();
@@ -84,6 +87,35 @@ inline float decrementSharpness(float s)
     if (s >= kSharpInfinite) return kSharpInfinite;
     if (s > 1.0f) return s - 1.0f;
     return 0.0f;
+}
+
+// Child sharpness of the n edges around one vertex, given the parents'
+// (Sdc::Crease::SubdivideEdgeSharpnessesAroundVertex, sdc/crease.cpp:
+// 138-198).  Uniform: each decremented by 1.  Chaikin: a semi-sharp edge
+// takes 3/4 of its own sharpness plus 1/4 of the mean of the other
+// semi-sharp edges at the vertex, minus 1; with no other semi-sharp edge it
+// is decremented; smooth and infinitely sharp edges keep their state.
+inline void subdivideSharpnessAroundVertex(int creasing, int n, const float *parent,
+                                           float *child)
+{
+    int count = 0;
+    float sum = 0.0f;
+    if (creasing == CREASE_CHAIKIN && n >= 2)
+        for (int i = 0; i < n; ++i)
+            if (parent[i] > 0.0f && parent[i] < kSharpInfinite) {
+                ++count;
+                sum += parent[i];
+            }
+    for (int i = 0; i < n; ++i) {
+        const float p = parent[i];
+        if (count < 2 || p <= 0.0f || p >= kSharpInfinite) {
+            child[i] = decrementSharpness(p);
+        } else {
+            const float others = (sum - p) / (float)(count - 1);
+            const float c = (0.75f * p + 0.25f * others) - 1.0f;
+            child[i] = c > 0.0f ? c : 0.0f;
+        }
+    }
 }
 
 }  // namespace osdfar

@@ -101,8 +101,36 @@ def test_uniform_matches_far_live(level, intermediate, mode):
     assert_same_rows(mine, theirs)
 
 
-def _my_patch_eval(mesh, iso, f, u, v, nout=3):
-    t = far.Topology(mesh)
+@needs_ref
+@pytest.mark.parametrize("level", [1, 2, 4])
+def test_uniform_chaikin_creasing_matches_far_live(level):
+    """Sdc CREASE_CHAIKIN: child edge sharpness from the edges around each
+    vertex, vertex and edge points with Chaikin's child rules; dense creases
+    so that most creased vertices have several semi-sharp edges."""
+    m = meshes.cube_sphere(3, seed=7, crease_fraction=0.6)
+    t = far.Topology(m, creasing=1)
+    t.refine_uniform(level)
+    mine = t.stencil_table(intermediate=True)
+    theirs = ref.stencils_uniform(ref.Topology(m, creasing=1), level, intermediate=True)
+    assert_same_rows(mine, theirs)
+    uniform = far.Topology(m)
+    uniform.refine_uniform(level)
+    u = uniform.stencil_table(intermediate=True)
+    assert not (np.array_equal(u["indices"], mine["indices"]) and
+                np.allclose(u["weights"], mine["weights"], atol=1e-6))   # the rule matters here
+
+
+def test_creasing_method_errors():
+    m = meshes.cube_sphere(2, seed=1)
+    with pytest.raises(ValueError):
+        far.Topology(m, creasing=2)
+    t = far.Topology(m)
+    t.refine_uniform(1)
+    assert far.load_far().osdfar_topology_set_creasing(t.h, 1) == -2   # after refinement
+
+
+def _my_patch_eval(mesh, iso, f, u, v, nout=3, creasing=0):
+    t = far.Topology(mesh, creasing=creasing)
     pt = t.patch_table(iso)
     st = pt.point_stencils(double=True)
     ctrl = mesh.positions.astype(np.float64)
@@ -153,12 +181,13 @@ def test_patch_map_covers_every_ptex_face_and_is_consistent():
 
 
 @needs_ref
-@pytest.mark.parametrize("n,seed,crease,iso", [(3, 1, 0.0, 2), (5, 2, 0.3, 4), (4, 5, 0.05, 3)])
-def test_patch_table_matches_far_live(n, seed, crease, iso):
+@pytest.mark.parametrize("n,seed,crease,iso,creasing", [(3, 1, 0.0, 2, 0), (5, 2, 0.3, 4, 0),
+                                                         (4, 5, 0.05, 3, 0), (4, 3, 0.6, 3, 1)])
+def test_patch_table_matches_far_live(n, seed, crease, iso, creasing):
     m = meshes.cube_sphere(n, seed=seed, crease_fraction=crease)
     f, u, v = meshes.random_patch_locations(m.nfaces, 2000, seed=seed + 10)
-    pt, _, outs = _my_patch_eval(m, iso, f, u, v)
-    P = ref.Patches(ref.Topology(m), iso, double=True)
+    pt, _, outs = _my_patch_eval(m, iso, f, u, v, creasing=creasing)
+    P = ref.Patches(ref.Topology(m, creasing=creasing), iso, double=True)
     assert [tuple(a) for a in pt.arrays] == [tuple(a) for a in P.arrays]
     src = np.zeros((P.num_vertices_total, 3))
     src[:P.num_coarse] = m.positions
