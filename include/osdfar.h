@@ -89,8 +89,9 @@ void osdfar_stencil_table_destroy(osdfar_stencil_table *t);
 /* ---- feature-adaptive patch tables --------------------------------------
  * PatchTableFactory::Create(refiner, Options(isoLevel)) with
  * ENDCAP_GREGORY_BASIS (far/patchTableFactory.h:40-160) for closed,
- * manifold Catmark meshes with quad base faces and semi-sharp creases
- * (returns NULL otherwise).  Refines the topology uniformly to isoLevel.
+ * manifold Catmark meshes with semi-sharp creases, quad or n-gon base faces
+ * (n-gons need isoLevel >= 1; their patches carry PatchParam's non-quad
+ * bit), uniform or Chaikin creasing (returns NULL otherwise).  Refines the topology uniformly to isoLevel.
  * The patch points live in a vertex buffer laid out as
  *   [control vertices | refined vertices | end-cap points]
  * whose non-control rows are given by osdfar_patch_table_point_stencils. */

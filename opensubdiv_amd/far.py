@@ -135,13 +135,25 @@ def set_threads(n):
     load_far().osdfar_set_threads(int(n))
 
 
+def face_arrays(mesh):
+    """(face-vertex indices, vertices per face) of a mesh: face_verts (F, n)
+    for faces of one size, or flat face_verts with face_sizes for mixed
+    sizes (n-gons)."""
+    sizes = getattr(mesh, "face_sizes", None)
+    fv = np.ascontiguousarray(mesh.face_verts, np.int32).reshape(-1)
+    if sizes is None:
+        nvpf = np.full(mesh.nfaces, np.asarray(mesh.face_verts).shape[1], np.int32)
+    else:
+        nvpf = np.ascontiguousarray(sizes, np.int32)
+    return fv, nvpf
+
+
 class Topology:
     """Control mesh + uniform refinement levels (TopologyRefiner)."""
 
     def __init__(self, mesh, boundary=1, creasing=0):
         L = load_far()
-        fv = np.ascontiguousarray(mesh.face_verts, np.int32).reshape(-1)
-        nvpf = np.full(mesh.nfaces, mesh.face_verts.shape[1], np.int32)
+        fv, nvpf = face_arrays(mesh)
         cp = np.ascontiguousarray(mesh.crease_pairs, np.int32).reshape(-1)
         cw = np.ascontiguousarray(mesh.crease_weights, np.float32)
         cv = np.ascontiguousarray(mesh.corner_verts, np.int32)
@@ -298,8 +310,7 @@ def hash64(data, seed=0):
 def topology_key(mesh, recipe="", boundary=1):
     """64-bit key of a mesh's topology (the arrays Topology() compiles) and a
     recipe string naming what was built from it."""
-    fv = np.ascontiguousarray(mesh.face_verts, np.int32).reshape(-1)
-    nvpf = np.full(mesh.nfaces, mesh.face_verts.shape[1], np.int32)
+    fv, nvpf = face_arrays(mesh)
     cp = np.ascontiguousarray(mesh.crease_pairs, np.int32).reshape(-1)
     cw = np.ascontiguousarray(mesh.crease_weights, np.float32)
     cv = np.ascontiguousarray(mesh.corner_verts, np.int32)

@@ -254,7 +254,7 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
     {
         const Level &L0 = refiner.level(0);
         for (int f = 0; f < L0.nf; ++f)
-            if (L0.faceSize(f) != 4) return false;             // quad base faces only
+            if (L0.faceSize(f) != 4 && isoLevel < 1) return false;   // n-gons need a level
         for (int e = 0; e < L0.ne; ++e)
             if (L0.efOff[e + 1] - L0.efOff[e] != 2 || L0.esharp[e] >= kSharpInfinite) return false;
         for (int v = 0; v < L0.nv; ++v)
@@ -280,7 +280,7 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
         for (int f = 0; f < L.nf; ++f) {
             if (!complete[f]) continue;
             if (d < maxL && faceHasFeatures(L, f)) {
-                for (int j = 0; j < 4; ++j) next[L.fvOff[f] + j] = 1;
+                for (int j = 0; j < L.faceSize(f); ++j) next[L.fvOff[f] + j] = 1;
             } else {
                 leaves.push_back({ d, f, !isExtraordinary(L, f) });
                 maxDepth = std::max(maxDepth, d);
@@ -334,8 +334,10 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
             v = o.v;
         }
         p.field0 = (uint32_t)ptex & 0xfffffffu;
-        p.field1 = (uint32_t)lf.level | ((uint32_t)(regular ? 1 : 0) << 5) |
-                   ((uint32_t)v << 12) | ((uint32_t)u << 22);
+        // far/patchBuilder.cpp:1272-1380: depth = level; under an n-gon the
+        // (u,v) bits count from level 1, flagged by the non-quad bit
+        p.field1 = (uint32_t)lf.level | ((uint32_t)refiner.ptexNonQuad[ptex] << 4) |
+                   ((uint32_t)(regular ? 1 : 0) << 5) | ((uint32_t)v << 12) | ((uint32_t)u << 22);
         p.sharpness = 0.0f;
         return p;
     };
@@ -406,12 +408,15 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
         const uint32_t f1 = params[p].field1;
         const int ptex = (int)(params[p].field0 & 0xfffffffu);
         const int depth = (int)(f1 & 0xf), u = (int)((f1 >> 22) & 0x3ff), v = (int)((f1 >> 12) & 0x3ff);
+        // a patch at the root depth (0, or 1 under an n-gon) covers its
+        // whole ptex face (far/patchMap.cpp:155-180)
+        const int rootDepth = (int)((f1 >> 4) & 1);
         int node = rootOf(ptex);
-        if (depth == 0) {
+        if (depth == rootDepth) {
             for (int q = 0; q < 4; ++q) nodes[node].child[q] = -(p + 1);
             continue;
         }
-        for (int k = depth - 1; k >= 0; --k) {
+        for (int k = depth - rootDepth - 1; k >= 0; --k) {
             const int q = (((v >> k) & 1) << 1) | ((u >> k) & 1);
             if (k == 0) {
                 nodes[node].child[q] = -(p + 1);
@@ -583,19 +588,27 @@ void uniformVertexLocations(const Refiner &r, int level, std::vector<int> &ptex,
     ptex.assign(L.nv, -1);
     u.assign(L.nv, 0.0);
     v.assign(L.nv, 0.0);
-    const double scale = 1.0 / (double)(1 << level);
     const int du[4] = { 0, 1, 1, 0 }, dv[4] = { 0, 0, 1, 1 };
     for (int f = 0; f < L.nf; ++f) {
-        int p = 0, fu = 0, fv = 0, nonquadShift = 0;
+        int p = 0, fu = 0, fv = 0;
         if (level == 0) {
             p = r.ptexBase[f];
+            if (L.faceSize(f) != 4) {
+                // corner k of an n-gon is the origin of its ptex face k
+                for (int k = 0; k < L.faceSize(f); ++k) {
+                    const int vert = L.fv[L.fvOff[f] + k];
+                    if (ptex[vert] >= 0) continue;
+                    ptex[vert] = p + k;
+                }
+                continue;
+            }
         } else {
             const FaceOrigin &o = r.origins[level][f];
             p = o.ptex;
             fu = o.u;
             fv = o.v;
         }
-        (void)nonquadShift;
+        const double scale = 1.0 / (double)(1 << (level - (level ? r.ptexNonQuad[p] : 0)));
         for (int k = 0; k < 4; ++k) {
             const int vert = L.fv[L.fvOff[f] + k];
             if (ptex[vert] >= 0) continue;

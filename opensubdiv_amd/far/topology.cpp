@@ -119,6 +119,10 @@ bool Refiner::create(const TopologyDesc &d)
         ptexBase[f] = ptexCount;
         ptexCount += L.faceSize(f) == 4 ? 1 : L.faceSize(f);
     }
+    ptexNonQuad.assign(ptexCount, 0);
+    for (int f = 0; f < L.nf; ++f)
+        if (L.faceSize(f) != 4)
+            for (int j = 0; j < L.faceSize(f); ++j) ptexNonQuad[ptexBase[f] + j] = 1;
     return true;
 }
 

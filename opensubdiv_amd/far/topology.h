@@ -75,6 +75,9 @@ public:
     // ptex index of each base face (one per quad; n-gons have n ptex faces)
     std::vector<int> ptexBase;
     int ptexCount = 0;
+    // 1 for the ptex faces of n-gon base faces (one per corner), whose
+    // patches carry PatchParam's non-quad bit
+    std::vector<uint8_t> ptexNonQuad;
     int creasing = CREASE_UNIFORM;
 
 private:

@@ -73,7 +73,9 @@ class Topology:
 
     def __init__(self, mesh, boundary=1, creasing=0):
         fv = np.ascontiguousarray(mesh.face_verts, np.int32).reshape(-1)
-        nvpf = np.full(mesh.nfaces, 4, np.int32)
+        sizes = getattr(mesh, "face_sizes", None)
+        nvpf = np.full(mesh.nfaces, 4, np.int32) if sizes is None else \
+            np.ascontiguousarray(sizes, np.int32)
         cp = np.ascontiguousarray(mesh.crease_pairs, np.int32).reshape(-1)
         cw = np.ascontiguousarray(mesh.crease_weights, np.float32)
         cv = np.ascontiguousarray(mesh.corner_verts, np.int32)

@@ -249,3 +249,151 @@ def test_vertex_locations_reproduce_uniform_limit_positions():
     if ref.available():
         lim, _, _ = ref.limit_points_uniform(ref.Topology(m), 3, m.positions)
         assert bbox_rel_err(P, lim) < 1e-6
+
+
+class PolyMesh:
+    """A mesh with faces of any size (flat face_verts + face_sizes)."""
+
+    def __init__(self, faces, creases=(), corners=()):
+        self.face_sizes = np.array([len(f) for f in faces], np.int32)
+        self.face_verts = np.concatenate([np.asarray(f, np.int32) for f in faces])
+        self.nfaces = len(faces)
+        self.nverts = int(self.face_verts.max()) + 1
+        rng = np.random.default_rng(len(faces))
+        self.positions = rng.normal(size=(self.nverts, 3))
+        self.crease_pairs = np.array([c[:2] for c in creases], np.int32).reshape(-1, 2)
+        self.crease_weights = np.array([c[2] for c in creases], np.float32)
+        self.corner_verts = np.array([c[0] for c in corners], np.int32)
+        self.corner_weights = np.array([c[1] for c in corners], np.float32)
+
+
+def _check_oriented(m):
+    """Every edge of a closed mesh appears once in each direction; of an open
+    one, at most once per direction."""
+    seen = set()
+    k = 0
+    for n in m.face_sizes:
+        f = m.face_verts[k:k + n]
+        k += n
+        for i in range(n):
+            e = (int(f[i]), int(f[(i + 1) % n]))
+            assert e not in seen
+            seen.add(e)
+
+
+def pentagonal_prism():
+    bottom = [4, 3, 2, 1, 0]
+    top = [5, 6, 7, 8, 9]
+    sides = [[i, (i + 1) % 5, 5 + (i + 1) % 5, 5 + i] for i in range(5)]
+    return PolyMesh([bottom, top] + sides)
+
+
+def cube_with_triangles():
+    q = meshes.unit_cube(0).face_verts.tolist()
+    a, b, c, d = q[0]
+    e, f, g, h = q[1]
+    return PolyMesh([[a, b, c], [a, c, d], [e, f, g, h]] + q[2:],
+                    creases=[(a, c, 1.5)])
+
+
+def open_grid(n=4, creases=()):
+    idx = lambda i, j: i * (n + 1) + j
+    faces = [[idx(i, j), idx(i, j + 1), idx(i + 1, j + 1), idx(i + 1, j)]
+             for i in range(n) for j in range(n)]
+    return PolyMesh(faces, creases=creases)
+
+
+def sharp_cube():
+    q = meshes.unit_cube(0).face_verts.tolist()
+    top = q[1]
+    loop = [(top[i], top[(i + 1) % 4], 10.0) for i in range(4)]
+    return PolyMesh(q, creases=loop + [(q[0][0], q[0][1], 2.3)],
+                    corners=[(q[0][2], 2.5), (q[0][3], 10.0)])
+
+
+POLY_MESHES = {"pentagonal-prism": pentagonal_prism, "cube-with-triangles": cube_with_triangles,
+               "open-grid": open_grid,
+               "open-grid-creased": lambda: open_grid(5, creases=[(7, 8, 2.0), (8, 9, 0.6)]),
+               "sharp-cube": sharp_cube}
+
+
+@needs_ref
+@pytest.mark.parametrize("name", list(POLY_MESHES))
+@pytest.mark.parametrize("boundary,creasing", [(1, 0), (2, 0), (1, 1)])
+def test_uniform_general_topology_matches_far_live(name, boundary, creasing):
+    """n-gon base faces, boundaries (edge-only and edge-and-corner
+    interpolation), infinitely sharp creases and corners, both creasing
+    methods: uniform tables with every intermediate level, row for row."""
+    m = POLY_MESHES[name]()
+    _check_oriented(m)
+    t = far.Topology(m, boundary=boundary, creasing=creasing)
+    t.refine_uniform(3)
+    mine = t.stencil_table(intermediate=True)
+    theirs = ref.stencils_uniform(ref.Topology(m, boundary=boundary, creasing=creasing), 3,
+                                  intermediate=True)
+    assert_same_rows(mine, theirs)
+
+
+@needs_ref
+@pytest.mark.parametrize("name,iso", [("pentagonal-prism", 3), ("cube-with-triangles", 2),
+                                      ("cube-with-triangles", 4)])
+def test_patch_table_with_ngon_base_faces_matches_far_live(name, iso):
+    """Adaptive patch tables of meshes with n-gon base faces: the patches of
+    an n-gon's sub-faces carry the non-quad bit, their (u,v) count from
+    level 1 and the patch map roots them at depth 1; same arrays and the
+    same limit surface as Far at random locations of every ptex face."""
+    m = POLY_MESHES[name]()
+    nptex = int(sum(1 if n == 4 else n for n in m.face_sizes))
+    f, u, v = meshes.random_patch_locations(nptex, 3000, seed=iso)
+    u[:nptex], v[:nptex] = 0.5, 0.5
+    f[:nptex] = np.arange(nptex)
+    pt, coords, outs = _my_patch_eval(m, iso, f, u, v)
+    assert ((pt.params["field1"] >> 4) & 1).any()          # non-quad patches present
+    P = ref.Patches(ref.Topology(m), iso, double=True)
+    assert [tuple(a) for a in pt.arrays] == [tuple(a) for a in P.arrays]
+    np.testing.assert_array_equal(pt.params["field1"], P.params["field1"])
+    src = np.zeros((P.num_vertices_total, 3))
+    src[:P.num_coarse] = m.positions
+    st = P.stencils
+    src[P.num_coarse:] = oracle.apply_stencils(m.positions, st["sizes"], st["offsets"],
+                                               st["indices"], st["weights"], np.float64)
+    h = P.find(f, u, v)
+    assert (h[:, 0] >= 0).all()
+    np.testing.assert_array_equal(coords["patchIndex"], h[:, 1])
+    theirs = np.zeros(len(f), PATCH_COORD_DTYPE)
+    theirs["arrayIndex"], theirs["patchIndex"], theirs["vertIndex"] = h[:, 0], h[:, 1], h[:, 2]
+    theirs["s"], theirs["t"] = u, v
+    o = [np.zeros((len(f), 3)) for _ in range(3)]
+    assert oracle.eval_patches_f64(src, (0, 3, 3), o, [(0, 3, 3)] * 3, theirs, P.arrays,
+                                   P.indices, P.params)
+    for a, b in zip(outs, o):
+        assert bbox_rel_err(a, b) < 1e-6
+
+
+@needs_ref
+@pytest.mark.parametrize("name", ["pentagonal-prism", "cube-with-triangles"])
+def test_limit_stencils_and_vertex_locations_with_ngons_match_far_live(name):
+    """Limit stencils (P, derivatives) at random ptex locations against
+    Far::LimitStencilTableFactory, and the limit stencils at every level-3
+    vertex location against PrimvarRefiner::Limit of the uniform level."""
+    m = POLY_MESHES[name]()
+    nptex = int(sum(1 if n == 4 else n for n in m.face_sizes))
+    f, u, v = meshes.random_patch_locations(nptex, 500, seed=3)
+    pt = far.Topology(m).patch_table(3)
+    mine = pt.limit_stencils(f, u, v, second=True, double=True)
+    theirs = ref.stencils_limit(ref.Topology(m), 3, f, u, v, d2=True, double=True)
+    ctrl = m.positions.astype(np.float64)
+    for k in ("weights", "du", "dv", "duu", "duv", "dvv"):
+        a = oracle.apply_stencils(ctrl, mine["sizes"], mine["offsets"], mine["indices"], mine[k],
+                                  np.float64)
+        b = oracle.apply_stencils(ctrl, theirs["sizes"], theirs["offsets"], theirs["indices"],
+                                  theirs[k], np.float64)
+        assert bbox_rel_err(a, b) < 1e-9, k
+    t = far.Topology(m)
+    t.refine_uniform(3)
+    p, u3, v3 = t.vertex_locations(3)
+    ls = far.Topology(m).patch_table(3).limit_stencils(p, u3, v3, double=True)
+    P = oracle.apply_stencils(ctrl, ls["sizes"], ls["offsets"], ls["indices"], ls["weights"],
+                              np.float64)
+    lim, _, _ = ref.limit_points_uniform(ref.Topology(m), 3, ctrl)
+    assert bbox_rel_err(P, lim) < 1e-9
