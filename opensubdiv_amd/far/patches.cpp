@@ -206,13 +206,18 @@ bool gregoryPoints(const Level &L, int f, Row out[20])
 }
 
 // Far's feature test for refining face f of a level (scope: manifold,
-// interior): non-quad, extraordinary corner, or a corner with vertex
-// sharpness or an incident sharp edge.
-bool faceHasFeatures(const Level &L, int f)
+// interior; far/topologyRefiner.cpp:627-690): non-quad, a base-level corner
+// shared with a non-quad face, an extraordinary corner, or a corner with
+// vertex sharpness or an incident sharp edge.
+bool faceHasFeatures(const Level &L, int f, bool baseLevel)
 {
     if (L.faceSize(f) != 4) return true;
     for (int k = L.fvOff[f]; k < L.fvOff[f + 1]; ++k) {
         const int v = L.fv[k];
+        // a base-level corner shared with an n-gon (Far's _incidIrregFace tag)
+        if (baseLevel)
+            for (int i = L.vfOff[v]; i < L.vfOff[v + 1]; ++i)
+                if (L.faceSize(L.vf[i]) != 4) return true;
         if (L.valence(v) != 4 || (L.vfOff[v + 1] - L.vfOff[v]) != 4) return true;
         if (L.vsharp[v] > 0.0f) return true;
         for (int i = L.veOff[v]; i < L.veOff[v + 1]; ++i)
@@ -279,7 +284,7 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
         if (d < maxL) next.assign(refiner.level(d + 1).nf, 0);
         for (int f = 0; f < L.nf; ++f) {
             if (!complete[f]) continue;
-            if (d < maxL && faceHasFeatures(L, f)) {
+            if (d < maxL && faceHasFeatures(L, f, d == 0)) {
                 for (int j = 0; j < L.faceSize(f); ++j) next[L.fvOff[f] + j] = 1;
             } else {
                 leaves.push_back({ d, f, !isExtraordinary(L, f) });

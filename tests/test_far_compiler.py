@@ -311,7 +311,17 @@ def sharp_cube():
                     corners=[(q[0][2], 2.5), (q[0][3], 10.0)])
 
 
-POLY_MESHES = {"pentagonal-prism": pentagonal_prism, "cube-with-triangles": cube_with_triangles,
+def sphere_with_split_quad():
+    """A cube-sphere with one interior quad split along a diagonal: the two
+    other corners keep valence 4 but touch triangles."""
+    m = meshes.cube_sphere(4, seed=2, crease_fraction=0.0)
+    q = m.face_verts.tolist()
+    a, b, c, d = q[5]
+    return PolyMesh(q[:5] + [[a, b, c], [a, c, d]] + q[6:])
+
+
+POLY_MESHES = {"sphere-with-split-quad": sphere_with_split_quad,
+               "pentagonal-prism": pentagonal_prism, "cube-with-triangles": cube_with_triangles,
                "open-grid": open_grid,
                "open-grid-creased": lambda: open_grid(5, creases=[(7, 8, 2.0), (8, 9, 0.6)]),
                "sharp-cube": sharp_cube}
@@ -336,7 +346,7 @@ def test_uniform_general_topology_matches_far_live(name, boundary, creasing):
 
 @needs_ref
 @pytest.mark.parametrize("name,iso", [("pentagonal-prism", 3), ("cube-with-triangles", 2),
-                                      ("cube-with-triangles", 4)])
+                                      ("cube-with-triangles", 4), ("sphere-with-split-quad", 3)])
 def test_patch_table_with_ngon_base_faces_matches_far_live(name, iso):
     """Adaptive patch tables of meshes with n-gon base faces: the patches of
     an n-gon's sub-faces carry the non-quad bit, their (u,v) count from
