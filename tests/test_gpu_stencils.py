@@ -408,3 +408,52 @@ def test_tiled_layout_edge_tables(hip):
             continue
         want = oracle.apply_stencils(src, t["sizes"], t["offsets"], t["indices"], t["weights"])
         np.testing.assert_array_equal(host(out, shape=(n, L)), want, err_msg=f"case {k}")
+
+
+@pytest.mark.parametrize("layout", [0, 2])
+def test_hbr_regression_shapes_on_the_device(hip, layout):
+    """The reference's regression shapes (regression/hbr_regression): level-3
+    tables (every intermediate level) from the table compiler, evaluated on
+    the device in float32 in place after the control points, equal the
+    oracle's float32 sums bit for bit and Hbr's baseline positions within
+    1e-5 of the bounding-box diagonal (matched one-to-one by nearest
+    neighbour: Hbr numbers vertices in another order)."""
+    from scipy.spatial import cKDTree
+    from opensubdiv_amd import HipStencilTable, far
+    g = golden("hbr_catmark.npz")
+    names = sorted({k.split("/")[0] for k in g.files})
+    old = hip.osdhip_config(b"table_layout", layout)
+    try:
+        for name in names:
+            class Shape:
+                pass
+            m = Shape()
+            for k in ("positions", "face_sizes", "face_verts", "crease_pairs", "crease_weights",
+                      "corner_verts", "corner_weights"):
+                setattr(m, k, g[f"{name}/{k}"])
+            m.nverts, m.nfaces = len(m.positions), len(m.face_sizes)
+            b, c = (int(x) for x in g[f"{name}/options"])
+            t = far.Topology(m, boundary=b, creasing=c)
+            t.refine_uniform(3)
+            st = t.stencil_table(intermediate=True)
+            n = len(st["sizes"])
+            ctrl = np.ascontiguousarray(m.positions, np.float32)
+            want = oracle.apply_stencils(ctrl, st["sizes"], st["offsets"], st["indices"],
+                                         st["weights"])
+            buf = np.full((m.nverts + n, 3), np.nan, np.float32)
+            buf[:m.nverts] = ctrl
+            d = dev(buf)
+            h = HipStencilTable.Create(st)
+            assert table_eval(hip, h.h, d, (0, 3, 3), [d], [(3 * m.nverts, 3, 3)], 0, n) == 0
+            got = host(d, shape=(m.nverts + n, 3))
+            np.testing.assert_array_equal(got[m.nverts:], want, err_msg=name)
+            for level in ((1,) if name == "catmark_chaikin1" else (1, 2, 3)):
+                nv = t.level_info(level)[0]
+                lo = m.nverts + sum(t.level_info(k)[0] for k in range(1, level))
+                base = g[f"{name}/level{level}"].astype(np.float64)
+                diag = np.linalg.norm(base.max(0) - base.min(0))
+                dist, nearest = cKDTree(base).query(got[lo:lo + nv].astype(np.float64))
+                assert len(np.unique(nearest)) == nv, (name, level)
+                assert dist.max() <= 1e-5 * diag, (name, level, dist.max())
+    finally:
+        hip.osdhip_config(b"table_layout", old)
