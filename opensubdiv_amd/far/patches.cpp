@@ -456,8 +456,9 @@ bool PatchTable::findPatch(int ptex, double u, double v, PatchHandleRec &h) cons
             const int p = -c - 1;
             h.arrayIndex = (arrays.size() > 1 && p >= arrays[0].numPatches) ? 1 : 0;
             h.patchIndex = p;
-            h.vertIndex = arrays[h.arrayIndex].indexBase +
-                          arrays[h.arrayIndex].stride * (p - arrays[h.arrayIndex].primitiveIdBase);
+            // far/patchMap.cpp:100-125: the first point of the patch
+            // counted within its array (j * patch size), not in the index buffer
+            h.vertIndex = arrays[h.arrayIndex].stride * (p - arrays[h.arrayIndex].primitiveIdBase);
             return true;
         }
         node = c;

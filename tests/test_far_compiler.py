@@ -152,6 +152,10 @@ def test_patch_table_matches_far_fixture():
     pt, coords, outs = _my_patch_eval(m, 3, g["ptex"], g["u"], g["v"])
     far_arrays = g["arrays"].view(PATCH_ARRAY_DTYPE).reshape(-1)
     assert [tuple(a) for a in pt.arrays] == [tuple(a) for a in far_arrays]
+    # Far::PatchMap's handles for the same (ptex, u, v): identical records
+    far_coords = g["coords"].view(PATCH_COORD_DTYPE).reshape(-1)
+    for k in ("arrayIndex", "patchIndex", "vertIndex", "s", "t"):
+        np.testing.assert_array_equal(coords[k], far_coords[k], err_msg=k)
     # limit surface identical to Far's within the float32 noise of the fixture
     assert bbox_rel_err(outs[0], g["cpu_p"]) < 2e-7
     assert bbox_rel_err(outs[1], g["cpu_du"]) < 1e-5
@@ -174,8 +178,9 @@ def test_patch_map_covers_every_ptex_face_and_is_consistent():
     s, tt = u * (1 << depth) - pu, v * (1 << depth) - pv
     assert np.all((s >= 0) & (s <= 1) & (tt >= 0) & (tt <= 1))
     a = pt.arrays[c["arrayIndex"]]
-    np.testing.assert_array_equal(c["vertIndex"],
-                                  a["indexBase"] + a["stride"] * (c["patchIndex"] - a["primitiveIdBase"]))
+    # Far's handles count vertIndex within the patch's array (patchMap.cpp:100-125)
+    np.testing.assert_array_equal(c["vertIndex"], a["stride"] * (c["patchIndex"] -
+                                                                 a["primitiveIdBase"]))
     assert len(pt.find([m.nfaces + 5], [0.5], [0.5])) == 1
     assert pt.find([m.nfaces + 5], [0.5], [0.5])["arrayIndex"][0] == -1
 
