@@ -69,6 +69,7 @@ def build_inputs(need_refine=True, need_patches=True):
     if need_patches:
         out["points"] = pt.point_stencils()
         f, s, t = meshes.random_patch_locations(mesh.nfaces, 4194304, seed=2)
+        out["queries"] = (f, s, t)
         out["coords"] = pt.find(f, s, t)
         assert (out["coords"]["arrayIndex"] >= 0).all()
     log(f"[bench] inputs built in {time.time() - t0:.1f}s: limit table "
@@ -308,6 +309,38 @@ def run_cube(args, lib, with_cpu):
     return res
 
 
+def device_patch_map(args, inputs, stream):
+    """The same PatchCoords made on the device (osdhip_patch_map_find, one
+    launch for every query) against the host map that made them: identical
+    records; lookups/s beside the host map's (OpenMP, all host threads)."""
+    import torch
+    from opensubdiv_amd import far
+    from opensubdiv_amd.osd import PATCH_COORD_DTYPE, HipPatchMap
+    pt, coords = inputs["patches"], inputs["coords"]
+    f, s, t = inputs["queries"]
+    n = len(f)
+    pm = HipPatchMap(pt.arrays, pt.params, stream.cuda_stream)
+    d_f = torch.from_numpy(np.ascontiguousarray(f, np.int32)).to("cuda")
+    d_u = torch.from_numpy(np.ascontiguousarray(s, np.float64)).to("cuda")
+    d_v = torch.from_numpy(np.ascontiguousarray(t, np.float64)).to("cuda")
+    d_out = torch.empty(n * 5, dtype=torch.int32, device="cuda")
+
+    def step():
+        if not pm.FindPatches(n, d_f, d_u, d_v, d_out, stream.cuda_stream):
+            raise RuntimeError("osdhip_patch_map_find failed")
+    wall, ms = time_launches(step, 10, 2, stream)
+    got = d_out.cpu().numpy().view(PATCH_COORD_DTYPE)
+    same = bool(got.tobytes() == np.ascontiguousarray(coords).tobytes())
+    far.set_threads(0)
+    t0 = time.perf_counter()
+    pt.find(f, s, t)
+    host_s = time.perf_counter() - t0
+    pm.close()
+    return {"lookups_per_s": n / (float(np.mean(ms)) * 1e-3), "kernel_ms": float(np.mean(ms)),
+            "identical_to_host_map": same, "host_lookups_per_s": n / host_s,
+            "host_threads": os.cpu_count()}
+
+
 def run_patches(args, lib, inputs, with_cpu):
     import torch
     from opensubdiv_amd import _lib
@@ -369,6 +402,7 @@ def run_patches(args, lib, inputs, with_cpu):
     lib.osdhip_config(b"patch_sort_min", 0)
     res = {"points_per_s": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
            "kernel_ms": avg, "patch_coords": n, "patches": len(pt.params),
+           "patch_map": device_patch_map(args, inputs, stream),
            "gregory_patches": int(sum(a["numPatches"] for a in pt.arrays if a["desc"] == 9)),
            "algorithmic_bytes": alg, "GBps": alg / (avg * 1e-3) / 1e9,
            "kernel": f"k_patches<true,false,3> ({best})", "autotune": variants}

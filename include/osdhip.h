@@ -171,6 +171,28 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
                         const osdhip_patch_param *params,
                         void *stream);
 
+/* ---- PatchCoords on the device: Far::PatchMap ------------------------------
+ * Far::PatchMap (far/patchMap.h:58-120, far/patchMap.cpp:88-200) held on the
+ * device.  osdhip_patch_map_create builds, on the host, the quadtree Far's
+ * constructor builds from a patch table -- here from the Osd flattening of
+ * its PatchArrays and PatchParams (HOST pointers; osd/cpuPatchTable.cpp:
+ * 35-158) -- and uploads it with the patch handles; NULL for an empty or
+ * inconsistent table.  osdhip_patch_map_find is PatchMap::FindPatch
+ * (far/patchMap.h:193-231) for n (ptex face, u, v) queries in one launch
+ * (DEVICE pointers): coords[i] = {handle, s = (float)u, t = (float)v} with
+ * Far's handle {arrayIndex, patchIndex, vertIndex = first point within the
+ * array}; a query outside the map (no such face, u or v outside [0, 1])
+ * gets {-1, -1, -1}.  Same records as Far's PatchMap, including the
+ * upper-half rule at every median (u >= median goes to the upper half). */
+typedef struct osdhip_patch_map osdhip_patch_map;
+osdhip_patch_map *osdhip_patch_map_create(const osdhip_patch_array *arrays, int numArrays,
+                                          const osdhip_patch_param *params, int numPatches,
+                                          void *stream);
+int osdhip_patch_map_find(const osdhip_patch_map *map, int n, const int *ptexFaces,
+                          const double *u, const double *v, osdhip_patch_coord *coords,
+                          void *stream);
+void osdhip_patch_map_destroy(osdhip_patch_map *map);
+
 /* ---- the step after EvalPatches: limit normals ------------------------------
  * normals[i] = cross(dPdu[i], dPdv[i]) over xyz (the first three elements of
  * each descriptor), scaled to unit length by 1 / sqrtf(dot) when `normalize`

@@ -69,6 +69,9 @@ _HIP_SIGS = {
     "osdhip_stencil_table_csr": [_vp, _vp, _vp, _vp, _vp],
     "osdhip_stencil_table_layout": [_vp, _vp, _vp, _vp, _vp],
     "osdhip_limit_normals": [_vp, _D, _vp, _D, _vp, _D, _i, _i, _vp],
+    "osdhip_patch_map_create": [_vp, _i, _vp, _i, _vp],
+    "osdhip_patch_map_find": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
+    "osdhip_patch_map_destroy": [_vp],
     "osdhip_stencil_table_eval": [_vp, _vp, _D, _i, _vp, _vp, _i, _i, _vp],
     "osdhip_config": [C.c_char_p, _i],
     "osdhip_malloc": [C.POINTER(C.c_void_p), C.c_size_t],
@@ -118,9 +121,11 @@ def load_hip():
             fn.restype = C.c_char_p
         elif name == "osdhip_batch_epoch_bytes":
             fn.restype = C.c_longlong
-        elif name in ("osdhip_stencil_table_create", "osdhip_batch_create"):
+        elif name in ("osdhip_stencil_table_create", "osdhip_batch_create",
+                      "osdhip_patch_map_create"):
             fn.restype = C.c_void_p
-        elif name in ("osdhip_stencil_table_destroy", "osdhip_batch_destroy"):
+        elif name in ("osdhip_stencil_table_destroy", "osdhip_batch_destroy",
+                      "osdhip_patch_map_destroy"):
             fn.restype = None
         else:
             fn.restype = C.c_int

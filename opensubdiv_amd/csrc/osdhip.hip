@@ -27,6 +27,7 @@
 #include "../../include/osdhip.h"
 #include "common.h"
 #include "patch_kernels.h"
+#include "patch_map.h"
 #include "stencil_csr.h"
 #include "stencil_sliced.h"
 #include "stencil_tiled.h"
@@ -728,6 +729,128 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
         if (rc == OSDHIP_OK) rc = rf;
     }
     return rc;
+}
+
+struct osdhip_patch_map {
+    osdhip::v4i *nodes = nullptr;
+    int *handles = nullptr;
+    int minFace = 0, maxFace = -1, maxDepth = 0;
+};
+
+void osdhip_patch_map_destroy(osdhip_patch_map *map)
+{
+    if (!map) return;
+    if (map->nodes) (void)hipFree(map->nodes);
+    if (map->handles) (void)hipFree(map->handles);
+    delete map;
+}
+
+osdhip_patch_map *osdhip_patch_map_create(const osdhip_patch_array *arrays, int numArrays,
+                                          const osdhip_patch_param *params, int numPatches,
+                                          void *stream)
+{
+    if (!arrays || !params || numArrays <= 0 || numPatches <= 0) return nullptr;
+    // handles: far/patchMap.cpp:100-125 (patches in array order, vertIndex
+    // counted within the array)
+    std::vector<int> handles(3 * (size_t)numPatches, -1);
+    long long total = 0;
+    for (int a = 0; a < numArrays; ++a) {
+        const osdhip_patch_array &A = arrays[a];
+        if (A.numPatches < 0 || A.stride < 0 || A.primitiveIdBase != total) return nullptr;
+        total += A.numPatches;
+        if (total > numPatches) return nullptr;
+        for (int j = 0; j < A.numPatches; ++j) {
+            const size_t p = (size_t)A.primitiveIdBase + (size_t)j;
+            handles[3 * p] = a;
+            handles[3 * p + 1] = (int)p;
+            handles[3 * p + 2] = j * A.stride;
+        }
+    }
+    if (total != numPatches) return nullptr;
+    // quadtree: far/patchMap.cpp:130-200, one root per face in
+    // [minFace, maxFace]; a patch at its root depth (0, or 1 under an
+    // n-gon) fills all four quadrants of the root
+    int minFace = (int)(params[0].field0 & 0xfffffffu), maxFace = minFace;
+    for (int p = 1; p < numPatches; ++p) {
+        const int f = (int)(params[p].field0 & 0xfffffffu);
+        minFace = std::min(minFace, f);
+        maxFace = std::max(maxFace, f);
+    }
+    const osdhip::v4i empty = { osdhip::kMapEmpty, osdhip::kMapEmpty, osdhip::kMapEmpty,
+                                osdhip::kMapEmpty };
+    std::vector<osdhip::v4i> nodes((size_t)(maxFace - minFace) + 1, empty);
+    int maxDepth = 0;
+    for (int p = 0; p < numPatches; ++p) {
+        const unsigned f1 = params[p].field1;
+        const int depth = (int)(f1 & 0xfu), rootDepth = (int)((f1 >> 4) & 1u);
+        if (depth < rootDepth) return nullptr;
+        maxDepth = std::max(maxDepth, depth);
+        int node = (int)(params[p].field0 & 0xfffffffu) - minFace;
+        const int leaf = -(p + 1);
+        if (depth == rootDepth) {
+            nodes[(size_t)node] = osdhip::v4i{ leaf, leaf, leaf, leaf };
+            continue;
+        }
+        const int u = (int)((f1 >> 22) & 0x3ffu), v = (int)((f1 >> 12) & 0x3ffu);
+        for (int j = rootDepth + 1; j <= depth; ++j) {
+            const int q = (((v >> (depth - j)) & 1) << 1) | ((u >> (depth - j)) & 1);
+            if (j == depth) {
+                nodes[(size_t)node][q] = leaf;
+            } else {
+                int c = nodes[(size_t)node][q];
+                if (c < 0) {
+                    c = (int)nodes.size();
+                    nodes[(size_t)node][q] = c;
+                    nodes.push_back(empty);
+                }
+                node = c;
+            }
+        }
+    }
+    osdhip_patch_map *m = new (std::nothrow) osdhip_patch_map;
+    if (!m) return nullptr;
+    m->minFace = minFace;
+    m->maxFace = maxFace;
+    m->maxDepth = maxDepth;
+    hipStream_t st = (hipStream_t)stream;
+    const bool ok =
+        hipMalloc((void **)&m->nodes, nodes.size() * sizeof(osdhip::v4i)) == hipSuccess &&
+        hipMalloc((void **)&m->handles, handles.size() * sizeof(int)) == hipSuccess &&
+        hipMemcpyAsync(m->nodes, nodes.data(), nodes.size() * sizeof(osdhip::v4i),
+                       hipMemcpyHostToDevice, st) == hipSuccess &&
+        hipMemcpyAsync(m->handles, handles.data(), handles.size() * sizeof(int),
+                       hipMemcpyHostToDevice, st) == hipSuccess &&
+        hipStreamSynchronize(st) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        osdhip_patch_map_destroy(m);
+        return nullptr;
+    }
+    return m;
+}
+
+int osdhip_patch_map_find(const osdhip_patch_map *map, int n, const int *ptexFaces,
+                          const double *u, const double *v, osdhip_patch_coord *coords,
+                          void *stream)
+{
+    if (!map || n < 0) return OSDHIP_ERR_ARGS;
+    if (n == 0) return OSDHIP_OK;
+    if (!ptexFaces || !u || !v || !coords) return OSDHIP_ERR_ARGS;
+    osdhip::PatchMapArgs a;
+    a.nodes = map->nodes;
+    a.handles = map->handles;
+    a.minFace = map->minFace;
+    a.maxFace = map->maxFace;
+    a.maxDepth = map->maxDepth;
+    a.n = n;
+    a.ptex = ptexFaces;
+    a.u = u;
+    a.v = v;
+    a.coords = coords;
+    const int grid = (int)(((long long)n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(osdhip::k_patch_map_find, dim3(grid), dim3(kBlock), 0, (hipStream_t)stream,
+                       a);
+    return last_error();
 }
 
 int osdhip_limit_normals(const float *dPdu, osdhip_buffer_desc duDesc, const float *dPdv,

@@ -261,6 +261,34 @@ def limit_normals(du, duDesc, dv, dvDesc, normals, normalsDesc, count, normalize
     return rc == _lib.OK
 
 
+class HipPatchMap:
+    """Far::PatchMap on the device (osdhip_patch_map_*): built from a patch
+    table's PatchArrays and PatchParams (numpy, host), FindPatches maps
+    device (ptex face, u, v) arrays to device PatchCoords in one launch."""
+
+    def __init__(self, arrays, params, deviceContext=None):
+        a = np.ascontiguousarray(arrays, PATCH_ARRAY_DTYPE)
+        p = np.ascontiguousarray(params, PATCH_PARAM_DTYPE)
+        self.h = _lib.load_hip().osdhip_patch_map_create(a.ctypes.data, len(a), p.ctypes.data,
+                                                         len(p), _stream(deviceContext))
+        if not self.h:
+            raise ValueError("osdhip_patch_map_create: empty or inconsistent patch table")
+
+    def FindPatches(self, n, ptexFaces, u, v, patchCoords, deviceContext=None):
+        """Device int32 faces, float64 u / v, PatchCoord output; returns bool."""
+        return _lib.load_hip().osdhip_patch_map_find(self.h, int(n), _ptr(ptexFaces), _ptr(u),
+                                                     _ptr(v), _ptr(patchCoords),
+                                                     _stream(deviceContext)) == _lib.OK
+
+    def close(self):
+        if getattr(self, "h", None) and _lib._hip is not None:
+            _lib._hip.osdhip_patch_map_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
 class HipEvaluator:
     """Static evaluator (osd/cudaEvaluator.h:94-1265); all methods return
     bool like the reference.  ``deviceContext`` may be a hipStream_t (int);

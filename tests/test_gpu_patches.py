@@ -315,3 +315,75 @@ def test_invalid_phantom_indices_read_no_memory_outside_the_buffer(hip):
                         dev(arrays), dev(holes), dev(params)) == 0
     for k in range(3):
         np.testing.assert_array_equal(host(outs[k], shape=(n, L)), want[k], err_msg=f"output {k}")
+
+
+def _queries(rng, nptex, n):
+    f = rng.integers(0, nptex, n).astype(np.int32)
+    u, v = rng.random(n), rng.random(n)
+    # medians of every depth (upper-half rule), the unit square's edges,
+    # points just outside it, NaN and faces outside the map
+    grid = np.array([0.0, 1.0, 0.5, 0.25, 0.75, 0.125, 0.0625, 1 - 2 ** -30, 2 ** -30])
+    k = len(grid)
+    u[:k * k], v[:k * k] = np.repeat(grid, k), np.tile(grid, k)
+    u[k * k:k * k + 4] = [-1e-12, 1 + 1e-12, np.nan, 0.5]
+    v[k * k:k * k + 4] = [0.5, 0.5, 0.5, -0.0]
+    f[k * k + 4:k * k + 8] = [-1, nptex, nptex + 100, 2 ** 30]
+    return f, u, v
+
+
+def test_device_patch_map_matches_far_patch_map_records(hip):
+    """Far::PatchMap's own records for the golden cube-sphere queries."""
+    from opensubdiv_amd.osd import HipPatchMap
+    g = golden("cubesphere_patches.npz")
+    pm = HipPatchMap(g["arrays"].view(PATCH_ARRAY_DTYPE).reshape(-1),
+                     g["params"].view(PATCH_PARAM_DTYPE).reshape(-1))
+    n = len(g["ptex"])
+    out = dev(np.full(n, -7, PATCH_COORD_DTYPE))
+    assert pm.FindPatches(n, dev(g["ptex"].astype(np.int32)), dev(g["u"].astype(np.float64)),
+                          dev(g["v"].astype(np.float64)), out)
+    got = host(out, PATCH_COORD_DTYPE)
+    want = g["coords"].view(PATCH_COORD_DTYPE).reshape(-1)
+    for k in PATCH_COORD_DTYPE.names:
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    pm.close()
+
+
+@pytest.mark.parametrize("mesh", ["creased-sphere", "pentagonal-prism"])
+def test_device_patch_map_matches_host_map(hip, mesh):
+    """Tables of the table compiler (creased adaptive levels; an n-gon mesh
+    whose sub-face patches are rooted at depth 1): the device map gives the
+    host map's records for random queries, every median, the square's
+    edges and points outside it, NaN and faces outside the map."""
+    from opensubdiv_amd import far, meshes
+    from opensubdiv_amd.osd import HipPatchMap
+    if mesh == "creased-sphere":
+        m = meshes.cube_sphere(5, seed=3, crease_fraction=0.2)
+        nptex = m.nfaces
+    else:
+        from test_far_compiler import pentagonal_prism
+        m = pentagonal_prism()
+        nptex = int(sum(1 if k == 4 else k for k in m.face_sizes))
+    pt = far.Topology(m).patch_table(3)
+    rng = np.random.default_rng(21)
+    f, u, v = _queries(rng, nptex, 20000)
+    want = pt.find(f, u, v)
+    pm = HipPatchMap(pt.arrays, pt.params)
+    out = dev(np.zeros(len(f), PATCH_COORD_DTYPE))
+    assert pm.FindPatches(len(f), dev(f), dev(u), dev(v), out)
+    got = host(out, PATCH_COORD_DTYPE)
+    for k in PATCH_COORD_DTYPE.names:
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    assert (got["arrayIndex"] >= 0).sum() > 19000 and (got["arrayIndex"] == -1).sum() >= 7
+    # argument errors; an empty query set is a no-op
+    assert hip.osdhip_patch_map_find(pm.h, 0, None, None, None, None, stream()) == 0
+    assert hip.osdhip_patch_map_find(pm.h, 5, None, ptr(out), ptr(out), ptr(out), stream()) == \
+        _lib.ERR_ARGS
+    assert hip.osdhip_patch_map_find(None, 5, ptr(out), ptr(out), ptr(out), ptr(out), stream()) \
+        == _lib.ERR_ARGS
+    pm.close()
+    with pytest.raises(ValueError):
+        HipPatchMap(np.zeros(0, PATCH_ARRAY_DTYPE), pt.params)
+    bad = pt.arrays.copy()
+    bad["numPatches"][0] += 1                      # arrays do not cover the params
+    with pytest.raises(ValueError):
+        HipPatchMap(bad, pt.params)
