@@ -193,6 +193,20 @@ int osdhip_patch_map_find(const osdhip_patch_map *map, int n, const int *ptexFac
                           void *stream);
 void osdhip_patch_map_destroy(osdhip_patch_map *map);
 
+/* Bfr::Tessellation (bfr/tessellation.h:51-240) of quad parameterizations
+ * at a uniform rate, on the device, for numFaces ptex faces at once: each
+ * face's (rate + 1)^2 coordinates in Bfr's order -- the boundary
+ * counter-clockwise from (0,0), then the interior rings from the outside
+ * in -- with the same float values as Tessellation::GetCoords<float>.
+ * Written face-major as (u, v) pairs to `uv` (may be NULL) and, with a
+ * patch map, as the PatchCoords of (ptexFaces[f], u, v) to `coords`, ready
+ * for osdhip_eval_patches (no host round trip).  rate in [1, 4096];
+ * OSDHIP_ERR_ARGS when nothing is to be written, coords without a map or a
+ * map without coords / faces, or more than 2^31 - 1 coordinates. */
+int osdhip_tess_uniform_quads(int numFaces, const int *ptexFaces, int rate, float *uv,
+                              const osdhip_patch_map *map, osdhip_patch_coord *coords,
+                              void *stream);
+
 /* ---- the step after EvalPatches: limit normals ------------------------------
  * normals[i] = cross(dPdu[i], dPdv[i]) over xyz (the first three elements of
  * each descriptor), scaled to unit length by 1 / sqrtf(dot) when `normalize`

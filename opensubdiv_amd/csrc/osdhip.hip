@@ -853,6 +853,38 @@ int osdhip_patch_map_find(const osdhip_patch_map *map, int n, const int *ptexFac
     return last_error();
 }
 
+int osdhip_tess_uniform_quads(int numFaces, const int *ptexFaces, int rate, float *uv,
+                              const osdhip_patch_map *map, osdhip_patch_coord *coords,
+                              void *stream)
+{
+    if (numFaces < 0 || rate < 1 || rate > 4096) return OSDHIP_ERR_ARGS;
+    if (numFaces == 0) return OSDHIP_OK;
+    if ((map && (!coords || !ptexFaces)) || (!map && coords) || (!uv && !map))
+        return OSDHIP_ERR_ARGS;
+    const long long perFace = (long long)(rate + 1) * (long long)(rate + 1);
+    const long long total = perFace * (long long)numFaces;
+    if (total > 0x7fffffffLL) return OSDHIP_ERR_ARGS;
+    osdhip::TessArgs a;
+    a.numFaces = numFaces;
+    a.rate = rate;
+    a.perFace = (int)perFace;
+    a.ptex = ptexFaces;
+    a.uv = uv;
+    a.map.nodes = map ? map->nodes : nullptr;
+    a.map.handles = map ? map->handles : nullptr;
+    a.map.minFace = map ? map->minFace : 0;
+    a.map.maxFace = map ? map->maxFace : -1;
+    a.map.maxDepth = map ? map->maxDepth : 0;
+    a.map.n = (int)total;
+    a.map.ptex = nullptr;
+    a.map.u = a.map.v = nullptr;
+    a.map.coords = coords;
+    const int grid = (int)((total + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(osdhip::k_tess_uniform_quads, dim3(grid), dim3(kBlock), 0,
+                       (hipStream_t)stream, a);
+    return last_error();
+}
+
 int osdhip_limit_normals(const float *dPdu, osdhip_buffer_desc duDesc, const float *dPdv,
                          osdhip_buffer_desc dvDesc, float *normals, osdhip_buffer_desc nDesc,
                          int count, int normalize, void *stream)

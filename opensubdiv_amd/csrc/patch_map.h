@@ -21,12 +21,9 @@ struct PatchMapArgs {
     osdhip_patch_coord *coords;
 };
 
-__global__ __launch_bounds__(kBlock) void k_patch_map_find(PatchMapArgs a)
+__device__ __forceinline__ osdhip_patch_coord patch_map_lookup(const PatchMapArgs &a, int f,
+                                                              double u, double v)
 {
-    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
-    if (i >= a.n) return;
-    const int f = a.ptex[i];
-    double u = a.u[i], v = a.v[i];
     osdhip_patch_coord c;
     c.arrayIndex = c.patchIndex = c.vertIndex = -1;
     c.s = (float)u;
@@ -55,7 +52,92 @@ __global__ __launch_bounds__(kBlock) void k_patch_map_find(PatchMapArgs a)
             node = ch;
         }
     }
-    a.coords[i] = c;
+    return c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_patch_map_find(PatchMapArgs a)
+{
+    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (i >= a.n) return;
+    a.coords[i] = patch_map_lookup(a, a.ptex[i], a.u[i], a.v[i]);
+}
+
+// One (u, v) of Bfr::Tessellation's uniform pattern of a quad at `rate`
+// (bfr/tessellation.cpp:158-174, 929-990): the boundary counter-clockwise
+// from (0,0), 4 * rate points, then the interior rings from the outside in,
+// each counter-clockwise, the innermost of an even rate being the centre.
+// Along an iso-line the coordinate accumulates its step point by point in
+// float, as the reference does, so the values are bit-identical.
+__device__ __forceinline__ float tess_acc(float x, float d, int i)
+{
+    for (int j = 0; j < i; ++j) x += d;
+    return x;
+}
+
+__device__ __forceinline__ void tess_quad_coord(int rate, int k, float &u, float &v)
+{
+    const float d = 1.0f / (float)rate;
+    if (k < 4 * rate) {
+        const int seg = k / rate, i = k - seg * rate;
+        if (seg == 0) { u = tess_acc(0.0f, d, i); v = 0.0f; }
+        else if (seg == 1) { u = 1.0f; v = tess_acc(0.0f, d, i); }
+        else if (seg == 2) { u = tess_acc(1.0f, -d, i); v = 1.0f; }
+        else { u = 0.0f; v = tess_acc(1.0f, -d, i); }
+        return;
+    }
+    k -= 4 * rate;
+    float u0 = d, v0 = d;
+    int ur = rate - 2, vr = rate - 2;
+    for (int r = 0; r < rate / 2; ++r, ur -= 2, vr -= 2, u0 += d, v0 += d) {
+        if (ur > 0 && vr > 0) {
+            if (k < 2 * (ur + vr)) {
+                const float u1 = 1.0f - u0, v1 = 1.0f - v0;
+                if (k < ur) { u = tess_acc(u0, d, k); v = v0; }
+                else if ((k -= ur) < vr) { u = u1; v = tess_acc(v0, d, k); }
+                else if ((k -= vr) < ur) { u = tess_acc(u1, -d, k); v = v1; }
+                else { k -= ur; u = u0; v = tess_acc(v1, -d, k); }
+                return;
+            }
+            k -= 2 * (ur + vr);
+        } else if (ur > 0 || vr > 0) {
+            const int n = (ur > 0 ? ur : vr) + 1;
+            if (k < n) {
+                if (ur > 0) { u = tess_acc(u0, d, k); v = v0; }
+                else { u = u0; v = tess_acc(v0, d, k); }
+                return;
+            }
+            k -= n;
+        } else {
+            u = 0.5f;
+            v = 0.5f;
+            return;
+        }
+    }
+    u = v = 0.0f;   // not reached for k < (rate + 1)^2
+}
+
+struct TessArgs {
+    int numFaces, rate, perFace;
+    const int *ptex;
+    float *uv;
+    PatchMapArgs map;        // map.nodes == NULL: no PatchCoords
+};
+
+__global__ __launch_bounds__(kBlock) void k_tess_uniform_quads(TessArgs a)
+{
+    const long long idx = (long long)blockIdx.x * kBlock + threadIdx.x;
+    if (idx >= (long long)a.numFaces * a.perFace) return;
+    const int f = (int)(idx / a.perFace), k = (int)(idx - (long long)f * a.perFace);
+    float u, v;
+    tess_quad_coord(a.rate, k, u, v);
+    if (a.uv) {
+        a.uv[2 * idx] = u;
+        a.uv[2 * idx + 1] = v;
+    }
+    if (a.map.nodes) {
+        const osdhip_patch_coord c = patch_map_lookup(a.map, a.ptex[f], (double)u, (double)v);
+        a.map.coords[idx] = c;
+    }
 }
 
 }  // namespace osdhip

@@ -71,6 +71,7 @@ def lib():
             "oracle_eval_patches_f64": [vp, dd, vp, dd, vp, dd, vp, dd, i, vp, vp, vp, vp],
             "oracle_eval_patches_omp_f32": [vp, dd, vp, dd, vp, dd, vp, dd, i, vp, vp, vp, vp, i],
             "oracle_limit_normals_f32": [vp, dd, vp, dd, vp, dd, i, i],
+            "oracle_tess_uniform_quad_f32": [i, vp],
         }
         for k, a in sigs.items():
             f = getattr(L, k)
@@ -189,3 +190,12 @@ def limit_normals(du, du_desc, dv, dv_desc, n, n_desc, count, normalize=True):
     steps (far_tutorial_2_3.cpp:73-87). Returns bool."""
     return bool(lib().oracle_limit_normals_f32(_p(du), D(du_desc), _p(dv), D(dv_desc), _p(n),
                                                D(n_desc), int(count), int(bool(normalize))))
+
+
+def tess_uniform_quad(rate):
+    """(rate + 1)^2 x 2 float32 (u, v) of Bfr's uniform quad pattern
+    (bfr/tessellation.cpp:158-174, 929-990)."""
+    uv = np.zeros(((rate + 1) ** 2, 2), np.float32)
+    n = lib().oracle_tess_uniform_quad_f32(int(rate), _p(uv))
+    assert n == len(uv), n
+    return uv

@@ -350,3 +350,59 @@ int oracle_limit_normals_f32(const float *du, oracle_desc duDesc, const float *d
     }
     return 1;
 }
+
+/* Bfr::Tessellation::GetCoords<float> of a quad at a uniform rate
+ * (bfr/tessellation.cpp:158-174 iso-lines whose coordinate accumulates its
+ * step point by point; 929-945 the boundary, counter-clockwise from (0,0);
+ * 947-990 the interior rings from the outside in, the centre last for an
+ * even rate).  Writes (rate + 1)^2 (u, v) pairs; returns the count, or -1
+ * for rate < 1. */
+static int tess_iso_v(float *uv, int n, float u, float v, float du)
+{
+    for (int i = 0; i < n; ++i, u += du) {
+        uv[2 * i] = u;
+        uv[2 * i + 1] = v;
+    }
+    return n;
+}
+
+static int tess_iso_u(float *uv, int n, float u, float v, float dv)
+{
+    for (int i = 0; i < n; ++i, v += dv) {
+        uv[2 * i] = u;
+        uv[2 * i + 1] = v;
+    }
+    return n;
+}
+
+int oracle_tess_uniform_quad_f32(int rate, float *uv)
+{
+    if (rate < 1) return -1;
+    int n = 0;
+    n += tess_iso_v(uv + 2 * n, rate, 0.0f, 0.0f, 1.0f / (float)rate);
+    n += tess_iso_u(uv + 2 * n, rate, 1.0f, 0.0f, 1.0f / (float)rate);
+    n += tess_iso_v(uv + 2 * n, rate, 1.0f, 1.0f, -1.0f / (float)rate);
+    n += tess_iso_u(uv + 2 * n, rate, 0.0f, 1.0f, -1.0f / (float)rate);
+    const int rings = rate / 2;
+    const float du = 1.0f / (float)rate, dv = 1.0f / (float)rate;
+    float u = du, v = dv;
+    int ur = rate - 2, vr = rate - 2;
+    for (int i = 0; i < rings; ++i, ur -= 2, vr -= 2, u += du, v += dv) {
+        if (ur > 0 && vr > 0) {
+            const float u1 = 1.0f - u, v1 = 1.0f - v;
+            n += tess_iso_v(uv + 2 * n, ur, u, v, du);
+            n += tess_iso_u(uv + 2 * n, vr, u1, v, dv);
+            n += tess_iso_v(uv + 2 * n, ur, u1, v1, -du);
+            n += tess_iso_u(uv + 2 * n, vr, u, v1, -dv);
+        } else if (ur > 0) {
+            n += tess_iso_v(uv + 2 * n, ur + 1, u, v, du);
+        } else if (vr > 0) {
+            n += tess_iso_u(uv + 2 * n, vr + 1, u, v, dv);
+        } else {
+            uv[2 * n] = 0.5f;
+            uv[2 * n + 1] = 0.5f;
+            n += 1;
+        }
+    }
+    return n;
+}

@@ -169,4 +169,8 @@ int oracle_limit_normals_f32(const float *du, oracle_desc duDesc, const float *d
 #ifdef __cplusplus
 }
 #endif
+/* Bfr::Tessellation::GetCoords<float> of a quad at a uniform rate
+ * (bfr/tessellation.cpp:158-174, 929-990): (rate + 1)^2 (u, v) pairs. */
+int oracle_tess_uniform_quad_f32(int rate, float *uv);
+
 #endif

@@ -50,6 +50,7 @@ def lib():
             "ref_cpu_eval_patches": ([vp] * 8 + [i, vp, vp, vp, vp], i),
             "ref_cpu_eval_patches_d2": ([vp] * 14 + [i, vp, vp, vp, vp], i),
             "ref_omp_eval_patches": ([vp] * 8 + [i, vp, vp, vp, vp, i], i),
+            "ref_tess_uniform_coords": ([i, i, vp], i),
         }
         for k, (a, r) in sig.items():
             f = getattr(L, k)
@@ -256,3 +257,12 @@ def cpu_eval_patches_d2(src, sd, outs, descs, coords, arrays, indices, params):
         a += [_p(o), _p(x)]
     return bool(lib().ref_cpu_eval_patches_d2(_p(src), _p(sdd), *a, len(coords), _p(coords),
                                               _p(arrays), _p(indices), _p(params)))
+
+
+def tess_uniform_coords(face_size, rate):
+    """Bfr::Tessellation(Parameterization(CATMARK, face_size), rate).GetCoords
+    as float32 (u, v) pairs."""
+    uv = np.zeros((face_size * (rate + 1) ** 2 + 8, 2), np.float32)
+    n = lib().ref_tess_uniform_coords(int(face_size), int(rate), _p(uv))
+    assert n > 0, n
+    return uv[:n].copy()

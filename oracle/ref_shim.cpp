@@ -23,6 +23,7 @@
 #include <opensubdiv/osd/cpuEvaluator.h>
 #include <opensubdiv/osd/cpuPatchTable.h>
 #include <opensubdiv/osd/ompEvaluator.h>
+#include <opensubdiv/bfr/tessellation.h>
 
 #include <cstring>
 #include <vector>
@@ -469,6 +470,18 @@ int ref_omp_eval_patches(const float *src, const int *sd, float *dst, const int 
                                           (const Osd::PatchCoord *)coords,
                                           (const Osd::PatchArray *)arrays, indices,
                                           (const Osd::PatchParam *)params);
+}
+
+// Bfr::Tessellation's coordinates (bfr/tessellation.h:51-240) of a Catmark
+// face of faceSize corners at a uniform rate, float (u,v) pairs; returns
+// the count, or -1 for an invalid pattern.
+int ref_tess_uniform_coords(int faceSize, int rate, float *uv)
+{
+    OpenSubdiv::Bfr::Parameterization p(OpenSubdiv::Sdc::SCHEME_CATMARK, faceSize);
+    if (!p.IsValid()) return -1;
+    OpenSubdiv::Bfr::Tessellation t(p, rate);
+    if (!t.IsValid()) return -1;
+    return t.GetCoords(uv);
 }
 
 }  // extern "C"

@@ -387,3 +387,56 @@ def test_device_patch_map_matches_host_map(hip, mesh):
     bad["numPatches"][0] += 1                      # arrays do not cover the params
     with pytest.raises(ValueError):
         HipPatchMap(bad, pt.params)
+
+
+def test_device_tessellation_matches_bfr_and_feeds_eval_patches(hip):
+    """Bfr's uniform quad pattern on the device for every ptex face of a
+    creased cube-sphere at several rates: (u, v) bit-identical to the
+    oracle's Bfr restatement, PatchCoords identical to the host patch map's
+    for the same points, and EvalPatches on them identical to the oracle."""
+    from opensubdiv_amd import far, meshes
+    from opensubdiv_amd.osd import HipPatchMap
+    m = meshes.cube_sphere(4, seed=5, crease_fraction=0.2)
+    pt = far.Topology(m).patch_table(3)
+    pm = HipPatchMap(pt.arrays, pt.params)
+    faces = np.arange(m.nfaces, dtype=np.int32)[::-1].copy()      # any face order
+    d_faces = dev(faces)
+    for rate in (1, 2, 3, 8, 17):
+        per = (rate + 1) ** 2
+        n = len(faces) * per
+        uv = empty(2 * n)
+        coords = dev(np.zeros(n, PATCH_COORD_DTYPE))
+        assert hip.osdhip_tess_uniform_quads(len(faces), ptr(d_faces), rate, ptr(uv), pm.h,
+                                             ptr(coords), stream()) == 0
+        want_uv = np.tile(oracle.tess_uniform_quad(rate), (len(faces), 1))
+        got_uv = host(uv, shape=(n, 2))
+        assert got_uv.tobytes() == want_uv.tobytes(), rate
+        want = pt.find(np.repeat(faces, per), want_uv[:, 0].astype(np.float64),
+                       want_uv[:, 1].astype(np.float64))
+        got = host(coords, PATCH_COORD_DTYPE)
+        assert got.tobytes() == want.tobytes(), rate
+    # the last rate's coords straight into EvalPatches
+    st = pt.point_stencils()
+    ctrl = m.positions.astype(np.float32)
+    pts = oracle.apply_stencils(ctrl, st["sizes"], st["offsets"], st["indices"], st["weights"])
+    src = np.ascontiguousarray(np.concatenate([ctrl, pts]))
+    outs = [empty(n * 3) for _ in range(3)]
+    assert eval_patches(hip, dev(src), (0, 3, 3), outs, [(0, 3, 3)] * 3, n, coords,
+                        dev(pt.arrays), dev(pt.indices), dev(pt.params)) == 0
+    ref_out = [np.zeros((n, 3), np.float32) for _ in range(3)]
+    assert oracle.eval_patches(src, (0, 3, 3), ref_out, [(0, 3, 3)] * 3, want, pt.arrays,
+                               pt.indices, pt.params)
+    for k in range(3):
+        np.testing.assert_array_equal(host(outs[k], shape=(n, 3)), ref_out[k])
+    # uv only (no map); argument errors
+    assert hip.osdhip_tess_uniform_quads(len(faces), None, 4, ptr(uv), None, None, stream()) == 0
+    assert hip.osdhip_tess_uniform_quads(len(faces), ptr(d_faces), 0, ptr(uv), None, None,
+                                         stream()) == _lib.ERR_ARGS
+    assert hip.osdhip_tess_uniform_quads(len(faces), ptr(d_faces), 4, None, None, None,
+                                         stream()) == _lib.ERR_ARGS
+    assert hip.osdhip_tess_uniform_quads(len(faces), None, 4, None, pm.h, ptr(coords),
+                                         stream()) == _lib.ERR_ARGS
+    assert hip.osdhip_tess_uniform_quads(len(faces), ptr(d_faces), 4, None, None, ptr(coords),
+                                         stream()) == _lib.ERR_ARGS
+    assert hip.osdhip_tess_uniform_quads(0, None, 4, None, None, None, stream()) == 0
+    pm.close()

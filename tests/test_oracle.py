@@ -405,3 +405,42 @@ def test_limit_normals_point_outward_on_the_golden_sphere():
     cos = (_normals(du, dv, True) * radial).sum(1)
     assert cos.min() > math.cos(math.radians(10))
     assert ((_normals(dv, du, True) * radial).sum(1) < 0).all()
+
+
+# ------------------------------------------------ Bfr tessellation pattern
+
+def test_tess_uniform_quad_matches_bfr_vectors():
+    """The oracle's Bfr quad pattern against Bfr's own coordinates (fixture
+    written by the reference's Bfr::Tessellation; live for more rates when
+    the reference library is built), bit for bit, plus what the pattern
+    fixes: (rate + 1)^2 points, the boundary first (4 rate points on the
+    square's edges, counter-clockwise from the origin), every grid point
+    i / rate once."""
+    g = golden("bfr_tess_quad.npz")
+    for key in g.files:
+        rate = int(key[4:])
+        uv = oracle.tess_uniform_quad(rate)
+        assert uv.tobytes() == g[key].tobytes(), key
+        assert len(uv) == (rate + 1) ** 2
+        edge = uv[:4 * rate]
+        assert np.all((edge[:, 0] == 0) | (edge[:, 0] == 1) | (edge[:, 1] == 0) |
+                      (edge[:, 1] == 1))
+        assert tuple(uv[0]) == (0.0, 0.0) and tuple(uv[rate]) == (1.0, 0.0)
+        ij = np.rint(uv.astype(np.float64) * rate).astype(int)
+        assert np.abs(uv * rate - ij).max() < 1e-4 * rate
+        assert len({(a, b) for a, b in ij}) == len(uv)
+    from oracle import ref
+    if ref.available():
+        for rate in range(1, 65):
+            assert oracle.tess_uniform_quad(rate).tobytes() == \
+                ref.tess_uniform_coords(4, rate).tobytes(), rate
+
+
+def test_tess_planted_mistake_is_caught():
+    """A plausible restatement error -- the iso-line coordinate computed as
+    u0 + i * d instead of accumulated -- fails the bit comparison."""
+    g = golden("bfr_tess_quad.npz")
+    rate = 33
+    d = np.float32(1.0) / np.float32(rate)
+    wrong = (np.arange(rate, dtype=np.float32) * d).astype(np.float32)
+    assert wrong.tobytes() != g["rate33"][:rate, 0].tobytes()
