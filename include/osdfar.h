@@ -6,7 +6,8 @@
  * tables, limit stencil tables with derivatives, patch tables
  * (B-spline patches + Gregory-basis end caps) and a patch map.  All pointers
  * are HOST pointers.  Functions returning int return 0 on success and a
- * negative value on invalid input.
+ * negative value on invalid input (OSDFAR_ERR_HOST when host memory ran
+ * out); no function throws.
  *
  * Interfaces restated (OpenSubdiv v3.6.0, paths relative to its tree):
  *   far/topologyDescriptor.h:40-80      TopologyDescriptor (input mesh)
@@ -20,6 +21,8 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+#define OSDFAR_ERR_HOST (-9)   /* host allocation failed */
 
 typedef struct osdfar_topology osdfar_topology;
 typedef struct osdfar_stencil_table osdfar_stencil_table;

@@ -415,15 +415,19 @@ void freeBatch(osdhip_batch *b)
 
 extern "C" {
 
-osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
+// osdhip_batch_create's body; *owned is the batch under construction, so
+// that the caller frees it if a host allocation throws.
+static osdhip_batch *batch_create(int length, int numStencilSegments,
                                   const osdhip_batch_stencils *stencils, int numPatchSegments,
-                                  const osdhip_batch_patches *patches, void *stream)
+                                  const osdhip_batch_patches *patches, void *stream,
+                                  osdhip_batch **owned)
 {
     if (length < 1 || length > 8 || numStencilSegments < 0 || numPatchSegments < 0 ||
         (numStencilSegments > 0 && !stencils) || (numPatchSegments > 0 && !patches))
         return nullptr;
     osdhip_batch *b = new (std::nothrow) osdhip_batch;
     if (!b) return nullptr;
+    *owned = b;
     b->length = length;
     int dev = 0;
     hipDeviceProp_t prop;
@@ -568,6 +572,20 @@ osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
     return b;
 }
 
+osdhip_batch *osdhip_batch_create(int length, int numStencilSegments,
+                                  const osdhip_batch_stencils *stencils, int numPatchSegments,
+                                  const osdhip_batch_patches *patches, void *stream)
+{
+    osdhip_batch *owned = nullptr;
+    try {
+        return batch_create(length, numStencilSegments, stencils, numPatchSegments, patches,
+                            stream, &owned);
+    } catch (...) {
+        freeBatch(owned);
+        return nullptr;
+    }
+}
+
 void osdhip_batch_destroy(osdhip_batch *b) { freeBatch(b); }
 
 int osdhip_batch_launch(osdhip_batch *b, void *stream)
@@ -706,106 +724,112 @@ int osdhip_batch_info(const osdhip_batch *b, long long *stencilRows, long long *
 
 int osdhip_batch_tune(osdhip_batch *b, int reps, void *stream, char *report, int reportBytes)
 {
-    if (!b || reps < 1) return OSDHIP_ERR_ARGS;
-    hipStream_t st = (hipStream_t)stream;
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return last_error();
-    std::string rep = "{";
-    auto timePhase = [&](int phase, float &best) -> int {
-        // one untimed run, then the median of reps timed runs
-        int rc = osdhip_batch_run(b, phase, stream);
-        std::vector<float> ms;
-        for (int r = 0; rc == OSDHIP_OK && r < reps; ++r) {
-            (void)hipEventRecord(e0, st);
-            rc = osdhip_batch_run(b, phase, stream);
-            (void)hipEventRecord(e1, st);
-            if (rc == OSDHIP_OK) rc = (int)hipEventSynchronize(e1);
-            float t = 0.0f;
-            if (rc == OSDHIP_OK) rc = (int)hipEventElapsedTime(&t, e0, e1);
-            ms.push_back(t);
+    try {
+        if (!b || reps < 1) return OSDHIP_ERR_ARGS;
+        hipStream_t st = (hipStream_t)stream;
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return last_error();
+        std::string rep = "{";
+        auto timePhase = [&](int phase, float &best) -> int {
+            // one untimed run, then the median of reps timed runs
+            int rc = osdhip_batch_run(b, phase, stream);
+            std::vector<float> ms;
+            for (int r = 0; rc == OSDHIP_OK && r < reps; ++r) {
+                (void)hipEventRecord(e0, st);
+                rc = osdhip_batch_run(b, phase, stream);
+                (void)hipEventRecord(e1, st);
+                if (rc == OSDHIP_OK) rc = (int)hipEventSynchronize(e1);
+                float t = 0.0f;
+                if (rc == OSDHIP_OK) rc = (int)hipEventElapsedTime(&t, e0, e1);
+                ms.push_back(t);
+            }
+            if (rc != OSDHIP_OK) return rc;
+            std::sort(ms.begin(), ms.end());
+            best = ms[ms.size() / 2];
+            return OSDHIP_OK;
+        };
+        int rc = OSDHIP_OK;
+        char buf[160];
+        // phase A: (frames per chunk, gather group) x item slices x scheduling
+        if (b->nS) {
+            const std::vector<std::pair<int, int>> fg =
+                b->tiled ? std::vector<std::pair<int, int>>{ { 1, 4 }, { 1, 8 }, { 2, 4 }, { 2, 8 },
+                                                             { 4, 4 }, { 4, 8 } }
+                         : std::vector<std::pair<int, int>>{ { 0, 4 }, { 1, 4 }, { 1, 8 }, { 2, 4 },
+                                                             { 4, 2 } };
+            const int slices[3] = { 4, 16, 64 };
+            BatchConfig bestc = b->cfg;
+            float bestms = 1e30f;
+            rep += "\"phase_a\":[";
+            bool first = true;
+            for (auto &v : fg)
+                for (int sl : slices)
+                    for (int pers = 0; pers < 2; ++pers) {
+                        b->cfg.aFrames = v.first;
+                        b->cfg.aGroup = v.second;
+                        b->cfg.aItemSlices = sl;
+                        b->cfg.aPersistent = pers;
+                        float t = 0.0f;
+                        rc = timePhase(1, t);
+                        if (rc != OSDHIP_OK) break;
+                        std::snprintf(buf, sizeof(buf),
+                                      "%s{\"frames\":%d,\"group\":%d,\"slices\":%d,\"persistent\":%d,\"ms\":%.4f}",
+                                      first ? "" : ",", v.first, v.second, sl, pers, t);
+                        rep += buf;
+                        first = false;
+                        if (t < bestms) { bestms = t; bestc = b->cfg; }
+                    }
+            rep += "]";
+            BatchConfig keep = b->cfg;
+            keep.aFrames = bestc.aFrames;
+            keep.aGroup = bestc.aGroup;
+            keep.aItemSlices = bestc.aItemSlices;
+            keep.aPersistent = bestc.aPersistent;
+            b->cfg = keep;
         }
-        if (rc != OSDHIP_OK) return rc;
-        std::sort(ms.begin(), ms.end());
-        best = ms[ms.size() / 2];
-        return OSDHIP_OK;
-    };
-    int rc = OSDHIP_OK;
-    char buf[160];
-    // phase A: (frames per chunk, gather group) x item slices x scheduling
-    if (b->nS) {
-        const std::vector<std::pair<int, int>> fg =
-            b->tiled ? std::vector<std::pair<int, int>>{ { 1, 4 }, { 1, 8 }, { 2, 4 }, { 2, 8 },
-                                                         { 4, 4 }, { 4, 8 } }
-                     : std::vector<std::pair<int, int>>{ { 0, 4 }, { 1, 4 }, { 1, 8 }, { 2, 4 },
-                                                         { 4, 2 } };
-        const int slices[3] = { 4, 16, 64 };
-        BatchConfig bestc = b->cfg;
-        float bestms = 1e30f;
-        rep += "\"phase_a\":[";
-        bool first = true;
-        for (auto &v : fg)
-            for (int sl : slices)
-                for (int pers = 0; pers < 2; ++pers) {
-                    b->cfg.aFrames = v.first;
-                    b->cfg.aGroup = v.second;
-                    b->cfg.aItemSlices = sl;
-                    b->cfg.aPersistent = pers;
-                    float t = 0.0f;
-                    rc = timePhase(1, t);
-                    if (rc != OSDHIP_OK) break;
-                    std::snprintf(buf, sizeof(buf),
-                                  "%s{\"frames\":%d,\"group\":%d,\"slices\":%d,\"persistent\":%d,\"ms\":%.4f}",
-                                  first ? "" : ",", v.first, v.second, sl, pers, t);
-                    rep += buf;
-                    first = false;
-                    if (t < bestms) { bestms = t; bestc = b->cfg; }
-                }
-        rep += "]";
-        BatchConfig keep = b->cfg;
-        keep.aFrames = bestc.aFrames;
-        keep.aGroup = bestc.aGroup;
-        keep.aItemSlices = bestc.aItemSlices;
-        keep.aPersistent = bestc.aPersistent;
-        b->cfg = keep;
+        // phase B: frames per chunk x scheduling (needs phase A's outputs: run it once first)
+        if (rc == OSDHIP_OK && b->nP) {
+            if (b->nS) rc = osdhip_batch_run(b, 1, stream);
+            const int fcs[3] = { 1, 2, 4 };
+            BatchConfig bestc = b->cfg;
+            float bestms = 1e30f;
+            rep += b->nS ? ",\"phase_b\":[" : "\"phase_b\":[";
+            bool first = true;
+            for (int fc : fcs)
+                for (int pers = 0; rc == OSDHIP_OK && pers < 2; ++pers)
+                    for (int stg = 0; rc == OSDHIP_OK && stg < 2; ++stg) {
+                        b->cfg.bFrames = fc;
+                        b->cfg.bPersistent = pers;
+                        b->cfg.bStage = stg;
+                        float t = 0.0f;
+                        rc = timePhase(2, t);
+                        if (rc != OSDHIP_OK) break;
+                        std::snprintf(buf, sizeof(buf),
+                                      "%s{\"frames\":%d,\"persistent\":%d,\"stage\":%d,\"ms\":%.4f}",
+                                      first ? "" : ",", fc, pers, stg, t);
+                        rep += buf;
+                        first = false;
+                        if (t < bestms) { bestms = t; bestc = b->cfg; }
+                    }
+            rep += "]";
+            b->cfg.bFrames = bestc.bFrames;
+            b->cfg.bPersistent = bestc.bPersistent;
+            b->cfg.bStage = bestc.bStage;
+        }
+        rep += "}";
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc == OSDHIP_OK) rc = capture(b);
+        if (report && reportBytes > 0) {
+            std::strncpy(report, rep.c_str(), (size_t)reportBytes - 1);
+            report[reportBytes - 1] = '\0';
+        }
+        return rc;
+    } catch (const std::bad_alloc &) {
+        return (int)hipErrorOutOfMemory;
+    } catch (...) {
+        return (int)hipErrorUnknown;
     }
-    // phase B: frames per chunk x scheduling (needs phase A's outputs: run it once first)
-    if (rc == OSDHIP_OK && b->nP) {
-        if (b->nS) rc = osdhip_batch_run(b, 1, stream);
-        const int fcs[3] = { 1, 2, 4 };
-        BatchConfig bestc = b->cfg;
-        float bestms = 1e30f;
-        rep += b->nS ? ",\"phase_b\":[" : "\"phase_b\":[";
-        bool first = true;
-        for (int fc : fcs)
-            for (int pers = 0; rc == OSDHIP_OK && pers < 2; ++pers)
-                for (int stg = 0; rc == OSDHIP_OK && stg < 2; ++stg) {
-                    b->cfg.bFrames = fc;
-                    b->cfg.bPersistent = pers;
-                    b->cfg.bStage = stg;
-                    float t = 0.0f;
-                    rc = timePhase(2, t);
-                    if (rc != OSDHIP_OK) break;
-                    std::snprintf(buf, sizeof(buf),
-                                  "%s{\"frames\":%d,\"persistent\":%d,\"stage\":%d,\"ms\":%.4f}",
-                                  first ? "" : ",", fc, pers, stg, t);
-                    rep += buf;
-                    first = false;
-                    if (t < bestms) { bestms = t; bestc = b->cfg; }
-                }
-        rep += "]";
-        b->cfg.bFrames = bestc.bFrames;
-        b->cfg.bPersistent = bestc.bPersistent;
-        b->cfg.bStage = bestc.bStage;
-    }
-    rep += "}";
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc == OSDHIP_OK) rc = capture(b);
-    if (report && reportBytes > 0) {
-        std::strncpy(report, rep.c_str(), (size_t)reportBytes - 1);
-        report[reportBytes - 1] = '\0';
-    }
-    return rc;
 }
 
 }  // extern "C"

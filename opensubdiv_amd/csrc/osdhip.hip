@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <vector>
 
@@ -304,11 +305,13 @@ static bool build_tiled(osdhip_stencil_table *t, hipStream_t st)
         ok = hipGetLastError() == hipSuccess;
     }
     if (ok) {
-        std::vector<int> nu((size_t)S);
-        ok = hipMemcpyAsync(nu.data(), t->nuniq, (size_t)S * sizeof(int), hipMemcpyDeviceToHost,
-                            st) == hipSuccess &&
+        std::unique_ptr<int[]> nu(new (std::nothrow) int[(size_t)S]);
+        ok = nu && hipMemcpyAsync(nu.get(), t->nuniq, (size_t)S * sizeof(int),
+                                  hipMemcpyDeviceToHost, st) == hipSuccess &&
              hipStreamSynchronize(st) == hipSuccess;
-        t->listedSlices = (int)std::count_if(nu.begin(), nu.end(), [](int u) { return u >= 0; });
+        if (ok)
+            t->listedSlices = (int)std::count_if(nu.get(), nu.get() + S,
+                                                 [](int u) { return u >= 0; });
     }
     if (ok) {
         t->sliceEntries = exclusive_scan(entries, t->sliceOff, S, st);
@@ -400,7 +403,9 @@ osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *si
     hipStream_t st = (hipStream_t)stream;
 
     // CSR mirror (offsets regenerated when absent: exclusive prefix of sizes)
-    std::vector<int> off(numStencils);
+    std::unique_ptr<int[]> offBuf(new (std::nothrow) int[(size_t)std::max(numStencils, 1)]);
+    if (!offBuf) { free_table(t); return nullptr; }
+    int *const off = offBuf.get();
     long long run = 0;
     for (int i = 0; i < numStencils; ++i) {
         if (sizes[i] < 0) { free_table(t); return nullptr; }
@@ -411,7 +416,7 @@ osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *si
     for (int i = 0; i < numStencils; ++i) nnz = std::max(nnz, (long long)off[i] + sizes[i]);
     t->nnz = nnz;
     bool ok = upload(&t->sizes, sizes, (size_t)numStencils, st, t->bytes) &&
-              upload(&t->offsets, off.data(), (size_t)numStencils, st, t->bytes) &&
+              upload(&t->offsets, off, (size_t)numStencils, st, t->bytes) &&
               upload(&t->indices, indices, (size_t)nnz, st, t->bytes);
     for (int k = 0; ok && k < numWeightArrays; ++k)
         ok = upload(&t->w[k], weights[k], (size_t)nnz, st, t->bytes);
@@ -749,84 +754,88 @@ osdhip_patch_map *osdhip_patch_map_create(const osdhip_patch_array *arrays, int 
                                           const osdhip_patch_param *params, int numPatches,
                                           void *stream)
 {
-    if (!arrays || !params || numArrays <= 0 || numPatches <= 0) return nullptr;
-    // handles: far/patchMap.cpp:100-125 (patches in array order, vertIndex
-    // counted within the array)
-    std::vector<int> handles(3 * (size_t)numPatches, -1);
-    long long total = 0;
-    for (int a = 0; a < numArrays; ++a) {
-        const osdhip_patch_array &A = arrays[a];
-        if (A.numPatches < 0 || A.stride < 0 || A.primitiveIdBase != total) return nullptr;
-        total += A.numPatches;
-        if (total > numPatches) return nullptr;
-        for (int j = 0; j < A.numPatches; ++j) {
-            const size_t p = (size_t)A.primitiveIdBase + (size_t)j;
-            handles[3 * p] = a;
-            handles[3 * p + 1] = (int)p;
-            handles[3 * p + 2] = j * A.stride;
-        }
-    }
-    if (total != numPatches) return nullptr;
-    // quadtree: far/patchMap.cpp:130-200, one root per face in
-    // [minFace, maxFace]; a patch at its root depth (0, or 1 under an
-    // n-gon) fills all four quadrants of the root
-    int minFace = (int)(params[0].field0 & 0xfffffffu), maxFace = minFace;
-    for (int p = 1; p < numPatches; ++p) {
-        const int f = (int)(params[p].field0 & 0xfffffffu);
-        minFace = std::min(minFace, f);
-        maxFace = std::max(maxFace, f);
-    }
-    const osdhip::v4i empty = { osdhip::kMapEmpty, osdhip::kMapEmpty, osdhip::kMapEmpty,
-                                osdhip::kMapEmpty };
-    std::vector<osdhip::v4i> nodes((size_t)(maxFace - minFace) + 1, empty);
-    int maxDepth = 0;
-    for (int p = 0; p < numPatches; ++p) {
-        const unsigned f1 = params[p].field1;
-        const int depth = (int)(f1 & 0xfu), rootDepth = (int)((f1 >> 4) & 1u);
-        if (depth < rootDepth) return nullptr;
-        maxDepth = std::max(maxDepth, depth);
-        int node = (int)(params[p].field0 & 0xfffffffu) - minFace;
-        const int leaf = -(p + 1);
-        if (depth == rootDepth) {
-            nodes[(size_t)node] = osdhip::v4i{ leaf, leaf, leaf, leaf };
-            continue;
-        }
-        const int u = (int)((f1 >> 22) & 0x3ffu), v = (int)((f1 >> 12) & 0x3ffu);
-        for (int j = rootDepth + 1; j <= depth; ++j) {
-            const int q = (((v >> (depth - j)) & 1) << 1) | ((u >> (depth - j)) & 1);
-            if (j == depth) {
-                nodes[(size_t)node][q] = leaf;
-            } else {
-                int c = nodes[(size_t)node][q];
-                if (c < 0) {
-                    c = (int)nodes.size();
-                    nodes[(size_t)node][q] = c;
-                    nodes.push_back(empty);
-                }
-                node = c;
+    try {
+        if (!arrays || !params || numArrays <= 0 || numPatches <= 0) return nullptr;
+        // handles: far/patchMap.cpp:100-125 (patches in array order, vertIndex
+        // counted within the array)
+        std::vector<int> handles(3 * (size_t)numPatches, -1);
+        long long total = 0;
+        for (int a = 0; a < numArrays; ++a) {
+            const osdhip_patch_array &A = arrays[a];
+            if (A.numPatches < 0 || A.stride < 0 || A.primitiveIdBase != total) return nullptr;
+            total += A.numPatches;
+            if (total > numPatches) return nullptr;
+            for (int j = 0; j < A.numPatches; ++j) {
+                const size_t p = (size_t)A.primitiveIdBase + (size_t)j;
+                handles[3 * p] = a;
+                handles[3 * p + 1] = (int)p;
+                handles[3 * p + 2] = j * A.stride;
             }
         }
-    }
-    osdhip_patch_map *m = new (std::nothrow) osdhip_patch_map;
-    if (!m) return nullptr;
-    m->minFace = minFace;
-    m->maxFace = maxFace;
-    m->maxDepth = maxDepth;
-    hipStream_t st = (hipStream_t)stream;
-    const bool ok =
-        hipMalloc((void **)&m->nodes, nodes.size() * sizeof(osdhip::v4i)) == hipSuccess &&
-        hipMalloc((void **)&m->handles, handles.size() * sizeof(int)) == hipSuccess &&
-        hipMemcpyAsync(m->nodes, nodes.data(), nodes.size() * sizeof(osdhip::v4i),
-                       hipMemcpyHostToDevice, st) == hipSuccess &&
-        hipMemcpyAsync(m->handles, handles.data(), handles.size() * sizeof(int),
-                       hipMemcpyHostToDevice, st) == hipSuccess &&
-        hipStreamSynchronize(st) == hipSuccess;
-    if (!ok) {
-        (void)hipGetLastError();
-        osdhip_patch_map_destroy(m);
+        if (total != numPatches) return nullptr;
+        // quadtree: far/patchMap.cpp:130-200, one root per face in
+        // [minFace, maxFace]; a patch at its root depth (0, or 1 under an
+        // n-gon) fills all four quadrants of the root
+        int minFace = (int)(params[0].field0 & 0xfffffffu), maxFace = minFace;
+        for (int p = 1; p < numPatches; ++p) {
+            const int f = (int)(params[p].field0 & 0xfffffffu);
+            minFace = std::min(minFace, f);
+            maxFace = std::max(maxFace, f);
+        }
+        const osdhip::v4i empty = { osdhip::kMapEmpty, osdhip::kMapEmpty, osdhip::kMapEmpty,
+                                    osdhip::kMapEmpty };
+        std::vector<osdhip::v4i> nodes((size_t)(maxFace - minFace) + 1, empty);
+        int maxDepth = 0;
+        for (int p = 0; p < numPatches; ++p) {
+            const unsigned f1 = params[p].field1;
+            const int depth = (int)(f1 & 0xfu), rootDepth = (int)((f1 >> 4) & 1u);
+            if (depth < rootDepth) return nullptr;
+            maxDepth = std::max(maxDepth, depth);
+            int node = (int)(params[p].field0 & 0xfffffffu) - minFace;
+            const int leaf = -(p + 1);
+            if (depth == rootDepth) {
+                nodes[(size_t)node] = osdhip::v4i{ leaf, leaf, leaf, leaf };
+                continue;
+            }
+            const int u = (int)((f1 >> 22) & 0x3ffu), v = (int)((f1 >> 12) & 0x3ffu);
+            for (int j = rootDepth + 1; j <= depth; ++j) {
+                const int q = (((v >> (depth - j)) & 1) << 1) | ((u >> (depth - j)) & 1);
+                if (j == depth) {
+                    nodes[(size_t)node][q] = leaf;
+                } else {
+                    int c = nodes[(size_t)node][q];
+                    if (c < 0) {
+                        c = (int)nodes.size();
+                        nodes[(size_t)node][q] = c;
+                        nodes.push_back(empty);
+                    }
+                    node = c;
+                }
+            }
+        }
+        osdhip_patch_map *m = new (std::nothrow) osdhip_patch_map;
+        if (!m) return nullptr;
+        m->minFace = minFace;
+        m->maxFace = maxFace;
+        m->maxDepth = maxDepth;
+        hipStream_t st = (hipStream_t)stream;
+        const bool ok =
+            hipMalloc((void **)&m->nodes, nodes.size() * sizeof(osdhip::v4i)) == hipSuccess &&
+            hipMalloc((void **)&m->handles, handles.size() * sizeof(int)) == hipSuccess &&
+            hipMemcpyAsync(m->nodes, nodes.data(), nodes.size() * sizeof(osdhip::v4i),
+                           hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipMemcpyAsync(m->handles, handles.data(), handles.size() * sizeof(int),
+                           hipMemcpyHostToDevice, st) == hipSuccess &&
+            hipStreamSynchronize(st) == hipSuccess;
+        if (!ok) {
+            (void)hipGetLastError();
+            osdhip_patch_map_destroy(m);
+            return nullptr;
+        }
+        return m;
+    } catch (...) {
         return nullptr;
     }
-    return m;
 }
 
 int osdhip_patch_map_find(const osdhip_patch_map *map, int n, const int *ptexFaces,

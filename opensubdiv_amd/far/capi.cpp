@@ -63,22 +63,26 @@ osdfar_topology *osdfar_topology_create(int numVertices, int numFaces,
                                         const float *cornerWeights,
                                         int boundaryInterpolation)
 {
-    if (numVertices <= 0 || numFaces <= 0 || !numVertsPerFace || !vertIndicesPerFace) return nullptr;
-    TopologyDesc d;
-    d.numVertices = numVertices;
-    d.numFaces = numFaces;
-    d.numVertsPerFace = numVertsPerFace;
-    d.vertIndicesPerFace = vertIndicesPerFace;
-    d.numCreases = creaseVertexIndexPairs && creaseWeights ? numCreases : 0;
-    d.creaseVertexIndexPairs = creaseVertexIndexPairs;
-    d.creaseWeights = creaseWeights;
-    d.numCorners = cornerVertexIndices && cornerWeights ? numCorners : 0;
-    d.cornerVertexIndices = cornerVertexIndices;
-    d.cornerWeights = cornerWeights;
-    d.boundaryInterpolation = boundaryInterpolation;
-    std::unique_ptr<osdfar_topology> t(new (std::nothrow) osdfar_topology);
-    if (!t || !t->refiner.create(d)) return nullptr;
-    return t.release();
+    try {
+        if (numVertices <= 0 || numFaces <= 0 || !numVertsPerFace || !vertIndicesPerFace) return nullptr;
+        TopologyDesc d;
+        d.numVertices = numVertices;
+        d.numFaces = numFaces;
+        d.numVertsPerFace = numVertsPerFace;
+        d.vertIndicesPerFace = vertIndicesPerFace;
+        d.numCreases = creaseVertexIndexPairs && creaseWeights ? numCreases : 0;
+        d.creaseVertexIndexPairs = creaseVertexIndexPairs;
+        d.creaseWeights = creaseWeights;
+        d.numCorners = cornerVertexIndices && cornerWeights ? numCorners : 0;
+        d.cornerVertexIndices = cornerVertexIndices;
+        d.cornerWeights = cornerWeights;
+        d.boundaryInterpolation = boundaryInterpolation;
+        std::unique_ptr<osdfar_topology> t(new (std::nothrow) osdfar_topology);
+        if (!t || !t->refiner.create(d)) return nullptr;
+        return t.release();
+    } catch (...) {
+        return nullptr;
+    }
 }
 
 int osdfar_topology_set_creasing(osdfar_topology *topo, int method)
@@ -93,10 +97,14 @@ void osdfar_topology_destroy(osdfar_topology *topo) { delete topo; }
 
 int osdfar_topology_cache_stencils(osdfar_topology *topo, int level)
 {
-    if (!topo || level < 0 || level > 10 || topo->refiner.maxLevel() > level) return -1;
-    topo->refiner.refineUniform(level);
-    computeLevelStencils(topo->refiner, INTERPOLATE_VERTEX, topo->levels);
-    return 0;
+    try {
+        if (!topo || level < 0 || level > 10 || topo->refiner.maxLevel() > level) return -1;
+        topo->refiner.refineUniform(level);
+        computeLevelStencils(topo->refiner, INTERPOLATE_VERTEX, topo->levels);
+        return 0;
+    } catch (...) {
+        return OSDFAR_ERR_HOST;
+    }
 }
 
 int osdfar_topology_drop_stencils(osdfar_topology *topo)
@@ -108,10 +116,14 @@ int osdfar_topology_drop_stencils(osdfar_topology *topo)
 
 int osdfar_refine_uniform(osdfar_topology *topo, int maxLevel)
 {
-    if (!topo || maxLevel < 0 || maxLevel > 10) return -1;
-    if (topo->refiner.maxLevel() > maxLevel) return -1;
-    topo->refiner.refineUniform(maxLevel);
-    return 0;
+    try {
+        if (!topo || maxLevel < 0 || maxLevel > 10) return -1;
+        if (topo->refiner.maxLevel() > maxLevel) return -1;
+        topo->refiner.refineUniform(maxLevel);
+        return 0;
+    } catch (...) {
+        return OSDFAR_ERR_HOST;
+    }
 }
 
 int osdfar_level_info(const osdfar_topology *topo, int level, int *nv, int *nf, int *ne)
@@ -136,23 +148,27 @@ osdfar_stencil_table *osdfar_stencil_table_create(const osdfar_topology *topo, i
                                                   int intermediateLevels, int controlVerts,
                                                   int useDouble)
 {
-    if (!topo || (mode != 0 && mode != 1)) return nullptr;
-    std::unique_ptr<osdfar_stencil_table> t(new (std::nothrow) osdfar_stencil_table);
-    if (!t) return nullptr;
-    StencilOptions o;
-    o.interpolationMode = mode;
-    o.generateIntermediateLevels = intermediateLevels != 0;
-    o.generateControlVerts = controlVerts != 0;
-    t->isDouble = useDouble != 0;
-    const bool cached = mode == 0 && (int)topo->levels.size() == topo->refiner.maxLevel() + 1;
-    if (cached) {
-        if (t->isDouble) stencilTableFromLevels(topo->refiner, topo->levels, o, t->d);
-        else stencilTableFromLevels(topo->refiner, topo->levels, o, t->f);
-    } else {
-        if (t->isDouble) buildStencilTable(topo->refiner, o, t->d);
-        else buildStencilTable(topo->refiner, o, t->f);
+    try {
+        if (!topo || (mode != 0 && mode != 1)) return nullptr;
+        std::unique_ptr<osdfar_stencil_table> t(new (std::nothrow) osdfar_stencil_table);
+        if (!t) return nullptr;
+        StencilOptions o;
+        o.interpolationMode = mode;
+        o.generateIntermediateLevels = intermediateLevels != 0;
+        o.generateControlVerts = controlVerts != 0;
+        t->isDouble = useDouble != 0;
+        const bool cached = mode == 0 && (int)topo->levels.size() == topo->refiner.maxLevel() + 1;
+        if (cached) {
+            if (t->isDouble) stencilTableFromLevels(topo->refiner, topo->levels, o, t->d);
+            else stencilTableFromLevels(topo->refiner, topo->levels, o, t->f);
+        } else {
+            if (t->isDouble) buildStencilTable(topo->refiner, o, t->d);
+            else buildStencilTable(topo->refiner, o, t->f);
+        }
+        return t.release();
+    } catch (...) {
+        return nullptr;
     }
-    return t.release();
 }
 
 int osdfar_stencil_table_data(const osdfar_stencil_table *t, const int **sizes,
@@ -195,14 +211,18 @@ void osdfar_stencil_table_destroy(osdfar_stencil_table *t) { delete t; }
 
 osdfar_patch_table *osdfar_patch_table_create(osdfar_topology *topo, int isoLevel)
 {
-    if (!topo || isoLevel < 0 || isoLevel > 10) return nullptr;
-    if (topo->refiner.maxLevel() > isoLevel) return nullptr;
-    std::unique_ptr<osdfar_patch_table> t(new (std::nothrow) osdfar_patch_table);
-    const bool cached = (int)topo->levels.size() == topo->refiner.maxLevel() + 1 &&
-                        topo->refiner.maxLevel() == isoLevel;
-    if (!t || !t->table.build(topo->refiner, isoLevel, cached ? &topo->levels : nullptr))
+    try {
+        if (!topo || isoLevel < 0 || isoLevel > 10) return nullptr;
+        if (topo->refiner.maxLevel() > isoLevel) return nullptr;
+        std::unique_ptr<osdfar_patch_table> t(new (std::nothrow) osdfar_patch_table);
+        const bool cached = (int)topo->levels.size() == topo->refiner.maxLevel() + 1 &&
+                            topo->refiner.maxLevel() == isoLevel;
+        if (!t || !t->table.build(topo->refiner, isoLevel, cached ? &topo->levels : nullptr))
+            return nullptr;
+        return t.release();
+    } catch (...) {
         return nullptr;
-    return t.release();
+    }
 }
 
 void osdfar_patch_table_destroy(osdfar_patch_table *t) { delete t; }
@@ -235,33 +255,41 @@ int osdfar_patch_table_copy(const osdfar_patch_table *t, int *arrays, int *indic
 
 osdfar_stencil_table *osdfar_patch_table_point_stencils(const osdfar_patch_table *t, int useDouble)
 {
-    if (!t) return nullptr;
-    std::unique_ptr<osdfar_stencil_table> s(new (std::nothrow) osdfar_stencil_table);
-    if (!s) return nullptr;
-    s->isDouble = useDouble != 0;
-    if (s->isDouble) t->table.pointStencilTable(s->d);
-    else t->table.pointStencilTable(s->f);
-    return s.release();
+    try {
+        if (!t) return nullptr;
+        std::unique_ptr<osdfar_stencil_table> s(new (std::nothrow) osdfar_stencil_table);
+        if (!s) return nullptr;
+        s->isDouble = useDouble != 0;
+        if (s->isDouble) t->table.pointStencilTable(s->d);
+        else t->table.pointStencilTable(s->f);
+        return s.release();
+    } catch (...) {
+        return nullptr;
+    }
 }
 
 int osdfar_patch_map_find(const osdfar_patch_table *t, int n, const int *ptex, const double *u,
                           const double *v, void *patchCoords)
 {
-    if (!t || n < 0 || (n > 0 && (!ptex || !u || !v || !patchCoords))) return -1;
-    struct Coord { int arrayIndex, patchIndex, vertIndex; float s, t; };
-    Coord *out = (Coord *)patchCoords;
-    int found = 0;
-#pragma omp parallel for reduction(+ : found)
-    for (int i = 0; i < n; ++i) {
-        PatchHandleRec h;
-        if (t->table.findPatch(ptex[i], u[i], v[i], h)) {
-            out[i] = { h.arrayIndex, h.patchIndex, h.vertIndex, (float)u[i], (float)v[i] };
-            ++found;
-        } else {
-            out[i] = { -1, -1, -1, (float)u[i], (float)v[i] };
+    try {
+        if (!t || n < 0 || (n > 0 && (!ptex || !u || !v || !patchCoords))) return -1;
+        struct Coord { int arrayIndex, patchIndex, vertIndex; float s, t; };
+        Coord *out = (Coord *)patchCoords;
+        int found = 0;
+    #pragma omp parallel for reduction(+ : found)
+        for (int i = 0; i < n; ++i) {
+            PatchHandleRec h;
+            if (t->table.findPatch(ptex[i], u[i], v[i], h)) {
+                out[i] = { h.arrayIndex, h.patchIndex, h.vertIndex, (float)u[i], (float)v[i] };
+                ++found;
+            } else {
+                out[i] = { -1, -1, -1, (float)u[i], (float)v[i] };
+            }
         }
+        return found;
+    } catch (...) {
+        return OSDFAR_ERR_HOST;
     }
-    return found;
 }
 
 osdfar_stencil_table *osdfar_limit_stencil_table_create(const osdfar_patch_table *t, int n,
@@ -269,27 +297,35 @@ osdfar_stencil_table *osdfar_limit_stencil_table_create(const osdfar_patch_table
                                                         const double *v, int secondDerivatives,
                                                         int useDouble)
 {
-    if (!t || n < 0 || (n > 0 && (!ptex || !u || !v))) return nullptr;
-    std::unique_ptr<osdfar_stencil_table> s(new (std::nothrow) osdfar_stencil_table);
-    if (!s) return nullptr;
-    s->isDouble = useDouble != 0;
-    s->numWeightArrays = secondDerivatives ? 6 : 3;
-    if (s->isDouble) t->table.limitStencils(n, ptex, u, v, secondDerivatives != 0, s->d);
-    else t->table.limitStencils(n, ptex, u, v, secondDerivatives != 0, s->f);
-    return s.release();
+    try {
+        if (!t || n < 0 || (n > 0 && (!ptex || !u || !v))) return nullptr;
+        std::unique_ptr<osdfar_stencil_table> s(new (std::nothrow) osdfar_stencil_table);
+        if (!s) return nullptr;
+        s->isDouble = useDouble != 0;
+        s->numWeightArrays = secondDerivatives ? 6 : 3;
+        if (s->isDouble) t->table.limitStencils(n, ptex, u, v, secondDerivatives != 0, s->d);
+        else t->table.limitStencils(n, ptex, u, v, secondDerivatives != 0, s->f);
+        return s.release();
+    } catch (...) {
+        return nullptr;
+    }
 }
 
 int osdfar_vertex_locations(const osdfar_topology *topo, int level, int *ptex, double *u,
                             double *v)
 {
-    if (!topo || level < 0 || level > topo->refiner.maxLevel() || !ptex || !u || !v) return -1;
-    std::vector<int> p;
-    std::vector<double> uu, vv;
-    uniformVertexLocations(topo->refiner, level, p, uu, vv);
-    std::memcpy(ptex, p.data(), p.size() * sizeof(int));
-    std::memcpy(u, uu.data(), uu.size() * sizeof(double));
-    std::memcpy(v, vv.data(), vv.size() * sizeof(double));
-    return 0;
+    try {
+        if (!topo || level < 0 || level > topo->refiner.maxLevel() || !ptex || !u || !v) return -1;
+        std::vector<int> p;
+        std::vector<double> uu, vv;
+        uniformVertexLocations(topo->refiner, level, p, uu, vv);
+        std::memcpy(ptex, p.data(), p.size() * sizeof(int));
+        std::memcpy(u, uu.data(), uu.size() * sizeof(double));
+        std::memcpy(v, vv.data(), vv.size() * sizeof(double));
+        return 0;
+    } catch (...) {
+        return OSDFAR_ERR_HOST;
+    }
 }
 
 }  // extern "C"

@@ -168,135 +168,147 @@ unsigned long long osdfar_topology_key(int numVertices, int numFaces, const int 
                                        const int *cornerVertexIndices, const float *cornerWeights,
                                        int boundaryInterpolation, const char *recipe)
 {
-    if (numVertices < 0 || numFaces < 0 || numCreases < 0 || numCorners < 0) return 0;
-    if ((numFaces > 0 && !numVertsPerFace) || (numCreases > 0 && !(creaseVertexIndexPairs &&
-                                                                  creaseWeights)) ||
-        (numCorners > 0 && !(cornerVertexIndices && cornerWeights)))
+    try {
+        if (numVertices < 0 || numFaces < 0 || numCreases < 0 || numCorners < 0) return 0;
+        if ((numFaces > 0 && !numVertsPerFace) || (numCreases > 0 && !(creaseVertexIndexPairs &&
+                                                                      creaseWeights)) ||
+            (numCorners > 0 && !(cornerVertexIndices && cornerWeights)))
+            return 0;
+        long long nfv = 0;
+        for (int f = 0; f < numFaces; ++f) {
+            if (numVertsPerFace[f] < 0) return 0;
+            nfv += numVertsPerFace[f];
+        }
+        if (nfv > 0 && !vertIndicesPerFace) return 0;
+        // each field is hashed on its own and chained, with its length, so that
+        // moving bytes between fields changes the key
+        const int counts[5] = {numVertices, numFaces, numCreases, numCorners, boundaryInterpolation};
+        uint64_t h = xxh64(counts, sizeof(counts), 0);
+        auto chain = [&h](const void *p, size_t n) { h = xxh64(p, n, h ^ (uint64_t)n); };
+        chain(numVertsPerFace, (size_t)numFaces * sizeof(int));
+        chain(vertIndicesPerFace, (size_t)nfv * sizeof(int));
+        chain(creaseVertexIndexPairs, (size_t)numCreases * 2 * sizeof(int));
+        chain(creaseWeights, (size_t)numCreases * sizeof(float));
+        chain(cornerVertexIndices, (size_t)numCorners * sizeof(int));
+        chain(cornerWeights, (size_t)numCorners * sizeof(float));
+        const char *r = recipe ? recipe : "";
+        chain(r, std::strlen(r));
+        return h;
+    } catch (...) {
         return 0;
-    long long nfv = 0;
-    for (int f = 0; f < numFaces; ++f) {
-        if (numVertsPerFace[f] < 0) return 0;
-        nfv += numVertsPerFace[f];
     }
-    if (nfv > 0 && !vertIndicesPerFace) return 0;
-    // each field is hashed on its own and chained, with its length, so that
-    // moving bytes between fields changes the key
-    const int counts[5] = {numVertices, numFaces, numCreases, numCorners, boundaryInterpolation};
-    uint64_t h = xxh64(counts, sizeof(counts), 0);
-    auto chain = [&h](const void *p, size_t n) { h = xxh64(p, n, h ^ (uint64_t)n); };
-    chain(numVertsPerFace, (size_t)numFaces * sizeof(int));
-    chain(vertIndicesPerFace, (size_t)nfv * sizeof(int));
-    chain(creaseVertexIndexPairs, (size_t)numCreases * 2 * sizeof(int));
-    chain(creaseWeights, (size_t)numCreases * sizeof(float));
-    chain(cornerVertexIndices, (size_t)numCorners * sizeof(int));
-    chain(cornerWeights, (size_t)numCorners * sizeof(float));
-    const char *r = recipe ? recipe : "";
-    chain(r, std::strlen(r));
-    return h;
 }
 
 int osdfar_table_file_write(const char *path, unsigned long long key, int numSections,
                             const osdfar_section *sections)
 {
-    if (!path || numSections < 0 || numSections > kMaxSections || (numSections && !sections))
-        return -1;
-    std::vector<Entry> dir((size_t)numSections);
-    size_t off = align_up(sizeof(Header) + (size_t)numSections * sizeof(Entry));
-    for (int i = 0; i < numSections; ++i) {
-        const osdfar_section &s = sections[i];
-        const int es = elem_size(s.dtype);
-        if (!s.name || !es || s.count < 0 || s.count > (1LL << 48) || (s.count > 0 && !s.data))
+    try {
+        if (!path || numSections < 0 || numSections > kMaxSections || (numSections && !sections))
             return -1;
-        const size_t len = std::strlen(s.name);
-        if (len == 0 || len >= (size_t)kNameBytes) return -1;
-        for (int j = 0; j < i; ++j)
-            if (std::strcmp(dir[j].name, s.name) == 0) return -1;   // names are unique
-        Entry &e = dir[i];
-        std::memset(&e, 0, sizeof(e));
-        std::memcpy(e.name, s.name, len);
-        e.dtype = s.dtype;
-        e.elemSize = es;
-        e.count = s.count;
-        e.offset = (int64_t)off;
-        e.hash = xxh64(s.data, (size_t)s.count * (size_t)es, 0);
-        off = align_up(off + (size_t)s.count * (size_t)es);
-    }
-    Header hd;
-    std::memset(&hd, 0, sizeof(hd));
-    std::memcpy(hd.magic, kMagic, sizeof(kMagic));
-    hd.version = kVersion;
-    hd.numSections = (uint32_t)numSections;
-    hd.key = key;
-    hd.fileBytes = off;
-    hd.dirHash = xxh64(dir.data(), dir.size() * sizeof(Entry), 0);
+        std::vector<Entry> dir((size_t)numSections);
+        size_t off = align_up(sizeof(Header) + (size_t)numSections * sizeof(Entry));
+        for (int i = 0; i < numSections; ++i) {
+            const osdfar_section &s = sections[i];
+            const int es = elem_size(s.dtype);
+            if (!s.name || !es || s.count < 0 || s.count > (1LL << 48) || (s.count > 0 && !s.data))
+                return -1;
+            const size_t len = std::strlen(s.name);
+            if (len == 0 || len >= (size_t)kNameBytes) return -1;
+            for (int j = 0; j < i; ++j)
+                if (std::strcmp(dir[j].name, s.name) == 0) return -1;   // names are unique
+            Entry &e = dir[i];
+            std::memset(&e, 0, sizeof(e));
+            std::memcpy(e.name, s.name, len);
+            e.dtype = s.dtype;
+            e.elemSize = es;
+            e.count = s.count;
+            e.offset = (int64_t)off;
+            e.hash = xxh64(s.data, (size_t)s.count * (size_t)es, 0);
+            off = align_up(off + (size_t)s.count * (size_t)es);
+        }
+        Header hd;
+        std::memset(&hd, 0, sizeof(hd));
+        std::memcpy(hd.magic, kMagic, sizeof(kMagic));
+        hd.version = kVersion;
+        hd.numSections = (uint32_t)numSections;
+        hd.key = key;
+        hd.fileBytes = off;
+        hd.dirHash = xxh64(dir.data(), dir.size() * sizeof(Entry), 0);
 
-    // write under a name of this process and call, then rename into place:
-    // a reader sees either no file or a whole one
-    const std::string tmp = std::string(path) + ".tmp." + std::to_string((long)getpid()) + "." +
-                            std::to_string(g_tmpCounter.fetch_add(1));
-    FILE *f = std::fopen(tmp.c_str(), "wb");
-    if (!f) return -2;
-    static const unsigned char zeros[kAlign] = {};
-    bool ok = write_all(f, &hd, sizeof(hd)) && write_all(f, dir.data(), dir.size() * sizeof(Entry));
-    size_t pos = sizeof(hd) + dir.size() * sizeof(Entry);
-    for (int i = 0; ok && i < numSections; ++i) {
-        ok = write_all(f, zeros, (size_t)dir[i].offset - pos);
-        const size_t n = (size_t)dir[i].count * (size_t)dir[i].elemSize;
-        ok = ok && write_all(f, sections[i].data, n);
-        pos = (size_t)dir[i].offset + n;
+        // write under a name of this process and call, then rename into place:
+        // a reader sees either no file or a whole one
+        const std::string tmp = std::string(path) + ".tmp." + std::to_string((long)getpid()) + "." +
+                                std::to_string(g_tmpCounter.fetch_add(1));
+        FILE *f = std::fopen(tmp.c_str(), "wb");
+        if (!f) return -2;
+        static const unsigned char zeros[kAlign] = {};
+        bool ok = write_all(f, &hd, sizeof(hd)) && write_all(f, dir.data(), dir.size() * sizeof(Entry));
+        size_t pos = sizeof(hd) + dir.size() * sizeof(Entry);
+        for (int i = 0; ok && i < numSections; ++i) {
+            ok = write_all(f, zeros, (size_t)dir[i].offset - pos);
+            const size_t n = (size_t)dir[i].count * (size_t)dir[i].elemSize;
+            ok = ok && write_all(f, sections[i].data, n);
+            pos = (size_t)dir[i].offset + n;
+        }
+        ok = ok && write_all(f, zeros, off - pos);
+        ok = (std::fclose(f) == 0) && ok;
+        if (!ok || std::rename(tmp.c_str(), path) != 0) {
+            std::remove(tmp.c_str());
+            return -2;
+        }
+        return 0;
+    } catch (...) {
+        return OSDFAR_ERR_HOST;
     }
-    ok = ok && write_all(f, zeros, off - pos);
-    ok = (std::fclose(f) == 0) && ok;
-    if (!ok || std::rename(tmp.c_str(), path) != 0) {
-        std::remove(tmp.c_str());
-        return -2;
-    }
-    return 0;
 }
 
 osdfar_table_file *osdfar_table_file_open(const char *path, unsigned long long key, int verify)
 {
-    if (!path) return nullptr;
-    const int fd = open(path, O_RDONLY);
-    if (fd < 0) return nullptr;
-    struct stat st;
-    if (fstat(fd, &st) != 0 || (size_t)st.st_size < sizeof(Header)) {
+    try {
+        if (!path) return nullptr;
+        const int fd = open(path, O_RDONLY);
+        if (fd < 0) return nullptr;
+        struct stat st;
+        if (fstat(fd, &st) != 0 || (size_t)st.st_size < sizeof(Header)) {
+            close(fd);
+            return nullptr;
+        }
+        const size_t bytes = (size_t)st.st_size;
+        void *map = mmap(nullptr, bytes, PROT_READ, MAP_PRIVATE, fd, 0);
         close(fd);
+        if (map == MAP_FAILED) return nullptr;
+        auto fail = [&]() -> osdfar_table_file * {
+            munmap(map, bytes);
+            return nullptr;
+        };
+        const unsigned char *base = (const unsigned char *)map;
+        Header hd;
+        std::memcpy(&hd, base, sizeof(hd));
+        if (std::memcmp(hd.magic, kMagic, sizeof(kMagic)) != 0 || hd.version != kVersion ||
+            hd.key != key || hd.fileBytes != bytes || hd.numSections > (uint32_t)kMaxSections ||
+            sizeof(Header) + (size_t)hd.numSections * sizeof(Entry) > bytes)
+            return fail();
+        std::vector<Entry> dir(hd.numSections);
+        if (hd.numSections) std::memcpy(dir.data(), base + sizeof(Header), dir.size() * sizeof(Entry));
+        if (xxh64(dir.data(), dir.size() * sizeof(Entry), 0) != hd.dirHash) return fail();
+        for (const Entry &e : dir) {
+            if (std::memchr(e.name, 0, kNameBytes) == nullptr || e.elemSize != elem_size(e.dtype) ||
+                e.count < 0 || e.offset < 0 || (size_t)e.offset % kAlign != 0 ||
+                (size_t)e.offset > bytes || (size_t)e.count > (bytes - (size_t)e.offset) / (size_t)e.elemSize)
+                return fail();
+            if (verify && xxh64(base + e.offset, (size_t)e.count * (size_t)e.elemSize, 0) != e.hash)
+                return fail();
+        }
+        osdfar_table_file *t = new (std::nothrow) osdfar_table_file;
+        if (!t) return fail();
+        t->map = map;
+        t->bytes = bytes;
+        t->key = hd.key;
+        t->dir = std::move(dir);
+        return t;
+    } catch (...) {
         return nullptr;
     }
-    const size_t bytes = (size_t)st.st_size;
-    void *map = mmap(nullptr, bytes, PROT_READ, MAP_PRIVATE, fd, 0);
-    close(fd);
-    if (map == MAP_FAILED) return nullptr;
-    auto fail = [&]() -> osdfar_table_file * {
-        munmap(map, bytes);
-        return nullptr;
-    };
-    const unsigned char *base = (const unsigned char *)map;
-    Header hd;
-    std::memcpy(&hd, base, sizeof(hd));
-    if (std::memcmp(hd.magic, kMagic, sizeof(kMagic)) != 0 || hd.version != kVersion ||
-        hd.key != key || hd.fileBytes != bytes || hd.numSections > (uint32_t)kMaxSections ||
-        sizeof(Header) + (size_t)hd.numSections * sizeof(Entry) > bytes)
-        return fail();
-    std::vector<Entry> dir(hd.numSections);
-    if (hd.numSections) std::memcpy(dir.data(), base + sizeof(Header), dir.size() * sizeof(Entry));
-    if (xxh64(dir.data(), dir.size() * sizeof(Entry), 0) != hd.dirHash) return fail();
-    for (const Entry &e : dir) {
-        if (std::memchr(e.name, 0, kNameBytes) == nullptr || e.elemSize != elem_size(e.dtype) ||
-            e.count < 0 || e.offset < 0 || (size_t)e.offset % kAlign != 0 ||
-            (size_t)e.offset > bytes || (size_t)e.count > (bytes - (size_t)e.offset) / (size_t)e.elemSize)
-            return fail();
-        if (verify && xxh64(base + e.offset, (size_t)e.count * (size_t)e.elemSize, 0) != e.hash)
-            return fail();
-    }
-    osdfar_table_file *t = new (std::nothrow) osdfar_table_file;
-    if (!t) return fail();
-    t->map = map;
-    t->bytes = bytes;
-    t->key = hd.key;
-    t->dir = std::move(dir);
-    return t;
 }
 
 int osdfar_table_file_sections(const osdfar_table_file *f)
