@@ -91,10 +91,15 @@ void osdfar_stencil_table_destroy(osdfar_stencil_table *t);
 
 /* ---- feature-adaptive patch tables --------------------------------------
  * PatchTableFactory::Create(refiner, Options(isoLevel)) with
- * ENDCAP_GREGORY_BASIS (far/patchTableFactory.h:40-160) for closed,
- * manifold Catmark meshes with semi-sharp creases, quad or n-gon base faces
- * (n-gons need isoLevel >= 1; their patches carry PatchParam's non-quad
- * bit), uniform or Chaikin creasing (returns NULL otherwise).  Refines the topology uniformly to isoLevel.
+ * ENDCAP_GREGORY_BASIS (far/patchTableFactory.h:40-160) for manifold
+ * Catmark meshes with semi-sharp creases, quad or n-gon base faces (n-gons
+ * need isoLevel >= 1; their patches carry PatchParam's non-quad bit),
+ * uniform or Chaikin creasing, and boundaries whose points are regular:
+ * shared by two faces, or single-face corners under edge-and-corner
+ * interpolation (boundary patches carry Far's boundary mask and fill their
+ * missing points with the face's first vertex).  Returns NULL otherwise
+ * (irregular boundary points, interior infinitely sharp features).
+ * PatchParams carry Far's transition masks.  Refines the topology uniformly to isoLevel.
  * The patch points live in a vertex buffer laid out as
  *   [control vertices | refined vertices | end-cap points]
  * whose non-control rows are given by osdfar_patch_table_point_stencils. */

@@ -209,6 +209,22 @@ bool gregoryPoints(const Level &L, int f, Row out[20])
 // interior; far/topologyRefiner.cpp:627-690): non-quad, a base-level corner
 // shared with a non-quad face, an extraordinary corner, or a corner with
 // vertex sharpness or an incident sharp edge.
+bool isBoundaryEdge(const Level &L, int e) { return L.efOff[e + 1] - L.efOff[e] == 1; }
+
+// A corner a B-spline patch reproduces (Far's regular tags, far/patchBuilder.cpp:
+// 581-700 for this scope): interior with 4 faces; on a boundary with 2 faces
+// (the boundary crease); or a boundary corner of 1 face made sharp by
+// edge-and-corner interpolation.
+bool regularCorner(const Level &L, int v)
+{
+    const int nf = L.vfOff[v + 1] - L.vfOff[v], ne = L.valence(v);
+    int nb = 0;
+    for (int i = L.veOff[v]; i < L.veOff[v + 1]; ++i) nb += isBoundaryEdge(L, L.ve[i]);
+    if (nb == 0) return nf == 4 && ne == 4;
+    if (nf == 2) return ne == 3 && nb == 2 && L.vsharp[v] <= 0.0f;
+    return nf == 1 && ne == 2 && L.vsharp[v] >= kSharpInfinite;
+}
+
 bool faceHasFeatures(const Level &L, int f, bool baseLevel)
 {
     if (L.faceSize(f) != 4) return true;
@@ -218,10 +234,13 @@ bool faceHasFeatures(const Level &L, int f, bool baseLevel)
         if (baseLevel)
             for (int i = L.vfOff[v]; i < L.vfOff[v + 1]; ++i)
                 if (L.faceSize(L.vf[i]) != 4) return true;
-        if (L.valence(v) != 4 || (L.vfOff[v + 1] - L.vfOff[v]) != 4) return true;
-        if (L.vsharp[v] > 0.0f) return true;
-        for (int i = L.veOff[v]; i < L.veOff[v + 1]; ++i)
-            if (L.esharp[L.ve[i]] > 0.0f) return true;
+        if (!regularCorner(L, v)) return true;
+        // semi-sharp vertex or edge; the infinitely sharp boundary is regular
+        if (L.vsharp[v] > 0.0f && L.vsharp[v] < kSharpInfinite) return true;
+        for (int i = L.veOff[v]; i < L.veOff[v + 1]; ++i) {
+            const float s = L.esharp[L.ve[i]];
+            if (s > 0.0f && s < kSharpInfinite) return true;
+        }
     }
     return false;
 }
@@ -229,7 +248,7 @@ bool faceHasFeatures(const Level &L, int f, bool baseLevel)
 bool isExtraordinary(const Level &L, int f)
 {
     for (int k = L.fvOff[f]; k < L.fvOff[f + 1]; ++k)
-        if (L.valence(L.fv[k]) != 4) return true;
+        if (!regularCorner(L, L.fv[k])) return true;
     return false;
 }
 
@@ -238,16 +257,58 @@ bool isExtraordinary(const Level &L, int f)
 // the face diagonally opposite at that corner fill the other three slots.
 const int kCornerPoint[4][4] = { { 5, 4, 0, 1 }, { 6, 2, 3, 7 }, { 10, 11, 15, 14 }, { 9, 13, 12, 8 } };
 
-bool regularPoints(const Level &L, int f, int pts[16])
+// Boundary edges of face f as PatchParam's boundary mask: bit c for the
+// face's edge c, from corner c to corner c + 1 (far/patchBuilder.cpp:718-762).
+int boundaryMask(const Level &L, int f)
 {
+    int m = 0;
+    for (int c = 0; c < 4; ++c) m |= isBoundaryEdge(L, L.fe[L.fvOff[f] + c]) << c;
+    return m;
+}
+
+// Corner k (0..3) of face g at vertex v.
+int cornerOf(const Level &L, int g, int v)
+{
+    for (int k = 0; k < 4; ++k)
+        if (L.fv[L.fvOff[g] + k] == v) return k;
+    return -1;
+}
+
+bool regularPoints(const Level &L, int f, int mask, int pts[16])
+{
+    // far/patchBuilder.cpp:876-966: a corner without boundary edges takes the
+    // three points of the face diagonally opposite; with a boundary edge the
+    // missing points are the face's first vertex (fillMissingBoundaryPoints)
+    const int b = L.fvOff[f];
+    const int fill = L.fv[b];
     std::vector<int> ring;
     for (int c = 0; c < 4; ++c) {
-        const int v = L.fv[L.fvOff[f] + c];
-        if (orderedRing(L, v, f, ring) != 4) return false;
+        const int v = L.fv[b + c];
+        const bool lead = (mask >> c) & 1, trail = (mask >> ((c + 3) & 3)) & 1;
         pts[kCornerPoint[c][0]] = v;
-        pts[kCornerPoint[c][1]] = ring[4];
-        pts[kCornerPoint[c][2]] = ring[5];
-        pts[kCornerPoint[c][3]] = ring[6];
+        if (!lead && !trail) {
+            if (orderedRing(L, v, f, ring) != 4) return false;
+            pts[kCornerPoint[c][1]] = ring[4];
+            pts[kCornerPoint[c][2]] = ring[5];
+            pts[kCornerPoint[c][3]] = ring[6];
+        } else if (lead && trail) {
+            pts[kCornerPoint[c][1]] = pts[kCornerPoint[c][2]] = pts[kCornerPoint[c][3]] = fill;
+        } else if (lead) {
+            // the other face at v lies across the trailing edge; its corner
+            // before v
+            const int g = otherFace(L, L.fe[b + ((c + 3) & 3)], f);
+            const int k = g < 0 ? -1 : cornerOf(L, g, v);
+            if (k < 0) return false;
+            pts[kCornerPoint[c][1]] = L.fv[L.fvOff[g] + ((k + 3) & 3)];
+            pts[kCornerPoint[c][2]] = pts[kCornerPoint[c][3]] = fill;
+        } else {
+            // across the leading edge; its corner after v
+            const int g = otherFace(L, L.fe[b + c], f);
+            const int k = g < 0 ? -1 : cornerOf(L, g, v);
+            if (k < 0) return false;
+            pts[kCornerPoint[c][1]] = pts[kCornerPoint[c][2]] = fill;
+            pts[kCornerPoint[c][3]] = L.fv[L.fvOff[g] + ((k + 1) & 3)];
+        }
     }
     return true;
 }
@@ -260,10 +321,17 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
         const Level &L0 = refiner.level(0);
         for (int f = 0; f < L0.nf; ++f)
             if (L0.faceSize(f) != 4 && isoLevel < 1) return false;   // n-gons need a level
-        for (int e = 0; e < L0.ne; ++e)
-            if (L0.efOff[e + 1] - L0.efOff[e] != 2 || L0.esharp[e] >= kSharpInfinite) return false;
+        // boundaries need interpolated boundary edges; interior edges and
+        // vertices must not be infinitely sharp (only boundary corners are)
+        bool boundary = false;
+        for (int e = 0; e < L0.ne; ++e) {
+            const int nf = L0.efOff[e + 1] - L0.efOff[e];
+            if (nf == 1) boundary = true;
+            else if (nf != 2 || L0.esharp[e] >= kSharpInfinite) return false;
+        }
+        if (boundary && refiner.boundaryInterpolation == 0) return false;
         for (int v = 0; v < L0.nv; ++v)
-            if (L0.vsharp[v] >= kSharpInfinite) return false;
+            if (L0.vsharp[v] >= kSharpInfinite && L0.vfOff[v + 1] - L0.vfOff[v] != 1) return false;
     }
     refiner.refineUniform(isoLevel);                           // may reallocate the levels
     const int maxL = isoLevel;
@@ -276,21 +344,36 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
 
     numControl = refiner.level(0).nv;
     maxDepth = 0;
-    struct Leaf { int level, face; bool regular; };
+    struct Leaf { int level, face; bool regular; int transition; };
     std::vector<Leaf> leaves;
-    std::vector<char> complete(refiner.level(0).nf, 1), next;
+    std::vector<char> complete(refiner.level(0).nf, 1), next, selected;
     for (int d = 0; d <= maxL; ++d) {
         const Level &L = refiner.level(d);
+        const size_t first = leaves.size();
         if (d < maxL) next.assign(refiner.level(d + 1).nf, 0);
+        selected.assign(L.nf, 0);
         for (int f = 0; f < L.nf; ++f) {
             if (!complete[f]) continue;
             if (d < maxL && faceHasFeatures(L, f, d == 0)) {
+                selected[f] = 1;
                 for (int j = 0; j < L.faceSize(f); ++j) next[L.fvOff[f] + j] = 1;
             } else {
-                leaves.push_back({ d, f, !isExtraordinary(L, f) });
+                leaves.push_back({ d, f, !isExtraordinary(L, f), 0 });
                 maxDepth = std::max(maxDepth, d);
             }
         }
+        // PatchParam's transition mask (far/patchBuilder.cpp:1374-1379): the
+        // edges of a leaf that the next level splits, i.e. that a selected
+        // face shares
+        if (d < maxL)
+            for (size_t i = first; i < leaves.size(); ++i) {
+                const int f = leaves[i].face;
+                for (int k = 0; k < 4; ++k) {
+                    const int e = L.fe[L.fvOff[f] + k];
+                    for (int j = L.efOff[e]; j < L.efOff[e + 1]; ++j)
+                        if (selected[L.ef[j]]) leaves[i].transition |= 1 << k;
+                }
+            }
         if (d < maxL) complete.swap(next);
     }
 
@@ -303,7 +386,8 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
     for (const Leaf &lf : leaves) {
         if (lf.regular) {
             int pts[16];
-            if (!regularPoints(refiner.level(lf.level), lf.face, pts)) return false;
+            const Level &L = refiner.level(lf.level);
+            if (!regularPoints(L, lf.face, boundaryMask(L, lf.face), pts)) return false;
             if (lf.level > 0)
                 for (int p : pts) bufIndex[lf.level][p] = 0;
             regPts.insert(regPts.end(), pts, pts + 16);
@@ -338,11 +422,13 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
             u = o.u;
             v = o.v;
         }
-        p.field0 = (uint32_t)ptex & 0xfffffffu;
+        p.field0 = ((uint32_t)ptex & 0xfffffffu) | ((uint32_t)lf.transition << 28);
         // far/patchBuilder.cpp:1272-1380: depth = level; under an n-gon the
         // (u,v) bits count from level 1, flagged by the non-quad bit
+        const uint32_t bmask = regular ? (uint32_t)boundaryMask(refiner.level(lf.level), lf.face) : 0u;
         p.field1 = (uint32_t)lf.level | ((uint32_t)refiner.ptexNonQuad[ptex] << 4) |
-                   ((uint32_t)(regular ? 1 : 0) << 5) | ((uint32_t)v << 12) | ((uint32_t)u << 22);
+                   ((uint32_t)(regular ? 1 : 0) << 5) | (bmask << 7) | ((uint32_t)v << 12) |
+                   ((uint32_t)u << 22);
         p.sharpness = 0.0f;
         return p;
     };

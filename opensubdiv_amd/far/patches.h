@@ -1,5 +1,6 @@
-// Feature-adaptive patch tables for Catmark meshes (closed, manifold, quad
-// or n-gon base faces, semi-sharp creases), their patch-point stencils, the patch
+// Feature-adaptive patch tables for Catmark meshes (manifold, quad or n-gon
+// base faces, semi-sharp creases, boundaries with regular boundary points),
+// their patch-point stencils, the patch
 // map and limit stencil tables -- the inputs of EvalPatches and of
 // derivative EvalStencils.
 //
@@ -38,8 +39,9 @@ struct PatchHandleRec {
 
 class PatchTable {
 public:
-    // Returns false for meshes outside the compiler's scope (boundaries,
-    // infinitely sharp features, n-gon base faces with isoLevel 0).  `levels`, when given,
+    // Returns false for meshes outside the compiler's scope (irregular
+    // boundary points, interior infinitely sharp features, n-gon base faces
+    // with isoLevel 0).  `levels`, when given,
     // holds the vertex-mode level stencils of the refiner up to at least
     // isoLevel (computed here otherwise).
     bool build(Refiner &refiner, int isoLevel,

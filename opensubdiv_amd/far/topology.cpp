@@ -111,6 +111,7 @@ bool Refiner::create(const TopologyDesc &d)
             if (L.vfOff[v + 1] - L.vfOff[v] == 1) L.vsharp[v] = kSharpInfinite;
 
     creasing = d.creasing == CREASE_CHAIKIN ? CREASE_CHAIKIN : CREASE_UNIFORM;
+    boundaryInterpolation = d.boundaryInterpolation;
 
     // ptex faces (far/ptexIndices.cpp): one per quad, one per corner of an n-gon
     ptexBase.resize(L.nf);

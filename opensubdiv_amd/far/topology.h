@@ -79,6 +79,7 @@ public:
     // patches carry PatchParam's non-quad bit
     std::vector<uint8_t> ptexNonQuad;
     int creasing = CREASE_UNIFORM;
+    int boundaryInterpolation = 1;
 
 private:
     void refineOnce();

@@ -194,6 +194,8 @@ def test_patch_table_matches_far_live(n, seed, crease, iso, creasing):
     pt, _, outs = _my_patch_eval(m, iso, f, u, v, creasing=creasing)
     P = ref.Patches(ref.Topology(m, creasing=creasing), iso, double=True)
     assert [tuple(a) for a in pt.arrays] == [tuple(a) for a in P.arrays]
+    # PatchParams bit for bit: face, transition mask, depth, u/v, boundary
+    assert pt.params.tobytes() == P.params.tobytes()
     src = np.zeros((P.num_vertices_total, 3))
     src[:P.num_coarse] = m.positions
     st = P.stencils
@@ -366,7 +368,7 @@ def test_patch_table_with_ngon_base_faces_matches_far_live(name, iso):
     assert ((pt.params["field1"] >> 4) & 1).any()          # non-quad patches present
     P = ref.Patches(ref.Topology(m), iso, double=True)
     assert [tuple(a) for a in pt.arrays] == [tuple(a) for a in P.arrays]
-    np.testing.assert_array_equal(pt.params["field1"], P.params["field1"])
+    assert pt.params.tobytes() == P.params.tobytes()
     src = np.zeros((P.num_vertices_total, 3))
     src[:P.num_coarse] = m.positions
     st = P.stencils
@@ -457,3 +459,71 @@ def test_uniform_levels_match_hbr_regression_baselines(name):
         dist, nearest = cKDTree(base).query(mine)
         assert len(np.unique(nearest)) == nv, f"level {level}: not one-to-one"
         assert dist.max() < 1e-6, f"level {level}: {dist.max()}"
+
+
+def disc(n):
+    """n quads around an interior vertex of valence n; the outer vertices of
+    one quad are boundary corners, the shared ones regular boundary points."""
+    return PolyMesh([[0, 1 + 2 * i, 2 + 2 * i, 1 + 2 * ((i + 1) % n)] for i in range(n)])
+
+
+@needs_ref
+@pytest.mark.parametrize("name,iso", [("grid", 2), ("grid-creased", 3), ("disc5", 3),
+                                      ("disc3", 2)])
+def test_patch_table_with_boundaries_matches_far_live(name, iso):
+    """Open meshes whose boundary points are regular (edge-and-corner
+    interpolation): boundary B-spline patches with Far's boundary masks and
+    filled phantom points, interior end caps, transition masks.  Same arrays,
+    PatchParams, B-spline point indices and patch map as Far, and the same
+    limit surface.  (Far shares Gregory end-cap points between neighbouring
+    caps, so those indices differ; their values do not.)"""
+    m = {"grid": lambda: open_grid(4),
+         "grid-creased": lambda: open_grid(5, creases=[(7, 8, 2.0), (8, 9, 0.6)]),
+         "disc5": lambda: disc(5), "disc3": lambda: disc(3)}[name]()
+    nptex = m.nfaces
+    t = far.Topology(m, boundary=2)
+    pt = t.patch_table(iso)
+    P = ref.Patches(ref.Topology(m, boundary=2), iso, double=True)
+    assert [tuple(a) for a in pt.arrays] == [tuple(a) for a in P.arrays]
+    assert pt.params.tobytes() == P.params.tobytes()
+    assert (((pt.params["field1"] >> 7) & 0xF) != 0).any()          # boundary patches present
+    nreg = int(pt.arrays[0]["numPatches"]) * 16 if pt.arrays[0]["desc"] == 6 else 0
+    np.testing.assert_array_equal(pt.indices[:nreg], P.indices[:nreg])
+    rng = np.random.default_rng(iso)
+    f = rng.integers(0, nptex, 3000).astype(np.int32)
+    u, v = rng.random(3000), rng.random(3000)
+    u[:4], v[:4] = [0.0, 1.0, 1.0, 0.0], [0.0, 0.0, 1.0, 1.0]
+    ctrl = m.positions.astype(np.float64)
+    st = pt.point_stencils(double=True)
+    buf = np.ascontiguousarray(np.concatenate([ctrl, oracle.apply_stencils(
+        ctrl, st["sizes"], st["offsets"], st["indices"], st["weights"], np.float64)]))
+    mine = pt.find(f, u, v)
+    h = P.find(f, u, v)
+    np.testing.assert_array_equal(mine["patchIndex"], h[:, 1])
+    outs = [np.zeros((len(f), 3)) for _ in range(3)]
+    assert oracle.eval_patches_f64(buf, (0, 3, 3), outs, [(0, 3, 3)] * 3, mine, pt.arrays,
+                                   pt.indices, pt.params)
+    src = np.zeros((P.num_vertices_total, 3))
+    src[:P.num_coarse] = ctrl
+    S = P.stencils
+    src[P.num_coarse:] = oracle.apply_stencils(ctrl, S["sizes"], S["offsets"], S["indices"],
+                                               S["weights"], np.float64)
+    theirs = np.zeros(len(f), PATCH_COORD_DTYPE)
+    theirs["arrayIndex"], theirs["patchIndex"], theirs["vertIndex"] = h[:, 0], h[:, 1], h[:, 2]
+    theirs["s"], theirs["t"] = u, v
+    o = [np.zeros((len(f), 3)) for _ in range(3)]
+    assert oracle.eval_patches_f64(src, (0, 3, 3), o, [(0, 3, 3)] * 3, theirs, P.arrays,
+                                   P.indices, P.params)
+    for a, b in zip(outs, o):
+        assert bbox_rel_err(a, b) < 1e-12
+
+
+def test_patch_table_rejects_irregular_boundaries():
+    """Boundary points that need end caps of their own (a smooth boundary
+    corner under edge-only interpolation, a boundary point shared by three
+    faces) are outside the compiler's scope: no table rather than a wrong one."""
+    with pytest.raises(Exception):
+        far.Topology(open_grid(4), boundary=1).patch_table(2)
+    fan = PolyMesh([[0, 1, 2, 3], [0, 3, 4, 5], [0, 5, 6, 7]])      # centre on the boundary
+    with pytest.raises(Exception):
+        far.Topology(fan, boundary=2).patch_table(2)
