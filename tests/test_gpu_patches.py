@@ -440,3 +440,33 @@ def test_device_tessellation_matches_bfr_and_feeds_eval_patches(hip):
                                          stream()) == _lib.ERR_ARGS
     assert hip.osdhip_tess_uniform_quads(0, None, 4, None, None, None, stream()) == 0
     pm.close()
+
+
+@pytest.mark.parametrize("mesh", ["grid", "disc5"])
+def test_boundary_patch_tables_bit_exact(hip, mesh):
+    """Adaptive tables of open meshes (boundary B-spline patches with Far's
+    boundary masks and filled phantom points, interior end caps): P, dPdu,
+    dPdv at random and corner locations equal the oracle's bit for bit."""
+    from opensubdiv_amd import far
+    from test_far_compiler import disc, open_grid
+    m = open_grid(5, creases=[(7, 8, 2.0)]) if mesh == "grid" else disc(5)
+    pt = far.Topology(m, boundary=2).patch_table(3)
+    rng = np.random.default_rng(3)
+    n = 4000
+    f = rng.integers(0, m.nfaces, n).astype(np.int32)
+    u, v = rng.random(n), rng.random(n)
+    u[:4], v[:4] = [0.0, 1.0, 1.0, 0.0], [0.0, 0.0, 1.0, 1.0]
+    coords = pt.find(f, u, v)
+    assert (coords["arrayIndex"] >= 0).all()
+    st = pt.point_stencils()
+    ctrl = m.positions.astype(np.float32)
+    pts = oracle.apply_stencils(ctrl, st["sizes"], st["offsets"], st["indices"], st["weights"])
+    src = np.ascontiguousarray(np.concatenate([ctrl, pts]))
+    want = [np.zeros((n, 3), np.float32) for _ in range(3)]
+    assert oracle.eval_patches(src, (0, 3, 3), want, [(0, 3, 3)] * 3, coords, pt.arrays,
+                               pt.indices, pt.params)
+    outs = [empty(n * 3) for _ in range(3)]
+    assert eval_patches(hip, dev(src), (0, 3, 3), outs, [(0, 3, 3)] * 3, n, dev(coords),
+                        dev(pt.arrays), dev(pt.indices), dev(pt.params)) == 0
+    for k in range(3):
+        np.testing.assert_array_equal(host(outs[k], shape=(n, 3)), want[k])
