@@ -97,8 +97,9 @@ void osdfar_stencil_table_destroy(osdfar_stencil_table *t);
  * uniform or Chaikin creasing, and boundaries whose points are regular:
  * shared by two faces, or single-face corners under edge-and-corner
  * interpolation (boundary patches carry Far's boundary mask and fill their
- * missing points with the face's first vertex).  Returns NULL otherwise
- * (irregular boundary points, interior infinitely sharp features).
+ * missing points with the face's first vertex), and interior infinitely
+ * sharp creases and corners (isolated to isoLevel, as Far does with
+ * inf-sharp patches off).  Returns NULL for irregular boundary points.
  * PatchParams carry Far's transition masks.  Refines the topology uniformly to isoLevel.
  * The patch points live in a vertex buffer laid out as
  *   [control vertices | refined vertices | end-cap points]

@@ -235,11 +235,15 @@ bool faceHasFeatures(const Level &L, int f, bool baseLevel)
             for (int i = L.vfOff[v]; i < L.vfOff[v + 1]; ++i)
                 if (L.faceSize(L.vf[i]) != 4) return true;
         if (!regularCorner(L, v)) return true;
-        // semi-sharp vertex or edge; the infinitely sharp boundary is regular
-        if (L.vsharp[v] > 0.0f && L.vsharp[v] < kSharpInfinite) return true;
+        // semi-sharp or interior infinitely sharp vertex or edge (Far selects
+        // inf-sharp creases and corners when inf-sharp patches are off,
+        // far/topologyRefiner.cpp:340-360); the infinitely sharp boundary
+        // and its corners are regular
+        const bool corner1 = L.vfOff[v + 1] - L.vfOff[v] == 1;
+        if (L.vsharp[v] > 0.0f && !(corner1 && L.vsharp[v] >= kSharpInfinite)) return true;
         for (int i = L.veOff[v]; i < L.veOff[v + 1]; ++i) {
-            const float s = L.esharp[L.ve[i]];
-            if (s > 0.0f && s < kSharpInfinite) return true;
+            const int e = L.ve[i];
+            if (L.esharp[e] > 0.0f && !isBoundaryEdge(L, e)) return true;
         }
     }
     return false;
@@ -327,11 +331,9 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
         for (int e = 0; e < L0.ne; ++e) {
             const int nf = L0.efOff[e + 1] - L0.efOff[e];
             if (nf == 1) boundary = true;
-            else if (nf != 2 || L0.esharp[e] >= kSharpInfinite) return false;
+            else if (nf != 2) return false;
         }
         if (boundary && refiner.boundaryInterpolation == 0) return false;
-        for (int v = 0; v < L0.nv; ++v)
-            if (L0.vsharp[v] >= kSharpInfinite && L0.vfOff[v + 1] - L0.vfOff[v] != 1) return false;
     }
     refiner.refineUniform(isoLevel);                           // may reallocate the levels
     const int maxL = isoLevel;

@@ -527,3 +527,54 @@ def test_patch_table_rejects_irregular_boundaries():
     fan = PolyMesh([[0, 1, 2, 3], [0, 3, 4, 5], [0, 5, 6, 7]])      # centre on the boundary
     with pytest.raises(Exception):
         far.Topology(fan, boundary=2).patch_table(2)
+
+
+def sphere_with_inf_crease():
+    """A cube-sphere with an infinitely sharp crease along two edges."""
+    m = meshes.cube_sphere(4, seed=3, crease_fraction=0.0)
+    q = m.face_verts
+    pm = PolyMesh(q.tolist(), creases=[(int(q[k][0]), int(q[k][1]), 10.0) for k in (2, 3)])
+    pm.positions = m.positions
+    return pm
+
+
+@needs_ref
+@pytest.mark.parametrize("name,iso", [("sharp-cube", 2), ("sharp-cube", 3),
+                                      ("sphere-inf-crease", 3)])
+def test_patch_table_with_inf_sharp_features_matches_far_live(name, iso):
+    """Interior infinitely sharp creases and corners (inf-sharp patches off,
+    Far's default): faces along them are isolated to the iso level and then
+    patched as Far does.  Same arrays, PatchParams, patch map and limit
+    surface.  (Far's sparse levels keep some refined vertices no patch
+    uses, so its point numbering can differ from ours.)"""
+    m = sharp_cube() if name == "sharp-cube" else sphere_with_inf_crease()
+    pt = far.Topology(m).patch_table(iso)
+    P = ref.Patches(ref.Topology(m), iso, double=True)
+    assert [tuple(a) for a in pt.arrays] == [tuple(a) for a in P.arrays]
+    assert pt.params.tobytes() == P.params.tobytes()
+    rng = np.random.default_rng(iso)
+    f = rng.integers(0, m.nfaces, 3000).astype(np.int32)
+    u, v = rng.random(3000), rng.random(3000)
+    ctrl = m.positions.astype(np.float64)
+    st = pt.point_stencils(double=True)
+    buf = np.ascontiguousarray(np.concatenate([ctrl, oracle.apply_stencils(
+        ctrl, st["sizes"], st["offsets"], st["indices"], st["weights"], np.float64)]))
+    mine = pt.find(f, u, v)
+    h = P.find(f, u, v)
+    np.testing.assert_array_equal(mine["patchIndex"], h[:, 1])
+    outs = [np.zeros((len(f), 3)) for _ in range(3)]
+    assert oracle.eval_patches_f64(buf, (0, 3, 3), outs, [(0, 3, 3)] * 3, mine, pt.arrays,
+                                   pt.indices, pt.params)
+    src = np.zeros((P.num_vertices_total, 3))
+    src[:P.num_coarse] = ctrl
+    S = P.stencils
+    src[P.num_coarse:] = oracle.apply_stencils(ctrl, S["sizes"], S["offsets"], S["indices"],
+                                               S["weights"], np.float64)
+    theirs = np.zeros(len(f), PATCH_COORD_DTYPE)
+    theirs["arrayIndex"], theirs["patchIndex"], theirs["vertIndex"] = h[:, 0], h[:, 1], h[:, 2]
+    theirs["s"], theirs["t"] = u, v
+    o = [np.zeros((len(f), 3)) for _ in range(3)]
+    assert oracle.eval_patches_f64(src, (0, 3, 3), o, [(0, 3, 3)] * 3, theirs, P.arrays,
+                                   P.indices, P.params)
+    for a, b in zip(outs, o):
+        assert bbox_rel_err(a, b) < 1e-12
