@@ -154,7 +154,9 @@ def compile_mesh(spec: MeshSpec, length=7, threads=1, cache=None) -> HostMesh:
                              tf.array("patch.params", PATCH_PARAM_DTYPE),
                              tf.array("coords", PATCH_COORD_DTYPE), frames, tf)
             except (KeyError, ValueError):
-                tf.close()
+                tf.close()                  # a file of another layout: a miss
+                cache.hits -= 1
+                cache.misses += 1
             else:
                 return h
     topo = far.Topology(mesh)
