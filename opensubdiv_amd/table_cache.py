@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import threading
 
 import numpy as np
 
@@ -71,6 +72,18 @@ class TableCache:
         self.verify = bool(verify)
         os.makedirs(self.directory, exist_ok=True)
         self.hits = self.misses = 0
+        self._lock = threading.Lock()         # loads run on several host threads
+
+    def count(self, hit, undo_hit=False):
+        """Records a hit or a miss (undo_hit: a loaded file turned out
+        unusable, so its hit becomes a miss)."""
+        with self._lock:
+            if undo_hit:
+                self.hits -= 1
+            if hit:
+                self.hits += 1
+            else:
+                self.misses += 1
 
     @staticmethod
     def key(mesh, recipe, boundary=1):
@@ -81,10 +94,7 @@ class TableCache:
 
     def load(self, key):
         tf = far.TableFile.open(self.path(key), key, self.verify)
-        if tf is None:
-            self.misses += 1
-        else:
-            self.hits += 1
+        self.count(tf is not None)
         return tf
 
     def store(self, key, arrays):

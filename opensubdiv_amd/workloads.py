@@ -155,8 +155,7 @@ def compile_mesh(spec: MeshSpec, length=7, threads=1, cache=None) -> HostMesh:
                              tf.array("coords", PATCH_COORD_DTYPE), frames, tf)
             except (KeyError, ValueError):
                 tf.close()                  # a file of another layout: a miss
-                cache.hits -= 1
-                cache.misses += 1
+                cache.count(False, undo_hit=True)
             else:
                 return h
     topo = far.Topology(mesh)
