@@ -249,10 +249,17 @@ bool faceHasFeatures(const Level &L, int f, bool baseLevel)
     return false;
 }
 
+// Whether a leaf face needs an end cap.  A smooth single-face boundary
+// corner is isolated like an irregular point but then patched as a sharp
+// corner, as Far does by default (generateLegacySharpCornerPatches,
+// far/patchBuilder.cpp:676-686).
 bool isExtraordinary(const Level &L, int f)
 {
-    for (int k = L.fvOff[f]; k < L.fvOff[f + 1]; ++k)
-        if (!regularCorner(L, L.fv[k])) return true;
+    for (int k = L.fvOff[f]; k < L.fvOff[f + 1]; ++k) {
+        const int v = L.fv[k];
+        if (L.vfOff[v + 1] - L.vfOff[v] == 1 && L.valence(v) == 2) continue;
+        if (!regularCorner(L, v)) return true;
+    }
     return false;
 }
 

@@ -468,12 +468,14 @@ def disc(n):
 
 
 @needs_ref
+@pytest.mark.parametrize("boundary", [1, 2])
 @pytest.mark.parametrize("name,iso", [("grid", 2), ("grid-creased", 3), ("disc5", 3),
                                       ("disc3", 2)])
-def test_patch_table_with_boundaries_matches_far_live(name, iso):
-    """Open meshes whose boundary points are regular (edge-and-corner
-    interpolation): boundary B-spline patches with Far's boundary masks and
-    filled phantom points, interior end caps, transition masks.  Same arrays,
+def test_patch_table_with_boundaries_matches_far_live(name, iso, boundary):
+    """Open meshes under edge-only and edge-and-corner interpolation:
+    boundary B-spline patches with Far's boundary masks and filled phantom
+    points (smooth single-face corners isolated, then patched as sharp
+    corners, Far's default), interior end caps, transition masks.  Same arrays,
     PatchParams, B-spline point indices and patch map as Far, and the same
     limit surface.  (Far shares Gregory end-cap points between neighbouring
     caps, so those indices differ; their values do not.)"""
@@ -481,9 +483,9 @@ def test_patch_table_with_boundaries_matches_far_live(name, iso):
          "grid-creased": lambda: open_grid(5, creases=[(7, 8, 2.0), (8, 9, 0.6)]),
          "disc5": lambda: disc(5), "disc3": lambda: disc(3)}[name]()
     nptex = m.nfaces
-    t = far.Topology(m, boundary=2)
+    t = far.Topology(m, boundary=boundary)
     pt = t.patch_table(iso)
-    P = ref.Patches(ref.Topology(m, boundary=2), iso, double=True)
+    P = ref.Patches(ref.Topology(m, boundary=boundary), iso, double=True)
     assert [tuple(a) for a in pt.arrays] == [tuple(a) for a in P.arrays]
     assert pt.params.tobytes() == P.params.tobytes()
     assert (((pt.params["field1"] >> 7) & 0xF) != 0).any()          # boundary patches present
@@ -519,11 +521,9 @@ def test_patch_table_with_boundaries_matches_far_live(name, iso):
 
 
 def test_patch_table_rejects_irregular_boundaries():
-    """Boundary points that need end caps of their own (a smooth boundary
-    corner under edge-only interpolation, a boundary point shared by three
-    faces) are outside the compiler's scope: no table rather than a wrong one."""
-    with pytest.raises(Exception):
-        far.Topology(open_grid(4), boundary=1).patch_table(2)
+    """Boundary points that need end caps of their own (a boundary point
+    shared by three faces) are outside the compiler's scope: no table rather
+    than a wrong one."""
     fan = PolyMesh([[0, 1, 2, 3], [0, 3, 4, 5], [0, 5, 6, 7]])      # centre on the boundary
     with pytest.raises(Exception):
         far.Topology(fan, boundary=2).patch_table(2)
