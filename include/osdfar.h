@@ -94,13 +94,13 @@ void osdfar_stencil_table_destroy(osdfar_stencil_table *t);
  * ENDCAP_GREGORY_BASIS (far/patchTableFactory.h:40-160) for manifold
  * Catmark meshes with semi-sharp creases, quad or n-gon base faces (n-gons
  * need isoLevel >= 1; their patches carry PatchParam's non-quad bit),
- * uniform or Chaikin creasing, and boundaries whose points are shared by at
- * most two faces (boundary patches carry Far's boundary mask and fill their
- * missing points with the face's first vertex; a smooth single-face corner
- * is isolated, then patched as a sharp corner, Far's default), and interior infinitely
+ * uniform or Chaikin creasing, and boundaries (boundary patches carry Far's
+ * boundary mask and fill their missing points with the face's first vertex;
+ * a smooth single-face corner is isolated, then patched as a sharp corner,
+ * Far's default; irregular boundary points get end caps with the boundary
+ * limit masks), and interior infinitely
  * sharp creases and corners (isolated to isoLevel, as Far does with
- * inf-sharp patches off).  Returns NULL for a boundary point shared by
- * three or more faces (it would need a boundary end cap).
+ * inf-sharp patches off).  Returns NULL for non-manifold input.
  * PatchParams carry Far's transition masks.  Refines the topology uniformly to isoLevel.
  * The patch points live in a vertex buffer laid out as
  *   [control vertices | refined vertices | end-cap points]

@@ -121,86 +121,262 @@ void interiorPointWeights(int n, std::vector<double> &p, std::vector<double> &ep
     }
 }
 
+bool isBoundaryEdge(const Level &L, int e) { return L.efOff[e + 1] - L.efOff[e] == 1; }
+
+// Corner k (0..3) of face g at vertex v.
+int cornerOf(const Level &L, int g, int v)
+{
+    for (int k = 0; k < 4; ++k)
+        if (L.fv[L.fvOff[g] + k] == v) return k;
+    return -1;
+}
+
+// Limit point and edge-tangent points of a smooth boundary corner with
+// valence (edges) n >= 2 whose patch face is face `faceInRing` of its ring
+// (the boundary limit masks of far/catmarkPatchBuilder.cpp:201-309):
+// weights[0] for the corner, weights[i] for ring point i - 1 (edges 0..n-1
+// at the even positions, faces between them at the odd ones).
+void boundaryPointWeights(int n, int faceInRing, std::vector<double> &p, std::vector<double> &ep,
+                          std::vector<double> &em)
+{
+    const int W = 2 * n, N = W - 1, k = n - 1;
+    const double theta = M_PI / k;
+    p.assign(W, 0.0);
+    ep.assign(W, 0.0);
+    em.assign(W, 0.0);
+    p[0] = 4.0 / 6.0;
+    p[1] = 1.0 / 6.0;
+    p[N] = 1.0 / 6.0;
+    // tangent across the boundary (interior direction)
+    std::vector<double> t(W, 0.0);
+    const double c = std::cos(theta), sn = std::sin(theta);
+    const double d3kc = 1.0 / (3.0 * k + c);
+    const double gamma = -4.0 * sn * d3kc;
+    const double a0 = -((1.0 + 2.0 * c) * std::sqrt(1.0 + c)) * d3kc / std::sqrt(1.0 - c);
+    const double b0 = sn * d3kc;
+    t[0] = gamma / 3.0;
+    t[1] = a0 / 3.0;
+    t[2] = b0 / 3.0;
+    t[N] = a0 / 3.0;
+    for (int i = 1; i < n - 1; ++i) {
+        const double si = std::sin(theta * i), si1 = std::sin(theta * (i + 1));
+        t[1 + 2 * i] = 4.0 * si * d3kc / 3.0;
+        t[2 + 2 * i] = (si + si1) * d3kc / 3.0;
+    }
+    // each edge point mixes the tangent along the boundary with the one
+    // across it by the angle of its edge; on a boundary edge it lies on it
+    auto edgePoint = [&](std::vector<double> &e, bool onBoundary, int lead, int angleIndex) {
+        if (onBoundary) {
+            e[0] = 2.0 / 3.0;
+            e[lead] = 1.0 / 3.0;
+            return;
+        }
+        const double ca = std::cos(theta * angleIndex), sa = std::sin(theta * angleIndex);
+        for (int i = 0; i < W; ++i) e[i] = t[i] * sa;
+        e[0] += p[0];
+        e[1] += p[1] + ca / 6.0;
+        e[N] += p[N] - ca / 6.0;
+    };
+    edgePoint(ep, faceInRing == 0, 1, faceInRing);
+    edgePoint(em, faceInRing == k - 1, N, (faceInRing + 1) % n);
+}
+
 // The 20 Gregory points of an end-cap face (Loop, Schaefer, Ni, Castano,
 // "Approximating subdivision surfaces with Gregory patches for hardware
-// tessellation", as converted in far/catmarkPatchBuilder.cpp:1003-1416) for
-// interior corners: rows over level vertices, point 5c+{0,1,2,3,4} =
-// P, E+, E-, F+, F- of corner c.
+// tessellation", as converted in far/catmarkPatchBuilder.cpp:727-1416):
+// rows over level vertices, point 5c+{0,1,2,3,4} = P, E+, E-, F+, F- of
+// corner c.  Corners are interior (regular or not), on a boundary (two
+// faces: regular; more: the boundary limit masks), a single-face smooth
+// corner, or sharp.
 bool gregoryPoints(const Level &L, int f, Row out[20])
 {
-    int cv[4];
-    std::vector<int> ring[4];
-    int val[4];
-    bool reg[4];
-    double cs[4];
+    struct Corner {
+        int v, numFaces, valence, faceInRing;
+        bool boundary, sharp, regular, epOnB, emOnB, fpReg, fmReg, fpCopy, fmCopy;
+        double cosA;
+        std::vector<int> ring;
+    } C[4];
+    const int b = L.fvOff[f];
     for (int c = 0; c < 4; ++c) {
-        cv[c] = L.fv[L.fvOff[f] + c];
-        val[c] = orderedRing(L, cv[c], f, ring[c]);
-        if (val[c] < 3) return false;                 // boundary / valence 2: out of scope
-        reg[c] = val[c] == 4;
-        cs[c] = reg[c] ? 0.0 : std::cos(2.0 * M_PI / val[c]);
+        Corner &K = C[c];
+        K.v = L.fv[b + c];
+        int nb = 0;
+        for (int i = L.veOff[K.v]; i < L.veOff[K.v + 1]; ++i) nb += isBoundaryEdge(L, L.ve[i]);
+        K.boundary = nb > 0;
+        if (!K.boundary) {
+            K.numFaces = orderedRing(L, K.v, f, K.ring);
+            if (K.numFaces < 3) return false;
+            K.valence = K.numFaces;
+            K.faceInRing = 0;
+            K.sharp = false;
+        } else {
+            if (nb != 2) return false;
+            // ring from the face whose edge (v, next) is on the boundary to the
+            // face whose edge (v, prev) is
+            int s = f;
+            for (int guard = 0; guard < 256; ++guard) {
+                const int j = cornerOf(L, s, K.v);
+                const int eNext = L.fe[L.fvOff[s] + j];
+                if (isBoundaryEdge(L, eNext)) break;
+                s = otherFace(L, eNext, s);
+                if (s < 0 || L.faceSize(s) != 4) return false;
+            }
+            K.ring.clear();
+            K.faceInRing = -1;
+            int g = s, k = 0;
+            for (;; ++k) {
+                if (L.faceSize(g) != 4 || k > 255) return false;
+                const int j = cornerOf(L, g, K.v), gb = L.fvOff[g];
+                if (g == f) K.faceInRing = k;
+                K.ring.push_back(L.fv[gb + ((j + 1) & 3)]);
+                K.ring.push_back(L.fv[gb + ((j + 2) & 3)]);
+                const int ePrev = L.fe[gb + ((j + 3) & 3)];
+                if (isBoundaryEdge(L, ePrev)) {
+                    K.ring.push_back(L.fv[gb + ((j + 3) & 3)]);
+                    break;
+                }
+                g = otherFace(L, ePrev, g);
+                if (g < 0) return false;
+            }
+            K.numFaces = k + 1;
+            K.valence = K.numFaces + 1;
+            if (K.faceInRing < 0) return false;
+            K.sharp = K.numFaces == 1 && L.vsharp[K.v] >= kSharpInfinite;
+        }
+        K.regular = ((K.numFaces << (K.boundary ? 1 : 0)) == 4) && !K.sharp;
+        K.cosA = K.regular ? 0.0
+                           : std::cos((K.boundary ? M_PI : 2.0 * M_PI) / (double)K.numFaces);
     }
     for (int c = 0; c < 4; ++c) {
+        Corner &K = C[c];
+        K.epOnB = K.emOnB = K.fpCopy = K.fmCopy = false;
+        K.fpReg = K.regular && C[(c + 1) & 3].regular;
+        K.fmReg = K.regular && C[(c + 3) & 3].regular;
+        if (K.boundary) {
+            K.epOnB = K.faceInRing == 0;
+            K.emOnB = K.faceInRing == K.numFaces - 1;
+            if (K.numFaces > 1) {
+                if (K.epOnB) { K.fpReg = K.fmReg; K.fpCopy = !K.fpReg; }
+                if (K.emOnB) { K.fmReg = K.fpReg; K.fmCopy = !K.fmReg; }
+            } else {
+                K.fpReg = K.fmReg = true;
+            }
+        }
+    }
+    // corner and edge points
+    for (int c = 0; c < 4; ++c) {
+        const Corner &K = C[c];
+        const int cv = K.v, vn = C[(c + 1) & 3].v, vp = C[(c + 3) & 3].v;
         Row &P = out[5 * c], &Ep = out[5 * c + 1], &Em = out[5 * c + 2];
-        const std::vector<int> &r = ring[c];
-        if (reg[c]) {
-            P.add(cv[c], 4.0 / 9.0);
+        const std::vector<int> &r = K.ring;
+        if (K.sharp) {
+            P.add(cv, 1.0);
+            Ep.add(cv, 2.0 / 3.0);
+            Ep.add(vn, 1.0 / 3.0);
+            Em.add(cv, 2.0 / 3.0);
+            Em.add(vp, 1.0 / 3.0);
+        } else if (!K.boundary && K.regular) {
+            P.add(cv, 4.0 / 9.0);
             for (int k = 0; k < 4; ++k) {
                 P.add(r[2 * k], 1.0 / 9.0);
                 P.add(r[2 * k + 1], 1.0 / 36.0);
             }
-            Ep.add(cv[c], 4.0 / 9.0);
+            Ep.add(cv, 4.0 / 9.0);
             Ep.add(r[0], 2.0 / 9.0);
             Ep.add(r[2], 1.0 / 9.0);
             Ep.add(r[6], 1.0 / 9.0);
             Ep.add(r[1], 1.0 / 18.0);
             Ep.add(r[7], 1.0 / 18.0);
-            Em.add(cv[c], 4.0 / 9.0);
+            Em.add(cv, 4.0 / 9.0);
             Em.add(r[2], 2.0 / 9.0);
             Em.add(r[0], 1.0 / 9.0);
             Em.add(r[4], 1.0 / 9.0);
             Em.add(r[1], 1.0 / 18.0);
             Em.add(r[3], 1.0 / 18.0);
-        } else {
+        } else if (!K.boundary) {
             std::vector<double> pw, epw, emw;
-            interiorPointWeights(val[c], pw, epw, emw);
-            P.add(cv[c], pw[0]);
-            Ep.add(cv[c], epw[0]);
-            Em.add(cv[c], emw[0]);
+            interiorPointWeights(K.valence, pw, epw, emw);
+            P.add(cv, pw[0]);
+            Ep.add(cv, epw[0]);
+            Em.add(cv, emw[0]);
             for (int i = 1; i < (int)pw.size(); ++i) {
                 P.add(r[i - 1], pw[i]);
                 Ep.add(r[i - 1], epw[i]);
                 Em.add(r[i - 1], emw[i]);
             }
+        } else if (K.regular) {
+            // two faces: the boundary crease limit and its tangents
+            P.add(cv, 2.0 / 3.0);
+            P.add(r[0], 1.0 / 6.0);
+            P.add(r[4], 1.0 / 6.0);
+            Row &eB = K.epOnB ? Ep : Em, &eI = K.epOnB ? Em : Ep;
+            eB.add(cv, 2.0 / 3.0);
+            eB.add(r[K.epOnB ? 0 : 4], 1.0 / 3.0);
+            eI.add(cv, 4.0 / 9.0);
+            eI.add(r[2], 2.0 / 9.0);
+            eI.add(r[0], 1.0 / 9.0);
+            eI.add(r[4], 1.0 / 9.0);
+            eI.add(r[1], 1.0 / 18.0);
+            eI.add(r[3], 1.0 / 18.0);
+        } else if (K.numFaces > 1) {
+            std::vector<double> pw, epw, emw;
+            boundaryPointWeights(K.valence, K.faceInRing, pw, epw, emw);
+            const int N = (int)pw.size() - 1;
+            P.add(cv, pw[0]);
+            P.add(r[0], pw[1]);
+            P.add(r[2 * (K.valence - 1)], pw[N]);
+            Ep.add(cv, epw[0]);
+            Em.add(cv, emw[0]);
+            if (K.epOnB) Ep.add(r[0], epw[1]);
+            else for (int i = 1; i <= N; ++i) Ep.add(r[i - 1], epw[i]);
+            if (K.emOnB) Em.add(r[2 * (K.valence - 1)], emw[N]);
+            else for (int i = 1; i <= N; ++i) Em.add(r[i - 1], emw[i]);
+        } else {
+            // a smooth single-face corner
+            P.add(cv, 4.0 / 6.0);
+            P.add(vn, 1.0 / 6.0);
+            P.add(vp, 1.0 / 6.0);
+            Ep.add(cv, 2.0 / 3.0);
+            Ep.add(vn, 1.0 / 3.0);
+            Em.add(cv, 2.0 / 3.0);
+            Em.add(vp, 1.0 / 3.0);
         }
     }
+    // face points
     for (int c = 0; c < 4; ++c) {
+        const Corner &K = C[c];
         const int cn = (c + 1) & 3, co = (c + 2) & 3, cp = (c + 3) & 3;
-        const std::vector<int> &r = ring[c];
-        const int n = val[c];
+        const std::vector<int> &r = K.ring;
+        const int n = K.valence;
         for (int side = 0; side < 2; ++side) {
             Row &F = out[5 * c + 3 + side];
-            const int far = side == 0 ? cn : cp;
-            if (reg[c] && reg[far]) {
-                F.add(cv[c], 4.0 / 9.0);
-                F.add(cv[cp], 2.0 / 9.0);
-                F.add(cv[cn], 2.0 / 9.0);
-                F.add(cv[co], 1.0 / 9.0);
+            const bool reg = side == 0 ? K.fpReg : K.fmReg;
+            const bool copied = side == 0 ? K.fpCopy : K.fmCopy;
+            if (reg) {
+                F.add(K.v, 4.0 / 9.0);
+                F.add(C[cp].v, 2.0 / 9.0);
+                F.add(C[cn].v, 2.0 / 9.0);
+                F.add(C[co].v, 1.0 / 9.0);
                 continue;
             }
-            const double cN = cs[c], cF = cs[far];
+            if (copied) continue;
+            const int far = side == 0 ? cn : cp;
+            const double cN = K.cosA, cF = C[far].cosA;
             const Row &eNear = out[5 * c + 1 + side];                 // E+ or E- of c
             const Row &eFar = out[5 * far + (side == 0 ? 2 : 1)];    // E- of next / E+ of prev
             F.addRow(out[5 * c], cF / 3.0);
             F.addRow(eNear, (3.0 - 2.0 * cN - cF) / 3.0);
             F.addRow(eFar, 2.0 * cN / 3.0);
             const double sgn = side == 0 ? 1.0 : -1.0;
-            const int eInt = side, ePrev = (eInt + n - 1) % n, eNext = (eInt + 1) % n;
+            const int eInt = side == 0 ? K.faceInRing : (K.faceInRing + 1) % n;
+            const int ePrev = (eInt + n - 1) % n, eNext = (eInt + 1) % n;
             F.add(r[2 * ePrev], -sgn / 9.0);
             F.add(r[2 * ePrev + 1], -sgn / 18.0);
             F.add(r[2 * eInt + 1], sgn / 18.0);
             F.add(r[2 * eNext], sgn / 9.0);
         }
+        if (K.fpCopy) out[5 * c + 3] = out[5 * c + 4];
+        if (K.fmCopy) out[5 * c + 4] = out[5 * c + 3];
     }
     return true;
 }
@@ -209,7 +385,6 @@ bool gregoryPoints(const Level &L, int f, Row out[20])
 // interior; far/topologyRefiner.cpp:627-690): non-quad, a base-level corner
 // shared with a non-quad face, an extraordinary corner, or a corner with
 // vertex sharpness or an incident sharp edge.
-bool isBoundaryEdge(const Level &L, int e) { return L.efOff[e + 1] - L.efOff[e] == 1; }
 
 // A corner a B-spline patch reproduces (Far's regular tags, far/patchBuilder.cpp:
 // 581-700 for this scope): interior with 4 faces; on a boundary with 2 faces
@@ -277,13 +452,6 @@ int boundaryMask(const Level &L, int f)
     return m;
 }
 
-// Corner k (0..3) of face g at vertex v.
-int cornerOf(const Level &L, int g, int v)
-{
-    for (int k = 0; k < 4; ++k)
-        if (L.fv[L.fvOff[g] + k] == v) return k;
-    return -1;
-}
 
 bool regularPoints(const Level &L, int f, int mask, int pts[16])
 {

@@ -39,8 +39,8 @@ struct PatchHandleRec {
 
 class PatchTable {
 public:
-    // Returns false for meshes outside the compiler's scope (irregular
-    // boundary points, n-gon base faces with isoLevel 0).  `levels`, when given,
+    // Returns false for meshes outside the compiler's scope (non-manifold
+    // vertices, n-gon base faces with isoLevel 0).  `levels`, when given,
     // holds the vertex-mode level stencils of the refiner up to at least
     // isoLevel (computed here otherwise).
     bool build(Refiner &refiner, int isoLevel,

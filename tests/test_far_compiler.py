@@ -470,18 +470,23 @@ def disc(n):
 @needs_ref
 @pytest.mark.parametrize("boundary", [1, 2])
 @pytest.mark.parametrize("name,iso", [("grid", 2), ("grid-creased", 3), ("disc5", 3),
-                                      ("disc3", 2)])
+                                      ("disc3", 2), ("fan3", 3), ("fan4", 2)])
 def test_patch_table_with_boundaries_matches_far_live(name, iso, boundary):
     """Open meshes under edge-only and edge-and-corner interpolation:
     boundary B-spline patches with Far's boundary masks and filled phantom
     points (smooth single-face corners isolated, then patched as sharp
-    corners, Far's default), interior end caps, transition masks.  Same arrays,
+    corners, Far's default), end caps at interior and at irregular boundary
+    points, transition masks.  Same arrays,
     PatchParams, B-spline point indices and patch map as Far, and the same
     limit surface.  (Far shares Gregory end-cap points between neighbouring
     caps, so those indices differ; their values do not.)"""
     m = {"grid": lambda: open_grid(4),
          "grid-creased": lambda: open_grid(5, creases=[(7, 8, 2.0), (8, 9, 0.6)]),
-         "disc5": lambda: disc(5), "disc3": lambda: disc(3)}[name]()
+         "disc5": lambda: disc(5), "disc3": lambda: disc(3),
+         # an irregular boundary point (3 or 4 faces): end caps with the
+         # boundary limit masks
+         "fan3": lambda: PolyMesh([[0, 1, 2, 3], [0, 3, 4, 5], [0, 5, 6, 7]]),
+         "fan4": lambda: PolyMesh([[0, 1, 2, 3], [0, 3, 4, 5], [0, 5, 6, 7], [0, 7, 8, 9]])}[name]()
     nptex = m.nfaces
     t = far.Topology(m, boundary=boundary)
     pt = t.patch_table(iso)
@@ -518,15 +523,6 @@ def test_patch_table_with_boundaries_matches_far_live(name, iso, boundary):
                                    P.indices, P.params)
     for a, b in zip(outs, o):
         assert bbox_rel_err(a, b) < 1e-12
-
-
-def test_patch_table_rejects_irregular_boundaries():
-    """Boundary points that need end caps of their own (a boundary point
-    shared by three faces) are outside the compiler's scope: no table rather
-    than a wrong one."""
-    fan = PolyMesh([[0, 1, 2, 3], [0, 3, 4, 5], [0, 5, 6, 7]])      # centre on the boundary
-    with pytest.raises(Exception):
-        far.Topology(fan, boundary=2).patch_table(2)
 
 
 def sphere_with_inf_crease():

@@ -442,14 +442,15 @@ def test_device_tessellation_matches_bfr_and_feeds_eval_patches(hip):
     pm.close()
 
 
-@pytest.mark.parametrize("mesh", ["grid", "disc5"])
+@pytest.mark.parametrize("mesh", ["grid", "disc5", "fan4"])
 def test_boundary_patch_tables_bit_exact(hip, mesh):
     """Adaptive tables of open meshes (boundary B-spline patches with Far's
     boundary masks and filled phantom points, interior end caps): P, dPdu,
     dPdv at random and corner locations equal the oracle's bit for bit."""
     from opensubdiv_amd import far
-    from test_far_compiler import disc, open_grid
-    m = open_grid(5, creases=[(7, 8, 2.0)]) if mesh == "grid" else disc(5)
+    from test_far_compiler import PolyMesh, disc, open_grid
+    m = {"grid": lambda: open_grid(5, creases=[(7, 8, 2.0)]), "disc5": lambda: disc(5),
+         "fan4": lambda: PolyMesh([[0, 1, 2, 3], [0, 3, 4, 5], [0, 5, 6, 7], [0, 7, 8, 9]])}[mesh]()
     pt = far.Topology(m, boundary=2).patch_table(3)
     rng = np.random.default_rng(3)
     n = 4000
