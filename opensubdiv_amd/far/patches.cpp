@@ -581,7 +581,48 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
         for (int &b : bufIndex[d])
             if (b == 0) b = next_index++;
     numRefined = next_index - numControl;
-    numLocal = (int)greg.size();
+    // End-cap points: corner points shared by every cap at the vertex, edge
+    // points by the two caps on an interior edge, face points never -- Far's
+    // shareEndCapPatchPoints (far/patchTableFactory.cpp:1686-1860): a point
+    // reuses the first cap's local point, numbered in order of appearance.
+    std::vector<int> gregLocal(greg.size(), -1), newRows;
+    {
+        std::vector<std::vector<int>> cornerPt(maxL + 1), edgePt(maxL + 1);
+        for (size_t i = 0; i < gregLeaves.size(); ++i) {
+            const Leaf &lf = *gregLeaves[i];
+            const Level &L = refiner.level(lf.level);
+            std::vector<int> &cp = cornerPt[lf.level], &ep = edgePt[lf.level];
+            if (cp.empty()) cp.assign(L.nv, -1);
+            if (ep.empty()) ep.assign(2 * (size_t)L.ne, -1);
+            const int b = L.fvOff[lf.face];
+            for (int k = 0; k < 20; ++k) {
+                const int c = k / 5, kind = k % 5;
+                int *slot = nullptr;
+                if (kind == 0) {
+                    slot = &cp[L.fv[b + c]];
+                } else if (kind <= 2) {
+                    // E+ near corner c of edge c; E- near corner c of edge c - 1
+                    const int fe = kind == 1 ? c : (c + 3) & 3;
+                    const int e = L.fe[b + fe];
+                    if (!isBoundaryEdge(L, e)) {
+                        const bool reversed = L.ev[2 * e] != L.fv[b + fe];
+                        const int end = (kind == 2) != reversed;
+                        slot = &ep[2 * (size_t)e + end];
+                    }
+                }
+                int id;
+                if (slot && *slot >= 0) {
+                    id = *slot;
+                } else {
+                    id = (int)newRows.size();
+                    newRows.push_back((int)(20 * i + k));
+                    if (slot) *slot = id;
+                }
+                gregLocal[20 * i + k] = id;
+            }
+        }
+    }
+    numLocal = (int)newRows.size();
 
     // patch arrays / indices / params, and point stencils
     arrays.clear();
@@ -622,7 +663,7 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
     }
     if (nGreg) arrays.push_back({ kGregory, kGregory, nGreg, (int)indices.size(), 20, nReg });
     for (int i = 0; i < nGreg; ++i) {
-        for (int k = 0; k < 20; ++k) indices.push_back(numControl + numRefined + 20 * i + k);
+        for (int k = 0; k < 20; ++k) indices.push_back(numControl + numRefined + gregLocal[20 * i + k]);
         params.push_back(paramOf(*gregLeaves[i], false));
         patchDepth_.push_back(gregLeaves[i]->level);
     }
@@ -644,7 +685,8 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
         const LevelStencils &src = LS[maxL];
         std::vector<int> pos(numControl, -1), list;
         std::vector<double> val;
-        for (const Row &row : greg) {
+        for (int rowIndex : newRows) {
+            const Row &row = greg[(size_t)rowIndex];
             for (auto &x : row.e)
                 for (int64_t k = src.off[x.first]; k < src.off[x.first + 1]; ++k) {
                     const int c = src.idx[k];

@@ -119,7 +119,7 @@ class HostMesh:
 def mesh_recipe(spec: MeshSpec) -> str:
     """What compile_mesh builds from a spec's topology (the table cache's
     recipe string): everything but the frames, which are not cached."""
-    return (f"cube-sphere tables v2: refine vertex L{spec.level}; patches iso {spec.level} "
+    return (f"cube-sphere tables v3: refine vertex L{spec.level}; patches iso {spec.level} "
             f"gregory-basis end caps; coords {spec.ncoords} random @ seed {spec.seed + 1} "
             f"in patch order")
 

@@ -476,10 +476,9 @@ def test_patch_table_with_boundaries_matches_far_live(name, iso, boundary):
     boundary B-spline patches with Far's boundary masks and filled phantom
     points (smooth single-face corners isolated, then patched as sharp
     corners, Far's default), end caps at interior and at irregular boundary
-    points, transition masks.  Same arrays,
-    PatchParams, B-spline point indices and patch map as Far, and the same
-    limit surface.  (Far shares Gregory end-cap points between neighbouring
-    caps, so those indices differ; their values do not.)"""
+    points, transition masks.  Same arrays, PatchParams, point indices
+    (end-cap points shared between neighbouring caps as Far shares them),
+    patch map and limit surface as Far."""
     m = {"grid": lambda: open_grid(4),
          "grid-creased": lambda: open_grid(5, creases=[(7, 8, 2.0), (8, 9, 0.6)]),
          "disc5": lambda: disc(5), "disc3": lambda: disc(3),
@@ -494,8 +493,9 @@ def test_patch_table_with_boundaries_matches_far_live(name, iso, boundary):
     assert [tuple(a) for a in pt.arrays] == [tuple(a) for a in P.arrays]
     assert pt.params.tobytes() == P.params.tobytes()
     assert (((pt.params["field1"] >> 7) & 0xF) != 0).any()          # boundary patches present
-    nreg = int(pt.arrays[0]["numPatches"]) * 16 if pt.arrays[0]["desc"] == 6 else 0
-    np.testing.assert_array_equal(pt.indices[:nreg], P.indices[:nreg])
+    # every point index, end-cap points shared between caps as Far shares them
+    np.testing.assert_array_equal(pt.indices, P.indices)
+    assert pt.num_local == P.num_vertices_total - P.num_coarse - pt.num_refined
     rng = np.random.default_rng(iso)
     f = rng.integers(0, nptex, 3000).astype(np.int32)
     u, v = rng.random(3000), rng.random(3000)
