@@ -1,28 +1,38 @@
 """Benchmark of the MI355X Osd evaluator (one GPU).
 
-    python bench.py [--steps K] [--warmup W] [--no-cpu] [--only cfg1]
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--only cfg3|cfg4|cfg1|...]
 
 Workloads (synthetic, seeded; tables built by the project's own table
 compiler, which reproduces Far's tables):
 
-* cfg1 (headline): cube-sphere, 6 x 128 x 128 quads (98,306 vertices),
-  float32 xyz on the unit sphere with 1% seeded radial noise, 5% of edges
-  creased with sharpness U[0,4).  Limit stencil table with du/dv weights at
-  the 6,291,458 vertex locations of uniform level 3 (Far
-  LimitStencilTableFactory over the feature-adaptive iso-3 patch table).
-  One step = one fused EvalStencils launch producing P, dP/du, dP/dv for
-  every location (one frame, inputs resident in HBM).
-  value = limit points per second.
+* cfg3 (headline): 256 cube-sphere meshes (seed 0), n log-uniform in
+  [8, 128], 5% creased edges, uniform refinement level 2-4, 1-16 frames of
+  float32 xyz + 4 primvars, 64K-1M PatchCoords each.  One step = every
+  mesh's refinement of every frame, its patch points, and P / dPdu / dPdv at
+  its coords for every frame, in one graph launch per output stage.
+  value = refined rows + patch coords per second, over all frames.
+* cfg4: the serving loop over 32 resident topologies drawn like cfg3,
+  Poisson arrivals at a fixed fraction of the measured capacity; p50 / p99
+  latency beside points/s.
+* cfg1: cube-sphere, 6 x 128 x 128 quads (98,306 vertices), float32 xyz on
+  the unit sphere with 1% seeded radial noise, 5% of edges creased with
+  sharpness U[0,4).  Limit stencil table with du/dv weights at the 6,291,458
+  vertex locations of uniform level 3; one fused EvalStencils launch.
 * cfg1_refine: the uniform level-3 refinement table of the same mesh, P.
 * cfg2: EvalPatches (P, dP/du, dP/dv) at 4,194,304 PatchCoords (ptex face
   uniform, (u,v) uniform [0,1)^2, PatchMap lookup) on the iso-3
   feature-adaptive patch table (B-spline + Gregory end caps) of the mesh.
+* cfg0: the tutorial cube, uniform level 3 (a CPU configuration).
 
-The JSON line carries the roofline of the headline kernel (algorithmic HBM
-bytes per launch / average launch time from HIP events on the launch
-stream) and the CPU baseline: Osd::CpuEvaluator built from the reference
-sources when that library is present (else the C restatement), timed on
-this machine's host cores over the same table; OmpEvaluator beside it.
+The JSON line carries the roofline of the headline step (algorithmic HBM
+bytes per step / the step's time from HIP events on the launch stream, as a
+fraction of 8 TB/s and of the copy rate measured in the same run), the CPU
+baseline (Osd::CpuEvaluator and OmpEvaluator built from the reference
+sources when that library is present, else the C restatement, on a bounded
+sample of the same workload), parity (bit-exact checks against the
+reference CPU evaluator and position-sensitive fingerprints of every output
+buffer) and each workload's error against an FP64 evaluation of FP64
+tables.
 """
 from __future__ import annotations
 
@@ -49,7 +59,7 @@ def log(*a):
 
 # --------------------------------------------------------------- inputs
 
-def build_inputs(need_refine=True, need_patches=True):
+def build_inputs(need_refine=True, need_patches=True, fp64=True):
     from opensubdiv_amd import far, meshes
     t0 = time.time()
     mesh = meshes.cube_sphere(128, seed=1, radial_noise=0.01, crease_fraction=0.05,
@@ -60,14 +70,20 @@ def build_inputs(need_refine=True, need_patches=True):
     if need_refine:
         out["refine"] = topo.stencil_table()
         log(f"[bench] refinement table built ({time.time() - t0:.1f}s)")
+    if need_refine and fp64:
+        out["refine64"] = topo.stencil_table(double=True)
     ptex, u, v = topo.vertex_locations(3)
     pt = far.Topology(mesh).patch_table(3)
     out["patches"] = pt
     log(f"[bench] patch table built ({time.time() - t0:.1f}s)")
     out["limit"] = pt.limit_stencils(ptex, u, v)
+    if fp64:
+        out["limit64"] = pt.limit_stencils(ptex, u, v, double=True)
     log(f"[bench] limit table built ({time.time() - t0:.1f}s)")
     if need_patches:
         out["points"] = pt.point_stencils()
+        if fp64:
+            out["points64"] = pt.point_stencils(double=True)
         f, s, t = meshes.random_patch_locations(mesh.nfaces, 4194304, seed=2)
         out["queries"] = (f, s, t)
         out["coords"] = pt.find(f, s, t)
@@ -128,6 +144,13 @@ def cpu_time(run, budget_s, max_runs=9):
     return float(np.median(times)), len(times)
 
 
+def traffic_entry(key):
+    """The committed counter profile's entry for a workload, if any."""
+    if os.path.exists(PROFILE_TRAFFIC):
+        return json.load(open(PROFILE_TRAFFIC)).get(key)
+    return None
+
+
 def traffic_for(key, kernel):
     """HBM bytes per launch from the committed counter profile, when it was
     measured on the same kernel (template arguments included)."""
@@ -161,8 +184,28 @@ def kernel_labels(nw, lc):
             "split": f"k_stencils_sliced<{nw},{lc}>", "csr_lds": f"k_stencils_csr<{nw},{lc}>"}
 
 
-def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu):
-    """EvalStencils of a (limit) stencil table through HipStencilTable."""
+def bbox_rel_err(x, ref):
+    """max |x - ref| over the bounding-box diagonal of ref (per column set)."""
+    x = np.asarray(x, np.float64).reshape(len(ref), -1)
+    ref = np.asarray(ref, np.float64).reshape(len(ref), -1)
+    if ref.size == 0:
+        return 0.0
+    diag = float(np.linalg.norm(ref.max(axis=0) - ref.min(axis=0))) or 1.0
+    return float(np.abs(x - ref).max() / diag)
+
+
+def omp_threads():
+    return min(os.cpu_count() or 1, 16)
+
+
+OMP_NOTE = ("reference OmpEvaluator: default OpenMP schedule, and its per-thread accumulators "
+            "share cache lines (osd/ompKernel.cpp:90-111), so it can run slower than one core")
+
+
+def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu, table64=None):
+    """EvalStencils of a (limit) stencil table through HipStencilTable.
+    table64: the same table in double (Far StencilTableReal<double>) for
+    the error against an FP64 evaluation."""
     import torch
     from opensubdiv_amd import HipStencilTable, _lib
     stream = torch.cuda.current_stream()
@@ -172,7 +215,6 @@ def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu):
     n = len(table_np["sizes"])
     nnz = len(table_np["indices"])
     t0 = time.time()
-    lib.osdhip_config(b"table_layout", 0)
     htable = HipStencilTable.Create(table_np, s)
     log(f"[bench] {name}: device table {htable.device_bytes() / 1e6:.0f} MB in "
         f"{time.time() - t0:.1f}s")
@@ -180,20 +222,19 @@ def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu):
     outs = [torch.empty(n * L, dtype=torch.float32, device="cuda") for _ in range(nw)]
     alg = stencil_bytes(n, nnz, nw, L, src_np.shape[0], nw)
     # autotune: a few untimed launches of each kernel variant (record slices
-    # with gather groups of 8 / 4, split-stream slices, CSR with LDS
-    # staging), then the timed region runs exactly `steps` launches of the
-    # fastest
+    # with gather groups of 16 / 8 / 4, split-stream slices, CSR with LDS
+    # staging; per-table options), then the timed region runs exactly
+    # `steps` launches of the fastest
     VARIANTS = {"rec_g8": (0, 8, 0), "rec_g4": (0, 4, 0), "rec_g16": (0, 16, 0),
                 "split": (1, 8, 0), "csr_lds": (0, 8, 1)}
     variants = {}
     for vname, (layout, group, kernel) in VARIANTS.items():
-        lib.osdhip_config(b"slice_group", group)
-        lib.osdhip_config(b"stencil_kernel", kernel)
         tab = htable
         if layout != 0:
-            lib.osdhip_config(b"table_layout", layout)
-            tab = HipStencilTable.Create(table_np, s)
-            lib.osdhip_config(b"table_layout", 0)
+            tab = HipStencilTable.Create(table_np, s, layout=layout, group=group)
+        else:
+            htable.set("group", group)
+            htable.set("kernel", kernel)
         fn = table_eval_fn(lib, tab, d_src, D(0, L, L), outs, [D(0, L, L)] * nw, s)
         _, ms = time_launches(fn, 5, 2, stream)
         avg = float(np.median(ms))
@@ -205,22 +246,33 @@ def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu):
     layout, group, kernel = VARIANTS[best]
     if layout != 0:
         del htable
-        lib.osdhip_config(b"table_layout", layout)
-        htable = HipStencilTable.Create(table_np, s)
-        lib.osdhip_config(b"table_layout", 0)
-    lib.osdhip_config(b"slice_group", group)
-    lib.osdhip_config(b"stencil_kernel", kernel)
+        htable = HipStencilTable.Create(table_np, s, layout=layout, group=group)
+    else:
+        htable.set("group", group)
+        htable.set("kernel", kernel)
     step = table_eval_fn(lib, htable, d_src, D(0, L, L), outs, [D(0, L, L)] * nw, s)
     wall, ms = time_launches(step, args.steps, args.warmup, stream)
-    lib.osdhip_config(b"slice_group", 8)
-    lib.osdhip_config(b"stencil_kernel", 0)
     got = [o.cpu().numpy().reshape(n, L) for o in outs]
     avg = float(np.mean(ms))
     res = {"points_per_s": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
            "kernel_ms": avg, "stencils": n, "table_entries": nnz, "floats_per_point": L,
            "outputs": nw, "algorithmic_bytes": alg, "GBps": alg / (avg * 1e-3) / 1e9,
+           "frac_of_peak": alg / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
            "kernel": f"{kernel_label[best]} ({best})",
            "autotune": variants}
+    if table64 is not None:
+        import oracle
+        src64 = np.ascontiguousarray(src_np, np.float64)
+        errs = {}
+        for k, key in enumerate(("weights", "du", "dv")[:nw]):
+            want = np.zeros((n, L), np.float64)
+            assert oracle.eval_stencils_f64(src64, (0, L, L), want, (0, L, L), table64["sizes"],
+                                            table64["offsets"], table64["indices"],
+                                            table64[key])
+            errs[("P", "dPdu", "dPdv")[k]] = bbox_rel_err(got[k], want)
+        res["fp64_error"] = {"bbox_rel_max": max(errs.values()), "per_output": errs,
+                             "reference": "FP64 evaluation of the table compiled in double "
+                                          "(Far StencilTableReal<double> semantics)"}
     if with_cpu:
         from oracle import ref
         import oracle
@@ -248,19 +300,30 @@ def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu):
         res["cpu_baseline"] = {"value": n / med, "unit": "points/s", "cores": 1, "kind": kind,
                                "sample": f"full table ({n} stencils, {nw} outputs) x {runs} "
                                          f"runs, median {med * 1e3:.1f} ms"}
-        if nw == 1 and ref.available():
-            threads = min(os.cpu_count() or 1, 16)
-            omp = lambda: ref.cpu_eval_stencils(src_np, (0, L, L), dst[0], (0, L, L),  # noqa
-                                                table_np, threads=threads)
-            m2, r2 = cpu_time(omp, args.cpu_budget / 2)
-            res["cpu_baseline"]["omp"] = {"value": n / m2, "cores": threads,
-                                          "kind": "reference OmpEvaluator",
-                                          "sample": f"{r2} runs, median {m2 * 1e3:.1f} ms"}
-            want = [np.zeros((n, L), np.float32)]
-            ref.cpu_eval_stencils(src_np, (0, L, L), want[0], (0, L, L), table_np)
+        res["parity"] = {"bit_exact_vs_cpu": all(np.array_equal(a, b) for a, b in zip(got, dst))}
+        threads = omp_threads()
+        odst = [np.zeros((n, L), np.float32) for _ in range(nw)]
+        if ref.available():
+            okind = "reference OmpEvaluator"
+            if nw == 1:
+                omp = lambda: ref.cpu_eval_stencils(src_np, (0, L, L), odst[0], (0, L, L),  # noqa
+                                                    table_np, threads=threads)
+            else:
+                omp = lambda: ref.cpu_eval_stencils_d1(src_np, (0, L, L), odst,  # noqa
+                                                       [(0, L, L)] * 3, table_np,
+                                                       threads=threads)
         else:
-            want = dst
-        res["parity"] = {"bit_exact_vs_cpu": all(np.array_equal(a, b) for a, b in zip(got, want))}
+            okind = "port (OpenMP restatement)"
+            omp = (lambda: oracle.eval_stencils(src_np, (0, L, L), odst[0], (0, L, L),  # noqa
+                                                table_np["sizes"], table_np["offsets"],
+                                                table_np["indices"], table_np["weights"],
+                                                threads=threads)) if nw == 1 else None
+        if omp is not None:
+            m2, r2 = cpu_time(omp, args.cpu_budget / 2)
+            res["cpu_baseline"]["omp"] = {
+                "value": n / m2, "cores": threads, "kind": okind, "note": OMP_NOTE,
+                "sample": f"full table x {r2} runs, median {m2 * 1e3:.1f} ms",
+                "bit_exact_vs_cpu": all(np.array_equal(a, b) for a, b in zip(odst, dst))}
     del htable
     return res
 
@@ -378,71 +441,188 @@ def run_patches(args, lib, inputs, with_cpu):
     d3 = D(0, L, L)
     z = D(0, 0, 0)
 
-    def step():
-        rc = lib.osdhip_eval_patches(d_src.data_ptr(), d3, outs[0].data_ptr(), d3,
-                                     outs[1].data_ptr(), d3, outs[2].data_ptr(), d3,
-                                     None, z, None, z, None, z, n, d_coords.data_ptr(),
-                                     d_arrays.data_ptr(), d_indices.data_ptr(),
-                                     d_params.data_ptr(), s)
-        if rc != 0:
-            raise RuntimeError(f"osdhip_eval_patches failed: {rc}")
+    def make_step(entry):
+        fn = getattr(lib, entry)
+
+        def step():
+            rc = fn(d_src.data_ptr(), d3, outs[0].data_ptr(), d3, outs[1].data_ptr(), d3,
+                    outs[2].data_ptr(), d3, None, z, None, z, None, z, n, d_coords.data_ptr(),
+                    d_arrays.data_ptr(), d_indices.data_ptr(), d_params.data_ptr(), s)
+            if rc != 0:
+                raise RuntimeError(f"{entry} failed: {rc}")
+        return step
     alg = patch_bytes(n, L, 3, len(pt.indices), len(pt.params), len(pt.arrays), nctrl + npts)
     # autotune: caller order vs patch-sorted order (radix sort inside each
-    # launch, included in the timed step)
+    # call, included in the timed step)
+    entries = {"caller_order": "osdhip_eval_patches", "patch_sorted": "osdhip_eval_patches_sorted"}
     variants = {}
-    for vname, smin in (("caller_order", 0), ("patch_sorted", 16384)):
-        lib.osdhip_config(b"patch_sort_min", smin)
-        _, ms = time_launches(step, 5, 2, stream)
+    for vname, entry in entries.items():
+        _, ms = time_launches(make_step(entry), 5, 2, stream)
         avg = float(np.median(ms))
         variants[vname] = {"step_ms": avg, "GBps": alg / (avg * 1e-3) / 1e9}
     best = min(variants, key=lambda k: variants[k]["step_ms"])
-    lib.osdhip_config(b"patch_sort_min", 0 if best == "caller_order" else 16384)
-    wall, ms = time_launches(step, args.steps, args.warmup, stream)
+    wall, ms = time_launches(make_step(entries[best]), args.steps, args.warmup, stream)
     avg = float(np.mean(ms))
-    lib.osdhip_config(b"patch_sort_min", 0)
     res = {"points_per_s": n * args.steps / wall, "ms_per_step": wall * 1e3 / args.steps,
            "kernel_ms": avg, "patch_coords": n, "patches": len(pt.params),
            "patch_map": device_patch_map(args, inputs, stream),
            "gregory_patches": int(sum(a["numPatches"] for a in pt.arrays if a["desc"] == 9)),
            "algorithmic_bytes": alg, "GBps": alg / (avg * 1e-3) / 1e9,
+           "frac_of_peak": alg / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS,
            "kernel": f"k_patches<true,false,3> ({best})", "autotune": variants}
+    got = [o.cpu().numpy().reshape(n, L) for o in outs]
+    if inputs.get("points64") is not None:
+        import oracle
+        p64 = inputs["points64"]
+        ctrl64 = np.ascontiguousarray(mesh.positions, np.float64)
+        pts64 = oracle.apply_stencils(ctrl64, p64["sizes"], p64["offsets"], p64["indices"],
+                                      p64["weights"], np.float64)
+        src64 = np.ascontiguousarray(np.concatenate([ctrl64, pts64]))
+        want = [np.zeros((n, L), np.float64) for _ in range(3)]
+        assert oracle.eval_patches_f64(src64, (0, L, L), want, [(0, L, L)] * 3, coords,
+                                       pt.arrays, pt.indices, pt.params)
+        errs = {k: bbox_rel_err(g, w) for k, g, w in zip(("P", "dPdu", "dPdv"), got, want)}
+        res["fp64_error"] = {"bbox_rel_max": max(errs.values()), "per_output": errs,
+                             "reference": "FP64 patch points (double point stencils) and "
+                                          "FP64 basis (Far EvaluateBasis<double> semantics)"}
     if with_cpu:
         src_np = d_src.cpu().numpy().reshape(-1, 3)
         import oracle
         from oracle import ref
-        got = [o.cpu().numpy().reshape(n, L) for o in outs]
         dst = [np.zeros((n, L), np.float32) for _ in range(3)]
+        odst = [np.zeros((n, L), np.float32) for _ in range(3)]
+        threads = omp_threads()
         if ref.available():
-            kind = "reference"
+            kind, okind = "reference", "reference OmpEvaluator"
             run = lambda: ref.cpu_eval_patches(src_np, (0, L, L), dst, [(0, L, L)] * 3,  # noqa
                                                coords, pt.arrays, pt.indices, pt.params)
+            omp = lambda: ref.cpu_eval_patches(src_np, (0, L, L), odst, [(0, L, L)] * 3,  # noqa
+                                               coords, pt.arrays, pt.indices, pt.params,
+                                               threads=threads)
         else:
-            kind = "port"
+            kind, okind = "port", "port (OpenMP restatement)"
             run = lambda: oracle.eval_patches(src_np, (0, L, L), dst, [(0, L, L)] * 3,  # noqa
                                               coords, pt.arrays, pt.indices, pt.params)
+            omp = lambda: oracle.eval_patches(src_np, (0, L, L), odst, [(0, L, L)] * 3,  # noqa
+                                              coords, pt.arrays, pt.indices, pt.params,
+                                              threads=threads)
         med, runs = cpu_time(run, args.cpu_budget, max_runs=3)
         res["cpu_baseline"] = {"value": n / med, "unit": "points/s", "cores": 1, "kind": kind,
                                "sample": f"all {n} coords x {runs} runs, median "
                                          f"{med * 1e3:.1f} ms"}
+        m2, r2 = cpu_time(omp, args.cpu_budget / 2, max_runs=5)
+        res["cpu_baseline"]["omp"] = {
+            "value": n / m2, "cores": threads, "kind": okind, "note": OMP_NOTE,
+            "sample": f"all coords x {r2} runs, median {m2 * 1e3:.1f} ms",
+            "bit_exact_vs_cpu": all(np.array_equal(a, b) for a, b in zip(odst, dst))}
         res["parity"] = {"bit_exact_vs_cpu": all(np.array_equal(a, b) for a, b in zip(got, dst))}
     return res
 
 
-def buffer_checksums(dms):
-    """Exact order-independent checksum of every output buffer: the sum of
-    its float bit patterns as int64 (equal bits -> equal sums)."""
+def fingerprint(buf, chunk=1 << 26):
+    """Position-sensitive fingerprint of a float32 device buffer: the sum
+    over i of bits[i] * w(i) modulo 2^64, with w(i) an odd 31-bit hash of
+    the position.  Equal buffers give equal fingerprints; a permutation of
+    rows, a shifted range or any changed value changes it (with
+    overwhelming probability).  Computed on the device in chunks."""
     import torch
-    chunk = 1 << 26
+    v = buf.view(torch.int32).reshape(-1)
+    total = 0
+    for i in range(0, v.numel(), chunk):
+        x = v[i:i + chunk].to(torch.int64) & 0xFFFFFFFF
+        w = torch.arange(i, i + x.numel(), device=x.device, dtype=torch.int64)
+        w = ((w * 0x9E3779B1 + 0x7F4A7C15) & 0x7FFFFFFF) | 1
+        total = (total + int((x * w).sum().item())) % (1 << 64)
+        del x, w
+    return total
+
+
+def mesh_fingerprints(dms):
+    """Fingerprints of every output buffer of every mesh: refined rows,
+    patch points (the vertex buffer past the control points) and P, dPdu,
+    dPdv."""
     out = []
     for d in dms:
         bufs = [d.refined, d.vbuf[d.nctrl * d.spec.frames * d.lp:]] + d.out
-        sums = []
-        for b in bufs:
-            v = b.view(torch.int32)
-            sums.append(sum(int(v[i:i + chunk].to(torch.int64).sum().item())
-                            for i in range(0, v.numel(), chunk)))
-        out.append(sums)
+        out.append([fingerprint(b) for b in bufs])
     return out
+
+
+def check_mesh_vs_reference(d, frames, fp64=True):
+    """One batched mesh against the reference CPU evaluator at full size:
+    its tables recompiled on the host (the compiler is deterministic), then
+    for each frame in `frames` the refinement rows, the patch points and
+    P / dPdu / dPdv at every coord are compared bit for bit with
+    Osd::CpuEvaluator (the C restatement where the reference library is
+    absent) on the same inputs; and the error of the device results against
+    an FP64 evaluation of FP64 tables (frame frames[0])."""
+    import oracle
+    from oracle import ref
+    from opensubdiv_amd import far, meshes
+    from opensubdiv_amd.workloads import compile_mesh
+    sp, L, F, Lp = d.spec, d.length, d.spec.frames, d.lp
+    h = compile_mesh(sp, L, threads=0)
+    try:
+        mesh = meshes.cube_sphere(sp.n, seed=sp.seed, radial_noise=0.01, crease_fraction=0.05,
+                                  sharpness_max=4.0)
+        topo = far.Topology(mesh)
+        topo.refine_uniform(sp.level)
+        table = topo.stencil_table()
+        refined = d.refined.view(F, d.nrefine, L)
+        vpts = d.vbuf[d.nctrl * F * Lp:].view(d.npts, F, Lp)
+        outs = [o.view(F, d.ncoords, L) for o in d.out]
+        use_ref = ref.available()
+        same = True
+        for f in frames:
+            ctrl = np.ascontiguousarray(h.frames[f])
+            dst = np.zeros((d.nrefine, L), np.float32)
+            pts = np.zeros((d.npts, L), np.float32)
+            po = [np.zeros((d.ncoords, L), np.float32) for _ in range(3)]
+            if use_ref:
+                ok = ref.cpu_eval_stencils(ctrl, (0, L, L), dst, (0, L, L), table) and \
+                    ref.cpu_eval_stencils(ctrl, (0, L, L), pts, (0, L, L), h.points)
+                src = np.ascontiguousarray(np.concatenate([ctrl, pts]))
+                ok = ok and ref.cpu_eval_patches(src, (0, L, L), po, [(0, L, L)] * 3, h.coords,
+                                                 h.arrays, h.indices, h.params)
+            else:
+                dst = oracle.apply_stencils(ctrl, table["sizes"], table["offsets"],
+                                            table["indices"], table["weights"])
+                pts = oracle.apply_stencils(ctrl, h.points["sizes"], h.points["offsets"],
+                                            h.points["indices"], h.points["weights"])
+                src = np.ascontiguousarray(np.concatenate([ctrl, pts]))
+                ok = oracle.eval_patches(src, (0, L, L), po, [(0, L, L)] * 3, h.coords, h.arrays,
+                                         h.indices, h.params)
+            same &= bool(ok)
+            same &= np.array_equal(refined[f].cpu().numpy().view(np.uint32), dst.view(np.uint32))
+            same &= np.array_equal(np.ascontiguousarray(vpts[:, f, :L].cpu().numpy()).view(
+                np.uint32), pts.view(np.uint32))
+            for o, w in zip(outs, po):
+                same &= np.array_equal(o[f].cpu().numpy().view(np.uint32), w.view(np.uint32))
+        res = {"mesh": sp.index, "n": sp.n, "level": sp.level, "frames_checked": list(frames),
+               "refined_rows": d.nrefine, "patch_points": d.npts, "coords": d.ncoords,
+               "bit_exact_vs_cpu": bool(same),
+               "checker": "reference CpuEvaluator" if use_ref else "oracle restatement"}
+        if fp64:
+            f = frames[0]
+            c64 = np.ascontiguousarray(h.frames[f], np.float64)
+            t64 = topo.stencil_table(double=True)
+            r64 = oracle.apply_stencils(c64, t64["sizes"], t64["offsets"], t64["indices"],
+                                        t64["weights"], np.float64)
+            pt = far.Topology(mesh).patch_table(sp.level)
+            p64t = pt.point_stencils(double=True)
+            p64 = oracle.apply_stencils(c64, p64t["sizes"], p64t["offsets"], p64t["indices"],
+                                        p64t["weights"], np.float64)
+            s64 = np.ascontiguousarray(np.concatenate([c64, p64]))
+            w64 = [np.zeros((d.ncoords, L), np.float64) for _ in range(3)]
+            assert oracle.eval_patches_f64(s64, (0, L, L), w64, [(0, L, L)] * 3, h.coords,
+                                           h.arrays, h.indices, h.params)
+            errs = {"refined": bbox_rel_err(refined[f].cpu().numpy(), r64)}
+            for k, o, w in zip(("P", "dPdu", "dPdv"), outs, w64):
+                errs[k] = bbox_rel_err(o[f].cpu().numpy(), w)
+            res["fp64_error"] = {"bbox_rel_max": max(errs.values()), "per_output": errs}
+        return res
+    finally:
+        h.release()
 
 
 def table_cache(args):
@@ -457,7 +637,15 @@ def cache_summary(cache):
     return None if cache is None else {"hits": cache.hits, "misses": cache.misses}
 
 
-def run_batched(args, lib, device):
+def sample_for_check(members, max_rows=3_000_000):
+    """The mesh of a stage checked against the reference CPU evaluator: the
+    largest whose refinement has at most `max_rows` rows (bounded host
+    time), else the smallest."""
+    small = [d for d in members if d.nrefine <= max_rows]
+    return max(small, key=lambda d: d.nrefine) if small else min(members, key=lambda d: d.nrefine)
+
+
+def run_batched(args, lib, device, copy_gbps=None):
     """cfg3: 256 cube-sphere meshes (n log-uniform in [8,128], 5% creased
     edges, level 2-4, 1-16 frames of xyz + 4 primvars, 64K-1M patch-ordered
     PatchCoords) in one graph launch per step, against the same work one
@@ -471,14 +659,10 @@ def run_batched(args, lib, device):
     seed = args.seed
     specs = workloads.sample_meshes(args.meshes, seed)
     t0 = time.time()
-    lib.osdhip_config(b"table_csr_mirror", 0)
-    old_layout = lib.osdhip_config(b"table_layout", args.table_layout)
     workers = min(16, os.cpu_count() or 1)
     cache = table_cache(args)
     dms = workloads.build_device_meshes(specs, L, s, workers=workers, log=log, own_outputs=False,
-                                        cache=cache)
-    lib.osdhip_config(b"table_csr_mirror", 1)
-    lib.osdhip_config(b"table_layout", old_layout)
+                                        cache=cache, layout=args.table_layout)
     torch.cuda.synchronize()
     build_s = time.time() - t0
     # outputs (refined vertices, P / dPdu / dPdv of every frame) stream
@@ -495,27 +679,61 @@ def run_batched(args, lib, device):
         f"through a {4 * arena.numel() / 1e9:.1f} GB arena in {nstages} stage(s)")
     batch = Batch(L, [x for d in dms for x in d.stencil_segments()],
                   [d.patch_segment() for d in dms], s)
-    # parity at full size: every stage, batched kernels vs one mesh and
-    # frame per launch, exact checksums of every output buffer
-    parity = True
-    for st in range(nstages):
-        members = [d for d in dms if d.stage == st]
-        batch.run_stage(st, 3, s)
-        a = buffer_checksums(members)
-        batch.run_plain_stage(st, s)
-        b = buffer_checksums(members)
-        parity &= a == b
-    log(f"[cfg3] parity batched == plain: {parity}")
-    t0 = time.time()
-    report = batch.tune(reps=2, stream=s)
-    log(f"[cfg3] tuned in {time.time() - t0:.1f}s: {batch.config()}")
+    if args.config:
+        for kv in args.config.split(","):
+            k, v = kv.split("=")
+            batch.set(k.strip(), int(v))
+    parity = None
+    if not args.no_parity:
+        # parity at full size, every stage: (1) one sampled mesh's batched
+        # outputs against the reference CPU evaluator, bit for bit, plus its
+        # error against FP64; (2) position-sensitive fingerprints of every
+        # output buffer, batched kernels vs one mesh and frame per launch
+        parity = {"stages": []}
+        for st in range(nstages):
+            members = [d for d in dms if d.stage == st]
+            batch.run_stage(st, 3, s)
+            torch.cuda.synchronize()
+            a = mesh_fingerprints(members)
+            chk = sample_for_check(members)
+            frames = sorted({0, chk.spec.frames - 1})
+            sample = check_mesh_vs_reference(chk, frames)
+            batch.run_plain_stage(st, s)
+            torch.cuda.synchronize()
+            b = mesh_fingerprints(members)
+            parity["stages"].append({"meshes": len(members), "buffers": 5 * len(members),
+                                     "batched_equals_plain": a == b, "sampled_mesh": sample})
+            log(f"[cfg3] stage {st}: batched == plain fingerprints {a == b}; mesh "
+                f"{chk.spec.index} (n {chk.spec.n}, level {chk.spec.level}) bit-exact vs CPU "
+                f"{sample['bit_exact_vs_cpu']}, fp64 error {sample['fp64_error']['bbox_rel_max']:.2e}")
+        parity["batched_equals_plain_all"] = all(x["batched_equals_plain"]
+                                                 for x in parity["stages"])
+        parity["sampled_bit_exact_vs_cpu_all"] = all(x["sampled_mesh"]["bit_exact_vs_cpu"]
+                                                     for x in parity["stages"])
+        parity["method"] = ("per stage: position-sensitive fingerprints (sum of bits x position "
+                            "hash mod 2^64) of all 5 output buffers of every mesh, batched vs one "
+                            "mesh and frame per launch; and one sampled mesh per stage compared "
+                            "bit for bit with the reference CpuEvaluator on its first and last "
+                            "frames")
+    report = None
+    if not args.no_tune:
+        t0 = time.time()
+        report = batch.tune(reps=2, stream=s)
+        log(f"[cfg3] tuned in {time.time() - t0:.1f}s: {batch.config()}")
     points = sum(d.points_per_step() for d in dms)
     alg = sum(d.algorithmic_bytes() for d in dms)
     wall, ms = time_launches(lambda: batch.launch(s), args.steps, args.warmup, stream)
     kms = float(np.median(ms))
-    plain_steps = max(2, min(args.steps, 3))
-    pwall, pms = time_launches(lambda: batch.run_plain(s), plain_steps, 1, stream)
-    pkms = float(np.median(pms))
+    achieved = alg / (kms * 1e-3) / 1e9
+    plain = None
+    if not args.no_plain:
+        plain_steps = max(2, min(args.steps, 3))
+        pwall, pms = time_launches(lambda: batch.run_plain(s), plain_steps, 1, stream)
+        pkms = float(np.median(pms))
+        plain = {"points_per_s": points * plain_steps / pwall, "step_ms": pkms,
+                 "launches_per_step": sum(2 * d.spec.frames + d.spec.frames for d in dms),
+                 "speedup_batched": pkms / kms}
+    tr = traffic_entry("cfg3")
     res = {
         "metric": METRIC, "value": points * args.steps / wall, "unit": "points/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -533,20 +751,30 @@ def run_batched(args, lib, device):
                    "table_entries": sum(d.nnz_refine + d.nnz_points for d in dms),
                    "device_bytes": dev_bytes, "build_s": build_s,
                    "table_cache": cache_summary(cache)},
-        "roofline": {"bound": "hbm", "achieved": alg / (kms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_batch_stencils + k_batch_patches (graph)",
-                     "algorithmic_bytes": alg, "kernel_ms": kms},
-        "plain": {"points_per_s": points * plain_steps / pwall, "step_ms": pkms,
-                  "launches_per_step": sum(2 * d.spec.frames + d.spec.frames for d in dms),
-                  "speedup_batched": pkms / kms},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": tr.get("bytes_per_step") if tr else None,
+                     "copy_rate_GBps": copy_gbps,
+                     "frac_of_copy_rate": achieved / copy_gbps if copy_gbps else None,
+                     "kernel": "one step: k_batch_stencils_tiled (phase A) + k_batch_patches* "
+                               "(phase B), graph launches of every stage",
+                     "algorithmic_bytes": alg, "kernel_ms": kms,
+                     "output_bytes": out_bytes},
+        "plain": plain,
         "autotune": report, "tuned": batch.config(),
-        "parity": {"batched_equals_plain_checksums": parity,
-                   "method": "per stage: int64 sums of every output buffer's bit patterns"},
+        "parity": parity,
     }
+    if tr:
+        res["roofline"]["traffic_source"] = tr.get("source")
+    if parity:
+        errs = [x["sampled_mesh"]["fp64_error"]["bbox_rel_max"] for x in parity["stages"]]
+        res["fp64_error"] = {"bbox_rel_max": max(errs), "per_stage": errs,
+                             "reference": "sampled mesh per stage: FP64 tables, FP64 control "
+                                          "points (promoted), FP64 basis"}
     if not args.no_cpu:
-        res["cpu_baseline"] = batched_cpu_baseline(dms, specs, args.cpu_budget)
+        res["cpu_baseline"] = batched_cpu_baseline(dms, args.cpu_budget)
     batch.close()
+    del arena, dms
     return res
 
 
@@ -565,11 +793,7 @@ def run_served(args, lib, device):
     cache = table_cache(args)
     with ThreadPoolExecutor(max_workers=workers) as ex:
         hosts = list(ex.map(lambda sp: workloads.compile_mesh(sp, L, 1, cache), specs))
-    lib.osdhip_config(b"table_csr_mirror", 0)
-    old_layout = lib.osdhip_config(b"table_layout", args.table_layout)
-    residents = serve.residents_from_hosts(hosts)
-    lib.osdhip_config(b"table_csr_mirror", 1)
-    lib.osdhip_config(b"table_layout", old_layout)
+    residents = serve.residents_from_hosts(hosts, layout=args.table_layout)
     for h in hosts:
         h.release()
     torch.cuda.synchronize()
@@ -634,51 +858,70 @@ def run_served(args, lib, device):
     }
 
 
-def batched_cpu_baseline(dms, specs, budget_s):
+def batched_cpu_baseline(dms, budget_s):
     """Reference CpuEvaluator on a bounded sample of the batched workload:
-    the refinement stencils and EvalPatches of the smallest meshes, one
-    frame each, until about `budget_s` seconds of CPU work."""
+    the refinement stencils, patch points and EvalPatches of the smallest
+    meshes, one frame each, until about `budget_s` seconds of CPU work; then
+    OmpEvaluator on the same sample (all host threads, at most 16)."""
     from oracle import ref
     import oracle
-    from opensubdiv_amd import far, meshes
+    from opensubdiv_amd.workloads import compile_mesh
+    use_ref = ref.available()
+    threads = omp_threads()
     t_end = time.time() + budget_s
-    pts, secs, used = 0, 0.0, 0
+    pts, secs, osecs, used = 0, 0.0, 0.0, 0
     for d in sorted(dms, key=lambda x: x.nrefine):
         if time.time() > t_end:
             break
-        sp, L, F = d.spec, d.length, d.spec.frames
-        mesh = meshes.cube_sphere(sp.n, seed=sp.seed, radial_noise=0.01, crease_fraction=0.05,
-                                  sharpness_max=4.0)
-        topo = far.Topology(mesh)
-        topo.refine_uniform(sp.level)
-        table = topo.stencil_table()
-        vb = np.ascontiguousarray(d.vbuf.cpu().numpy().reshape(-1, F, d.lp)[:, 0, :L])
-        ctrl = np.ascontiguousarray(vb[:d.nctrl])
-        dst = np.zeros((d.nrefine, L), np.float32)
-        n = d.ncoords
-        outs = [np.zeros((n, L), np.float32) for _ in range(3)]
-        coords = d.coords.cpu().numpy().view(oracle_coord_dtype())
-        arrays = d.arrays.cpu().numpy().view(oracle_array_dtype())
-        params = d.params.cpu().numpy().view(oracle_param_dtype())
-        indices = d.indices.cpu().numpy()
-        t0 = time.perf_counter()
-        if ref.available():
-            ref.cpu_eval_stencils(ctrl, (0, L, L), dst, (0, L, L), table)
-            ref.cpu_eval_patches(vb, (0, L, L), outs, [(0, L, L)] * 3, coords, arrays, indices,
-                                 params)
-            kind = "reference"
-        else:
-            oracle.eval_stencils(ctrl, (0, L, L), dst, (0, L, L), table["sizes"],
-                                 table["offsets"], table["indices"], table["weights"])
-            oracle.eval_patches(vb, (0, L, L), outs, [(0, L, L)] * 3, coords, arrays, indices,
-                                params)
-            kind = "port"
-        secs += time.perf_counter() - t0
+        sp, L = d.spec, d.length
+        h = compile_mesh(sp, L, threads=0)
+        try:
+            ctrl = np.ascontiguousarray(h.frames[0])
+            from opensubdiv_amd import far, meshes
+            mesh = meshes.cube_sphere(sp.n, seed=sp.seed, radial_noise=0.01,
+                                      crease_fraction=0.05, sharpness_max=4.0)
+            topo = far.Topology(mesh)
+            topo.refine_uniform(sp.level)
+            table = topo.stencil_table()
+            n = len(h.coords)
+            for omp in (0, threads):
+                dst = np.zeros((len(table["sizes"]), L), np.float32)
+                pbuf = np.zeros((len(h.points["sizes"]), L), np.float32)
+                outs = [np.zeros((n, L), np.float32) for _ in range(3)]
+                t0 = time.perf_counter()
+                if use_ref:
+                    ref.cpu_eval_stencils(ctrl, (0, L, L), dst, (0, L, L), table, threads=omp)
+                    ref.cpu_eval_stencils(ctrl, (0, L, L), pbuf, (0, L, L), h.points,
+                                          threads=omp)
+                    src = np.concatenate([ctrl, pbuf])
+                    ref.cpu_eval_patches(src, (0, L, L), outs, [(0, L, L)] * 3, h.coords,
+                                         h.arrays, h.indices, h.params, threads=omp)
+                else:
+                    oracle.eval_stencils(ctrl, (0, L, L), dst, (0, L, L), table["sizes"],
+                                         table["offsets"], table["indices"], table["weights"],
+                                         threads=omp)
+                    oracle.eval_stencils(ctrl, (0, L, L), pbuf, (0, L, L), h.points["sizes"],
+                                         h.points["offsets"], h.points["indices"],
+                                         h.points["weights"], threads=omp)
+                    src = np.concatenate([ctrl, pbuf])
+                    oracle.eval_patches(src, (0, L, L), outs, [(0, L, L)] * 3, h.coords,
+                                        h.arrays, h.indices, h.params, threads=omp)
+                dt = time.perf_counter() - t0
+                if omp:
+                    osecs += dt
+                else:
+                    secs += dt
+        finally:
+            h.release()
         pts += d.nrefine + n
         used += 1
+    kind = "reference" if use_ref else "port"
     return {"value": pts / secs if secs else None, "unit": "points/s", "cores": 1, "kind": kind,
-            "sample": f"{used} smallest meshes, frame 0: refinement stencils + EvalPatches "
-                      f"(P, dPdu, dPdv) of {pts} points in {secs:.1f}s"}
+            "sample": f"{used} smallest meshes, frame 0: refinement stencils, patch points and "
+                      f"EvalPatches (P, dPdu, dPdv); {pts} points in {secs:.1f}s",
+            "omp": {"value": pts / osecs if osecs else None, "cores": threads,
+                    "kind": "reference OmpEvaluator" if use_ref else "port (OpenMP restatement)",
+                    "note": OMP_NOTE, "sample": f"the same {pts} points in {osecs:.1f}s"}}
 
 
 def oracle_coord_dtype():
@@ -696,34 +939,65 @@ def oracle_param_dtype():
     return PATCH_PARAM_DTYPE
 
 
-def copy_probe(args):
+def measure_copy(steps=10, warmup=2):
     """1 GiB device-to-device copy: reads and writes exactly 2^30 bytes per
     launch, so its FETCH_SIZE / WRITE_SIZE calibrate the counters and its
-    rate is the measured copy bandwidth (read + write bytes / time)."""
+    rate (read + write bytes / time) is the measured copy bandwidth."""
     import torch
     n = 1 << 28
     a = torch.rand(n, device="cuda")
     b = torch.empty_like(a)
     stream = torch.cuda.current_stream()
-    wall, ms = time_launches(lambda: b.copy_(a), args.steps, args.warmup, stream)
+    _, ms = time_launches(lambda: b.copy_(a), steps, warmup, stream)
     kms = float(np.median(ms))
-    _, rms = time_launches(lambda: a.sum(), args.steps, args.warmup, stream)
+    _, rms = time_launches(lambda: a.sum(), steps, warmup, stream)
     rkms = float(np.median(rms))
-    print(json.dumps({"probe": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n,
-                      "kernel_ms": kms, "GBps": 8 * n / (kms * 1e-3) / 1e9,
-                      "read_probe": {"bytes_read": 4 * n, "kernel_ms": rkms,
-                                     "GBps": 4 * n / (rkms * 1e-3) / 1e9, "op": "torch.sum"},
+    del a, b
+    torch.cuda.empty_cache()
+    return {"bytes_read": 4 * n, "bytes_written": 4 * n, "kernel_ms": kms,
+            "GBps": 8 * n / (kms * 1e-3) / 1e9,
+            "read_probe": {"bytes_read": 4 * n, "kernel_ms": rkms,
+                           "GBps": 4 * n / (rkms * 1e-3) / 1e9, "op": "torch.sum"}}
+
+
+def copy_probe(args):
+    print(json.dumps({"probe": "copy", **measure_copy(args.steps, args.warmup),
                       "steps": args.steps}), flush=True)
+
+
+def run_single_mesh(args, lib, device, want, with_cpu):
+    """cfg1 (limit stencils), cfg1_refine, cfg2 (EvalPatches) and cfg0 (the
+    tutorial cube): the one-mesh workloads."""
+    inputs = build_inputs(need_refine="cfg1_refine" in want, need_patches="cfg2" in want,
+                          fp64=not args.no_fp64)
+    src = inputs["mesh"].positions.astype(np.float32)
+    out = {}
+    if "cfg1" in want:
+        out["cfg1"] = run_stencils(args, lib, "cfg1 limit stencils", inputs["limit"], src, 3,
+                                   kernel_labels(3, 3), with_cpu, inputs.get("limit64"))
+        out["cfg1"]["traffic"] = traffic_for("cfg1_limit", out["cfg1"]["kernel"].split(" ")[0])
+    if "cfg1_refine" in want:
+        out["cfg1_refine"] = run_stencils(args, lib, "cfg1 refinement", inputs["refine"], src, 1,
+                                          kernel_labels(1, 3), with_cpu, inputs.get("refine64"))
+        out["cfg1_refine"]["traffic"] = traffic_for(
+            "cfg1_refine", out["cfg1_refine"]["kernel"].split(" ")[0])
+    if "cfg2" in want:
+        out["cfg2"] = run_patches(args, lib, inputs, with_cpu)
+    if "cfg0" in want:
+        out["cfg0"] = run_cube(args, lib, with_cpu)
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 error checks")
     ap.add_argument("--cpu-budget", type=float, default=10.0, help="seconds per CPU baseline")
-    ap.add_argument("--only", default=None, choices=[None, "cfg1", "cfg1_refine", "cfg2", "cfg3", "cfg4"])
+    ap.add_argument("--only", default=None,
+                    choices=[None, "cfg0", "cfg1", "cfg1_refine", "cfg2", "cfg3", "cfg4"])
     ap.add_argument("--requests", type=int, default=1500, help="cfg4: requests served")
     ap.add_argument("--load", type=float, default=0.7,
                     help="cfg4: arrival rate as a fraction of the measured capacity")
@@ -738,7 +1012,12 @@ def main():
                          "rest are compiled and stored")
     ap.add_argument("--table-layout", type=int, default=2, choices=[0, 2],
                     help="cfg3/cfg4: stencil table layout (0 record slices, 2 locality-ordered)")
-
+    ap.add_argument("--config", default=None,
+                    help="cfg3: batch knobs key=value,... set before (or instead of) tuning")
+    ap.add_argument("--no-tune", action="store_true", help="cfg3: skip the autotuner")
+    ap.add_argument("--no-parity", action="store_true", help="cfg3: skip the parity checks")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="cfg3: skip timing the one-mesh-per-launch path")
     ap.add_argument("--copy-probe", action="store_true",
                     help="time a plain device-to-device copy of 1 GiB and a 1 GiB read "
                          "(torch.sum) instead: measured HBM bandwidths, and the reference "
@@ -759,50 +1038,35 @@ def main():
     lib = _lib.load_hip()
     name = ctypes.create_string_buffer(256)
     lib.osdhip_device_info(name, 256)
-    if args.only == "cfg3":
-        print(json.dumps(run_batched(args, lib, name.value.decode())), flush=True)
-        return
-    if args.only == "cfg4":
-        print(json.dumps(run_served(args, lib, name.value.decode())), flush=True)
-        return
-    want = {"cfg1", "cfg1_refine", "cfg2"} if args.only is None else {args.only, "cfg1"}
-    inputs = build_inputs(need_refine="cfg1_refine" in want, need_patches="cfg2" in want)
+    device = name.value.decode()
     with_cpu = not args.no_cpu
-    src = inputs["mesh"].positions.astype(np.float32)
-    workloads = {}
-    workloads["cfg1"] = run_stencils(args, lib, "cfg1 limit stencils", inputs["limit"], src, 3,
-                                     kernel_labels(3, 3), with_cpu)
-    if "cfg1_refine" in want:
-        workloads["cfg1_refine"] = run_stencils(args, lib, "cfg1 refinement", inputs["refine"],
-                                                src, 1, kernel_labels(1, 3), with_cpu)
-    if "cfg2" in want:
-        workloads["cfg2"] = run_patches(args, lib, inputs, with_cpu)
-    if args.only is None:
-        workloads["cfg0"] = run_cube(args, lib, with_cpu)
-    head = workloads["cfg1"]
-    out = {
-        "metric": METRIC,
-        "value": head["points_per_s"],
-        "unit": "points/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": head["ms_per_step"],
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "cfg1: cube-sphere 6x128x128 quads (98,306 verts), 5% creased "
-                               "edges, limit stencils (P, du, dv) at the 6,291,458 uniform "
-                               "level-3 vertex locations, one fused launch per frame",
-                   "stencils": head["stencils"], "table_entries": head["table_entries"],
-                   "floats_per_point": 3, "device": name.value.decode()},
-        "roofline": {"bound": "hbm", "achieved": head["GBps"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": head["GBps"] / HBM_PEAK_GBS,
-                     "traffic": traffic_for("cfg1_limit", head["kernel"].split(" ")[0]),
-                     "kernel": head["kernel"],
-                     "algorithmic_bytes": head["algorithmic_bytes"],
-                     "kernel_ms": head["kernel_ms"]},
-        "cpu_baseline": head.get("cpu_baseline"),
-        "parity": head.get("parity"),
-        "workloads": {k: v for k, v in workloads.items() if k != "cfg1"},
-    }
+    copy = measure_copy()
+    log(f"[bench] 1 GiB device copy: {copy['GBps']:.0f} GB/s (read + write)")
+    if args.only == "cfg4":
+        print(json.dumps(run_served(args, lib, device)), flush=True)
+        return
+    if args.only in ("cfg0", "cfg1", "cfg1_refine", "cfg2"):
+        print(json.dumps(run_single_mesh(args, lib, device, {args.only}, with_cpu)[args.only]),
+              flush=True)
+        return
+    # the headline: cfg3, the batched workload
+    out = run_batched(args, lib, device, copy["GBps"])
+    out["copy_probe"] = copy
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    if args.only == "cfg3":
+        print(json.dumps(out), flush=True)
+        return
+    log("[bench] cfg4 (served)")
+    served = run_served(args, lib, device)
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    out["served"] = dict(served["served"], parity=served["parity"],
+                         value_points_per_s=served["value"])
+    out["workloads"] = {"cfg4": served}
+    log("[bench] cfg1 / cfg1_refine / cfg2 / cfg0")
+    out["workloads"].update(run_single_mesh(args, lib, device,
+                                            {"cfg0", "cfg1", "cfg1_refine", "cfg2"}, with_cpu))
     print(json.dumps(out), flush=True)
 
 

@@ -49,14 +49,24 @@ public:
     template <class FAR_STENCIL_TABLE>
     static HipStencilTable *Create(FAR_STENCIL_TABLE const *table, void *deviceContext = NULL)
     {
+        return Create(table, (const osdhip_table_options *)NULL, deviceContext);
+    }
+
+    /// The same with explicit device options (layout, CSR arrays kept,
+    /// kernel; osdhip_table_options in osdhip.h).  NULL options = defaults;
+    /// options out of range give NULL.
+    template <class FAR_STENCIL_TABLE>
+    static HipStencilTable *Create(FAR_STENCIL_TABLE const *table,
+                                   const osdhip_table_options *options, void *deviceContext)
+    {
         std::vector<const float *> w;
         w.push_back(data(table->GetWeights()));
         collectDerivatives(table, w, hip_detail::HasDu<FAR_STENCIL_TABLE>(),
                            hip_detail::HasDuu<FAR_STENCIL_TABLE>());
         const int n = table->GetNumStencils();
-        osdhip_stencil_table *t = osdhip_stencil_table_create(
+        osdhip_stencil_table *t = osdhip_stencil_table_create_ex(
             n, data(table->GetSizes()), table->GetOffsets().empty() ? NULL : data(table->GetOffsets()),
-            data(table->GetControlIndices()), (int)w.size(), w.data(), deviceContext);
+            data(table->GetControlIndices()), (int)w.size(), w.data(), options, deviceContext);
         if (!t && n > 0) return NULL;
         return new HipStencilTable(t, n);
     }

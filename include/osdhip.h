@@ -118,16 +118,56 @@ typedef struct osdhip_stencil_table osdhip_stencil_table;
 /* HOST arrays in; numWeightArrays = 1 (P), 3 (P, du, dv) or 6 (+ duu, duv,
  * dvv); weights[k] = host pointer of array k; offsets may be NULL (then the
  * exclusive prefix sum of sizes).  Returns NULL on invalid input or when a
- * device allocation fails. */
+ * device allocation fails.  Default options (osdhip_table_options_init). */
 osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *sizes,
                                                   const int *offsets, const int *indices,
                                                   int numWeightArrays,
                                                   const float *const *weights, void *stream);
+
+/* How a table is laid out on the device and evaluated.  Everything is fixed
+ * per table (nothing process-wide), so tables built with different options
+ * can be used concurrently.  Results are the same bits whatever the options.
+ *   layout:  0 = one interleaved record {index, weights} per entry in 64-row
+ *            slices (default); 1 = index and weights in separate streams;
+ *            2 = locality-ordered slices: rows grouped by their dominant
+ *            control vertex, each slice's distinct control vertices staged
+ *            in LDS (for multi-frame batches and large refinement tables).
+ *   keepCsr: 1 (default) = the Far CSR arrays stay on the device
+ *            (osdhip_stencil_table_csr, the CSR kernel); 0 = only the slice
+ *            layout is kept (the CSR getters return NULL).
+ *   group:   entries per gather group of the slice kernels: 4, 8 (default)
+ *            or 16 (layout 2 uses 4, or 8 for 8 and 16).
+ *   kernel:  osdhip_stencil_table_eval kernel: 0 = slice layout (default),
+ *            1 = CSR with LDS staging (needs keepCsr). */
+typedef struct {
+    int layout;
+    int keepCsr;
+    int group;
+    int kernel;
+} osdhip_table_options;
+
+void osdhip_table_options_init(osdhip_table_options *options);
+/* osdhip_stencil_table_create with explicit options (NULL = defaults);
+ * NULL also for options out of range. */
+osdhip_stencil_table *osdhip_stencil_table_create_ex(int numStencils, const int *sizes,
+                                                     const int *offsets, const int *indices,
+                                                     int numWeightArrays,
+                                                     const float *const *weights,
+                                                     const osdhip_table_options *options,
+                                                     void *stream);
+/* Changes an evaluation option of one table, "group" or "kernel" (the
+ * layout options are fixed at creation); returns the previous value,
+ * OSDHIP_ERR_ARGS for a value out of range, OSDHIP_ERR_UNSUPPORTED for
+ * another key.  Not to be called while the table is being evaluated. */
+int osdhip_stencil_table_set(osdhip_stencil_table *table, const char *key, int value);
+/* The table's options as built. */
+int osdhip_stencil_table_options(const osdhip_stencil_table *table,
+                                 osdhip_table_options *options);
 void osdhip_stencil_table_destroy(osdhip_stencil_table *table);
 int osdhip_stencil_table_info(const osdhip_stencil_table *table, int *numStencils,
                               long long *numEntries, int *numWeightArrays,
                               long long *deviceBytes);
-/* The table's device layout: layout (see "table_layout" below), number of
+/* The table's device layout: layout (see osdhip_table_options), number of
  * 64-row slices and, for layout 2, the slices whose control vertices are
  * staged in LDS and the total length of their control lists.  No
  * counterpart in the reference: a diagnostic of this library's layouts. */
@@ -170,6 +210,25 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
                         const int *patchIndices,
                         const osdhip_patch_param *params,
                         void *stream);
+/* osdhip_eval_patches evaluated in patch order: the coords are radix-sorted
+ * by patch (in stream-ordered scratch memory, hipMallocAsync) and each
+ * result still goes to its coord's own row.  Same results, bit for bit.
+ * Pays off when many coords share patches in no particular order; for
+ * uniformly random coords on MI355X the sort costs more than it saves.
+ * No counterpart in the reference. */
+int osdhip_eval_patches_sorted(const float *src, osdhip_buffer_desc srcDesc,
+                               float *dst, osdhip_buffer_desc dstDesc,
+                               float *du, osdhip_buffer_desc duDesc,
+                               float *dv, osdhip_buffer_desc dvDesc,
+                               float *duu, osdhip_buffer_desc duuDesc,
+                               float *duv, osdhip_buffer_desc duvDesc,
+                               float *dvv, osdhip_buffer_desc dvvDesc,
+                               int numPatchCoords,
+                               const osdhip_patch_coord *coords,
+                               const osdhip_patch_array *arrays,
+                               const int *patchIndices,
+                               const osdhip_patch_param *params,
+                               void *stream);
 
 /* ---- PatchCoords on the device: Far::PatchMap ------------------------------
  * Far::PatchMap (far/patchMap.h:58-120, far/patchMap.cpp:88-200) held on the
@@ -218,31 +277,6 @@ int osdhip_tess_uniform_quads(int numFaces, const int *ptexFaces, int rate, floa
 int osdhip_limit_normals(const float *dPdu, osdhip_buffer_desc duDesc, const float *dPdv,
                          osdhip_buffer_desc dvDesc, float *normals, osdhip_buffer_desc nDesc,
                          int count, int normalize, void *stream);
-
-/* Tuning knobs; returns the previous value (or an error code).
- *   "stencil_kernel": 0 = slice layout (default), 1 = CSR with LDS staging
- *                     for osdhip_stencil_table_eval.
- *   "table_layout":   slice layout of tables created afterwards: 0 = one
- *                     interleaved record {index, weights} per entry
- *                     (default), 1 = index and weights in separate streams,
- *                     2 = locality-ordered slices: rows grouped by their
- *                     dominant control vertex, each slice's distinct
- *                     control vertices staged in LDS (results identical;
- *                     for multi-frame batches and large refinement tables).
- *   "slice_group":    entries per gather group of the record kernel, 4, 8
- *                     or 16 (default 8; layout 2 uses 4, or 8 for 8 and 16).
- *   "table_csr_mirror": 1 (default) = tables created afterwards keep the Far
- *                     CSR arrays on the device (osdhip_stencil_table_csr,
- *                     the CSR kernel); 0 = only the slice layout is kept
- *                     (the CSR getters then return NULL and the table is
- *                     always evaluated with the slice kernels).
- *   "patch_sort_min": osdhip_eval_patches evaluates coord sets of at least
- *                     this many coords in patch order (a radix sort in
- *                     stream-ordered scratch memory); 0 = never (default:
- *                     on MI355X the sort costs more than the locality it
- *                     buys for uniformly random coords).  Results are
- *                     identical either way. */
-int osdhip_config(const char *key, int value);
 
 /* ---- device memory and streams -------------------------------------------
  * Used by the C++ HipVertexBuffer / HipStencilTable / HipPatchTable

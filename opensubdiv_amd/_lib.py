@@ -32,6 +32,12 @@ def desc(d) -> Desc:
     return Desc(int(d.offset), int(d.length), int(d.stride))
 
 
+class TableOptions(C.Structure):
+    """osdhip_table_options (layout, keepCsr, group, kernel)."""
+    _fields_ = [("layout", C.c_int), ("keepCsr", C.c_int), ("group", C.c_int),
+                ("kernel", C.c_int)]
+
+
 class BatchStencils(C.Structure):
     """osdhip_batch_stencils."""
     _fields_ = [("table", C.c_void_p), ("src", C.c_void_p),
@@ -63,7 +69,13 @@ _HIP_SIGS = {
                                 _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _vp],
     "osdhip_eval_patches": [_vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D,
                             _i, _vp, _vp, _vp, _vp, _vp],
+    "osdhip_eval_patches_sorted": [_vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp,
+                                   _D, _i, _vp, _vp, _vp, _vp, _vp],
     "osdhip_stencil_table_create": [_i, _vp, _vp, _vp, _i, _vp, _vp],
+    "osdhip_stencil_table_create_ex": [_i, _vp, _vp, _vp, _i, _vp, _vp, _vp],
+    "osdhip_table_options_init": [_vp],
+    "osdhip_stencil_table_set": [_vp, C.c_char_p, _i],
+    "osdhip_stencil_table_options": [_vp, _vp],
     "osdhip_stencil_table_destroy": [_vp],
     "osdhip_stencil_table_info": [_vp, _vp, _vp, _vp, _vp],
     "osdhip_stencil_table_csr": [_vp, _vp, _vp, _vp, _vp],
@@ -74,7 +86,6 @@ _HIP_SIGS = {
     "osdhip_patch_map_destroy": [_vp],
     "osdhip_tess_uniform_quads": [_i, _vp, _i, _vp, _vp, _vp, _vp],
     "osdhip_stencil_table_eval": [_vp, _vp, _D, _i, _vp, _vp, _i, _i, _vp],
-    "osdhip_config": [C.c_char_p, _i],
     "osdhip_malloc": [C.POINTER(C.c_void_p), C.c_size_t],
     "osdhip_free": [_vp],
     "osdhip_memcpy_htod": [_vp, _vp, C.c_size_t, _vp],
@@ -122,11 +133,11 @@ def load_hip():
             fn.restype = C.c_char_p
         elif name == "osdhip_batch_epoch_bytes":
             fn.restype = C.c_longlong
-        elif name in ("osdhip_stencil_table_create", "osdhip_batch_create",
-                      "osdhip_patch_map_create"):
+        elif name in ("osdhip_stencil_table_create", "osdhip_stencil_table_create_ex",
+                      "osdhip_batch_create", "osdhip_patch_map_create"):
             fn.restype = C.c_void_p
         elif name in ("osdhip_stencil_table_destroy", "osdhip_batch_destroy",
-                      "osdhip_patch_map_destroy"):
+                      "osdhip_patch_map_destroy", "osdhip_table_options_init"):
             fn.restype = None
         else:
             fn.restype = C.c_int

@@ -37,14 +37,6 @@
 
 namespace osdhip {
 
-// Tuning knobs (osdhip_config).
-static int g_stencil_kernel = 0;     // table eval: 0 sliced, 1 CSR
-static int g_patch_sort_min = 0;     // EvalPatches: patch-sort coords when n >= this (0 = never)
-static int g_table_layout = 0;       // new tables: 0 record-interleaved slices, 1 split streams,
-                                     // 2 locality-ordered slices
-static int g_csr_mirror = 1;         // new tables keep the Far CSR arrays on the device
-static int g_slice_group = 8;        // record kernel: entries per gather group (4, 8, 16)
-
 template <int NWT, int G>
 static void launch_rec_g(int lc, const RecArgs &a, int grid, hipStream_t st)
 {
@@ -386,11 +378,42 @@ int osdhip_eval_stencils_d2(const float *src, osdhip_buffer_desc srcDesc,
     return eval_csr<6>(src, srcDesc, outs, ods, sizes, offsets, indices, ws, start, end, stream);
 }
 
+void osdhip_table_options_init(osdhip_table_options *o)
+{
+    if (!o) return;
+    o->layout = 0;
+    o->keepCsr = 1;
+    o->group = 8;
+    o->kernel = 0;
+}
+
+static bool table_options_ok(const osdhip_table_options &o)
+{
+    return o.layout >= 0 && o.layout <= 2 && (o.keepCsr == 0 || o.keepCsr == 1) &&
+           (o.group == 4 || o.group == 8 || o.group == 16) && (o.kernel == 0 || o.kernel == 1) &&
+           !(o.kernel == 1 && !o.keepCsr);
+}
+
 osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *sizes,
                                                   const int *offsets, const int *indices,
                                                   int numWeightArrays,
                                                   const float *const *weights, void *stream)
 {
+    return osdhip_stencil_table_create_ex(numStencils, sizes, offsets, indices, numWeightArrays,
+                                          weights, nullptr, stream);
+}
+
+osdhip_stencil_table *osdhip_stencil_table_create_ex(int numStencils, const int *sizes,
+                                                     const int *offsets, const int *indices,
+                                                     int numWeightArrays,
+                                                     const float *const *weights,
+                                                     const osdhip_table_options *options,
+                                                     void *stream)
+{
+    osdhip_table_options opt;
+    osdhip_table_options_init(&opt);
+    if (options) opt = *options;
+    if (!table_options_ok(opt)) return nullptr;
     if (numStencils < 0 || (numStencils > 0 && (!sizes || !indices || !weights)) ||
         (numWeightArrays != 1 && numWeightArrays != 3 && numWeightArrays != 6))
         return nullptr;
@@ -400,6 +423,9 @@ osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *si
     if (!t) return nullptr;
     t->n = numStencils;
     t->nw = numWeightArrays;
+    t->keepCsr = opt.keepCsr;
+    t->group = opt.group;
+    t->kernel = opt.kernel;
     hipStream_t st = (hipStream_t)stream;
 
     // CSR mirror (offsets regenerated when absent: exclusive prefix of sizes)
@@ -425,7 +451,7 @@ osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *si
     const int S = (numStencils + kWave - 1) / kWave;
     const int rows = S * kWave;
     t->nslices = S;
-    t->layout = g_table_layout;
+    t->layout = opt.layout;
     long long *entries = nullptr;
     void *scanTmp = nullptr;
     size_t scanBytes = 0;
@@ -489,7 +515,7 @@ osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *si
     if (scanTmp) (void)hipFree(scanTmp);
     if (ok) ok = hipStreamSynchronize(st) == hipSuccess;
     if (!ok) { free_table(t); return nullptr; }
-    if (!g_csr_mirror) {
+    if (!t->keepCsr) {
         // slice layout only: the CSR arrays were needed to build it
         void *csr[] = { t->offsets, t->indices, t->w[0], t->w[1], t->w[2], t->w[3], t->w[4], t->w[5] };
         for (void *p : csr)
@@ -502,6 +528,31 @@ osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *si
 }
 
 void osdhip_stencil_table_destroy(osdhip_stencil_table *t) { free_table(t); }
+
+int osdhip_stencil_table_set(osdhip_stencil_table *t, const char *key, int value)
+{
+    if (!t || !key) return OSDHIP_ERR_ARGS;
+    osdhip_table_options o;
+    if (osdhip_stencil_table_options(t, &o) != OSDHIP_OK) return OSDHIP_ERR_ARGS;
+    int *field = nullptr;
+    if (!std::strcmp(key, "group")) { o.group = value; field = &t->group; }
+    else if (!std::strcmp(key, "kernel")) { o.kernel = value; field = &t->kernel; }
+    else return OSDHIP_ERR_UNSUPPORTED;
+    if (!table_options_ok(o) || (o.kernel == 1 && !t->indices)) return OSDHIP_ERR_ARGS;
+    const int old = *field;
+    *field = value;
+    return old;
+}
+
+int osdhip_stencil_table_options(const osdhip_stencil_table *t, osdhip_table_options *o)
+{
+    if (!t || !o) return OSDHIP_ERR_ARGS;
+    o->layout = t->layout;
+    o->keepCsr = t->keepCsr;
+    o->group = t->group;
+    o->kernel = t->kernel;
+    return OSDHIP_OK;
+}
 
 int osdhip_stencil_table_info(const osdhip_stencil_table *t, int *numStencils,
                               long long *numEntries, int *numWeightArrays, long long *deviceBytes)
@@ -549,7 +600,7 @@ int osdhip_stencil_table_eval(const osdhip_stencil_table *t, const float *src,
     if (end <= start) return OSDHIP_OK;
     if (!src || start < 0 || !outputs_ok(srcDesc, numOutputs, outputs, outputDescs, nullptr))
         return OSDHIP_ERR_ARGS;
-    if (g_stencil_kernel == 1 && t->indices) {
+    if (t->kernel == 1 && t->indices) {
         const float *ws[6] = { t->w[0], t->w[1], t->w[2], t->w[3], t->w[4], t->w[5] };
         if (numOutputs == 1)
             return eval_csr<1>(src, srcDesc, outputs, outputDescs, t->sizes, t->offsets, t->indices,
@@ -590,7 +641,7 @@ int osdhip_stencil_table_eval(const osdhip_stencil_table *t, const float *src,
                 a.dst[k] = outputs[k] ? outputs[k] + outputDescs[k].offset + e0 : nullptr;
                 a.dstStride[k] = outputs[k] ? outputDescs[k].stride : 0;
             }
-            const int g = g_slice_group == 4 ? 4 : 8;
+            const int g = t->group == 4 ? 4 : 8;
             if (t->nw == 1) launch_tiled<1>(lc, g, a, tgrid, st);
             else if (t->nw == 3) launch_tiled<3>(lc, g, a, tgrid, st);
             else launch_tiled<6>(lc, g, a, tgrid, st);
@@ -621,9 +672,9 @@ int osdhip_stencil_table_eval(const osdhip_stencil_table *t, const float *src,
             }
             // the kernel computes every weight array of the table; only the
             // requested outputs are written
-            if (t->nw == 1) launch_rec<1>(lc, g_slice_group, a, grid, st);
-            else if (t->nw == 3) launch_rec<3>(lc, g_slice_group, a, grid, st);
-            else launch_rec<6>(lc, g_slice_group, a, grid, st);
+            if (t->nw == 1) launch_rec<1>(lc, t->group, a, grid, st);
+            else if (t->nw == 3) launch_rec<3>(lc, t->group, a, grid, st);
+            else launch_rec<6>(lc, t->group, a, grid, st);
             int rc = last_error();
             if (rc != OSDHIP_OK) return rc;
         }
@@ -658,23 +709,16 @@ int osdhip_stencil_table_eval(const osdhip_stencil_table *t, const float *src,
     return OSDHIP_OK;
 }
 
-int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
-                        float *dst, osdhip_buffer_desc dstDesc,
-                        float *du, osdhip_buffer_desc duDesc,
-                        float *dv, osdhip_buffer_desc dvDesc,
-                        float *duu, osdhip_buffer_desc duuDesc,
-                        float *duv, osdhip_buffer_desc duvDesc,
-                        float *dvv, osdhip_buffer_desc dvvDesc,
-                        int numPatchCoords,
-                        const osdhip_patch_coord *coords,
-                        const osdhip_patch_array *arrays,
-                        const int *patchIndices,
-                        const osdhip_patch_param *params,
+static int eval_patches(const float *src, osdhip_buffer_desc srcDesc, float *const *outs,
+                        const osdhip_buffer_desc *odsIn, int numPatchCoords,
+                        const osdhip_patch_coord *coords, const osdhip_patch_array *arrays,
+                        const int *patchIndices, const osdhip_patch_param *params, bool sorted,
                         void *stream)
 {
+    float *dst = outs[0], *du = outs[1], *dv = outs[2], *duu = outs[3], *duv = outs[4],
+          *dvv = outs[5];
+    osdhip_buffer_desc ods[6] = { odsIn[0], odsIn[1], odsIn[2], odsIn[3], odsIn[4], odsIn[5] };
     if (!src) return OSDHIP_ERR_ARGS;
-    float *outs[6] = { dst, du, dv, duu, duv, dvv };
-    osdhip_buffer_desc ods[6] = { dstDesc, duDesc, dvDesc, duuDesc, duvDesc, dvvDesc };
     if (!outputs_ok(srcDesc, 6, outs, ods, nullptr)) return OSDHIP_ERR_ARGS;
     if (numPatchCoords <= 0) return OSDHIP_OK;
     if (!coords || !arrays || !patchIndices || !params) return OSDHIP_ERR_ARGS;
@@ -695,7 +739,7 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
     // coord index) pairs in stream-ordered scratch memory, then evaluate in
     // that order.  Outputs still go to each coord's own row.
     void *ws = nullptr;
-    if (g_patch_sort_min > 0 && numPatchCoords >= g_patch_sort_min) {
+    if (sorted) {
         const int n = numPatchCoords;
         size_t tmpBytes = 0;
         hipcub::DoubleBuffer<unsigned> kb(nullptr, nullptr);
@@ -734,6 +778,46 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
         if (rc == OSDHIP_OK) rc = rf;
     }
     return rc;
+}
+
+int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
+                        float *dst, osdhip_buffer_desc dstDesc,
+                        float *du, osdhip_buffer_desc duDesc,
+                        float *dv, osdhip_buffer_desc dvDesc,
+                        float *duu, osdhip_buffer_desc duuDesc,
+                        float *duv, osdhip_buffer_desc duvDesc,
+                        float *dvv, osdhip_buffer_desc dvvDesc,
+                        int numPatchCoords,
+                        const osdhip_patch_coord *coords,
+                        const osdhip_patch_array *arrays,
+                        const int *patchIndices,
+                        const osdhip_patch_param *params,
+                        void *stream)
+{
+    float *outs[6] = { dst, du, dv, duu, duv, dvv };
+    const osdhip_buffer_desc ods[6] = { dstDesc, duDesc, dvDesc, duuDesc, duvDesc, dvvDesc };
+    return eval_patches(src, srcDesc, outs, ods, numPatchCoords, coords, arrays, patchIndices,
+                        params, false, stream);
+}
+
+int osdhip_eval_patches_sorted(const float *src, osdhip_buffer_desc srcDesc,
+                               float *dst, osdhip_buffer_desc dstDesc,
+                               float *du, osdhip_buffer_desc duDesc,
+                               float *dv, osdhip_buffer_desc dvDesc,
+                               float *duu, osdhip_buffer_desc duuDesc,
+                               float *duv, osdhip_buffer_desc duvDesc,
+                               float *dvv, osdhip_buffer_desc dvvDesc,
+                               int numPatchCoords,
+                               const osdhip_patch_coord *coords,
+                               const osdhip_patch_array *arrays,
+                               const int *patchIndices,
+                               const osdhip_patch_param *params,
+                               void *stream)
+{
+    float *outs[6] = { dst, du, dv, duu, duv, dvv };
+    const osdhip_buffer_desc ods[6] = { dstDesc, duDesc, dvDesc, duuDesc, duvDesc, dvvDesc };
+    return eval_patches(src, srcDesc, outs, ods, numPatchCoords, coords, arrays, patchIndices,
+                        params, true, stream);
 }
 
 struct osdhip_patch_map {
@@ -907,42 +991,6 @@ int osdhip_limit_normals(const float *dPdu, osdhip_buffer_desc duDesc, const flo
                        duDesc.offset, duDesc.stride, dPdv, dvDesc.offset, dvDesc.stride, normals,
                        nDesc.offset, nDesc.stride, count, normalize ? 1 : 0);
     return last_error();
-}
-
-int osdhip_config(const char *key, int value)
-{
-    if (!key) return OSDHIP_ERR_ARGS;
-    if (std::strcmp(key, "stencil_kernel") == 0) {
-        if (value != 0 && value != 1) return OSDHIP_ERR_ARGS;
-        int old = g_stencil_kernel;
-        g_stencil_kernel = value;
-        return old;
-    }
-    if (std::strcmp(key, "table_layout") == 0) {
-        if (value < 0 || value > 2) return OSDHIP_ERR_ARGS;
-        int old = g_table_layout;
-        g_table_layout = value;
-        return old;
-    }
-    if (std::strcmp(key, "table_csr_mirror") == 0) {
-        if (value != 0 && value != 1) return OSDHIP_ERR_ARGS;
-        int old = g_csr_mirror;
-        g_csr_mirror = value;
-        return old;
-    }
-    if (std::strcmp(key, "slice_group") == 0) {
-        if (value != 4 && value != 8 && value != 16) return OSDHIP_ERR_ARGS;
-        int old = g_slice_group;
-        g_slice_group = value;
-        return old;
-    }
-    if (std::strcmp(key, "patch_sort_min") == 0) {
-        if (value < 0) return OSDHIP_ERR_ARGS;
-        int old = g_patch_sort_min;
-        g_patch_sort_min = value;
-        return old;
-    }
-    return OSDHIP_ERR_UNSUPPORTED;
 }
 
 int osdhip_malloc(void **ptr, size_t bytes)

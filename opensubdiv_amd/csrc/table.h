@@ -9,6 +9,9 @@
 // the device from the CSR arrays.
 struct osdhip_stencil_table {
     int n = 0, nw = 0, nslices = 0, layout = 0;
+    int keepCsr = 1;                 // the Far CSR arrays stay on the device
+    int group = 8;                   // slice kernels: entries per gather group
+    int kernel = 0;                  // table eval: 0 slice layout, 1 CSR
     long long nnz = 0, sliceEntries = 0, bytes = 0;
     int *sizes = nullptr, *offsets = nullptr, *indices = nullptr;
     float *w[6] = {};

@@ -116,22 +116,29 @@ class HipStencilTable:
 
     Accepts any object with GetSizes/GetOffsets/GetControlIndices/GetWeights
     (and optionally GetDuWeights ... GetDvvWeights) returning array-likes, or
-    a dict with keys sizes/offsets/indices/weights[/du/dv/duu/duv/dvv]."""
+    a dict with keys sizes/offsets/indices/weights[/du/dv/duu/duv/dvv].
+
+    Device options (osdhip_table_options, fixed per table): ``layout`` 0
+    (record slices), 1 (split streams) or 2 (locality-ordered slices staged
+    in LDS); ``keep_csr`` keeps the Far CSR arrays on the device; ``group``
+    (4, 8, 16) and ``kernel`` (0 slices, 1 CSR) select the evaluation
+    kernel and can be changed later with ``set``."""
 
     _KEYS = ["weights", "du", "dv", "duu", "duv", "dvv"]
     _GETTERS = ["GetWeights", "GetDuWeights", "GetDvWeights", "GetDuuWeights",
                 "GetDuvWeights", "GetDvvWeights"]
 
-    def __init__(self, table, stream=None):
+    def __init__(self, table, stream=None, layout=0, keep_csr=True, group=8, kernel=0):
         import ctypes
+        opts = _lib.TableOptions(int(layout), int(bool(keep_csr)), int(group), int(kernel))
         if hasattr(table, "weights_ptr"):            # far.RawStencilTable
             lib = _lib.load_hip()
             wptrs = (ctypes.c_void_p * 1)(table.weights_ptr)
             self._n, self._nw = int(table.num_stencils), 1
-            self.h = lib.osdhip_stencil_table_create(self._n, table.sizes_ptr, None,
-                                                     table.indices_ptr, 1,
-                                                     ctypes.cast(wptrs, ctypes.c_void_p),
-                                                     _stream(stream))
+            self.h = lib.osdhip_stencil_table_create_ex(self._n, table.sizes_ptr, None,
+                                                        table.indices_ptr, 1,
+                                                        ctypes.cast(wptrs, ctypes.c_void_p),
+                                                        ctypes.byref(opts), _stream(stream))
             if not self.h:
                 raise MemoryError("osdhip_stencil_table_create failed")
             self._csr_pointers(lib)
@@ -157,9 +164,10 @@ class HipStencilTable:
         wptrs = (ctypes.c_void_p * nw)(*[w.ctypes.data for w in ws])
         self._n = int(sizes.shape[0])
         self._nw = nw
-        self.h = lib.osdhip_stencil_table_create(
+        self.h = lib.osdhip_stencil_table_create_ex(
             self._n, sizes.ctypes.data, None if offsets is None else offsets.ctypes.data,
-            indices.ctypes.data, nw, ctypes.cast(wptrs, ctypes.c_void_p), _stream(stream))
+            indices.ctypes.data, nw, ctypes.cast(wptrs, ctypes.c_void_p), ctypes.byref(opts),
+            _stream(stream))
         if not self.h:
             raise MemoryError("osdhip_stencil_table_create failed")
         self._csr_pointers(lib)
@@ -174,8 +182,24 @@ class HipStencilTable:
         self._w = list(w6)
 
     @classmethod
-    def Create(cls, table, deviceContext=None):
-        return cls(table, deviceContext)
+    def Create(cls, table, deviceContext=None, **options):
+        return cls(table, deviceContext, **options)
+
+    def set(self, key, value):
+        """Changes the evaluation option "group" or "kernel" of this table;
+        returns the previous value."""
+        rc = _lib.load_hip().osdhip_stencil_table_set(self.h, key.encode(), int(value))
+        if rc < 0:
+            raise ValueError(f"osdhip_stencil_table_set({key!r}, {value}) failed: {rc}")
+        return rc
+
+    def options(self):
+        import ctypes
+        o = _lib.TableOptions()
+        _lib.check(_lib.load_hip().osdhip_stencil_table_options(self.h, ctypes.byref(o)),
+                   "osdhip_stencil_table_options")
+        return {"layout": o.layout, "keep_csr": bool(o.keepCsr), "group": o.group,
+                "kernel": o.kernel}
 
     def GetSizesBuffer(self): return self._csr[0]
     def GetOffsetsBuffer(self): return self._csr[1]

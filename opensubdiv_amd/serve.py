@@ -297,15 +297,18 @@ def poisson_requests(n, rate, ntopo, seed, limit_fraction=0.5, nframes=8):
     return reqs
 
 
-def residents_from_hosts(hosts, stream=None):
+def residents_from_hosts(hosts, stream=None, layout=2, keep_csr=False):
     """Resident topologies from compiled workloads.HostMesh objects (their
-    `frames` become the frame pool requests cycle through)."""
+    `frames` become the frame pool requests cycle through); stencil tables
+    in slice layout `layout` (0 or 2), with or without their CSR arrays."""
     from .osd import HipStencilTable, _device_tensor
     out = []
     for h in hosts:
+        opts = dict(layout=layout, keep_csr=keep_csr)
         out.append(Resident(h.spec.index, h.nctrl, int(len(h.points["sizes"])),
                             int(h.refine.num_stencils), int(len(h.coords)),
-                            HipStencilTable(h.refine, stream), HipStencilTable(h.points, stream),
+                            HipStencilTable(h.refine, stream, **opts),
+                            HipStencilTable(h.points, stream, **opts),
                             _device_tensor(h.coords), _device_tensor(h.arrays),
                             _device_tensor(h.indices), _device_tensor(h.params), h.frames))
     return out

@@ -47,6 +47,7 @@ def lib():
             "ref_cpu_eval_stencils_d1": ([vp] * 14 + [i, i], i),
             "ref_cpu_eval_stencils_d2": ([vp] * 23 + [i, i], i),
             "ref_omp_eval_stencils": ([vp, vp, vp, vp, vp, vp, vp, vp, i, i, i], i),
+            "ref_omp_eval_stencils_d1": ([vp] * 14 + [i, i, i], i),
             "ref_cpu_eval_patches": ([vp] * 8 + [i, vp, vp, vp, vp], i),
             "ref_cpu_eval_patches_d2": ([vp] * 14 + [i, vp, vp, vp, vp], i),
             "ref_omp_eval_patches": ([vp] * 8 + [i, vp, vp, vp, vp, i], i),
@@ -203,7 +204,8 @@ def cpu_eval_stencils(src, sd, dst, dd, table, start=0, end=None, threads=0):
     return bool(lib().ref_cpu_eval_stencils(*args))
 
 
-def cpu_eval_stencils_d1(src, sd, outs, descs, table, start=0, end=None):
+def cpu_eval_stencils_d1(src, sd, outs, descs, table, start=0, end=None, threads=0):
+    """CpuEvaluator::EvalStencils with du/dv, or OmpEvaluator's (threads > 0)."""
     end = len(table["sizes"]) if end is None else end
     a = []
     keep = []
@@ -212,10 +214,12 @@ def cpu_eval_stencils_d1(src, sd, outs, descs, table, start=0, end=None):
         keep.append(dd)
         a += [_p(o), _p(dd)]
     sdd = _desc(sd)
-    return bool(lib().ref_cpu_eval_stencils_d1(_p(src), _p(sdd), *a, _p(table["sizes"]),
-                                               _p(table["offsets"]), _p(table["indices"]),
-                                               _p(table["weights"]), _p(table["du"]),
-                                               _p(table["dv"]), start, end))
+    args = (_p(src), _p(sdd), *a, _p(table["sizes"]), _p(table["offsets"]),
+            _p(table["indices"]), _p(table["weights"]), _p(table["du"]), _p(table["dv"]),
+            start, end)
+    if threads:
+        return bool(lib().ref_omp_eval_stencils_d1(*args, threads))
+    return bool(lib().ref_cpu_eval_stencils_d1(*args))
 
 
 def cpu_eval_stencils_d2(src, sd, outs, descs, table, start=0, end=None):

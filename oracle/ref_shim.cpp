@@ -430,6 +430,17 @@ int ref_omp_eval_stencils(const float *src, const int *sd, float *dst, const int
                                            indices, weights, start, end);
 }
 
+int ref_omp_eval_stencils_d1(const float *src, const int *sd, float *dst, const int *dd,
+                             float *du, const int *dud, float *dv, const int *dvd,
+                             const int *sizes, const int *offsets, const int *indices,
+                             const float *w, const float *duw, const float *dvw,
+                             int start, int end, int nthreads)
+{
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    return Osd::OmpEvaluator::EvalStencils(src, D(sd), dst, D(dd), du, D(dud), dv, D(dvd),
+                                           sizes, offsets, indices, w, duw, dvw, start, end);
+}
+
 int ref_cpu_eval_patches(const float *src, const int *sd, float *dst, const int *dd,
                          float *du, const int *dud, float *dv, const int *dvd,
                          int n, const void *coords, const void *arrays,

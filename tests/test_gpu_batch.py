@@ -18,25 +18,21 @@ from opensubdiv_amd.workloads import DeviceMesh, MeshSpec, compile_mesh
 pytestmark = pytest.mark.gpu
 
 # small meshes: n x n quads per cube face, refinement / isolation level,
-# frames, coords -- Gregory end caps at the valence-3 corners; the last one
-# has fewer coords than patches, so its waves of coords reference too many
-# points to be staged in LDS
+# frames, coords -- Gregory end caps at the valence-3 corners; mesh 5 has
+# fewer coords than patches, so its waves of coords reference too many
+# points to be staged in LDS; mesh 6 is refined to level 4, whose rows are
+# the longest of the batched workload (16 entries, the most the tiled kernel
+# keeps in registers) and whose slices list the most control vertices
 SPECS = [MeshSpec(0, 3, 2, 3, 700, 11), MeshSpec(1, 2, 1, 1, 300, 12),
          MeshSpec(2, 5, 2, 5, 2500, 13), MeshSpec(3, 4, 3, 2, 1800, 14),
-         MeshSpec(4, 2, 2, 4, 64, 15), MeshSpec(5, 12, 2, 2, 700, 16)]
+         MeshSpec(4, 2, 2, 4, 64, 15), MeshSpec(5, 12, 2, 2, 700, 16),
+         MeshSpec(6, 6, 4, 3, 2000, 17)]
 
 
 def build(length, layout=0):
     import torch
-    lib = _lib.load_hip()
-    old = lib.osdhip_config(b"table_csr_mirror", 0)
-    old_layout = lib.osdhip_config(b"table_layout", layout)
-    try:
-        hosts = [compile_mesh(s, length) for s in SPECS]
-        dms = [DeviceMesh(h) for h in hosts]
-    finally:
-        lib.osdhip_config(b"table_csr_mirror", old)
-        lib.osdhip_config(b"table_layout", old_layout)
+    hosts = [compile_mesh(s, length) for s in SPECS]
+    dms = [DeviceMesh(h, layout=layout) for h in hosts]
     for h in hosts:
         h.release()
     torch.cuda.synchronize()
@@ -265,14 +261,9 @@ def test_batch_knob_and_segment_errors(workload):
     with pytest.raises(ValueError):
         Batch(length, [seg], [])
     # tables of both slice layouts in one batch
-    lib = _lib.load_hip()
-    old = lib.osdhip_config(b"table_layout", 2 - batch.layout)
-    try:
-        t1 = {"sizes": t["sizes"], "offsets": t["offsets"], "indices": t["indices"],
-              "weights": t["weights"]}
-        other = HipStencilTable(t1)
-    finally:
-        lib.osdhip_config(b"table_layout", old)
+    t1 = {"sizes": t["sizes"], "offsets": t["offsets"], "indices": t["indices"],
+          "weights": t["weights"]}
+    other = HipStencilTable(t1, layout=2 - batch.layout)
     seg2 = dict(seg)
     seg2["table"] = other
     with pytest.raises(ValueError):
@@ -287,12 +278,8 @@ def test_staged_outputs_share_an_arena():
     import torch
     from opensubdiv_amd.workloads import stage_outputs
     length = 7
-    old = _lib.load_hip().osdhip_config(b"table_csr_mirror", 0)
-    try:
-        hosts = [compile_mesh(s, length) for s in SPECS]
-        dms = [DeviceMesh(h, own_outputs=False) for h in hosts]
-    finally:
-        _lib.load_hip().osdhip_config(b"table_csr_mirror", old)
+    hosts = [compile_mesh(s, length) for s in SPECS]
+    dms = [DeviceMesh(h, own_outputs=False, layout=0) for h in hosts]
     need = [d.output_floats() for d in dms]
     arena, nstages = stage_outputs(dms, max(need) + 64)
     assert nstages >= 3 and arena.numel() < sum(need)
@@ -328,19 +315,12 @@ def test_meshes_loaded_from_the_table_cache_match_compiled(tmp_path):
     from opensubdiv_amd.table_cache import TableCache
     length = 7
     cache = TableCache(tmp_path)
-    lib = _lib.load_hip()
-    old = lib.osdhip_config(b"table_csr_mirror", 0)
-    old_layout = lib.osdhip_config(b"table_layout", 2)
-    try:
-        for s in SPECS:
-            compile_mesh(s, length, cache=cache).release()
-        assert (cache.hits, cache.misses) == (0, len(SPECS))
-        cached = [compile_mesh(s, length, cache=cache) for s in SPECS]
-        assert cache.hits == len(SPECS)
-        dms = [DeviceMesh(h) for h in cached]
-    finally:
-        lib.osdhip_config(b"table_csr_mirror", old)
-        lib.osdhip_config(b"table_layout", old_layout)
+    for s in SPECS:
+        compile_mesh(s, length, cache=cache).release()
+    assert (cache.hits, cache.misses) == (0, len(SPECS))
+    cached = [compile_mesh(s, length, cache=cache) for s in SPECS]
+    assert cache.hits == len(SPECS)
+    dms = [DeviceMesh(h, layout=2) for h in cached]
     for h in cached:
         h.release()
     _, fresh = build(length, 2)

@@ -15,15 +15,16 @@ D = _lib.Desc
 NODESC = D(0, 0, 0)
 
 
-def eval_patches(lib, src, sd, outs, descs, n, coords, arrays, indices, params):
+def eval_patches(lib, src, sd, outs, descs, n, coords, arrays, indices, params, sorted=False):
     args = []
     for k in range(6):
         if k < len(outs) and outs[k] is not None:
             args += [ptr(outs[k]), D(*descs[k])]
         else:
             args += [None, NODESC]
-    return lib.osdhip_eval_patches(ptr(src), D(*sd), *args, n, ptr(coords), ptr(arrays),
-                                   ptr(indices), ptr(params), stream())
+    fn = lib.osdhip_eval_patches_sorted if sorted else lib.osdhip_eval_patches
+    return fn(ptr(src), D(*sd), *args, n, ptr(coords), ptr(arrays), ptr(indices), ptr(params),
+              stream())
 
 
 def golden_patches():
@@ -191,35 +192,29 @@ def test_python_mirror_eval_patches(hip):
 
 @pytest.mark.parametrize("deriv", [0, 1, 2])
 def test_patch_sorted_order_bit_exact(hip, deriv):
-    """Patch-ordered evaluation (radix sort of the coords by patch) returns
-    every coord's result in its own row, bit-identical to caller order and to
-    the oracle; forced on a small set and by the default threshold on a
-    large one (coords drawn with repeats and in random order)."""
+    """Patch-ordered evaluation (osdhip_eval_patches_sorted: radix sort of
+    the coords by patch) returns every coord's result in its own row,
+    bit-identical to caller order and to the oracle, on a small and a large
+    set (coords drawn with repeats and in random order)."""
     g, src, _, arrays, indices, params = golden_patches()
     rng = np.random.default_rng(77 + deriv)
     base = g["coords"]
     nout = {0: 1, 1: 3, 2: 6}[deriv]
-    old = hip.osdhip_config(b"patch_sort_min", 16384)
-    try:
-        for n, smin in ((1000, 1), (70000, 16384)):
-            coords = base[rng.integers(0, len(base), size=n)]
-            want = [np.zeros((n, 3), np.float32) for _ in range(nout)]
-            assert oracle.eval_patches(g["src"], (0, 3, 3), want, [(0, 3, 3)] * nout, coords,
-                                       g["arrays"], g["indices"], g["params"])
-            dc = dev(coords)
-            res = {}
-            for mode in (smin, 0):
-                assert hip.osdhip_config(b"patch_sort_min", mode) >= 0
-                outs = [empty(n * 3) for _ in range(nout)]
-                assert eval_patches(hip, src, (0, 3, 3), outs, [(0, 3, 3)] * nout, n, dc,
-                                    arrays, indices, params) == 0
-                res[mode] = [host(o, shape=(n, 3)) for o in outs]
-            for k in range(nout):
-                np.testing.assert_array_equal(res[smin][k], want[k], err_msg=f"sorted {n} {k}")
-                np.testing.assert_array_equal(res[0][k], want[k], err_msg=f"caller {n} {k}")
-        assert hip.osdhip_config(b"patch_sort_min", -1) == _lib.ERR_ARGS
-    finally:
-        hip.osdhip_config(b"patch_sort_min", old)
+    for n in (1000, 70000):
+        coords = base[rng.integers(0, len(base), size=n)]
+        want = [np.zeros((n, 3), np.float32) for _ in range(nout)]
+        assert oracle.eval_patches(g["src"], (0, 3, 3), want, [(0, 3, 3)] * nout, coords,
+                                   g["arrays"], g["indices"], g["params"])
+        dc = dev(coords)
+        res = {}
+        for srt in (True, False):
+            outs = [empty(n * 3) for _ in range(nout)]
+            assert eval_patches(hip, src, (0, 3, 3), outs, [(0, 3, 3)] * nout, n, dc,
+                                arrays, indices, params, sorted=srt) == 0
+            res[srt] = [host(o, shape=(n, 3)) for o in outs]
+        for k in range(nout):
+            np.testing.assert_array_equal(res[True][k], want[k], err_msg=f"sorted {n} {k}")
+            np.testing.assert_array_equal(res[False][k], want[k], err_msg=f"caller {n} {k}")
 
 
 def normals(lib, du, dd, dv, vd, n, nd, count, normalize):

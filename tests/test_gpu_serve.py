@@ -10,20 +10,16 @@ from opensubdiv_amd.workloads import MeshSpec, compile_mesh
 
 pytestmark = pytest.mark.gpu
 
+# the last topology is refined to level 4 (the widest stencil slices)
 SPECS = [MeshSpec(0, 3, 2, 4, 900, 21), MeshSpec(1, 2, 3, 4, 400, 22),
-         MeshSpec(2, 4, 2, 4, 1500, 23)]
+         MeshSpec(2, 4, 2, 4, 1500, 23), MeshSpec(3, 4, 4, 2, 2000, 24)]
 L = 7
 
 
 @pytest.fixture(scope="module", params=[0, 2], ids=["rec", "tiled"])
 def residents(request):
     hosts = [compile_mesh(s, L) for s in SPECS]
-    lib = _lib.load_hip()
-    old = lib.osdhip_config(b"table_layout", request.param)
-    try:
-        res = residents_from_hosts(hosts)
-    finally:
-        lib.osdhip_config(b"table_layout", old)
+    res = residents_from_hosts(hosts, layout=request.param)
     yield res
     for h in hosts:
         h.release()
@@ -128,3 +124,47 @@ def test_epoch_cap_splits_the_burst_with_same_results(residents):
         assert n == 1 or pts <= 3000
     for r in reqs2:
         assert np.array_equal(want[r.rid].view(np.uint32), got[r.rid].view(np.uint32))
+
+
+def test_level4_requests_match_the_oracle(residents):
+    """Both request kinds on the level-4 topology, served in a mix with the
+    others, against the oracle's CpuEvaluator-order sums on the same frame:
+    refinement rows, and P / dPdu / dPdv at the topology's coords (patch
+    points refined from the same frame first)."""
+    import oracle
+    from opensubdiv_amd import far, meshes
+    t = len(SPECS) - 1
+    reqs = make_requests()
+    server = Server(residents, L, device_budget=64 << 20, host_budget=64 << 20, slots=3)
+    done, mix = server.run(reqs, keep={r.rid for r in reqs}, max_seconds=60)
+    assert len(done) == len(reqs)
+    h = compile_mesh(SPECS[t], L)
+    try:
+        mesh = meshes.cube_sphere(SPECS[t].n, seed=SPECS[t].seed, radial_noise=0.01,
+                                  crease_fraction=0.05, sharpness_max=4.0)
+        topo = far.Topology(mesh)
+        topo.refine_uniform(SPECS[t].level)
+        ref = topo.stencil_table()
+        checked = set()
+        for r in reqs:
+            if r.topo != t:
+                continue
+            ctrl = np.ascontiguousarray(residents[t].frames[r.frame % residents[t].frames.shape[0]])
+            if r.kind == "refine":
+                want = oracle.apply_stencils(ctrl, ref["sizes"], ref["offsets"], ref["indices"],
+                                             ref["weights"]).reshape(-1)
+            else:
+                pts = oracle.apply_stencils(ctrl, h.points["sizes"], h.points["offsets"],
+                                            h.points["indices"], h.points["weights"])
+                src = np.ascontiguousarray(np.concatenate([ctrl, pts]))
+                n = len(h.coords)
+                outs = [np.zeros((n, L), np.float32) for _ in range(3)]
+                assert oracle.eval_patches(src, (0, L, L), outs, [(0, L, L)] * 3, h.coords,
+                                           h.arrays, h.indices, h.params)
+                want = np.concatenate([o.reshape(-1) for o in outs])
+            assert np.array_equal(mix[r.rid].view(np.uint32), want.view(np.uint32)), \
+                f"request {r.rid} {r.kind}"
+            checked.add(r.kind)
+        assert checked == {"refine", "limit"}
+    finally:
+        h.release()

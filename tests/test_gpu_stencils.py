@@ -235,39 +235,52 @@ def test_device_table_bit_exact(hip, variant, nw, L):
     nsrc, n = 600, 4099
     t = random_table(rng, n, nsrc, 18, nw)
     keys = ("weights", "du", "dv", "duu", "duv", "dvv")[:nw]
-    old = [hip.osdhip_config(b"table_layout", layout), hip.osdhip_config(b"slice_group", group),
-           hip.osdhip_config(b"stencil_kernel", kernel)]
-    try:
-        table = HipStencilTable.Create(t)
-        assert table.GetNumWeightArrays() == nw
-        src = rng.normal(size=(nsrc, L)).astype(np.float32)
-        s_t = dev(src)
-        cases = [(0, n, nw), (37, 2000, nw), (64, 128, nw), (4000, 4099, nw)]
-        if nw > 1:
-            cases.append((5, 3000, 1))          # P only from a derivative table
-        for start, end, nout in cases:
-            outs = [empty(n * L) for _ in range(nout)]
-            assert table_eval(hip, table.h, s_t, (0, L, L), outs, [(0, L, L)] * nout, start,
-                              end) == 0
-            for k in range(nout):
-                want = oracle.apply_stencils(src, t["sizes"], t["offsets"], t["indices"],
-                                             t[keys[k]])[start:end]
-                got = host(outs[k], shape=(n, L))
-                np.testing.assert_array_equal(got[:end - start], want,
-                                              err_msg=f"{keys[k]} [{start},{end})")
-                assert np.all(np.isnan(got[end - start:]))
-    finally:
-        hip.osdhip_config(b"table_layout", old[0])
-        hip.osdhip_config(b"slice_group", old[1])
-        hip.osdhip_config(b"stencil_kernel", old[2])
+    table = HipStencilTable.Create(t, layout=layout, group=group, kernel=kernel)
+    assert table.options() == {"layout": layout, "keep_csr": True, "group": group,
+                               "kernel": kernel}
+    assert table.GetNumWeightArrays() == nw
+    src = rng.normal(size=(nsrc, L)).astype(np.float32)
+    s_t = dev(src)
+    cases = [(0, n, nw), (37, 2000, nw), (64, 128, nw), (4000, 4099, nw)]
+    if nw > 1:
+        cases.append((5, 3000, 1))          # P only from a derivative table
+    for start, end, nout in cases:
+        outs = [empty(n * L) for _ in range(nout)]
+        assert table_eval(hip, table.h, s_t, (0, L, L), outs, [(0, L, L)] * nout, start,
+                          end) == 0
+        for k in range(nout):
+            want = oracle.apply_stencils(src, t["sizes"], t["offsets"], t["indices"],
+                                         t[keys[k]])[start:end]
+            got = host(outs[k], shape=(n, L))
+            np.testing.assert_array_equal(got[:end - start], want,
+                                          err_msg=f"{keys[k]} [{start},{end})")
+            assert np.all(np.isnan(got[end - start:]))
 
 
-def test_device_table_knob_errors(hip):
-    assert hip.osdhip_config(b"table_layout", 3) == _lib.ERR_ARGS
-    assert hip.osdhip_config(b"table_layout", -1) == _lib.ERR_ARGS
-    assert hip.osdhip_config(b"slice_group", 5) == _lib.ERR_ARGS
-    assert hip.osdhip_config(b"slice_group", 32) == _lib.ERR_ARGS
-    assert hip.osdhip_config(b"no_such_knob", 0) == _lib.ERR_UNSUPPORTED
+def test_device_table_option_errors(hip):
+    """Options are checked at creation and per table: out-of-range layouts,
+    groups or kernels are rejected, and a table's evaluation options change
+    only for that table."""
+    import ctypes
+    from opensubdiv_amd import HipStencilTable
+    t = {"sizes": np.full(70, 2, np.int32), "offsets": np.arange(0, 140, 2, dtype=np.int32),
+         "indices": np.arange(140, dtype=np.int32) % 9,
+         "weights": np.linspace(0, 1, 140).astype(np.float32)}
+    for bad in (dict(layout=3), dict(layout=-1), dict(group=5), dict(group=32), dict(kernel=2),
+                dict(kernel=1, keep_csr=False)):
+        with pytest.raises(MemoryError):
+            HipStencilTable.Create(t, **bad)
+    a = HipStencilTable.Create(t, group=4)
+    b = HipStencilTable.Create(t, layout=2, keep_csr=False)
+    assert a.set("group", 16) == 4 and a.options()["group"] == 16
+    assert b.options() == {"layout": 2, "keep_csr": False, "group": 8, "kernel": 0}
+    assert hip.osdhip_stencil_table_set(a.h, b"group", 5) == _lib.ERR_ARGS
+    assert hip.osdhip_stencil_table_set(b.h, b"kernel", 1) == _lib.ERR_ARGS   # no CSR kept
+    assert hip.osdhip_stencil_table_set(a.h, b"layout", 2) == _lib.ERR_UNSUPPORTED
+    assert hip.osdhip_stencil_table_set(None, b"group", 4) == _lib.ERR_ARGS
+    o = _lib.TableOptions()
+    hip.osdhip_table_options_init(ctypes.byref(o))
+    assert (o.layout, o.keepCsr, o.group, o.kernel) == (0, 1, 8, 0)
 
 
 def test_device_table_cfg0_and_errors(hip):
@@ -294,13 +307,9 @@ def test_device_table_cfg0_and_errors(hip):
     np.testing.assert_array_equal(host(out2, shape=(n, 3)), g["cpu_out"])
 
 
-def _tiled_table(hip, t):
+def _tiled_table(hip, t, group=8):
     from opensubdiv_amd import HipStencilTable
-    old = hip.osdhip_config(b"table_layout", 2)
-    try:
-        return HipStencilTable.Create(t)
-    finally:
-        hip.osdhip_config(b"table_layout", old)
+    return HipStencilTable.Create(t, layout=2, group=group)
 
 
 @pytest.mark.parametrize("group", [4, 8])
@@ -310,42 +319,40 @@ def test_tiled_layout_mesh_tables_stage_in_lds(hip, group):
     limit table mixes listed and gathering slices, and P / all six limit outputs,
     full tables and sub-ranges, equal the reference's CpuEvaluator outputs
     bit for bit."""
-    old = hip.osdhip_config(b"slice_group", group)
-    try:
-        g = golden("cubesphere_l2.npz")
-        table = _tiled_table(hip, {k: g[k] for k in ("sizes", "offsets", "indices", "weights")})
-        info = table.layout()
-        assert info["layout"] == 2 and info["listed_slices"] == info["slices"] > 0
-        n = table.GetNumStencils()
-        src = dev(g["src"])
-        d7 = empty(n * 7)
-        assert table_eval(hip, table.h, src, (0, 7, 7), [d7], [(0, 7, 7)], 0, n) == 0
-        np.testing.assert_array_equal(host(d7, shape=(n, 7)), g["cpu_out7"])
-        d3 = empty(n * 7, 0.0)
-        assert table_eval(hip, table.h, src, (0, 3, 7), [d3], [(0, 3, 7)], 0, n) == 0
-        o = host(d3, shape=(n, 7))
-        np.testing.assert_array_equal(o[:, :3], g["cpu_out3"])
-        assert np.all(o[:, 3:] == 0)
-        for start, end in [(0, 1), (63, 65), (100, 1500), (1900, n)]:
-            d = empty(n * 7)
-            assert table_eval(hip, table.h, src, (0, 7, 7), [d], [(0, 7, 7)], start, end) == 0
-            o = host(d, shape=(n, 7))
-            np.testing.assert_array_equal(o[:end - start], g["cpu_out7"][start:end])
-            assert np.all(np.isnan(o[end - start:]))
+    g = golden("cubesphere_l2.npz")
+    table = _tiled_table(hip, {k: g[k] for k in ("sizes", "offsets", "indices", "weights")},
+                         group)
+    info = table.layout()
+    assert info["layout"] == 2 and info["listed_slices"] == info["slices"] > 0
+    n = table.GetNumStencils()
+    src = dev(g["src"])
+    d7 = empty(n * 7)
+    assert table_eval(hip, table.h, src, (0, 7, 7), [d7], [(0, 7, 7)], 0, n) == 0
+    np.testing.assert_array_equal(host(d7, shape=(n, 7)), g["cpu_out7"])
+    d3 = empty(n * 7, 0.0)
+    assert table_eval(hip, table.h, src, (0, 3, 7), [d3], [(0, 3, 7)], 0, n) == 0
+    o = host(d3, shape=(n, 7))
+    np.testing.assert_array_equal(o[:, :3], g["cpu_out3"])
+    assert np.all(o[:, 3:] == 0)
+    for start, end in [(0, 1), (63, 65), (100, 1500), (1900, n)]:
+        d = empty(n * 7)
+        assert table_eval(hip, table.h, src, (0, 7, 7), [d], [(0, 7, 7)], start, end) == 0
+        o = host(d, shape=(n, 7))
+        np.testing.assert_array_equal(o[:end - start], g["cpu_out7"][start:end])
+        assert np.all(np.isnan(o[end - start:]))
 
-        g = golden("cubesphere_limit.npz")
-        keys = ("weights", "du", "dv", "duu", "duv", "dvv")
-        table = _tiled_table(hip, {k: g[k] for k in ("sizes", "offsets", "indices") + keys})
-        info = table.layout()
-        assert 0 < info["listed_slices"] < info["slices"]      # both paths in one table
-        n = table.GetNumStencils()
-        src = dev(g["src"])
-        outs = [empty(n * 3) for _ in range(6)]
-        assert table_eval(hip, table.h, src, (0, 3, 3), outs, [(0, 3, 3)] * 6, 0, n) == 0
-        for k, name in enumerate(("cpu_p", "cpu_du", "cpu_dv", "cpu_duu", "cpu_duv", "cpu_dvv")):
-            np.testing.assert_array_equal(host(outs[k], shape=(n, 3)), g[name], err_msg=name)
-    finally:
-        hip.osdhip_config(b"slice_group", old)
+    g = golden("cubesphere_limit.npz")
+    keys = ("weights", "du", "dv", "duu", "duv", "dvv")
+    table = _tiled_table(hip, {k: g[k] for k in ("sizes", "offsets", "indices") + keys},
+                         group)
+    info = table.layout()
+    assert 0 < info["listed_slices"] < info["slices"]      # both paths in one table
+    n = table.GetNumStencils()
+    src = dev(g["src"])
+    outs = [empty(n * 3) for _ in range(6)]
+    assert table_eval(hip, table.h, src, (0, 3, 3), outs, [(0, 3, 3)] * 6, 0, n) == 0
+    for k, name in enumerate(("cpu_p", "cpu_du", "cpu_dv", "cpu_duu", "cpu_duv", "cpu_dvv")):
+        np.testing.assert_array_equal(host(outs[k], shape=(n, 3)), g[name], err_msg=name)
 
 
 @pytest.mark.parametrize("nw,L", [(1, 3), (1, 8), (3, 5), (6, 2)])
@@ -422,38 +429,34 @@ def test_hbr_regression_shapes_on_the_device(hip, layout):
     from opensubdiv_amd import HipStencilTable, far
     g = golden("hbr_catmark.npz")
     names = sorted({k.split("/")[0] for k in g.files})
-    old = hip.osdhip_config(b"table_layout", layout)
-    try:
-        for name in names:
-            class Shape:
-                pass
-            m = Shape()
-            for k in ("positions", "face_sizes", "face_verts", "crease_pairs", "crease_weights",
-                      "corner_verts", "corner_weights"):
-                setattr(m, k, g[f"{name}/{k}"])
-            m.nverts, m.nfaces = len(m.positions), len(m.face_sizes)
-            b, c = (int(x) for x in g[f"{name}/options"])
-            t = far.Topology(m, boundary=b, creasing=c)
-            t.refine_uniform(3)
-            st = t.stencil_table(intermediate=True)
-            n = len(st["sizes"])
-            ctrl = np.ascontiguousarray(m.positions, np.float32)
-            want = oracle.apply_stencils(ctrl, st["sizes"], st["offsets"], st["indices"],
-                                         st["weights"])
-            buf = np.full((m.nverts + n, 3), np.nan, np.float32)
-            buf[:m.nverts] = ctrl
-            d = dev(buf)
-            h = HipStencilTable.Create(st)
-            assert table_eval(hip, h.h, d, (0, 3, 3), [d], [(3 * m.nverts, 3, 3)], 0, n) == 0
-            got = host(d, shape=(m.nverts + n, 3))
-            np.testing.assert_array_equal(got[m.nverts:], want, err_msg=name)
-            for level in ((1,) if name == "catmark_chaikin1" else (1, 2, 3)):
-                nv = t.level_info(level)[0]
-                lo = m.nverts + sum(t.level_info(k)[0] for k in range(1, level))
-                base = g[f"{name}/level{level}"].astype(np.float64)
-                diag = np.linalg.norm(base.max(0) - base.min(0))
-                dist, nearest = cKDTree(base).query(got[lo:lo + nv].astype(np.float64))
-                assert len(np.unique(nearest)) == nv, (name, level)
-                assert dist.max() <= 1e-5 * diag, (name, level, dist.max())
-    finally:
-        hip.osdhip_config(b"table_layout", old)
+    for name in names:
+        class Shape:
+            pass
+        m = Shape()
+        for k in ("positions", "face_sizes", "face_verts", "crease_pairs", "crease_weights",
+                  "corner_verts", "corner_weights"):
+            setattr(m, k, g[f"{name}/{k}"])
+        m.nverts, m.nfaces = len(m.positions), len(m.face_sizes)
+        b, c = (int(x) for x in g[f"{name}/options"])
+        t = far.Topology(m, boundary=b, creasing=c)
+        t.refine_uniform(3)
+        st = t.stencil_table(intermediate=True)
+        n = len(st["sizes"])
+        ctrl = np.ascontiguousarray(m.positions, np.float32)
+        want = oracle.apply_stencils(ctrl, st["sizes"], st["offsets"], st["indices"],
+                                     st["weights"])
+        buf = np.full((m.nverts + n, 3), np.nan, np.float32)
+        buf[:m.nverts] = ctrl
+        d = dev(buf)
+        h = HipStencilTable.Create(st, layout=layout)
+        assert table_eval(hip, h.h, d, (0, 3, 3), [d], [(3 * m.nverts, 3, 3)], 0, n) == 0
+        got = host(d, shape=(m.nverts + n, 3))
+        np.testing.assert_array_equal(got[m.nverts:], want, err_msg=name)
+        for level in ((1,) if name == "catmark_chaikin1" else (1, 2, 3)):
+            nv = t.level_info(level)[0]
+            lo = m.nverts + sum(t.level_info(k)[0] for k in range(1, level))
+            base = g[f"{name}/level{level}"].astype(np.float64)
+            diag = np.linalg.norm(base.max(0) - base.min(0))
+            dist, nearest = cKDTree(base).query(got[lo:lo + nv].astype(np.float64))
+            assert len(np.unique(nearest)) == nv, (name, level)
+            assert dist.max() <= 1e-5 * diag, (name, level, dist.max())

@@ -339,6 +339,50 @@ def test_stencils_start_end_row_convention_and_errors():
     assert not oracle.eval_stencils(src, (0, 3, 3), out, (0, 2, 3), sizes, offsets, idx, w, 0, 4)
 
 
+@pytest.mark.parametrize("threads", [1, 2, 3, 8])
+def test_omp_stencils_bit_exact_vs_cpu_evaluator(threads):
+    """The OpenMP restatement (osd/ompKernel.cpp:70-122) against the real
+    CpuEvaluator's vectors: same bits whatever the thread count, since every
+    row still sums in index order."""
+    g = golden("cube_l3.npz")
+    n = len(g["sizes"])
+    out = np.zeros((n, 3), np.float32)
+    assert oracle.eval_stencils(g["src"], (0, 3, 3), out, (0, 3, 3), g["sizes"], g["offsets"],
+                                g["indices"], g["weights"], threads=threads)
+    np.testing.assert_array_equal(out, g["cpu_out"])
+    g = golden("cubesphere_l2.npz")
+    n = len(g["sizes"])
+    out7 = np.zeros((n, 7), np.float32)
+    assert oracle.eval_stencils(g["src"], (0, 7, 7), out7, (0, 7, 7), g["sizes"], g["offsets"],
+                                g["indices"], g["weights"], threads=threads)
+    np.testing.assert_array_equal(out7, g["cpu_out7"])
+    # OMP writes stencil i to row i - start (osd/ompKernel.cpp:113)
+    part = np.zeros((n, 7), np.float32)
+    assert oracle.eval_stencils(g["src"], (0, 7, 7), part, (0, 7, 7), g["sizes"], g["offsets"],
+                                g["indices"], g["weights"], 100, 900, threads=threads)
+    np.testing.assert_array_equal(part[:800], g["cpu_out7"][100:900])
+    assert np.all(part[800:] == 0)
+    assert not oracle.eval_stencils(g["src"], (0, 7, 7), part, (0, 3, 7), g["sizes"],
+                                    g["offsets"], g["indices"], g["weights"], threads=threads)
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_omp_patches_bit_exact_vs_cpu_evaluator(threads):
+    """The OpenMP EvalPatches restatement (osd/ompEvaluator.cpp:163-350)
+    against the real CpuEvaluator's P / dPdu / dPdv and P-only vectors."""
+    g, coords, arrays, idx, params = _patch_fixture()
+    n = len(coords)
+    o = [np.zeros((n, 3), np.float32) for _ in range(3)]
+    assert oracle.eval_patches(g["src"], (0, 3, 3), o, [(0, 3, 3)] * 3, coords, arrays, idx,
+                               params, threads=threads)
+    for x, k in zip(o, ("cpu_p", "cpu_du", "cpu_dv")):
+        np.testing.assert_array_equal(x, g[k], err_msg=k)
+    p_only = np.zeros((n, 3), np.float32)
+    assert oracle.eval_patches(g["src"], (0, 3, 3), [p_only, None, None], [(0, 3, 3), None, None],
+                               coords, arrays, idx, params, threads=threads)
+    np.testing.assert_array_equal(p_only, g["cpu_p_only"])
+
+
 def test_planted_mistake_is_caught_by_the_closed_form():
     """A plausible oracle bug (B-spline basis with columns reversed, i.e.
     transposed s/t roles) must fail the Cox-de Boor check."""
