@@ -195,8 +195,13 @@ int osdhip_stencil_table_eval(const osdhip_stencil_table *table, const float *sr
  * limit value / derivative at coords[i], as CpuEvaluator::EvalPatches
  * (osd/cpuEvaluator.cpp:157-381).  Any output pointer may be NULL (skipped);
  * src == NULL or an output length != srcDesc.length returns OSDHIP_ERR_ARGS.
- * Supported patch types: REGULAR (B-spline, with boundary fold),
- * GREGORY_BASIS, QUADS (bilinear); other types produce zeros. */
+ * Patch types: REGULAR (B-spline, with boundary fold), GREGORY_BASIS,
+ * QUADS (bilinear), LOOP (box spline, with boundary phantom points),
+ * GREGORY_TRIANGLE and TRIANGLES (linear), every type
+ * OsdEvaluatePatchBasis evaluates (osd/patchBasis.h:1447-1610); any other
+ * type produces zeros, as CpuEvaluator's Clear() with no point added.
+ * Second derivatives on rotated triangles keep the reference's sign
+ * (-4^depth, osd/patchBasis.h:1596). */
 int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
                         float *dst, osdhip_buffer_desc dstDesc,
                         float *du, osdhip_buffer_desc duDesc,

@@ -994,27 +994,23 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
         const osdhip_patch_param pp = ld(sg.params + c.patchIndex);
         const unsigned f1 = pp.field1;
         const int type = ((f1 >> 5) & 1u) ? arr.regDesc : arr.desc;
-        const int depth = (int)(f1 & 0xfu);
-        const int nonquad = (int)((f1 >> 4) & 1u);
         const int boundary = (int)((f1 >> 7) & 0x1fu);
         if (type == OSDHIP_PATCH_REGULAR && boundary == 0) continue;   // the regular kernel's
-        const int pv = (int)((f1 >> 12) & 0x3ffu);
-        const int pu = (int)((f1 >> 22) & 0x3ffu);
-        const float frac_inv = (float)(1 << (depth - nonquad));
-        const float s = c.s * frac_inv - (float)pu;
-        const float t = c.t * frac_inv - (float)pv;
-        const int n = type == OSDHIP_PATCH_REGULAR ? 16
-                    : type == OSDHIP_PATCH_GREGORY_BASIS ? 20
-                    : type == OSDHIP_PATCH_QUADS ? 4 : 0;
+        float s = c.s, t = c.t, sign;
+        const float k1 = patch_normalize(type, f1, s, t, sign);
+        const int n = patch_points(type);
         const G_ int *cvs = sg.indices + arr.indexBase + arr.stride * (c.patchIndex - arr.primitiveIdBase);
         Basis<true, false> B;
         if (n == 16) basis_bspline<true, false>(s, t, boundary, B);
         else if (n == 20) basis_gregory<true, false>(s, t, B);
         else if (n == 4) basis_linear<true, false>(s, t, B);
+        else if (n == 12) basis_box_tri<true, false>(s, t, boundary, B);
+        else if (n == 18) basis_gregory_tri<true, false>(s, t, B);
+        else if (n == 3) basis_linear_tri<true, false>(s, t, B);
+        // a negative (invalid) point index reads point 0, as in k_patches
         int cv[20];
 #pragma unroll
-        for (int j = 0; j < 20; ++j) cv[j] = j < n ? cvs[j] : 0;
-        const float k1 = (float)(1 << depth);
+        for (int j = 0; j < 20; ++j) cv[j] = j < n ? max(cvs[j], 0) : 0;
 
         for (int f0 = 0; f0 < sg.frames; f0 += FC) {
             const int nq = min(FC, sg.frames - f0);
@@ -1027,8 +1023,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
                     for (int k = 0; k < LC; ++k) acc[o][q][k] = 0.0f;
             const G_ float *base = sg.src + (size_t)f0 * sg.srcF;
             // the point count follows from the type: 16 / 20 points without
-            // per-point predicates, bilinear separately (reference order)
-            if (n >= 16) {
+            // per-point predicates, the others (at most 18) predicated, all
+            // in the reference's point order
+            if (n == 16 || n == 20) {
 #pragma unroll
                 for (int j = 0; j < 16; ++j)
                     batch_patch_point<LC, FC>(base + (size_t)(unsigned)cv[j] * sg.srcV, sg.srcF,
@@ -1039,11 +1036,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_patches(BatchArgs a)
                         batch_patch_point<LC, FC>(base + (size_t)(unsigned)cv[j] * sg.srcV,
                                                   sg.srcF, nq, B.p[j], B.ds[j], B.dt[j], acc);
                 }
-            } else if (n == 4) {
+            } else {
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    batch_patch_point<LC, FC>(base + (size_t)(unsigned)cv[j] * sg.srcV, sg.srcF,
-                                              nq, B.p[j], B.ds[j], B.dt[j], acc);
+                for (int j = 0; j < 18; ++j)
+                    if (j < n)
+                        batch_patch_point<LC, FC>(base + (size_t)(unsigned)cv[j] * sg.srcV,
+                                                  sg.srcF, nq, B.p[j], B.ds[j], B.dt[j], acc);
             }
 #pragma unroll
             for (int q = 0; q < FC; ++q) {

@@ -44,6 +44,38 @@ __device__ __forceinline__ void accumulate_point(const float (&v)[LC], const Bas
     }
 }
 
+// The first NA = min(NP, 16) points' control values are loaded first (all
+// NA loads in flight), then the basis (`eval` fills B), then the products
+// are summed in point order, then the remaining points' (as the Gregory
+// path does with its last 4, which bounds the registers held at once).
+template <int NP, int LC, bool D1, bool D2, int NO, class EVAL>
+__device__ __forceinline__ void gather_points(const float *src, size_t sstride, const int *cvs,
+                                              Basis<D1, D2> &B, float (&acc)[NO][LC], EVAL eval)
+{
+    constexpr int NA = NP < 16 ? NP : 16;
+    float v[NA][LC];
+#pragma unroll
+    for (int j = 0; j < NA; ++j) {
+        const float *p = src + (size_t)max(cvs[j], 0) * sstride;
+#pragma unroll
+        for (int k = 0; k < LC; ++k) v[j][k] = p[k];
+    }
+    eval();
+#pragma unroll
+    for (int j = 0; j < NA; ++j) accumulate_point<D1, D2, LC>(v[j], B, j, acc);
+    if constexpr (NP > NA) {
+        float r[NP - NA][LC];
+#pragma unroll
+        for (int j = NA; j < NP; ++j) {
+            const float *p = src + (size_t)max(cvs[j], 0) * sstride;
+#pragma unroll
+            for (int k = 0; k < LC; ++k) r[j - NA][k] = p[k];
+        }
+#pragma unroll
+        for (int j = NA; j < NP; ++j) accumulate_point<D1, D2, LC>(r[j - NA], B, j, acc);
+    }
+}
+
 // Sort keys for the patch-ordered evaluation: the low 24 bits of the patch
 // index (any key gives a valid permutation; these give full locality up to
 // 16.7 M patches), value = coord index.
@@ -71,28 +103,23 @@ __global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
     const osdhip_patch_param pp = a.params[c.patchIndex];
     const unsigned f1 = pp.field1;
     const int type = ((f1 >> 5) & 1u) ? arr.regDesc : arr.desc;
-    const int depth = (int)(f1 & 0xfu);
-    const int nonquad = (int)((f1 >> 4) & 1u);
     const int boundary = (int)((f1 >> 7) & 0x1fu);
-    const int pv = (int)((f1 >> 12) & 0x3ffu);
-    const int pu = (int)((f1 >> 22) & 0x3ffu);
 
-    // PatchParam::Normalize: ptex-face (s,t) -> patch-local (s,t)
-    const float frac_inv = (float)(1 << (depth - nonquad));
-    const float s = c.s * frac_inv - (float)pu;
-    const float t = c.t * frac_inv - (float)pv;
+    // PatchParam Normalize / NormalizeTriangle: ptex-face (s,t) ->
+    // patch-local (s,t), and the signed 2^depth derivative factor
+    float s = c.s, t = c.t, sign;
+    const float k1 = patch_normalize(type, f1, s, t, sign);
 
     // A negative point index (INDEX_INVALID of a patch built without
     // fillMissingBoundaryPoints, far/patchBuilder.h:177-183) reads point 0
     // instead of memory before the buffer; such points carry zero weight
     // after the boundary fold.
     // The point count follows from the type alone (REGULAR 16, GREGORY_BASIS
-    // 20, QUADS 4, anything else 0 -> zero outputs, as the reference's
-    // Clear() without AddWithWeight), so the index and control-point loads
-    // are issued before the basis is evaluated and overlap with it.
-    const int n = type == OSDHIP_PATCH_REGULAR ? 16
-                : type == OSDHIP_PATCH_GREGORY_BASIS ? 20
-                : type == OSDHIP_PATCH_QUADS ? 4 : 0;
+    // 20, QUADS 4, LOOP 12, GREGORY_TRIANGLE 18, TRIANGLES 3, anything else
+    // 0 -> zero outputs, as the reference's Clear() without AddWithWeight),
+    // so the index and control-point loads are issued before the basis is
+    // evaluated and overlap with it.
+    const int n = patch_points(type);
     const int *cvs = a.indices + arr.indexBase + arr.stride * (c.patchIndex - arr.primitiveIdBase);
     constexpr int NO = 1 + (D1 ? 2 : 0) + (D2 ? 3 : 0);
     float acc[NO][LC];
@@ -137,11 +164,20 @@ __global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
         basis_linear<D1, D2>(s, t, B);
 #pragma unroll
         for (int j = 0; j < 4; ++j) accumulate_point<D1, D2, LC>(v[j], B, j, acc);
+    } else if (n == 12) {
+        gather_points<12, LC>(a.src, sstride, cvs, B, acc,
+                              [&] { basis_box_tri<D1, D2>(s, t, boundary, B); });
+    } else if (n == 18) {
+        gather_points<18, LC>(a.src, sstride, cvs, B, acc,
+                              [&] { basis_gregory_tri<D1, D2>(s, t, B); });
+    } else if (n == 3) {
+        gather_points<3, LC>(a.src, sstride, cvs, B, acc,
+                             [&] { basis_linear_tri<D1, D2>(s, t, B); });
     }
-    // 2^depth (first) and 4^depth (second derivatives): exact power-of-two
-    // factors, applied to the sums instead of to every weight.
-    const float k1 = (float)(1 << depth);
-    const float k2 = k1 * k1;
+    // +-2^depth (first) and +-4^depth (second derivatives): exact signed
+    // power-of-two factors, applied to the sums instead of to every weight
+    // (the second-derivative factor is sign * k1 * k1, as the reference's).
+    const float k2 = sign * k1 * k1;
 #pragma unroll
     for (int o = 0; o < NO; ++o) {
         float *d = a.dst[o];

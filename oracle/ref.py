@@ -30,6 +30,8 @@ def lib():
         sig = {
             "ref_topo_create": ([i, i, vp, vp, i, vp, vp, i, vp, vp, i, i], vp),
             "ref_topo_destroy": ([vp], None),
+            "ref_topo_set_scheme": ([vp, i], None),
+            "ref_patches_create_uniform": ([vp, i, i], vp),
             "ref_stencils_uniform": ([vp, i, i, i, i, i, i], vp),
             "ref_stencils_limit": ([vp, i, i, vp, vp, vp, i, i], vp),
             "ref_stencils_info": ([vp, vp, vp, vp], None),
@@ -69,14 +71,18 @@ def _desc(dsc):
     return np.ascontiguousarray(dsc, np.int32)
 
 
-class Topology:
-    """Far::TopologyDescriptor for a Catmark mesh (Sdc defaults of the
-    regression tools: VTX_BOUNDARY_EDGE_ONLY, CREASE_UNIFORM)."""
+SCHEME_BILINEAR, SCHEME_CATMARK, SCHEME_LOOP = 0, 1, 2
 
-    def __init__(self, mesh, boundary=1, creasing=0):
+
+class Topology:
+    """Far::TopologyDescriptor for a Catmark (or, with scheme=SCHEME_LOOP, an
+    all-triangle Loop) mesh, with the Sdc defaults of the regression tools:
+    VTX_BOUNDARY_EDGE_ONLY, CREASE_UNIFORM."""
+
+    def __init__(self, mesh, boundary=1, creasing=0, scheme=SCHEME_CATMARK):
         fv = np.ascontiguousarray(mesh.face_verts, np.int32).reshape(-1)
         sizes = getattr(mesh, "face_sizes", None)
-        nvpf = np.full(mesh.nfaces, 4, np.int32) if sizes is None else \
+        nvpf = np.full(mesh.nfaces, fv.size // mesh.nfaces, np.int32) if sizes is None else \
             np.ascontiguousarray(sizes, np.int32)
         cp = np.ascontiguousarray(mesh.crease_pairs, np.int32).reshape(-1)
         cw = np.ascontiguousarray(mesh.crease_weights, np.float32)
@@ -87,6 +93,8 @@ class Topology:
                                        _p(cp) if len(cw) else None, _p(cw) if len(cw) else None,
                                        len(cv), _p(cv) if len(cv) else None,
                                        _p(cwv) if len(cv) else None, boundary, creasing)
+        if scheme != SCHEME_CATMARK:
+            lib().ref_topo_set_scheme(self.h, int(scheme))
 
     def __del__(self):
         if getattr(self, "h", None) and _L is not None:
@@ -136,10 +144,17 @@ class Patches:
     Osd::CpuPatchTable flattening + Far::PatchMap."""
 
     def __init__(self, topo, iso_level, endcap=3, inf_sharp=False, single_crease=False,
-                 legacy_sharp_corner=True, double=False):
+                 legacy_sharp_corner=True, double=False, uniform=False, triangulate=False):
+        """uniform=True: RefineUniform(iso_level) and the uniform linear
+        table (QUADS, or TRIANGLES for Loop meshes / triangulate=True);
+        no patch map then."""
         L = lib()
-        self.h = L.ref_patches_create(topo.h, iso_level, endcap, int(inf_sharp),
-                                      int(single_crease), int(legacy_sharp_corner), int(double))
+        if uniform:
+            self.h = L.ref_patches_create_uniform(topo.h, iso_level, int(triangulate))
+        else:
+            self.h = L.ref_patches_create(topo.h, iso_level, endcap, int(inf_sharp),
+                                          int(single_crease), int(legacy_sharp_corner),
+                                          int(double))
         v = [C.c_int() for _ in range(6)]
         L.ref_patches_info(self.h, *[C.byref(x) for x in v])
         self.num_coarse, ns, nnz, na, ni, npar = [x.value for x in v]

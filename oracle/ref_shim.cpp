@@ -39,6 +39,7 @@ struct Topo {
     std::vector<float> creaseW, cornerW;
     int boundary = 1;   // Sdc::Options::VTX_BOUNDARY_EDGE_ONLY
     int creasing = 0;   // Sdc::Options::CREASE_UNIFORM
+    int scheme = Sdc::SCHEME_CATMARK;
 
     Far::TopologyRefiner *makeRefiner() const {
         Far::TopologyDescriptor d;
@@ -56,7 +57,7 @@ struct Topo {
         o.SetVtxBoundaryInterpolation((Sdc::Options::VtxBoundaryInterpolation)boundary);
         o.SetCreasingMethod((Sdc::Options::CreasingMethod)creasing);
         typedef Far::TopologyRefinerFactory<Far::TopologyDescriptor> F;
-        return F::Create(d, F::Options(Sdc::SCHEME_CATMARK, o));
+        return F::Create(d, F::Options((Sdc::SchemeType)scheme, o));
     }
 };
 
@@ -126,6 +127,10 @@ void *ref_topo_create(int nverts, int nfaces, const int *nvpf, const int *fv,
 }
 
 void ref_topo_destroy(void *t) { delete (Topo *)t; }
+
+// Subdivision scheme of later refiners: Sdc::SchemeType (0 bilinear,
+// 1 Catmark, 2 Loop; sdc/types.h:36-40).
+void ref_topo_set_scheme(void *t, int scheme) { ((Topo *)t)->scheme = scheme; }
 
 // Uniform refinement to `level`, then StencilTableFactory::Create.
 void *ref_stencils_uniform(void *topo, int level, int mode, int genIntermediate,
@@ -275,6 +280,34 @@ void *ref_patches_create(void *topo, int isoLevel, int endcap, int infSharp,
     return p;
 }
 
+// Uniform linear patch table (far/patchTableFactory.cpp:745-850,
+// BuildUniformPolygons): RefineUniform(level), one QUADS or TRIANGLES patch
+// per face of the last level (TRIANGLES for Loop meshes, or Catmark quads
+// split in two with triangulateQuads), indices into [coarse | refined
+// levels], the stencil table with intermediate levels that produces them,
+// and the table's PatchMap.
+void *ref_patches_create_uniform(void *topo, int level, int triangulateQuads)
+{
+    Patches *p = new Patches;
+    p->refiner = ((Topo *)topo)->makeRefiner();
+    if (!p->refiner) { delete p; return nullptr; }
+    p->nCoarse = p->refiner->GetLevel(0).GetNumVertices();
+    p->refiner->RefineUniform(Far::TopologyRefiner::UniformOptions(level));
+    Far::PatchTableFactory::Options po(level);
+    po.triangulateQuads = triangulateQuads != 0;
+    po.generateVaryingTables = false;
+    p->table = Far::PatchTableFactory::Create(*p->refiner, po);
+    Far::StencilTableFactory::Options so;
+    so.generateOffsets = true;
+    so.generateIntermediateLevels = true;
+    auto const *st = Far::StencilTableFactory::Create(*p->refiner, so);
+    p->st.fromTable(st);
+    delete st;
+    p->flat = new Osd::CpuPatchTable(p->table);
+    p->map = new Far::PatchMap(*p->table);
+    return p;
+}
+
 void ref_patches_info(void *h, int *nCoarse, int *nStencils, int *nnz,
                       int *nArrays, int *nIndices, int *nParams)
 {
@@ -311,6 +344,7 @@ int ref_patches_find(void *h, int n, const int *faces, const double *u,
 {
     Patches *p = (Patches *)h;
     int found = 0;
+    if (!p->map) return -1;
     for (int i = 0; i < n; ++i) {
         auto const *hd = p->map->FindPatch(faces[i], u[i], v[i]);
         if (hd) {

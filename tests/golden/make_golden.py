@@ -19,6 +19,18 @@ reference's outputs; nothing of the reference's source goes into them.
                         stencils, 4096 PatchMap coords, CpuEvaluator
                         EvalPatches P/du/dv and 2nd derivatives, and Far
                         EvaluateBasis<double> weights for the first 256
+  loop_patches.npz      triangular patch types: Loop-scheme meshes (a
+                        creased triangulated cube-sphere, closed, iso 2; an
+                        open triangle grid with EDGE_AND_CORNER boundaries,
+                        iso 3; a one-quad-wide strip, whose patches touch
+                        both boundaries, iso 2) whose adaptive tables hold
+                        LOOP box-spline patches (edge and vertex boundary
+                        masks) and
+                        GREGORY_TRIANGLE end caps, and a uniform level-2
+                        Loop table of TRIANGLES; per mesh: tables, refined
+                        control points, PatchMap coords, CpuEvaluator
+                        EvalPatches outputs (P/du/dv and the 2nd-derivative
+                        arity) and Far EvaluateBasis<double> weights
 """
 from __future__ import annotations
 
@@ -142,6 +154,80 @@ def cubesphere_patches():
          cpu2_dvv=o2[5], basis_n=npts, basis_wP=wP, basis_wDs=wDs, basis_wDt=wDt)
 
 
+def tri_sphere(n, seed):
+    """The cube-sphere's quads split along one diagonal (a closed Loop
+    mesh, valences 4 to 6), 5 % of its edges creased."""
+    m = meshes.cube_sphere(n, seed=seed, crease_fraction=0.0)
+    q = m.face_verts
+    t = np.concatenate([q[:, [0, 1, 2]], q[:, [0, 2, 3]]]).astype(np.int32)
+    tm = meshes.QuadMesh(m.nverts, t, m.positions)
+    rng = np.random.default_rng(seed + 1)
+    e = tm.edges()
+    k = int(0.05 * len(e))
+    pick = np.sort(rng.choice(len(e), k, replace=False))
+    tm.crease_pairs = e[pick].astype(np.int32)
+    tm.crease_weights = rng.uniform(0, 4, k).astype(np.float32)
+    return tm
+
+
+def tri_grid(nx, ny, seed):
+    """An open nx x ny grid of quads split into triangles, heights noisy."""
+    X, Y = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="ij")
+    ids = X * (ny + 1) + Y
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    q = np.stack([ids[i, j], ids[i + 1, j], ids[i + 1, j + 1], ids[i, j + 1]], -1).reshape(-1, 4)
+    t = np.concatenate([q[:, [0, 1, 2]], q[:, [0, 2, 3]]]).astype(np.int32)
+    rng = np.random.default_rng(seed)
+    p = np.stack([X.ravel() / nx, Y.ravel() / nx, 0.1 * rng.uniform(-1, 1, X.size)], -1)
+    return meshes.QuadMesh(X.size, t, p)
+
+
+def loop_patches():
+    out = {}
+    cases = [("sphere", tri_sphere(3, 21), dict(boundary=1), dict(iso_level=2)),
+             ("grid", tri_grid(5, 5, 22), dict(boundary=2), dict(iso_level=3)),
+             ("strip", tri_grid(5, 1, 24), dict(boundary=1), dict(iso_level=2)),
+             ("uniform", tri_sphere(2, 23), dict(boundary=1), dict(iso_level=2, uniform=True))]
+    for k, (name, m, topt, popt) in enumerate(cases):
+        topo = ref.Topology(m, scheme=ref.SCHEME_LOOP, **topt)
+        P = ref.Patches(topo, **popt)
+        st = P.stencils
+        nC, nT = P.num_coarse, P.num_vertices_total
+        src = np.zeros((nT, 3), np.float32)
+        src[:nC] = m.positions.astype(np.float32)
+        assert ref.cpu_eval_stencils(src, (0, 3, 3), src, (nC * 3, 3, 3), st)
+        rng = np.random.default_rng(30 + k)
+        nq = 1024
+        f = rng.integers(0, m.nfaces, nq).astype(np.int32)
+        u, v = rng.uniform(size=nq), rng.uniform(size=nq)
+        flip = u + v > 1
+        u[flip], v[flip] = 1 - u[flip], 1 - v[flip]
+        h = P.find(f, u, v)
+        assert (h[:, 0] >= 0).all()
+        coords = np.zeros(nq, PATCH_COORD_DTYPE)
+        coords["arrayIndex"], coords["patchIndex"], coords["vertIndex"] = h[:, 0], h[:, 1], h[:, 2]
+        coords["s"], coords["t"] = u.astype(np.float32), v.astype(np.float32)
+        o = [np.zeros((nq, 3), np.float32) for _ in range(3)]
+        assert ref.cpu_eval_patches(src, (0, 3, 3), o, [(0, 3, 3)] * 3, coords, P.arrays,
+                                    P.indices, P.params)
+        o2 = [np.zeros((nq, 3), np.float32) for _ in range(6)]
+        assert ref.cpu_eval_patches_d2(src, (0, 3, 3), o2, [(0, 3, 3)] * 6, coords, P.arrays,
+                                       P.indices, P.params)
+        nb = 128
+        npts, wP, wDs, wDt = P.eval_basis_f64(h[:nb], coords["s"][:nb].astype(np.float64),
+                                              coords["t"][:nb].astype(np.float64))
+        types = sorted(set(P.arrays["regDesc"].tolist()) | set(P.arrays["desc"].tolist()))
+        print(name, "types", types, "patches", len(P.params), "coords", nq)
+        out.update({f"{name}_{key}": val for key, val in dict(
+            src=src, arrays=P.arrays.view(np.int32).reshape(-1, 6), indices=P.indices,
+            params=P.params.view(np.uint32).reshape(-1, 3),
+            coords=coords.view(np.int32).reshape(-1, 5),
+            cpu_p=o[0], cpu_du=o[1], cpu_dv=o[2],
+            cpu2_p=o2[0], cpu2_du=o2[1], cpu2_dv=o2[2], cpu2_duu=o2[3], cpu2_duv=o2[4],
+            cpu2_dvv=o2[5], basis_n=npts, basis_wP=wP, basis_wDs=wDs, basis_wDt=wDt).items()})
+    save("loop_patches.npz", **out)
+
+
 if __name__ == "__main__":
     if not ref.available():
         sys.exit("oracle/_ref/libosdref.so not built (make -C oracle)")
@@ -149,3 +235,4 @@ if __name__ == "__main__":
     cubesphere_l2()
     cubesphere_limit()
     cubesphere_patches()
+    loop_patches()

@@ -278,8 +278,15 @@ def test_staged_outputs_share_an_arena():
     import torch
     from opensubdiv_amd.workloads import stage_outputs
     length = 7
+    # the level-4 mesh (the largest outputs) between smaller ones, so that
+    # an arena of its size holds three stages: meshes before it, it alone,
+    # meshes after it
     hosts = [compile_mesh(s, length) for s in SPECS]
     dms = [DeviceMesh(h, own_outputs=False, layout=0) for h in hosts]
+    need = [d.output_floats() for d in dms]
+    big = dms[need.index(max(need))]
+    dms = [d for d in dms if d is not big]
+    dms.insert(len(dms) // 2, big)
     need = [d.output_floats() for d in dms]
     arena, nstages = stage_outputs(dms, max(need) + 64)
     assert nstages >= 3 and arena.numel() < sum(need)

@@ -137,16 +137,89 @@ def test_synthetic_tables_all_types_bit_exact(hip, L, deriv):
 
 
 def test_unsupported_patch_type_gives_zeros(hip):
+    """Types OsdEvaluatePatchBasis does not evaluate (POINTS, LINES, the
+    legacy GREGORY / GREGORY_BOUNDARY): zero outputs, as CpuEvaluator's
+    Clear() with no point added."""
     rng = np.random.default_rng(5)
-    arrays = np.array([(5, 5, 4, 0, 12, 0)], PATCH_ARRAY_DTYPE)   # LOOP: not a quad type
-    params = np.array([(0, 0, 0.0)] * 4, PATCH_PARAM_DTYPE)
-    indices = np.zeros(48, np.int32)
-    coords = np.array([(0, i, 0, 0.5, 0.5) for i in range(4)], PATCH_COORD_DTYPE)
-    src = rng.normal(size=(2, 3)).astype(np.float32)
-    p = empty(12)
-    assert eval_patches(hip, dev(src), (0, 3, 3), [p], [(0, 3, 3)], 4, dev(coords), dev(arrays),
-                        dev(indices), dev(params)) == 0
-    assert np.all(host(p) == 0)
+    for ptype in (1, 2, 7, 8):
+        arrays = np.array([(ptype, ptype, 4, 0, 16, 0)], PATCH_ARRAY_DTYPE)
+        params = np.array([(0, 0, 0.0)] * 4, PATCH_PARAM_DTYPE)
+        indices = np.zeros(64, np.int32)
+        coords = np.array([(0, i, 0, 0.5, 0.5) for i in range(4)], PATCH_COORD_DTYPE)
+        src = rng.normal(size=(2, 3)).astype(np.float32)
+        p = dev(np.full(12, np.nan, np.float32))
+        assert eval_patches(hip, dev(src), (0, 3, 3), [p], [(0, 3, 3)], 4, dev(coords),
+                            dev(arrays), dev(indices), dev(params)) == 0
+        assert np.all(host(p) == 0), ptype
+
+
+LOOP_CASES = ("sphere", "grid", "strip", "uniform")
+
+
+@pytest.mark.parametrize("name", LOOP_CASES)
+def test_golden_loop_patches_bit_exact(hip, name):
+    """Loop-scheme tables -- LOOP box-spline patches with boundary masks,
+    GREGORY_TRIANGLE end caps, uniform TRIANGLES, rotated sub-triangles --
+    against the reference CpuEvaluator's outputs, P only, P/du/dv and the
+    2nd-derivative arity, and against the oracle with an interleaved
+    stride-9 output."""
+    g = golden("loop_patches.npz")
+    src, coords, arrays = dev(g[name + "_src"]), dev(g[name + "_coords"]), dev(g[name + "_arrays"])
+    indices, params = dev(g[name + "_indices"]), dev(g[name + "_params"])
+    n = len(g[name + "_coords"])
+    outs = [empty(n * 3) for _ in range(3)]
+    assert eval_patches(hip, src, (0, 3, 3), outs, [(0, 3, 3)] * 3, n, coords, arrays, indices,
+                        params) == 0
+    for o, k in zip(outs, ("cpu_p", "cpu_du", "cpu_dv")):
+        np.testing.assert_array_equal(host(o, shape=(n, 3)), g[f"{name}_{k}"], err_msg=k)
+    outs = [empty(n * 3) for _ in range(6)]
+    assert eval_patches(hip, src, (0, 3, 3), outs, [(0, 3, 3)] * 6, n, coords, arrays, indices,
+                        params) == 0
+    for o, k in zip(outs, ("cpu2_p", "cpu2_du", "cpu2_dv", "cpu2_duu", "cpu2_duv", "cpu2_dvv")):
+        np.testing.assert_array_equal(host(o, shape=(n, 3)), g[f"{name}_{k}"], err_msg=k)
+    p = empty(n * 3)
+    assert eval_patches(hip, src, (0, 3, 3), [p], [(0, 3, 3)], n, coords, arrays, indices,
+                        params) == 0
+    np.testing.assert_array_equal(host(p, shape=(n, 3)), g[name + "_cpu_p"])
+    inter = empty(n * 9)
+    assert eval_patches(hip, src, (0, 3, 3), [inter, inter, inter],
+                        [(0, 3, 9), (3, 3, 9), (6, 3, 9)], n, coords, arrays, indices,
+                        params) == 0
+    got = host(inter, shape=(n, 9))
+    for j, k in enumerate(("cpu_p", "cpu_du", "cpu_dv")):
+        np.testing.assert_array_equal(got[:, 3 * j:3 * j + 3], g[f"{name}_{k}"], err_msg=k)
+
+
+@pytest.mark.parametrize("name", LOOP_CASES)
+def test_golden_loop_patches_in_a_batch(hip, name):
+    """The batched engine's phase B on Loop-scheme tables (no regular
+    Catmark patches, so every coord takes the generic batch kernel), two
+    frames (the second the first scaled by 2, a power of two, so its
+    results are the first's scaled exactly): same bits as the reference
+    CpuEvaluator's, run directly and as a graph launch."""
+    import torch
+    from opensubdiv_amd.batch import Batch
+    g = golden("loop_patches.npz")
+    src1 = g[name + "_src"]
+    nsrc, n = len(src1), len(g[name + "_coords"])
+    src = dev(np.stack([src1, 2 * src1], axis=1))              # (v, f, k), frames interleaved
+    outs = [empty(2 * n * 3) for _ in range(3)]
+    seg = dict(n=n, coords=dev(g[name + "_coords"]), arrays=dev(g[name + "_arrays"]),
+               indices=dev(g[name + "_indices"]), params=dev(g[name + "_params"]), src=src,
+               src_v=6, src_f=3, P=outs[0], dPdu=outs[1], dPdv=outs[2], out_v=3, out_f=n * 3,
+               frames=2)
+    b = Batch(3, [], [seg])
+    assert nsrc > 0
+    for run in (b.run, b.launch):
+        for o in outs:
+            o.fill_(float("nan"))
+        run()
+        torch.cuda.synchronize()
+        for o, k in zip(outs, ("cpu_p", "cpu_du", "cpu_dv")):
+            got = host(o, shape=(2, n, 3))
+            np.testing.assert_array_equal(got[0], g[f"{name}_{k}"], err_msg=k)
+            np.testing.assert_array_equal(got[1], 2 * g[f"{name}_{k}"], err_msg=k)
+    b.close()
 
 
 def test_patch_argument_errors(hip):

@@ -20,6 +20,8 @@ from conftest import bbox_rel_err, golden
 from opensubdiv_amd.osd import PATCH_ARRAY_DTYPE, PATCH_COORD_DTYPE, PATCH_PARAM_DTYPE
 
 REGULAR, GREGORY, QUADS = 6, 9, 3
+TRIANGLES, LOOP, GREGORY_TRIANGLE = 4, 5, 10
+LOOP_CASES = ("sphere", "grid", "strip", "uniform")
 
 
 # ------------------------------------------------------------ golden vectors
@@ -294,7 +296,195 @@ def test_normalization_maps_ptex_space_to_patch_space():
 
 
 def test_unsupported_patch_type_returns_zero_points():
-    assert oracle.patch_basis(5, 0, field1(), 0.3, 0.3)[0] == 0
+    # POINTS, LINES and the legacy GREGORY / GREGORY_BOUNDARY types, which
+    # OsdEvaluatePatchBasis does not evaluate
+    for ptype in (0, 1, 2, 7, 8, 11):
+        assert oracle.patch_basis(ptype, 0, field1(), 0.3, 0.3)[0] == 0
+
+
+# ------------------------------------------------- triangular patch types
+
+def _loop_fixture(name):
+    g = golden("loop_patches.npz")
+    coords = np.ascontiguousarray(g[name + "_coords"]).view(PATCH_COORD_DTYPE).reshape(-1)
+    arrays = np.ascontiguousarray(g[name + "_arrays"]).view(PATCH_ARRAY_DTYPE).reshape(-1)
+    params = np.ascontiguousarray(g[name + "_params"]).view(PATCH_PARAM_DTYPE).reshape(-1)
+    return g, coords, arrays, np.ascontiguousarray(g[name + "_indices"]), params
+
+
+@pytest.mark.parametrize("name", LOOP_CASES)
+def test_loop_patches_vs_cpu_evaluator(name):
+    """LOOP box-spline patches (boundary masks included), GREGORY_TRIANGLE
+    end caps and uniform TRIANGLES: bit-exact against the reference's
+    CpuEvaluator in all three arities."""
+    g, coords, arrays, idx, params = _loop_fixture(name)
+    types = set(arrays["desc"].tolist()) | set(arrays["regDesc"].tolist())
+    assert types == ({TRIANGLES} if name == "uniform" else {LOOP, GREGORY_TRIANGLE})
+    n = len(coords)
+    o = [np.zeros((n, 3), np.float32) for _ in range(3)]
+    assert oracle.eval_patches(g[name + "_src"], (0, 3, 3), o, [(0, 3, 3)] * 3, coords, arrays,
+                               idx, params)
+    for x, k in zip(o, ("cpu_p", "cpu_du", "cpu_dv")):
+        np.testing.assert_array_equal(x, g[f"{name}_{k}"], err_msg=k)
+    o2 = [np.zeros((n, 3), np.float32) for _ in range(6)]
+    assert oracle.eval_patches(g[name + "_src"], (0, 3, 3), o2, [(0, 3, 3)] * 6, coords, arrays,
+                               idx, params)
+    for x, k in zip(o2, ("cpu2_p", "cpu2_du", "cpu2_dv", "cpu2_duu", "cpu2_duv", "cpu2_dvv")):
+        np.testing.assert_array_equal(x, g[f"{name}_{k}"], err_msg=k)
+
+
+def test_loop_fixture_covers_boundaries_and_rotated_triangles():
+    masks, rotated = set(), 0
+    for name in LOOP_CASES:
+        g, coords, arrays, idx, params = _loop_fixture(name)
+        f1 = params["field1"][coords["patchIndex"]].astype(np.int64)
+        depth, u, v = f1 & 0xF, (f1 >> 22) & 0x3FF, (f1 >> 12) & 0x3FF
+        masks |= set(((f1 >> 7) & 0x1F).tolist())
+        rotated += int((u + v >= (1 << depth)).sum())
+    # edge masks, vertex-only masks (upper bits 1) and edge + opposite
+    # vertex masks (upper bits 2), and rotated sub-triangles
+    assert {1, 2, 3, 4, 6} <= masks and {9, 10, 12} <= masks and {17, 18} <= masks
+    assert rotated > 100
+
+
+@pytest.mark.parametrize("name", LOOP_CASES)
+def test_triangle_basis_f64_vs_far_evaluate_basis(name):
+    g, coords, arrays, idx, params = _loop_fixture(name)
+    for i in range(len(g[name + "_basis_n"])):
+        c = coords[i]
+        a = arrays[c["arrayIndex"]]
+        p = params[c["patchIndex"]]
+        ptype = a["regDesc"] if (p["field1"] >> 5) & 1 else a["desc"]
+        n, wP, wDs, wDt, *_ = oracle.patch_basis(ptype, p["field0"], p["field1"],
+                                                 float(c["s"]), float(c["t"]), 1, np.float64)
+        assert n == g[name + "_basis_n"][i]
+        np.testing.assert_allclose(wP[:n], g[name + "_basis_wP"][i, :n], atol=1e-13)
+        np.testing.assert_allclose(wDs[:n], g[name + "_basis_wDs"][i, :n], atol=1e-11)
+        np.testing.assert_allclose(wDt[:n], g[name + "_basis_wDt"][i, :n], atol=1e-11)
+
+
+def bernstein_tri(i, j, u, v, du=0, dv=0):
+    """Quartic Bernstein triangle polynomial 4!/(i! j! k!) u^i v^j w^k
+    (k = 4 - i - j, w = 1 - u - v) and its u/v derivatives, by sympy-free
+    differentiation of the monomial expansion in (u, v)."""
+    k = 4 - i - j
+    c = math.factorial(4) / (math.factorial(i) * math.factorial(j) * math.factorial(k))
+    total = 0.0
+    # w^k = sum_a sum_b k!/(a! b! (k-a-b)!) (-u)^a (-v)^b
+    for a in range(k + 1):
+        for b in range(k + 1 - a):
+            m = math.factorial(k) / (math.factorial(a) * math.factorial(b) *
+                                     math.factorial(k - a - b)) * (-1) ** (a + b)
+            pu, pv = i + a, j + b
+            if pu < du or pv < dv:
+                continue
+            fu = math.perm(pu, du) * u ** (pu - du)
+            fv = math.perm(pv, dv) * v ** (pv - dv)
+            total += m * fu * fv
+    return c * total
+
+
+# Gregory triangle point -> exponents (i of u, j of v) of the Bezier point it
+# stands for: corner k's point, its two edge neighbours and its interior
+# pair, for corners (0,0), (1,0), (0,1), then the three edge midpoints.
+GREG_TRI_UV = {0: (0, 0), 1: (1, 0), 2: (0, 1), 5: (4, 0), 6: (3, 1), 7: (3, 0),
+               10: (0, 4), 11: (0, 3), 12: (1, 3), 15: (2, 0), 16: (2, 2), 17: (0, 2)}
+GREG_TRI_PAIRS = {(3, 4): (1, 1), (8, 9): (2, 1), (13, 14): (1, 2)}
+
+
+@pytest.mark.parametrize("s,t", [(0.2, 0.3), (0.6, 0.1), (0.05, 0.9), (1 / 3, 1 / 3)])
+def test_gregory_triangle_reduces_to_quartic_bezier_triangle(s, t):
+    """Each Gregory boundary point carries its Bernstein weight and each
+    interior pair sums to one (the approximate derivatives hold the rational
+    factors fixed, so derivatives follow the same rule)."""
+    n, *w = oracle.patch_basis(GREGORY_TRIANGLE, 0, field1(), s, t, 2, np.float64)
+    assert n == 18
+    for o, (du, dv) in enumerate([(0, 0), (1, 0), (0, 1), (2, 0), (1, 1), (0, 2)]):
+        for p, (i, j) in GREG_TRI_UV.items():
+            assert abs(w[o][p] - bernstein_tri(i, j, s, t, du, dv)) < 1e-11, (o, p)
+        for (p, q), (i, j) in GREG_TRI_PAIRS.items():
+            assert abs(w[o][p] + w[o][q] - bernstein_tri(i, j, s, t, du, dv)) < 1e-11, (o, p)
+    assert 0.0 < w[0][3] / (w[0][3] + w[0][4]) < 1.0
+
+
+def test_box_spline_corners_are_loop_limit_masks():
+    """At the three corners of a regular patch the box spline is the Loop
+    limit mask of a valence-6 vertex: 1/2 on the vertex, 1/12 on each of its
+    six neighbours, 0 on the other five points."""
+    corners = {}
+    for s, t in [(0.0, 0.0), (1.0, 0.0), (0.0, 1.0)]:
+        n, wP, *_ = oracle.patch_basis(LOOP, 0, field1(), s, t, 0, np.float64)
+        assert n == 12
+        w = wP[:12]
+        center = int(np.argmax(w))
+        assert abs(w[center] - 0.5) < 1e-15
+        assert np.sum(np.abs(w - 1 / 12) < 1e-15) == 6
+        assert np.sum(w == 0) == 5
+        corners[(s, t)] = center
+    assert len(set(corners.values())) == 3
+
+
+@pytest.mark.parametrize("ptype", [LOOP, GREGORY_TRIANGLE, TRIANGLES])
+@pytest.mark.parametrize("depth,u,v", [(0, 0, 0), (2, 1, 1), (2, 2, 1), (3, 3, 5)])
+def test_triangle_bases_unity_and_ptex_space_derivatives(ptype, depth, u, v):
+    """Partition of unity, zero-sum derivatives, and first and second
+    derivatives equal to finite differences in ptex-face (s, t), for upright
+    and rotated (u + v >= 2^depth) sub-triangles: pins the triangle
+    normalization, the 2^depth / 4^depth scaling and the sign flip.  The
+    Gregory triangle's approximate derivatives hold its rational factors
+    fixed, so there the interior pairs are compared summed.
+
+    Reference quirk kept: OsdEvaluatePatchBasis scales second derivatives by
+    derivSign * d1Scale^2 (osd/patchBasis.h:1590-1596), so on a rotated
+    triangle they are the negated ptex-space second derivatives (the chain
+    rule gives +4^depth there); the golden vectors hold the reference's
+    sign."""
+    f1 = field1(depth, u, v, regular=int(ptype == LOOP))
+    rotated = u + v >= (1 << depth)
+    rng = np.random.default_rng(depth * 7 + u + ptype)
+    h = 1e-5
+    for _ in range(4):
+        a, b = rng.uniform(0.1, 0.8), rng.uniform(0.05, 0.15)
+        sl, tl = a * (1 - b) * 0.9 + 0.02, b                 # inside the local triangle
+        if rotated:
+            s, t = ((1 << depth) - u - sl) / (1 << depth), ((1 << depth) - v - tl) / (1 << depth)
+        else:
+            s, t = (u + sl) / (1 << depth), (v + tl) / (1 << depth)
+        n, wP, wDs, wDt, wDss, wDst, wDtt = oracle.patch_basis(ptype, 0, f1, s, t, 2,
+                                                                np.float64)
+        assert abs(wP[:n].sum() - 1) < 1e-13
+        for w in (wDs, wDt, wDss, wDst, wDtt):
+            assert abs(w[:n].sum()) < 1e-8
+
+        def merge(w):
+            w = np.array(w[:n])
+            if ptype == GREGORY_TRIANGLE:
+                for p, q in GREG_TRI_PAIRS:
+                    w[p], w[q] = w[p] + w[q], 0.0
+            return w
+
+        def P(ds=0.0, dt=0.0, k=0):
+            return merge(oracle.patch_basis(ptype, 0, f1, s + ds, t + dt, 1, np.float64)[1 + k])
+        wDs, wDt, wDss, wDst, wDtt = map(merge, (wDs, wDt, wDss, wDst, wDtt))
+        scale = float(1 << depth)
+        sign2 = -1.0 if rotated else 1.0
+        np.testing.assert_allclose(wDs[:n], (P(h) - P(-h)) / (2 * h), atol=1e-5 * scale)
+        np.testing.assert_allclose(wDt[:n], (P(0, h) - P(0, -h)) / (2 * h), atol=1e-5 * scale)
+        np.testing.assert_allclose(wDss[:n], sign2 * (P(h, 0, 1) - P(-h, 0, 1)) / (2 * h),
+                                   atol=1e-4 * scale * scale)
+        np.testing.assert_allclose(wDst[:n], sign2 * (P(0, h, 1) - P(0, -h, 1)) / (2 * h),
+                                   atol=1e-4 * scale * scale)
+        np.testing.assert_allclose(wDtt[:n], sign2 * (P(0, h, 2) - P(0, -h, 2)) / (2 * h),
+                                   atol=1e-4 * scale * scale)
+
+
+def test_linear_triangle_closed_form():
+    n, wP, wDs, wDt, *_ = oracle.patch_basis(TRIANGLES, 0, field1(regular=0), 0.2, 0.7, 1,
+                                             np.float64)
+    assert n == 3
+    np.testing.assert_allclose(wP[:3], [0.1, 0.2, 0.7], atol=1e-15)
+    np.testing.assert_array_equal(wDs[:3], [-1, 1, 0])
+    np.testing.assert_array_equal(wDt[:3], [-1, 0, 1])
 
 
 # ---------------------------------------------------------------- brute force
