@@ -452,8 +452,8 @@ def run_patches(args, lib, inputs, with_cpu):
                 raise RuntimeError(f"{entry} failed: {rc}")
         return step
     alg = patch_bytes(n, L, 3, len(pt.indices), len(pt.params), len(pt.arrays), nctrl + npts)
-    # autotune: caller order vs patch-sorted order (radix sort inside each
-    # call, included in the timed step)
+    # autotune: caller order vs patch-sorted order (counting sort by patch
+    # inside each call, included in the timed step)
     entries = {"caller_order": "osdhip_eval_patches", "patch_sorted": "osdhip_eval_patches_sorted"}
     variants = {}
     for vname, entry in entries.items():

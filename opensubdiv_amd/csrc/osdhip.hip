@@ -29,6 +29,7 @@
 #include "common.h"
 #include "patch_kernels.h"
 #include "patch_map.h"
+#include "patch_sort.h"
 #include "stencil_csr.h"
 #include "stencil_sliced.h"
 #include "stencil_tiled.h"
@@ -709,11 +710,15 @@ int osdhip_stencil_table_eval(const osdhip_stencil_table *t, const float *src,
     return OSDHIP_OK;
 }
 
+// How eval_patches orders its coords: caller order, or patch order with the
+// sort's workspace from hipMallocAsync / from the caller (`ws`).
+enum class PatchOrder { Caller, SortedMalloc, SortedWorkspace };
+
 static int eval_patches(const float *src, osdhip_buffer_desc srcDesc, float *const *outs,
                         const osdhip_buffer_desc *odsIn, int numPatchCoords,
                         const osdhip_patch_coord *coords, const osdhip_patch_array *arrays,
-                        const int *patchIndices, const osdhip_patch_param *params, bool sorted,
-                        void *stream)
+                        const int *patchIndices, const osdhip_patch_param *params,
+                        PatchOrder order, char *ws, void *stream)
 {
     float *dst = outs[0], *du = outs[1], *dv = outs[2], *duu = outs[3], *duv = outs[4],
           *dvv = outs[5];
@@ -735,29 +740,25 @@ static int eval_patches(const float *src, osdhip_buffer_desc srcDesc, float *con
     a.params = params;
     hipStream_t st = (hipStream_t)stream;
 
-    // Patch-ordered evaluation for large coord sets: radix-sort (patch key,
-    // coord index) pairs in stream-ordered scratch memory, then evaluate in
-    // that order.  Outputs still go to each coord's own row.
-    void *ws = nullptr;
-    if (sorted) {
-        const int n = numPatchCoords;
-        size_t tmpBytes = 0;
-        hipcub::DoubleBuffer<unsigned> kb(nullptr, nullptr);
-        hipcub::DoubleBuffer<int> vb(nullptr, nullptr);
-        hipcub::DeviceRadixSort::SortPairs(nullptr, tmpBytes, kb, vb, n, 0, 24, st);
-        const size_t arr = ((size_t)n * 4 + 255) & ~(size_t)255;
-        if (hipMallocAsync(&ws, 4 * arr + tmpBytes, st) == hipSuccess) {
-            char *b = (char *)ws;
-            kb = hipcub::DoubleBuffer<unsigned>((unsigned *)b, (unsigned *)(b + arr));
-            vb = hipcub::DoubleBuffer<int>((int *)(b + 2 * arr), (int *)(b + 3 * arr));
-            hipLaunchKernelGGL(k_patch_keys, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
-                               coords, n, kb.Current(), vb.Current());
-            if (hipcub::DeviceRadixSort::SortPairs(b + 4 * arr, tmpBytes, kb, vb, n, 0, 24, st) ==
-                hipSuccess)
-                a.order = vb.Current();
+    // Patch-ordered evaluation: a counting sort of the coords by patch in
+    // stream-ordered scratch memory, then evaluation in that order.
+    // Outputs still go to each coord's own row.
+    void *owned = nullptr;
+    if (order == PatchOrder::SortedMalloc) {
+        if (hipMallocAsync(&owned, PatchSortLayout(numPatchCoords).bytes, st) == hipSuccess) {
+            ws = (char *)owned;
         } else {
-            (void)hipGetLastError();
+            (void)hipGetLastError();          // no scratch: caller order, same results
+            owned = nullptr;
             ws = nullptr;
+        }
+    }
+    if (order != PatchOrder::Caller && ws) {
+        a.order = queue_patch_sort(coords, numPatchCoords, ws, st);
+        int rc = last_error();
+        if (rc != OSDHIP_OK) {
+            if (owned) (void)hipFreeAsync(owned, st);
+            return rc;
         }
     }
     int rc = OSDHIP_OK;
@@ -773,8 +774,8 @@ static int eval_patches(const float *src, osdhip_buffer_desc srcDesc, float *con
         else launch_patches<false, false>(lc, a, st);
         rc = last_error();
     }
-    if (ws) {
-        int rf = (int)hipFreeAsync(ws, st);
+    if (owned) {
+        int rf = (int)hipFreeAsync(owned, st);
         if (rc == OSDHIP_OK) rc = rf;
     }
     return rc;
@@ -797,7 +798,7 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
     float *outs[6] = { dst, du, dv, duu, duv, dvv };
     const osdhip_buffer_desc ods[6] = { dstDesc, duDesc, dvDesc, duuDesc, duvDesc, dvvDesc };
     return eval_patches(src, srcDesc, outs, ods, numPatchCoords, coords, arrays, patchIndices,
-                        params, false, stream);
+                        params, PatchOrder::Caller, nullptr, stream);
 }
 
 int osdhip_eval_patches_sorted(const float *src, osdhip_buffer_desc srcDesc,
@@ -817,7 +818,7 @@ int osdhip_eval_patches_sorted(const float *src, osdhip_buffer_desc srcDesc,
     float *outs[6] = { dst, du, dv, duu, duv, dvv };
     const osdhip_buffer_desc ods[6] = { dstDesc, duDesc, dvDesc, duuDesc, duvDesc, dvvDesc };
     return eval_patches(src, srcDesc, outs, ods, numPatchCoords, coords, arrays, patchIndices,
-                        params, true, stream);
+                        params, PatchOrder::SortedMalloc, nullptr, stream);
 }
 
 struct osdhip_patch_map {

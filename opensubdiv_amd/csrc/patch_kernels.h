@@ -76,18 +76,6 @@ __device__ __forceinline__ void gather_points(const float *src, size_t sstride, 
     }
 }
 
-// Sort keys for the patch-ordered evaluation: the low 24 bits of the patch
-// index (any key gives a valid permutation; these give full locality up to
-// 16.7 M patches), value = coord index.
-__global__ __launch_bounds__(kBlock) void k_patch_keys(const osdhip_patch_coord *coords, int n,
-                                                       unsigned *keys, int *vals)
-{
-    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
-    if (i >= n) return;
-    keys[i] = (unsigned)coords[i].patchIndex & 0xffffffu;
-    vals[i] = i;
-}
-
 template <bool D1, bool D2, int LC>
 __global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
 {

@@ -265,8 +265,8 @@ def test_python_mirror_eval_patches(hip):
 
 @pytest.mark.parametrize("deriv", [0, 1, 2])
 def test_patch_sorted_order_bit_exact(hip, deriv):
-    """Patch-ordered evaluation (osdhip_eval_patches_sorted: radix sort of
-    the coords by patch) returns every coord's result in its own row,
+    """Patch-ordered evaluation (osdhip_eval_patches_sorted: counting sort
+    of the coords by patch) returns every coord's result in its own row,
     bit-identical to caller order and to the oracle, on a small and a large
     set (coords drawn with repeats and in random order)."""
     g, src, _, arrays, indices, params = golden_patches()
@@ -288,6 +288,35 @@ def test_patch_sorted_order_bit_exact(hip, deriv):
         for k in range(nout):
             np.testing.assert_array_equal(res[True][k], want[k], err_msg=f"sorted {n} {k}")
             np.testing.assert_array_equal(res[False][k], want[k], err_msg=f"caller {n} {k}")
+
+
+def test_patch_sorted_order_beyond_the_bucket_count(hip):
+    """More patches than the sort's 2^20 buckets (patch p and p + 2^20 share
+    a bucket), coords drawn with repeats: still every coord's own result,
+    bit-identical to caller order and to the oracle."""
+    rng = np.random.default_rng(91)
+    npatch, nsrc, n = (1 << 20) + 4096, 5000, 60000
+    arrays = np.array([(3, 3, npatch, 0, 4, 0)], PATCH_ARRAY_DTYPE)
+    params = np.zeros(npatch, PATCH_PARAM_DTYPE)                 # depth 0, not regular
+    indices = rng.integers(0, nsrc, size=4 * npatch).astype(np.int32)
+    pick = np.concatenate([rng.integers(0, npatch, size=n // 2),
+                           (1 << 20) + rng.integers(0, 4096, size=n // 4),
+                           rng.integers(0, 4096, size=n - n // 2 - n // 4)])
+    coords = np.zeros(n, PATCH_COORD_DTYPE)
+    coords["patchIndex"] = rng.permutation(pick)
+    coords["s"], coords["t"] = rng.uniform(size=n), rng.uniform(size=n)
+    src = rng.normal(size=(nsrc, 3)).astype(np.float32)
+    want = [np.zeros((n, 3), np.float32) for _ in range(3)]
+    assert oracle.eval_patches(src, (0, 3, 3), want, [(0, 3, 3)] * 3, coords, arrays, indices,
+                               params)
+    d = [dev(x) for x in (src, coords, arrays, indices, params)]
+    for srt in (True, False):
+        outs = [empty(n * 3) for _ in range(3)]
+        assert eval_patches(hip, d[0], (0, 3, 3), outs, [(0, 3, 3)] * 3, n, *d[1:],
+                            sorted=srt) == 0
+        for k in range(3):
+            np.testing.assert_array_equal(host(outs[k], shape=(n, 3)), want[k],
+                                          err_msg=f"sorted={srt} {k}")
 
 
 def normals(lib, du, dd, dv, vd, n, nd, count, normalize):
