@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_patches(PatchArgs a)
 
     const size_t sstride = (size_t)a.srcStride;
     Basis<D1, D2> B;
-    if (n >= 16) {
+    if (n == 16 || n == 20) {
         float v[16][LC];
 #pragma unroll
         for (int j = 0; j < 16; ++j) {
