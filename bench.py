@@ -662,7 +662,8 @@ def run_batched(args, lib, device, copy_gbps=None):
     workers = min(16, os.cpu_count() or 1)
     cache = table_cache(args)
     dms = workloads.build_device_meshes(specs, L, s, workers=workers, log=log, own_outputs=False,
-                                        cache=cache, layout=args.table_layout)
+                                        cache=cache, layout=args.table_layout,
+                                        order=args.table_order)
     torch.cuda.synchronize()
     build_s = time.time() - t0
     # outputs (refined vertices, P / dPdu / dPdv of every frame) stream
@@ -745,6 +746,7 @@ def run_batched(args, lib, device, copy_gbps=None):
                                "step (stencils then patches)",
                    "meshes": len(dms), "seed": seed, "floats_per_point": L, "device": device,
                    "stages": nstages, "table_layout": args.table_layout,
+                   "table_order": args.table_order,
                    "output_bytes_per_step": out_bytes,
                    "refined_rows_per_step": sum(d.spec.frames * d.nrefine for d in dms),
                    "patch_points_per_step": sum(d.spec.frames * d.ncoords for d in dms),
@@ -1012,6 +1014,8 @@ def main():
                          "rest are compiled and stored")
     ap.add_argument("--table-layout", type=int, default=2, choices=[0, 2],
                     help="cfg3/cfg4: stencil table layout (0 record slices, 2 locality-ordered)")
+    ap.add_argument("--table-order", type=int, default=0, choices=[0, 1],
+                    help="cfg3: layout-2 row order (0 by dominant control vertex, 1 Far's order)")
     ap.add_argument("--config", default=None,
                     help="cfg3: batch knobs key=value,... set before (or instead of) tuning")
     ap.add_argument("--no-tune", action="store_true", help="cfg3: skip the autotuner")

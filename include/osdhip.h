@@ -138,12 +138,18 @@ osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *si
  *   group:   entries per gather group of the slice kernels: 4, 8 (default)
  *            or 16 (layout 2 uses 4, or 8 for 8 and 16).
  *   kernel:  osdhip_stencil_table_eval kernel: 0 = slice layout (default),
- *            1 = CSR with LDS staging (needs keepCsr). */
+ *            1 = CSR with LDS staging (needs keepCsr).
+ *   order:   layout 2's row order: 0 = by dominant control vertex (default;
+ *            the fewest control vertices per slice), 1 = Far's row order
+ *            (a slice's outputs are 64 consecutive rows; slices with more
+ *            distinct control vertices than the LDS list holds gather
+ *            them from memory). */
 typedef struct {
     int layout;
     int keepCsr;
     int group;
     int kernel;
+    int order;
 } osdhip_table_options;
 
 void osdhip_table_options_init(osdhip_table_options *options);

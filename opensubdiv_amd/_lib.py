@@ -33,9 +33,9 @@ def desc(d) -> Desc:
 
 
 class TableOptions(C.Structure):
-    """osdhip_table_options (layout, keepCsr, group, kernel)."""
+    """osdhip_table_options (layout, keepCsr, group, kernel, order)."""
     _fields_ = [("layout", C.c_int), ("keepCsr", C.c_int), ("group", C.c_int),
-                ("kernel", C.c_int)]
+                ("kernel", C.c_int), ("order", C.c_int)]
 
 
 class BatchStencils(C.Structure):

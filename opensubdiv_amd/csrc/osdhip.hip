@@ -279,7 +279,7 @@ static bool build_tiled(osdhip_stencil_table *t, hipStream_t st)
               dalloc(&t->nuniq, (size_t)S, t->bytes) && dalloc(&t->ctrlOff, (size_t)S, t->bytes);
     if (ok) {
         hipLaunchKernelGGL(k_row_dominant, dim3(gr), dim3(kBlock), 0, st, t->sizes, t->offsets,
-                           t->indices, W[0], n, rows, keys, vals);
+                           t->indices, W[0], n, rows, t->order, keys, vals);
         ok = hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, keys, keysOut, vals, t->perm, rows,
                                                 0, 32, st) == hipSuccess &&
              hipMalloc(&sortTmp, std::max<size_t>(sortBytes, 1)) == hipSuccess &&
@@ -386,13 +386,14 @@ void osdhip_table_options_init(osdhip_table_options *o)
     o->keepCsr = 1;
     o->group = 8;
     o->kernel = 0;
+    o->order = 0;
 }
 
 static bool table_options_ok(const osdhip_table_options &o)
 {
     return o.layout >= 0 && o.layout <= 2 && (o.keepCsr == 0 || o.keepCsr == 1) &&
            (o.group == 4 || o.group == 8 || o.group == 16) && (o.kernel == 0 || o.kernel == 1) &&
-           !(o.kernel == 1 && !o.keepCsr);
+           !(o.kernel == 1 && !o.keepCsr) && (o.order == 0 || o.order == 1);
 }
 
 osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *sizes,
@@ -427,6 +428,7 @@ osdhip_stencil_table *osdhip_stencil_table_create_ex(int numStencils, const int 
     t->keepCsr = opt.keepCsr;
     t->group = opt.group;
     t->kernel = opt.kernel;
+    t->order = opt.order;
     hipStream_t st = (hipStream_t)stream;
 
     // CSR mirror (offsets regenerated when absent: exclusive prefix of sizes)
@@ -552,6 +554,7 @@ int osdhip_stencil_table_options(const osdhip_stencil_table *t, osdhip_table_opt
     o->keepCsr = t->keepCsr;
     o->group = t->group;
     o->kernel = t->kernel;
+    o->order = t->order;
     return OSDHIP_OK;
 }
 

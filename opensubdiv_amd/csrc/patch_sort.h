@@ -15,26 +15,11 @@
 
 #include "../../include/osdhip.h"
 #include "common.h"
+#include "patch_sort_layout.h"
 
 namespace osdhip {
 
-constexpr int kPatchBuckets = 1 << 20;
-constexpr int kScanChunk = 4 * kBlock;                    // buckets per scan block
-constexpr int kScanChunks = kPatchBuckets / kScanChunk;   // 1024
-static_assert(kScanChunks == kScanChunk, "one block scans the chunk sums");
-
-struct PatchSortLayout {
-    size_t counts, offsets, sums, order, bytes;
-    explicit PatchSortLayout(long long n)
-    {
-        auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-        counts = 0;
-        offsets = up(counts + 4 * (size_t)kPatchBuckets);
-        sums = up(offsets + 4 * (size_t)kPatchBuckets);
-        order = up(sums + 4 * (size_t)kScanChunks);
-        bytes = up(order + 4 * (size_t)n);
-    }
-};
+static_assert(kScanChunk == 4 * kBlock, "a scan block takes 4 buckets per thread");
 
 __device__ __forceinline__ unsigned patch_bucket(int patchIndex)
 {

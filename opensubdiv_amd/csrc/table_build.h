@@ -89,21 +89,27 @@ __global__ __launch_bounds__(kBlock) void k_fill_split(
 // |weight|, the first one on ties; a stable sort keeps Far order among
 // equal keys), so the 64 rows of a slice surround a few control vertices and
 // share most of their support: 26-38 distinct control vertices per slice on
-// creased cube-spheres at levels 2-4, against 200+ in Far order.  Per slice
+// creased cube-spheres at levels 2-4, against 30 (level 4) to 100 (level 2)
+// on average in Far order -- which keeps a slice's 64 outputs consecutive
+// (order option 1).  Per slice
 // the distinct control vertices are listed in ascending order; when there
 // are at most kTileZero of them the slice's records hold list positions
 // (LDS slots), otherwise control indices.
 
-// Sort key per slice row: the dominant control vertex (rows past n and
-// empty rows: 0xffffffff, after every real key); value: the row.
+// Sort key per slice row: the dominant control vertex (empty rows:
+// 0xffffffff), or with farOrder the row itself (Far's order kept); rows
+// past n: 0xffffffff, after every real key; value: the row.
 __global__ __launch_bounds__(kBlock) void k_row_dominant(const int *sizes, const int *offsets,
                                                          const int *indices, const float *w0, int n,
-                                                         int rows, unsigned *keys, int *vals)
+                                                         int rows, int farOrder, unsigned *keys,
+                                                         int *vals)
 {
     const int r = (int)(blockIdx.x * kBlock + threadIdx.x);
     if (r >= rows) return;
     unsigned key = 0xffffffffu;
-    if (r < n && sizes[r] > 0) {
+    if (farOrder) {
+        if (r < n) key = (unsigned)r;
+    } else if (r < n && sizes[r] > 0) {
         const long long off = offsets[r];
         key = (unsigned)indices[off];
         float best = fabsf(w0[off]);

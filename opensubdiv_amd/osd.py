@@ -122,15 +122,17 @@ class HipStencilTable:
     (record slices), 1 (split streams) or 2 (locality-ordered slices staged
     in LDS); ``keep_csr`` keeps the Far CSR arrays on the device; ``group``
     (4, 8, 16) and ``kernel`` (0 slices, 1 CSR) select the evaluation
-    kernel and can be changed later with ``set``."""
+    kernel and can be changed later with ``set``; ``order`` (layout 2) 0
+    groups rows by dominant control vertex, 1 keeps Far's row order."""
 
     _KEYS = ["weights", "du", "dv", "duu", "duv", "dvv"]
     _GETTERS = ["GetWeights", "GetDuWeights", "GetDvWeights", "GetDuuWeights",
                 "GetDuvWeights", "GetDvvWeights"]
 
-    def __init__(self, table, stream=None, layout=0, keep_csr=True, group=8, kernel=0):
+    def __init__(self, table, stream=None, layout=0, keep_csr=True, group=8, kernel=0, order=0):
         import ctypes
-        opts = _lib.TableOptions(int(layout), int(bool(keep_csr)), int(group), int(kernel))
+        opts = _lib.TableOptions(int(layout), int(bool(keep_csr)), int(group), int(kernel),
+                                 int(order))
         if hasattr(table, "weights_ptr"):            # far.RawStencilTable
             lib = _lib.load_hip()
             wptrs = (ctypes.c_void_p * 1)(table.weights_ptr)
@@ -199,7 +201,7 @@ class HipStencilTable:
         _lib.check(_lib.load_hip().osdhip_stencil_table_options(self.h, ctypes.byref(o)),
                    "osdhip_stencil_table_options")
         return {"layout": o.layout, "keep_csr": bool(o.keepCsr), "group": o.group,
-                "kernel": o.kernel}
+                "kernel": o.kernel, "order": o.order}
 
     def GetSizesBuffer(self): return self._csr[0]
     def GetOffsetsBuffer(self): return self._csr[1]

@@ -181,7 +181,8 @@ def compile_mesh(spec: MeshSpec, length=7, threads=1, cache=None) -> HostMesh:
 class DeviceMesh:
     """One mesh's device data and its batch segments (see module docstring)."""
 
-    def __init__(self, h: HostMesh, stream=None, own_outputs=True, layout=2, keep_csr=False):
+    def __init__(self, h: HostMesh, stream=None, own_outputs=True, layout=2, keep_csr=False,
+                 order=0):
         import torch
         from .osd import HipStencilTable, _device_tensor
         s = h.spec
@@ -207,9 +208,12 @@ class DeviceMesh:
         if own_outputs:
             self.bind_outputs(torch.empty(self.output_floats(), dtype=torch.float32, device=dev))
         # stencil tables in slice layout `layout` (0 record slices, 2
-        # locality-ordered); the batch kernels need no CSR arrays
-        self.refine_table = HipStencilTable(h.refine, stream, layout=layout, keep_csr=keep_csr)
-        self.point_table = HipStencilTable(h.points, stream, layout=layout, keep_csr=keep_csr)
+        # locality-ordered, rows by dominant vertex (order 0) or in Far's
+        # order (order 1)); the batch kernels need no CSR arrays
+        self.refine_table = HipStencilTable(h.refine, stream, layout=layout, keep_csr=keep_csr,
+                                            order=order)
+        self.point_table = HipStencilTable(h.points, stream, layout=layout, keep_csr=keep_csr,
+                                           order=order)
         self.coords = _device_tensor(h.coords)
         self.arrays = _device_tensor(h.arrays)
         self.indices = _device_tensor(h.indices)
@@ -296,19 +300,19 @@ def stage_outputs(dms, arena_floats):
 
 
 def build_device_meshes(specs, length=7, stream=None, workers=8, log=None, own_outputs=True,
-                        cache=None, layout=2):
+                        cache=None, layout=2, order=0):
     """Compiles the meshes on `workers` host threads (largest first), or
     loads them from a table_cache.TableCache, and uploads each as it
-    completes (tables in slice layout `layout`); returns DeviceMeshes in
-    spec order (with their own output buffers, or none for
+    completes (tables in slice layout `layout`, row order `order`); returns
+    DeviceMeshes in spec order (with their own output buffers, or none for
     stage_outputs)."""
-    order = sorted(specs, key=lambda s: -spec_rows(s)[1])
+    largest_first = sorted(specs, key=lambda s: -spec_rows(s)[1])
     done = {}
     with ThreadPoolExecutor(max_workers=max(1, int(workers))) as ex:
-        futs = {ex.submit(compile_mesh, s, length, 1, cache): s for s in order}
+        futs = {ex.submit(compile_mesh, s, length, 1, cache): s for s in largest_first}
         for k, fut in enumerate(as_completed(futs)):
             h = fut.result()
-            done[h.spec.index] = DeviceMesh(h, stream, own_outputs, layout=layout)
+            done[h.spec.index] = DeviceMesh(h, stream, own_outputs, layout=layout, order=order)
             h.release()
             del h
             if log and (k + 1) % 16 == 0:

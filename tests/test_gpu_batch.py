@@ -29,10 +29,10 @@ SPECS = [MeshSpec(0, 3, 2, 3, 700, 11), MeshSpec(1, 2, 1, 1, 300, 12),
          MeshSpec(6, 6, 4, 3, 2000, 17)]
 
 
-def build(length, layout=0):
+def build(length, layout=0, order=0):
     import torch
     hosts = [compile_mesh(s, length) for s in SPECS]
-    dms = [DeviceMesh(h, layout=layout) for h in hosts]
+    dms = [DeviceMesh(h, layout=layout, order=order) for h in hosts]
     for h in hosts:
         h.release()
     torch.cuda.synchronize()
@@ -66,17 +66,19 @@ def same(a, b, what):
             assert np.array_equal(u.view(np.uint32), v.view(np.uint32)), f"{what}: mesh {i} buffer {k}"
 
 
-# (floats per vertex, table layout): record slices and locality-ordered
-# slices (LDS-staged control points)
-@pytest.fixture(scope="module", params=[(7, 0), (3, 0), (7, 2), (3, 2)],
-                ids=["L7-rec", "L3-rec", "L7-tiled", "L3-tiled"])
+# (floats per vertex, table layout, row order): record slices and
+# locality-ordered slices (LDS-staged control points), rows by dominant
+# vertex or in Far's order
+@pytest.fixture(scope="module", params=[(7, 0, 0), (3, 0, 0), (7, 2, 0), (3, 2, 0), (7, 2, 1)],
+                ids=["L7-rec", "L3-rec", "L7-tiled", "L3-tiled", "L7-tiledfar"])
 def workload(request):
-    length, layout = request.param
-    hosts, dms = build(length, layout)
+    length, layout, order = request.param
+    hosts, dms = build(length, layout, order)
     if layout == 2:
         for d in dms:
             info = d.refine_table.layout()
-            assert info["layout"] == 2 and info["listed_slices"] >= 0.9 * info["slices"]
+            assert info["layout"] == 2 and info["listed_slices"] > 0
+            assert order == 1 or info["listed_slices"] >= 0.9 * info["slices"]
     batch = Batch(length, [s for d in dms for s in d.stencil_segments()],
                   [d.patch_segment() for d in dms])
     batch.layout = layout

@@ -215,11 +215,13 @@ def table_eval(hip, h, src, sd, outs, descs, start, end):
                                          ctypes.cast(ds, ctypes.c_void_p), start, end, stream())
 
 
-# (table_layout, slice_group, stencil_kernel): record slices with gather
-# groups of 8 and 4, split-stream slices, the CSR kernel on the mirror, and
-# locality-ordered slices (random tables: mostly gathers from memory)
-VARIANTS = {"rec8": (0, 8, 0), "rec4": (0, 4, 0), "rec16": (0, 16, 0), "split": (1, 8, 0),
-            "csr": (0, 8, 1), "tiled8": (2, 8, 0), "tiled4": (2, 4, 0)}
+# (table_layout, slice_group, stencil_kernel, row order): record slices with
+# gather groups of 8 and 4, split-stream slices, the CSR kernel on the
+# mirror, and locality-ordered slices, rows by dominant vertex or in Far's
+# order (random tables: mostly gathers from memory)
+VARIANTS = {"rec8": (0, 8, 0, 0), "rec4": (0, 4, 0, 0), "rec16": (0, 16, 0, 0),
+            "split": (1, 8, 0, 0), "csr": (0, 8, 1, 0), "tiled8": (2, 8, 0, 0),
+            "tiled4": (2, 4, 0, 0), "tiledfar": (2, 8, 0, 1)}
 
 
 @pytest.mark.parametrize("variant", sorted(VARIANTS))
@@ -230,14 +232,14 @@ def test_device_table_bit_exact(hip, variant, nw, L):
     inside a 64-row slice, empty rows, and P-only evaluation of a derivative
     table."""
     from opensubdiv_amd import HipStencilTable
-    layout, group, kernel = VARIANTS[variant]
+    layout, group, kernel, order = VARIANTS[variant]
     rng = np.random.default_rng(300 + 10 * nw + L + 7 * layout + group + kernel)
     nsrc, n = 600, 4099
     t = random_table(rng, n, nsrc, 18, nw)
     keys = ("weights", "du", "dv", "duu", "duv", "dvv")[:nw]
-    table = HipStencilTable.Create(t, layout=layout, group=group, kernel=kernel)
+    table = HipStencilTable.Create(t, layout=layout, group=group, kernel=kernel, order=order)
     assert table.options() == {"layout": layout, "keep_csr": True, "group": group,
-                               "kernel": kernel}
+                               "kernel": kernel, "order": order}
     assert table.GetNumWeightArrays() == nw
     src = rng.normal(size=(nsrc, L)).astype(np.float32)
     s_t = dev(src)
@@ -273,14 +275,14 @@ def test_device_table_option_errors(hip):
     a = HipStencilTable.Create(t, group=4)
     b = HipStencilTable.Create(t, layout=2, keep_csr=False)
     assert a.set("group", 16) == 4 and a.options()["group"] == 16
-    assert b.options() == {"layout": 2, "keep_csr": False, "group": 8, "kernel": 0}
+    assert b.options() == {"layout": 2, "keep_csr": False, "group": 8, "kernel": 0, "order": 0}
     assert hip.osdhip_stencil_table_set(a.h, b"group", 5) == _lib.ERR_ARGS
     assert hip.osdhip_stencil_table_set(b.h, b"kernel", 1) == _lib.ERR_ARGS   # no CSR kept
     assert hip.osdhip_stencil_table_set(a.h, b"layout", 2) == _lib.ERR_UNSUPPORTED
     assert hip.osdhip_stencil_table_set(None, b"group", 4) == _lib.ERR_ARGS
     o = _lib.TableOptions()
     hip.osdhip_table_options_init(ctypes.byref(o))
-    assert (o.layout, o.keepCsr, o.group, o.kernel) == (0, 1, 8, 0)
+    assert (o.layout, o.keepCsr, o.group, o.kernel, o.order) == (0, 1, 8, 0, 0)
 
 
 def test_device_table_cfg0_and_errors(hip):
@@ -307,23 +309,25 @@ def test_device_table_cfg0_and_errors(hip):
     np.testing.assert_array_equal(host(out2, shape=(n, 3)), g["cpu_out"])
 
 
-def _tiled_table(hip, t, group=8):
+def _tiled_table(hip, t, group=8, order=0):
     from opensubdiv_amd import HipStencilTable
-    return HipStencilTable.Create(t, layout=2, group=group)
+    return HipStencilTable.Create(t, layout=2, group=group, order=order)
 
 
-@pytest.mark.parametrize("group", [4, 8])
-def test_tiled_layout_mesh_tables_stage_in_lds(hip, group):
-    """Locality-ordered slices on cube-sphere tables: every slice of the
-    level-2 refinement table lists its control vertices (LDS path), the
-    limit table mixes listed and gathering slices, and P / all six limit outputs,
-    full tables and sub-ranges, equal the reference's CpuEvaluator outputs
-    bit for bit."""
+@pytest.mark.parametrize("group,order", [(4, 0), (8, 0), (8, 1)])
+def test_tiled_layout_mesh_tables_stage_in_lds(hip, group, order):
+    """Locality-ordered slices on cube-sphere tables: with rows by dominant
+    vertex every slice of the level-2 refinement table lists its control
+    vertices (LDS path), the limit table mixes listed and gathering slices;
+    rows in Far's order mix both; P / all six limit outputs, full tables
+    and sub-ranges, equal the reference's CpuEvaluator outputs bit for
+    bit."""
     g = golden("cubesphere_l2.npz")
     table = _tiled_table(hip, {k: g[k] for k in ("sizes", "offsets", "indices", "weights")},
-                         group)
+                         group, order)
     info = table.layout()
-    assert info["layout"] == 2 and info["listed_slices"] == info["slices"] > 0
+    assert info["layout"] == 2 and 0 < info["listed_slices"] <= info["slices"]
+    assert order == 1 or info["listed_slices"] == info["slices"]
     n = table.GetNumStencils()
     src = dev(g["src"])
     d7 = empty(n * 7)
@@ -344,9 +348,10 @@ def test_tiled_layout_mesh_tables_stage_in_lds(hip, group):
     g = golden("cubesphere_limit.npz")
     keys = ("weights", "du", "dv", "duu", "duv", "dvv")
     table = _tiled_table(hip, {k: g[k] for k in ("sizes", "offsets", "indices") + keys},
-                         group)
+                         group, order)
     info = table.layout()
-    assert 0 < info["listed_slices"] < info["slices"]      # both paths in one table
+    # rows by dominant vertex: both paths in one table
+    assert order == 1 or 0 < info["listed_slices"] < info["slices"]
     n = table.GetNumStencils()
     src = dev(g["src"])
     outs = [empty(n * 3) for _ in range(6)]
