@@ -12,7 +12,9 @@
  * Return codes: OSDHIP_OK (0) on success; OSDHIP_ERR_ARGS (-1) when the
  * arguments are rejected the way Osd::CpuEvaluator rejects them (length
  * mismatch, NULL src/dst); OSDHIP_ERR_UNSUPPORTED (-2) for a request the
- * library does not implement; a positive value is a hipError_t.
+ * library does not implement; OSDHIP_ERR_TOO_LARGE (-3) for a request
+ * beyond a fixed-size entry point's limit; a positive value is a
+ * hipError_t.
  *
  * Interfaces replaced (OpenSubdiv v3.6.0, paths relative to its tree):
  *   osd/cudaEvaluator.cpp:33-68     extern "C" CudaEvalStencils /
@@ -36,6 +38,7 @@ extern "C" {
 #define OSDHIP_OK 0
 #define OSDHIP_ERR_ARGS (-1)
 #define OSDHIP_ERR_UNSUPPORTED (-2)
+#define OSDHIP_ERR_TOO_LARGE (-3)
 
 /* Osd::BufferDescriptor (osd/bufferDescriptor.h:61-103), in floats. */
 typedef struct { int offset, length, stride; } osdhip_buffer_desc;
@@ -240,6 +243,31 @@ int osdhip_eval_patches_sorted(const float *src, osdhip_buffer_desc srcDesc,
                                const int *patchIndices,
                                const osdhip_patch_param *params,
                                void *stream);
+
+/* osdhip_eval_patches_sorted without any allocation: the same kernels and
+ * results, the sort's workspace in a fixed device array the library holds
+ * on each GPU it runs on (OSDHIP_NOALLOC_SCRATCH_BYTES, kept until the
+ * process ends).  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  At most OSDHIP_NOALLOC_MAX_PATCH_COORDS
+ * coords per call (OSDHIP_ERR_TOO_LARGE beyond).  Calls share their GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race;
+ * calls ordered on one stream do not. */
+#define OSDHIP_NOALLOC_MAX_PATCH_COORDS (1 << 22)
+#define OSDHIP_NOALLOC_SCRATCH_BYTES (25u << 20)
+int osdhip_eval_patches_sorted_noalloc(const float *src, osdhip_buffer_desc srcDesc,
+                                       float *dst, osdhip_buffer_desc dstDesc,
+                                       float *du, osdhip_buffer_desc duDesc,
+                                       float *dv, osdhip_buffer_desc dvDesc,
+                                       float *duu, osdhip_buffer_desc duuDesc,
+                                       float *duv, osdhip_buffer_desc duvDesc,
+                                       float *dvv, osdhip_buffer_desc dvvDesc,
+                                       int numPatchCoords,
+                                       const osdhip_patch_coord *coords,
+                                       const osdhip_patch_array *arrays,
+                                       const int *patchIndices,
+                                       const osdhip_patch_param *params,
+                                       void *stream);
 
 /* ---- PatchCoords on the device: Far::PatchMap ------------------------------
  * Far::PatchMap (far/patchMap.h:58-120, far/patchMap.cpp:88-200) held on the

@@ -71,6 +71,8 @@ _HIP_SIGS = {
                             _i, _vp, _vp, _vp, _vp, _vp],
     "osdhip_eval_patches_sorted": [_vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp,
                                    _D, _i, _vp, _vp, _vp, _vp, _vp],
+    "osdhip_eval_patches_sorted_noalloc": [_vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D, _vp, _D,
+                                           _vp, _D, _i, _vp, _vp, _vp, _vp, _vp],
     "osdhip_stencil_table_create": [_i, _vp, _vp, _vp, _i, _vp, _vp],
     "osdhip_stencil_table_create_ex": [_i, _vp, _vp, _vp, _i, _vp, _vp, _vp],
     "osdhip_table_options_init": [_vp],

@@ -804,6 +804,22 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
                         params, PatchOrder::Caller, nullptr, stream);
 }
 
+extern "C++" {
+namespace osdhip {
+// Patch-ordered EvalPatches in a workspace the caller provides
+// (PatchSortLayout(numPatchCoords).bytes), for osdhip_eval_patches_sorted_noalloc.
+int eval_patches_in_workspace(const float *src, osdhip_buffer_desc srcDesc, float *const *outs,
+                              const osdhip_buffer_desc *ods, int numPatchCoords,
+                              const osdhip_patch_coord *coords, const osdhip_patch_array *arrays,
+                              const int *patchIndices, const osdhip_patch_param *params,
+                              char *ws, void *stream)
+{
+    return eval_patches(src, srcDesc, outs, ods, numPatchCoords, coords, arrays, patchIndices,
+                        params, PatchOrder::SortedWorkspace, ws, stream);
+}
+}  // namespace osdhip
+}  // extern "C++"
+
 int osdhip_eval_patches_sorted(const float *src, osdhip_buffer_desc srcDesc,
                                float *dst, osdhip_buffer_desc dstDesc,
                                float *du, osdhip_buffer_desc duDesc,
