@@ -452,7 +452,7 @@ def run_patches(args, lib, inputs, with_cpu):
                 raise RuntimeError(f"{entry} failed: {rc}")
         return step
     alg = patch_bytes(n, L, 3, len(pt.indices), len(pt.params), len(pt.arrays), nctrl + npts)
-    # autotune: caller order vs patch-sorted order (counting sort by patch
+    # autotune: caller order vs patch-sorted order (binning by patch
     # inside each call, included in the timed step)
     entries = {"caller_order": "osdhip_eval_patches", "patch_sorted": "osdhip_eval_patches_sorted"}
     variants = {}

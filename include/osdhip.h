@@ -224,12 +224,13 @@ int osdhip_eval_patches(const float *src, osdhip_buffer_desc srcDesc,
                         const int *patchIndices,
                         const osdhip_patch_param *params,
                         void *stream);
-/* osdhip_eval_patches evaluated in patch order: a counting sort of the
- * coords by patch (patchIndex mod 2^20; 8 MB + 4 bytes per coord of
- * stream-ordered scratch memory, hipMallocAsync; caller order, with the same
- * results, when that allocation fails), then each result still goes to its
- * coord's own row.  Same results, bit for bit.  Pays off when many coords
- * share patches in no particular order.  No counterpart in the reference. */
+/* osdhip_eval_patches evaluated in patch order: the coords are binned by
+ * patch (1024 consecutive patches per bin; up to 4 MB + 4 bytes per coord
+ * of stream-ordered scratch memory, hipMallocAsync; caller order, with the
+ * same results, when that allocation fails), then each result still goes to
+ * its coord's own row.  Same results, bit for bit.  Pays off when many
+ * coords share patches in no particular order.  No counterpart in the
+ * reference. */
 int osdhip_eval_patches_sorted(const float *src, osdhip_buffer_desc srcDesc,
                                float *dst, osdhip_buffer_desc dstDesc,
                                float *du, osdhip_buffer_desc duDesc,

@@ -743,7 +743,7 @@ static int eval_patches(const float *src, osdhip_buffer_desc srcDesc, float *con
     a.params = params;
     hipStream_t st = (hipStream_t)stream;
 
-    // Patch-ordered evaluation: a counting sort of the coords by patch in
+    // Patch-ordered evaluation: the coords binned by patch (patch_sort.h) in
     // stream-ordered scratch memory, then evaluation in that order.
     // Outputs still go to each coord's own row.
     void *owned = nullptr;

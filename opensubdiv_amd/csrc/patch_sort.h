@@ -1,15 +1,17 @@
-// Patch-ordered evaluation of PatchCoords: a counting sort of the coords by
-// patch (bucket = patchIndex mod 2^20, one patch per bucket up to 2^20
-// patches) that produces the order k_patches visits them in.  Each coord's
-// result depends on that coord only and goes to its own row, so any visit
-// order gives the same bits; in patch order the lanes of a wave share index
-// rows and control points in L2 instead of missing on each.
+// Patch-ordered evaluation of PatchCoords: the coords are binned by patch
+// (bin = (patchIndex >> 10) mod 1024: 1024 consecutive patches per bin,
+// which share control points and whose index rows and points fit in one
+// XCD's L2) into the order k_patches visits them in.  Each coord's result
+// depends on that coord only and goes to its own row, so any visit order
+// gives the same bits.
 //
-// Workspace (device memory, PatchSortLayout): counts[2^20], offsets[2^20],
-// 1024 chunk sums, order[n].  Every launch writes what it reads first
-// (counts are cleared by a fill kernel, not a memset, so the sequence is
-// also safe to capture into a graph), so a workspace may be reused by
-// calls that do not overlap.
+// One pass over the coords counts bins per tile of coords in LDS (no
+// global atomics), a two-level scan turns the bin-major histograms into
+// each (bin, tile)'s first position, and a second pass places each coord
+// (LDS atomics per tile).  Workspace (device memory, PatchSortLayout):
+// histograms, chunk sums, order[n]; every launch writes what it reads, with
+// kernels only (no memset), so the sequence may be captured into a graph
+// and a workspace reused by calls that do not overlap.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -19,24 +21,30 @@
 
 namespace osdhip {
 
-static_assert(kScanChunk == 4 * kBlock, "a scan block takes 4 buckets per thread");
+static_assert(kScanChunk == 4 * kBlock, "a scan block takes 4 entries per thread");
+static_assert(kMaxBinBlocks <= kScanChunk, "one block scans the chunk sums");
+static_assert(kPatchBins % kBlock == 0, "bins are cleared and written kBlock at a time");
 
-__device__ __forceinline__ unsigned patch_bucket(int patchIndex)
+__device__ __forceinline__ int patch_bin(int patchIndex)
 {
-    return (unsigned)patchIndex & (unsigned)(kPatchBuckets - 1);
+    return patchIndex < 0 ? 0 : (patchIndex >> kPatchBinShift) & (kPatchBins - 1);
 }
 
-__global__ __launch_bounds__(kBlock) void k_zero_u32(unsigned *p, int n)
+// Block b: bin counts of coords [b * tile, min(n, (b + 1) * tile)) into
+// hist[bin * blocks + b].
+__global__ __launch_bounds__(kBlock) void k_patch_bin_hist(const osdhip_patch_coord *coords, int n,
+                                                           int tile, int blocks, unsigned *hist)
 {
-    for (int i = (int)(blockIdx.x * kBlock + threadIdx.x); i < n; i += (int)(gridDim.x * kBlock))
-        p[i] = 0u;
-}
-
-__global__ __launch_bounds__(kBlock) void k_patch_bucket_count(const osdhip_patch_coord *coords,
-                                                               int n, unsigned *counts)
-{
-    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
-    if (i < n) atomicAdd(&counts[patch_bucket(coords[i].patchIndex)], 1u);
+    __shared__ unsigned h[kPatchBins];
+    for (int k = (int)threadIdx.x; k < kPatchBins; k += kBlock) h[k] = 0u;
+    __syncthreads();
+    const long long i0 = (long long)blockIdx.x * tile;
+    const long long i1 = i0 + tile < n ? i0 + tile : n;
+    for (long long i = i0 + threadIdx.x; i < i1; i += kBlock)
+        atomicAdd(&h[patch_bin(coords[i].patchIndex)], 1u);
+    __syncthreads();
+    for (int k = (int)threadIdx.x; k < kPatchBins; k += kBlock)
+        hist[(size_t)k * blocks + blockIdx.x] = h[k];
 }
 
 // Exclusive scan of the kBlock * 4 values at v (4 per thread, thread t owns
@@ -73,63 +81,73 @@ __device__ __forceinline__ unsigned block_exclusive_scan4(const unsigned *v, uns
     return total;
 }
 
-// Block b: exclusive scan of counts[b * 1024, +1024) into offsets, and the
+// Block b: exclusive scan, in place, of v[b * 1024, +1024), and the
 // chunk's total into sums[b].
-__global__ __launch_bounds__(kBlock) void k_scan_chunks(const unsigned *counts, unsigned *offsets,
-                                                        unsigned *sums)
+__global__ __launch_bounds__(kBlock) void k_scan_chunks(unsigned *v, unsigned *sums)
 {
     __shared__ unsigned ws[kWavesPerBlock];
+    __shared__ unsigned in[kScanChunk];
     const size_t base = (size_t)blockIdx.x * kScanChunk;
-    const unsigned total = block_exclusive_scan4(counts + base, offsets + base, ws);
+    for (int k = (int)threadIdx.x; k < kScanChunk; k += kBlock) in[k] = v[base + k];
+    __syncthreads();
+    const unsigned total = block_exclusive_scan4(in, v + base, ws);
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// One block: exclusive scan of the 1024 chunk sums in place.
-__global__ __launch_bounds__(kBlock) void k_scan_sums(unsigned *sums)
+// One block: exclusive scan of the first `chunks` chunk sums in place.
+__global__ __launch_bounds__(kBlock) void k_scan_sums(unsigned *sums, int chunks)
 {
     __shared__ unsigned ws[kWavesPerBlock];
-    __shared__ unsigned in[kScanChunks];
-    for (int i = (int)threadIdx.x; i < kScanChunks; i += kBlock) in[i] = sums[i];
+    __shared__ unsigned in[kScanChunk];
+    for (int k = (int)threadIdx.x; k < kScanChunk; k += kBlock) in[k] = k < chunks ? sums[k] : 0u;
     __syncthreads();
-    block_exclusive_scan4(in, sums, ws);
+    block_exclusive_scan4(in, in, ws);          // each thread rewrites only what it read
+    __syncthreads();
+    for (int k = (int)threadIdx.x; k < chunks; k += kBlock) sums[k] = in[k];
 }
 
-__global__ __launch_bounds__(kBlock) void k_scan_add(unsigned *offsets, const unsigned *sums)
+// Adds each chunk's offset to its entries.
+__global__ __launch_bounds__(kBlock) void k_scan_add(unsigned *v, const unsigned *sums)
 {
-    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
-    offsets[i] += sums[i / kScanChunk];
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    v[i] += sums[i / kScanChunk];
 }
 
-// Each coord takes the next free position of its bucket (the order within
-// a bucket is whatever the atomics give; it does not change any result).
-__global__ __launch_bounds__(kBlock) void k_patch_bucket_scatter(const osdhip_patch_coord *coords,
-                                                                 int n, unsigned *cursor,
-                                                                 int *order)
+// Block b places coords [b * tile, ...) of each bin from the bin's first
+// free position for this tile (the order within a tile's share of a bin is
+// whatever the LDS atomics give; it does not change any result).
+__global__ __launch_bounds__(kBlock) void k_patch_bin_scatter(const osdhip_patch_coord *coords,
+                                                              int n, int tile, int blocks,
+                                                              const unsigned *offsets, int *order)
 {
-    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
-    if (i >= n) return;
-    const unsigned pos = atomicAdd(&cursor[patch_bucket(coords[i].patchIndex)], 1u);
-    order[pos] = i;
+    __shared__ unsigned cur[kPatchBins];
+    for (int k = (int)threadIdx.x; k < kPatchBins; k += kBlock)
+        cur[k] = offsets[(size_t)k * blocks + blockIdx.x];
+    __syncthreads();
+    const long long i0 = (long long)blockIdx.x * tile;
+    const long long i1 = i0 + tile < n ? i0 + tile : n;
+    for (long long i = i0 + threadIdx.x; i < i1; i += kBlock) {
+        const unsigned pos = atomicAdd(&cur[patch_bin(coords[i].patchIndex)], 1u);
+        order[pos] = (int)i;
+    }
 }
 
-// Queues the sort of n coords on st in the workspace at ws (laid out by
+// Queues the binning of n coords on st in the workspace at ws (laid out by
 // PatchSortLayout(n)); returns the order array.
 inline int *queue_patch_sort(const osdhip_patch_coord *coords, int n, char *ws, hipStream_t st)
 {
     const PatchSortLayout L(n);
-    unsigned *counts = (unsigned *)(ws + L.counts), *offsets = (unsigned *)(ws + L.offsets);
-    unsigned *sums = (unsigned *)(ws + L.sums);
+    unsigned *hist = (unsigned *)(ws + L.hist), *sums = (unsigned *)(ws + L.sums);
     int *order = (int *)(ws + L.order);
-    const int g = (n + kBlock - 1) / kBlock;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1024), dim3(kBlock), 0, st, counts, kPatchBuckets);
-    hipLaunchKernelGGL(k_patch_bucket_count, dim3(g), dim3(kBlock), 0, st, coords, n, counts);
-    hipLaunchKernelGGL(k_scan_chunks, dim3(kScanChunks), dim3(kBlock), 0, st, counts, offsets,
-                       sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, st, sums);
-    hipLaunchKernelGGL(k_scan_add, dim3(kPatchBuckets / kBlock), dim3(kBlock), 0, st, offsets,
-                       sums);
-    hipLaunchKernelGGL(k_patch_bucket_scatter, dim3(g), dim3(kBlock), 0, st, coords, n, offsets,
-                       order);
+    const int chunks = L.blocks;             // kPatchBins * blocks entries = blocks chunks
+    hipLaunchKernelGGL(k_patch_bin_hist, dim3(L.blocks), dim3(kBlock), 0, st, coords, n, L.tile,
+                       L.blocks, hist);
+    hipLaunchKernelGGL(k_scan_chunks, dim3(chunks), dim3(kBlock), 0, st, hist, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, st, sums, chunks);
+    hipLaunchKernelGGL(k_scan_add, dim3(chunks * (kScanChunk / kBlock)), dim3(kBlock), 0, st,
+                       hist, sums);
+    hipLaunchKernelGGL(k_patch_bin_scatter, dim3(L.blocks), dim3(kBlock), 0, st, coords, n,
+                       L.tile, L.blocks, hist, order);
     return order;
 }
 

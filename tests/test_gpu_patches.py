@@ -265,8 +265,8 @@ def test_python_mirror_eval_patches(hip):
 
 @pytest.mark.parametrize("deriv", [0, 1, 2])
 def test_patch_sorted_order_bit_exact(hip, deriv):
-    """Patch-ordered evaluation (osdhip_eval_patches_sorted: counting sort
-    of the coords by patch) returns every coord's result in its own row,
+    """Patch-ordered evaluation (osdhip_eval_patches_sorted: the coords
+    binned by patch) returns every coord's result in its own row,
     bit-identical to caller order and to the oracle, on a small and a large
     set (coords drawn with repeats and in random order)."""
     g, src, _, arrays, indices, params = golden_patches()
@@ -291,8 +291,8 @@ def test_patch_sorted_order_bit_exact(hip, deriv):
 
 
 def test_patch_sorted_order_beyond_the_bucket_count(hip):
-    """More patches than the sort's 2^20 buckets (patch p and p + 2^20 share
-    a bucket), coords drawn with repeats: still every coord's own result,
+    """More patches than the binning's 2^20 (patch p and p + 2^20 share a
+    bin), coords drawn with repeats: still every coord's own result,
     bit-identical to caller order and to the oracle."""
     rng = np.random.default_rng(91)
     npatch, nsrc, n = (1 << 20) + 4096, 5000, 60000
