@@ -127,6 +127,15 @@ def load_hip():
         return _hip
     if not os.path.exists(HIP_LIB):
         raise RuntimeError(f"libosdhip.so not built ({HIP_LIB}); run opensubdiv_amd.build")
+    # One HIP runtime per process.  torch ships its own libamdhip64 (same
+    # soname as the system one libosdhip.so links against): imported first,
+    # its runtime is the one libosdhip then binds to; loaded the other way
+    # round, the process would hold both, and streams or device pointers of
+    # one are invalid in the other.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(HIP_LIB)
     for name, args in _HIP_SIGS.items():
         fn = getattr(lib, name)
