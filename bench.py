@@ -804,7 +804,7 @@ def run_served(args, lib, device):
     log(f"[cfg4] {len(residents)} resident topologies in {build_s:.1f}s; budgets "
         f"{dev_budget >> 30} GiB device, {host_budget >> 30} GiB pinned host")
     server = serve.Server(residents, L, dev_budget, host_budget, slots=args.slots,
-                          max_epoch_points=int(args.epoch_points))
+                          max_epoch_points=int(args.epoch_points), copy_order=args.copy_order)
 
     def points(r):
         t = residents[r.topo]
@@ -847,6 +847,7 @@ def run_served(args, lib, device):
                    "topologies": len(residents), "floats_per_point": L, "device": device,
                    "device_budget_bytes": dev_budget, "host_budget_bytes": host_budget,
                    "slots": args.slots, "max_epoch_points": int(args.epoch_points),
+                   "copy_order": args.copy_order,
                    "table_layout": args.table_layout,
                    "build_s": build_s, "table_cache": cache_summary(cache)},
         "served": {"requests": len(done), "rate_per_s": rate, "capacity_per_s": capacity,
@@ -1004,6 +1005,9 @@ def main():
     ap.add_argument("--load", type=float, default=0.7,
                     help="cfg4: arrival rate as a fraction of the measured capacity")
     ap.add_argument("--slots", type=int, default=4, help="cfg4: epochs in flight")
+    ap.add_argument("--copy-order", default="srpt", choices=["srpt", "edf"],
+                    help="cfg4: which result download goes next (fewest bytes left, or "
+                         "earliest ideal completion)")
     ap.add_argument("--epoch-points", type=float, default=32e6,
                     help="cfg4: cap on the points batched into one epoch")
     ap.add_argument("--meshes", type=int, default=256, help="cfg3: meshes in the batch")

@@ -13,11 +13,16 @@ the patch table).  The server:
 * batches the requests admitted since the last launch (up to a cap on the
   points per epoch) into one epoch
   (osdhip_batch_epoch: phase A stencils, phase B patches) on a free stream
-  slot, with the frame uploads before it and the result downloads after it
-  on the same stream;
+  slot, with the frame uploads before it;
 * keeps up to `slots` epochs in flight, so requests that arrive while others
   are on the GPU are admitted as soon as a slot and memory free up, and their
   work overlaps on the device;
+* downloads results on copy streams of their own, in chunks, a few chunks in
+  flight, always the next chunk of the request with the fewest bytes left
+  (or, as an option, the earliest ideal completion: its arrival plus all its
+  bytes at the link rate): the host link, not the GPU, bounds serving, and
+  this way a small request is not queued behind another's gigabyte of
+  results;
 * completes a request when its download has finished; latency = completion
   time - arrival time.
 
@@ -98,11 +103,16 @@ class Request:
     dev: list = field(default_factory=list)      # device arena blocks
     host: tuple | None = None                    # host arena block
     points: int = 0
+    pieces: list = field(default_factory=list)   # (device, host offset, floats) left to copy
+    left: int = 0                                # floats left to copy
+    inflight: int = 0                            # chunks on the copy stream
+    deadline: float = 0.0                        # arrival + its bytes at the link rate
 
 
 class Server:
     def __init__(self, residents, length, device_budget, host_budget, slots=4, item_slices=8,
-                 max_epoch_points=None):
+                 max_epoch_points=None, chunk_bytes=16 << 20, copy_depth=4, copy_streams=2,
+                 copy_order="srpt", link_bytes_per_s=45e9):
         import torch
         self.torch = torch
         self.lib = _lib.load_hip()
@@ -122,6 +132,24 @@ class Server:
         for _ in range(int(slots)):
             self.slots.append({"stream": torch.cuda.Stream(), "event": torch.cuda.Event(),
                                "reqs": None, "staging": None, "scratch": None, "cap": 0})
+        # result downloads: copy_streams streams (default priority, like every
+        # stream here) taking chunks of chunk_bytes in turn, at most
+        # copy_depth chunks in flight
+        self.copy_streams = [torch.cuda.Stream() for _ in range(max(1, int(copy_streams)))]
+        self.copy_turn = 0
+        # which download's chunk goes next: "srpt" the fewest bytes left
+        # (the lower mean and p99 latency on the served workload); "edf" the
+        # earliest ideal completion (arrival + all its bytes at
+        # link_bytes_per_s), which bounds how long a large request can be
+        # overtaken by later small ones (the lower maximum)
+        if copy_order not in ("srpt", "edf"):
+            raise ValueError(f"copy_order {copy_order!r}")
+        self.copy_order = copy_order
+        self.link = float(link_bytes_per_s)
+        self.chunk = max(1, int(chunk_bytes) // 4)
+        self.copy_depth = int(copy_depth)
+        self.copying = []                        # requests whose results are downloading
+        self.inflight = []                       # (event, request) per chunk, in issue order
         # host frames in pinned memory, one block per topology
         self.frames = [torch.from_numpy(np.ascontiguousarray(r.frames)).pin_memory()
                        for r in residents]
@@ -211,22 +239,66 @@ class Server:
                                              slot["scratch"].data_ptr(), slot["cap"],
                                              st.cuda_stream)
             _lib.check(rc, "osdhip_batch_epoch")
-            for r in reqs:
-                ho, _ = r.host
-                if r.kind == "refine":
-                    o, _ = r.dev[1]
-                    n = self.res[r.topo].nrefine * self.L
-                    self.host[ho:ho + n].copy_(self.dev[o:o + n], non_blocking=True)
-                else:
-                    n = self.res[r.topo].ncoords * self.L
-                    for k in range(3):
-                        o, _ = r.dev[1 + k]
-                        self.host[ho + k * n:ho + (k + 1) * n].copy_(self.dev[o:o + n],
-                                                                     non_blocking=True)
             slot["event"].record(st)
         slot["reqs"] = reqs
         self.epochs.append(len(reqs))
         self.epoch_points.append(sum(self.points(r) for r in reqs))
+
+    # ---- result downloads -----------------------------------------------
+    def _pieces(self, r: Request):
+        """(device offset, host offset, floats) of a request's results."""
+        ho, _ = r.host
+        if r.kind == "refine":
+            o, _ = r.dev[1]
+            return [(o, ho, self.res[r.topo].nrefine * self.L)]
+        n = self.res[r.topo].ncoords * self.L
+        return [(r.dev[1 + k][0], ho + k * n, n) for k in range(3)]
+
+    def _start_download(self, r: Request):
+        r.pieces = self._pieces(r)
+        r.left = sum(p[2] for p in r.pieces)
+        r.deadline = r.arrival + 4.0 * r.left / self.link
+        self.copying.append(r)
+
+    def _issue_chunks(self):
+        """Keeps copy_depth chunks in flight: each the next chunk of the
+        downloading request with the fewest floats left."""
+        torch = self.torch
+        while len(self.inflight) < self.copy_depth:
+            cand = [r for r in self.copying if r.pieces]
+            if not cand:
+                return
+            r = min(cand, key=(lambda x: x.left) if self.copy_order == "srpt"
+                    else (lambda x: x.deadline))
+            o, ho, n = r.pieces[0]
+            m = min(n, self.chunk)
+            cs = self.copy_streams[self.copy_turn]
+            self.copy_turn = (self.copy_turn + 1) % len(self.copy_streams)
+            with torch.cuda.stream(cs):
+                self.host[ho:ho + m].copy_(self.dev[o:o + m], non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(cs)
+            if m == n:
+                r.pieces.pop(0)
+            else:
+                r.pieces[0] = (o + m, ho + m, n - m)
+            r.left -= m
+            r.inflight += 1
+            self.inflight.append((ev, r))
+
+    def _finished_downloads(self):
+        """Requests whose every chunk has landed."""
+        out, keep = [], []
+        for ev, r in self.inflight:
+            if ev.query():
+                r.inflight -= 1
+                if r.inflight == 0 and not r.pieces:
+                    self.copying.remove(r)
+                    out.append(r)
+            else:
+                keep.append((ev, r))
+        self.inflight = keep
+        return out
 
     def result(self, r: Request):
         """Host copy of a completed request's output (before release)."""
@@ -250,17 +322,20 @@ class Server:
             while i < len(pending) and pending[i].arrival <= now:
                 waiting.append(pending[i])
                 i += 1
-            # completions
+            # computed epochs: their slots free up, their results start downloading
             for sl in self.slots:
                 if sl["reqs"] is not None and sl["event"].query():
-                    tdone = time.perf_counter() - t0
                     for r in sl["reqs"]:
-                        r.done = tdone
-                        if keep is not None and r.rid in keep:
-                            results[r.rid] = self.result(r)
-                        self.release(r)
-                        done.append(r)
+                        self._start_download(r)
                     sl["reqs"] = None
+            # downloaded requests complete
+            for r in self._finished_downloads():
+                r.done = time.perf_counter() - t0
+                if keep is not None and r.rid in keep:
+                    results[r.rid] = self.result(r)
+                self.release(r)
+                done.append(r)
+            self._issue_chunks()
             # admission + launch on a free slot
             free = [sl for sl in self.slots if sl["reqs"] is None]
             if waiting and free:
@@ -274,9 +349,10 @@ class Server:
                     batch.append(r)
                 if batch:
                     self.launch(free[0], batch)
-                elif not any(sl["reqs"] is not None for sl in self.slots):
+                elif not any(sl["reqs"] is not None for sl in self.slots) and not self.copying:
                     raise MemoryError(f"request {waiting[0].rid} exceeds the serving budgets")
-            if i < len(pending) and not waiting and all(sl["reqs"] is None for sl in self.slots):
+            if i < len(pending) and not waiting and not self.copying and \
+                    all(sl["reqs"] is None for sl in self.slots):
                 # idle until the next arrival
                 dt = pending[i].arrival - (time.perf_counter() - t0)
                 if dt > 0:

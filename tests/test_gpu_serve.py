@@ -126,6 +126,32 @@ def test_epoch_cap_splits_the_burst_with_same_results(residents):
         assert np.array_equal(want[r.rid].view(np.uint32), got[r.rid].view(np.uint32))
 
 
+def test_chunked_downloads_keep_results_and_let_small_requests_finish_first(residents):
+    """Results downloaded in many small chunks (shortest remaining transfer
+    first) are the same bits as whole downloads; a small request that
+    arrives with a large one is not held behind the large one's transfer."""
+    reqs = make_requests()
+    whole = Server(residents, L, device_budget=64 << 20, host_budget=64 << 20, slots=3,
+                   chunk_bytes=1 << 30)
+    _, want = whole.run(reqs, keep={r.rid for r in reqs}, max_seconds=60)
+    chunked = Server(residents, L, device_budget=64 << 20, host_budget=64 << 20, slots=3,
+                     chunk_bytes=1 << 12, copy_depth=2, copy_order="srpt")
+    reqs2 = make_requests()
+    done, got = chunked.run(reqs2, keep={r.rid for r in reqs2}, max_seconds=60)
+    assert len(done) == len(reqs2)
+    assert chunked.dev_arena.used() == 0 and chunked.host_arena.used() == 0
+    for r in reqs2:
+        assert np.array_equal(want[r.rid].view(np.uint32), got[r.rid].view(np.uint32))
+    # one large and one small request arriving together, in one epoch
+    big = max(range(len(SPECS)), key=lambda t: residents[t].nrefine)
+    small = min(range(len(SPECS)), key=lambda t: residents[t].nrefine)
+    pair = [Request(100, big, "refine", 0.0, 0), Request(101, small, "refine", 0.0, 1)]
+    assert chunked.host_floats(pair[0]) > 8 * chunked.host_floats(pair[1])
+    done, _ = chunked.run(pair, max_seconds=60)
+    t = {r.rid: r.done for r in done}
+    assert t[101] <= t[100]
+
+
 def test_level4_requests_match_the_oracle(residents):
     """Both request kinds on the level-4 topology, served in a mix with the
     others, against the oracle's CpuEvaluator-order sums on the same frame:
