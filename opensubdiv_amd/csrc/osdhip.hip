@@ -83,18 +83,18 @@ static void launch_tiled(int lc, int group, const TiledArgs &a, int grid, hipStr
     else launch_tiled_g<NWT, 8>(lc, a, grid, st);
 }
 
-template <int NW, int CAP, bool VEC>
+template <int NW, bool VEC>
 static void launch_csr(int lc, const StencilArgs &a, int grid, hipStream_t st)
 {
     switch (lc) {
-    case 1: hipLaunchKernelGGL((k_stencils_csr<NW, 1, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 2: hipLaunchKernelGGL((k_stencils_csr<NW, 2, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 3: hipLaunchKernelGGL((k_stencils_csr<NW, 3, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 4: hipLaunchKernelGGL((k_stencils_csr<NW, 4, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 5: hipLaunchKernelGGL((k_stencils_csr<NW, 5, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 6: hipLaunchKernelGGL((k_stencils_csr<NW, 6, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    case 7: hipLaunchKernelGGL((k_stencils_csr<NW, 7, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
-    default: hipLaunchKernelGGL((k_stencils_csr<NW, 8, CAP, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((k_stencils_csr<NW, 1, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_stencils_csr<NW, 2, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_stencils_csr<NW, 3, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_stencils_csr<NW, 4, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 5: hipLaunchKernelGGL((k_stencils_csr<NW, 5, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 6: hipLaunchKernelGGL((k_stencils_csr<NW, 6, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 7: hipLaunchKernelGGL((k_stencils_csr<NW, 7, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_stencils_csr<NW, 8, VEC>), dim3(grid), dim3(kBlock), 0, st, a); break;
     }
 }
 
@@ -174,11 +174,8 @@ static int eval_csr(const float *src, osdhip_buffer_desc sd, float *const *outs,
             a.dst[w] = outs[w] ? outs[w] + ods[w].offset + e0 : nullptr;
             a.dstStride[w] = outs[w] ? ods[w].stride : 0;
         }
-        // SLOTS = CAP + 4 entries per wave: 32 KB per 256-thread block for
-        // NW = 1, i.e. five blocks per 160 KB CU
-        constexpr int CAP = NW == 6 ? 508 : 1020;
-        if (vec) launch_csr<NW, CAP, true>(lc, a, grid, st);
-        else launch_csr<NW, CAP, false>(lc, a, grid, st);
+        if (vec) launch_csr<NW, true>(lc, a, grid, st);
+        else launch_csr<NW, false>(lc, a, grid, st);
         int rc = last_error();
         if (rc != OSDHIP_OK) return rc;
     }

@@ -3,10 +3,15 @@
 //
 // One 64-lane wavefront owns 64 consecutive stencils.  Their table entries
 // form one slice [lo, hi) of the index / weight streams, which the wave
-// copies into LDS with coalesced, non-temporal 16-byte loads (all loads of
-// the slice are issued before the first is consumed, so each wave keeps up
-// to ~10 KB of HBM traffic in flight).  Each lane then reduces its own
-// stencil from LDS, issuing four control-point gathers at a time through L2.
+// walks in windows of kCsrWindow (512) entries: each window is copied into LDS
+// with coalesced, non-temporal 16-byte loads, then every lane adds the
+// entries of its own stencil that fall in the window, issuing four
+// control-point gathers at a time through L2.  Small windows keep the LDS
+// and register footprint per wave low, so many waves per CU hide the
+// loads' latency.  Slice entry e of a window sits at LDS position
+// e + e / 16: lane l reading entry j of a stencil that starts 16 l entries
+// in (stencils of 16 entries) then hits bank 17 l + j, a different one for
+// every lane.
 // Entries are accumulated in table order without FMA contraction, which
 // reproduces CpuEvalStencils bit for bit.
 #pragma once
@@ -29,14 +34,17 @@ struct StencilArgs {
 
 constexpr int kGatherGroup = 4;
 
+// LDS position of slice entry e (one padding slot after every 16 entries).
+__device__ __forceinline__ int csr_lds_pos(int e) { return e + (e >> 4); }
+
 // Accumulates entries [j0, sz) of one stencil; idxAt(j) / wAt(w, j) fetch
 // the j-th entry.  Gathers of a group are issued before any is consumed.
 template <int NW, int LC, class IDX, class WT>
 __device__ __forceinline__ void accumulate_stencil(const float *__restrict__ src, size_t sstride,
-                                                   int sz, IDX idxAt, WT wAt,
+                                                   int j0, int sz, IDX idxAt, WT wAt,
                                                    float (&acc)[NW][LC])
 {
-    for (int j = 0; j < sz; j += kGatherGroup) {
+    for (int j = j0; j < sz; j += kGatherGroup) {
         float v[kGatherGroup][LC];
 #pragma unroll
         for (int u = 0; u < kGatherGroup; ++u) {
@@ -60,15 +68,20 @@ __device__ __forceinline__ void accumulate_stencil(const float *__restrict__ src
     }
 }
 
+// 16-byte chunks per lane in one LDS window of a wave's table slice, and
+// the window's entries.
+constexpr int kCsrChunks = 2;
+constexpr int kCsrWindow = 4 * kWave * kCsrChunks;
+
 // NW weight arrays (1: P, 3: P/du/dv, 6: + duu/duv/dvv); LC floats per vertex
-// in this launch; CAP = table entries staged per wave; VEC = index and weight
-// pointers are 16-byte aligned (16-byte staging loads).
-template <int NW, int LC, int CAP, bool VEC>
+// in this launch; VEC = index and weight pointers are 16-byte aligned
+// (16-byte staging loads).
+template <int NW, int LC, bool VEC>
 __global__ __launch_bounds__(kBlock) void k_stencils_csr(StencilArgs a)
 {
-    constexpr int SLOTS = CAP + 4;          // room for the 16-byte alignment slack
-    __shared__ __attribute__((aligned(16))) int s_idx[kWavesPerBlock][SLOTS];
-    __shared__ __attribute__((aligned(16))) float s_w[kWavesPerBlock][NW][SLOTS];
+    constexpr int PADDED = kCsrWindow + kCsrWindow / 16;
+    __shared__ int s_idx[kWavesPerBlock][PADDED];
+    __shared__ float s_w[kWavesPerBlock][NW][PADDED];
 
     const int lane = (int)threadIdx.x & (kWave - 1);
     const int wv = wave_in_block();
@@ -93,8 +106,8 @@ __global__ __launch_bounds__(kBlock) void k_stencils_csr(StencilArgs a)
         lo = min(lo, __shfl_xor(lo, m));
         hi = max(hi, __shfl_xor(hi, m));
     }
-    const int base = VEC ? (lo & ~3) : lo;
-    const int span = hi - base;
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    hi = __builtin_amdgcn_readfirstlane(hi);
 
     float acc[NW][LC];
 #pragma unroll
@@ -104,59 +117,51 @@ __global__ __launch_bounds__(kBlock) void k_stencils_csr(StencilArgs a)
 
     const float *__restrict__ src = a.src;
     const size_t sstride = (size_t)a.srcStride;
-
-    if (span <= SLOTS) {
-        if constexpr (VEC) {
-            // [base, base + 4*lane + 256*it) chunks; a chunk is loaded as one
-            // 16-byte access when it lies inside [base, hi), else per entry.
-            constexpr int IT = (SLOTS + 4 * kWave - 1) / (4 * kWave);
-            v4i ri[IT];
-            v4f rw[NW][IT];
+    const long long base = VEC ? ((long long)lo & ~3LL) : lo;
+    for (long long w0 = base; w0 < hi; w0 += kCsrWindow) {
+        // window [w0, w0 + kCsrWindow) of the slice; lane l stages entries
+        // w0 + 4 (l + 64 c) .. + 3 for chunks c
+        v4i ri[kCsrChunks];
+        v4f rw[NW][kCsrChunks];
 #pragma unroll
-            for (int it = 0; it < IT; ++it) {
-                const int e = it * 4 * kWave + 4 * lane;
-                if (e + 4 <= span) {
-                    ri[it] = __builtin_nontemporal_load((const v4i *)(a.indices + base + e));
-#pragma unroll
-                    for (int w = 0; w < NW; ++w)
-                        rw[w][it] = __builtin_nontemporal_load((const v4f *)(a.w[w] + base + e));
-                } else if (e < span) {
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const bool in = e + q < span;
-                        ri[it][q] = in ? a.indices[base + e + q] : 0;
-#pragma unroll
-                        for (int w = 0; w < NW; ++w) rw[w][it][q] = in ? a.w[w][base + e + q] : 0.0f;
-                    }
-                }
-            }
-#pragma unroll
-            for (int it = 0; it < IT; ++it) {
-                const int e = it * 4 * kWave + 4 * lane;
-                if (e < span) {
-                    *(v4i *)&s_idx[wv][e] = ri[it];
-#pragma unroll
-                    for (int w = 0; w < NW; ++w) *(v4f *)&s_w[wv][w][e] = rw[w][it];
-                }
-            }
-        } else {
-            for (int e = lane; e < span; e += kWave) {
-                s_idx[wv][e] = __builtin_nontemporal_load(a.indices + base + e);
+        for (int c = 0; c < kCsrChunks; ++c) {
+            const long long e = w0 + 4 * (lane + kWave * c);
+            if (VEC && e + 4 <= hi) {
+                ri[c] = __builtin_nontemporal_load((const v4i *)(a.indices + e));
 #pragma unroll
                 for (int w = 0; w < NW; ++w)
-                    s_w[wv][w][e] = __builtin_nontemporal_load(a.w[w] + base + e);
+                    rw[w][c] = __builtin_nontemporal_load((const v4f *)(a.w[w] + e));
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const bool in = e + q >= lo && e + q < hi;
+                    ri[c][q] = in ? a.indices[e + q] : 0;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) rw[w][c][q] = in ? a.w[w][e + q] : 0.0f;
+                }
+            }
+        }
+        wave_lds_sync();                             // the previous window's reads are done
+        // dword stores (a chunk's positions need not be 16-byte aligned)
+#pragma unroll
+        for (int c = 0; c < kCsrChunks; ++c) {
+            const int pe = csr_lds_pos(4 * (lane + kWave * c));
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                s_idx[wv][pe + q] = ri[c][q];
+#pragma unroll
+                for (int w = 0; w < NW; ++w) s_w[wv][w][pe + q] = rw[w][c][q];
             }
         }
         wave_lds_sync();
-        const int rb = off - base;
-        accumulate_stencil<NW, LC>(
-            src, sstride, sz, [&](int j) { return s_idx[wv][rb + j]; },
-            [&](int w, int j) { return s_w[wv][w][rb + j]; }, acc);
-    } else {
-        // Slice larger than the staging buffer: read the table directly.
-        accumulate_stencil<NW, LC>(
-            src, sstride, sz, [&](int j) { return a.indices[off + j]; },
-            [&](int w, int j) { return a.w[w][off + j]; }, acc);
+        // this lane's entries j in [w0, w0 + kCsrWindow), in table order
+        const long long rb = (long long)off - w0;   // window position of entry 0
+        const int j0 = (int)max(0LL, -rb);
+        const int j1 = (int)min((long long)sz, (long long)kCsrWindow - rb);
+        if (j0 < j1)
+            accumulate_stencil<NW, LC>(
+                src, sstride, j0, j1, [&](int j) { return s_idx[wv][csr_lds_pos((int)rb + j)]; },
+                [&](int w, int j) { return s_w[wv][w][csr_lds_pos((int)rb + j)]; }, acc);
     }
 
     if (active) {
@@ -171,5 +176,6 @@ __global__ __launch_bounds__(kBlock) void k_stencils_csr(StencilArgs a)
         }
     }
 }
+
 
 }  // namespace osdhip
