@@ -10,6 +10,9 @@ All generators are deterministic functions of their seed (numpy PCG64).
   12 n^2 edges, 8 valence-3 corners), projected to the unit sphere with
   seeded radial noise; a seeded fraction of edges is creased with sharpness
   uniform in [0, smax).
+* ``cube_sphere_uvs`` -- a face-varying UV channel for ``cube_sphere``: one
+  chart per cube face, laid out in a 3 x 2 atlas, so values split along the
+  twelve cube edges (seams) and the eight corners carry three values each.
 * ``deform_frames`` -- seeded sinusoidal displacement along the vertex
   normal, one frame per call index, plus extra uniform primvars.
 """
@@ -104,6 +107,26 @@ def cube_sphere(n: int, seed: int = 0, radial_noise: float = 0.01,
         mesh.crease_pairs = e[pick].astype(np.int32)
         mesh.crease_weights = rng.uniform(0.0, sharpness_max, size=k).astype(np.float32)
     return mesh
+
+
+def cube_sphere_uvs(n: int):
+    """Face-varying UVs of ``cube_sphere(n)``: (numValues, value index per
+    face-vertex (6 n^2, 4), (numValues, 2) float64 values).  Chart c holds the
+    (n + 1)^2 lattice values of cube face c at u = (c % 3 + a / n) / 3,
+    v = (c // 3 + b / n) / 2; faces index them in the order cube_sphere emits
+    its quads."""
+    n = int(n)
+    per = (n + 1) * (n + 1)
+    a, b = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    idx, uv = [], []
+    for c in range(6):
+        lid = c * per + a * (n + 1) + b
+        idx.append(np.stack([lid[ii, jj], lid[ii + 1, jj], lid[ii + 1, jj + 1], lid[ii, jj + 1]],
+                            axis=-1).reshape(-1, 4))
+        uv.append(np.stack([((c % 3) + a / n) / 3.0, ((c // 3) + b / n) / 2.0],
+                           axis=-1).reshape(-1, 2))
+    return 6 * per, np.concatenate(idx).astype(np.int32), np.concatenate(uv)
 
 
 def vertex_normals(mesh: QuadMesh) -> np.ndarray:

@@ -31,6 +31,11 @@ def lib():
             "ref_topo_create": ([i, i, vp, vp, i, vp, vp, i, vp, vp, i, i], vp),
             "ref_topo_destroy": ([vp], None),
             "ref_topo_set_scheme": ([vp, i], None),
+            "ref_topo_set_fvar": ([vp, i, vp, i], None),
+            "ref_patches_create_fvar": ([vp, i, i], vp),
+            "ref_patches_fvar_info": ([vp] * 9, None),
+            "ref_patches_fvar_stencils": ([vp], vp),
+            "ref_patches_fvar_copy": ([vp, vp, vp, vp], None),
             "ref_patches_create_uniform": ([vp, i, i], vp),
             "ref_stencils_uniform": ([vp, i, i, i, i, i, i], vp),
             "ref_stencils_limit": ([vp, i, i, vp, vp, vp, i, i], vp),
@@ -72,6 +77,9 @@ def _desc(dsc):
 
 
 SCHEME_BILINEAR, SCHEME_CATMARK, SCHEME_LOOP = 0, 1, 2
+# Sdc::Options::FVarLinearInterpolation (sdc/options.h:60-67)
+(FVAR_LINEAR_NONE, FVAR_LINEAR_CORNERS_ONLY, FVAR_LINEAR_CORNERS_PLUS1, FVAR_LINEAR_CORNERS_PLUS2,
+ FVAR_LINEAR_BOUNDARIES, FVAR_LINEAR_ALL) = range(6)
 
 
 class Topology:
@@ -95,6 +103,13 @@ class Topology:
                                        _p(cwv) if len(cv) else None, boundary, creasing)
         if scheme != SCHEME_CATMARK:
             lib().ref_topo_set_scheme(self.h, int(scheme))
+
+    def set_fvar(self, num_values, value_indices, linear_interp=FVAR_LINEAR_ALL):
+        """One face-varying channel (value index per face-vertex) with
+        Sdc::Options::FVarLinearInterpolation `linear_interp`."""
+        vi = np.ascontiguousarray(value_indices, np.int32).reshape(-1)
+        self._keep = self._keep + (vi,)
+        lib().ref_topo_set_fvar(self.h, int(num_values), _p(vi), int(linear_interp))
 
     def __del__(self):
         if getattr(self, "h", None) and _L is not None:
@@ -144,12 +159,19 @@ class Patches:
     Osd::CpuPatchTable flattening + Far::PatchMap."""
 
     def __init__(self, topo, iso_level, endcap=3, inf_sharp=False, single_crease=False,
-                 legacy_sharp_corner=True, double=False, uniform=False, triangulate=False):
+                 legacy_sharp_corner=True, double=False, uniform=False, triangulate=False,
+                 fvar=False, fvar_legacy_linear=True):
         """uniform=True: RefineUniform(iso_level) and the uniform linear
         table (QUADS, or TRIANGLES for Loop meshes / triangulate=True);
-        no patch map then."""
+        no patch map then.  fvar=True: the topology's face-varying channel
+        too (Gregory end caps), in self.fvar: its stencil table (base values
+        to refined and local values), flattened arrays, indices and params."""
         L = lib()
-        if uniform:
+        if fvar:
+            self.h = L.ref_patches_create_fvar(topo.h, iso_level, int(fvar_legacy_linear))
+            if not self.h:
+                raise RuntimeError("face-varying patch table not created")
+        elif uniform:
             self.h = L.ref_patches_create_uniform(topo.h, iso_level, int(triangulate))
         else:
             self.h = L.ref_patches_create(topo.h, iso_level, endcap, int(inf_sharp),
@@ -165,6 +187,19 @@ class Patches:
         self.indices = np.zeros(ni, np.int32)
         self.params = np.zeros(npar, PATCH_PARAM_DTYPE)
         L.ref_patches_copy(self.h, _p(self.arrays), _p(self.indices), _p(self.params))
+        self.fvar = None
+        if fvar:
+            v = [C.c_int() for _ in range(8)]
+            L.ref_patches_fvar_info(self.h, *[C.byref(x) for x in v])
+            nbase, _, _, fna, fni, fnp, reg_t, irreg_t = [x.value for x in v]
+            fa = np.zeros(fna, PATCH_ARRAY_DTYPE)
+            fi = np.zeros(fni, np.int32)
+            fp = np.zeros(fnp, PATCH_PARAM_DTYPE)
+            L.ref_patches_fvar_copy(self.h, _p(fa), _p(fi), _p(fp))
+            self.fvar = {"num_base": nbase, "arrays": fa, "indices": fi, "params": fp,
+                         "regular_type": reg_t, "irregular_type": irreg_t,
+                         "stencils": _read_stencils(L.ref_patches_fvar_stencils(self.h), 1,
+                                                    owner_destroy=False)}
 
     @property
     def num_vertices_total(self):

@@ -40,6 +40,10 @@ struct Topo {
     int boundary = 1;   // Sdc::Options::VTX_BOUNDARY_EDGE_ONLY
     int creasing = 0;   // Sdc::Options::CREASE_UNIFORM
     int scheme = Sdc::SCHEME_CATMARK;
+    // one optional face-varying channel: value index per face-vertex
+    int fvarValues = 0;
+    std::vector<int> fvarIdx;
+    int fvarInterp = Sdc::Options::FVAR_LINEAR_ALL;
 
     Far::TopologyRefiner *makeRefiner() const {
         Far::TopologyDescriptor d;
@@ -53,7 +57,15 @@ struct Topo {
         d.numCorners = (int)cornerW.size();
         d.cornerVertexIndices = cornerVerts.empty() ? nullptr : cornerVerts.data();
         d.cornerWeights = cornerW.empty() ? nullptr : cornerW.data();
+        Far::TopologyDescriptor::FVarChannel ch;
+        if (fvarValues > 0) {
+            ch.numValues = fvarValues;
+            ch.valueIndices = fvarIdx.data();
+            d.numFVarChannels = 1;
+            d.fvarChannels = &ch;
+        }
         Sdc::Options o;
+        o.SetFVarLinearInterpolation((Sdc::Options::FVarLinearInterpolation)fvarInterp);
         o.SetVtxBoundaryInterpolation((Sdc::Options::VtxBoundaryInterpolation)boundary);
         o.SetCreasingMethod((Sdc::Options::CreasingMethod)creasing);
         typedef Far::TopologyRefinerFactory<Far::TopologyDescriptor> F;
@@ -92,7 +104,9 @@ struct Patches {
     Far::PatchMap *map = nullptr;
     Osd::CpuPatchTable *flat = nullptr;
     Stencils st;   // coarse -> (refined + local points)
+    Stencils fst;  // face-varying channel 0: base values -> (refined + local values)
     int nCoarse = 0;
+    int nFVarBase = 0;
     ~Patches() { delete map; delete flat; delete table; delete refiner; }
 };
 
@@ -131,6 +145,17 @@ void ref_topo_destroy(void *t) { delete (Topo *)t; }
 // Subdivision scheme of later refiners: Sdc::SchemeType (0 bilinear,
 // 1 Catmark, 2 Loop; sdc/types.h:36-40).
 void ref_topo_set_scheme(void *t, int scheme) { ((Topo *)t)->scheme = scheme; }
+
+// One face-varying channel on later refiners (far/topologyDescriptor.h:
+// 73-84): numValues values, one value index per face-vertex, and
+// Sdc::Options::FVarLinearInterpolation (sdc/options.h:60-67).
+void ref_topo_set_fvar(void *t, int numValues, const int *valueIndices, int linearInterp)
+{
+    Topo *p = (Topo *)t;
+    p->fvarValues = numValues;
+    p->fvarIdx.assign(valueIndices, valueIndices + p->fv.size());
+    p->fvarInterp = linearInterp;
+}
 
 // Uniform refinement to `level`, then StencilTableFactory::Create.
 void *ref_stencils_uniform(void *topo, int level, int mode, int genIntermediate,
@@ -306,6 +331,91 @@ void *ref_patches_create_uniform(void *topo, int level, int triangulateQuads)
     p->flat = new Osd::CpuPatchTable(p->table);
     p->map = new Far::PatchMap(*p->table);
     return p;
+}
+
+// Feature-adaptive patch table with the topology's face-varying channel,
+// set up as examples/glEvalLimit/glEvalLimit.cpp:684-750 does: vertex and
+// face-varying (INTERPOLATE_FACE_VARYING, channel 0) stencil tables with
+// intermediate levels, PatchTableFactory with generateFVarTables (and
+// generateFVarLegacyLinearPatches as given: Far's default is true, linear
+// QUADS face-varying patches; false builds B-spline / Gregory face-varying
+// patches unless the channel interpolates linearly everywhere), local points
+// appended to both stencil tables.
+void *ref_patches_create_fvar(void *topo, int isoLevel, int legacyLinear)
+{
+    Topo *tp = (Topo *)topo;
+    if (tp->fvarValues <= 0) return nullptr;
+    Patches *p = new Patches;
+    p->refiner = tp->makeRefiner();
+    if (!p->refiner) { delete p; return nullptr; }
+    p->nCoarse = p->refiner->GetLevel(0).GetNumVertices();
+    p->nFVarBase = p->refiner->GetLevel(0).GetNumFVarValues(0);
+    Far::PatchTableFactory::Options po(isoLevel);
+    po.SetEndCapType(Far::PatchTableFactory::Options::ENDCAP_GREGORY_BASIS);
+    po.generateVaryingTables = false;
+    po.generateFVarTables = true;
+    po.generateFVarLegacyLinearPatches = legacyLinear != 0;
+    p->refiner->RefineAdaptive(po.GetRefineAdaptiveOptions());
+    p->table = Far::PatchTableFactory::Create(*p->refiner, po);
+    Far::StencilTableFactory::Options so;
+    so.generateOffsets = true;
+    so.generateIntermediateLevels = true;
+    Far::StencilTable const *vs = Far::StencilTableFactory::Create(*p->refiner, so);
+    if (auto const *local = p->table->GetLocalPointStencilTable()) {
+        auto const *all = Far::StencilTableFactory::AppendLocalPointStencilTable(*p->refiner, vs, local);
+        delete vs;
+        vs = all;
+    }
+    so.interpolationMode = Far::StencilTableFactory::INTERPOLATE_FACE_VARYING;
+    so.fvarChannel = 0;
+    Far::StencilTable const *fs = Far::StencilTableFactory::Create(*p->refiner, so);
+    if (auto const *local = p->table->GetLocalPointFaceVaryingStencilTable(0)) {
+        auto const *all = Far::StencilTableFactory::AppendLocalPointStencilTableFaceVarying(
+            *p->refiner, fs, local, 0);
+        delete fs;
+        fs = all;
+    }
+    p->st.fromTable(vs);
+    p->fst.fromTable(fs);
+    p->fst.nControl = p->nFVarBase;
+    delete vs;
+    delete fs;
+    p->flat = new Osd::CpuPatchTable(p->table);
+    p->map = new Far::PatchMap(*p->table);
+    return p;
+}
+
+// Face-varying channel 0 of a table from ref_patches_create_fvar: base value
+// count, its stencil table's size, the flattened array / index / param counts
+// (osd/cpuPatchTable.cpp:54-115) and the regular / irregular patch types.
+void ref_patches_fvar_info(void *h, int *nBase, int *nStencils, int *nnz, int *nArrays,
+                           int *nIndices, int *nParams, int *regType, int *irregType)
+{
+    Patches *p = (Patches *)h;
+    *nBase = p->nFVarBase;
+    *nStencils = (int)p->fst.sizes.size();
+    *nnz = (int)p->fst.indices.size();
+    *nArrays = (int)p->flat->GetNumPatchArrays();
+    *nIndices = (int)p->flat->GetFVarPatchIndexSize(0);
+    *nParams = (int)p->flat->GetFVarPatchParamSize(0);
+    *regType = (int)p->table->GetFVarPatchDescriptorRegular(0).GetType();
+    *irregType = (int)p->table->GetFVarPatchDescriptorIrregular(0).GetType();
+}
+
+void *ref_patches_fvar_stencils(void *h) { return &((Patches *)h)->fst; }
+
+void ref_patches_fvar_copy(void *h, int *arrays, int *indices, void *params)
+{
+    Patches *p = (Patches *)h;
+    if (arrays)
+        std::memcpy(arrays, p->flat->GetFVarPatchArrayBuffer(0),
+                    p->flat->GetNumPatchArrays() * sizeof(Osd::PatchArray));
+    if (indices)
+        std::memcpy(indices, p->flat->GetFVarPatchIndexBuffer(0),
+                    p->flat->GetFVarPatchIndexSize(0) * sizeof(int));
+    if (params)
+        std::memcpy(params, p->flat->GetFVarPatchParamBuffer(0),
+                    p->flat->GetFVarPatchParamSize(0) * sizeof(Osd::PatchParam));
 }
 
 void ref_patches_info(void *h, int *nCoarse, int *nStencils, int *nnz,

@@ -5,11 +5,17 @@
 // adaptively to level 3 with Gregory-basis end caps; Refine() runs on
 // both, then EvalPatches (P, du, dv) over the same patch coordinates.  The
 // refined vertex / varying buffers and every evaluated float are compared
-// bit for bit.  Prints "drop-in bit-exact" and exits 0 on success.
+// bit for bit.  A second cube carries a face-varying UV channel (one chart
+// per face, smooth UVs): Far's face-varying stencils refine it on both
+// evaluators, then EvalPatchesFaceVarying (P, du, dv) runs through
+// CpuPatchTable's and HipPatchTable::Create's flattening of the same Far
+// table.  Prints "drop-in bit-exact" and exits 0 on success.
 //
 // Built against the OpenSubdiv headers and its Sdc/Vtr/Far/Osd-CPU objects
 // (the oracle's reference build) and libosdhip.
 #include <opensubdiv/far/patchMap.h>
+#include <opensubdiv/far/patchTableFactory.h>
+#include <opensubdiv/far/stencilTableFactory.h>
 #include <opensubdiv/far/topologyDescriptor.h>
 #include <opensubdiv/osd/cpuEvaluator.h>
 #include <opensubdiv/osd/cpuPatchTable.h>
@@ -79,6 +85,129 @@ static int compare(const char *what, const float *a, const float *b, size_t n)
         }
     std::printf("%s: %zu floats, %zu differ\n", what, n, bad);
     return bad ? 1 : 0;
+}
+
+// Face-varying channel 0 of the cube of makeRefiner, one UV chart per face
+// (24 values, every edge a seam), FVAR_LINEAR_CORNERS_PLUS1, refined
+// adaptively to level 3 with B-spline / Gregory face-varying patches
+// (glEvalLimit's set-up, examples/glEvalLimit/glEvalLimit.cpp:684-750).
+static int faceVaryingCheck()
+{
+    static int nverts[6] = { 4, 4, 4, 4, 4, 4 };
+    static int faces[24] = { 0, 1, 3, 2, 2, 3, 5, 4, 4, 5, 7, 6,
+                             6, 7, 1, 0, 1, 7, 5, 3, 6, 0, 2, 4 };
+    static int uvIdx[24];
+    for (int i = 0; i < 24; ++i) uvIdx[i] = i;
+    Far::TopologyDescriptor d;
+    d.numVertices = 8;
+    d.numFaces = 6;
+    d.numVertsPerFace = nverts;
+    d.vertIndicesPerFace = faces;
+    Far::TopologyDescriptor::FVarChannel ch;
+    ch.numValues = 24;
+    ch.valueIndices = uvIdx;
+    d.numFVarChannels = 1;
+    d.fvarChannels = &ch;
+    typedef Far::TopologyRefinerFactory<Far::TopologyDescriptor> F;
+    Sdc::Options so;
+    so.SetVtxBoundaryInterpolation(Sdc::Options::VTX_BOUNDARY_EDGE_ONLY);
+    so.SetFVarLinearInterpolation(Sdc::Options::FVAR_LINEAR_CORNERS_PLUS1);
+    Far::TopologyRefiner *refiner = F::Create(d, F::Options(Sdc::SCHEME_CATMARK, so));
+    Far::PatchTableFactory::Options po(3);
+    po.SetEndCapType(Far::PatchTableFactory::Options::ENDCAP_GREGORY_BASIS);
+    po.generateFVarTables = true;
+    po.generateFVarLegacyLinearPatches = false;
+    refiner->RefineAdaptive(po.GetRefineAdaptiveOptions());
+    Far::PatchTable const *far = Far::PatchTableFactory::Create(*refiner, po);
+    Far::StencilTableFactory::Options sto;
+    sto.generateOffsets = true;
+    sto.generateIntermediateLevels = true;
+    sto.interpolationMode = Far::StencilTableFactory::INTERPOLATE_FACE_VARYING;
+    Far::StencilTable const *st = Far::StencilTableFactory::Create(*refiner, sto);
+    if (Far::StencilTable const *local = far->GetLocalPointFaceVaryingStencilTable()) {
+        Far::StencilTable const *all =
+            Far::StencilTableFactory::AppendLocalPointStencilTableFaceVarying(*refiner, st, local);
+        delete st;
+        st = all;
+    }
+    const int nbase = 24, nall = nbase + st->GetNumStencils();
+    float uv[24][2];
+    for (int f = 0; f < 6; ++f)
+        for (int k = 0; k < 4; ++k) {
+            uv[4 * f + k][0] = (float)(f % 3) / 3.0f + ((k == 1 || k == 2) ? 0.25f : 0.0f);
+            uv[4 * f + k][1] = (float)(f / 3) / 2.0f + (k >= 2 ? 0.375f : 0.0f);
+        }
+    CpuVB *cuv = CpuVB::Create(2, nall);
+    Osd::HipVertexBuffer *huv = Osd::HipVertexBuffer::Create(2, nall);
+    cuv->UpdateData(&uv[0][0], 0, nbase);
+    huv->UpdateData(&uv[0][0], 0, nbase);
+    Osd::HipStencilTable *hst = Osd::HipStencilTable::Create(st);
+    Osd::BufferDescriptor s2(0, 2, 2), r2(nbase * 2, 2, 2);
+    bool okc = Osd::CpuEvaluator::EvalStencils(cuv, s2, cuv, r2, st);
+    bool okh = Osd::HipEvaluator::EvalStencils(huv, s2, huv, r2, hst);
+    Osd::HipEvaluator::Synchronize();
+    int fails = 0;
+    if (!okc || !okh) {
+        std::printf("face-varying EvalStencils failed cpu=%d hip=%d\n", okc, okh);
+        return 1;
+    }
+    std::vector<float> h((size_t)nall * 2);
+    osdhip_memcpy_dtoh(h.data(), huv->BindHipBuffer(), h.size() * 4, NULL);
+    fails += compare("refined face-varying values", cuv->BindCpuBuffer(), h.data(), h.size());
+
+    Osd::CpuPatchTable *cpt = Osd::CpuPatchTable::Create(far);
+    Osd::HipPatchTable *hpt = Osd::HipPatchTable::Create(far);
+    Far::PatchMap pmap(*far);
+    std::vector<Osd::PatchCoord> coords;
+    for (int f = 0; f < 6; ++f)
+        for (int i = 0; i < 13; ++i)
+            for (int j = 0; j < 13; ++j) {
+                float s = i / 12.0f, t = j / 12.0f;
+                Far::PatchTable::PatchHandle const *ph = pmap.FindPatch(f, s, t);
+                if (!ph) continue;
+                Osd::PatchCoord c;
+                c.handle.arrayIndex = ph->arrayIndex;
+                c.handle.patchIndex = ph->patchIndex;
+                c.handle.vertIndex = ph->vertIndex;
+                c.s = s;
+                c.t = t;
+                coords.push_back(c);
+            }
+    const int n = (int)coords.size();
+    CpuVB *cc = CpuVB::Create(5, n);
+    std::memcpy(cc->BindCpuBuffer(), coords.data(), coords.size() * sizeof(Osd::PatchCoord));
+    Osd::HipVertexBuffer *hc = Osd::HipVertexBuffer::Create(5, n);
+    hc->UpdateData((const float *)coords.data(), 0, n);
+    CpuVB *cout[3];
+    Osd::HipVertexBuffer *hout[3];
+    for (int k = 0; k < 3; ++k) {
+        cout[k] = CpuVB::Create(2, n);
+        hout[k] = Osd::HipVertexBuffer::Create(2, n);
+    }
+    okc = Osd::CpuEvaluator::EvalPatchesFaceVarying(cuv, s2, cout[0], s2, cout[1], s2, cout[2], s2,
+                                                    n, cc, cpt, 0);
+    okh = Osd::HipEvaluator::EvalPatchesFaceVarying(huv, s2, hout[0], s2, hout[1], s2, hout[2], s2,
+                                                    n, hc, hpt, 0);
+    Osd::HipEvaluator::Synchronize();
+    if (!okc || !okh) {
+        std::printf("EvalPatchesFaceVarying failed cpu=%d hip=%d\n", okc, okh);
+        return 1;
+    }
+    const char *names[3] = { "EvalPatchesFaceVarying P", "EvalPatchesFaceVarying du",
+                             "EvalPatchesFaceVarying dv" };
+    std::vector<float> hr((size_t)n * 2);
+    for (int k = 0; k < 3; ++k) {
+        osdhip_memcpy_dtoh(hr.data(), hout[k]->BindHipBuffer(), hr.size() * 4, NULL);
+        fails += compare(names[k], cout[k]->BindCpuBuffer(), hr.data(), hr.size());
+        delete cout[k];
+        delete hout[k];
+    }
+    std::printf("face-varying patch coords: %d (%d fvar channel, patch types %d / %d)\n", n,
+                far->GetNumFVarChannels(), (int)far->GetFVarPatchDescriptorRegular(0).GetType(),
+                (int)far->GetFVarPatchDescriptorIrregular(0).GetType());
+    delete cc; delete hc; delete cuv; delete huv; delete hst; delete cpt; delete hpt;
+    delete st; delete far; delete refiner;
+    return fails;
 }
 
 int main()
@@ -185,6 +314,7 @@ int main()
     delete cc; delete cp; delete cu; delete cv;
     delete hc; delete hp; delete hu; delete hvv;
     std::printf("patch coords: %d\n", n);
+    fails += faceVaryingCheck();
     if (fails) return 1;
     std::printf("drop-in bit-exact\n");
     return 0;

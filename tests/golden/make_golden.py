@@ -3,7 +3,7 @@ reference (Far factories + Osd::CpuEvaluator), built from the reference
 sources into oracle/_ref/libosdref.so by oracle/Makefile.
 
 Run in the build container (where the reference tree exists):
-    make -C oracle && python tests/golden/make_golden.py
+    make -C oracle && python tests/golden/make_golden.py [fixture function ...]
 
 Each fixture holds inputs (tables, control points, coords) and the
 reference's outputs; nothing of the reference's source goes into them.
@@ -31,6 +31,17 @@ reference's outputs; nothing of the reference's source goes into them.
                         control points, PatchMap coords, CpuEvaluator
                         EvalPatches outputs (P/du/dv and the 2nd-derivative
                         arity) and Far EvaluateBasis<double> weights
+  cubesphere_fvar.npz   a face-varying UV channel (one chart per cube face,
+                        seams along the cube edges) on the creased
+                        cube-sphere n=4, iso 3, two ways: Far's defaults
+                        (FVAR_LINEAR_ALL, legacy linear QUADS patches) and
+                        smooth UVs (FVAR_LINEAR_CORNERS_PLUS1, B-spline and
+                        Gregory face-varying patches with face-varying local
+                        points); per variant: the face-varying stencil table,
+                        the refined UV buffer, the flattened face-varying
+                        patch arrays / indices / params, PatchMap coords and
+                        CpuEvaluator EvalPatchesFaceVarying outputs (P/du/dv
+                        and the 2nd-derivative arity)
 """
 from __future__ import annotations
 
@@ -154,6 +165,54 @@ def cubesphere_patches():
          cpu2_dvv=o2[5], basis_n=npts, basis_wP=wP, basis_wDs=wDs, basis_wDt=wDt)
 
 
+def cubesphere_fvar():
+    m = meshes.cube_sphere(4, seed=14)
+    nval, fidx, uv = meshes.cube_sphere_uvs(4)
+    out = {"face_verts": m.face_verts, "crease_pairs": m.crease_pairs,
+           "crease_weights": m.crease_weights, "fvar_indices": fidx,
+           "uv": uv.astype(np.float32)}
+    f, s, t = meshes.random_patch_locations(m.nfaces, 2048, seed=16)
+    for name, interp, legacy in (("linear", ref.FVAR_LINEAR_ALL, True),
+                                 ("smooth", ref.FVAR_LINEAR_CORNERS_PLUS1, False)):
+        topo = ref.Topology(m)
+        topo.set_fvar(nval, fidx, interp)
+        P = ref.Patches(topo, 3, fvar=True, fvar_legacy_linear=legacy)
+        fv = P.fvar
+        st = fv["stencils"]
+        nb, ns = fv["num_base"], len(st["sizes"])
+        assert nb == nval
+        buf = np.zeros((nb + ns, 2), np.float32)
+        buf[:nb] = out["uv"]
+        assert ref.cpu_eval_stencils(buf, (0, 2, 2), buf, (nb * 2, 2, 2), st)
+        h = P.find(f, s, t)
+        assert (h[:, 0] >= 0).all()
+        coords = np.zeros(len(f), PATCH_COORD_DTYPE)
+        coords["arrayIndex"], coords["patchIndex"], coords["vertIndex"] = h[:, 0], h[:, 1], h[:, 2]
+        coords["s"], coords["t"] = s.astype(np.float32), t.astype(np.float32)
+        n = len(coords)
+        o = [np.zeros((n, 2), np.float32) for _ in range(3)]
+        assert ref.cpu_eval_patches(buf, (0, 2, 2), o, [(0, 2, 2)] * 3, coords, fv["arrays"],
+                                    fv["indices"], fv["params"])
+        o2 = [np.zeros((n, 2), np.float32) for _ in range(6)]
+        assert ref.cpu_eval_patches_d2(buf, (0, 2, 2), o2, [(0, 2, 2)] * 6, coords,
+                                       fv["arrays"], fv["indices"], fv["params"])
+        out.update({
+            f"{name}_st_sizes": st["sizes"], f"{name}_st_offsets": st["offsets"],
+            f"{name}_st_indices": st["indices"], f"{name}_st_weights": st["weights"],
+            f"{name}_buffer": buf,
+            f"{name}_arrays": fv["arrays"].view(np.int32).reshape(-1, 6),
+            f"{name}_indices": fv["indices"],
+            f"{name}_params": fv["params"].view(np.uint32).reshape(-1, 3),
+            f"{name}_vertex_arrays": P.arrays.view(np.int32).reshape(-1, 6),
+            f"{name}_vertex_indices": P.indices,
+            f"{name}_vertex_params": P.params.view(np.uint32).reshape(-1, 3),
+            f"{name}_coords": coords.view(np.int32).reshape(-1, 5),
+            f"{name}_cpu_p": o[0], f"{name}_cpu_du": o[1], f"{name}_cpu_dv": o[2],
+            f"{name}_cpu2_p": o2[0], f"{name}_cpu2_du": o2[1], f"{name}_cpu2_dv": o2[2],
+            f"{name}_cpu2_duu": o2[3], f"{name}_cpu2_duv": o2[4], f"{name}_cpu2_dvv": o2[5]})
+    save("cubesphere_fvar.npz", **out)
+
+
 def tri_sphere(n, seed):
     """The cube-sphere's quads split along one diagonal (a closed Loop
     mesh, valences 4 to 6), 5 % of its edges creased."""
@@ -231,8 +290,10 @@ def loop_patches():
 if __name__ == "__main__":
     if not ref.available():
         sys.exit("oracle/_ref/libosdref.so not built (make -C oracle)")
-    cube_l3()
-    cubesphere_l2()
-    cubesphere_limit()
-    cubesphere_patches()
-    loop_patches()
+    # names on the command line write only those fixtures
+    makers = [cube_l3, cubesphere_l2, cubesphere_limit, cubesphere_patches, loop_patches,
+              cubesphere_fvar]
+    wanted = set(sys.argv[1:])
+    for fn in makers:
+        if not wanted or fn.__name__ in wanted:
+            fn()

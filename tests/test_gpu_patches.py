@@ -263,6 +263,53 @@ def test_python_mirror_eval_patches(hip):
     assert not HipEvaluator.EvalPatchesFaceVarying(src, d, P2, d, n, coords, ft, 1)
 
 
+@pytest.mark.parametrize("name", ["linear", "smooth"])
+def test_golden_face_varying_channel_bit_exact(hip, name):
+    """A real Far face-varying UV channel on the cube-sphere, end to end on
+    the device as glEvalLimit runs it: the face-varying stencil table refines
+    the UVs in a HipVertexBuffer (EvalStencils), then EvalPatchesFaceVarying
+    evaluates the channel at PatchMap coords -- both bit-exact with the
+    reference CpuEvaluator; linear QUADS patches (Far's defaults) and smooth
+    B-spline / Gregory patches with face-varying local points."""
+    from opensubdiv_amd import (BufferDescriptor, HipEvaluator, HipPatchTable, HipStencilTable,
+                                HipVertexBuffer)
+    g = golden("cubesphere_fvar.npz")
+    nb, nt = len(g["uv"]), len(g[name + "_buffer"])
+    vb = HipVertexBuffer.Create(2, nt)
+    vb.UpdateData(g["uv"], 0, nb)
+    st = HipStencilTable.Create({"sizes": g[name + "_st_sizes"], "offsets": g[name + "_st_offsets"],
+                                 "indices": g[name + "_st_indices"],
+                                 "weights": g[name + "_st_weights"]})
+    assert HipEvaluator.EvalStencils(vb, BufferDescriptor(0, 2, 2), vb,
+                                     BufferDescriptor(nb * 2, 2, 2), st)
+    assert HipEvaluator.Synchronize()
+    np.testing.assert_array_equal(vb.numpy(), g[name + "_buffer"])
+
+    def view(key, dtype):
+        return np.ascontiguousarray(g[key]).view(dtype).reshape(-1)
+
+    table = HipPatchTable(view(name + "_vertex_arrays", PATCH_ARRAY_DTYPE),
+                          g[name + "_vertex_indices"],
+                          view(name + "_vertex_params", PATCH_PARAM_DTYPE),
+                          fvar_channels=[(view(name + "_arrays", PATCH_ARRAY_DTYPE),
+                                          g[name + "_indices"],
+                                          view(name + "_params", PATCH_PARAM_DTYPE))])
+    n = len(g[name + "_coords"])
+    coords = dev(g[name + "_coords"])
+    d = BufferDescriptor(0, 2, 2)
+    outs = [empty(n * 2) for _ in range(6)]
+    assert HipEvaluator.EvalPatchesFaceVarying(vb, d, outs[0], d, outs[1], d, outs[2], d,
+                                               n, coords, table, 0)
+    assert HipEvaluator.Synchronize()
+    for o, k in zip(outs[:3], ("cpu_p", "cpu_du", "cpu_dv")):
+        np.testing.assert_array_equal(host(o, shape=(n, 2)), g[name + "_" + k], err_msg=k)
+    args = [x for o in outs[1:] for x in (o, d)]
+    assert HipEvaluator.EvalPatchesFaceVarying(vb, d, outs[0], d, *args, n, coords, table, 0)
+    assert HipEvaluator.Synchronize()
+    for o, k in zip(outs, ("cpu2_p", "cpu2_du", "cpu2_dv", "cpu2_duu", "cpu2_duv", "cpu2_dvv")):
+        np.testing.assert_array_equal(host(o, shape=(n, 2)), g[name + "_" + k], err_msg=k)
+
+
 @pytest.mark.parametrize("deriv", [0, 1, 2])
 def test_patch_sorted_order_bit_exact(hip, deriv):
     """Patch-ordered evaluation (osdhip_eval_patches_sorted: the coords

@@ -101,6 +101,73 @@ def test_patches_vs_cpu_evaluator():
     np.testing.assert_array_equal(p_only, g["cpu_p_only"])
 
 
+FVAR_CASES = ("linear", "smooth")
+
+
+@pytest.mark.parametrize("name", FVAR_CASES)
+def test_face_varying_channel_vs_cpu_evaluator(name):
+    """A real Far face-varying UV channel (seams along the cube edges): the
+    face-varying stencil table refines the UVs as Far's own table does, and
+    EvalPatchesFaceVarying through the channel's flattened arrays reproduces
+    CpuEvaluator bit for bit -- linear QUADS patches (Far's defaults) and
+    smooth B-spline / Gregory patches with face-varying local points."""
+    g = golden("cubesphere_fvar.npz")
+    nb = len(g["uv"])
+    buf = np.zeros_like(g[name + "_buffer"])
+    buf[:nb] = g["uv"]
+    assert oracle.eval_stencils(buf, (0, 2, 2), buf, (nb * 2, 2, 2), g[name + "_st_sizes"],
+                                g[name + "_st_offsets"], g[name + "_st_indices"],
+                                g[name + "_st_weights"])
+    np.testing.assert_array_equal(buf, g[name + "_buffer"])
+    arrays = np.ascontiguousarray(g[name + "_arrays"]).view(PATCH_ARRAY_DTYPE).reshape(-1)
+    params = np.ascontiguousarray(g[name + "_params"]).view(PATCH_PARAM_DTYPE).reshape(-1)
+    coords = np.ascontiguousarray(g[name + "_coords"]).view(PATCH_COORD_DTYPE).reshape(-1)
+    types = set(arrays["regDesc"].tolist()) | set(arrays["desc"].tolist())
+    assert types == ({QUADS} if name == "linear" else {REGULAR, GREGORY})
+    n = len(coords)
+    o = [np.zeros((n, 2), np.float32) for _ in range(6)]
+    assert oracle.eval_patches(buf, (0, 2, 2), o[:3], [(0, 2, 2)] * 3, coords, arrays,
+                               g[name + "_indices"], params)
+    for x, k in zip(o[:3], ("cpu_p", "cpu_du", "cpu_dv")):
+        np.testing.assert_array_equal(x, g[name + "_" + k], err_msg=k)
+    assert oracle.eval_patches(buf, (0, 2, 2), o, [(0, 2, 2)] * 6, coords, arrays,
+                               g[name + "_indices"], params)
+    for x, k in zip(o, ("cpu2_p", "cpu2_du", "cpu2_dv", "cpu2_duu", "cpu2_duv", "cpu2_dvv")):
+        np.testing.assert_array_equal(x, g[name + "_" + k], err_msg=k)
+
+
+def test_linear_face_varying_patches_interpolate_the_face_values():
+    """Closed form for the linear channel: a QUADS face-varying patch is the
+    bilinear blend of its four refined face values, so at a patch corner the
+    result is that corner's value, and on a chart the UVs stay inside the
+    chart's atlas cell (seams do not blend across charts)."""
+    g = golden("cubesphere_fvar.npz")
+    arrays = np.ascontiguousarray(g["linear_arrays"]).view(PATCH_ARRAY_DTYPE).reshape(-1)
+    params = np.ascontiguousarray(g["linear_params"]).view(PATCH_PARAM_DTYPE).reshape(-1)
+    coords = np.ascontiguousarray(g["linear_coords"]).view(PATCH_COORD_DTYPE).reshape(-1)
+    buf, idx = g["linear_buffer"], g["linear_indices"]
+    p = g["linear_cpu_p"]
+    # every coord's UV lies in the atlas cell of the cube face its ptex face is on
+    ptex = params["field0"][coords["patchIndex"]] & 0xfffffff
+    chart = ptex // 16
+    cu, cv = chart % 3, chart // 3
+    eps = 1e-6
+    assert np.all((p[:, 0] >= cu / 3 - eps) & (p[:, 0] <= (cu + 1) / 3 + eps))
+    assert np.all((p[:, 1] >= cv / 2 - eps) & (p[:, 1] <= (cv + 1) / 2 + eps))
+    # bilinear closed form at the coords, in float64, within float rounding
+    for k in range(0, len(coords), 97):
+        c = coords[k]
+        a = arrays[c["arrayIndex"]]
+        base = a["indexBase"] + 4 * (c["patchIndex"] - a["primitiveIdBase"])
+        cv4 = buf[idx[base:base + 4]].astype(np.float64)
+        f1 = int(params["field1"][c["patchIndex"]])
+        depth, u0, v0 = f1 & 15, (f1 >> 22) & 1023, (f1 >> 12) & 1023
+        s = float(c["s"]) * (1 << depth) - u0
+        t = float(c["t"]) * (1 << depth) - v0
+        w = np.array([(1 - s) * (1 - t), s * (1 - t), s * t, (1 - s) * t])
+        np.testing.assert_allclose(w @ cv4, p[k], rtol=0, atol=2e-7)
+
+
 def test_patch_basis_f64_vs_far_evaluate_basis():
     g, coords, arrays, idx, params = _patch_fixture()
     for i in range(len(g["basis_n"])):
