@@ -117,6 +117,26 @@ int osdfar_patch_table_copy(const osdfar_patch_table *t, int *arrays, int *indic
 /* Stencils of the refined and end-cap points (Far's appended local-point
  * stencil table, far/stencilTableFactory.cpp:256-408). */
 osdfar_stencil_table *osdfar_patch_table_point_stencils(const osdfar_patch_table *t, int useDouble);
+/* A face-varying channel on the topology (Far::TopologyDescriptor::
+ * FVarChannel, far/topologyDescriptor.h:73-84): numValues values and one
+ * value index per base face-vertex (numValues 0 removes it).  Later patch
+ * tables carry the channel with Far's default options: linear refinement
+ * everywhere (Sdc FVAR_LINEAR_ALL) and a linear QUADS face-varying patch per
+ * patch over the four values of its face (generateFVarLegacyLinearPatches,
+ * far/patchTableFactory.cpp:1255-1266), PatchParams equal to the vertex
+ * patches'.  Returns -1 for bad arguments. */
+int osdfar_topology_set_fvar(osdfar_topology *topo, int numValues, const int *valueIndices);
+/* The channel's buffer is [base values | refined values]; info gives their
+ * counts and the flattened sizes (Osd::CpuPatchTable's face-varying buffers,
+ * osd/cpuPatchTable.cpp:54-115), copy the arrays (6 ints each), indices and
+ * params, and the stencil table the refined values from the base values
+ * (Far's INTERPOLATE_FACE_VARYING table, far/stencilTableFactory.h:55-80). */
+int osdfar_patch_table_fvar_info(const osdfar_patch_table *t, int channel, int *numBase,
+                                 int *numRefined, int *numArrays, int *numIndices);
+int osdfar_patch_table_fvar_copy(const osdfar_patch_table *t, int channel, int *arrays,
+                                 int *indices, void *params);
+osdfar_stencil_table *osdfar_patch_table_fvar_stencils(const osdfar_patch_table *t, int channel,
+                                                       int useDouble);
 /* PatchMap::FindPatch (far/patchMap.h:193-231) for n (ptex face, u, v)
  * queries; writes Osd::PatchCoord records {arrayIndex, patchIndex,
  * vertIndex, s, t} (arrayIndex = -1 when no patch covers the location).

@@ -44,6 +44,10 @@ def load_far():
         "osdfar_patch_table_info": ([vp, vp, vp, vp, vp, vp, vp, vp], i),
         "osdfar_patch_table_copy": ([vp, vp, vp, vp], i),
         "osdfar_patch_table_point_stencils": ([vp, i], vp),
+        "osdfar_topology_set_fvar": ([vp, i, vp], i),
+        "osdfar_patch_table_fvar_info": ([vp, i, vp, vp, vp, vp], i),
+        "osdfar_patch_table_fvar_copy": ([vp, i, vp, vp, vp], i),
+        "osdfar_patch_table_fvar_stencils": ([vp, i, i], vp),
         "osdfar_patch_map_find": ([vp, i, vp, vp, vp, vp], i),
         "osdfar_limit_stencil_table_create": ([vp, i, vp, vp, vp, i, i], vp),
         "osdfar_vertex_locations": ([vp, i, vp, vp, vp], i),
@@ -172,6 +176,13 @@ class Topology:
     def refine_uniform(self, level):
         assert load_far().osdfar_refine_uniform(self.h, level) == 0
 
+    def set_fvar(self, num_values, value_indices):
+        """A face-varying channel: one value index per base face-vertex
+        (linear refinement and linear patches, Far's defaults)."""
+        vi = np.ascontiguousarray(value_indices, np.int32).reshape(-1)
+        if load_far().osdfar_topology_set_fvar(self.h, int(num_values), _p(vi)) != 0:
+            raise ValueError("face-varying channel rejected")
+
     def cache_stencils(self, level):
         """Refine to `level` and keep the level stencils, shared by the next
         stencil_table() and patch_table(level) calls."""
@@ -245,6 +256,29 @@ class PatchTable:
         self.indices = np.zeros(ni, np.int32)
         self.params = np.zeros(npch, PATCH_PARAM_DTYPE)
         L.osdfar_patch_table_copy(self.h, _p(self.arrays), _p(self.indices), _p(self.params))
+        # face-varying channel 0, when the topology has one: buffer layout
+        # [base values | refined values]
+        self.fvar = None
+        v = [C.c_int() for _ in range(4)]
+        if L.osdfar_patch_table_fvar_info(self.h, 0, *[C.byref(x) for x in v]) == 0:
+            nb, nr, fna, fni = [x.value for x in v]
+            fa = np.zeros(fna, PATCH_ARRAY_DTYPE)
+            fi = np.zeros(fni, np.int32)
+            fp = np.zeros(npch, PATCH_PARAM_DTYPE)
+            L.osdfar_patch_table_fvar_copy(self.h, 0, _p(fa), _p(fi), _p(fp))
+            self.fvar = {"num_base": nb, "num_refined": nr, "arrays": fa, "indices": fi,
+                         "params": fp}
+
+    def fvar_stencils(self, double=False):
+        """Stencils of the face-varying channel's refined values over its
+        base values."""
+        h = load_far().osdfar_patch_table_fvar_stencils(self.h, 0, int(double))
+        if not h:
+            raise ValueError("no face-varying channel")
+        try:
+            return _read_table(h, double)
+        finally:
+            load_far().osdfar_stencil_table_destroy(h)
 
     @property
     def num_vertices_total(self):
@@ -280,7 +314,11 @@ class PatchTable:
             load_far().osdfar_stencil_table_destroy(h)
 
     def flat(self):
-        return {"arrays": self.arrays, "indices": self.indices, "params": self.params}
+        out = {"arrays": self.arrays, "indices": self.indices, "params": self.params}
+        if self.fvar is not None:
+            out["fvar_channels"] = [(self.fvar["arrays"], self.fvar["indices"],
+                                     self.fvar["params"])]
+        return out
 
     def __del__(self):
         if getattr(self, "h", None) and _far is not None:

@@ -11,6 +11,8 @@ using namespace osdfar;
 struct osdfar_topology {
     Refiner refiner;
     std::vector<LevelStencils> levels;     // cached vertex-mode level stencils
+    int fvarValues = 0;                    // face-varying channel (0: none)
+    std::vector<int> fvarIndices;          // one value index per base face-vertex
 };
 
 struct osdfar_stencil_table {
@@ -219,6 +221,9 @@ osdfar_patch_table *osdfar_patch_table_create(osdfar_topology *topo, int isoLeve
                             topo->refiner.maxLevel() == isoLevel;
         if (!t || !t->table.build(topo->refiner, isoLevel, cached ? &topo->levels : nullptr))
             return nullptr;
+        if (topo->fvarValues > 0 &&
+            !t->table.buildFaceVarying(topo->refiner, topo->fvarValues, topo->fvarIndices.data()))
+            return nullptr;
         return t.release();
     } catch (...) {
         return nullptr;
@@ -251,6 +256,61 @@ int osdfar_patch_table_copy(const osdfar_patch_table *t, int *arrays, int *indic
     if (indices) std::memcpy(indices, p.indices.data(), p.indices.size() * sizeof(int));
     if (params) std::memcpy(params, p.params.data(), p.params.size() * sizeof(PatchParamRec));
     return 0;
+}
+
+int osdfar_topology_set_fvar(osdfar_topology *topo, int numValues, const int *valueIndices)
+{
+    try {
+        if (!topo || numValues < 0 || (numValues > 0 && !valueIndices)) return -1;
+        const Level &L0 = topo->refiner.level(0);
+        for (size_t k = 0; numValues > 0 && k < L0.fv.size(); ++k)
+            if (valueIndices[k] < 0 || valueIndices[k] >= numValues) return -1;
+        topo->fvarValues = numValues;
+        if (numValues > 0) topo->fvarIndices.assign(valueIndices, valueIndices + L0.fv.size());
+        else topo->fvarIndices.clear();
+        return 0;
+    } catch (...) {
+        return -3;
+    }
+}
+
+int osdfar_patch_table_fvar_info(const osdfar_patch_table *t, int channel, int *numBase,
+                                 int *numRefined, int *numArrays, int *numIndices)
+{
+    if (!t || channel < 0 || channel >= (int)t->table.fvar.size()) return -1;
+    const PatchTable::FVarChannel &c = t->table.fvar[channel];
+    if (numBase) *numBase = c.numBase;
+    if (numRefined) *numRefined = c.numRefined;
+    if (numArrays) *numArrays = (int)c.arrays.size();
+    if (numIndices) *numIndices = (int)c.indices.size();
+    return 0;
+}
+
+int osdfar_patch_table_fvar_copy(const osdfar_patch_table *t, int channel, int *arrays,
+                                 int *indices, void *params)
+{
+    if (!t || channel < 0 || channel >= (int)t->table.fvar.size()) return -1;
+    const PatchTable::FVarChannel &c = t->table.fvar[channel];
+    if (arrays) std::memcpy(arrays, c.arrays.data(), c.arrays.size() * sizeof(PatchArrayRec));
+    if (indices) std::memcpy(indices, c.indices.data(), c.indices.size() * sizeof(int));
+    if (params) std::memcpy(params, c.params.data(), c.params.size() * sizeof(PatchParamRec));
+    return 0;
+}
+
+osdfar_stencil_table *osdfar_patch_table_fvar_stencils(const osdfar_patch_table *t, int channel,
+                                                       int useDouble)
+{
+    try {
+        if (!t || channel < 0 || channel >= (int)t->table.fvar.size()) return nullptr;
+        std::unique_ptr<osdfar_stencil_table> s(new (std::nothrow) osdfar_stencil_table);
+        if (!s) return nullptr;
+        s->isDouble = useDouble != 0;
+        if (s->isDouble) t->table.fvarStencilTable(channel, s->d);
+        else t->table.fvarStencilTable(channel, s->f);
+        return s.release();
+    } catch (...) {
+        return nullptr;
+    }
 }
 
 osdfar_stencil_table *osdfar_patch_table_point_stencils(const osdfar_patch_table *t, int useDouble)

@@ -629,6 +629,8 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
     indices.clear();
     params.clear();
     patchDepth_.clear();
+    patchLeaf_.clear();
+    fvar.clear();
     auto paramOf = [&](const Leaf &lf, bool regular) {
         PatchParamRec p;
         int ptex, u = 0, v = 0;
@@ -660,12 +662,14 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
         }
         params.push_back(paramOf(lf, true));
         patchDepth_.push_back(lf.level);
+        patchLeaf_.push_back({ lf.level, lf.face });
     }
     if (nGreg) arrays.push_back({ kGregory, kGregory, nGreg, (int)indices.size(), 20, nReg });
     for (int i = 0; i < nGreg; ++i) {
         for (int k = 0; k < 20; ++k) indices.push_back(numControl + numRefined + gregLocal[20 * i + k]);
         params.push_back(paramOf(*gregLeaves[i], false));
         patchDepth_.push_back(gregLeaves[i]->level);
+        patchLeaf_.push_back({ gregLeaves[i]->level, gregLeaves[i]->face });
     }
 
     // point stencils: refined vertices (level order), then end-cap points

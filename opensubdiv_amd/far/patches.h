@@ -78,8 +78,27 @@ public:
     int limitStencils(int n, const int *ptex, const double *u, const double *v, bool second,
                       StencilTableT<REAL> &out) const;
 
+    // Face-varying channels (fvar.cpp): linear refinement and linear QUADS
+    // patches, Far's defaults.  The channel's buffer is
+    // [base values | refined values], its stencils give the refined values
+    // from the base ones; arrays / params mirror the vertex patches.
+    struct FVarChannel {
+        int numBase = 0, numRefined = 0;
+        std::vector<PatchArrayRec> arrays;
+        std::vector<int> indices;
+        std::vector<PatchParamRec> params;
+        LevelStencils stencils;
+    };
+    std::vector<FVarChannel> fvar;
+    // Adds a channel: numValues values, one value index per face-vertex of
+    // the base level.  False for out-of-range indices.
+    bool buildFaceVarying(const Refiner &r, int numValues, const int *valueIndices);
+    template <typename REAL>
+    void fvarStencilTable(int channel, StencilTableT<REAL> &out) const;
+
 private:
     std::vector<int> patchDepth_;           // per patch (for derivative scaling)
+    std::vector<std::pair<int, int>> patchLeaf_;   // per patch: (level, face)
 };
 
 // One (ptex, u, v) per vertex of a uniform refinement level: the location

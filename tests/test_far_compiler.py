@@ -574,3 +574,169 @@ def test_patch_table_with_inf_sharp_features_matches_far_live(name, iso):
                                    P.indices, P.params)
     for a, b in zip(outs, o):
         assert bbox_rel_err(a, b) < 1e-12
+
+
+# ------------------------------------------------------------ face-varying
+
+def _fvar_surface(arrays, indices, params, stencils, base, coords):
+    """A face-varying channel evaluated in float64: base values refined by
+    the channel's stencils, then P / d/du / d/dv at the coords."""
+    base = np.asarray(base, np.float64)
+    L = base.shape[1]
+    buf = np.ascontiguousarray(np.concatenate([base, oracle.apply_stencils(
+        base, stencils["sizes"], stencils["offsets"], stencils["indices"],
+        np.asarray(stencils["weights"], np.float64), np.float64)]))
+    outs = [np.zeros((len(coords), L)) for _ in range(3)]
+    assert oracle.eval_patches_f64(buf, (0, L, L), outs, [(0, L, L)] * 3, coords, arrays,
+                                   indices, params)
+    return outs
+
+
+def test_fvar_channel_matches_far_fixture():
+    """The compiler's face-varying channel (linear refinement, linear QUADS
+    patches: Far's defaults) against Far's on the cube-sphere UV atlas with
+    seams along every cube edge: the same flattened arrays and PatchParams,
+    and the same UVs and UV derivatives at the fixture's PatchMap coords."""
+    g = golden("cubesphere_fvar.npz")
+    m = mesh_from(g)
+    nval, fidx, uv = meshes.cube_sphere_uvs(4)
+    np.testing.assert_array_equal(fidx, g["fvar_indices"])
+    t = far.Topology(m)
+    t.set_fvar(nval, fidx)
+    pt = t.patch_table(3)
+    fv = pt.fvar
+    far_arrays = g["linear_arrays"].view(PATCH_ARRAY_DTYPE).reshape(-1)
+    assert [tuple(a) for a in fv["arrays"]] == [tuple(a) for a in far_arrays]
+    assert fv["params"].tobytes() == np.ascontiguousarray(g["linear_params"]).tobytes()
+    assert fv["params"].tobytes() == pt.params.tobytes()
+    assert fv["num_base"] == nval and len(fv["indices"]) == 4 * len(pt.params)
+    coords = np.ascontiguousarray(g["linear_coords"]).view(PATCH_COORD_DTYPE).reshape(-1)
+    mine = _fvar_surface(fv["arrays"], fv["indices"], fv["params"], pt.fvar_stencils(double=True),
+                         g["uv"], coords)
+    theirs = _fvar_surface(far_arrays, g["linear_indices"],
+                           g["linear_params"].view(PATCH_PARAM_DTYPE).reshape(-1),
+                           {"sizes": g["linear_st_sizes"], "offsets": g["linear_st_offsets"],
+                            "indices": g["linear_st_indices"],
+                            "weights": g["linear_st_weights"]}, g["uv"], coords)
+    # dyadic weights on quads: the same surface to double rounding
+    for a, b in zip(mine, theirs):
+        assert np.abs(a - b).max() < 1e-12
+    # and the reference's float evaluation to float rounding (UVs and their
+    # derivatives are O(1): absolute error)
+    for a, k in zip(mine, ("cpu_p", "cpu_du", "cpu_dv")):
+        assert np.abs(a - g["linear_" + k]).max() < 2e-6, k
+
+
+def test_fvar_channel_stencils_are_bilinear_interpolants():
+    """Closed form: under linear refinement, a refined value inside a quad
+    base face is the bilinear interpolant of the face's four values at the
+    value's parametric location -- checked at every corner of every patch,
+    and shown to catch a planted mistake (edge children taking one end's
+    value instead of the midpoint)."""
+    m = meshes.cube_sphere(3, seed=2)
+    nval, fidx, uv = meshes.cube_sphere_uvs(3)
+    t = far.Topology(m)
+    t.set_fvar(nval, fidx)
+    pt = t.patch_table(3)
+    fv = pt.fvar
+    st = pt.fvar_stencils(double=True)
+    base = uv.astype(np.float64)
+    p = fv["params"]
+    ptex = p["field0"] & 0xfffffff
+    depth = p["field1"] & 0xF
+    pu, pv = (p["field1"] >> 22) & 0x3FF, (p["field1"] >> 12) & 0x3FF
+    corner = np.array([[0, 0], [1, 0], [1, 1], [0, 1]])
+
+    def closed_form_error(weights):
+        buf = np.concatenate([base, oracle.apply_stencils(base, st["sizes"], st["offsets"],
+                                                          st["indices"], weights, np.float64)])
+        err = 0.0
+        for k in range(len(p)):
+            c = base[fidx[ptex[k]]]                   # the ptex (base) face's 4 values
+            for j in range(4):
+                s = (pu[k] + corner[j, 0]) / float(1 << depth[k])
+                tt = (pv[k] + corner[j, 1]) / float(1 << depth[k])
+                w = np.array([(1 - s) * (1 - tt), s * (1 - tt), s * tt, (1 - s) * tt])
+                err = max(err, float(np.abs(w @ c - buf[fv["indices"][4 * k + j]]).max()))
+        return err
+
+    assert closed_form_error(st["weights"]) < 1e-14
+    planted = np.array(st["weights"], np.float64)
+    for r in np.nonzero(st["sizes"] == 2)[0]:         # edge points on base edges
+        o = int(st["offsets"][r])
+        planted[o:o + 2] = [1.0, 0.0]
+    assert closed_form_error(planted) > 1e-3
+
+
+def _uv_channels(m, seed):
+    """Three UV channels of a mesh: one value per face-vertex (every edge a
+    seam), one per vertex (no seam), and a mix (faces split into two charts)."""
+    rng = np.random.default_rng(seed)
+    nfv = np.asarray(m.face_verts).size
+    per_face = (nfv, np.arange(nfv, dtype=np.int32), rng.random((nfv, 2)))
+    shared = (m.nverts, np.asarray(m.face_verts, np.int32).reshape(-1), rng.random((m.nverts, 2)))
+    sizes = getattr(m, "face_sizes", None)
+    if sizes is None:
+        sizes = np.full(m.nfaces, 4)
+    chart = np.repeat(np.arange(m.nfaces) % 2, sizes)
+    mixed_idx = np.asarray(m.face_verts, np.int32).reshape(-1) + chart * m.nverts
+    mixed = (2 * m.nverts, mixed_idx.astype(np.int32), rng.random((2 * m.nverts, 2)))
+    return {"seams": per_face, "shared": shared, "two-charts": mixed}
+
+
+@needs_ref
+@pytest.mark.parametrize("name,iso,boundary", [("pentagonal-prism", 3, 1),
+                                               ("cube-with-triangles", 2, 1),
+                                               ("open-grid-creased", 3, 2),
+                                               ("sphere-with-split-quad", 3, 1),
+                                               ("cubesphere", 3, 1)])
+def test_fvar_channel_matches_far_live(name, iso, boundary):
+    """Face-varying channels on n-gon, open and creased meshes, with seams
+    everywhere, nowhere and along chart borders: the same arrays and
+    PatchParams as Far's channel and the same UVs (and derivatives) at
+    random PatchMap locations of every ptex face: to double rounding on quad
+    meshes, within the rounding of Far's float weights (1/n face points) on
+    n-gon meshes."""
+    m = meshes.cube_sphere(3, seed=4) if name == "cubesphere" else POLY_MESHES[name]()
+    sizes = getattr(m, "face_sizes", None)
+    nptex = m.nfaces if sizes is None else int(sum(1 if n == 4 else n for n in sizes))
+    f, u, v = meshes.random_patch_locations(nptex, 1500, seed=iso + 7)
+    for kind, (nval, fidx, vals) in _uv_channels(m, iso).items():
+        t = far.Topology(m, boundary=boundary)
+        t.set_fvar(nval, fidx)
+        pt = t.patch_table(iso)
+        rt = ref.Topology(m, boundary=boundary)
+        rt.set_fvar(nval, fidx, ref.FVAR_LINEAR_ALL)
+        P = ref.Patches(rt, iso, fvar=True, fvar_legacy_linear=True)
+        assert [tuple(a) for a in pt.fvar["arrays"]] == [tuple(a) for a in P.fvar["arrays"]], kind
+        assert pt.fvar["params"].tobytes() == P.fvar["params"].tobytes(), kind
+        coords = pt.find(f, u, v)
+        assert (coords["arrayIndex"] >= 0).all()
+        h = P.find(f, u, v)
+        np.testing.assert_array_equal(coords["patchIndex"], h[:, 1])
+        mine = _fvar_surface(pt.fvar["arrays"], pt.fvar["indices"], pt.fvar["params"],
+                             pt.fvar_stencils(double=True), vals, coords)
+        theirs = _fvar_surface(P.fvar["arrays"], P.fvar["indices"], P.fvar["params"],
+                               P.fvar["stencils"], vals, coords)
+        # UVs are O(1); quad meshes have dyadic weights (agreement to double
+        # rounding), n-gon face points 1/n, which Far's float table rounds
+        tol = 1e-12 if sizes is None or (np.asarray(sizes) == 4).all() else 1e-6
+        for a, b in zip(mine, theirs):
+            assert np.abs(a - b).max() < tol, kind
+        # a channel with seams needs more values than one without
+        if kind == "seams":
+            assert pt.fvar["num_refined"] > 0
+
+
+def test_fvar_channel_errors():
+    m = meshes.cube_sphere(2, seed=1)
+    t = far.Topology(m)
+    nval, fidx, _ = meshes.cube_sphere_uvs(2)
+    bad = fidx.copy()
+    bad[0, 0] = nval
+    with pytest.raises(ValueError):
+        t.set_fvar(nval, bad)
+    pt = t.patch_table(2)
+    assert pt.fvar is None
+    with pytest.raises(ValueError):
+        pt.fvar_stencils()
