@@ -414,6 +414,17 @@ int osdhip_batch_set(osdhip_batch *batch, const char *key, int value);
 int osdhip_batch_get(const osdhip_batch *batch, const char *key);
 int osdhip_batch_info(const osdhip_batch *batch, long long *stencilRows, long long *frameRows,
                       long long *patchCoords, long long *patchFrames, long long *tableBytes);
+/* Inspection of a patch segment's point staging lists, built at batch
+ * creation (no counterpart in the reference): per wave of 64 coords the
+ * number of distinct points its regular-interior coords use (-1: more than
+ * 256, not staged) and the offset of their ascending list in `pts`, and per
+ * coord 4 words of slot bytes (byte j of a regular-interior coord: its
+ * point j's position in the wave's list).  `numPts` (required) receives the
+ * length of `pts`, -1 when the segment has no lists; null outputs are
+ * skipped.  Host buffers of nw = ceil(ncoords / 64) (nuniq, ptsOff),
+ * numPts (pts) and 4 * ncoords (slots) elements; synchronous. */
+int osdhip_batch_patch_staging(const osdhip_batch *batch, int segment, int *nuniq,
+                               long long *ptsOff, int *pts, unsigned *slots, long long *numPts);
 /* Autotuner: times every variant of each phase (median of `reps` runs with
  * HIP events on `stream`), keeps the fastest and re-captures the graph.
  * `report` (optional) receives the timings as JSON. */

@@ -109,6 +109,7 @@ _HIP_SIGS = {
     "osdhip_batch_get": [_vp, C.c_char_p],
     "osdhip_batch_info": [_vp, _vp, _vp, _vp, _vp, _vp],
     "osdhip_batch_tune": [_vp, _i, _vp, _vp, _i],
+    "osdhip_batch_patch_staging": [_vp, _i, _vp, _vp, _vp, _vp, _vp],
     "osdhip_batch_epoch_bytes": [_i, _i, _vp, _i, _vp, _i],
     "osdhip_batch_epoch": [_i, _i, _vp, _i, _vp, _i, _vp, C.c_longlong, _vp, C.c_longlong, _vp],
 }

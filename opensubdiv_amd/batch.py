@@ -101,6 +101,28 @@ class Batch:
         keys = ("stencil_rows", "frame_rows", "patch_coords", "patch_frames", "table_bytes")
         return dict(zip(keys, (x.value for x in v)))
 
+    def patch_staging(self, segment):
+        """Patch segment `segment`'s point staging lists (numpy arrays):
+        nuniq per 64-coord wave (-1: not staged), ptsOff, pts (ascending
+        per wave) and 4 slot words per coord; None without lists."""
+        import numpy as np
+        n = ctypes.c_longlong()
+        _lib.check(self._lib.osdhip_batch_patch_staging(self.h, int(segment), None, None, None,
+                                                        None, ctypes.byref(n)),
+                   "osdhip_batch_patch_staging")
+        if n.value < 0:
+            return None
+        ncoords = int(self._keep[1][segment]["n"])
+        nw = (ncoords + 63) // 64
+        nuniq = np.zeros(nw, np.int32)
+        off = np.zeros(nw, np.int64)
+        pts = np.zeros(max(1, n.value), np.int32)
+        slots = np.zeros(4 * ncoords, np.uint32)
+        _lib.check(self._lib.osdhip_batch_patch_staging(
+            self.h, int(segment), nuniq.ctypes.data, off.ctypes.data, pts.ctypes.data,
+            slots.ctypes.data, ctypes.byref(n)), "osdhip_batch_patch_staging")
+        return {"nuniq": nuniq, "ptsOff": off, "pts": pts[:n.value], "slots": slots}
+
     def close(self):
         if getattr(self, "h", None):
             self._lib.osdhip_batch_destroy(self.h)

@@ -722,6 +722,40 @@ int osdhip_batch_info(const osdhip_batch *b, long long *stencilRows, long long *
     return OSDHIP_OK;
 }
 
+int osdhip_batch_patch_staging(const osdhip_batch *b, int segment, int *nuniq, long long *ptsOff,
+                               int *pts, unsigned *slots, long long *numPts)
+{
+    try {
+        if (!b || segment < 0 || segment >= b->nP || !numPts) return OSDHIP_ERR_ARGS;
+        const BatchPatchSeg &g = b->hP[(size_t)segment];
+        const int nw = (g.ncoords + kWave - 1) / kWave;
+        if (!g.nuniq || nw == 0) {
+            *numPts = -1;
+            return OSDHIP_OK;
+        }
+        std::vector<int> nu((size_t)nw);
+        std::vector<long long> off((size_t)nw);
+        hipError_t e = hipMemcpy(nu.data(), g.nuniq, (size_t)nw * sizeof(int), hipMemcpyDeviceToHost);
+        if (e == hipSuccess)
+            e = hipMemcpy(off.data(), g.ptsOff, (size_t)nw * sizeof(long long), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return (int)e;
+        const long long total = off.back() + std::max(0, nu.back());
+        *numPts = total;
+        if (nuniq) std::copy(nu.begin(), nu.end(), nuniq);
+        if (ptsOff) std::copy(off.begin(), off.end(), ptsOff);
+        if (pts && total > 0 &&
+            (e = hipMemcpy(pts, g.pts, (size_t)total * sizeof(int), hipMemcpyDeviceToHost)) != hipSuccess)
+            return (int)e;
+        if (slots &&
+            (e = hipMemcpy(slots, g.slots, (size_t)g.ncoords * 4 * sizeof(unsigned),
+                           hipMemcpyDeviceToHost)) != hipSuccess)
+            return (int)e;
+        return OSDHIP_OK;
+    } catch (...) {
+        return OSDHIP_ERR_ARGS;
+    }
+}
+
 int osdhip_batch_tune(osdhip_batch *b, int reps, void *stream, char *report, int reportBytes)
 {
     try {

@@ -125,6 +125,51 @@ def test_plain_path_matches_oracle(workload):
                                               err_msg=f"mesh {s.index} patch output {k} f{f}")
 
 
+def test_patch_staging_lists_stay_in_bounds(workload):
+    """The point staging lists k_patch_slots builds at batch creation, read
+    back and checked on the host before any kernel relies on them: per wave
+    a count in [-1, 256] and contiguous offsets; listed points strictly
+    ascending and inside the segment's point buffer; every regular-interior
+    coord's 16 slot bytes below its wave's count, naming exactly the
+    patch's 16 points.  (The staged patch kernel reads LDS slots and the
+    point buffer through these lists.)"""
+    from opensubdiv_amd.osd import PATCH_ARRAY_DTYPE, PATCH_COORD_DTYPE, PATCH_PARAM_DTYPE
+    length, hosts, dms, batch = workload
+    staged = 0
+    for seg, d in enumerate(dms):
+        lists = batch.patch_staging(seg)
+        assert lists is not None
+        nu, off, pts, slots = lists["nuniq"], lists["ptsOff"], lists["pts"], lists["slots"]
+        nw = (d.ncoords + 63) // 64
+        assert len(nu) == nw and ((nu >= -1) & (nu <= 256)).all()
+        np.testing.assert_array_equal(off, np.concatenate([[0], np.cumsum(np.maximum(nu, 0))[:-1]]))
+        assert len(pts) == int(np.maximum(nu, 0).sum())
+        npoints = d.nctrl + d.npts                       # rows of the segment's point buffer
+        coords = d.coords.cpu().numpy().view(PATCH_COORD_DTYPE).reshape(-1)
+        arrays = d.arrays.cpu().numpy().view(PATCH_ARRAY_DTYPE).reshape(-1)
+        params = d.params.cpu().numpy().view(PATCH_PARAM_DTYPE).reshape(-1)
+        indices = d.indices.cpu().numpy()
+        sb = slots.view(np.uint8).reshape(-1, 16)
+        for w in range(nw):
+            if nu[w] < 0:
+                continue
+            lst = pts[off[w]:off[w] + nu[w]]
+            assert ((lst >= 0) & (lst < npoints)).all(), (seg, w)
+            assert (np.diff(lst) > 0).all(), (seg, w)
+            for i in range(64 * w, min(64 * w + 64, d.ncoords)):
+                c = coords[i]
+                a = arrays[c["arrayIndex"]]
+                f1 = int(params[c["patchIndex"]]["field1"])
+                ptype = a["regDesc"] if (f1 >> 5) & 1 else a["desc"]
+                if ptype != 6 or (f1 >> 7) & 0x1F:
+                    continue                              # not regular-interior: no slots
+                b = a["indexBase"] + a["stride"] * (c["patchIndex"] - a["primitiveIdBase"])
+                assert (sb[i] < nu[w]).all(), (seg, i)
+                np.testing.assert_array_equal(lst[sb[i]], indices[b:b + 16], err_msg=f"{seg} {i}")
+                staged += 1
+    assert staged > 0
+
+
 def test_graph_launch_matches_plain_and_replays(workload):
     length, hosts, dms, batch = workload
     poison(dms)
