@@ -107,6 +107,29 @@ void osdfar_stencil_table_destroy(osdfar_stencil_table *t);
  * whose non-control rows are given by osdfar_patch_table_point_stencils. */
 typedef struct osdfar_patch_table osdfar_patch_table;
 osdfar_patch_table *osdfar_patch_table_create(osdfar_topology *topo, int isoLevel);
+/* Options of Far::PatchTableFactory / TopologyRefiner::RefineAdaptive the
+ * compiler follows (osdfar_patch_table_create uses the defaults):
+ *   secondaryLevel      smooth extraordinary vertices are isolated to this
+ *                       level only, sharp features to isoLevel; irregular
+ *                       faces left at a shallower level get Gregory end caps
+ *                       there (AdaptiveOptions::secondaryLevel,
+ *                       far/topologyRefiner.h:156-182; default 15 = isoLevel)
+ *   legacySharpCorners  1 (Far's default): smooth single-face boundary
+ *                       corners are patched as sharp corners; 0: end caps
+ *   shareEndCapPoints   1 (Far's default): end-cap points shared between
+ *                       neighbouring caps; 0: 20 points per cap
+ * With secondaryLevel 2, isoLevel 6, legacySharpCorners 0 and
+ * shareEndCapPoints 0 the table is the one Bfr's surfaces are compared with
+ * (regression/bfr_evaluate/farPatchEvaluator.cpp:52-84): evaluated through
+ * the patch map it gives Bfr::Surface::Evaluate's limit surface (for meshes
+ * without interior infinitely sharp features). */
+typedef struct {
+    int secondaryLevel;
+    int legacySharpCorners;
+    int shareEndCapPoints;
+} osdfar_patch_options;
+osdfar_patch_table *osdfar_patch_table_create_ex(osdfar_topology *topo, int isoLevel,
+                                                 const osdfar_patch_options *options);
 void osdfar_patch_table_destroy(osdfar_patch_table *t);
 int osdfar_patch_table_info(const osdfar_patch_table *t, int *numControl, int *numRefined,
                             int *numLocal, int *numArrays, int *numIndices, int *numPatches,

@@ -291,6 +291,23 @@ int osdhip_patch_map_find(const osdhip_patch_map *map, int n, const int *ptexFac
                           const double *u, const double *v, osdhip_patch_coord *coords,
                           void *stream);
 void osdhip_patch_map_destroy(osdhip_patch_map *map);
+/* Bfr-style per-face queries (Bfr::Surface::Evaluate, bfr/surface.h:
+ * 227-266): query i is base face faces[i] at (u[i], v[i]) in Bfr's
+ * parameterization of that face (bfr/parameterization.h:60-200) -- a
+ * quad's unit square, or for an n-gon its n quad sub-faces tiled on a grid
+ * (sub-face k on tile (k % uDim, k / uDim), uDim = 2, 3 for n < 10, else
+ * 1 + floor(sqrt(n - 1)); normalized coordinates twice the offset into the
+ * tile).  Each is mapped to its ptex face (ptexBase[f] for a quad,
+ * ptexBase[f] + k for sub-face k; faceSizes and ptexBase: numFaces device
+ * ints per base face) and located as osdhip_patch_map_find does; coords[i]
+ * carries the ptex (s, t).  On a patch table built with Bfr's approximation
+ * options (osdfar_patch_options), osdhip_eval_patches at these coords
+ * gives Bfr::Surface::Evaluate's P, Du, Dv (derivatives with respect to
+ * the quad's (u, v) or the sub-face's normalized coordinates, as Bfr's). */
+int osdhip_patch_map_find_faces(const osdhip_patch_map *map, int n, const int *faces,
+                                const double *u, const double *v, const int *faceSizes,
+                                const int *ptexBase, int numFaces, osdhip_patch_coord *coords,
+                                void *stream);
 
 /* Bfr::Tessellation (bfr/tessellation.h:51-240) of quad parameterizations
  * at a uniform rate, on the device, for numFaces ptex faces at once: each

@@ -963,6 +963,35 @@ int osdhip_patch_map_find(const osdhip_patch_map *map, int n, const int *ptexFac
     return last_error();
 }
 
+int osdhip_patch_map_find_faces(const osdhip_patch_map *map, int n, const int *faces,
+                                const double *u, const double *v, const int *faceSizes,
+                                const int *ptexBase, int numFaces, osdhip_patch_coord *coords,
+                                void *stream)
+{
+    if (!map || n < 0 || numFaces < 0) return OSDHIP_ERR_ARGS;
+    if (n == 0) return OSDHIP_OK;
+    if (!faces || !u || !v || !coords || !faceSizes || !ptexBase) return OSDHIP_ERR_ARGS;
+    osdhip::FaceQueryArgs a;
+    a.map.nodes = map->nodes;
+    a.map.handles = map->handles;
+    a.map.minFace = map->minFace;
+    a.map.maxFace = map->maxFace;
+    a.map.maxDepth = map->maxDepth;
+    a.map.n = n;
+    a.map.ptex = nullptr;
+    a.map.u = u;
+    a.map.v = v;
+    a.map.coords = coords;
+    a.faces = faces;
+    a.faceSizes = faceSizes;
+    a.ptexBase = ptexBase;
+    a.numFaces = numFaces;
+    const int grid = (int)(((long long)n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(osdhip::k_patch_map_find_faces, dim3(grid), dim3(kBlock), 0,
+                       (hipStream_t)stream, a);
+    return last_error();
+}
+
 int osdhip_tess_uniform_quads(int numFaces, const int *ptexFaces, int rate, float *uv,
                               const osdhip_patch_map *map, osdhip_patch_coord *coords,
                               void *stream)

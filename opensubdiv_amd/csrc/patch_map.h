@@ -62,6 +62,45 @@ __global__ __launch_bounds__(kBlock) void k_patch_map_find(PatchMapArgs a)
     a.coords[i] = patch_map_lookup(a, a.ptex[i], a.u[i], a.v[i]);
 }
 
+// Queries given per base face in Bfr's parameterization of the face
+// (bfr/parameterization.cpp:45-62, 140-175): a quad's own unit square; an
+// n-gon's n quad sub-faces, sub-face k on the tile (k % uDim, k / uDim) of
+// a uDim-wide grid, its normalized coordinates twice the offset into the
+// tile (offsets above 0.75 belong to the next tile).  Sub-face k of base
+// face f is ptex face ptexBase[f] + k, with the same normalized (s, t).
+struct FaceQueryArgs {
+    PatchMapArgs map;        // map.ptex unused
+    const int *faces;
+    const int *faceSizes, *ptexBase;
+    int numFaces;
+};
+
+__global__ __launch_bounds__(kBlock) void k_patch_map_find_faces(FaceQueryArgs a)
+{
+    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (i >= a.map.n) return;
+    const int f = a.faces[i];
+    double u = a.map.u[i], v = a.map.v[i];
+    int ptex = -1;
+    if (f >= 0 && f < a.numFaces) {
+        const int n = a.faceSizes[f];
+        if (n == 4) {
+            ptex = a.ptexBase[f];
+        } else if (n >= 3) {
+            const int uDim = n < 10 ? 2 + (n > 4) : 1 + (int)sqrtf((float)(n - 1));
+            int uTile = (int)u, vTile = (int)v;
+            double uFrac = u - (double)uTile, vFrac = v - (double)vTile;
+            if (uFrac > 0.75) { ++uTile; uFrac -= 1.0; }
+            if (vFrac > 0.75) { ++vTile; vFrac -= 1.0; }
+            const int k = uDim * vTile + uTile;
+            if (uTile >= 0 && uTile < uDim && k >= 0 && k < n) ptex = a.ptexBase[f] + k;
+            u = 2.0 * uFrac;
+            v = 2.0 * vFrac;
+        }
+    }
+    a.map.coords[i] = patch_map_lookup(a.map, ptex, u, v);
+}
+
 // One (u, v) of Bfr::Tessellation's uniform pattern of a quad at `rate`
 // (bfr/tessellation.cpp:158-174, 929-990): the boundary counter-clockwise
 // from (0,0), 4 * rate points, then the interior rings from the outside in,

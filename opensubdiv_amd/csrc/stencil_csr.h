@@ -32,7 +32,7 @@ struct StencilArgs {
     int nblocks;            // logical blocks that hold stencils
 };
 
-constexpr int kGatherGroup = 4;
+constexpr int kGatherGroup = 8;
 
 // LDS position of slice entry e (one padding slot after every 16 entries).
 __device__ __forceinline__ int csr_lds_pos(int e) { return e + (e >> 4); }

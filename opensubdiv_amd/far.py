@@ -40,6 +40,7 @@ def load_far():
         "osdfar_stencil_table_copy": ([vp, vp, vp, vp, vp, i, vp, vp], i),
         "osdfar_stencil_table_destroy": ([vp], None),
         "osdfar_patch_table_create": ([vp, i], vp),
+        "osdfar_patch_table_create_ex": ([vp, i, vp], vp),
         "osdfar_patch_table_destroy": ([vp], None),
         "osdfar_patch_table_info": ([vp, vp, vp, vp, vp, vp, vp, vp], i),
         "osdfar_patch_table_copy": ([vp, vp, vp, vp], i),
@@ -227,8 +228,18 @@ class Topology:
         assert load_far().osdfar_vertex_locations(self.h, level, _p(p), _p(u), _p(v)) == 0
         return p, u, v
 
-    def patch_table(self, iso_level):
-        return PatchTable(self, iso_level)
+    def patch_table(self, iso_level, secondary_level=None, legacy_sharp_corners=True,
+                    share_endcap_points=True):
+        """Feature-adaptive patch table (osdfar_patch_table_create_ex)."""
+        return PatchTable(self, iso_level, secondary_level, legacy_sharp_corners,
+                          share_endcap_points)
+
+    def bfr_patch_table(self, approx_smooth=2, approx_sharp=6):
+        """The patch table Bfr's per-face surfaces equal (Bfr::SurfaceFactory
+        approximation levels: smooth extraordinary vertices to approx_smooth,
+        sharp features to approx_sharp; no legacy sharp corners, unshared
+        end-cap points)."""
+        return PatchTable(self, approx_sharp, approx_smooth, False, False)
 
     def __del__(self):
         if getattr(self, "h", None) and _far is not None:
@@ -241,11 +252,14 @@ class PatchTable:
     patch-point stencils and patch map.  Vertex buffer layout:
     [control | refined | end-cap points]."""
 
-    def __init__(self, topo, iso_level):
+    def __init__(self, topo, iso_level, secondary_level=None, legacy_sharp_corners=True,
+                 share_endcap_points=True):
         from .osd import PATCH_ARRAY_DTYPE, PATCH_PARAM_DTYPE
         L = load_far()
         self._topo = topo
-        self.h = L.osdfar_patch_table_create(topo.h, iso_level)
+        opts = (C.c_int * 3)(15 if secondary_level is None else int(secondary_level),
+                             int(bool(legacy_sharp_corners)), int(bool(share_endcap_points)))
+        self.h = L.osdfar_patch_table_create_ex(topo.h, iso_level, C.cast(opts, C.c_void_p))
         if not self.h:
             raise ValueError("mesh outside the patch compiler's scope")
         v = [C.c_int() for _ in range(7)]

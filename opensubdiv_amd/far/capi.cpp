@@ -213,13 +213,26 @@ void osdfar_stencil_table_destroy(osdfar_stencil_table *t) { delete t; }
 
 osdfar_patch_table *osdfar_patch_table_create(osdfar_topology *topo, int isoLevel)
 {
+    return osdfar_patch_table_create_ex(topo, isoLevel, nullptr);
+}
+
+osdfar_patch_table *osdfar_patch_table_create_ex(osdfar_topology *topo, int isoLevel,
+                                                 const osdfar_patch_options *options)
+{
     try {
         if (!topo || isoLevel < 0 || isoLevel > 10) return nullptr;
         if (topo->refiner.maxLevel() > isoLevel) return nullptr;
+        PatchOptions opt;
+        if (options) {
+            if (options->secondaryLevel < 0 || options->secondaryLevel > 15) return nullptr;
+            opt.secondaryLevel = options->secondaryLevel;
+            opt.legacySharpCorners = options->legacySharpCorners != 0;
+            opt.shareEndCapPoints = options->shareEndCapPoints != 0;
+        }
         std::unique_ptr<osdfar_patch_table> t(new (std::nothrow) osdfar_patch_table);
         const bool cached = (int)topo->levels.size() == topo->refiner.maxLevel() + 1 &&
                             topo->refiner.maxLevel() == isoLevel;
-        if (!t || !t->table.build(topo->refiner, isoLevel, cached ? &topo->levels : nullptr))
+        if (!t || !t->table.build(topo->refiner, isoLevel, cached ? &topo->levels : nullptr, opt))
             return nullptr;
         if (topo->fvarValues > 0 &&
             !t->table.buildFaceVarying(topo->refiner, topo->fvarValues, topo->fvarIndices.data()))

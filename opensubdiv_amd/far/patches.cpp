@@ -384,7 +384,10 @@ bool gregoryPoints(const Level &L, int f, Row out[20])
 // Far's feature test for refining face f of a level (scope: manifold,
 // interior; far/topologyRefiner.cpp:627-690): non-quad, a base-level corner
 // shared with a non-quad face, an extraordinary corner, or a corner with
-// vertex sharpness or an incident sharp edge.
+// vertex sharpness or an incident sharp edge.  `reduced`: the feature set
+// of levels at or past the secondary level (FeatureMask::ReduceFeatures,
+// far/topologyRefiner.cpp:367-380): extraordinary corners (interior or on a
+// boundary) no longer select the face, sharp features still do.
 
 // A corner a B-spline patch reproduces (Far's regular tags, far/patchBuilder.cpp:
 // 581-700 for this scope): interior with 4 faces; on a boundary with 2 faces
@@ -400,7 +403,7 @@ bool regularCorner(const Level &L, int v)
     return nf == 1 && ne == 2 && L.vsharp[v] >= kSharpInfinite;
 }
 
-bool faceHasFeatures(const Level &L, int f, bool baseLevel)
+bool faceHasFeatures(const Level &L, int f, bool baseLevel, bool reduced)
 {
     if (L.faceSize(f) != 4) return true;
     for (int k = L.fvOff[f]; k < L.fvOff[f + 1]; ++k) {
@@ -409,7 +412,7 @@ bool faceHasFeatures(const Level &L, int f, bool baseLevel)
         if (baseLevel)
             for (int i = L.vfOff[v]; i < L.vfOff[v + 1]; ++i)
                 if (L.faceSize(L.vf[i]) != 4) return true;
-        if (!regularCorner(L, v)) return true;
+        if (!reduced && !regularCorner(L, v)) return true;
         // semi-sharp or interior infinitely sharp vertex or edge (Far selects
         // inf-sharp creases and corners when inf-sharp patches are off,
         // far/topologyRefiner.cpp:340-360); the infinitely sharp boundary
@@ -425,14 +428,15 @@ bool faceHasFeatures(const Level &L, int f, bool baseLevel)
 }
 
 // Whether a leaf face needs an end cap.  A smooth single-face boundary
-// corner is isolated like an irregular point but then patched as a sharp
-// corner, as Far does by default (generateLegacySharpCornerPatches,
-// far/patchBuilder.cpp:676-686).
-bool isExtraordinary(const Level &L, int f)
+// corner is isolated like an irregular point but then, with legacy
+// sharp-corner patches (Far's default, generateLegacySharpCornerPatches,
+// far/patchBuilder.cpp:676-686), patched as a sharp corner; without them it
+// gets an end cap too.
+bool isExtraordinary(const Level &L, int f, bool legacySharpCorners)
 {
     for (int k = L.fvOff[f]; k < L.fvOff[f + 1]; ++k) {
         const int v = L.fv[k];
-        if (L.vfOff[v + 1] - L.vfOff[v] == 1 && L.valence(v) == 2) continue;
+        if (legacySharpCorners && L.vfOff[v + 1] - L.vfOff[v] == 1 && L.valence(v) == 2) continue;
         if (!regularCorner(L, v)) return true;
     }
     return false;
@@ -494,7 +498,8 @@ bool regularPoints(const Level &L, int f, int mask, int pts[16])
 
 }  // namespace
 
-bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelStencils> *levels)
+bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelStencils> *levels,
+                       const PatchOptions &opt)
 {
     {
         const Level &L0 = refiner.level(0);
@@ -531,11 +536,14 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
         selected.assign(L.nf, 0);
         for (int f = 0; f < L.nf; ++f) {
             if (!complete[f]) continue;
-            if (d < maxL && faceHasFeatures(L, f, d == 0)) {
+            // levels at or past the secondary level use the reduced
+            // feature set (far/topologyRefiner.cpp:412-470)
+            const bool reduced = d >= std::min(opt.secondaryLevel, isoLevel);
+            if (d < maxL && faceHasFeatures(L, f, d == 0, reduced)) {
                 selected[f] = 1;
                 for (int j = 0; j < L.faceSize(f); ++j) next[L.fvOff[f] + j] = 1;
             } else {
-                leaves.push_back({ d, f, !isExtraordinary(L, f), 0 });
+                leaves.push_back({ d, f, !isExtraordinary(L, f, opt.legacySharpCorners), 0 });
                 maxDepth = std::max(maxDepth, d);
             }
         }
@@ -610,6 +618,7 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
                         slot = &ep[2 * (size_t)e + end];
                     }
                 }
+                if (!opt.shareEndCapPoints) slot = nullptr;
                 int id;
                 if (slot && *slot >= 0) {
                     id = *slot;
@@ -686,17 +695,26 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
                 S.off.push_back((int64_t)S.idx.size());
             }
     {
-        const LevelStencils &src = LS[maxL];
+        // an end cap's rows are over the vertices of its own level (the last
+        // one, or a shallower one under a secondary level); level 0 is the
+        // control vertices themselves
         std::vector<int> pos(numControl, -1), list;
         std::vector<double> val;
+        auto add = [&](int c, double w) {
+            if (pos[c] < 0) { pos[c] = (int)list.size(); list.push_back(c); val.push_back(0.0); }
+            val[pos[c]] += w;
+        };
         for (int rowIndex : newRows) {
             const Row &row = greg[(size_t)rowIndex];
-            for (auto &x : row.e)
-                for (int64_t k = src.off[x.first]; k < src.off[x.first + 1]; ++k) {
-                    const int c = src.idx[k];
-                    if (pos[c] < 0) { pos[c] = (int)list.size(); list.push_back(c); val.push_back(0.0); }
-                    val[pos[c]] += x.second * src.w[k];
-                }
+            const int lvl = gregLeaves[(size_t)rowIndex / 20]->level;
+            if (lvl == 0) {
+                for (auto &x : row.e) add(x.first, x.second);
+            } else {
+                const LevelStencils &src = LS[lvl];
+                for (auto &x : row.e)
+                    for (int64_t k = src.off[x.first]; k < src.off[x.first + 1]; ++k)
+                        add(src.idx[k], x.second * src.w[k]);
+            }
             std::vector<int> ord(list.size());
             for (size_t k = 0; k < ord.size(); ++k) ord[k] = (int)k;
             std::sort(ord.begin(), ord.end(), [&](int a, int b) { return list[a] < list[b]; });

@@ -37,6 +37,18 @@ struct PatchHandleRec {
     int arrayIndex, patchIndex, vertIndex;
 };
 
+// Options of Far::PatchTableFactory / RefineAdaptive the compiler follows.
+struct PatchOptions {
+    // smooth extraordinary vertices are isolated to this level only (sharp
+    // features to isoLevel): AdaptiveOptions::secondaryLevel
+    int secondaryLevel = 15;
+    // smooth single-face boundary corners patched as sharp corners
+    // (generateLegacySharpCornerPatches, Far's default) or given end caps
+    bool legacySharpCorners = true;
+    // end-cap points shared between neighbouring caps (shareEndCapPatchPoints)
+    bool shareEndCapPoints = true;
+};
+
 class PatchTable {
 public:
     // Returns false for meshes outside the compiler's scope (non-manifold
@@ -44,7 +56,8 @@ public:
     // holds the vertex-mode level stencils of the refiner up to at least
     // isoLevel (computed here otherwise).
     bool build(Refiner &refiner, int isoLevel,
-               const std::vector<LevelStencils> *levels = nullptr);
+               const std::vector<LevelStencils> *levels = nullptr,
+               const PatchOptions &opt = PatchOptions());
 
     int numControl = 0, numRefined = 0, numLocal = 0, maxDepth = 0;
     std::vector<PatchArrayRec> arrays;      // [REGULAR (16), GREGORY_BASIS (20)]

@@ -306,6 +306,15 @@ class HipPatchMap:
                                                      _ptr(v), _ptr(patchCoords),
                                                      _stream(deviceContext)) == _lib.OK
 
+    def FindFaceCoords(self, n, faces, u, v, faceSizes, ptexBase, patchCoords,
+                       deviceContext=None):
+        """Bfr-style queries: device int32 base faces, float64 (u, v) in
+        Bfr's parameterization of each face, int32 face sizes and first
+        ptex index per base face; PatchCoord output; returns bool."""
+        return _lib.load_hip().osdhip_patch_map_find_faces(
+            self.h, int(n), _ptr(faces), _ptr(u), _ptr(v), _ptr(faceSizes), _ptr(ptexBase),
+            int(len(faceSizes)), _ptr(patchCoords), _stream(deviceContext)) == _lib.OK
+
     def close(self):
         if getattr(self, "h", None) and _lib._hip is not None:
             _lib._hip.osdhip_patch_map_destroy(self.h)

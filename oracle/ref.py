@@ -43,6 +43,8 @@ def lib():
             "ref_stencils_copy": ([vp, vp, vp, vp, i, vp, vp], None),
             "ref_stencils_destroy": ([vp], None),
             "ref_patches_create": ([vp, i, i, i, i, i, i], vp),
+            "ref_patches_create_ex": ([vp, i, i, i, i, i, i], vp),
+            "ref_bfr_evaluate": ([vp, i, i, vp, i, vp, vp, vp, vp, vp, vp], i),
             "ref_patches_info": ([vp, vp, vp, vp, vp, vp, vp], None),
             "ref_patches_stencils": ([vp], vp),
             "ref_patches_copy": ([vp, vp, vp, vp], None),
@@ -160,14 +162,20 @@ class Patches:
 
     def __init__(self, topo, iso_level, endcap=3, inf_sharp=False, single_crease=False,
                  legacy_sharp_corner=True, double=False, uniform=False, triangulate=False,
-                 fvar=False, fvar_legacy_linear=True):
+                 fvar=False, fvar_legacy_linear=True, secondary_level=None,
+                 share_endcap_points=True):
         """uniform=True: RefineUniform(iso_level) and the uniform linear
         table (QUADS, or TRIANGLES for Loop meshes / triangulate=True);
         no patch map then.  fvar=True: the topology's face-varying channel
         too (Gregory end caps), in self.fvar: its stencil table (base values
         to refined and local values), flattened arrays, indices and params."""
         L = lib()
-        if fvar:
+        if secondary_level is not None:
+            # Gregory end caps; secondary isolation level
+            self.h = L.ref_patches_create_ex(topo.h, iso_level, int(secondary_level),
+                                             int(inf_sharp), int(legacy_sharp_corner),
+                                             int(share_endcap_points), int(double))
+        elif fvar:
             self.h = L.ref_patches_create_fvar(topo.h, iso_level, int(fvar_legacy_linear))
             if not self.h:
                 raise RuntimeError("face-varying patch table not created")
@@ -229,6 +237,21 @@ class Patches:
         if getattr(self, "h", None) and _L is not None:
             _L.ref_patches_destroy(self.h)
             self.h = None
+
+
+def bfr_evaluate(topo, positions, faces, u, v, approx_smooth=2, approx_sharp=6):
+    """Bfr::Surface<double> of each query's base face at (u, v) in the face's
+    parameterization: (P, Du, Dv) as (n, 3) float64 arrays and the number of
+    queries on faces with a valid surface."""
+    pos = np.ascontiguousarray(positions, np.float64)
+    faces = np.ascontiguousarray(faces, np.int32)
+    u = np.ascontiguousarray(u, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    n = len(faces)
+    out = [np.full((n, 3), np.nan) for _ in range(3)]
+    done = lib().ref_bfr_evaluate(topo.h, int(approx_smooth), int(approx_sharp), _p(pos), n,
+                                  _p(faces), _p(u), _p(v), *[_p(o) for o in out])
+    return out[0], out[1], out[2], done
 
 
 def limit_points_uniform(topo, level, coarse_xyz):

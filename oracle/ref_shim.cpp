@@ -14,6 +14,7 @@
 //   far/patchMap.h:58-74             PatchMap::FindPatch
 //   far/patchTable.h:429-445         PatchTable::EvaluateBasis<double>
 //   osd/cpuEvaluator.h, osd/ompEvaluator.h   raw EvalStencils / EvalPatches
+//   bfr/refinerSurfaceFactory.h, bfr/surface.h  Bfr::Surface per-face evaluation
 #include <opensubdiv/far/topologyDescriptor.h>
 #include <opensubdiv/far/topologyRefinerFactory.h>
 #include <opensubdiv/far/stencilTableFactory.h>
@@ -24,6 +25,8 @@
 #include <opensubdiv/osd/cpuPatchTable.h>
 #include <opensubdiv/osd/ompEvaluator.h>
 #include <opensubdiv/bfr/tessellation.h>
+#include <opensubdiv/bfr/refinerSurfaceFactory.h>
+#include <opensubdiv/bfr/surface.h>
 
 #include <cstring>
 #include <vector>
@@ -303,6 +306,98 @@ void *ref_patches_create(void *topo, int isoLevel, int endcap, int infSharp,
     p->flat = new Osd::CpuPatchTable(p->table);
     p->map = new Far::PatchMap(*p->table);
     return p;
+}
+
+// Feature-adaptive patch table with Far's secondary isolation level
+// (TopologyRefiner::AdaptiveOptions::secondaryLevel, far/topologyRefiner.h:
+// 156-182): features other than smooth extraordinary vertices are isolated
+// to isoLevel, smooth extraordinary vertices only to secondaryLevel.
+// Gregory end caps, end-cap points shared or not, legacy sharp-corner
+// patches on or off -- the set-up regression/bfr_evaluate/
+// farPatchEvaluator.cpp:52-84 compares Bfr with (useInfSharpPatch as
+// given).
+void *ref_patches_create_ex(void *topo, int isoLevel, int secondaryLevel, int infSharp,
+                            int legacySharpCorner, int shareEndCapPoints, int useDouble)
+{
+    Patches *p = new Patches;
+    p->refiner = ((Topo *)topo)->makeRefiner();
+    if (!p->refiner) { delete p; return nullptr; }
+    p->nCoarse = p->refiner->GetLevel(0).GetNumVertices();
+    Far::PatchTableFactory::Options po(isoLevel);
+    po.SetEndCapType(Far::PatchTableFactory::Options::ENDCAP_GREGORY_BASIS);
+    po.useInfSharpPatch = infSharp != 0;
+    po.generateLegacySharpCornerPatches = legacySharpCorner != 0;
+    po.shareEndCapPatchPoints = shareEndCapPoints != 0;
+    po.generateVaryingTables = false;
+    if (useDouble) po.SetPatchPrecision<double>();
+    Far::TopologyRefiner::AdaptiveOptions ro = po.GetRefineAdaptiveOptions();
+    ro.SetIsolationLevel(isoLevel);
+    ro.SetSecondaryLevel(secondaryLevel);
+    p->refiner->RefineAdaptive(ro);
+    p->table = Far::PatchTableFactory::Create(*p->refiner, po);
+    p->st.isDouble = useDouble != 0;
+    auto build = [&](auto tag) {
+        typedef decltype(tag) REAL;
+        typename Far::StencilTableFactoryReal<REAL>::Options so;
+        so.generateOffsets = true;
+        so.generateIntermediateLevels = true;
+        auto *base = Far::StencilTableFactoryReal<REAL>::Create(*p->refiner, so);
+        auto const *local = p->table->template GetLocalPointStencilTable<REAL>();
+        auto const *all = base;
+        if (local) {
+            all = Far::StencilTableFactoryReal<REAL>::AppendLocalPointStencilTable(*p->refiner, base, local);
+            delete base;
+        }
+        p->st.fromTable(all);
+        delete all;
+    };
+    if (useDouble) build(double()); else build(float());
+    p->flat = new Osd::CpuPatchTable(p->table);
+    p->map = new Far::PatchMap(*p->table);
+    return p;
+}
+
+// Bfr::Surface<double> of base face faces[i] evaluated at (u[i], v[i]) in
+// the face's parameterization (bfr/surface.h:227-266; the set-up of
+// tutorials/bfr/tutorial_1_1): RefinerSurfaceFactory over the unrefined
+// topology with approxLevelSmooth / approxLevelSharp, InitVertexSurface,
+// PreparePatchPoints from the 3-float control points, then Evaluate
+// (P, Du, Dv; 3 doubles each per query).  Returns the number of queries
+// whose face had a valid surface (others are left untouched).
+int ref_bfr_evaluate(void *topo, int approxSmooth, int approxSharp, const double *pos, int n,
+                     const int *faces, const double *u, const double *v, double *P, double *Du,
+                     double *Dv)
+{
+    Far::TopologyRefiner *r = ((Topo *)topo)->makeRefiner();
+    if (!r) return -1;
+    int done = 0;
+    {
+        Bfr::SurfaceFactory::Options so;
+        so.SetApproxLevelSmooth(approxSmooth);
+        so.SetApproxLevelSharp(approxSharp);
+        Bfr::RefinerSurfaceFactory<> factory(*r, so);
+        Bfr::Surface<double> surface;
+        std::vector<double> pts;
+        int current = -1;
+        bool valid = false;
+        for (int i = 0; i < n; ++i) {
+            if (faces[i] != current) {
+                current = faces[i];
+                valid = factory.InitVertexSurface(current, &surface);
+                if (valid) {
+                    pts.resize((size_t)surface.GetNumPatchPoints() * 3);
+                    surface.PreparePatchPoints(pos, 3, pts.data(), 3);
+                }
+            }
+            if (!valid) continue;
+            const double uv[2] = { u[i], v[i] };
+            surface.Evaluate(uv, pts.data(), 3, P + 3 * (size_t)i, Du + 3 * (size_t)i,
+                             Dv + 3 * (size_t)i);
+            ++done;
+        }
+    }
+    delete r;
+    return done;
 }
 
 // Uniform linear patch table (far/patchTableFactory.cpp:745-850,

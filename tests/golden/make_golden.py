@@ -31,6 +31,12 @@ reference's outputs; nothing of the reference's source goes into them.
                         control points, PatchMap coords, CpuEvaluator
                         EvalPatches outputs (P/du/dv and the 2nd-derivative
                         arity) and Far EvaluateBasis<double> weights
+  bfr_surfaces.npz      Bfr::Surface<double>::Evaluate (approximation levels
+                        2 smooth / 6 sharp) of three meshes -- a creased
+                        cube-sphere, a pentagonal prism (n-gon faces, so
+                        Bfr's quad sub-face parameterization), an open
+                        creased grid -- at seeded (face, u, v) queries in
+                        Bfr's parameterization: meshes, queries, P/Du/Dv
   cubesphere_fvar.npz   a face-varying UV channel (one chart per cube face,
                         seams along the cube edges) on the creased
                         cube-sphere n=4, iso 3, two ways: Far's defaults
@@ -213,6 +219,43 @@ def cubesphere_fvar():
     save("cubesphere_fvar.npz", **out)
 
 
+def bfr_surfaces():
+    import math
+    sys.path.insert(0, os.path.dirname(HERE))
+    import test_far_compiler as T
+    out = {}
+    cases = (("sphere", meshes.cube_sphere(4, seed=5, crease_fraction=0.1), 1),
+             ("prism", T.POLY_MESHES["pentagonal-prism"](), 1),
+             ("grid", T.POLY_MESHES["open-grid-creased"](), 2))
+    for k, (name, m, boundary) in enumerate(cases):
+        sizes = getattr(m, "face_sizes", None)
+        sizes = np.full(m.nfaces, 4, np.int32) if sizes is None else np.asarray(sizes, np.int32)
+        rng = np.random.default_rng(40 + k)
+        n = 1500
+        faces = rng.integers(0, m.nfaces, n).astype(np.int32)
+        s, t = rng.random(n), rng.random(n)
+        u, v = s.copy(), t.copy()
+        for i in range(n):                                 # n-gon: a random sub-face's tile
+            fs = int(sizes[faces[i]])
+            if fs != 4:
+                sub = int(rng.integers(fs))
+                udim = (2 + (fs > 4)) if fs < 10 else 1 + int(math.sqrt(fs - 1))
+                u[i] = sub % udim + 0.5 * s[i] * 0.999
+                v[i] = sub // udim + 0.5 * t[i] * 0.999
+        P, Du, Dv, done = ref.bfr_evaluate(ref.Topology(m, boundary=boundary), m.positions,
+                                           faces, u, v)
+        assert done == n
+        out.update({f"{name}_face_sizes": sizes,
+                    f"{name}_face_verts": np.asarray(m.face_verts, np.int32).reshape(-1),
+                    f"{name}_crease_pairs": np.asarray(m.crease_pairs, np.int32).reshape(-1, 2),
+                    f"{name}_crease_weights": np.asarray(m.crease_weights, np.float32),
+                    f"{name}_positions": np.asarray(m.positions, np.float64),
+                    f"{name}_boundary": np.int32(boundary),
+                    f"{name}_faces": faces, f"{name}_u": u, f"{name}_v": v,
+                    f"{name}_P": P, f"{name}_Du": Du, f"{name}_Dv": Dv})
+    save("bfr_surfaces.npz", **out)
+
+
 def tri_sphere(n, seed):
     """The cube-sphere's quads split along one diagonal (a closed Loop
     mesh, valences 4 to 6), 5 % of its edges creased."""
@@ -292,7 +335,7 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref/libosdref.so not built (make -C oracle)")
     # names on the command line write only those fixtures
     makers = [cube_l3, cubesphere_l2, cubesphere_limit, cubesphere_patches, loop_patches,
-              cubesphere_fvar]
+              cubesphere_fvar, bfr_surfaces]
     wanted = set(sys.argv[1:])
     for fn in makers:
         if not wanted or fn.__name__ in wanted:

@@ -740,3 +740,123 @@ def test_fvar_channel_errors():
     assert pt.fvar is None
     with pytest.raises(ValueError):
         pt.fvar_stencils()
+
+
+# ------------------------------------------------ secondary level, Bfr options
+
+SECONDARY_CASES = {
+    # name: (mesh, isoLevel, secondaryLevel, boundary rule, legacy sharp corners)
+    "sphere-smooth": (lambda: meshes.cube_sphere(3, seed=5, crease_fraction=0.0), 6, 2, 1, True),
+    "sphere-creased": (lambda: meshes.cube_sphere(4, seed=5, crease_fraction=0.1), 6, 2, 1, True),
+    "sphere-l4-s1": (lambda: meshes.cube_sphere(4, seed=7, crease_fraction=0.2), 4, 1, 1, True),
+    "sphere-bfr": (lambda: meshes.cube_sphere(4, seed=5, crease_fraction=0.1), 6, 2, 1, False),
+    "prism": (lambda: POLY_MESHES["pentagonal-prism"](), 5, 2, 1, False),
+    "triangles": (lambda: POLY_MESHES["cube-with-triangles"](), 4, 1, 1, False),
+    "grid-creased": (lambda: POLY_MESHES["open-grid-creased"](), 5, 2, 1, False),
+    "disc5": (lambda: disc(5), 5, 2, 1, False),
+    "fan3-corner": (lambda: PolyMesh([[0, 1, 2, 3], [0, 3, 4, 5], [0, 5, 6, 7]]), 4, 2, 2, False),
+}
+
+
+@needs_ref
+@pytest.mark.parametrize("name", list(SECONDARY_CASES))
+def test_secondary_level_and_bfr_options_match_far_live(name):
+    """Adaptive tables with Far's secondary isolation level (smooth
+    extraordinary vertices isolated less deeply than sharp features, their
+    faces capped at the shallower level), with or without legacy sharp-corner
+    patches and shared end-cap points: same arrays and PatchParams as live
+    Far, and the same limit surface to double rounding."""
+    mk, iso, sec, boundary, legacy = SECONDARY_CASES[name]
+    m = mk()
+    pt = far.Topology(m, boundary=boundary).patch_table(iso, sec, legacy, legacy)
+    P = ref.Patches(ref.Topology(m, boundary=boundary), iso, secondary_level=sec,
+                    legacy_sharp_corner=legacy, share_endcap_points=legacy, double=True)
+    assert [tuple(a) for a in pt.arrays] == [tuple(a) for a in P.arrays]
+    assert pt.params.tobytes() == P.params.tobytes()
+    depths = pt.params["field1"][pt.params["field1"] >> 5 & 1 == 0] & 0xF
+    assert (depths < iso).any() or sec >= iso          # end caps above the last level
+    sizes = getattr(m, "face_sizes", None)
+    nptex = m.nfaces if sizes is None else int(sum(1 if n == 4 else n for n in sizes))
+    f, u, v = meshes.random_patch_locations(nptex, 2000, seed=3)
+    pos = np.asarray(m.positions, np.float64)
+
+    def surface(arrays, indices, params, st, coords):
+        src = np.ascontiguousarray(np.concatenate([pos, oracle.apply_stencils(
+            pos, st["sizes"], st["offsets"], st["indices"], st["weights"], np.float64)]))
+        o = [np.zeros((len(coords), 3)) for _ in range(3)]
+        assert oracle.eval_patches_f64(src, (0, 3, 3), o, [(0, 3, 3)] * 3, coords, arrays,
+                                       indices, params)
+        return o
+
+    mine = surface(pt.arrays, pt.indices, pt.params, pt.point_stencils(double=True),
+                   pt.find(f, u, v))
+    h = P.find(f, u, v)
+    c = np.zeros(len(f), PATCH_COORD_DTYPE)
+    c["arrayIndex"], c["patchIndex"], c["vertIndex"] = h[:, 0], h[:, 1], h[:, 2]
+    c["s"], c["t"] = u, v
+    theirs = surface(P.arrays, P.indices, P.params, P.stencils, c)
+    for a, b in zip(mine, theirs):
+        assert np.abs(a - b).max() < 1e-12
+
+
+class FixtureMesh:
+    """A mesh stored in a fixture (flat face vertices + face sizes)."""
+
+    def __init__(self, g, name):
+        self.face_sizes = g[name + "_face_sizes"]
+        self.face_verts = g[name + "_face_verts"]
+        self.nfaces = len(self.face_sizes)
+        self.positions = g[name + "_positions"]
+        self.nverts = len(self.positions)
+        self.crease_pairs = g[name + "_crease_pairs"]
+        self.crease_weights = g[name + "_crease_weights"]
+        self.corner_verts = np.zeros(0, np.int32)
+        self.corner_weights = np.zeros(0, np.float32)
+
+
+def bfr_to_ptex(sizes, faces, u, v):
+    """(base face, u, v) in Bfr's parameterization -> (ptex face, s, t): a
+    quad's own square; an n-gon's sub-face k on tile (k % uDim, k // uDim),
+    normalized coordinates twice the offset into the tile
+    (bfr/parameterization.cpp:45-62, 140-175)."""
+    sizes = np.asarray(sizes)
+    base = np.concatenate([[0], np.cumsum(np.where(sizes == 4, 1, sizes))[:-1]])
+    ptex, s, t = base[faces].copy(), np.array(u, np.float64), np.array(v, np.float64)
+    for i in np.nonzero(sizes[faces] != 4)[0]:
+        n = int(sizes[faces[i]])
+        udim = (2 + (n > 4)) if n < 10 else 1 + int(np.sqrt(n - 1))
+        ut, vt = int(u[i]), int(v[i])
+        uf, vf = u[i] - ut, v[i] - vt
+        if uf > 0.75:
+            ut, uf = ut + 1, uf - 1.0
+        if vf > 0.75:
+            vt, vf = vt + 1, vf - 1.0
+        ptex[i] += udim * vt + ut
+        s[i], t[i] = 2.0 * uf, 2.0 * vf
+    return ptex.astype(np.int32), s, t
+
+
+@pytest.mark.parametrize("name", ["sphere", "prism", "grid"])
+def test_bfr_patch_tables_reproduce_bfr_surfaces(name):
+    """Bfr::Surface::Evaluate (the reference's own, fixture written by it,
+    approximation levels 2 / 6) against the compiler's Bfr-option table
+    (secondary level 2, iso 6, no legacy sharp corners, unshared end caps)
+    evaluated at the same faces through the patch map: quads, n-gon
+    sub-faces and an open creased grid agree to the reference's own
+    Far-vs-Bfr agreement (Bfr's patch data is float)."""
+    g = golden("bfr_surfaces.npz")
+    m = FixtureMesh(g, name)
+    pt = far.Topology(m, boundary=int(g[name + "_boundary"])).bfr_patch_table()
+    ptex, s, t = bfr_to_ptex(m.face_sizes, g[name + "_faces"], g[name + "_u"], g[name + "_v"])
+    coords = pt.find(ptex, s, t)
+    assert (coords["arrayIndex"] >= 0).all()
+    st = pt.point_stencils(double=True)
+    pos = np.asarray(m.positions, np.float64)
+    src = np.ascontiguousarray(np.concatenate([pos, oracle.apply_stencils(
+        pos, st["sizes"], st["offsets"], st["indices"], st["weights"], np.float64)]))
+    o = [np.zeros((len(coords), 3)) for _ in range(3)]
+    assert oracle.eval_patches_f64(src, (0, 3, 3), o, [(0, 3, 3)] * 3, coords, pt.arrays,
+                                   pt.indices, pt.params)
+    for a, k in zip(o, ("P", "Du", "Dv")):
+        b = g[name + "_" + k]
+        assert np.abs(a - b).max() < 1e-6 * max(1.0, np.abs(b).max()), k

@@ -615,3 +615,52 @@ def test_boundary_patch_tables_bit_exact(hip, mesh):
                         dev(pt.arrays), dev(pt.indices), dev(pt.params)) == 0
     for k in range(3):
         np.testing.assert_array_equal(host(outs[k], shape=(n, 3)), want[k])
+
+
+@pytest.mark.parametrize("name", ["sphere", "prism", "grid"])
+def test_bfr_face_queries_on_the_device_match_bfr_surfaces(hip, name):
+    """Bfr-style per-face evaluation on the device: the compiler's table
+    with Bfr's approximation options, its patch points refined by
+    EvalStencils, the queries (base face, u, v in Bfr's parameterization of
+    the face -- quads and n-gon sub-faces) located by
+    osdhip_patch_map_find_faces, then EvalPatches -- against the reference's
+    Bfr::Surface::Evaluate (fixture), within the 1e-5 bounding-box
+    tolerance; the device's face -> ptex mapping equals the host's."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_far_compiler import FixtureMesh, bfr_to_ptex
+    from conftest import bbox_rel_err
+    from opensubdiv_amd import BufferDescriptor, HipEvaluator, HipPatchTable, HipStencilTable
+    from opensubdiv_amd import HipVertexBuffer, far
+    from opensubdiv_amd.osd import HipPatchMap
+    g = golden("bfr_surfaces.npz")
+    m = FixtureMesh(g, name)
+    pt = far.Topology(m, boundary=int(g[name + "_boundary"])).bfr_patch_table()
+    st = pt.point_stencils()
+    nc, ns = m.nverts, len(st["sizes"])
+    vb = HipVertexBuffer.Create(3, nc + ns)
+    vb.UpdateData(m.positions.astype(np.float32), 0, nc)
+    assert HipEvaluator.EvalStencils(vb, BufferDescriptor(0, 3, 3), vb,
+                                     BufferDescriptor(nc * 3, 3, 3), HipStencilTable.Create(st))
+    faces, u, v = g[name + "_faces"], g[name + "_u"], g[name + "_v"]
+    n = len(faces)
+    sizes = np.asarray(m.face_sizes, np.int32)
+    base = np.concatenate([[0], np.cumsum(np.where(sizes == 4, 1, sizes))[:-1]]).astype(np.int32)
+    pm = HipPatchMap(pt.arrays, pt.params)
+    coords = dev(np.zeros((n, 5), np.int32))
+    assert pm.FindFaceCoords(n, dev(faces), dev(u), dev(v), dev(sizes), dev(base), coords)
+    assert HipEvaluator.Synchronize()
+    got = host(coords).view(PATCH_COORD_DTYPE).reshape(-1)
+    ptex, s, t = bfr_to_ptex(sizes, faces, u, v)
+    want = pt.find(ptex, s, t)
+    for k in ("arrayIndex", "patchIndex", "vertIndex", "s", "t"):
+        np.testing.assert_array_equal(got[k], want[k], err_msg=k)
+    table = HipPatchTable(pt.arrays, pt.indices, pt.params)
+    d = BufferDescriptor(0, 3, 3)
+    P, du, dv = empty(n * 3), empty(n * 3), empty(n * 3)
+    assert HipEvaluator.EvalPatches(vb, d, P, d, du, d, dv, d, n, coords, table)
+    assert HipEvaluator.Synchronize()
+    for o, k in zip((P, du, dv), ("P", "Du", "Dv")):
+        assert bbox_rel_err(host(o, shape=(n, 3)).astype(np.float64), g[name + "_" + k]) < 1e-5, k
+    pm.close()
