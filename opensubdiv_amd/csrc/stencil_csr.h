@@ -5,7 +5,7 @@
 // form one slice [lo, hi) of the index / weight streams, which the wave
 // walks in windows of kCsrWindow (512) entries: each window is copied into LDS
 // with coalesced, non-temporal 16-byte loads, then every lane adds the
-// entries of its own stencil that fall in the window, issuing four
+// entries of its own stencil that fall in the window, issuing eight
 // control-point gathers at a time through L2.  Small windows keep the LDS
 // and register footprint per wave low, so many waves per CU hide the
 // loads' latency.  Slice entry e of a window sits at LDS position
