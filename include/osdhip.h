@@ -323,6 +323,27 @@ int osdhip_tess_uniform_quads(int numFaces, const int *ptexFaces, int rate, floa
                               const osdhip_patch_map *map, osdhip_patch_coord *coords,
                               void *stream);
 
+/* Bfr::Tessellation coordinates (bfr/tessellation.h:51-240) for any
+ * parameterization and rates, on the device.  scheme: 0 bilinear or 1
+ * Catmark (a quad's unit square; other face sizes: Bfr's quad sub-faces),
+ * 2 Loop (triangles).  rates as Tessellation(param, numRates, rates) takes
+ * them: 1 uniform rate, 2 inner rates of a quad, N outer (edge) rates, or N
+ * outer plus 1 (2 for a quad) inner rates.  osdhip_tess_count returns the
+ * coordinates per face (GetNumCoords), OSDHIP_ERR_ARGS when invalid.
+ * osdhip_tess_coords writes the pattern for numFaces faces: face-major
+ * (u, v) float pairs to `uv` (may be NULL), in GetCoords order with the
+ * same float values (boundary first, then the interior); with a patch map,
+ * also the PatchCoords of base face faces[f] at each coordinate, located
+ * as osdhip_patch_map_find_faces does (faceSizes / ptexBase / numBaseFaces
+ * as there).  OSDHIP_ERR_ARGS when nothing is to be written, for coords
+ * without a map (or a map without faces / coords), or more than 2^31 - 1
+ * coordinates. */
+int osdhip_tess_count(int scheme, int faceSize, int numRates, const int *rates);
+int osdhip_tess_coords(int scheme, int faceSize, int numRates, const int *rates, int numFaces,
+                       float *uv, const int *faces, const osdhip_patch_map *map,
+                       const int *faceSizes, const int *ptexBase, int numBaseFaces,
+                       osdhip_patch_coord *coords, void *stream);
+
 /* ---- the step after EvalPatches: limit normals ------------------------------
  * normals[i] = cross(dPdu[i], dPdv[i]) over xyz (the first three elements of
  * each descriptor), scaled to unit length by 1 / sqrtf(dot) when `normalize`

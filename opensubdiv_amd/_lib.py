@@ -88,6 +88,8 @@ _HIP_SIGS = {
     "osdhip_patch_map_destroy": [_vp],
     "osdhip_patch_map_find_faces": [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp],
     "osdhip_tess_uniform_quads": [_i, _vp, _i, _vp, _vp, _vp, _vp],
+    "osdhip_tess_count": [_i, _i, _i, _vp],
+    "osdhip_tess_coords": [_i, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp],
     "osdhip_stencil_table_eval": [_vp, _vp, _D, _i, _vp, _vp, _i, _i, _vp],
     "osdhip_malloc": [C.POINTER(C.c_void_p), C.c_size_t],
     "osdhip_free": [_vp],

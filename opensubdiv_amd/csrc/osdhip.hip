@@ -29,6 +29,7 @@
 #include "common.h"
 #include "patch_kernels.h"
 #include "patch_map.h"
+#include "tess.h"
 #include "patch_sort.h"
 #include "stencil_csr.h"
 #include "stencil_sliced.h"
@@ -990,6 +991,71 @@ int osdhip_patch_map_find_faces(const osdhip_patch_map *map, int n, const int *f
     hipLaunchKernelGGL(osdhip::k_patch_map_find_faces, dim3(grid), dim3(kBlock), 0,
                        (hipStream_t)stream, a);
     return last_error();
+}
+
+int osdhip_tess_count(int scheme, int faceSize, int numRates, const int *rates)
+{
+    try {
+        osdhip::TessPattern t;
+        return t.build(scheme, faceSize, numRates, rates) ? t.count : OSDHIP_ERR_ARGS;
+    } catch (...) {
+        return OSDHIP_ERR_ARGS;
+    }
+}
+
+int osdhip_tess_coords(int scheme, int faceSize, int numRates, const int *rates, int numFaces,
+                       float *uv, const int *faces, const osdhip_patch_map *map,
+                       const int *faceSizes, const int *ptexBase, int numBaseFaces,
+                       osdhip_patch_coord *coords, void *stream)
+{
+    try {
+        if (numFaces < 0) return OSDHIP_ERR_ARGS;
+        osdhip::TessPattern t;
+        if (!t.build(scheme, faceSize, numRates, rates)) return OSDHIP_ERR_ARGS;
+        if (numFaces == 0) return OSDHIP_OK;
+        const bool toCoords = map != nullptr;
+        if ((!uv && !toCoords) || (toCoords && (!coords || !faces || !faceSizes || !ptexBase ||
+                                                 numBaseFaces <= 0)) ||
+            (!toCoords && coords))
+            return OSDHIP_ERR_ARGS;
+        if ((long long)t.count * numFaces > 0x7fffffffLL) return OSDHIP_ERR_ARGS;
+        hipStream_t st = (hipStream_t)stream;
+        osdhip::TessRun *druns = nullptr;
+        const size_t bytes = t.runs.size() * sizeof(osdhip::TessRun);
+        if (hipMallocAsync((void **)&druns, bytes, st) != hipSuccess) return last_error();
+        // pageable source: the copy has read it when the call returns
+        hipError_t e = hipMemcpyAsync(druns, t.runs.data(), bytes, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) {
+            osdhip::TessRunArgs a;
+            a.runs = druns;
+            a.numRuns = (int)t.runs.size();
+            a.perFace = t.count;
+            a.numFaces = numFaces;
+            a.uv = uv;
+            a.q.map.nodes = toCoords ? map->nodes : nullptr;
+            a.q.map.handles = toCoords ? map->handles : nullptr;
+            a.q.map.minFace = toCoords ? map->minFace : 0;
+            a.q.map.maxFace = toCoords ? map->maxFace : -1;
+            a.q.map.maxDepth = toCoords ? map->maxDepth : 0;
+            a.q.map.n = 0;
+            a.q.map.ptex = nullptr;
+            a.q.map.u = a.q.map.v = nullptr;
+            a.q.map.coords = coords;
+            a.q.faces = faces;
+            a.q.faceSizes = faceSizes;
+            a.q.ptexBase = ptexBase;
+            a.q.numFaces = numBaseFaces;
+            const long long threads = (long long)numFaces * a.numRuns;
+            const int grid = (int)((threads + kBlock - 1) / kBlock);
+            hipLaunchKernelGGL(osdhip::k_tess_runs, dim3(grid), dim3(kBlock), 0, st, a);
+            e = hipGetLastError();
+        }
+        const hipError_t f = hipFreeAsync(druns, st);
+        if (e != hipSuccess) return (int)e;
+        return f == hipSuccess ? OSDHIP_OK : (int)f;
+    } catch (...) {
+        return OSDHIP_ERR_ARGS;
+    }
 }
 
 int osdhip_tess_uniform_quads(int numFaces, const int *ptexFaces, int rate, float *uv,

@@ -75,12 +75,9 @@ struct FaceQueryArgs {
     int numFaces;
 };
 
-__global__ __launch_bounds__(kBlock) void k_patch_map_find_faces(FaceQueryArgs a)
+__device__ __forceinline__ osdhip_patch_coord face_query(const FaceQueryArgs &a, int f, double u,
+                                                        double v)
 {
-    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
-    if (i >= a.map.n) return;
-    const int f = a.faces[i];
-    double u = a.map.u[i], v = a.map.v[i];
     int ptex = -1;
     if (f >= 0 && f < a.numFaces) {
         const int n = a.faceSizes[f];
@@ -98,7 +95,14 @@ __global__ __launch_bounds__(kBlock) void k_patch_map_find_faces(FaceQueryArgs a
             v = 2.0 * vFrac;
         }
     }
-    a.map.coords[i] = patch_map_lookup(a.map, ptex, u, v);
+    return patch_map_lookup(a.map, ptex, u, v);
+}
+
+__global__ __launch_bounds__(kBlock) void k_patch_map_find_faces(FaceQueryArgs a)
+{
+    const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (i >= a.map.n) return;
+    a.map.coords[i] = face_query(a, a.faces[i], a.map.u[i], a.map.v[i]);
 }
 
 // One (u, v) of Bfr::Tessellation's uniform pattern of a quad at `rate`

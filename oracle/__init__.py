@@ -72,6 +72,7 @@ def lib():
             "oracle_eval_patches_omp_f32": [vp, dd, vp, dd, vp, dd, vp, dd, i, vp, vp, vp, vp, i],
             "oracle_limit_normals_f32": [vp, dd, vp, dd, vp, dd, i, i],
             "oracle_tess_uniform_quad_f32": [i, vp],
+            "oracle_tess_coords_f32": [i, i, i, vp, vp],
         }
         for k, a in sigs.items():
             f = getattr(L, k)
@@ -190,6 +191,20 @@ def limit_normals(du, du_desc, dv, dv_desc, n, n_desc, count, normalize=True):
     steps (far_tutorial_2_3.cpp:73-87). Returns bool."""
     return bool(lib().oracle_limit_normals_f32(_p(du), D(du_desc), _p(dv), D(dv_desc), _p(n),
                                                D(n_desc), int(count), int(bool(normalize))))
+
+
+def tess_coords(scheme, face_size, rates):
+    """Bfr::Tessellation::GetCoords<float> of a face of `face_size` corners
+    under `scheme` (0 bilinear, 1 Catmark, 2 Loop) at `rates` (1 uniform,
+    2 inner of a quad, N outer, N + 1 / N + 2 outer and inner): (n, 2)
+    float32, or None for invalid arguments."""
+    r = np.ascontiguousarray(rates, np.int32)
+    n = lib().oracle_tess_coords_f32(int(scheme), int(face_size), len(r), _p(r), None)
+    if n < 0:
+        return None
+    uv = np.zeros((n, 2), np.float32)
+    assert lib().oracle_tess_coords_f32(int(scheme), int(face_size), len(r), _p(r), _p(uv)) == n
+    return uv
 
 
 def tess_uniform_quad(rate):

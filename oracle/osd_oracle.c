@@ -375,6 +375,222 @@ static int tess_iso_u(float *uv, int n, float u, float v, float dv)
     return n;
 }
 
+static int tess_uv_line(float *uv, int n, float u, float v, float du, float dv)
+{
+    for (int i = 0; i < n; ++i, u += du, v += dv) {
+        uv[2 * i] = u;
+        uv[2 * i + 1] = v;
+    }
+    return n;
+}
+
+static int tess_point(float *uv, float u, float v)
+{
+    uv[0] = u;
+    uv[1] = v;
+    return 1;
+}
+
+/* Bfr::Tessellation::GetCoords<float> for any parameterization and rates
+ * (bfr/tessellation.cpp): scheme 1 Catmark / 0 bilinear (quads: QUAD,
+ * other sizes: QUAD_SUBFACES) or 2 Loop (triangles: TRI); rates as
+ * Tessellation(param, numRates, rates) takes them -- 1 uniform rate, 2 inner
+ * rates of a quad, N outer rates, N + 1 (N + 2 for quads) outer and inner
+ * rates.  Rates initialised as initializeRates (:2077-2146, rates capped at
+ * 32767), interior point count as initializeInventoryFor* (:2165-2275),
+ * boundary coords per parameterization (quad :929-945, tri :1368-1381, qsub
+ * :1745-1760 via getRingEdgeCoords :1687-1733), interior coords (:2345-2371;
+ * quad :945-995, tri :1383-1422, qsub :1762-1822).  Writes the (u, v)
+ * pairs when uv is not NULL; returns the coordinate count, -1 for invalid
+ * arguments. */
+typedef struct {
+    int type;           /* 0 quad, 1 tri, 2 quad sub-faces */
+    int N, uDim;
+} tess_param;
+
+static void tess_vertex(const tess_param *P, int k, float *u, float *v)
+{
+    if (P->type == 0) { *u = (float)(k && k < 3); *v = (float)(k > 1); }
+    else if (P->type == 1) { *u = (float)(k == 1); *v = (float)(k == 2); }
+    else { *u = (float)(k % P->uDim); *v = (float)(k / P->uDim); }
+}
+
+static int tess_qsub_edge(const tess_param *P, float *uv, int edge, int edgeRes, int incFirst,
+                          float t0, float dt)
+{
+    const int n0 = (edgeRes - 1) / 2, n1 = (edgeRes - 1) - n0;
+    int n = 0;
+    if (incFirst || n0) {
+        float u0, v0;
+        tess_vertex(P, edge, &u0, &v0);
+        if (incFirst) n += tess_point(uv + 2 * n, u0 + t0, v0 + t0);
+        if (n0) n += tess_iso_v(uv + 2 * n, n0, u0 + t0 + dt, v0 + t0, dt);
+    }
+    if (n1) {
+        float u1, v1;
+        tess_vertex(P, (edge + 1) % P->N, &u1, &v1);
+        n += tess_iso_u(uv + 2 * n, n1, u1 + t0,
+                        v1 + ((edgeRes & 1) ? (0.5f - 0.5f * dt) : 0.5f), -dt);
+    }
+    return n;
+}
+
+int oracle_tess_coords_f32(int scheme, int faceSize, int numRates, const int *rates, float *uv)
+{
+    if (faceSize < 3 || numRates < 1 || !rates) return -1;
+    if (scheme == 2 && faceSize != 3) return -1;
+    for (int i = 0; i < numRates; ++i)
+        if (rates[i] < 1) return -1;
+    tess_param P;
+    P.N = faceSize;
+    P.type = scheme == 2 ? 1 : (faceSize == 4 ? 0 : 2);
+    P.uDim = faceSize < 10 ? 2 + (faceSize > 4) : 1 + (int)sqrtf((float)(faceSize - 1));
+    const int N = faceSize, isQuad = P.type == 0, maxRate = 32767;
+    int outer[64], inner[2], uniform = 1, total = 0;
+    int *out = outer;
+    int *heap = NULL;
+    if (N > 64) {
+        heap = (int *)malloc(sizeof(int) * (size_t)N);
+        if (!heap) return -1;
+        out = heap;
+    }
+#define CAP(x) ((x) < maxRate ? (x) : maxRate)
+    if (numRates < N) {
+        if (numRates == 2 && isQuad) {
+            inner[0] = CAP(rates[0]);
+            inner[1] = CAP(rates[1]);
+            out[0] = out[2] = inner[0];
+            out[1] = out[3] = inner[1];
+            uniform = inner[0] == inner[1];
+            total = 2 * (inner[0] + inner[1]);
+        } else {
+            inner[0] = inner[1] = CAP(rates[0]);
+            for (int i = 0; i < N; ++i) out[i] = inner[0];
+            total = inner[0] * N;
+        }
+    } else {
+        for (int i = 0; i < N; ++i) {
+            out[i] = CAP(rates[i]);
+            uniform = uniform && out[i] == out[0];
+            total += out[i];
+        }
+        if (numRates > N) {
+            inner[0] = CAP(rates[N]);
+            inner[1] = (numRates == 6 && isQuad) ? CAP(rates[5]) : inner[0];
+            uniform = uniform && inner[0] == out[0] && inner[1] == out[0];
+        } else if (isQuad) {
+            inner[0] = (out[0] + out[2]) / 2;
+            inner[1] = (out[1] + out[3]) / 2;
+        } else {
+            inner[0] = inner[1] = total / N;
+        }
+    }
+#undef CAP
+    /* interior point count (initializeInventoryFor*) */
+    int nInterior = 0;
+    if (P.type == 0) {
+        if (uniform) nInterior = inner[0] > 1 ? (inner[0] - 1) * (inner[0] - 1) : 0;
+        else if (inner[0] > 1 && inner[1] > 1) nInterior = (inner[0] - 1) * (inner[1] - 1);
+        else if (out[0] == inner[0] && inner[0] == out[2] && out[1] == inner[1] && inner[1] == out[3])
+            nInterior = 0;
+        else nInterior = 1;
+    } else if (P.type == 1) {
+        const int r = inner[0];
+        if (uniform) nInterior = r > 1 ? (r - 2) * (r - 1) / 2 : 0;
+        else nInterior = r > 2 ? (r - 2) * (r - 1) / 2 : 1;
+    } else {
+        const int r = inner[0];
+        if (r > 1) {
+            const int e = r - 2, H = e / 2;
+            nInterior = (e & 1) ? (H + 1) * (H + 1) * N + (N == 3 ? 0 : 1) : H * (H + 1) * N + 1;
+        } else {
+            nInterior = (uniform && N == 3) ? 0 : 1;
+        }
+    }
+    const int count = total + nInterior;
+    if (!uv) {
+        free(heap);
+        return count;
+    }
+    int n = 0;
+    /* boundary */
+    if (P.type == 0) {
+        n += tess_iso_v(uv + 2 * n, out[0], 0.0f, 0.0f, 1.0f / (float)out[0]);
+        n += tess_iso_u(uv + 2 * n, out[1], 1.0f, 0.0f, 1.0f / (float)out[1]);
+        n += tess_iso_v(uv + 2 * n, out[2], 1.0f, 1.0f, -1.0f / (float)out[2]);
+        n += tess_iso_u(uv + 2 * n, out[3], 0.0f, 1.0f, -1.0f / (float)out[3]);
+    } else if (P.type == 1) {
+        n += tess_iso_v(uv + 2 * n, out[0], 0.0f, 0.0f, 1.0f / (float)out[0]);
+        n += tess_uv_line(uv + 2 * n, out[1], 1.0f, 0.0f, -1.0f / (float)out[1],
+                          1.0f / (float)out[1]);
+        n += tess_iso_u(uv + 2 * n, out[2], 0.0f, 1.0f, -1.0f / (float)out[2]);
+    } else {
+        for (int i = 0; i < N; ++i)
+            n += tess_qsub_edge(&P, uv + 2 * n, i, out[i], 1, 0.0f, 1.0f / (float)out[i]);
+    }
+    /* interior */
+    if (nInterior == 1) {
+        n += tess_point(uv + 2 * n, P.type == 1 ? 1.0f / 3.0f : 0.5f,
+                        P.type == 1 ? 1.0f / 3.0f : 0.5f);
+    } else if (nInterior > 1 && P.type == 0) {
+        const int rings = inner[0] / 2 < inner[1] / 2 ? inner[0] / 2 : inner[1] / 2;
+        const float du = 1.0f / (float)inner[0], dv = 1.0f / (float)inner[1];
+        float u = du, v = dv;
+        int ur = inner[0] - 2, vr = inner[1] - 2;
+        for (int i = 0; i < rings; ++i, ur -= 2, vr -= 2, u += du, v += dv) {
+            if (ur > 0 && vr > 0) {
+                const float u1 = 1.0f - u, v1 = 1.0f - v;
+                n += tess_iso_v(uv + 2 * n, ur, u, v, du);
+                n += tess_iso_u(uv + 2 * n, vr, u1, v, dv);
+                n += tess_iso_v(uv + 2 * n, ur, u1, v1, -du);
+                n += tess_iso_u(uv + 2 * n, vr, u, v1, -dv);
+            } else if (ur > 0) {
+                n += tess_iso_v(uv + 2 * n, ur + 1, u, v, du);
+            } else if (vr > 0) {
+                n += tess_iso_u(uv + 2 * n, vr + 1, u, v, dv);
+            } else {
+                n += tess_point(uv + 2 * n, 0.5f, 0.5f);
+            }
+        }
+    } else if (nInterior > 1 && P.type == 1) {
+        const int r = inner[0], rings = r / 3;
+        const float dt = 1.0f / (float)r;
+        float u = dt, v = dt;
+        int rr = r - 3;
+        for (int i = 0; i < rings; ++i, rr -= 3, u += dt, v += dt) {
+            if (rr == 0) {
+                n += tess_point(uv + 2 * n, 1.0f / 3.0f, 1.0f / 3.0f);
+            } else {
+                const float u1 = 1.0f - u * 2.0f, v1 = 1.0f - v * 2.0f;
+                n += tess_iso_v(uv + 2 * n, rr, u, v, dt);
+                n += tess_uv_line(uv + 2 * n, rr, u1, v, -dt, dt);
+                n += tess_iso_u(uv + 2 * n, rr, u, v1, -dt);
+            }
+        }
+    } else if (nInterior > 1) {
+        const int r = inner[0], rings = r / 2;
+        const float dt = 1.0f / (float)r;
+        float t = dt;
+        int rr = r - 2;
+        for (int i = 0; i < rings; ++i, rr -= 2, t += dt) {
+            if (rr == 0) {
+                n += tess_point(uv + 2 * n, 0.5f, 0.5f);
+            } else if (rr == 1) {
+                for (int k = 0; k < N; ++k) {
+                    float u0, v0;
+                    tess_vertex(&P, k, &u0, &v0);
+                    n += tess_point(uv + 2 * n, u0 + t, v0 + t);
+                }
+                if (N != 3) n += tess_point(uv + 2 * n, 0.5f, 0.5f);
+            } else {
+                for (int k = 0; k < N; ++k) n += tess_qsub_edge(&P, uv + 2 * n, k, rr, 1, t, dt);
+            }
+        }
+    }
+    free(heap);
+    return n == count ? n : -1;
+}
+
 int oracle_tess_uniform_quad_f32(int rate, float *uv)
 {
     if (rate < 1) return -1;

@@ -172,5 +172,10 @@ int oracle_limit_normals_f32(const float *du, oracle_desc duDesc, const float *d
 /* Bfr::Tessellation::GetCoords<float> of a quad at a uniform rate
  * (bfr/tessellation.cpp:158-174, 929-990): (rate + 1)^2 (u, v) pairs. */
 int oracle_tess_uniform_quad_f32(int rate, float *uv);
+/* Bfr::Tessellation::GetCoords<float> for any parameterization (scheme 0
+ * bilinear / 1 Catmark / 2 Loop, faceSize) and rates as Tessellation takes
+ * them (bfr/tessellation.cpp:2023-2371); uv NULL returns the count only;
+ * -1 for invalid arguments. */
+int oracle_tess_coords_f32(int scheme, int faceSize, int numRates, const int *rates, float *uv);
 
 #endif

@@ -61,6 +61,7 @@ def lib():
             "ref_cpu_eval_patches_d2": ([vp] * 14 + [i, vp, vp, vp, vp], i),
             "ref_omp_eval_patches": ([vp] * 8 + [i, vp, vp, vp, vp, i], i),
             "ref_tess_uniform_coords": ([i, i, vp], i),
+            "ref_tess_coords": ([i, i, i, vp, vp], i),
         }
         for k, (a, r) in sig.items():
             f = getattr(L, k)
@@ -334,6 +335,18 @@ def cpu_eval_patches_d2(src, sd, outs, descs, coords, arrays, indices, params):
         a += [_p(o), _p(x)]
     return bool(lib().ref_cpu_eval_patches_d2(_p(src), _p(sdd), *a, len(coords), _p(coords),
                                               _p(arrays), _p(indices), _p(params)))
+
+
+def tess_coords(scheme, face_size, rates):
+    """Bfr::Tessellation(Parameterization(scheme, face_size), rates)
+    .GetCoords<float>: (n, 2) float32, or None when invalid."""
+    r = np.ascontiguousarray(rates, np.int32)
+    n = lib().ref_tess_coords(int(scheme), int(face_size), len(r), _p(r), None)
+    if n < 0:
+        return None
+    uv = np.zeros((n, 2), np.float32)
+    assert lib().ref_tess_coords(int(scheme), int(face_size), len(r), _p(r), _p(uv)) == n
+    return uv
 
 
 def tess_uniform_coords(face_size, rate):

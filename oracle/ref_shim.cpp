@@ -734,4 +734,16 @@ int ref_tess_uniform_coords(int faceSize, int rate, float *uv)
     return t.GetCoords(uv);
 }
 
+// Bfr::Tessellation(Parameterization(scheme, faceSize), numRates, rates)
+// .GetCoords<float> (bfr/tessellation.h:107-240); uv NULL returns the
+// count; -1 for an invalid parameterization or rates.
+int ref_tess_coords(int scheme, int faceSize, int numRates, const int *rates, float *uv)
+{
+    OpenSubdiv::Bfr::Parameterization p((OpenSubdiv::Sdc::SchemeType)scheme, faceSize);
+    if (!p.IsValid()) return -1;
+    OpenSubdiv::Bfr::Tessellation t(p, numRates, rates);
+    if (!t.IsValid()) return -1;
+    return uv ? t.GetCoords(uv) : t.GetNumCoords();
+}
+
 }  // extern "C"

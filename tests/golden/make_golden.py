@@ -37,6 +37,10 @@ reference's outputs; nothing of the reference's source goes into them.
                         Bfr's quad sub-face parameterization), an open
                         creased grid -- at seeded (face, u, v) queries in
                         Bfr's parameterization: meshes, queries, P/Du/Dv
+  tess_coords.npz       Bfr::Tessellation::GetCoords<float> for 300 seeded
+                        (scheme, face size, rates) patterns: quads, Loop
+                        triangles and Catmark n-gon sub-faces (3-12 sides),
+                        uniform, inner-only, outer and outer + inner rates
   cubesphere_fvar.npz   a face-varying UV channel (one chart per cube face,
                         seams along the cube edges) on the creased
                         cube-sphere n=4, iso 3, two ways: Far's defaults
@@ -256,6 +260,27 @@ def bfr_surfaces():
     save("bfr_surfaces.npz", **out)
 
 
+def tess_coords():
+    rng = np.random.default_rng(77)
+    kinds = [(1, 4), (0, 4), (2, 3), (1, 3), (1, 5), (1, 6), (1, 7), (1, 9), (1, 10), (1, 12)]
+    cfg, rates_flat, coords = [], [], []
+    for k in range(300):
+        scheme, N = kinds[k % len(kinds)]
+        if k < 60:
+            rates = [int(rng.integers(1, 17))]
+        else:
+            m = int(rng.choice([N, N + 1, N + 2 if N == 4 else N + 1, 2 if N == 4 else 1]))
+            rates = [int(x) for x in rng.integers(1, 13, m)]
+        uv = ref.tess_coords(scheme, N, rates)
+        assert uv is not None
+        cfg.append((scheme, N, len(rates), len(rates_flat), len(uv),
+                    sum(len(c) for c in coords)))
+        rates_flat.extend(rates)
+        coords.append(uv)
+    save("tess_coords.npz", configs=np.array(cfg, np.int32),
+         rates=np.array(rates_flat, np.int32), coords=np.concatenate(coords))
+
+
 def tri_sphere(n, seed):
     """The cube-sphere's quads split along one diagonal (a closed Loop
     mesh, valences 4 to 6), 5 % of its edges creased."""
@@ -335,7 +360,7 @@ if __name__ == "__main__":
         sys.exit("oracle/_ref/libosdref.so not built (make -C oracle)")
     # names on the command line write only those fixtures
     makers = [cube_l3, cubesphere_l2, cubesphere_limit, cubesphere_patches, loop_patches,
-              cubesphere_fvar, bfr_surfaces]
+              cubesphere_fvar, bfr_surfaces, tess_coords]
     wanted = set(sys.argv[1:])
     for fn in makers:
         if not wanted or fn.__name__ in wanted:

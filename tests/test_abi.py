@@ -71,3 +71,21 @@ def test_batch_struct_layouts_match_the_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_tess_count_is_host_only_and_matches_the_oracle():
+    """osdhip_tess_count builds a tessellation pattern on the host (no device
+    call): its coordinate count equals the oracle's (pinned against Bfr) for
+    every fixture pattern, and invalid rates are rejected."""
+    import numpy as np
+    import oracle
+    from conftest import golden
+    lib = _lib.load_hip()
+    g = golden("tess_coords.npz")
+    for scheme, n, nr, r0, nc, c0 in g["configs"]:
+        r = np.ascontiguousarray(g["rates"][r0:r0 + nr], np.int32)
+        assert lib.osdhip_tess_count(int(scheme), int(n), int(nr), r.ctypes.data) == nc
+    bad = np.array([3, 0, 2, 2], np.int32)
+    assert lib.osdhip_tess_count(1, 4, 4, bad.ctypes.data) == _lib.ERR_ARGS
+    assert lib.osdhip_tess_count(2, 4, 1, bad.ctypes.data) == _lib.ERR_ARGS
+    assert oracle.tess_coords(1, 4, [3, 0, 2, 2]) is None

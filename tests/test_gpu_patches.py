@@ -664,3 +664,59 @@ def test_bfr_face_queries_on_the_device_match_bfr_surfaces(hip, name):
     for o, k in zip((P, du, dv), ("P", "Du", "Dv")):
         assert bbox_rel_err(host(o, shape=(n, 3)).astype(np.float64), g[name + "_" + k]) < 1e-5, k
     pm.close()
+
+
+def test_device_tessellation_of_every_parameterization_matches_bfr(hip):
+    """osdhip_tess_coords for quads, Loop triangles and Catmark n-gon
+    sub-faces at uniform, inner-only, outer and outer + inner rates: the
+    coordinates of every face (three faces per pattern) are bit-identical to
+    the reference's Bfr::Tessellation::GetCoords<float> (fixture)."""
+    g = golden("tess_coords.npz")
+    nf = 3
+    for scheme, n, nr, r0, nc, c0 in g["configs"]:
+        rates = np.ascontiguousarray(g["rates"][r0:r0 + nr], np.int32)
+        uv = empty(nf * nc * 2)
+        rc = hip.osdhip_tess_coords(int(scheme), int(n), int(nr), rates.ctypes.data, nf, ptr(uv),
+                                    None, None, None, None, 0, None, stream())
+        assert rc == 0, (scheme, n, list(rates))
+        got = host(uv, shape=(nf, nc, 2))
+        want = g["coords"][c0:c0 + nc]
+        for f in range(nf):
+            assert got[f].tobytes() == want.tobytes(), (scheme, n, list(rates), f)
+
+
+def test_device_tessellation_to_patch_coords_of_base_faces(hip):
+    """Tessellating base faces straight into PatchCoords: an n-gon mesh's
+    quads and pentagons at per-edge rates through the device patch map,
+    equal to osdhip_patch_map_find_faces of the same (face, u, v)."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_far_compiler import POLY_MESHES
+    from opensubdiv_amd import far
+    from opensubdiv_amd.osd import HipPatchMap
+    m = POLY_MESHES["pentagonal-prism"]()
+    pt = far.Topology(m).patch_table(3)
+    sizes = np.asarray(m.face_sizes, np.int32)
+    base = np.concatenate([[0], np.cumsum(np.where(sizes == 4, 1, sizes))[:-1]]).astype(np.int32)
+    pm = HipPatchMap(pt.arrays, pt.params)
+    for fs, rates in ((4, [3, 5, 2, 4]), (5, [4, 2, 3, 5, 1, 3]), (4, [6])):
+        faces = np.nonzero(sizes == fs)[0].astype(np.int32)
+        r = np.ascontiguousarray(rates, np.int32)
+        nc = hip.osdhip_tess_count(1, fs, len(r), r.ctypes.data)
+        assert nc > 0
+        uv, coords = empty(len(faces) * nc * 2), dev(np.zeros((len(faces) * nc, 5), np.int32))
+        dfaces, dsizes, dbase = dev(faces), dev(sizes), dev(base)
+        assert hip.osdhip_tess_coords(1, fs, len(r), r.ctypes.data, len(faces), ptr(uv),
+                                      ptr(dfaces), pm.h, ptr(dsizes), ptr(dbase), len(sizes),
+                                      ptr(coords), stream()) == 0
+        uvh = host(uv, shape=(len(faces) * nc, 2)).astype(np.float64)
+        qf = np.repeat(faces, nc).astype(np.int32)
+        want = dev(np.zeros((len(qf), 5), np.int32))
+        assert pm.FindFaceCoords(len(qf), dev(qf), dev(np.ascontiguousarray(uvh[:, 0])),
+                                 dev(np.ascontiguousarray(uvh[:, 1])), dsizes, dbase, want)
+        got_h = host(coords).view(PATCH_COORD_DTYPE).reshape(-1)
+        want_h = host(want).view(PATCH_COORD_DTYPE).reshape(-1)
+        assert got_h.tobytes() == want_h.tobytes(), (fs, rates)
+        assert (got_h["arrayIndex"] >= 0).all()
+    pm.close()

@@ -745,3 +745,67 @@ def test_tess_planted_mistake_is_caught():
     d = np.float32(1.0) / np.float32(rate)
     wrong = (np.arange(rate, dtype=np.float32) * d).astype(np.float32)
     assert wrong.tobytes() != g["rate33"][:rate, 0].tobytes()
+
+
+def _tess_fixture():
+    g = golden("tess_coords.npz")
+    for scheme, n, nr, r0, nc, c0 in g["configs"]:
+        yield int(scheme), int(n), g["rates"][r0:r0 + nr], g["coords"][c0:c0 + nc]
+
+
+def test_tess_coords_match_bfr_for_every_parameterization():
+    """The oracle's Bfr::Tessellation restatement for quads, Loop triangles
+    and Catmark n-gon sub-faces (3-12 sides) at uniform, inner-only, outer
+    and outer + inner rates, bit for bit against the reference's own
+    GetCoords<float> (fixture of 300 patterns; live for more when the
+    reference library is built); plus what the pattern fixes: the uniform
+    counts in closed form (quad (r+1)^2, triangle (r+1)(r+2)/2, n-gon
+    sub-faces as Bfr counts them), the boundary first with the outer
+    rates' points on the face's boundary, every coordinate inside the
+    parameterization's domain."""
+    for scheme, n, rates, want in _tess_fixture():
+        uv = oracle.tess_coords(scheme, n, rates)
+        assert uv is not None and uv.tobytes() == want.tobytes(), (scheme, n, list(rates))
+        if len(rates) == 1:
+            r = int(rates[0])
+            if scheme != 2 and n == 4:
+                assert len(uv) == (r + 1) ** 2
+            elif scheme == 2:
+                assert len(uv) == (r + 1) * (r + 2) // 2
+        outer = list(rates) if len(rates) >= n else [int(rates[0])] * n
+        if len(rates) == 2 and n == 4:
+            outer = [int(rates[0]), int(rates[1])] * 2
+        nb = sum(outer[:n])
+        b = uv[:nb].astype(np.float64)
+        if scheme == 2:
+            assert np.all((np.abs(b[:, 0]) < 1e-6) | (np.abs(b[:, 1]) < 1e-6) |
+                          (np.abs(b.sum(axis=1) - 1) < 1e-6))
+            assert np.all(uv >= -1e-6) and np.all(uv.sum(axis=1) <= 1 + 1e-6)
+        elif n == 4:
+            assert ((b == 0) | (b == 1)).any(axis=1).all()
+            assert np.all((uv >= -1e-6) & (uv <= 1 + 1e-6))
+        else:
+            udim = (2 + (n > 4)) if n < 10 else 1 + int(np.sqrt(n - 1))
+            tile = np.floor(uv + 0.25)                  # sub-face tile of each coordinate
+            assert np.all(tile[:, 0] < udim) and np.all(tile.min(axis=0) >= 0)
+            assert np.all(uv - tile <= 0.5 + 1e-6)
+    from oracle import ref
+    if ref.available():
+        rng = np.random.default_rng(3)
+        for scheme, n in [(1, 4), (2, 3), (1, 3), (1, 5), (1, 8), (1, 11), (0, 6)]:
+            for _ in range(40):
+                m = int(rng.choice([1, n, n + 1]))
+                rates = rng.integers(1, 20, m)
+                a, b = oracle.tess_coords(scheme, n, rates), ref.tess_coords(scheme, n, rates)
+                assert (a is None) == (b is None)
+                assert a is None or a.tobytes() == b.tobytes(), (scheme, n, list(rates))
+
+
+def test_tess_coords_invalid_arguments():
+    """What Bfr rejects the oracle rejects: rates below 1, no rates, faces of
+    fewer than 3 sides, Loop faces other than triangles."""
+    assert oracle.tess_coords(1, 4, [0]) is None
+    assert oracle.tess_coords(1, 4, []) is None
+    assert oracle.tess_coords(1, 2, [3]) is None
+    assert oracle.tess_coords(2, 4, [3]) is None
+    assert oracle.tess_coords(1, 4, [3, 0, 2, 2]) is None
