@@ -181,7 +181,9 @@ def table_eval_fn(lib, table, src, srcdesc, outs, descs, stream):
 def kernel_labels(nw, lc):
     return {"rec_g8": f"k_stencils_rec<{nw},{lc},8>", "rec_g4": f"k_stencils_rec<{nw},{lc},4>",
             "rec_g16": f"k_stencils_rec<{nw},{lc},16>",
-            "split": f"k_stencils_sliced<{nw},{lc}>", "csr_lds": f"k_stencils_csr<{nw},{lc}>"}
+            "split": f"k_stencils_sliced<{nw},{lc}>", "csr_lds": f"k_stencils_csr<{nw},{lc}>",
+            "rec_pair": f"k_stencils_rec_paired<{lc},16>",
+            "tiled8": f"k_stencils_tiled<{nw},{lc},8>", "tiled4": f"k_stencils_tiled<{nw},{lc},4>"}
 
 
 def bbox_rel_err(x, ref):
@@ -223,10 +225,14 @@ def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu, 
     alg = stencil_bytes(n, nnz, nw, L, src_np.shape[0], nw)
     # autotune: a few untimed launches of each kernel variant (record slices
     # with gather groups of 16 / 8 / 4, split-stream slices, CSR with LDS
-    # staging; per-table options), then the timed region runs exactly
-    # `steps` launches of the fastest
+    # staging, locality-ordered slices with control points staged in LDS,
+    # and for one weight array paired records; per-table options),
+    # then the timed region runs exactly `steps` launches of the fastest
     VARIANTS = {"rec_g8": (0, 8, 0), "rec_g4": (0, 4, 0), "rec_g16": (0, 16, 0),
-                "split": (1, 8, 0), "csr_lds": (0, 8, 1)}
+                "split": (1, 8, 0), "csr_lds": (0, 8, 1), "tiled8": (2, 8, 0),
+                "tiled4": (2, 4, 0)}
+    if nw == 1:
+        VARIANTS["rec_pair"] = (3, 16, 0)
     variants = {}
     for vname, (layout, group, kernel) in VARIANTS.items():
         tab = htable

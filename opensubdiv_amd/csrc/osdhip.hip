@@ -62,6 +62,28 @@ static void launch_rec(int lc, int group, const RecArgs &a, int grid, hipStream_
     else launch_rec_g<NWT, 8>(lc, a, grid, st);
 }
 
+template <int G>
+static void launch_rec_paired_g(int lc, const RecArgs &a, int grid, hipStream_t st)
+{
+    switch (lc) {
+    case 1: hipLaunchKernelGGL((k_stencils_rec_paired<1, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((k_stencils_rec_paired<2, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((k_stencils_rec_paired<3, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 4: hipLaunchKernelGGL((k_stencils_rec_paired<4, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 5: hipLaunchKernelGGL((k_stencils_rec_paired<5, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 6: hipLaunchKernelGGL((k_stencils_rec_paired<6, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    case 7: hipLaunchKernelGGL((k_stencils_rec_paired<7, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    default: hipLaunchKernelGGL((k_stencils_rec_paired<8, G>), dim3(grid), dim3(kBlock), 0, st, a); break;
+    }
+}
+
+static void launch_rec_paired(int lc, int group, const RecArgs &a, int grid, hipStream_t st)
+{
+    if (group == 4) launch_rec_paired_g<4>(lc, a, grid, st);
+    else if (group == 16) launch_rec_paired_g<16>(lc, a, grid, st);
+    else launch_rec_paired_g<8>(lc, a, grid, st);
+}
+
 template <int NWT, int G>
 static void launch_tiled_g(int lc, const TiledArgs &a, int grid, hipStream_t st)
 {
@@ -389,7 +411,7 @@ void osdhip_table_options_init(osdhip_table_options *o)
 
 static bool table_options_ok(const osdhip_table_options &o)
 {
-    return o.layout >= 0 && o.layout <= 2 && (o.keepCsr == 0 || o.keepCsr == 1) &&
+    return o.layout >= 0 && o.layout <= 3 && (o.keepCsr == 0 || o.keepCsr == 1) &&
            (o.group == 4 || o.group == 8 || o.group == 16) && (o.kernel == 0 || o.kernel == 1) &&
            !(o.kernel == 1 && !o.keepCsr) && (o.order == 0 || o.order == 1);
 }
@@ -417,6 +439,7 @@ osdhip_stencil_table *osdhip_stencil_table_create_ex(int numStencils, const int 
     if (numStencils < 0 || (numStencils > 0 && (!sizes || !indices || !weights)) ||
         (numWeightArrays != 1 && numWeightArrays != 3 && numWeightArrays != 6))
         return nullptr;
+    if (opt.layout == 3 && numWeightArrays != 1) return nullptr;   // pairs of 8-byte records
     for (int k = 0; k < numWeightArrays; ++k)
         if (numStencils > 0 && !weights[k]) return nullptr;
     osdhip_stencil_table *t = new (std::nothrow) osdhip_stencil_table;
@@ -465,7 +488,7 @@ osdhip_stencil_table *osdhip_stencil_table_create_ex(int numStencils, const int 
         if (ok) {
             const int gs = (S + kBlock - 1) / kBlock, gr = (rows + kBlock - 1) / kBlock;
             hipLaunchKernelGGL(k_slice_widths, dim3(gs), dim3(kBlock), 0, st, t->sizes, numStencils,
-                               S, t->sliceWidth, entries);
+                               S, t->sliceWidth, entries, t->layout == 3 ? 1 : 0);
             hipLaunchKernelGGL(k_pad_sizes, dim3(gr), dim3(kBlock), 0, st, t->sizes, numStencils,
                                rows, t->rowSize);
             ok = hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, entries, t->sliceOff, S, st) ==
@@ -484,7 +507,13 @@ osdhip_stencil_table *osdhip_stencil_table_create_ex(int numStencils, const int 
         t->sliceEntries = lastOff + lastEntries;
         const int gr = (rows + kBlock - 1) / kBlock;
         const float *const *W = t->w;
-        if (ok && t->layout == 0) {
+        if (ok && t->layout == 3) {
+            ok = dalloc(&t->rec, (size_t)t->sliceEntries * 2, t->bytes);
+            if (ok)
+                hipLaunchKernelGGL(k_fill_records_paired, dim3(gr), dim3(kBlock), 0, st, t->sizes,
+                                   t->offsets, t->indices, W[0], numStencils, rows, t->sliceOff,
+                                   t->sliceWidth, t->rec);
+        } else if (ok && t->layout == 0) {
             ok = dalloc(&t->rec, (size_t)t->sliceEntries * rec_dwords(numWeightArrays), t->bytes);
             if (ok) {
                 if (numWeightArrays == 1)
@@ -677,6 +706,30 @@ int osdhip_stencil_table_eval(const osdhip_stencil_table *t, const float *src,
             if (t->nw == 1) launch_rec<1>(lc, t->group, a, grid, st);
             else if (t->nw == 3) launch_rec<3>(lc, t->group, a, grid, st);
             else launch_rec<6>(lc, t->group, a, grid, st);
+            int rc = last_error();
+            if (rc != OSDHIP_OK) return rc;
+        }
+        return OSDHIP_OK;
+    }
+    if (t->layout == 3) {
+        RecArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.srcStride = srcDesc.stride;
+        a.rowSize = t->rowSize;
+        a.sliceOff = t->sliceOff;
+        a.sliceWidth = t->sliceWidth;
+        a.rec = t->rec;
+        a.rowBegin = start;
+        a.rowEnd = end;
+        a.sliceBegin = sliceBegin;
+        a.nslices = nslices;
+        a.nblocks = nblocks;
+        for (int e0 = 0; e0 < srcDesc.length; e0 += 8) {
+            const int lc = std::min(8, srcDesc.length - e0);
+            a.src = src + srcDesc.offset + e0;
+            a.dst[0] = outputs[0] ? outputs[0] + outputDescs[0].offset + e0 : nullptr;
+            a.dstStride[0] = outputs[0] ? outputDescs[0].stride : 0;
+            launch_rec_paired(lc, t->group, a, grid, st);
             int rc = last_error();
             if (rc != OSDHIP_OK) return rc;
         }

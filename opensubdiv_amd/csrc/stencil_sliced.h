@@ -219,4 +219,71 @@ __global__ __launch_bounds__(kBlock) void k_stencils_rec(RecArgs a)
     }
 }
 
+// One weight array, paired records (layout 3): a lane loads the records of
+// entries 2p and 2p + 1 of its row with one 16-byte load, so a wave streams
+// 1 KiB of records per load instruction (the 8-byte records of layout 0
+// give half that).  G entries (G / 2 pair loads) are loaded, then all their
+// control-point gathers are issued; past the width the last pair is
+// re-read and not accumulated.  Same products and order as CpuEvalStencils.
+template <int LC, int G>
+__global__ __launch_bounds__(kBlock) void k_stencils_rec_paired(RecArgs a)
+{
+    static_assert(G % 2 == 0, "pairs");
+    const int lane = (int)threadIdx.x & (kWave - 1);
+    const int lb = xcd_logical_block();
+    if (lb >= a.nblocks) return;
+    const int sl = lb * kWavesPerBlock + wave_in_block();
+    if (sl >= a.nslices) return;                     // whole wave idle
+    const int s = a.sliceBegin + sl;
+    const int row = s * kWave + lane;
+    const bool active = row >= a.rowBegin && row < a.rowEnd;
+    const int sz = active ? a.rowSize[row] : 0;
+    const int W = a.sliceWidth[s];
+    const int P = (W + 1) >> 1;                      // column pairs
+    const unsigned *rp = a.rec + (size_t)(2 * a.sliceOff[s] + 4 * lane);
+
+    float acc[LC];
+#pragma unroll
+    for (int k = 0; k < LC; ++k) acc[k] = 0.0f;
+    const float *__restrict__ src = a.src;
+    const size_t sstride = (size_t)a.srcStride;
+    for (int p0 = 0; p0 < P; p0 += G / 2) {
+        int ii[G];
+        float ww[G];
+#pragma unroll
+        for (int u = 0; u < G / 2; ++u) {
+            const v4i q = __builtin_nontemporal_load(
+                (const v4i *)(rp + (size_t)min(p0 + u, P - 1) * (kWave * 4)));
+            ii[2 * u] = q.x;
+            ww[2 * u] = __int_as_float(q.y);
+            ii[2 * u + 1] = q.z;
+            ww[2 * u + 1] = __int_as_float(q.w);
+        }
+        float v[G][LC];
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const float *pp = src + (size_t)(unsigned)ii[u] * sstride;
+#pragma unroll
+            for (int k = 0; k < LC; ++k) v[u][k] = pp[k];
+        }
+#pragma unroll
+        for (int u = 0; u < G; ++u) {
+            const bool take = 2 * p0 + u < sz;
+#pragma unroll
+            for (int k = 0; k < LC; ++k) {
+                const float t = acc[k] + v[u][k] * ww[u];
+                acc[k] = take ? t : acc[k];
+            }
+        }
+    }
+    if (active) {
+        float *d = a.dst[0];
+        if (d) {
+            d += (size_t)(row - a.rowBegin) * (size_t)a.dstStride[0];
+#pragma unroll
+            for (int k = 0; k < LC; ++k) d[k] = acc[k];
+        }
+    }
+}
+
 }  // namespace osdhip

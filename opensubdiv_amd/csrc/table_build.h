@@ -9,7 +9,8 @@ namespace osdhip {
 
 // Per slice: widest stencil, and 64 * width entries (the scan input).
 __global__ __launch_bounds__(kBlock) void k_slice_widths(const int *sizes, int n, int S,
-                                                         int *width, long long *entries)
+                                                         int *width, long long *entries,
+                                                         int paired = 0)
 {
     const int s = (int)(blockIdx.x * kBlock + threadIdx.x);
     if (s >= S) return;
@@ -17,7 +18,8 @@ __global__ __launch_bounds__(kBlock) void k_slice_widths(const int *sizes, int n
     const int r1 = min(n, (s + 1) * kWave);
     for (int r = s * kWave; r < r1; ++r) w = max(w, sizes[r]);
     width[s] = w;
-    entries[s] = (long long)w * kWave;
+    // paired records: a slice holds whole column pairs
+    entries[s] = (long long)(paired ? w + (w & 1) : w) * kWave;
 }
 
 // Slice-row sizes padded to whole slices (rows past the end: 0).
@@ -56,6 +58,33 @@ __global__ __launch_bounds__(kBlock) void k_fill_records(
         unsigned *d = rec + (size_t)(sliceOff[s] + (long long)j * kWave + lane) * R;
 #pragma unroll
         for (int k = 0; k < R; ++k) d[k] = q[k];
+    }
+}
+
+// Paired records (one weight array): entries 2p and 2p + 1 of a row sit
+// side by side, {index, w, index, w}, so a lane reads both with one 16-byte
+// load; column pair p of a slice holds its 64 rows' pairs contiguously,
+// record (j, lane) at dword 2 * sliceOff + 256 * (j / 2) + 4 * lane +
+// 2 * (j % 2).  An odd width is padded to a whole pair.
+__global__ __launch_bounds__(kBlock) void k_fill_records_paired(
+    const int *sizes, const int *offsets, const int *indices, const float *w0, int n, int rows,
+    const long long *sliceOff, const int *sliceWidth, unsigned *rec)
+{
+    const int r = (int)(blockIdx.x * kBlock + threadIdx.x);
+    if (r >= rows) return;
+    const int s = r / kWave, lane = r % kWave;
+    const int W = sliceWidth[s], W2 = W + (W & 1);
+    const int sz = r < n ? sizes[r] : 0;
+    const long long off = r < n ? (long long)offsets[r] : 0;
+    for (int j = 0; j < W2; ++j) {
+        unsigned q0 = 0u, q1 = 0u;
+        if (j < sz) {
+            q0 = (unsigned)indices[off + j];
+            q1 = __float_as_uint(w0[off + j]);
+        }
+        unsigned *d = rec + (size_t)(2 * sliceOff[s] + 256LL * (j >> 1) + 4 * lane + 2 * (j & 1));
+        d[0] = q0;
+        d[1] = q1;
     }
 }
 
