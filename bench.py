@@ -49,6 +49,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0           # MI355X HBM3E, GB/s
+# MI355X FP32 VALU without packed math: 256 CUs x 4 SIMDs x 16 lanes at
+# 2.4 GHz, one multiply or add per lane per clock (the published 157.3
+# TFLOP/s vector figure counts packed FMAs, 4 flops per lane per clock)
+VALU_PEAK_OPS = 256 * 4 * 16 * 2.4e9
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
 METRIC = "limit points/s + HBM GB/s vs roofline; served p50/p99 latency at fixed QPS"
 
@@ -729,6 +733,8 @@ def run_batched(args, lib, device, copy_gbps=None):
         log(f"[cfg3] tuned in {time.time() - t0:.1f}s: {batch.config()}")
     points = sum(d.points_per_step() for d in dms)
     alg = sum(d.algorithmic_bytes() for d in dms)
+    ops_a = sum(d.spec.frames * d.length * 2 * (d.nnz_refine + d.nnz_points) for d in dms)
+    ops = sum(d.valu_ops() for d in dms)
     wall, ms = time_launches(lambda: batch.launch(s), args.steps, args.warmup, stream)
     kms = float(np.median(ms))
     achieved = alg / (kms * 1e-3) / 1e9
@@ -767,7 +773,8 @@ def run_batched(args, lib, device, copy_gbps=None):
                      "kernel": "one step: k_batch_stencils_tiled (phase A) + k_batch_patches* "
                                "(phase B), graph launches of every stage",
                      "algorithmic_bytes": alg, "kernel_ms": kms,
-                     "output_bytes": out_bytes},
+                     "output_bytes": out_bytes,
+                     "valu": valu_roofline(ops, ops_a, kms, report)},
         "plain": plain,
         "autotune": report, "tuned": batch.config(),
         "parity": parity,
@@ -784,6 +791,24 @@ def run_batched(args, lib, device, copy_gbps=None):
     batch.close()
     del arena, dms
     return res
+
+
+def valu_roofline(ops, ops_a, kms, report):
+    """The step against the FP32 VALU rate: bit-exact sums take a multiply
+    and an add per term (no FMA), and packed FP32 math is not used, so
+    `ops` / VALU_PEAK_OPS is the step's arithmetic floor.  Per phase from
+    the autotuner's best phase times when it ran."""
+    out = {"ops": ops, "unit": "unfused FP32 multiplies and adds (lane ops)",
+           "peak_ops_per_s": VALU_PEAK_OPS, "floor_ms": ops / VALU_PEAK_OPS * 1e3,
+           "frac": ops / (kms * 1e-3) / VALU_PEAK_OPS}
+    if report:
+        for ph, n in (("phase_a", ops_a), ("phase_b", ops - ops_a)):
+            runs = report.get(ph) or []
+            if runs:
+                best = min(r["ms"] for r in runs)
+                out[ph] = {"ops": n, "best_ms": best, "floor_ms": n / VALU_PEAK_OPS * 1e3,
+                           "frac": n / (best * 1e-3) / VALU_PEAK_OPS}
+    return out
 
 
 def run_served(args, lib, device):
@@ -1024,8 +1049,9 @@ def main():
                          "rest are compiled and stored")
     ap.add_argument("--table-layout", type=int, default=2, choices=[0, 2],
                     help="cfg3/cfg4: stencil table layout (0 record slices, 2 locality-ordered)")
-    ap.add_argument("--table-order", type=int, default=0, choices=[0, 1],
-                    help="cfg3: layout-2 row order (0 by dominant control vertex, 1 Far's order)")
+    ap.add_argument("--table-order", type=int, default=2, choices=[0, 1, 2],
+                    help="cfg3: layout-2 row order (2 by dominant control vertex, 0 then by "
+                         "size within row windows, 1 Far's order)")
     ap.add_argument("--config", default=None,
                     help="cfg3: batch knobs key=value,... set before (or instead of) tuning")
     ap.add_argument("--no-tune", action="store_true", help="cfg3: skip the autotuner")

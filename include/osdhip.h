@@ -145,11 +145,17 @@ osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *si
  *            or 16 (layout 2 uses 4, or 8 for 8 and 16).
  *   kernel:  osdhip_stencil_table_eval kernel: 0 = slice layout (default),
  *            1 = CSR with LDS staging (needs keepCsr).
- *   order:   layout 2's row order: 0 = by dominant control vertex (default;
- *            the fewest control vertices per slice), 1 = Far's row order
- *            (a slice's outputs are 64 consecutive rows; slices with more
- *            distinct control vertices than the LDS list holds gather
- *            them from memory). */
+ *   order:   layout 2's row order: 0 = by dominant control vertex, then
+ *            by row size within windows of 128 or 512 rows when that
+ *            leaves fewer padded record entries and no fewer slices
+ *            listing their control vertices (default: the fewest entries
+ *            streamed per evaluation); 2 = by dominant control vertex only
+ *            (the fewest control vertices per slice: for many-frame
+ *            batches, which re-read records from cache but stage control
+ *            vertices per frame chunk); 1 = Far's row order (a slice's
+ *            outputs are 64 consecutive rows; slices with more distinct
+ *            control vertices than the LDS list holds gather them from
+ *            memory). */
 typedef struct {
     int layout;
     int keepCsr;
@@ -185,6 +191,13 @@ int osdhip_stencil_table_info(const osdhip_stencil_table *table, int *numStencil
  * counterpart in the reference: a diagnostic of this library's layouts. */
 int osdhip_stencil_table_layout(const osdhip_stencil_table *table, int *layout, int *numSlices,
                                 int *listedSlices, long long *listEntries);
+/* Record entries the table's slices hold (64 x the widest row of every
+ * slice: the entries each evaluation streams, padding included) and, for
+ * layout 2 with rows by dominant vertex, the window (0, 128 or 512 rows)
+ * within which rows were regrouped by size to make a slice's rows alike.
+ * No counterpart in the reference: a diagnostic of this library's layouts. */
+int osdhip_stencil_table_slices(const osdhip_stencil_table *table, long long *recordEntries,
+                                int *sizeWindow);
 /* DEVICE pointers of the Far-layout arrays; weights receives 6 pointers
  * (NULL for arrays the table does not have). */
 int osdhip_stencil_table_csr(const osdhip_stencil_table *table, const int **sizes,

@@ -82,6 +82,7 @@ _HIP_SIGS = {
     "osdhip_stencil_table_info": [_vp, _vp, _vp, _vp, _vp],
     "osdhip_stencil_table_csr": [_vp, _vp, _vp, _vp, _vp],
     "osdhip_stencil_table_layout": [_vp, _vp, _vp, _vp, _vp],
+    "osdhip_stencil_table_slices": [_vp, _vp, _vp],
     "osdhip_limit_normals": [_vp, _D, _vp, _D, _vp, _D, _i, _i, _vp],
     "osdhip_patch_map_create": [_vp, _i, _vp, _i, _vp],
     "osdhip_patch_map_find": [_vp, _i, _vp, _vp, _vp, _vp, _vp],

@@ -171,8 +171,9 @@ void launchB(const PhaseB &L, bool staged, hipStream_t st)
 {
     if (staged) {
         launchKind(L, kB_S128, k_batch_patches_regular<LC, FC, kStagedOnly, 128>, st);
-        // 256 slots at most 2 frames per chunk (LDS: 4 waves x FC x LC x 256 floats)
-        launchKind(L, kB_S256, k_batch_patches_regular<LC, (FC > 2 ? 2 : FC), kStagedOnly, 256>, st);
+        // 256 slots one frame per chunk (LDS: 4 waves x FC x LC x 256 floats,
+        // 28 KB at LC 7, so LDS allows as many blocks per CU as registers do)
+        launchKind(L, kB_S256, k_batch_patches_regular<LC, 1, kStagedOnly, 256>, st);
         launchKind(L, kB_Unl, k_batch_patches_regular<LC, FC, kGatherUnlisted>, st);
     } else {
         launchKind(L, kB_All, k_batch_patches_regular<LC, FC, kGatherAll>, st);

@@ -279,8 +279,8 @@ static long long exclusive_scan(const T *counts, T *out, int n, hipStream_t st)
 }
 
 // Layout 2 from the uploaded CSR arrays: dominant-vertex row order (stable
-// radix sort), per-slice widths and control lists (count pass, scans, fill
-// pass).
+// radix sort), optionally regrouped by row size within windows of 128 or 512
+// rows, per-slice widths and control lists (count pass, scans, fill pass).
 static bool build_tiled(osdhip_stencil_table *t, hipStream_t st)
 {
     const int n = t->n, S = t->nslices, rows = S * kWave;
@@ -299,33 +299,72 @@ static bool build_tiled(osdhip_stencil_table *t, hipStream_t st)
               dalloc(&t->nuniq, (size_t)S, t->bytes) && dalloc(&t->ctrlOff, (size_t)S, t->bytes);
     if (ok) {
         hipLaunchKernelGGL(k_row_dominant, dim3(gr), dim3(kBlock), 0, st, t->sizes, t->offsets,
-                           t->indices, W[0], n, rows, t->order, keys, vals);
+                           t->indices, W[0], n, rows, t->order == 1 ? 1 : 0, keys, vals);
         ok = hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, keys, keysOut, vals, t->perm, rows,
                                                 0, 32, st) == hipSuccess &&
              hipMalloc(&sortTmp, std::max<size_t>(sortBytes, 1)) == hipSuccess &&
              hipcub::DeviceRadixSort::SortPairs(sortTmp, sortBytes, keys, keysOut, vals, t->perm,
                                                 rows, 0, 32, st) == hipSuccess;
     }
-#define OSDHIP_TILED(NWT, FILL)                                                                 \
-    hipLaunchKernelGGL((k_tiled_slices<NWT, FILL>), dim3(gs), dim3(kBlock), 0, st, t->perm,     \
+#define OSDHIP_TILED(NWT, FILL, PERM)                                                           \
+    hipLaunchKernelGGL((k_tiled_slices<NWT, FILL>), dim3(gs), dim3(kBlock), 0, st, PERM,        \
                        t->sizes, t->offsets, t->indices, W[0], W[1], W[2], W[3], W[4], W[5], n, \
                        S, t->sliceWidth, entries, t->nuniq, listLen, t->sliceOff, t->ctrlOff,   \
                        t->ctrl, t->rec)
-    if (ok) {
-        if (t->nw == 1) OSDHIP_TILED(1, false);
-        else if (t->nw == 3) OSDHIP_TILED(3, false);
-        else OSDHIP_TILED(6, false);
-        ok = hipGetLastError() == hipSuccess;
-    }
-    if (ok) {
+    // count pass over a row order: slice widths and entries, control-list
+    // counts; `listed` receives the slices that list their control vertices
+    auto countPass = [&](const int *perm, int &listed) -> bool {
+        if (t->nw == 1) OSDHIP_TILED(1, false, perm);
+        else if (t->nw == 3) OSDHIP_TILED(3, false, perm);
+        else OSDHIP_TILED(6, false, perm);
         std::unique_ptr<int[]> nu(new (std::nothrow) int[(size_t)S]);
-        ok = nu && hipMemcpyAsync(nu.get(), t->nuniq, (size_t)S * sizeof(int),
-                                  hipMemcpyDeviceToHost, st) == hipSuccess &&
-             hipStreamSynchronize(st) == hipSuccess;
-        if (ok)
-            t->listedSlices = (int)std::count_if(nu.get(), nu.get() + S,
-                                                 [](int u) { return u >= 0; });
+        const bool good = hipGetLastError() == hipSuccess && nu &&
+                          hipMemcpyAsync(nu.get(), t->nuniq, (size_t)S * sizeof(int),
+                                         hipMemcpyDeviceToHost, st) == hipSuccess &&
+                          hipStreamSynchronize(st) == hipSuccess;
+        if (good)
+            listed = (int)std::count_if(nu.get(), nu.get() + S, [](int u) { return u >= 0; });
+        return good;
+    };
+    int listed = 0;
+    if (ok) ok = countPass(t->perm, listed);
+    // Size windows (order 0): of the dominant order
+    // and its regroupings by size within 128- and 512-row windows, keep the
+    // one whose slices hold the fewest record entries among those that list
+    // as many slices as the dominant order.  The winner is kept in keysOut
+    // (free after the sort), candidates are built in vals.
+    t->sizeWindow = 0;
+    if (ok && t->order == 0 && S > 1) {
+        long long best = exclusive_scan(entries, t->sliceOff, S, st);
+        ok = best >= 0;
+        int *cand = vals, *keep = (int *)keysOut;
+        const int base = listed;
+        for (int K : { 128, 512 }) {
+            if (!ok) break;
+            const long long nwin = ((long long)rows + K - 1) / K;
+            if (K == 128)
+                hipLaunchKernelGGL(k_window_size_sort<128>, dim3((unsigned)nwin), dim3(kBlock), 0, st,
+                                   t->perm, t->sizes, n, (long long)rows, cand);
+            else
+                hipLaunchKernelGGL(k_window_size_sort<512>, dim3((unsigned)nwin), dim3(kBlock), 0, st,
+                                   t->perm, t->sizes, n, (long long)rows, cand);
+            int l = 0;
+            ok = hipGetLastError() == hipSuccess && countPass(cand, l);
+            const long long e = ok ? exclusive_scan(entries, t->sliceOff, S, st) : -1;
+            ok = ok && e >= 0;
+            if (ok && l >= base && e < best) {
+                best = e;
+                t->sizeWindow = K;
+                ok = hipMemcpyAsync(keep, cand, (size_t)rows * sizeof(int), hipMemcpyDeviceToDevice,
+                                    st) == hipSuccess;
+            }
+        }
+        if (ok && t->sizeWindow)
+            ok = hipMemcpyAsync(t->perm, keep, (size_t)rows * sizeof(int), hipMemcpyDeviceToDevice,
+                                st) == hipSuccess;
+        if (ok) ok = countPass(t->perm, listed);      // the counts of the kept order
     }
+    if (ok) t->listedSlices = listed;
     if (ok) {
         t->sliceEntries = exclusive_scan(entries, t->sliceOff, S, st);
         t->ctrlLen = exclusive_scan(listLen, t->ctrlOff, S, st);
@@ -334,9 +373,9 @@ static bool build_tiled(osdhip_stencil_table *t, hipStream_t st)
              dalloc(&t->ctrl, (size_t)t->ctrlLen, t->bytes);
     }
     if (ok) {
-        if (t->nw == 1) OSDHIP_TILED(1, true);
-        else if (t->nw == 3) OSDHIP_TILED(3, true);
-        else OSDHIP_TILED(6, true);
+        if (t->nw == 1) OSDHIP_TILED(1, true, t->perm);
+        else if (t->nw == 3) OSDHIP_TILED(3, true, t->perm);
+        else OSDHIP_TILED(6, true, t->perm);
         ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(st) == hipSuccess;
     }
 #undef OSDHIP_TILED
@@ -413,7 +452,7 @@ static bool table_options_ok(const osdhip_table_options &o)
 {
     return o.layout >= 0 && o.layout <= 3 && (o.keepCsr == 0 || o.keepCsr == 1) &&
            (o.group == 4 || o.group == 8 || o.group == 16) && (o.kernel == 0 || o.kernel == 1) &&
-           !(o.kernel == 1 && !o.keepCsr) && (o.order == 0 || o.order == 1);
+           !(o.kernel == 1 && !o.keepCsr) && (o.order >= 0 && o.order <= 2);
 }
 
 osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *sizes,
@@ -604,6 +643,15 @@ int osdhip_stencil_table_layout(const osdhip_stencil_table *t, int *layout, int 
     if (numSlices) *numSlices = t->nslices;
     if (listedSlices) *listedSlices = t->listedSlices;
     if (listEntries) *listEntries = t->ctrlLen;
+    return OSDHIP_OK;
+}
+
+int osdhip_stencil_table_slices(const osdhip_stencil_table *t, long long *recordEntries,
+                                int *sizeWindow)
+{
+    if (!t) return OSDHIP_ERR_ARGS;
+    if (recordEntries) *recordEntries = t->sliceEntries;
+    if (sizeWindow) *sizeWindow = t->sizeWindow;
     return OSDHIP_OK;
 }
 

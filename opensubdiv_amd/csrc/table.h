@@ -32,6 +32,7 @@ struct osdhip_stencil_table {
     int *ctrl = nullptr;
     long long ctrlLen = 0;
     int listedSlices = 0;
+    int sizeWindow = 0;              // layout 2, order 0: rows sorted by size per window (0: none)
 };
 
 namespace osdhip {

@@ -154,6 +154,45 @@ __global__ __launch_bounds__(kBlock) void k_row_dominant(const int *sizes, const
     vals[r] = r;
 }
 
+// Rows of the dominant-vertex order regrouped by size within windows of K
+// consecutive rows (one block per window): a slice costs its widest row in
+// every lane, and rows of one neighbourhood differ in size (9, 12, 16
+// entries at a regular vertex, edge and face), so sorting each window's rows
+// by size -- stable, by counting -- makes a slice's rows alike while its
+// rows still come from a few neighbouring control vertices.  Rows past n
+// (slice padding) stay after every real row.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_window_size_sort(const int *in, const int *sizes,
+                                                             int n, long long rows, int *out)
+{
+    __shared__ unsigned char key[K];
+    __shared__ int start[65];
+    const long long w0 = (long long)blockIdx.x * K;
+    const int m = (int)(rows - w0 < K ? rows - w0 : K);
+    for (int b = (int)threadIdx.x; b < 65; b += kBlock) start[b] = 0;
+    for (int i = (int)threadIdx.x; i < m; i += kBlock) {
+        const int r = in[w0 + i];
+        key[i] = (unsigned char)(r >= 0 && r < n ? min(sizes[r], 63) : 64);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {                    // counts, then their exclusive sums
+        for (int i = 0; i < m; ++i) ++start[key[i]];
+        int s = 0;
+        for (int b = 0; b < 65; ++b) {
+            const int c = start[b];
+            start[b] = s;
+            s += c;
+        }
+    }
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < m; i += kBlock) {
+        const int k = key[i];
+        int before = 0;
+        for (int j = 0; j < i; ++j) before += key[j] == k;
+        out[w0 + start[k] + before] = in[w0 + i];
+    }
+}
+
 // One wave per slice of the ordered rows perm[].  Count pass (FILL false):
 // slice width, record entries (64 * width), distinct control vertices
 // (nuniq = -1 when more than kTileZero) and list length.  Fill pass: the

@@ -123,7 +123,9 @@ class HipStencilTable:
     in LDS); ``keep_csr`` keeps the Far CSR arrays on the device; ``group``
     (4, 8, 16) and ``kernel`` (0 slices, 1 CSR) select the evaluation
     kernel and can be changed later with ``set``; ``order`` (layout 2) 0
-    groups rows by dominant control vertex, 1 keeps Far's row order."""
+    groups rows by dominant control vertex and then by size within row
+    windows when that streams fewer padded entries, 2 by dominant control
+    vertex only, 1 keeps Far's row order."""
 
     _KEYS = ["weights", "du", "dv", "duu", "duv", "dvv"]
     _GETTERS = ["GetWeights", "GetDuWeights", "GetDvWeights", "GetDuuWeights",
@@ -222,12 +224,19 @@ class HipStencilTable:
         return b.value
 
     def layout(self):
-        """Device layout: dict(layout, slices, listed_slices, list_entries)."""
+        """Device layout: dict(layout, slices, listed_slices, list_entries,
+        record_entries, size_window)."""
         import ctypes
         v = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong()]
-        _lib.check(_lib.load_hip().osdhip_stencil_table_layout(self.h, *[ctypes.byref(x) for x in v]),
+        lib = _lib.load_hip()
+        _lib.check(lib.osdhip_stencil_table_layout(self.h, *[ctypes.byref(x) for x in v]),
                    "osdhip_stencil_table_layout")
-        return dict(zip(("layout", "slices", "listed_slices", "list_entries"), (x.value for x in v)))
+        e, wdw = ctypes.c_longlong(), ctypes.c_int()
+        _lib.check(lib.osdhip_stencil_table_slices(self.h, ctypes.byref(e), ctypes.byref(wdw)),
+                   "osdhip_stencil_table_slices")
+        out = dict(zip(("layout", "slices", "listed_slices", "list_entries"), (x.value for x in v)))
+        out.update(record_entries=e.value, size_window=wdw.value)
+        return out
 
     def __del__(self):
         if getattr(self, "h", None) and _lib._hip is not None:

@@ -182,7 +182,7 @@ class DeviceMesh:
     """One mesh's device data and its batch segments (see module docstring)."""
 
     def __init__(self, h: HostMesh, stream=None, own_outputs=True, layout=2, keep_csr=False,
-                 order=0):
+                 order=2):
         import torch
         from .osd import HipStencilTable, _device_tensor
         s = h.spec
@@ -195,6 +195,7 @@ class DeviceMesh:
         self.nnz_refine = int(h.refine.nnz)
         self.nnz_points = int(len(h.points["indices"]))
         self.ncoords = int(len(h.coords))
+        self.coord_points = coord_point_count(h.arrays, h.params, h.coords)
         self.npatches = int(len(h.params))
         self.npatch_indices = int(len(h.indices))
         dev = "cuda"
@@ -208,8 +209,10 @@ class DeviceMesh:
         if own_outputs:
             self.bind_outputs(torch.empty(self.output_floats(), dtype=torch.float32, device=dev))
         # stencil tables in slice layout `layout` (0 record slices, 2
-        # locality-ordered, rows by dominant vertex (order 0) or in Far's
-        # order (order 1)); the batch kernels need no CSR arrays
+        # locality-ordered, rows by dominant vertex (order 2: a slice's
+        # control vertices are staged once per frame chunk, so fewer of them
+        # beats the fewer padded entries of order 0's size windows), or in
+        # Far's order (order 1)); the batch kernels need no CSR arrays
         self.refine_table = HipStencilTable(h.refine, stream, layout=layout, keep_csr=keep_csr,
                                             order=order)
         self.point_table = HipStencilTable(h.points, stream, layout=layout, keep_csr=keep_csr,
@@ -262,6 +265,16 @@ class DeviceMesh:
         """Stencil outputs (refined vertices) and PatchCoords, per frame."""
         return self.spec.frames * (self.nrefine + self.ncoords)
 
+    def valu_ops(self):
+        """Floating-point operations one step cannot avoid when every sum is
+        formed as the reference forms it -- a rounded product, then a
+        rounded add, per term, so no fused multiply-add: per frame and
+        float, a multiply and an add per stencil entry (refinement and
+        patch-point tables) and three of each per patch point of every
+        PatchCoord (P, dP/du, dP/dv)."""
+        F, L = self.spec.frames, self.length
+        return F * L * (2 * (self.nnz_refine + self.nnz_points) + 6 * self.coord_points)
+
     def algorithmic_bytes(self):
         """Minimum HBM traffic of one step: tables, coords and patch tables
         once (frames are extra columns); per frame, control points read and
@@ -274,6 +287,25 @@ class DeviceMesh:
         per_frame = (e * (self.nctrl + self.nrefine) + e * (self.nctrl + self.npts) +
                      e * (self.nctrl + self.npts) + 3 * e * self.ncoords)
         return once + F * per_frame
+
+
+# control points of each patch type (Far::PatchDescriptor::GetNumControlVertices)
+_TYPE_POINTS = {3: 4, 4: 3, 5: 12, 6: 16, 9: 20, 10: 18}
+
+
+def coord_point_count(arrays, params, coords):
+    """Control points summed over PatchCoords: each coord's patch type is
+    its array's descriptor, or the array's regular descriptor when the
+    patch's param says regular (bit 5 of field1)."""
+    if len(coords) == 0:
+        return 0
+    a = arrays[coords["arrayIndex"]]
+    f1 = params["field1"][coords["patchIndex"]].astype(np.int64)
+    ptype = np.where((f1 >> 5) & 1, a["regDesc"], a["desc"])
+    table = np.zeros(16, np.int64)
+    for t, n in _TYPE_POINTS.items():
+        table[t] = n
+    return int(table[ptype].sum())
 
 
 def stage_outputs(dms, arena_floats):

@@ -68,9 +68,11 @@ def same(a, b, what):
 
 # (floats per vertex, table layout, row order): record slices and
 # locality-ordered slices (LDS-staged control points), rows by dominant
-# vertex or in Far's order
-@pytest.fixture(scope="module", params=[(7, 0, 0), (3, 0, 0), (7, 2, 0), (3, 2, 0), (7, 2, 1)],
-                ids=["L7-rec", "L3-rec", "L7-tiled", "L3-tiled", "L7-tiledfar"])
+# vertex and size windows (0), in Far's order (1) or by dominant vertex
+# only (2)
+@pytest.fixture(scope="module",
+                params=[(7, 0, 0), (3, 0, 0), (7, 2, 0), (3, 2, 0), (7, 2, 1), (7, 2, 2)],
+                ids=["L7-rec", "L3-rec", "L7-tiled", "L3-tiled", "L7-tiledfar", "L7-tileddom"])
 def workload(request):
     length, layout, order = request.param
     hosts, dms = build(length, layout, order)

@@ -366,6 +366,74 @@ def test_tiled_layout_mesh_tables_stage_in_lds(hip, group, order):
         np.testing.assert_array_equal(host(outs[k], shape=(n, 3)), g[name], err_msg=name)
 
 
+def _dominant_order_slices(t):
+    """Host model of the dominant-vertex row order (stable by row; empty
+    rows last): the record entries (64 x each slice's widest row) and the
+    slices with at most 63 distinct control vertices."""
+    sizes, offs, idx, w = (np.asarray(t[k]) for k in ("sizes", "offsets", "indices", "weights"))
+    n = len(sizes)
+    dom = np.full(n, 2 ** 32 - 1, np.int64)
+    for r in np.nonzero(sizes)[0]:
+        o = offs[r]
+        dom[r] = idx[o + int(np.argmax(np.abs(w[o:o + sizes[r]])))]
+    perm = np.argsort(dom, kind="stable")
+    S = (n + 63) // 64
+    entries, listed = 0, 0
+    for s in range(S):
+        rows = perm[64 * s:64 * s + 64]
+        entries += 64 * int(sizes[rows].max())
+        u = set()
+        for r in rows:
+            u.update(idx[offs[r]:offs[r] + sizes[r]].tolist())
+        listed += len(u) <= 63
+    return entries, listed
+
+
+@pytest.mark.parametrize("n,level", [(12, 2), (8, 3), (6, 4)])
+def test_tiled_rows_regrouped_by_size(hip, n, level):
+    """Rows by dominant vertex, regrouped by size within 128- or 512-row
+    windows when that leaves fewer record entries and no fewer listed
+    slices: the device layout never streams more entries or lists fewer
+    slices than the plain dominant order (modelled on the host), and the
+    level-3 and level-4 refinement tables take a window; outputs equal the
+    oracle's CpuEvaluator-order sums bit for bit."""
+    from opensubdiv_amd import far, meshes
+    m = meshes.cube_sphere(n, seed=3, radial_noise=0.01, crease_fraction=0.05, sharpness_max=4.0)
+    topo = far.Topology(m)
+    topo.refine_uniform(level)
+    t = topo.stencil_table()
+    table = _tiled_table(hip, {k: t[k] for k in ("sizes", "offsets", "indices", "weights")})
+    info = table.layout()
+    entries, listed = _dominant_order_slices(t)
+    assert info["size_window"] in (0, 128, 512)
+    assert info["record_entries"] <= entries and info["listed_slices"] >= listed
+    if info["size_window"] == 0:
+        assert info["record_entries"] == entries
+    if level >= 3:
+        assert info["size_window"] > 0 and info["record_entries"] < entries
+    nr = len(t["sizes"])
+    rng = np.random.default_rng(40 + level)
+    ctrl = rng.standard_normal((m.nverts, 7)).astype(np.float32)
+    out = empty(nr * 7)
+    assert table_eval(hip, table.h, dev(ctrl.reshape(-1)), (0, 7, 7), [out], [(0, 7, 7)], 0,
+                      nr) == 0
+    want = oracle.apply_stencils(ctrl, t["sizes"], t["offsets"], t["indices"], t["weights"])
+    np.testing.assert_array_equal(host(out, shape=(nr, 7)), want)
+    # Far's order keeps its rows, order 2 the dominant order: no window
+    far_order = _tiled_table(hip, {k: t[k] for k in ("sizes", "offsets", "indices", "weights")},
+                             order=1)
+    assert far_order.layout()["size_window"] == 0
+    dom = _tiled_table(hip, {k: t[k] for k in ("sizes", "offsets", "indices", "weights")},
+                       order=2)
+    info = dom.layout()
+    assert info["size_window"] == 0
+    assert (info["record_entries"], info["listed_slices"]) == (entries, listed)
+    out = empty(nr * 7)
+    assert table_eval(hip, dom.h, dev(ctrl.reshape(-1)), (0, 7, 7), [out], [(0, 7, 7)], 0,
+                      nr) == 0
+    np.testing.assert_array_equal(host(out, shape=(nr, 7)), want)
+
+
 @pytest.mark.parametrize("nw,L", [(1, 3), (1, 8), (3, 5), (6, 2)])
 def test_tiled_layout_random_tables_both_paths(hip, nw, L):
     """Random tables over 40 control vertices (every slice listed, padding

@@ -57,3 +57,24 @@ def test_compile_mesh_outputs():
         assert np.abs(d).max() <= 0.2 + 1e-6 and math.isfinite(float(d.sum()))
     finally:
         h.release()
+
+
+def test_coord_point_count_follows_patch_types():
+    """Control points per PatchCoord: 16 for regular B-spline patches, 20
+    for Gregory-basis end caps (the valence-3 cube corners), as the patch
+    table's own per-patch index stride says."""
+    s = MeshSpec(0, 3, 2, 1, 4000, 9)
+    h = workloads.compile_mesh(s, length=3)
+    try:
+        n = workloads.coord_point_count(h.arrays, h.params, h.coords)
+        strides = h.arrays["stride"][h.coords["arrayIndex"]]
+        f1 = h.params["field1"][h.coords["patchIndex"]].astype(np.int64)
+        regular = ((f1 >> 5) & 1).astype(bool)
+        # a patch array's stride is its descriptor's point count; a patch
+        # flagged regular inside another array evaluates 16 points
+        want = int(np.where(regular, 16, strides).sum())
+        assert n == want
+        assert 16 * len(h.coords) < n < 20 * len(h.coords)
+        assert workloads.coord_point_count(h.arrays, h.params, h.coords[:0]) == 0
+    finally:
+        h.release()
