@@ -187,7 +187,8 @@ def kernel_labels(nw, lc):
             "rec_g16": f"k_stencils_rec<{nw},{lc},16>",
             "split": f"k_stencils_sliced<{nw},{lc}>", "csr_lds": f"k_stencils_csr<{nw},{lc}>",
             "rec_pair": f"k_stencils_rec_paired<{lc},16>",
-            "tiled8": f"k_stencils_tiled<{nw},{lc},8>", "tiled4": f"k_stencils_tiled<{nw},{lc},4>"}
+            "tiled8": f"k_stencils_tiled<{nw},{lc},8>", "tiled4": f"k_stencils_tiled<{nw},{lc},4>",
+            "tiled4_dom": f"k_stencils_tiled<{nw},{lc},4> (rows by dominant vertex only)"}
 
 
 def bbox_rel_err(x, ref):
@@ -232,16 +233,21 @@ def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu, 
     # staging, locality-ordered slices with control points staged in LDS,
     # and for one weight array paired records; per-table options),
     # then the timed region runs exactly `steps` launches of the fastest
-    VARIANTS = {"rec_g8": (0, 8, 0), "rec_g4": (0, 4, 0), "rec_g16": (0, 16, 0),
-                "split": (1, 8, 0), "csr_lds": (0, 8, 1), "tiled8": (2, 8, 0),
-                "tiled4": (2, 4, 0)}
+    # (layout, group, kernel, layout-2 row order)
+    VARIANTS = {"rec_g8": (0, 8, 0, 0), "rec_g4": (0, 4, 0, 0), "rec_g16": (0, 16, 0, 0),
+                "split": (1, 8, 0, 0), "csr_lds": (0, 8, 1, 0), "tiled8": (2, 8, 0, 0),
+                "tiled4": (2, 4, 0, 0), "tiled4_dom": (2, 4, 0, 2)}
     if nw == 1:
-        VARIANTS["rec_pair"] = (3, 16, 0)
+        VARIANTS["rec_pair"] = (3, 16, 0, 0)
+    if args.variant:                      # one variant only (profiling runs)
+        if args.variant not in VARIANTS:
+            raise SystemExit(f"--variant {args.variant}: not one of {sorted(VARIANTS)}")
+        VARIANTS = {args.variant: VARIANTS[args.variant]}
     variants = {}
-    for vname, (layout, group, kernel) in VARIANTS.items():
+    for vname, (layout, group, kernel, order) in VARIANTS.items():
         tab = htable
         if layout != 0:
-            tab = HipStencilTable.Create(table_np, s, layout=layout, group=group)
+            tab = HipStencilTable.Create(table_np, s, layout=layout, group=group, order=order)
         else:
             htable.set("group", group)
             htable.set("kernel", kernel)
@@ -249,14 +255,19 @@ def run_stencils(args, lib, name, table_np, src_np, nw, kernel_label, with_cpu, 
         _, ms = time_launches(fn, 5, 2, stream)
         avg = float(np.median(ms))
         variants[vname] = {"kernel_ms": avg, "GBps": alg / (avg * 1e-3) / 1e9}
+        if layout == 2:
+            info = tab.layout()
+            variants[vname].update(record_entries=info["record_entries"],
+                                   size_window=info["size_window"],
+                                   listed_slices=info["listed_slices"], slices=info["slices"])
         if tab is not htable:
             del tab, fn
             torch.cuda.synchronize()
     best = min(variants, key=lambda k: variants[k]["kernel_ms"])
-    layout, group, kernel = VARIANTS[best]
+    layout, group, kernel, order = VARIANTS[best]
     if layout != 0:
         del htable
-        htable = HipStencilTable.Create(table_np, s, layout=layout, group=group)
+        htable = HipStencilTable.Create(table_np, s, layout=layout, group=group, order=order)
     else:
         htable.set("group", group)
         htable.set("kernel", kernel)
@@ -1055,6 +1066,8 @@ def main():
     ap.add_argument("--config", default=None,
                     help="cfg3: batch knobs key=value,... set before (or instead of) tuning")
     ap.add_argument("--no-tune", action="store_true", help="cfg3: skip the autotuner")
+    ap.add_argument("--variant", default=None,
+                    help="cfg1 / cfg1_refine: time this table variant only (e.g. tiled4)")
     ap.add_argument("--no-parity", action="store_true", help="cfg3: skip the parity checks")
     ap.add_argument("--no-plain", action="store_true",
                     help="cfg3: skip timing the one-mesh-per-launch path")
