@@ -997,12 +997,16 @@ def measure_copy(steps=10, warmup=2):
     kms = float(np.median(ms))
     _, rms = time_launches(lambda: a.sum(), steps, warmup, stream)
     rkms = float(np.median(rms))
+    _, wms = time_launches(lambda: b.fill_(1.0), steps, warmup, stream)
+    wkms = float(np.median(wms))
     del a, b
     torch.cuda.empty_cache()
     return {"bytes_read": 4 * n, "bytes_written": 4 * n, "kernel_ms": kms,
             "GBps": 8 * n / (kms * 1e-3) / 1e9,
             "read_probe": {"bytes_read": 4 * n, "kernel_ms": rkms,
-                           "GBps": 4 * n / (rkms * 1e-3) / 1e9, "op": "torch.sum"}}
+                           "GBps": 4 * n / (rkms * 1e-3) / 1e9, "op": "torch.sum"},
+            "write_probe": {"bytes_written": 4 * n, "kernel_ms": wkms,
+                            "GBps": 4 * n / (wkms * 1e-3) / 1e9, "op": "torch.fill_"}}
 
 
 def copy_probe(args):
