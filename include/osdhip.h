@@ -133,8 +133,9 @@ osdhip_stencil_table *osdhip_stencil_table_create(int numStencils, const int *si
  *   layout:  0 = one interleaved record {index, weights} per entry in 64-row
  *            slices (default); 1 = index and weights in separate streams;
  *            2 = locality-ordered slices: rows grouped by their dominant
- *            control vertex, each slice's distinct control vertices staged
- *            in LDS (for multi-frame batches and large refinement tables);
+ *            control vertex (see order), each slice's distinct control
+ *            vertices staged in LDS (for multi-frame batches and large
+ *            refinement tables);
  *            3 = paired records for tables of one weight array: a row's
  *            entries 2p and 2p + 1 side by side, read with one 16-byte load
  *            (NULL for tables of 3 or 6 weight arrays; not for batches).

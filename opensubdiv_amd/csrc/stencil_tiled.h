@@ -1,7 +1,8 @@
 // EvalStencils on locality-ordered slices (HipStencilTable layout 2).
 //
 // The table's rows are re-ordered so that each 64-row slice surrounds a few
-// control vertices (rows sorted by their dominant control vertex), and each
+// control vertices (rows sorted by their dominant control vertex, possibly
+// regrouped by size within small windows of that order), and each
 // slice lists its distinct control vertices.  A wavefront evaluating a slice
 // first copies those vertices' primvars into LDS -- one vertex per lane,
 // element k of slot u at lds[k][u], so a gather of 64 different slots reads

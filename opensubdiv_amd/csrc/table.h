@@ -12,7 +12,8 @@ struct osdhip_stencil_table {
     int keepCsr = 1;                 // the Far CSR arrays stay on the device
     int group = 8;                   // slice kernels: entries per gather group
     int kernel = 0;                  // table eval: 0 slice layout, 1 CSR
-    int order = 0;                   // layout 2 rows: 0 dominant vertex, 1 Far order
+    int order = 0;                   // layout 2 rows: 0 dominant vertex + size windows,
+                                     // 1 Far order, 2 dominant vertex only
     long long nnz = 0, sliceEntries = 0, bytes = 0;
     int *sizes = nullptr, *offsets = nullptr, *indices = nullptr;
     float *w[6] = {};

@@ -376,7 +376,8 @@ def poisson_requests(n, rate, ntopo, seed, limit_fraction=0.5, nframes=8):
 def residents_from_hosts(hosts, stream=None, layout=2, keep_csr=False):
     """Resident topologies from compiled workloads.HostMesh objects (their
     `frames` become the frame pool requests cycle through); stencil tables
-    in slice layout `layout` (0 or 2), with or without their CSR arrays."""
+    in slice layout `layout` (0 or 2; layout 2 in its default row order,
+    which suits one frame per request), with or without their CSR arrays."""
     from .osd import HipStencilTable, _device_tensor
     out = []
     for h in hosts:

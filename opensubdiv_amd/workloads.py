@@ -332,7 +332,7 @@ def stage_outputs(dms, arena_floats):
 
 
 def build_device_meshes(specs, length=7, stream=None, workers=8, log=None, own_outputs=True,
-                        cache=None, layout=2, order=0):
+                        cache=None, layout=2, order=2):
     """Compiles the meshes on `workers` host threads (largest first), or
     loads them from a table_cache.TableCache, and uploads each as it
     completes (tables in slice layout `layout`, row order `order`); returns
