@@ -419,6 +419,13 @@ def test_tiled_rows_regrouped_by_size(hip, n, level):
                       nr) == 0
     want = oracle.apply_stencils(ctrl, t["sizes"], t["offsets"], t["indices"], t["weights"])
     np.testing.assert_array_equal(host(out, shape=(nr, 7)), want)
+    # record slices (layout 0) hold 64 x the widest row of each run of 64 rows
+    from opensubdiv_amd import HipStencilTable
+    rec = HipStencilTable.Create({k: t[k] for k in ("sizes", "offsets", "indices", "weights")})
+    sz = np.zeros(-(-nr // 64) * 64, np.int64)
+    sz[:nr] = t["sizes"]
+    assert rec.layout()["record_entries"] == 64 * int(sz.reshape(-1, 64).max(axis=1).sum())
+    assert rec.layout()["size_window"] == 0
     # Far's order keeps its rows, order 2 the dominant order: no window
     far_order = _tiled_table(hip, {k: t[k] for k in ("sizes", "offsets", "indices", "weights")},
                              order=1)
