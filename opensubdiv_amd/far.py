@@ -153,10 +153,21 @@ def face_arrays(mesh):
     return fv, nvpf
 
 
-class Topology:
-    """Control mesh + uniform refinement levels (TopologyRefiner)."""
+SCHEME_BILINEAR, SCHEME_CATMARK = 0, 1     # Sdc::SchemeType values
 
-    def __init__(self, mesh, boundary=1, creasing=0):
+
+class Topology:
+    """Control mesh + uniform refinement levels (TopologyRefiner).
+
+    scheme: SCHEME_CATMARK (default) or SCHEME_BILINEAR.  Far's bilinear
+    scheme splits faces as Catmark does and refines with the linear masks of
+    varying interpolation, so its vertex stencil tables are the compiler's
+    varying tables; patch tables are built for Catmark only."""
+
+    def __init__(self, mesh, boundary=1, creasing=0, scheme=SCHEME_CATMARK):
+        if scheme not in (SCHEME_BILINEAR, SCHEME_CATMARK):
+            raise ValueError(f"subdivision scheme {scheme} not supported")
+        self.scheme = int(scheme)
         L = load_far()
         fv, nvpf = face_arrays(mesh)
         cp = np.ascontiguousarray(mesh.crease_pairs, np.int32).reshape(-1)
@@ -184,9 +195,14 @@ class Topology:
         if load_far().osdfar_topology_set_fvar(self.h, int(num_values), _p(vi)) != 0:
             raise ValueError("face-varying channel rejected")
 
+    def _catmark_only(self, what):
+        if self.scheme != SCHEME_CATMARK:
+            raise NotImplementedError(f"{what}: built for the Catmark scheme only")
+
     def cache_stencils(self, level):
         """Refine to `level` and keep the level stencils, shared by the next
         stencil_table() and patch_table(level) calls."""
+        self._catmark_only("cached level stencils")
         assert load_far().osdfar_topology_cache_stencils(self.h, level) == 0
 
     def drop_stencils(self):
@@ -204,16 +220,21 @@ class Topology:
         assert load_far().osdfar_level_face_vertices(self.h, level, _p(out)) == 0
         return out.reshape(nf, 4)
 
+    def _mode(self, mode):
+        # the bilinear scheme interpolates vertex data as varying data
+        return 1 if self.scheme == SCHEME_BILINEAR else int(mode)
+
     def stencil_table_raw(self, mode=0, intermediate=False, control=False):
         """Float table kept in the compiler's memory (RawStencilTable)."""
-        h = load_far().osdfar_stencil_table_create(self.h, mode, int(intermediate), int(control), 0)
+        h = load_far().osdfar_stencil_table_create(self.h, self._mode(mode), int(intermediate),
+                                                   int(control), 0)
         if not h:
             raise MemoryError("osdfar_stencil_table_create failed")
         return RawStencilTable(h)
 
     def stencil_table(self, mode=0, intermediate=False, control=False, double=False):
-        h = load_far().osdfar_stencil_table_create(self.h, mode, int(intermediate), int(control),
-                                                   int(double))
+        h = load_far().osdfar_stencil_table_create(self.h, self._mode(mode), int(intermediate),
+                                                   int(control), int(double))
         try:
             return _read_table(h, double)
         finally:
@@ -231,6 +252,7 @@ class Topology:
     def patch_table(self, iso_level, secondary_level=None, legacy_sharp_corners=True,
                     share_endcap_points=True):
         """Feature-adaptive patch table (osdfar_patch_table_create_ex)."""
+        self._catmark_only("patch tables")
         return PatchTable(self, iso_level, secondary_level, legacy_sharp_corners,
                           share_endcap_points)
 
@@ -239,6 +261,7 @@ class Topology:
         approximation levels: smooth extraordinary vertices to approx_smooth,
         sharp features to approx_sharp; no legacy sharp corners, unshared
         end-cap points)."""
+        self._catmark_only("patch tables")
         return PatchTable(self, approx_sharp, approx_smooth, False, False)
 
     def __del__(self):

@@ -352,6 +352,36 @@ def test_uniform_general_topology_matches_far_live(name, boundary, creasing):
 
 
 @needs_ref
+@pytest.mark.parametrize("name", ["cube-sphere"] + list(POLY_MESHES))
+def test_varying_tables_are_far_bilinear_tables(name):
+    """Far's bilinear scheme refines with the same quad split and numbering
+    as Catmark and with the linear masks of varying interpolation (face
+    point: the face's average; edge point: the midpoint; vertex point: the
+    vertex), so the compiler's varying tables are Far's uniform vertex
+    tables of the bilinear scheme, intermediate levels included, creases
+    and boundaries having no effect on either."""
+    m = (meshes.cube_sphere(3, seed=7, crease_fraction=0.25) if name == "cube-sphere"
+         else POLY_MESHES[name]())
+    t = far.Topology(m)
+    t.refine_uniform(3)
+    mine = t.stencil_table(mode=1, intermediate=True)
+    theirs = ref.stencils_uniform(ref.Topology(m, scheme=ref.SCHEME_BILINEAR), 3, mode=0,
+                                  intermediate=True)
+    assert_same_rows(mine, theirs)
+    # the scheme option gives the same table for vertex data, and refuses
+    # what is built for Catmark only
+    b = far.Topology(m, scheme=far.SCHEME_BILINEAR)
+    b.refine_uniform(3)
+    assert_same_rows(b.stencil_table(intermediate=True), theirs)
+    for call in (lambda: b.patch_table(2), lambda: b.bfr_patch_table(),
+                 lambda: b.cache_stencils(2)):
+        with pytest.raises(NotImplementedError):
+            call()
+    with pytest.raises(ValueError):
+        far.Topology(m, scheme=2)
+
+
+@needs_ref
 @pytest.mark.parametrize("name,iso", [("pentagonal-prism", 3), ("cube-with-triangles", 2),
                                       ("cube-with-triangles", 4), ("sphere-with-split-quad", 3)])
 def test_patch_table_with_ngon_base_faces_matches_far_live(name, iso):
