@@ -51,6 +51,15 @@ void osdfar_topology_destroy(osdfar_topology *topo);
  * -1 for an unknown method. */
 int osdfar_topology_set_creasing(osdfar_topology *topo, int method);
 
+/* Sdc::SchemeType (sdc/types.h:37-41; Far::TopologyRefinerFactory options):
+ * 0 bilinear, 1 Catmark (default), 2 Loop (triangles only).  Uniform
+ * stencil tables are built for all three (the bilinear scheme's vertex
+ * tables are its linear, varying-mode tables); patch tables, limit
+ * stencils and vertex locations for Catmark only.  Before the first
+ * refinement only: -2 afterwards, -1 for an unknown scheme or a Loop mesh
+ * with non-triangular faces. */
+int osdfar_topology_set_scheme(osdfar_topology *topo, int scheme);
+
 /* Refines uniformly to `level` and keeps every level's factorized vertex
  * stencils with the topology, so a later osdfar_stencil_table_create
  * (vertex mode) and osdfar_patch_table_create (isoLevel == level) reuse

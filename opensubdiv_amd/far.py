@@ -32,6 +32,7 @@ def load_far():
         "osdfar_topology_create": ([i, i, vp, vp, i, vp, vp, i, vp, vp, i], vp),
         "osdfar_topology_destroy": ([vp], None),
         "osdfar_topology_set_creasing": ([vp, i], i),
+        "osdfar_topology_set_scheme": ([vp, i], i),
         "osdfar_refine_uniform": ([vp, i], i),
         "osdfar_level_info": ([vp, i, vp, vp, vp], i),
         "osdfar_level_face_vertices": ([vp, i, vp], i),
@@ -153,19 +154,20 @@ def face_arrays(mesh):
     return fv, nvpf
 
 
-SCHEME_BILINEAR, SCHEME_CATMARK = 0, 1     # Sdc::SchemeType values
+SCHEME_BILINEAR, SCHEME_CATMARK, SCHEME_LOOP = 0, 1, 2     # Sdc::SchemeType values
 
 
 class Topology:
     """Control mesh + uniform refinement levels (TopologyRefiner).
 
-    scheme: SCHEME_CATMARK (default) or SCHEME_BILINEAR.  Far's bilinear
-    scheme splits faces as Catmark does and refines with the linear masks of
-    varying interpolation, so its vertex stencil tables are the compiler's
-    varying tables; patch tables are built for Catmark only."""
+    scheme: SCHEME_CATMARK (default), SCHEME_BILINEAR or SCHEME_LOOP
+    (triangle meshes).  Uniform stencil tables are built for all three: Far's
+    bilinear scheme splits faces as Catmark does and refines with the linear
+    masks of varying interpolation; Loop splits triangles.  Patch tables are
+    built for Catmark only."""
 
     def __init__(self, mesh, boundary=1, creasing=0, scheme=SCHEME_CATMARK):
-        if scheme not in (SCHEME_BILINEAR, SCHEME_CATMARK):
+        if scheme not in (SCHEME_BILINEAR, SCHEME_CATMARK, SCHEME_LOOP):
             raise ValueError(f"subdivision scheme {scheme} not supported")
         self.scheme = int(scheme)
         L = load_far()
@@ -183,6 +185,8 @@ class Topology:
             raise ValueError("topology rejected by the table compiler")
         if creasing and L.osdfar_topology_set_creasing(self.h, int(creasing)) != 0:
             raise ValueError(f"creasing method {creasing} not supported")
+        if scheme != SCHEME_CATMARK and L.osdfar_topology_set_scheme(self.h, int(scheme)) != 0:
+            raise ValueError(f"scheme {scheme} rejected (Loop needs triangles)")
         self.num_control = mesh.nverts
 
     def refine_uniform(self, level):
@@ -202,7 +206,6 @@ class Topology:
     def cache_stencils(self, level):
         """Refine to `level` and keep the level stencils, shared by the next
         stencil_table() and patch_table(level) calls."""
-        self._catmark_only("cached level stencils")
         assert load_far().osdfar_topology_cache_stencils(self.h, level) == 0
 
     def drop_stencils(self):
@@ -220,21 +223,16 @@ class Topology:
         assert load_far().osdfar_level_face_vertices(self.h, level, _p(out)) == 0
         return out.reshape(nf, 4)
 
-    def _mode(self, mode):
-        # the bilinear scheme interpolates vertex data as varying data
-        return 1 if self.scheme == SCHEME_BILINEAR else int(mode)
-
     def stencil_table_raw(self, mode=0, intermediate=False, control=False):
         """Float table kept in the compiler's memory (RawStencilTable)."""
-        h = load_far().osdfar_stencil_table_create(self.h, self._mode(mode), int(intermediate),
-                                                   int(control), 0)
+        h = load_far().osdfar_stencil_table_create(self.h, mode, int(intermediate), int(control), 0)
         if not h:
             raise MemoryError("osdfar_stencil_table_create failed")
         return RawStencilTable(h)
 
     def stencil_table(self, mode=0, intermediate=False, control=False, double=False):
-        h = load_far().osdfar_stencil_table_create(self.h, self._mode(mode), int(intermediate),
-                                                   int(control), int(double))
+        h = load_far().osdfar_stencil_table_create(self.h, mode, int(intermediate), int(control),
+                                                   int(double))
         try:
             return _read_table(h, double)
         finally:

@@ -95,6 +95,13 @@ int osdfar_topology_set_creasing(osdfar_topology *topo, int method)
     return 0;
 }
 
+int osdfar_topology_set_scheme(osdfar_topology *topo, int scheme)
+{
+    if (!topo) return -1;
+    if (topo->refiner.maxLevel() > 0 || !topo->levels.empty()) return -2;
+    return topo->refiner.setScheme(scheme) ? 0 : -1;
+}
+
 void osdfar_topology_destroy(osdfar_topology *topo) { delete topo; }
 
 int osdfar_topology_cache_stencils(osdfar_topology *topo, int level)
@@ -222,6 +229,7 @@ osdfar_patch_table *osdfar_patch_table_create_ex(osdfar_topology *topo, int isoL
     try {
         if (!topo || isoLevel < 0 || isoLevel > 10) return nullptr;
         if (topo->refiner.maxLevel() > isoLevel) return nullptr;
+        if (topo->refiner.scheme != SCHEME_CATMARK) return nullptr;   // Catmark patches only
         PatchOptions opt;
         if (options) {
             if (options->secondaryLevel < 0 || options->secondaryLevel > 15) return nullptr;
@@ -389,6 +397,7 @@ int osdfar_vertex_locations(const osdfar_topology *topo, int level, int *ptex, d
 {
     try {
         if (!topo || level < 0 || level > topo->refiner.maxLevel() || !ptex || !u || !v) return -1;
+        if (topo->refiner.scheme == SCHEME_LOOP) return -1;   // quad parameterizations only
         std::vector<int> p;
         std::vector<double> uu, vv;
         uniformVertexLocations(topo->refiner, level, p, uu, vv);

@@ -1,6 +1,8 @@
-// Factorized stencils of uniformly refined Catmark levels.
+// Factorized stencils of uniformly refined levels (Catmark, Loop, and the
+// bilinear scheme as Catmark topology with linear masks).
 //
-// Masks restate Sdc's Catmark rules (sdc/catmarkScheme.h:71-210) with
+// Masks restate Sdc's Catmark and Loop rules (sdc/catmarkScheme.h:71-210,
+// sdc/loopScheme.h:70-215) with
 // uniform or Chaikin creasing and fractional blending (sdc/scheme.h:392-600,
 // sdc/crease.cpp:44-198), applied level by level as Far's PrimvarRefiner
 // does (far/primvarRefiner.h:475-660): face points, then edge points (which
@@ -16,6 +18,7 @@
 #include "stencils.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #ifdef _OPENMP
 #include <omp.h>
@@ -295,11 +298,11 @@ void ruleMask(Rule r, int n, int nf, const float *es, VMask &m)
 
 }  // namespace
 
-// Rows of level l (parent rows PS, NULL at l == 1) into the sink; returns
-// the level's row count.
+// Rows of a Catmark (or, with the varying mode, bilinear) level l (parent
+// rows PS, NULL at l == 1) into the sink; returns the level's row count.
 template <class Sink>
-static int levelRows(const Refiner &r, int mode, int l, const LevelStencils *PS, double avgPrev,
-                     Sink &sink)
+static int catmarkLevelRows(const Refiner &r, int mode, int l, const LevelStencils *PS,
+                            double avgPrev, Sink &sink)
 {
     const int nControl = r.level(0).nv;
     const Level &P = r.level(l - 1);
@@ -399,6 +402,130 @@ static int levelRows(const Refiner &r, int mode, int l, const LevelStencils *PS,
         }
     }, sink);
     return nrows;
+}
+
+// Loop vertex-vertex masks (sdc/loopScheme.h:140-215): smooth -- the
+// vertex and, per incident edge, its other end, with Loop's weights for n
+// incident faces (5/8 and 1/16 at n = 6; otherwise from beta, evaluated in
+// double and rounded to float as the reference's float tables are);
+// crease 3/4 and 1/8 at the two crease ends; corner the vertex alone.
+static void loopSmoothVertexMask(int n, int nf, VMask &m)
+{
+    float ew = 0.0625f, vw = 0.625f;
+    if (nf != 6) {
+        const double dn = (double)nf, inv = 1.0 / dn;
+        const double c = std::cos(M_PI * 2.0 * inv);
+        const double beta = 0.25 * c + 0.375;
+        ew = (float)((0.625 - beta * beta) * inv);
+        vw = (float)(1.0 - (double)ew * dn);
+    }
+    m.v = vw;
+    std::fill(m.e, m.e + n, (double)ew);
+    std::fill(m.f, m.f + nf, 0.0);
+}
+
+static void loopRuleMask(Rule r, int n, int nf, const float *es, VMask &m)
+{
+    if (r == RULE_SMOOTH || r == RULE_DART) loopSmoothVertexMask(n, nf, m);
+    else if (r == RULE_CREASE) creaseVertexMask(n, nf, es, m);
+    else cornerVertexMask(n, nf, m);
+}
+
+// The vertex of triangle f opposite edge e.
+static int oppositeVertex(const Level &P, int f, int e)
+{
+    const int b = P.fvOff[f];
+    for (int j = 0; j < 3; ++j)
+        if (P.fe[b + j] == e) return P.fv[b + (j + 2) % 3];
+    return P.fv[b];
+}
+
+// Rows of a Loop level l: [vertex points | edge points] (no face points).
+// Edge points (sdc/loopScheme.h:70-130): 3/8 per end and 1/8 per vertex
+// opposite the edge in its two triangles, 1/2 per end on a crease;
+// fractional sharpness blends as for Catmark (sdc/scheme.h:392-600).
+template <class Sink>
+static int loopLevelRows(const Refiner &r, int mode, int l, const LevelStencils *PS,
+                         double avgPrev, Sink &sink)
+{
+    const int nControl = r.level(0).nv;
+    const Level &P = r.level(l - 1);
+    const Level &C = r.level(l);
+    const size_t eFromE = 3 * (size_t)P.nf;
+    const int nv = P.nv;
+    auto vertexRow = [&](int v, Accum &acc) {
+        if (mode == INTERPOLATE_VARYING) { acc.addRow(PS, v, 1.0); return; }
+        const int n = P.valence(v);
+        const int nfv = P.vfOff[v + 1] - P.vfOff[v];
+        float pesb[kMaxValence], cesb[kMaxValence];
+        const int nn = std::min(n, kMaxValence);
+        for (int i = 0; i < nn; ++i) pesb[i] = P.esharp[P.ve[P.veOff[v] + i]];
+        subdivideSharpnessAroundVertex(r.creasing, nn, pesb, cesb);
+        const float pvs = P.vsharp[v], cvs = decrementSharpness(pvs);
+        const Rule pr = ruleFrom(pvs, nn, pesb);
+        VMask m;
+        m.init(n, nfv);
+        loopRuleMask(pr, nn, nfv, pesb, m);
+        if (pr != RULE_SMOOTH && pr != RULE_DART) {
+            const Rule cr = ruleFrom(cvs, nn, cesb);
+            if (cr != pr) {
+                int cnt = 0;
+                double sum = 0.0;
+                if (pvs > 0.0f && cvs <= 0.0f) { cnt = 1; sum = pvs; }
+                for (int i = 0; i < nn; ++i)
+                    if (pesb[i] > 0.0f && cesb[i] <= 0.0f) { sum += pesb[i]; ++cnt; }
+                const double pw = cnt ? std::min(1.0, sum / cnt) : 0.0;
+                VMask cm;
+                cm.init(n, nfv);
+                loopRuleMask(cr, nn, nfv, cesb, cm);
+                const double cw = 1.0 - pw;
+                m.v = pw * m.v + cw * cm.v;
+                for (int i = 0; i < n; ++i) m.e[i] = pw * m.e[i] + cw * cm.e[i];
+            }
+        }
+        for (int i = 0; i < n; ++i)
+            if (m.e[i] != 0.0) acc.addRow(PS, P.otherVertex(P.ve[P.veOff[v] + i], v), m.e[i]);
+        acc.addRow(PS, v, m.v);
+    };
+    auto edgeRow = [&](int e, Accum &acc) {
+        const int a = P.ev[2 * e], b = P.ev[2 * e + 1];
+        const int nfe = P.efOff[e + 1] - P.efOff[e];
+        const float s = P.esharp[e];
+        const bool childCrease = C.esharp[eFromE + 2 * (size_t)e] > 0.0f &&
+                                 C.esharp[eFromE + 2 * (size_t)e + 1] > 0.0f;
+        if (mode == INTERPOLATE_VARYING || (s > 0.0f && (childCrease || nfe != 2))) {
+            acc.addRow(PS, a, 0.5);
+            acc.addRow(PS, b, 0.5);
+            return;
+        }
+        double vw = 0.375, fw = nfe == 2 ? 0.125 : 0.25 / (double)nfe;
+        if (s > 0.0f) {
+            const double p = (double)s, c = 1.0 - p;
+            vw = p * 0.5 + c * vw;
+            fw *= c;
+        }
+        acc.addRow(PS, a, vw);
+        acc.addRow(PS, b, vw);
+        for (int k = P.efOff[e]; k < P.efOff[e + 1]; ++k)
+            acc.addRow(PS, oppositeVertex(P, P.ef[k], e), fw);
+    };
+    const int nrows = nv + P.ne;
+    parallelRows(nrows, nControl, 3.0 * avgPrev + 6.0, [&](int i, Accum &acc) {
+        if (i < nv) vertexRow(i, acc);
+        else edgeRow(i - nv, acc);
+    }, sink);
+    return nrows;
+}
+
+// The level's rows in the refiner's scheme; the bilinear scheme refines as
+// Catmark does with the linear (varying) masks.
+template <class Sink>
+static int levelRows(const Refiner &r, int mode, int l, const LevelStencils *PS, double avgPrev,
+                     Sink &sink)
+{
+    if (r.scheme == SCHEME_LOOP) return loopLevelRows(r, mode, l, PS, avgPrev, sink);
+    return catmarkLevelRows(r, r.scheme == SCHEME_BILINEAR ? INTERPOLATE_VARYING : mode, l, PS,
+                            avgPrev, sink);
 }
 
 void computeLevelStencils(const Refiner &r, int mode, std::vector<LevelStencils> &LS)

@@ -139,8 +139,119 @@ int Refiner::numVerticesTotal() const
     return n;
 }
 
+bool Refiner::setScheme(int s)
+{
+    if (s != SCHEME_BILINEAR && s != SCHEME_CATMARK && s != SCHEME_LOOP) return false;
+    if (maxLevel() > 0 || levels.empty()) return false;
+    if (s == SCHEME_LOOP)
+        for (int f = 0; f < levels[0].nf; ++f)
+            if (levels[0].faceSize(f) != 3) return false;
+    scheme = s;
+    return true;
+}
+
+// Sharpness of a new level: the two child edges of parent edge e are child
+// edges eFromE + 2e + {0, 1} (at the parent's ev[0], ev[1]); vertex children
+// come first, so child vertex v is parent vertex v.
+void Refiner::childSharpness(const Level &P, Level &C, int eFromE) const
+{
+    auto edgeChild = [&](int pe, int corner) {
+        return eFromE + 2 * pe + (P.ev[2 * pe] == corner ? 0 : 1);
+    };
+    C.esharp.assign(C.ne, 0.0f);
+    if (creasing == CREASE_CHAIKIN) {
+        // the child edge at parent vertex v subdivides the parent's sharpness
+        // with the edges around v (vtr/refinement.cpp:870-930)
+        std::vector<float> ps, cs;
+        for (int v = 0; v < P.nv; ++v) {
+            const int n = P.valence(v);
+            ps.resize(n);
+            cs.resize(n);
+            for (int i = 0; i < n; ++i) ps[i] = P.esharp[P.ve[P.veOff[v] + i]];
+            subdivideSharpnessAroundVertex(creasing, n, ps.data(), cs.data());
+            for (int i = 0; i < n; ++i) C.esharp[edgeChild(P.ve[P.veOff[v] + i], v)] = cs[i];
+        }
+    } else {
+        for (int e = 0; e < P.ne; ++e)
+            C.esharp[eFromE + 2 * e] = C.esharp[eFromE + 2 * e + 1] = decrementSharpness(P.esharp[e]);
+    }
+    C.vsharp.assign(C.nv, 0.0f);
+    for (int v = 0; v < P.nv; ++v) C.vsharp[v] = decrementSharpness(P.vsharp[v]);
+}
+
+// Loop's triangle split (vtr/triRefinement.cpp:141-345).  Child vertices:
+// [children of vertices | of edges]; child faces: four per triangle, the
+// corner triangles at face vertices 0, 1, 2 and the middle one; child
+// edges: [three per triangle (the middle triangle's sides) | two per edge].
+// Face-child edge i joins the edge points of face edges i and i - 1; the
+// middle triangle starts at the edge point of face edge 1.
+void Refiner::refineTriangles()
+{
+    const int pl = maxLevel();
+    levels.emplace_back();
+    origins.emplace_back();
+    const Level &P = levels[pl];
+    Level &C = levels.back();
+    const int vFromV = 0, vFromE = P.nv;
+    C.nv = P.nv + P.ne;
+    C.nf = 4 * P.nf;
+    const int eFromF = 0, eFromE = 3 * P.nf;
+    C.ne = 3 * P.nf + 2 * P.ne;
+    C.fvOff.resize(C.nf + 1);
+    for (int i = 0; i <= C.nf; ++i) C.fvOff[i] = 3 * i;
+    C.fv.resize(3 * (size_t)C.nf);
+    C.fe.resize(3 * (size_t)C.nf);
+    C.ev.resize(2 * (size_t)C.ne);
+    auto edgeChild = [&](int pe, int corner) {
+        return eFromE + 2 * pe + (P.ev[2 * pe] == corner ? 0 : 1);
+    };
+    std::vector<FaceOrigin> &org = origins.back();
+    org.assign(C.nf, FaceOrigin{ 0, 0, 0 });
+    for (int f = 0; f < P.nf; ++f) {
+        const int b = P.fvOff[f];
+        int fv[3], ecv[3], atStart[3], atEnd[3], fce[3];
+        for (int i = 0; i < 3; ++i) {
+            fv[i] = P.fv[b + i];
+            const int pe = P.fe[b + i];                  // joins fv[i] and fv[i + 1]
+            ecv[i] = vFromE + pe;
+            atStart[i] = edgeChild(pe, P.fv[b + i]);
+            atEnd[i] = edgeChild(pe, P.fv[b + (i + 1) % 3]);
+            fce[i] = eFromF + 3 * f + i;
+        }
+        const int cfv[4][3] = { { vFromV + fv[0], ecv[0], ecv[2] },
+                                { ecv[0], vFromV + fv[1], ecv[1] },
+                                { ecv[2], ecv[1], vFromV + fv[2] },
+                                { ecv[1], ecv[2], ecv[0] } };
+        const int cfe[4][3] = { { atStart[0], fce[0], atEnd[2] },
+                                { atEnd[0], atStart[1], fce[1] },
+                                { fce[2], atEnd[1], atStart[2] },
+                                { fce[2], fce[0], fce[1] } };
+        for (int k = 0; k < 4; ++k)
+            for (int j = 0; j < 3; ++j) {
+                C.fv[3 * (4 * f + k) + j] = cfv[k][j];
+                C.fe[3 * (4 * f + k) + j] = cfe[k][j];
+            }
+        const int cev[3][2] = { { ecv[0], ecv[2] }, { ecv[1], ecv[0] }, { ecv[2], ecv[1] } };
+        for (int i = 0; i < 3; ++i) {
+            C.ev[2 * fce[i]] = cev[i][0];
+            C.ev[2 * fce[i] + 1] = cev[i][1];
+        }
+    }
+    for (int e = 0; e < P.ne; ++e)
+        for (int j = 0; j < 2; ++j) {
+            C.ev[2 * (eFromE + 2 * e + j)] = vFromE + e;
+            C.ev[2 * (eFromE + 2 * e + j) + 1] = vFromV + P.ev[2 * e + j];
+        }
+    childSharpness(P, C, eFromE);
+    C.buildIncidence();
+}
+
 void Refiner::refineOnce()
 {
+    if (scheme == SCHEME_LOOP) {
+        refineTriangles();
+        return;
+    }
     const int pl = maxLevel();
     levels.emplace_back();
     origins.emplace_back();
@@ -216,25 +327,7 @@ void Refiner::refineOnce()
             C.ev[2 * (eFromE + 2 * e + j)] = vFromE + e;
             C.ev[2 * (eFromE + 2 * e + j) + 1] = vFromV + P.ev[2 * e + j];
         }
-    C.esharp.assign(C.ne, 0.0f);
-    if (creasing == CREASE_CHAIKIN) {
-        // the child edge at parent vertex v subdivides the parent's sharpness
-        // with the edges around v (vtr/refinement.cpp:870-930)
-        std::vector<float> ps, cs;
-        for (int v = 0; v < P.nv; ++v) {
-            const int n = P.valence(v);
-            ps.resize(n);
-            cs.resize(n);
-            for (int i = 0; i < n; ++i) ps[i] = P.esharp[P.ve[P.veOff[v] + i]];
-            subdivideSharpnessAroundVertex(creasing, n, ps.data(), cs.data());
-            for (int i = 0; i < n; ++i) C.esharp[edgeChild(P.ve[P.veOff[v] + i], v)] = cs[i];
-        }
-    } else {
-        for (int e = 0; e < P.ne; ++e)
-            C.esharp[eFromE + 2 * e] = C.esharp[eFromE + 2 * e + 1] = decrementSharpness(P.esharp[e]);
-    }
-    C.vsharp.assign(C.nv, 0.0f);
-    for (int v = 0; v < P.nv; ++v) C.vsharp[vFromV + v] = decrementSharpness(P.vsharp[v]);
+    childSharpness(P, C, eFromE);
     C.buildIncidence();
 }
 

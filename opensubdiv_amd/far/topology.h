@@ -1,12 +1,14 @@
-// Host-side topology levels and uniform Catmull-Clark refinement for the
-// table compiler (libosdfar).  Scope: manifold Catmark meshes with n-gon
-// faces at level 0, semi-sharp and infinitely sharp creases, corner
-// sharpness, uniform or Chaikin creasing and boundary edges (sharpened).
+// Host-side topology levels and uniform refinement for the table compiler
+// (libosdfar).  Scope: manifold meshes -- Catmark and bilinear with n-gon
+// faces at level 0 (quad split), Loop with triangles (triangle split) --
+// with semi-sharp and infinitely sharp creases, corner sharpness, uniform
+// or Chaikin creasing and boundary edges (sharpened).
 //
 // Numbering follows OpenSubdiv's Vtr uniform refinement so refined vertex i
 // here is refined vertex i of Far (vtr/refinement.cpp:253-270: children of
-// vertices, then of faces, then of edges; vtr/quadRefinement.cpp:125-345 for
-// child faces / edges; vtr/level.cpp:1640-1790 for level-0 edges).
+// vertices, then of faces, then of edges; vtr/quadRefinement.cpp:125-345 and
+// vtr/triRefinement.cpp:141-345 for child faces / edges; vtr/level.cpp:
+// 1640-1790 for level-0 edges).
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -17,6 +19,9 @@ constexpr float kSharpInfinite = 10.0f;   // Sdc::Crease::SHARPNESS_INFINITE
 
 enum Rule { RULE_SMOOTH = 1, RULE_DART = 2, RULE_CREASE = 4, RULE_CORNER = 8 };
 enum Creasing { CREASE_UNIFORM = 0, CREASE_CHAIKIN = 1 };   // Sdc::Options::CreasingMethod
+// Sdc::SchemeType.  Bilinear refines as Catmark does (quad split, same
+// numbering) with linear masks; Loop splits triangles (vtr/triRefinement.cpp).
+enum Scheme { SCHEME_BILINEAR = 0, SCHEME_CATMARK = 1, SCHEME_LOOP = 2 };
 
 struct Level {
     int nv = 0, nf = 0, ne = 0;
@@ -80,9 +85,15 @@ public:
     std::vector<uint8_t> ptexNonQuad;
     int creasing = CREASE_UNIFORM;
     int boundaryInterpolation = 1;
+    int scheme = SCHEME_CATMARK;
+
+    // Before the first refinement; Loop needs triangles only.
+    bool setScheme(int s);
 
 private:
     void refineOnce();
+    void refineTriangles();
+    void childSharpness(const Level &P, Level &C, int eFromE) const;
 };
 
 inline float decrementSharpness(float s)

@@ -373,12 +373,82 @@ def test_varying_tables_are_far_bilinear_tables(name):
     b = far.Topology(m, scheme=far.SCHEME_BILINEAR)
     b.refine_uniform(3)
     assert_same_rows(b.stencil_table(intermediate=True), theirs)
-    for call in (lambda: b.patch_table(2), lambda: b.bfr_patch_table(),
-                 lambda: b.cache_stencils(2)):
+    for call in (lambda: b.patch_table(2), lambda: b.bfr_patch_table()):
         with pytest.raises(NotImplementedError):
             call()
     with pytest.raises(ValueError):
-        far.Topology(m, scheme=2)
+        far.Topology(m, scheme=far.SCHEME_LOOP)          # not a triangle mesh
+    with pytest.raises(ValueError):
+        far.Topology(m, scheme=3)
+
+
+def icosahedron():
+    p = (1 + 5 ** 0.5) / 2
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    m = PolyMesh(faces)
+    assert p > 1
+    return m
+
+
+def triangulated(mesh, creases=(), corners=()):
+    """Each quad of a quad mesh split along alternating diagonals."""
+    faces = []
+    for k, q in enumerate(np.asarray(mesh.face_verts).reshape(-1, 4)):
+        a, b, c, d = (int(x) for x in q)
+        faces += [(a, b, c), (a, c, d)] if k % 2 == 0 else [(a, b, d), (b, c, d)]
+    return PolyMesh(faces, creases, corners)
+
+
+def open_triangle_grid(n, creases=()):
+    idx = lambda i, j: i * (n + 1) + j
+    faces = []
+    for i in range(n):
+        for j in range(n):
+            faces += [(idx(i, j), idx(i, j + 1), idx(i + 1, j + 1)),
+                      (idx(i, j), idx(i + 1, j + 1), idx(i + 1, j))]
+    return PolyMesh(faces, creases)
+
+
+def tetra_sharp():
+    return PolyMesh([(0, 2, 1), (0, 1, 3), (1, 2, 3), (2, 0, 3)],
+                    creases=[(0, 1, 10.0), (1, 2, 1.5)], corners=[(3, 10.0), (2, 0.5)])
+
+
+def cube_sphere_triangles():
+    q = meshes.cube_sphere(3, seed=5, crease_fraction=0.25)
+    return triangulated(q, creases=[(int(a), int(b), float(w)) for (a, b), w in
+                                    zip(np.asarray(q.crease_pairs).reshape(-1, 2),
+                                        q.crease_weights)])
+
+
+LOOP_MESHES = {"icosahedron": icosahedron, "cube-sphere-triangulated": cube_sphere_triangles,
+               "open-grid": lambda: open_triangle_grid(4),
+               "open-grid-creased": lambda: open_triangle_grid(4, creases=[(6, 7, 2.0),
+                                                                         (7, 8, 0.6)]),
+               "tetrahedron-sharp": tetra_sharp}
+
+
+@needs_ref
+@pytest.mark.parametrize("name", list(LOOP_MESHES))
+@pytest.mark.parametrize("boundary,creasing,mode", [(1, 0, 0), (2, 0, 0), (1, 1, 0), (1, 0, 1)])
+def test_loop_uniform_tables_match_far_live(name, boundary, creasing, mode):
+    """Loop's triangle split and masks (smooth vertices of any valence,
+    creases, corners, boundaries under both interpolation rules, fractional
+    and Chaikin creasing, varying interpolation): uniform tables with every
+    intermediate level, row for row against live Far."""
+    m = LOOP_MESHES[name]()
+    t = far.Topology(m, boundary=boundary, creasing=creasing, scheme=far.SCHEME_LOOP)
+    t.refine_uniform(3)
+    mine = t.stencil_table(mode=mode, intermediate=True)
+    theirs = ref.stencils_uniform(ref.Topology(m, boundary=boundary, creasing=creasing,
+                                               scheme=ref.SCHEME_LOOP), 3, mode=mode,
+                                  intermediate=True)
+    assert_same_rows(mine, theirs)
+    # refined Loop levels: 4x the faces, V + E vertices
+    nv, nf, _ = t.level_info(1)
+    assert nf == 4 * m.nfaces
 
 
 @needs_ref
