@@ -546,3 +546,40 @@ def test_hbr_regression_shapes_on_the_device(hip, layout):
             dist, nearest = cKDTree(base).query(got[lo:lo + nv].astype(np.float64))
             assert len(np.unique(nearest)) == nv, (name, level)
             assert dist.max() <= 1e-5 * diag, (name, level, dist.max())
+
+
+@pytest.mark.parametrize("layout", [0, 2])
+def test_loop_tables_from_the_compiler_on_the_device(hip, layout):
+    """A Loop-scheme uniform table built by the table compiler (a triangulated,
+    creased cube-sphere refined to level 3), evaluated on the device in the
+    record and locality-ordered layouts, equals the oracle's CpuEvaluator-
+    order sums bit for bit."""
+    from opensubdiv_amd import HipStencilTable, far, meshes
+    q = meshes.cube_sphere(6, seed=4, crease_fraction=0.1)
+    tris = []
+    for k, f in enumerate(np.asarray(q.face_verts).reshape(-1, 4)):
+        a, b, c, d = (int(x) for x in f)
+        tris += [(a, b, c), (a, c, d)] if k % 2 == 0 else [(a, b, d), (b, c, d)]
+
+    class TriMesh:
+        face_verts = np.asarray(tris, np.int32)
+        nfaces = len(tris)
+        nverts = q.nverts
+        crease_pairs, crease_weights = q.crease_pairs, q.crease_weights
+        corner_verts = np.zeros(0, np.int32)
+        corner_weights = np.zeros(0, np.float32)
+
+    topo = far.Topology(TriMesh, scheme=far.SCHEME_LOOP)
+    topo.refine_uniform(3)
+    t = topo.stencil_table()
+    n = len(t["sizes"])
+    assert n == topo.level_info(3)[0]
+    table = HipStencilTable.Create({k: t[k] for k in ("sizes", "offsets", "indices", "weights")},
+                                   layout=layout)
+    rng = np.random.default_rng(17)
+    ctrl = rng.standard_normal((q.nverts, 7)).astype(np.float32)
+    out = empty(n * 7)
+    assert table_eval(hip, table.h, dev(ctrl.reshape(-1)), (0, 7, 7), [out], [(0, 7, 7)], 0,
+                      n) == 0
+    want = oracle.apply_stencils(ctrl, t["sizes"], t["offsets"], t["indices"], t["weights"])
+    np.testing.assert_array_equal(host(out, shape=(n, 7)), want)
