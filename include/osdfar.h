@@ -54,10 +54,11 @@ int osdfar_topology_set_creasing(osdfar_topology *topo, int method);
 /* Sdc::SchemeType (sdc/types.h:37-41; Far::TopologyRefinerFactory options):
  * 0 bilinear, 1 Catmark (default), 2 Loop (triangles only).  Uniform
  * stencil tables are built for all three (the bilinear scheme's vertex
- * tables are its linear, varying-mode tables); patch tables, limit
- * stencils and vertex locations for Catmark only.  Before the first
- * refinement only: -2 afterwards, -1 for an unknown scheme or a Loop mesh
- * with non-triangular faces. */
+ * tables are its linear, varying-mode tables); patch tables (and their
+ * limit stencils) for Catmark and bilinear -- QUADS patches, n-gons refined
+ * once -- and face-varying channels for Catmark; vertex locations for the
+ * quad-split schemes.  Before the first refinement only: -2 afterwards, -1
+ * for an unknown scheme or a Loop mesh with non-triangular faces. */
 int osdfar_topology_set_scheme(osdfar_topology *topo, int scheme);
 
 /* Refines uniformly to `level` and keeps every level's factorized vertex

@@ -164,7 +164,7 @@ class Topology:
     (triangle meshes).  Uniform stencil tables are built for all three: Far's
     bilinear scheme splits faces as Catmark does and refines with the linear
     masks of varying interpolation; Loop splits triangles.  Patch tables are
-    built for Catmark only."""
+    built for Catmark and bilinear (QUADS patches)."""
 
     def __init__(self, mesh, boundary=1, creasing=0, scheme=SCHEME_CATMARK):
         if scheme not in (SCHEME_BILINEAR, SCHEME_CATMARK, SCHEME_LOOP):
@@ -249,8 +249,10 @@ class Topology:
 
     def patch_table(self, iso_level, secondary_level=None, legacy_sharp_corners=True,
                     share_endcap_points=True):
-        """Feature-adaptive patch table (osdfar_patch_table_create_ex)."""
-        self._catmark_only("patch tables")
+        """Feature-adaptive patch table (osdfar_patch_table_create_ex):
+        Catmark, or the bilinear scheme's QUADS patches."""
+        if self.scheme == SCHEME_LOOP:
+            raise NotImplementedError("patch tables: built for the Catmark and bilinear schemes")
         return PatchTable(self, iso_level, secondary_level, legacy_sharp_corners,
                           share_endcap_points)
 

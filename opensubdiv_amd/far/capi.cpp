@@ -229,7 +229,9 @@ osdfar_patch_table *osdfar_patch_table_create_ex(osdfar_topology *topo, int isoL
     try {
         if (!topo || isoLevel < 0 || isoLevel > 10) return nullptr;
         if (topo->refiner.maxLevel() > isoLevel) return nullptr;
-        if (topo->refiner.scheme != SCHEME_CATMARK) return nullptr;   // Catmark patches only
+        // Catmark and bilinear patch tables; face-varying channels for Catmark
+        if (topo->refiner.scheme == SCHEME_LOOP) return nullptr;
+        if (topo->refiner.scheme != SCHEME_CATMARK && topo->fvarValues > 0) return nullptr;
         PatchOptions opt;
         if (options) {
             if (options->secondaryLevel < 0 || options->secondaryLevel > 15) return nullptr;

@@ -20,7 +20,7 @@ namespace osdfar {
 
 namespace {
 
-constexpr int kRegular = 6, kGregory = 9;
+constexpr int kQuads = 3, kRegular = 6, kGregory = 9;
 
 int otherFace(const Level &L, int e, int f)
 {
@@ -501,10 +501,18 @@ bool regularPoints(const Level &L, int f, int mask, int pts[16])
 bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelStencils> *levels,
                        const PatchOptions &opt)
 {
+    // The bilinear scheme (linear QUADS patches): Far refines only its
+    // irregular faces, one level, whatever the isolation level
+    // (far/topologyRefiner.cpp:400-420, selectLinearIrregularFaces); every
+    // leaf is a QUADS patch of its face's 4 vertices without boundary bits
+    // (far/patchBuilder.cpp:720-724).
+    const bool linear = refiner.scheme == SCHEME_BILINEAR;
+    if (refiner.scheme != SCHEME_CATMARK && !linear) return false;
+    bool hasNgon = false;
     {
         const Level &L0 = refiner.level(0);
-        for (int f = 0; f < L0.nf; ++f)
-            if (L0.faceSize(f) != 4 && isoLevel < 1) return false;   // n-gons need a level
+        for (int f = 0; f < L0.nf; ++f) hasNgon = hasNgon || L0.faceSize(f) != 4;
+        if (hasNgon && isoLevel < 1 && !linear) return false;   // n-gons need a level
         // boundaries need interpolated boundary edges; interior edges and
         // vertices must not be infinitely sharp (only boundary corners are)
         bool boundary = false;
@@ -515,8 +523,8 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
         }
         if (boundary && refiner.boundaryInterpolation == 0) return false;
     }
-    refiner.refineUniform(isoLevel);                           // may reallocate the levels
-    const int maxL = isoLevel;
+    const int maxL = linear ? (hasNgon ? 1 : 0) : isoLevel;
+    refiner.refineUniform(maxL);                               // may reallocate the levels
     std::vector<LevelStencils> ownLS;
     if (!levels || (int)levels->size() <= maxL) {
         computeLevelStencils(refiner, INTERPOLATE_VERTEX, ownLS);
@@ -539,11 +547,14 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
             // levels at or past the secondary level use the reduced
             // feature set (far/topologyRefiner.cpp:412-470)
             const bool reduced = d >= std::min(opt.secondaryLevel, isoLevel);
-            if (d < maxL && faceHasFeatures(L, f, d == 0, reduced)) {
+            const bool feature = linear ? d == 0 && L.faceSize(f) != 4
+                                        : faceHasFeatures(L, f, d == 0, reduced);
+            if (d < maxL && feature) {
                 selected[f] = 1;
                 for (int j = 0; j < L.faceSize(f); ++j) next[L.fvOff[f] + j] = 1;
             } else {
-                leaves.push_back({ d, f, !isExtraordinary(L, f, opt.legacySharpCorners), 0 });
+                leaves.push_back({ d, f, linear || !isExtraordinary(L, f, opt.legacySharpCorners),
+                                   0 });
                 maxDepth = std::max(maxDepth, d);
             }
         }
@@ -568,14 +579,19 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
     std::vector<int> regPts;
     std::vector<Row> greg;
     std::vector<const Leaf *> regLeaves, gregLeaves;
+    const int stride = linear ? 4 : 16;                        // points per regular patch
     for (const Leaf &lf : leaves) {
         if (lf.regular) {
             int pts[16];
             const Level &L = refiner.level(lf.level);
-            if (!regularPoints(L, lf.face, boundaryMask(L, lf.face), pts)) return false;
+            if (linear) {
+                for (int k = 0; k < 4; ++k) pts[k] = L.fv[L.fvOff[lf.face] + k];
+            } else if (!regularPoints(L, lf.face, boundaryMask(L, lf.face), pts)) {
+                return false;
+            }
             if (lf.level > 0)
-                for (int p : pts) bufIndex[lf.level][p] = 0;
-            regPts.insert(regPts.end(), pts, pts + 16);
+                for (int k = 0; k < stride; ++k) bufIndex[lf.level][pts[k]] = 0;
+            regPts.insert(regPts.end(), pts, pts + stride);
             regLeaves.push_back(&lf);
         } else {
             Row rows[20];
@@ -654,7 +670,8 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
         p.field0 = ((uint32_t)ptex & 0xfffffffu) | ((uint32_t)lf.transition << 28);
         // far/patchBuilder.cpp:1272-1380: depth = level; under an n-gon the
         // (u,v) bits count from level 1, flagged by the non-quad bit
-        const uint32_t bmask = regular ? (uint32_t)boundaryMask(refiner.level(lf.level), lf.face) : 0u;
+        const uint32_t bmask = regular && !linear
+                                   ? (uint32_t)boundaryMask(refiner.level(lf.level), lf.face) : 0u;
         p.field1 = (uint32_t)lf.level | ((uint32_t)refiner.ptexNonQuad[ptex] << 4) |
                    ((uint32_t)(regular ? 1 : 0) << 5) | (bmask << 7) | ((uint32_t)v << 12) |
                    ((uint32_t)u << 22);
@@ -662,11 +679,12 @@ bool PatchTable::build(Refiner &refiner, int isoLevel, const std::vector<LevelSt
         return p;
     };
     const int nReg = (int)regLeaves.size(), nGreg = (int)gregLeaves.size();
-    if (nReg) arrays.push_back({ kRegular, kRegular, nReg, 0, 16, 0 });
+    const int regType = linear ? kQuads : kRegular;
+    if (nReg) arrays.push_back({ regType, regType, nReg, 0, stride, 0 });
     for (int i = 0; i < nReg; ++i) {
         const Leaf &lf = *regLeaves[i];
-        for (int k = 0; k < 16; ++k) {
-            const int p = regPts[16 * i + k];
+        for (int k = 0; k < stride; ++k) {
+            const int p = regPts[stride * i + k];
             indices.push_back(lf.level == 0 ? p : bufIndex[lf.level][p]);
         }
         params.push_back(paramOf(lf, true));

@@ -129,8 +129,8 @@ def test_creasing_method_errors():
     assert far.load_far().osdfar_topology_set_creasing(t.h, 1) == -2   # after refinement
 
 
-def _my_patch_eval(mesh, iso, f, u, v, nout=3, creasing=0):
-    t = far.Topology(mesh, creasing=creasing)
+def _my_patch_eval(mesh, iso, f, u, v, nout=3, creasing=0, scheme=1):
+    t = far.Topology(mesh, creasing=creasing, scheme=scheme)
     pt = t.patch_table(iso)
     st = pt.point_stencils(double=True)
     ctrl = mesh.positions.astype(np.float64)
@@ -373,13 +373,49 @@ def test_varying_tables_are_far_bilinear_tables(name):
     b = far.Topology(m, scheme=far.SCHEME_BILINEAR)
     b.refine_uniform(3)
     assert_same_rows(b.stencil_table(intermediate=True), theirs)
-    for call in (lambda: b.patch_table(2), lambda: b.bfr_patch_table()):
-        with pytest.raises(NotImplementedError):
-            call()
+    with pytest.raises(NotImplementedError):
+        b.bfr_patch_table()
     with pytest.raises(ValueError):
         far.Topology(m, scheme=far.SCHEME_LOOP)          # not a triangle mesh
     with pytest.raises(ValueError):
         far.Topology(m, scheme=3)
+
+
+@needs_ref
+@pytest.mark.parametrize("name,iso", [("cube-sphere", 2), ("pentagonal-prism", 3),
+                                      ("cube-with-triangles", 0), ("sphere-with-split-quad", 2),
+                                      ("open-grid", 1)])
+def test_bilinear_patch_tables_match_far_live(name, iso):
+    """Bilinear-scheme patch tables: one QUADS array, quads at level 0 with
+    transition bits where they border a refined n-gon, the n-gons' sub-faces
+    at level 1 (whatever the isolation level), no boundary bits; the same
+    arrays, PatchParams, patch map results and surface as live Far."""
+    m = (meshes.cube_sphere(3, seed=7, crease_fraction=0.25) if name == "cube-sphere"
+         else POLY_MESHES[name]())
+    sizes = getattr(m, "face_sizes", np.full(m.nfaces, 4))
+    nptex = int(sum(1 if n == 4 else n for n in sizes))
+    f, u, v = meshes.random_patch_locations(nptex, 2000, seed=iso + 3)
+    pt, coords, outs = _my_patch_eval(m, iso, f, u, v, scheme=far.SCHEME_BILINEAR)
+    P = ref.Patches(ref.Topology(m, scheme=ref.SCHEME_BILINEAR), iso, double=True)
+    assert [tuple(a) for a in pt.arrays] == [tuple(a) for a in P.arrays]
+    assert pt.arrays[0][0] == 3 and pt.arrays[0][4] == 4            # QUADS, 4 points
+    assert pt.params.tobytes() == P.params.tobytes()
+    src = np.zeros((P.num_vertices_total, 3))
+    src[:P.num_coarse] = m.positions
+    st = P.stencils
+    if len(st["sizes"]):
+        src[P.num_coarse:] = oracle.apply_stencils(m.positions, st["sizes"], st["offsets"],
+                                                   st["indices"], st["weights"], np.float64)
+    h = P.find(f, u, v)
+    np.testing.assert_array_equal(coords["patchIndex"], h[:, 1])
+    theirs = np.zeros(len(f), PATCH_COORD_DTYPE)
+    theirs["arrayIndex"], theirs["patchIndex"], theirs["vertIndex"] = h[:, 0], h[:, 1], h[:, 2]
+    theirs["s"], theirs["t"] = u, v
+    o = [np.zeros((len(f), 3)) for _ in range(3)]
+    assert oracle.eval_patches_f64(src, (0, 3, 3), o, [(0, 3, 3)] * 3, theirs, P.arrays,
+                                   P.indices, P.params)
+    for a, b in zip(outs, o):
+        assert bbox_rel_err(a, b) < 1e-12
 
 
 def icosahedron():
