@@ -617,6 +617,38 @@ def test_boundary_patch_tables_bit_exact(hip, mesh):
         np.testing.assert_array_equal(host(outs[k], shape=(n, 3)), want[k])
 
 
+@pytest.mark.parametrize("mesh", ["cube-with-triangles", "pentagonal-prism", "open-grid"])
+def test_bilinear_patch_tables_bit_exact(hip, mesh):
+    """The compiler's bilinear-scheme patch tables (QUADS patches, n-gon
+    sub-faces at level 1) on the device: P, dPdu, dPdv at random locations
+    of every ptex face equal the oracle's bit for bit."""
+    from opensubdiv_amd import far
+    from test_far_compiler import POLY_MESHES
+    m = POLY_MESHES[mesh]()
+    pt = far.Topology(m, scheme=far.SCHEME_BILINEAR).patch_table(2)
+    assert all(a[0] == 3 for a in pt.arrays)                 # QUADS
+    nptex = int(sum(1 if k == 4 else k for k in m.face_sizes))
+    rng = np.random.default_rng(5)
+    n = 3000
+    f = rng.integers(0, nptex, n).astype(np.int32)
+    u, v = rng.random(n), rng.random(n)
+    coords = pt.find(f, u, v)
+    assert (coords["arrayIndex"] >= 0).all()
+    st = pt.point_stencils()
+    ctrl = m.positions.astype(np.float32)
+    pts = oracle.apply_stencils(ctrl, st["sizes"], st["offsets"], st["indices"], st["weights"]) \
+        if len(st["sizes"]) else np.zeros((0, 3), np.float32)
+    src = np.ascontiguousarray(np.concatenate([ctrl, pts]))
+    want = [np.zeros((n, 3), np.float32) for _ in range(3)]
+    assert oracle.eval_patches(src, (0, 3, 3), want, [(0, 3, 3)] * 3, coords, pt.arrays,
+                               pt.indices, pt.params)
+    outs = [empty(n * 3) for _ in range(3)]
+    assert eval_patches(hip, dev(src), (0, 3, 3), outs, [(0, 3, 3)] * 3, n, dev(coords),
+                        dev(pt.arrays), dev(pt.indices), dev(pt.params)) == 0
+    for k in range(3):
+        np.testing.assert_array_equal(host(outs[k], shape=(n, 3)), want[k])
+
+
 @pytest.mark.parametrize("name", ["sphere", "prism", "grid"])
 def test_bfr_face_queries_on_the_device_match_bfr_surfaces(hip, name):
     """Bfr-style per-face evaluation on the device: the compiler's table
