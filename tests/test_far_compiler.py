@@ -418,6 +418,28 @@ def test_bilinear_patch_tables_match_far_live(name, iso):
         assert bbox_rel_err(a, b) < 1e-12
 
 
+@needs_ref
+@pytest.mark.parametrize("name", ["pentagonal-prism", "cube-with-triangles"])
+def test_bilinear_limit_stencils_match_far_live(name):
+    """Limit stencils of the bilinear scheme (P and first derivatives of the
+    QUADS patches) at random ptex locations against
+    Far::LimitStencilTableFactory on the bilinear refiner."""
+    m = POLY_MESHES[name]()
+    nptex = int(sum(1 if n == 4 else n for n in m.face_sizes))
+    f, u, v = meshes.random_patch_locations(nptex, 500, seed=4)
+    pt = far.Topology(m, scheme=far.SCHEME_BILINEAR).patch_table(1)
+    mine = pt.limit_stencils(f, u, v, double=True)
+    theirs = ref.stencils_limit(ref.Topology(m, scheme=ref.SCHEME_BILINEAR), 1, f, u, v,
+                                double=True)
+    ctrl = m.positions.astype(np.float64)
+    for k in ("weights", "du", "dv"):
+        a = oracle.apply_stencils(ctrl, mine["sizes"], mine["offsets"], mine["indices"], mine[k],
+                                  np.float64)
+        b = oracle.apply_stencils(ctrl, theirs["sizes"], theirs["offsets"], theirs["indices"],
+                                  theirs[k], np.float64)
+        assert bbox_rel_err(a, b) < 1e-12, k
+
+
 def icosahedron():
     p = (1 + 5 ** 0.5) / 2
     faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
